@@ -1,0 +1,52 @@
+"""Pins the CPU oracle against the reference build's own outputs.
+
+The fixtures (tests/golden/*.npz, written by tests/golden/make_fixtures.py
+from the reference setup.out/core.out) hold record 0, the SHA-256 digest of
+E, H and the marker array of every one of 20 later records, the reference
+diag.dat and the MC_GAUSS_TEST build's gauss_test.dat.  The oracle must
+reproduce every record bit for bit, marker order included (so the J
+summation order is the reference's too), and the printed diagnostics to
+their 7 printed digits.
+"""
+import numpy as np
+import pytest
+
+from oracle_lib import (field_digest, load_fixture, marker_digest,
+                        oracle_from_fixture)
+
+CASES = ["periodic", "laser_mur", "mixed"]
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_bit_exact_records(case):
+    fx = load_fixture(case)
+    o = oracle_from_fixture(fx)
+    for r in range(1, len(fx["digest_E"]) + 1):
+        o.step()
+        E, H = o.fields()
+        M = o.markers()
+        assert field_digest(E) == fx["digest_E"][r - 1], (case, r, "E")
+        assert field_digest(H) == fx["digest_H"][r - 1], (case, r, "H")
+        assert marker_digest(M) == fx["digest_M"][r - 1], (case, r, "markers")
+        # sys_%06d.txt prints the time with %.14e
+        assert abs(o.time - fx["times"][r - 1]) <= 1e-13 * abs(o.time)
+        if r == 1:
+            np.testing.assert_array_equal(E, fx["E1"])
+            np.testing.assert_array_equal(H, fx["H1"])
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_diagnostics_match_reference_files(case):
+    fx = load_fixture(case)
+    o = oracle_from_fixture(fx)
+    rows, gs = [], []
+    for _ in range(len(fx["digest_E"])):
+        d, g = o.step(gauss=True)
+        rows.append(d)
+        gs.append(g)
+    rows = np.array(rows)[1:]            # diag.dat skips the t = 0 point
+    ref = fx["diag"][:, :6]
+    # diag.dat is printed with %e (7 significant digits)
+    np.testing.assert_allclose(rows, ref, rtol=6e-7, atol=0)
+    g = fx["gauss"][:, 1:]
+    np.testing.assert_allclose(np.array(gs), g, rtol=6e-7, atol=0)
