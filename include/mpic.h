@@ -1,0 +1,149 @@
+/*
+ * mpic.h -- C ABI of the MI355X PIC time step (libmpic.so).
+ *
+ * Replaces, for one GPU holding the whole domain, the physics API that the
+ * reference driver source/core/main.c:353-476 calls (cvarin/mandor3):
+ *
+ *   em_init / em_HHalfStep / em_HStep / em_EStep_start / em_EStep_finish /
+ *   em_energy                          source/core/em.h:32-39
+ *   plasma_init / plasma_move / plasma_temperature / jbc_finish /
+ *   plasma_rho                         source/core/plasma.h:82-87
+ *   plasma_load / plasma_save          source/core/plasma_IO.c:227,273
+ *   gauss_before / gauss_after         source/core/plasma_gaussTest.h:21-22
+ *
+ * The reference keeps E, H, J and the markers in process globals
+ * (source/core/main.c:90-91, source/core/plasma.c:40-43); here they live in
+ * device memory behind an opaque mpic_run_t, so several runs can coexist on
+ * one device (ensemble / serving).  Plain pointers and sizes only.
+ *
+ * Error convention: the reference logs and exits (ENSURE/DIE,
+ * source/log.h:45-60); every call here returns MPIC_OK (0) or a negative
+ * code, with a message in mpic_last_error().  The C host driver maps a
+ * failure to "log + exit(1)" like the reference.
+ *
+ * Layouts at this boundary are the reference's binData layouts:
+ *   fields  node block [0,n] per axis, interleaved {x,y,z} doubles, k
+ *           fastest (the payload of sys_E/H_*.bin, source/type_mesh.c:285-311)
+ *   markers marker_t records (source/type_marker.h:12-25): x y z vx vy vz
+ *           (double), rho qDivM (float), optional int id; recsize 56 or 64.
+ */
+#ifndef MPIC_H
+#define MPIC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MPIC_OK            0
+#define MPIC_ERR_ARG      -1
+#define MPIC_ERR_DEVICE   -2
+#define MPIC_ERR_ORDER    -3   /* stage called out of order (emStage) */
+#define MPIC_ERR_CAPACITY -4
+#define MPIC_ERR_STATE    -5
+
+/* Boundary codes, reference enum source/misc_parameters.h:38-42. */
+#define MPIC_BC_PERIODIC 0
+#define MPIC_BC_MIRROR   1
+#define MPIC_BC_OPEN     2     /* first-order Mur for fields, deletion for markers */
+
+/* Arithmetic of a run. */
+#define MPIC_FP64        0     /* plain form: FP64 everywhere, reference arithmetic */
+#define MPIC_FP32_SPLIT  1     /* int32 cell + FP32 offset markers, FP32 push/deposit/FDTD,
+                                  FP64 J flush and diagnostics */
+
+typedef struct mpic_run mpic_run_t;
+
+/* Domain description: the content of binData/domain.bin
+ * (source/misc_parameters.c:98-110) plus the start time of the record. */
+typedef struct {
+    int    n[3];          /* cells per axis (dmn_max) */
+    double h[3];          /* mesh steps */
+    double L[3];          /* box lengths */
+    double tau;           /* time step */
+    int    bc[6];         /* {xmin, ymin, zmin, xmax, ymax, zmax} */
+    double time;          /* Time of the loaded record */
+    int    precision;     /* MPIC_FP64 or MPIC_FP32_SPLIT */
+    long   max_markers;   /* device capacity in markers */
+    int    device;        /* HIP device ordinal */
+} mpic_config_t;
+
+const char *mpic_last_error(void);
+int  mpic_device_count(void);
+
+int  mpic_create(const mpic_config_t *cfg, mpic_run_t **run);
+int  mpic_destroy(mpic_run_t *run);
+/* Device memory the run holds (bytes); used for HBM admission. */
+int  mpic_footprint(const mpic_config_t *cfg, size_t *bytes);
+
+/* sysIO_loadEM / sysIO_save payloads (host buffers, 3*(n+1)^3 doubles). */
+int  mpic_set_fields(mpic_run_t *run, const double *E, const double *H);
+int  mpic_get_fields(mpic_run_t *run, double *E, double *H);
+/* J after jbc_finish, nodes [-2,n+2], interleaved, k fastest. */
+int  mpic_get_J(mpic_run_t *run, double *J);
+
+/* plasma_load: records in file order.  plasma_save: live markers in device
+ * order; *n in: capacity of 'rec', out: markers written. */
+int  mpic_markers_load(mpic_run_t *run, const void *rec, long n, int recsize);
+int  mpic_markers_save(mpic_run_t *run, void *rec, long *n, int recsize);
+long mpic_marker_count(mpic_run_t *run);
+
+/* Focused-laser soft source (source/scpic/em_sources.c, em_mirror.c:667-717):
+ * A = field amplitude (2*pi*a0), T0 = peak arrival at focus, delay = travel
+ * time from the source plane to the focus, dt = 1/e duration, phi0 = phase,
+ * forward = +x propagation, plane = source plane index (5 in the
+ * reference).  map: E map then H map, each (n[1]-9)*(n[2]-9) records of
+ * {y_cos, y_sin, z_cos, z_sin}, k fastest. */
+int  mpic_add_laser(mpic_run_t *run, double A, double T0, double delay,
+                    double dt, double phi0, int forward, int plane,
+                    const double *map);
+
+/* em_init + plasma_init (caps, Mur buffers, ghost nodes, walls applied to
+ * the loaded markers). */
+int  mpic_init(mpic_run_t *run);
+
+/* The stages of one time step, in the order of source/core/main.c:365-425. */
+int  mpic_em_h_half(mpic_run_t *run);                           /* em_HHalfStep   */
+int  mpic_em_energy(mpic_run_t *run, double *W_E, double *W_M); /* em_energy      */
+int  mpic_push_deposit(mpic_run_t *run);                        /* plasma_move    */
+int  mpic_temperature(mpic_run_t *run, double *WTx, double *WTy, double *WTz);
+int  mpic_em_h_step(mpic_run_t *run);                           /* em_HStep       */
+int  mpic_em_e_start(mpic_run_t *run);                          /* em_EStep_start */
+int  mpic_j_finish(mpic_run_t *run);                            /* jbc_finish     */
+int  mpic_em_e_finish(mpic_run_t *run);                         /* em_EStep_finish; Time += tau */
+
+/* A whole main-loop iteration; diag (may be NULL) = {Time, W_E, W_M, WTx,
+ * WTy, WTz} as handed to wDensity_addPoint (source/core/diag_WDensity.c:136).
+ * With diag == NULL nothing is copied back and the step stays asynchronous. */
+int  mpic_step(mpic_run_t *run, double *diag);
+/* n steps without host synchronisation (captured into a HIP graph once the
+ * launch topology is stable). */
+int  mpic_advance(mpic_run_t *run, int nsteps);
+int  mpic_sync(mpic_run_t *run);
+double mpic_time(mpic_run_t *run);
+
+/* CIC charge density with boundary fold (plasma_rho), nodes [-2,n+2]. */
+int  mpic_rho(mpic_run_t *run, double *rho);
+/* gauss_before / gauss_after residuals {max|drho/dt + div J|, max|div E - rho|}:
+ * call mpic_gauss_before() before mpic_push_deposit and mpic_gauss_after()
+ * after mpic_em_e_finish. */
+int  mpic_gauss_before(mpic_run_t *run);
+int  mpic_gauss_after(mpic_run_t *run, double *res);
+
+/* Per-step marker sort by cell (tile-bucketed, stable); drops deleted
+ * markers.  counts (may be NULL): per-cell live-marker counts, n0*n1*n2. */
+int  mpic_sort(mpic_run_t *run, int32_t *counts);
+/* Cell index of every live marker (device order) for parity checks:
+ * (i*n1 + j)*n2 + k of floor(x/h). */
+int  mpic_marker_cells(mpic_run_t *run, int32_t *cells, long *n);
+
+/* Kernel timing of the last mpic_advance (HIP events on the run's stream):
+ * milliseconds spent in push+deposit and in the field update. */
+int  mpic_last_timing(mpic_run_t *run, double *push_ms, double *field_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,250 @@
+"""mandor3_amd -- MI355X-native PIC time step of Mandor behind a C ABI.
+
+The compute path is libmpic.so (HIP, gfx950; see include/mpic.h).  This
+module is a thin ctypes binding used by the tests and the benchmark; the
+host driver of the product is C (mandor3_amd/host).  There is no CPU
+fallback: if libmpic.so is missing or no HIP device is present, the calls
+raise.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libmpic.so")
+
+BC_PERIODIC, BC_MIRROR, BC_OPEN = 0, 1, 2
+FP64, FP32_SPLIT = 0, 1
+
+MARKER_DTYPE = np.dtype([("x", "<f8"), ("y", "<f8"), ("z", "<f8"),
+                         ("vx", "<f8"), ("vy", "<f8"), ("vz", "<f8"),
+                         ("rho", "<f4"), ("qDivM", "<f4"), ("id", "<i4"),
+                         ("pad", "<i4")])
+
+
+class Config(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int * 3), ("h", ctypes.c_double * 3),
+                ("L", ctypes.c_double * 3), ("tau", ctypes.c_double),
+                ("bc", ctypes.c_int * 6), ("time", ctypes.c_double),
+                ("precision", ctypes.c_int), ("max_markers", ctypes.c_long),
+                ("device", ctypes.c_int)]
+
+
+class MpicError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """Loads libmpic.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise MpicError("libmpic.so is not built (python -m mandor3_amd.build)")
+        L = ctypes.CDLL(LIB_PATH)
+        P, I, D, LG = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_long
+        sig = {
+            "mpic_last_error": (ctypes.c_char_p, []),
+            "mpic_device_count": (I, []),
+            "mpic_create": (I, [ctypes.POINTER(Config), ctypes.POINTER(P)]),
+            "mpic_destroy": (I, [P]),
+            "mpic_footprint": (I, [ctypes.POINTER(Config), ctypes.POINTER(ctypes.c_size_t)]),
+            "mpic_set_fields": (I, [P, P, P]),
+            "mpic_get_fields": (I, [P, P, P]),
+            "mpic_get_J": (I, [P, P]),
+            "mpic_markers_load": (I, [P, P, LG, I]),
+            "mpic_markers_save": (I, [P, P, ctypes.POINTER(LG), I]),
+            "mpic_marker_count": (LG, [P]),
+            "mpic_add_laser": (I, [P, D, D, D, D, D, I, I, P]),
+            "mpic_init": (I, [P]),
+            "mpic_em_h_half": (I, [P]),
+            "mpic_em_energy": (I, [P, P, P]),
+            "mpic_push_deposit": (I, [P]),
+            "mpic_temperature": (I, [P, P, P, P]),
+            "mpic_em_h_step": (I, [P]),
+            "mpic_em_e_start": (I, [P]),
+            "mpic_j_finish": (I, [P]),
+            "mpic_em_e_finish": (I, [P]),
+            "mpic_step": (I, [P, P]),
+            "mpic_advance": (I, [P, I]),
+            "mpic_sync": (I, [P]),
+            "mpic_time": (D, [P]),
+            "mpic_rho": (I, [P, P]),
+            "mpic_gauss_before": (I, [P]),
+            "mpic_gauss_after": (I, [P, P]),
+            "mpic_sort": (I, [P, P]),
+            "mpic_marker_cells": (I, [P, P, ctypes.POINTER(LG)]),
+            "mpic_last_timing": (I, [P, P, P]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _check(rc, what):
+    if rc != 0:
+        msg = lib().mpic_last_error()
+        raise MpicError("%s failed (%d): %s" % (what, rc, msg.decode() if msg else ""))
+
+
+def device_count():
+    return lib().mpic_device_count()
+
+
+class Run:
+    """One simulation resident on the device (the reference's process state
+    of source/core/main.c, per run)."""
+
+    def __init__(self, n, h, L, tau, bc, time=0.0, precision=FP64,
+                 max_markers=1 << 20, device=0):
+        c = Config()
+        for a in range(3):
+            c.n[a] = int(n[a])
+            c.h[a] = float(h[a])
+            c.L[a] = float(L[a])
+        for b in range(6):
+            c.bc[b] = int(bc[b])
+        c.tau = float(tau)
+        c.time = float(time)
+        c.precision = int(precision)
+        c.max_markers = int(max_markers)
+        c.device = int(device)
+        self.cfg = c
+        self.n = tuple(int(v) for v in n)
+        self.h = tuple(float(v) for v in h)
+        self.tau = float(tau)
+        self.handle = ctypes.c_void_p()
+        _check(lib().mpic_create(ctypes.byref(c), ctypes.byref(self.handle)),
+               "mpic_create")
+
+    def close(self):
+        if self.handle:
+            lib().mpic_destroy(self.handle)
+            self.handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def nodes(self):
+        return int(np.prod([v + 1 for v in self.n]))
+
+    def set_fields(self, E, H):
+        E = np.ascontiguousarray(E, dtype=np.float64).reshape(-1)
+        H = np.ascontiguousarray(H, dtype=np.float64).reshape(-1)
+        if E.size != 3 * self.nodes or H.size != 3 * self.nodes:
+            raise ValueError("field blocks must hold 3*(n+1)^3 doubles")
+        _check(lib().mpic_set_fields(self.handle, _ptr(E), _ptr(H)), "mpic_set_fields")
+
+    def fields(self):
+        E = np.empty(3 * self.nodes)
+        H = np.empty(3 * self.nodes)
+        _check(lib().mpic_get_fields(self.handle, _ptr(E), _ptr(H)), "mpic_get_fields")
+        return E, H
+
+    def J(self):
+        J = np.empty(3 * int(np.prod([v + 5 for v in self.n])))
+        _check(lib().mpic_get_J(self.handle, _ptr(J)), "mpic_get_J")
+        return J
+
+    def load_markers(self, M):
+        M = np.ascontiguousarray(M, dtype=MARKER_DTYPE)
+        _check(lib().mpic_markers_load(self.handle, _ptr(M), M.size, 64),
+               "mpic_markers_load")
+
+    def markers(self):
+        cap = max(int(lib().mpic_marker_count(self.handle)), 1)
+        M = np.zeros(cap, dtype=MARKER_DTYPE)
+        n = ctypes.c_long(cap)
+        _check(lib().mpic_markers_save(self.handle, _ptr(M), ctypes.byref(n), 64),
+               "mpic_markers_save")
+        return M[:n.value]
+
+    def marker_count(self):
+        return int(lib().mpic_marker_count(self.handle))
+
+    def add_laser(self, params, lmap):
+        A, T0, delay, dt, phi0, fwd, plane = params
+        lmap = np.ascontiguousarray(lmap, dtype=np.float64)
+        _check(lib().mpic_add_laser(self.handle, A, T0, delay, dt, phi0, int(fwd),
+                                    int(plane), _ptr(lmap)), "mpic_add_laser")
+
+    def init(self):
+        _check(lib().mpic_init(self.handle), "mpic_init")
+
+    def step(self, diag=True):
+        if not diag:
+            _check(lib().mpic_step(self.handle, None), "mpic_step")
+            return None
+        d = np.zeros(6)
+        _check(lib().mpic_step(self.handle, _ptr(d)), "mpic_step")
+        return d
+
+    def stage(self, name):
+        _check(getattr(lib(), "mpic_" + name)(self.handle), "mpic_" + name)
+
+    def advance(self, nsteps):
+        _check(lib().mpic_advance(self.handle, int(nsteps)), "mpic_advance")
+
+    def sync(self):
+        _check(lib().mpic_sync(self.handle), "mpic_sync")
+
+    def timing(self):
+        a, b = np.zeros(1), np.zeros(1)
+        _check(lib().mpic_last_timing(self.handle, _ptr(a), _ptr(b)), "mpic_last_timing")
+        return float(a[0]), float(b[0])
+
+    @property
+    def time(self):
+        return lib().mpic_time(self.handle)
+
+    def rho(self):
+        r = np.empty(int(np.prod([v + 5 for v in self.n])))
+        _check(lib().mpic_rho(self.handle, _ptr(r)), "mpic_rho")
+        return r
+
+    def gauss_before(self):
+        _check(lib().mpic_gauss_before(self.handle), "mpic_gauss_before")
+
+    def gauss_after(self):
+        g = np.zeros(2)
+        _check(lib().mpic_gauss_after(self.handle, _ptr(g)), "mpic_gauss_after")
+        return g
+
+    def energy(self):
+        a, b = np.zeros(1), np.zeros(1)
+        _check(lib().mpic_em_energy(self.handle, _ptr(a), _ptr(b)), "mpic_em_energy")
+        return float(a[0]), float(b[0])
+
+    def temperature(self):
+        w = np.zeros(3)
+        _check(lib().mpic_temperature(self.handle, _ptr(w[0:]), _ptr(w[1:]), _ptr(w[2:])),
+               "mpic_temperature")
+        return w
+
+    def cells(self):
+        cap = max(self.marker_count(), 1)
+        c = np.zeros(cap, dtype=np.int32)
+        n = ctypes.c_long(cap)
+        _check(lib().mpic_marker_cells(self.handle, _ptr(c), ctypes.byref(n)),
+               "mpic_marker_cells")
+        return c[:n.value]
+
+    def sort(self, with_counts=False):
+        counts = np.zeros(int(np.prod(self.n)), dtype=np.int32) if with_counts else None
+        _check(lib().mpic_sort(self.handle, _ptr(counts) if with_counts else None),
+               "mpic_sort")
+        return counts

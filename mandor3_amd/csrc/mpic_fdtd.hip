@@ -1,0 +1,510 @@
+/*
+ * mpic_fdtd.hip -- Yee field update, boundary caps and walls (FP64).
+ *
+ * Built with -ffp-contract=off: every update keeps the reference's
+ * association (source/core/em.c:55-222, source/core/em_caps.c:405-809), so
+ * given the same J the fields are bit-identical to the CPU path.
+ *
+ * The reference advances a 3-node boundary layer ("caps") one half step
+ * ahead of the interior so that halo messages overlap with the push; on one
+ * GPU there is no halo, but the cap arithmetic (full-step H^{n+1/2}, the
+ * H^n = (H^{n-1/2} + H^{n+1/2})/2 average, the wall order and Mur buffers)
+ * decides the boundary values, so it is kept: Hc/Ec hold the cap values of
+ * the whole layer in one array (the per-cap copies of the reference hold
+ * identical values on their overlaps).
+ *
+ * Roofline: all kernels here are HBM-bound stencils.  Interior H update
+ * reads E (3 comps) and H (3) and writes H (3): 72 B/node; interior E
+ * update likewise 72 B/node; E -= tau*J reads J (3) and E (3), writes E
+ * (3): 72 B/node.
+ */
+#include "mpic_internal.h"
+
+#define TPB 256
+
+static inline unsigned nblk(long n) { return (unsigned)((n + TPB - 1) / TPB); }
+
+__device__ static inline bool box_decode(const BoxSet &bs, long t, int &i, int &j, int &k)
+{
+    if (t >= bs.off[bs.n]) return false;
+    int b = 0;
+    while (t >= bs.off[b + 1]) ++b;
+    long l = t - bs.off[b];
+    const Box &B = bs.b[b];
+    int dz = B.hi[2] - B.lo[2] + 1, dy = B.hi[1] - B.lo[1] + 1;
+    k = B.lo[2] + (int)(l % dz);
+    l /= dz;
+    j = B.lo[1] + (int)(l % dy);
+    i = B.lo[0] + (int)(l / dy);
+    return true;
+}
+
+/* Cap H^{n+1/2} = H + c3*(dEy/dz) - c2*(dEz/dy)..., source/core/em_caps.c:418-439. */
+__global__ void k_cap_H(Fields64 f, Grid g, BoxSet bs, double c1, double c2, double c3)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    int i, j, k;
+    if (!box_decode(bs, t, i, j, k)) return;
+    long o = gidx(g, i, j, k);
+    long oi = gidx(g, i - 1, j, k), oj = gidx(g, i, j - 1, k), ok = gidx(g, i, j, k - 1);
+    const double *Ex = f.E[0], *Ey = f.E[1], *Ez = f.E[2];
+    f.Hc[0][o] = f.H[0][o] + c3 * (Ey[o] - Ey[ok]) - c2 * (Ez[o] - Ez[oj]);
+    f.Hc[1][o] = f.H[1][o] + c1 * (Ez[o] - Ez[oi]) - c3 * (Ex[o] - Ex[ok]);
+    f.Hc[2][o] = f.H[2][o] + c2 * (Ex[o] - Ex[oj]) - c1 * (Ey[o] - Ey[oi]);
+}
+
+/* Cap E^{n+1} from the cap H, source/core/em_caps.c:580-599. */
+__global__ void k_cap_E(Fields64 f, Grid g, BoxSet bs, double c1, double c2, double c3)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    int i, j, k;
+    if (!box_decode(bs, t, i, j, k)) return;
+    long o = gidx(g, i, j, k);
+    long oi = gidx(g, i + 1, j, k), oj = gidx(g, i, j + 1, k), ok = gidx(g, i, j, k + 1);
+    const double *Hx = f.Hc[0], *Hy = f.Hc[1], *Hz = f.Hc[2];
+    f.Ec[0][o] = f.E[0][o] - c3 * (Hy[ok] - Hy[o]) + c2 * (Hz[oj] - Hz[o]);
+    f.Ec[1][o] = f.E[1][o] - c1 * (Hz[oi] - Hz[o]) + c3 * (Hx[ok] - Hx[o]);
+    f.Ec[2][o] = f.E[2][o] - c2 * (Hx[oj] - Hx[o]) + c1 * (Hy[oi] - Hy[o]);
+}
+
+int launch_cap_H(hipStream_t s, const Fields64 &f, const Grid &g, const BoxSet &bs)
+{
+    long n = bs.off[bs.n];
+    if (n <= 0) return MPIC_OK;
+    k_cap_H<<<nblk(n), TPB, 0, s>>>(f, g, bs, g.tau / g.h[0], g.tau / g.h[1], g.tau / g.h[2]);
+    MPIC_CHECK(hipGetLastError());
+    return MPIC_OK;
+}
+
+int launch_cap_E(hipStream_t s, const Fields64 &f, const Grid &g, const BoxSet &bs)
+{
+    long n = bs.off[bs.n];
+    if (n <= 0) return MPIC_OK;
+    k_cap_E<<<nblk(n), TPB, 0, s>>>(f, g, bs, g.tau / g.h[0], g.tau / g.h[1], g.tau / g.h[2]);
+    MPIC_CHECK(hipGetLastError());
+    return MPIC_OK;
+}
+
+/* Walls of one face: periodic, mirror and first-order Mur
+ * (source/core/emCap_periodic.c:12-68, emCap_mirror.c:17-72,
+ * emCap_Mur.c:55-175). */
+__global__ void k_face(Grid g, FaceOp op)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    int nq = op.qhi - op.qlo + 1;
+    long tot = (long)(op.phi - op.plo + 1) * nq;
+    if (t >= tot) return;
+    int p = op.plo + (int)(t / nq), q = op.qlo + (int)(t % nq);
+    int ea = op.ea, ep = (ea + 1) % 3, eq = (ea + 2) % 3;
+    int r[3];
+    r[ep] = p;
+    r[eq] = q;
+#define AT(a) (r[ea] = (a), gidx(g, r[0], r[1], r[2]))
+    const int N = op.N, top = op.top;
+    const int pl = top ? N : 0;
+    switch (op.kind) {
+    case FACE_PERIODIC_H: {
+        long o0 = AT(0), oN = AT(N), o1 = AT(1), oN1 = AT(N + 1);
+        for (int c = 0; c < 3; ++c) {
+            op.X[c][o0] = op.X[c][oN];
+            op.X[c][oN1] = op.X[c][o1];
+        }
+        break;
+    }
+    case FACE_PERIODIC_E: {
+        long o0 = AT(0), oN = AT(N), o1 = AT(1), oN1 = AT(N + 1);
+        for (int c = 0; c < 3; ++c) {
+            op.X[c][oN] = op.X[c][o0];
+            op.X[c][oN1] = op.X[c][o1];
+        }
+        break;
+    }
+    case FACE_WALL_H: {
+        long opl = AT(pl), oin = AT(pl + 1 - top), oex = AT(pl + top);
+        if (op.zeroN) op.X[ea][opl] = 0;
+        op.X[ep][oex] = op.X[ep][oin];
+        op.X[eq][oex] = op.X[eq][oin];
+        break;
+    }
+    case FACE_MIRROR_E: {
+        long opl = AT(pl), oin = AT(pl + 1 - top), oex = AT(pl + top);
+        op.X[ep][opl] = 0;
+        op.X[eq][opl] = 0;
+        op.X[ea][oex] = op.X[ea][oin];
+        break;
+    }
+    case FACE_MUR_E: {
+        long opl = AT(pl), os = AT(pl + 1 - 2 * top), oin = AT(pl + 1 - top), oex = AT(pl + top);
+        double *b = op.buf + 4 * t;
+        op.X[ea][oex] = op.X[ea][oin];
+        double vp = b[2] + op.alpha * (op.X[ep][os] - b[0]);
+        double vq = b[3] + op.alpha * (op.X[eq][os] - b[1]);
+        op.X[ep][opl] = vp;
+        op.X[eq][opl] = vq;
+        b[0] = vp;
+        b[1] = vq;
+        b[2] = op.X[ep][os];
+        b[3] = op.X[eq][os];
+        break;
+    }
+    case FACE_MUR_INIT: {
+        long opl = AT(pl), os = AT(pl + 1 - 2 * top), oin = AT(pl + 1 - top), oex = AT(pl + top);
+        double *b = op.buf + 4 * t;
+        b[0] = op.X[ep][opl];
+        b[1] = op.X[eq][opl];
+        b[2] = op.X[ep][os];
+        b[3] = op.X[eq][os];
+        op.X[ea][oex] = op.X[ea][oin];
+        break;
+    }
+    }
+#undef AT
+}
+
+int launch_face(hipStream_t s, const Grid &g, const FaceOp &op)
+{
+    long n = (long)(op.phi - op.plo + 1) * (op.qhi - op.qlo + 1);
+    if (n <= 0) return MPIC_OK;
+    k_face<<<nblk(n), TPB, 0, s>>>(g, op);
+    MPIC_CHECK(hipGetLastError());
+    return MPIC_OK;
+}
+
+__device__ static inline int in_boxes(const BoxSet &bs, int i, int j, int k)
+{
+    for (int b = 0; b < bs.n; ++b) {
+        const Box &B = bs.b[b];
+        if (i >= B.lo[0] && i <= B.hi[0] && j >= B.lo[1] && j <= B.hi[1] &&
+            k >= B.lo[2] && k <= B.hi[2])
+            return 1;
+    }
+    return 0;
+}
+
+/* Interior H half step (source/core/em.c:55-77) fused with the flush of the
+ * cap layer: H = (H + Hc)/2 after the first half step (cap_flushHalfH,
+ * em_caps.c:730-749) or H = Hc after the second (cap_flushH, :754-770).
+ * Nodes [0, n+1]^3; interior and flush regions are disjoint. */
+__global__ void k_h_update(Fields64 f, Grid g, BoxSet fl, int full, double c1, double c2,
+                           double c3)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int NX = g.n[0] + 2, NY = g.n[1] + 2, NZ = g.n[2] + 2;
+    if (t >= (long)NX * NY * NZ) return;
+    int k = (int)(t % NZ);
+    long r = t / NZ;
+    int j = (int)(r % NY), i = (int)(r / NY);
+    long o = gidx(g, i, j, k);
+    if (i >= 3 && i <= g.n[0] - 2 && j >= 3 && j <= g.n[1] - 2 && k >= 3 && k <= g.n[2] - 2) {
+        long oi = gidx(g, i - 1, j, k), oj = gidx(g, i, j - 1, k), ok = gidx(g, i, j, k - 1);
+        const double *Ex = f.E[0], *Ey = f.E[1], *Ez = f.E[2];
+        const double ex = Ex[o], ey = Ey[o], ez = Ez[o];
+        f.H[0][o] += c3 * (ey - Ey[ok]) - c2 * (ez - Ez[oj]);
+        f.H[1][o] += c1 * (ez - Ez[oi]) - c3 * (ex - Ex[ok]);
+        f.H[2][o] += c2 * (ex - Ex[oj]) - c1 * (ey - Ey[oi]);
+    } else if (in_boxes(fl, i, j, k)) {
+        if (full) {
+            f.H[0][o] = f.Hc[0][o];
+            f.H[1][o] = f.Hc[1][o];
+            f.H[2][o] = f.Hc[2][o];
+        } else {
+            f.H[0][o] = 0.5 * (f.H[0][o] + f.Hc[0][o]);
+            f.H[1][o] = 0.5 * (f.H[1][o] + f.Hc[1][o]);
+            f.H[2][o] = 0.5 * (f.H[2][o] + f.Hc[2][o]);
+        }
+    }
+}
+
+int launch_h_update(hipStream_t s, const Fields64 &f, const Grid &g, const BoxSet &flush, int full)
+{
+    long n = (long)(g.n[0] + 2) * (g.n[1] + 2) * (g.n[2] + 2);
+    k_h_update<<<nblk(n), TPB, 0, s>>>(f, g, flush, full, 0.5 * g.tau / g.h[0],
+                                        0.5 * g.tau / g.h[1], 0.5 * g.tau / g.h[2]);
+    MPIC_CHECK(hipGetLastError());
+    return MPIC_OK;
+}
+
+/* Interior E step (source/core/em.c:134-161) and/or the flush of the cap
+ * layer E = Ec (cap_flushE, em_caps.c:775-794). */
+__global__ void k_e_update(Fields64 f, Grid g, BoxSet fl, int interior, int do_flush,
+                           double c1, double c2, double c3)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int NX = g.n[0] + 2, NY = g.n[1] + 2, NZ = g.n[2] + 2;
+    if (t >= (long)NX * NY * NZ) return;
+    int k = (int)(t % NZ);
+    long r = t / NZ;
+    int j = (int)(r % NY), i = (int)(r / NY);
+    long o = gidx(g, i, j, k);
+    if (i >= 2 && i <= g.n[0] - 2 && j >= 2 && j <= g.n[1] - 2 && k >= 2 && k <= g.n[2] - 2) {
+        if (!interior) return;
+        long oi = gidx(g, i + 1, j, k), oj = gidx(g, i, j + 1, k), ok = gidx(g, i, j, k + 1);
+        const double *Hx = f.H[0], *Hy = f.H[1], *Hz = f.H[2];
+        const double hx = Hx[o], hy = Hy[o], hz = Hz[o];
+        f.E[0][o] -= c3 * (Hy[ok] - hy) - c2 * (Hz[oj] - hz);
+        f.E[1][o] -= c1 * (Hz[oi] - hz) - c3 * (Hx[ok] - hx);
+        f.E[2][o] -= c2 * (Hx[oj] - hx) - c1 * (Hy[oi] - hy);
+    } else if (do_flush && in_boxes(fl, i, j, k)) {
+        f.E[0][o] = f.Ec[0][o];
+        f.E[1][o] = f.Ec[1][o];
+        f.E[2][o] = f.Ec[2][o];
+    }
+}
+
+int launch_e_update(hipStream_t s, const Fields64 &f, const Grid &g, const BoxSet &flush,
+                    int interior, int do_flush)
+{
+    long n = (long)(g.n[0] + 2) * (g.n[1] + 2) * (g.n[2] + 2);
+    k_e_update<<<nblk(n), TPB, 0, s>>>(f, g, flush, interior, do_flush, g.tau / g.h[0],
+                                        g.tau / g.h[1], g.tau / g.h[2]);
+    MPIC_CHECK(hipGetLastError());
+    return MPIC_OK;
+}
+
+/* E -= tau*J over nodes [-1, n+1]^3, source/core/em.c:166-184. */
+__global__ void k_e_tauJ(Fields64 f, Grid g)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int NX = g.n[0] + 3, NY = g.n[1] + 3, NZ = g.n[2] + 3;
+    if (t >= (long)NX * NY * NZ) return;
+    int k = (int)(t % NZ) - 1;
+    long r = t / NZ;
+    int j = (int)(r % NY) - 1, i = (int)(r / NY) - 1;
+    long o = gidx(g, i, j, k);
+    const double tau = g.tau;
+    f.E[0][o] -= tau * f.J[0][o];
+    f.E[1][o] -= tau * f.J[1][o];
+    f.E[2][o] -= tau * f.J[2][o];
+}
+
+int launch_e_tauJ(hipStream_t s, const Fields64 &f, const Grid &g)
+{
+    long n = (long)(g.n[0] + 3) * (g.n[1] + 3) * (g.n[2] + 3);
+    k_e_tauJ<<<nblk(n), TPB, 0, s>>>(f, g);
+    MPIC_CHECK(hipGetLastError());
+    return MPIC_OK;
+}
+
+/* One-plane laser soft source, source/scpic/em_mirror.c:667-693.  The time
+ * factors (envelope, cos, sin of the carrier) are evaluated in FP64 on the
+ * host once per step and passed in. */
+__global__ void k_laser(double *Fy, double *Fz, Grid g, const double *map, int pos, int JMIN,
+                        int KMIN, int KMAX, int jmin, int jmax, int kmin, int kmax, double A,
+                        double cs, double sn)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    int nk = kmax - kmin + 1;
+    if (t >= (long)(jmax - jmin + 1) * nk) return;
+    int j = jmin + (int)(t / nk), k = kmin + (int)(t % nk);
+    long cnt = (long)(j - JMIN) * (KMAX - KMIN + 1) + k - KMIN;
+    const double *w = map + 4 * cnt;
+    long o = gidx(g, pos, j, k);
+    Fy[o] += A * (w[2] * cs + w[3] * sn);
+    Fz[o] += A * (w[0] * cs + w[1] * sn);
+}
+
+int launch_laser(hipStream_t s, double *Fy, double *Fz, const Grid &g, const double *map, int pos,
+                 int JMIN, int KMIN, int KMAX, int jmin, int jmax, int kmin, int kmax, double A,
+                 double cs, double sn)
+{
+    long n = (long)(jmax - jmin + 1) * (kmax - kmin + 1);
+    if (jmax < jmin || kmax < kmin) return MPIC_OK;
+    k_laser<<<nblk(n), TPB, 0, s>>>(Fy, Fz, g, map, pos, JMIN, KMIN, KMAX, jmin, jmax, kmin,
+                                     kmax, A, cs, sn);
+    MPIC_CHECK(hipGetLastError());
+    return MPIC_OK;
+}
+
+/* J boundary fold of one face b (order x-min, x-max, y-min, y-max, z-min,
+ * z-max), source/core/plasma_currentWalls.c:132-227. */
+__global__ void k_jfold(Fields64 f, Grid g, int b, int bc)
+{
+    const int ea = b >> 1, top = b & 1;
+    const int ep = (ea == 0) ? 1 : 0, eq = (ea == 2) ? 1 : 2;
+    const int np = g.n[ep] + 5, nq = g.n[eq] + 5;
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)np * nq) return;
+    int r[3];
+    r[ep] = -2 + (int)(t / nq);
+    r[eq] = -2 + (int)(t % nq);
+    const int N = g.n[ea];
+#define AT(a) (r[ea] = (a), gidx(g, r[0], r[1], r[2]))
+    if (bc == MPIC_BC_PERIODIC) {
+        for (int a = -2; a <= 2; ++a) {
+            long lo = AT(a), hi = AT(a + N);
+            for (int c = 0; c < 3; ++c) {
+                f.J[c][hi] += f.J[c][lo];
+                f.J[c][lo] = f.J[c][hi];
+            }
+        }
+    } else {
+        const int pl = top ? N : 0, so = top ? 1 : -1, in_ = 1 - top, ex_ = top;
+        double *Ja = f.J[ea], *Jp = f.J[ep], *Jq = f.J[eq];
+        long oin = AT(pl + in_), oex = AT(pl + ex_), om1 = AT(pl - so), op1 = AT(pl + so);
+        long om2 = AT(pl - 2 * so), op2 = AT(pl + 2 * so), oin2 = AT(pl + in_ - so),
+             oex2 = AT(pl + ex_ + so), o0 = AT(pl);
+        Ja[oin] -= Ja[oex];
+        Jp[om1] += Jp[op1];
+        Jq[om1] += Jq[op1];
+        Ja[oin2] -= Ja[oex2];
+        Jp[om2] += Jp[op2];
+        Jq[om2] += Jq[op2];
+        Jp[o0] = 0;
+        Jq[o0] = 0;
+        Ja[oex] = Ja[oin];
+        Jp[op1] = -Jp[om1];
+        Jq[op1] = -Jq[om1];
+        Ja[oex2] = Ja[oin2];
+        Jp[op2] = -Jp[om2];
+        Jq[op2] = -Jq[om2];
+    }
+#undef AT
+}
+
+int launch_jfold(hipStream_t s, const Fields64 &f, const Grid &g, int b, int bc)
+{
+    const int ea = b >> 1;
+    const int ep = (ea == 0) ? 1 : 0, eq = (ea == 2) ? 1 : 2;
+    if (bc == MPIC_BC_PERIODIC && (b & 1)) return MPIC_OK;
+    long n = (long)(g.n[ep] + 5) * (g.n[eq] + 5);
+    k_jfold<<<nblk(n), TPB, 0, s>>>(f, g, b, bc);
+    MPIC_CHECK(hipGetLastError());
+    return MPIC_OK;
+}
+
+/* Field energy sums, source/core/em.c:226-252 (FP64; block partial sums
+ * combined with FP64 atomics, so the order differs from the CPU loop). */
+__global__ void k_energy(Fields64 f, Grid g, double *acc)
+{
+    __shared__ double se[TPB], sb[TPB];
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long tot = (long)g.n[0] * g.n[1] * g.n[2];
+    double we = 0, wb = 0;
+    for (long c = t; c < tot; c += (long)gridDim.x * blockDim.x) {
+        int k = (int)(c % g.n[2]);
+        long r = c / g.n[2];
+        int j = (int)(r % g.n[1]), i = (int)(r / g.n[1]);
+        double ex = f.E[0][gidx(g, i + 1, j, k)], ey = f.E[1][gidx(g, i, j + 1, k)],
+               ez = f.E[2][gidx(g, i, j, k + 1)], hx = f.H[0][gidx(g, i, j + 1, k + 1)],
+               hy = f.H[1][gidx(g, i + 1, j, k + 1)], hz = f.H[2][gidx(g, i + 1, j + 1, k)];
+        we += ex * ex + ey * ey + ez * ez;
+        wb += hx * hx + hy * hy + hz * hz;
+    }
+    se[threadIdx.x] = we;
+    sb[threadIdx.x] = wb;
+    __syncthreads();
+    for (int w = TPB / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            se[threadIdx.x] += se[threadIdx.x + w];
+            sb[threadIdx.x] += sb[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        atomicAdd(acc + 0, se[0]);
+        atomicAdd(acc + 1, sb[0]);
+    }
+}
+
+int launch_energy(hipStream_t s, const Fields64 &f, const Grid &g, double *acc)
+{
+    long tot = (long)g.n[0] * g.n[1] * g.n[2];
+    long nb = (tot + TPB - 1) / TPB;
+    if (nb > 2048) nb = 2048;
+    k_energy<<<(unsigned)nb, TPB, 0, s>>>(f, g, acc);
+    MPIC_CHECK(hipGetLastError());
+    return MPIC_OK;
+}
+
+__global__ void k_zero(double *p, long n)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; t < n; t += (long)gridDim.x * blockDim.x) p[t] = 0.0;
+}
+
+int launch_zero(hipStream_t s, double *p, long n)
+{
+    long nb = (n + TPB - 1) / TPB;
+    if (nb > 65536) nb = 65536;
+    k_zero<<<(unsigned)nb, TPB, 0, s>>>(p, n);
+    MPIC_CHECK(hipGetLastError());
+    return MPIC_OK;
+}
+
+/* CIC rho fold of one face, source/core/plasma_rho.c:23-96. */
+__global__ void k_rho_fold(double *rho, Grid g, int b, int bc)
+{
+    const int ea = b >> 1, top = b & 1;
+    const int ep = (ea == 0) ? 1 : 0, eq = (ea == 2) ? 1 : 2;
+    const int np = g.n[ep] + 5, nq = g.n[eq] + 5;
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)np * nq) return;
+    int r[3];
+    r[ep] = -2 + (int)(t / nq);
+    r[eq] = -2 + (int)(t % nq);
+    const int N = g.n[ea];
+#define AT(a) (r[ea] = (a), gidx(g, r[0], r[1], r[2]))
+    if (bc == MPIC_BC_PERIODIC) {
+        for (int a = -2; a <= 2; ++a) {
+            long lo = AT(a), hi = AT(a + N);
+            rho[hi] += rho[lo];
+            rho[lo] = rho[hi];
+        }
+    } else {
+        const int pl = top ? N : 0, so = top ? 1 : -1;
+        long om1 = AT(pl - so), op1 = AT(pl + so), om2 = AT(pl - 2 * so), op2 = AT(pl + 2 * so),
+             o0 = AT(pl);
+        rho[om1] += rho[op1];
+        rho[om2] += rho[op2];
+        rho[o0] = 0;
+        rho[op1] = -rho[om1];
+        rho[op2] = -rho[om2];
+    }
+#undef AT
+}
+
+int launch_rho_fold(hipStream_t s, double *rho, const Grid &g, int b, int bc)
+{
+    const int ea = b >> 1;
+    const int ep = (ea == 0) ? 1 : 0, eq = (ea == 2) ? 1 : 2;
+    if (bc == MPIC_BC_PERIODIC && (b & 1)) return MPIC_OK;
+    long n = (long)(g.n[ep] + 5) * (g.n[eq] + 5);
+    k_rho_fold<<<nblk(n), TPB, 0, s>>>(rho, g, b, bc);
+    MPIC_CHECK(hipGetLastError());
+    return MPIC_OK;
+}
+
+/* Gauss / continuity residuals over nodes [0, n]^3,
+ * source/core/plasma_gaussTest.c:110-149.  acc = {max r_J, max r_E} as
+ * unsigned bit patterns of non-negative doubles (ordered like the values). */
+__global__ void k_gauss(Fields64 f, const double *rho1, const double *rho2, Grid g,
+                        unsigned long long *acc)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int NX = g.n[0] + 1, NY = g.n[1] + 1, NZ = g.n[2] + 1;
+    if (t >= (long)NX * NY * NZ) return;
+    int k = (int)(t % NZ);
+    long r = t / NZ;
+    int j = (int)(r % NY), i = (int)(r / NY);
+    long o = gidx(g, i, j, k);
+    const double h1 = g.h[0], h2 = g.h[1], h3 = g.h[2];
+    double t0 = (rho2[o] - rho1[o]) / g.tau + (f.J[0][gidx(g, i + 1, j, k)] - f.J[0][o]) / h1 +
+                (f.J[1][gidx(g, i, j + 1, k)] - f.J[1][o]) / h2 +
+                (f.J[2][gidx(g, i, j, k + 1)] - f.J[2][o]) / h3;
+    double t1 = (f.E[0][gidx(g, i + 1, j, k)] - f.E[0][o]) / h1 +
+                (f.E[1][gidx(g, i, j + 1, k)] - f.E[1][o]) / h2 +
+                (f.E[2][gidx(g, i, j, k + 1)] - f.E[2][o]) / h3 - rho2[o];
+    t0 = fabs(t0);
+    t1 = fabs(t1);
+    atomicMax(acc + 0, (unsigned long long)__double_as_longlong(t0));
+    atomicMax(acc + 1, (unsigned long long)__double_as_longlong(t1));
+}
+
+int launch_gauss(hipStream_t s, const Fields64 &f, const double *rho1, const double *rho2,
+                 const Grid &g, double *acc)
+{
+    long n = (long)(g.n[0] + 1) * (g.n[1] + 1) * (g.n[2] + 1);
+    k_gauss<<<nblk(n), TPB, 0, s>>>(f, rho1, rho2, g, (unsigned long long *)acc);
+    MPIC_CHECK(hipGetLastError());
+    return MPIC_OK;
+}

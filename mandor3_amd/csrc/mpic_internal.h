@@ -1,0 +1,124 @@
+/*
+ * mpic_internal.h -- device data layout and launch helpers of libmpic.
+ *
+ * HBM layout of one run (whole domain on one GPU):
+ *   fields   structure-of-arrays, one array per component, on a padded grid
+ *            of (n+5)^3 nodes covering node indices [-2, n+2] per axis
+ *            (k fastest).  E/H of the reference carry one ghost layer and J
+ *            two (source/core/main.c:109-115); one common padding keeps a
+ *            single index formula for every kernel.
+ *   caps     Hc/Ec: H^{n+1/2} and E^{n+1} of the boundary layer, computed
+ *            ahead of the interior exactly as the reference's boundary caps
+ *            (source/core/em_caps.c:405-725), stored on the same padded grid.
+ *   markers  structure-of-arrays (FP64 plain form): x y z ux uy uz (double),
+ *            w qm (float), id (int32).  A deleted marker has qm == 0.
+ */
+#ifndef MPIC_INTERNAL_H
+#define MPIC_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "mpic.h"
+
+#define MPIC_GHOST 2
+
+struct Grid {
+    int    n[3];
+    int    GY, GZ;          /* padded extents along y and z */
+    long   ncell;           /* padded node count */
+    double h[3];
+    double tau;
+};
+
+__host__ __device__ static inline long gidx(const Grid &g, int i, int j, int k)
+{
+    return ((long)(i + MPIC_GHOST) * g.GY + (j + MPIC_GHOST)) * g.GZ + (k + MPIC_GHOST);
+}
+
+struct Box {
+    int lo[3], hi[3];
+};
+
+/* Up to 6 disjoint boxes enumerated by one launch. */
+struct BoxSet {
+    int  n;
+    Box  b[6];
+    long off[7];            /* prefix sums of the box volumes */
+};
+
+struct Fields64 {
+    double *E[3], *H[3], *J[3], *Hc[3], *Ec[3];
+};
+
+struct Markers64 {
+    double  *x, *y, *z, *ux, *uy, *uz;
+    float   *w, *qm;
+    int32_t *id;
+};
+
+/* One wall of the boundary layer, applied to a 3-component array set. */
+enum FaceKind {
+    FACE_PERIODIC_H = 0,    /* X(0) = X(N), X(N+1) = X(1) */
+    FACE_PERIODIC_E = 1,    /* X(N) = X(0), X(N+1) = X(1) */
+    FACE_WALL_H     = 2,    /* mirror (zero normal) / Mur: tangential copied out */
+    FACE_MIRROR_E   = 3,
+    FACE_MUR_E      = 4,
+    FACE_MUR_INIT   = 5
+};
+
+struct FaceOp {
+    int     kind, ea, top, N, zeroN;
+    int     plo, phi, qlo, qhi;  /* transverse extents along ep = (ea+1)%3, eq = (ea+2)%3 */
+    double  alpha;
+    double *buf;
+    double *X[3];
+};
+
+struct PushConsts {
+    double h[3], tau;
+    double L[3];
+    double shell1[3], shell2[3];
+    int    bc[6];
+};
+
+#define MPIC_CHECK(call)                                                    \
+    do {                                                                    \
+        hipError_t e_ = (call);                                             \
+        if (e_ != hipSuccess) {                                             \
+            mpic_set_error("%s:%d %s: %s", __FILE__, __LINE__, #call,      \
+                           hipGetErrorString(e_));                          \
+            return MPIC_ERR_DEVICE;                                         \
+        }                                                                   \
+    } while (0)
+
+void mpic_set_error(const char *fmt, ...);
+
+/* Launchers (mpic_fdtd.hip, mpic_push64.hip). */
+int launch_cap_H(hipStream_t s, const Fields64 &f, const Grid &g, const BoxSet &bs);
+int launch_cap_E(hipStream_t s, const Fields64 &f, const Grid &g, const BoxSet &bs);
+int launch_face(hipStream_t s, const Grid &g, const FaceOp &op);
+int launch_h_update(hipStream_t s, const Fields64 &f, const Grid &g,
+                    const BoxSet &flush, int full);
+int launch_e_update(hipStream_t s, const Fields64 &f, const Grid &g,
+                    const BoxSet &flush, int interior, int do_flush);
+int launch_e_tauJ(hipStream_t s, const Fields64 &f, const Grid &g);
+int launch_laser(hipStream_t s, double *Fy, double *Fz, const Grid &g,
+                 const double *map, int pos, int JMIN, int KMIN, int KMAX,
+                 int jmin, int jmax, int kmin, int kmax, double A, double cs,
+                 double sn);
+int launch_jfold(hipStream_t s, const Fields64 &f, const Grid &g, int b, int bc);
+int launch_energy(hipStream_t s, const Fields64 &f, const Grid &g, double *acc);
+int launch_zero(hipStream_t s, double *p, long n);
+
+int launch_push64(hipStream_t s, const Markers64 &m, long n, const Fields64 &f,
+                  const Grid &g, const PushConsts &c, double *Wacc, int mode, unsigned *err);
+int launch_rho64(hipStream_t s, const Markers64 &m, long n, double *rho,
+                 const Grid &g);
+int launch_rho_fold(hipStream_t s, double *rho, const Grid &g, int b, int bc);
+int launch_gauss(hipStream_t s, const Fields64 &f, const double *rho1,
+                 const double *rho2, const Grid &g, double *acc);
+int launch_cells64(hipStream_t s, const Markers64 &m, long n, const Grid &g,
+                   int32_t *cell);
+
+#endif

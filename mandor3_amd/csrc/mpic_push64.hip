@@ -1,0 +1,343 @@
+/*
+ * mpic_push64.hip -- FP64 plain-form marker push and current deposit.
+ *
+ * One thread per marker: trilinear gather of E and H on the staggered Yee
+ * grid (source/misc_PIC.h:18-126), relativistic Boris rotation
+ * (source/core/plasma.c:115-171), charge-conserving deposit of the straight
+ * trajectory split at cell faces (source/core/plasma_currentKernel.c:20-279)
+ * and the particle walls (source/core/plasma_walls.c:27-104).  Built with
+ * -ffp-contract=off so each marker's arithmetic is the reference's, bit for
+ * bit; J is accumulated with FP64 atomics, so only the summation order of J
+ * differs from the CPU path.
+ *
+ * Roofline: HBM-bound.  Algorithmic bytes per marker-step: read 56 B +
+ * write 52 B of SoA marker data (x y z ux uy uz, w qm, id read once), plus
+ * per cell 48 B of E/H reads and 48 B of J read-modify-write divided by the
+ * markers per cell.
+ */
+#include "mpic_internal.h"
+
+#define TPB 256
+
+#define DBL_TO_INT(x) ((int)((x) + 128.0) - 128)
+
+struct F64View {
+    const double *E[3], *H[3];
+    double *J[3];
+    Grid g;
+};
+
+__device__ static inline double gather(const double *F, const Grid &g, int i, double sx, int j,
+                                       double sy, int k, double sz)
+{
+    return F[gidx(g, i, j, k)] * (1 - sx) * ((1 - sy) * (1 - sz)) +
+           F[gidx(g, i + 1, j, k)] * sx * ((1 - sy) * (1 - sz)) +
+           F[gidx(g, i, j + 1, k)] * (1 - sx) * (sy * (1 - sz)) +
+           F[gidx(g, i + 1, j + 1, k)] * sx * (sy * (1 - sz)) +
+           F[gidx(g, i, j, k + 1)] * (1 - sx) * ((1 - sy) * sz) +
+           F[gidx(g, i + 1, j, k + 1)] * sx * ((1 - sy) * sz) +
+           F[gidx(g, i, j + 1, k + 1)] * (1 - sx) * (sy * sz) +
+           F[gidx(g, i + 1, j + 1, k + 1)] * sx * (sy * sz);
+}
+
+__device__ static inline void jadd(double *p, double v)
+{
+    unsafeAtomicAdd(p, v);
+}
+
+/* Deposit of one straight segment inside cell (i,j,k). */
+__device__ static void deposit_seg(const F64View &v, int i, int j, int k, double x1, double x2,
+                                   double y1, double y2, double z1, double z2, double rp)
+{
+    const Grid &g = v.g;
+    const double h1 = g.h[0], h2 = g.h[1], h3 = g.h[2], tau = g.tau;
+    const double dX = x2 - x1, dY = y2 - y1, dZ = z2 - z1;
+    const double tX = 0.5 * (x1 + x2) - i * h1;
+    const double tY = 0.5 * (y1 + y2) - j * h2;
+    const double tZ = 0.5 * (z1 + z2) - k * h3;
+    const double yz = dY * dZ / 12, xz = dX * dZ / 12, xy = dX * dY / 12;
+    const double ax = rp * dX / (tau * h2 * h3);
+    jadd(v.J[0] + gidx(g, i + 1, j, k), ax * ((h2 - tY) * (h3 - tZ) + yz));
+    jadd(v.J[0] + gidx(g, i + 1, j + 1, k), ax * (tY * (h3 - tZ) - yz));
+    jadd(v.J[0] + gidx(g, i + 1, j, k + 1), ax * ((h2 - tY) * tZ - yz));
+    jadd(v.J[0] + gidx(g, i + 1, j + 1, k + 1), ax * (tY * tZ + yz));
+    const double ay = rp * dY / (tau * h1 * h3);
+    jadd(v.J[1] + gidx(g, i, j + 1, k), ay * ((h1 - tX) * (h3 - tZ) + xz));
+    jadd(v.J[1] + gidx(g, i + 1, j + 1, k), ay * (tX * (h3 - tZ) - xz));
+    jadd(v.J[1] + gidx(g, i, j + 1, k + 1), ay * ((h1 - tX) * tZ - xz));
+    jadd(v.J[1] + gidx(g, i + 1, j + 1, k + 1), ay * (tX * tZ + xz));
+    const double az = rp * dZ / (tau * h1 * h2);
+    jadd(v.J[2] + gidx(g, i, j, k + 1), az * ((h1 - tX) * (h2 - tY) + xy));
+    jadd(v.J[2] + gidx(g, i + 1, j, k + 1), az * (tX * (h2 - tY) - xy));
+    jadd(v.J[2] + gidx(g, i, j + 1, k + 1), az * ((h1 - tX) * tY - xy));
+    jadd(v.J[2] + gidx(g, i + 1, j + 1, k + 1), az * (tX * tY + xy));
+}
+
+/* Two or three face crossings, source/core/plasma_currentKernel.c:156-279. */
+__device__ static void deposit_multi(const F64View &v, int i1, int j1, int k1, int i2, int j2,
+                                     int k2, double x1, double y1, double z1, double x2,
+                                     double y2, double z2, double rp)
+{
+    const double h1 = v.g.h[0], h2 = v.g.h[1], h3 = v.g.h[2];
+    const int di = abs(i2 - i1), dj = abs(j2 - j1), dk = abs(k2 - k1);
+    if (di + dj + dk == 2) {
+        int i = i1, j = j1, k = k1;
+        const double sgX = di * ((i1 + i2 + 1) * 0.5 * h1 - x1) / (x2 - x1 + ((1 - di) << 16));
+        const double sgY = dj * ((j1 + j2 + 1) * 0.5 * h2 - y1) / (y2 - y1 + ((1 - dj) << 16));
+        const double sgZ = dk * ((k1 + k2 + 1) * 0.5 * h3 - z1) / (z2 - z1 + ((1 - dk) << 16));
+        double k1s = sgX + (1 - di) * sgY;
+        double k2s = sgZ + (1 - dk) * sgY;
+        int dI0[3] = {di * (i2 - i1), (1 - di) * (j2 - j1), 0};
+        int dI1[3] = {0, (1 - dk) * (j2 - j1), dk * (k2 - k1)};
+        int st = 0;
+        if (k1s > k2s) {
+            double t = k1s;
+            k1s = k2s;
+            k2s = t;
+            st = 1;
+        }
+        const double xA = x1 + k1s * (x2 - x1), yA = y1 + k1s * (y2 - y1), zA = z1 + k1s * (z2 - z1);
+        const double xB = x1 + k2s * (x2 - x1), yB = y1 + k2s * (y2 - y1), zB = z1 + k2s * (z2 - z1);
+        const int *fst = st ? dI1 : dI0, *snd = st ? dI0 : dI1;
+        deposit_seg(v, i, j, k, x1, xA, y1, yA, z1, zA, rp);
+        i += fst[0];
+        j += fst[1];
+        k += fst[2];
+        deposit_seg(v, i, j, k, xA, xB, yA, yB, zA, zB, rp);
+        i += snd[0];
+        j += snd[1];
+        k += snd[2];
+        deposit_seg(v, i, j, k, xB, x2, yB, y2, zB, z2, rp);
+        return;
+    }
+    const double xC = (i1 + i2 + 1) * 0.5 * h1, yC = (j1 + j2 + 1) * 0.5 * h2,
+                 zC = (k1 + k2 + 1) * 0.5 * h3;
+    double sg[3] = {(xC - x1) / (x2 - x1), (yC - y1) / (y2 - y1), (zC - z1) / (z2 - z1)};
+    double rc[3][3];
+    rc[0][0] = xC;
+    rc[0][1] = y1 + sg[0] * (y2 - y1);
+    rc[0][2] = z1 + sg[0] * (z2 - z1);
+    rc[1][0] = x1 + sg[1] * (x2 - x1);
+    rc[1][1] = yC;
+    rc[1][2] = z1 + sg[1] * (z2 - z1);
+    rc[2][0] = x1 + sg[2] * (x2 - x1);
+    rc[2][1] = y1 + sg[2] * (y2 - y1);
+    rc[2][2] = zC;
+    const int dI[3] = {i2 - i1, j2 - j1, k2 - k1};
+    int s1 = 0, s2 = 1, s3 = 2;
+    double a1 = sg[0], a2 = sg[1], a3 = sg[2];
+    if (a1 > a2) { int t = s1; s1 = s2; s2 = t; double d = a1; a1 = a2; a2 = d; }
+    if (a1 > a3) { int t = s1; s1 = s3; s3 = t; double d = a1; a1 = a3; a3 = d; }
+    if (a2 > a3) { int t = s2; s2 = s3; s3 = t; double d = a2; a2 = a3; a3 = d; }
+    int ii[3] = {i1, j1, k1};
+    deposit_seg(v, ii[0], ii[1], ii[2], x1, rc[s1][0], y1, rc[s1][1], z1, rc[s1][2], rp);
+    ii[s1] += dI[s1];
+    deposit_seg(v, ii[0], ii[1], ii[2], rc[s1][0], rc[s2][0], rc[s1][1], rc[s2][1], rc[s1][2],
+                rc[s2][2], rp);
+    ii[s2] += dI[s2];
+    deposit_seg(v, ii[0], ii[1], ii[2], rc[s2][0], rc[s3][0], rc[s2][1], rc[s3][1], rc[s2][2],
+                rc[s3][2], rp);
+    deposit_seg(v, i2, j2, k2, rc[s3][0], x2, rc[s3][1], y2, rc[s3][2], z2, rp);
+}
+
+/* Particle walls in the reference order xMin yMin zMin xMax yMax zMax.
+ * Returns 0 when an open wall deletes the marker. */
+__device__ static inline int walls(const PushConsts &c, double *r, double *u)
+{
+    for (int b = 0; b < 6; ++b) {
+        const int ax = b % 3, top = b / 3, bc = c.bc[b];
+        const double bnd = top ? c.shell2[ax] : c.shell1[ax];
+        if (bc == MPIC_BC_PERIODIC) {
+            if (top ? (r[ax] >= bnd) : (r[ax] <= bnd)) r[ax] += top ? -c.L[ax] : c.L[ax];
+        } else if (bc == MPIC_BC_MIRROR) {
+            if (top ? (r[ax] >= bnd) : (r[ax] <= bnd)) {
+                r[ax] = 2 * bnd - r[ax];
+                u[ax] *= -1;
+            }
+        } else if (bc == MPIC_BC_OPEN) {
+            const double lim = top ? bnd - c.h[ax] : bnd + c.h[ax];
+            if (top ? (r[ax] >= lim) : (r[ax] <= lim)) return 0;
+        }
+    }
+    return 1;
+}
+
+/* mode 0: full push; mode 1: walls only (plasma_init).  A marker whose
+ * cell lies outside [-1, n] (a state the reference never produces) is
+ * deleted and flagged in *err instead of touching memory off the grid. */
+__global__ void k_push64(Markers64 m, long n, F64View v, PushConsts c, double *Wacc, int mode,
+                         unsigned *err)
+{
+    __shared__ double sw[3][TPB];
+    long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double wx = 0, wy = 0, wz = 0;
+    if (p < n && m.qm[p] != 0.0f) {
+        double r[3] = {m.x[p], m.y[p], m.z[p]};
+        double u[3] = {m.ux[p], m.uy[p], m.uz[p]};
+        if (mode == 0) {
+            const Grid &g = v.g;
+            const double h1 = g.h[0], h2 = g.h[1], h3 = g.h[2], tau = g.tau;
+            const double x1 = r[0], y1 = r[1], z1 = r[2];
+            const float rhof = m.w[p], qmf = m.qm[p];
+            double sx, sy, sz, sx_, sy_, sz_;
+            int i, j, k, i_, j_, k_;
+            sx = x1 / h1; sx_ = sx + 0.5; i = DBL_TO_INT(sx); i_ = DBL_TO_INT(sx_); sx -= i; sx_ -= i_;
+            sy = y1 / h2; sy_ = sy + 0.5; j = DBL_TO_INT(sy); j_ = DBL_TO_INT(sy_); sy -= j; sy_ -= j_;
+            sz = z1 / h3; sz_ = sz + 0.5; k = DBL_TO_INT(sz); k_ = DBL_TO_INT(sz_); sz -= k; sz_ -= k_;
+            if (!(x1 / h1 > -1.0 && x1 / h1 < g.n[0] + 1.0 && y1 / h2 > -1.0 &&
+                  y1 / h2 < g.n[1] + 1.0 && z1 / h3 > -1.0 && z1 / h3 < g.n[2] + 1.0)) {
+                atomicOr(err, 1u);
+                m.qm[p] = 0.0f;
+                m.w[p] = 0.0f;
+                goto reduce;
+            }
+            {
+            const double Ex = gather(v.E[0], g, i_, sx_, j, sy, k, sz),
+                         Ey = gather(v.E[1], g, i, sx, j_, sy_, k, sz),
+                         Ez = gather(v.E[2], g, i, sx, j, sy, k_, sz_),
+                         Hx = gather(v.H[0], g, i, sx, j_, sy_, k_, sz_),
+                         Hy = gather(v.H[1], g, i_, sx_, j, sy, k_, sz_),
+                         Hz = gather(v.H[2], g, i_, sx_, j_, sy_, k, sz);
+            const double tauMass = 0.5 * tau * qmf, px = u[0] + tauMass * Ex,
+                         py = u[1] + tauMass * Ey, pz = u[2] + tauMass * Ez,
+                         gamma = sqrt(1 + px * px + py * py + pz * pz);
+            const double W = (gamma - 1.0) * rhof / (px * px + py * py + pz * pz + 1e-19) * h1 * h2 *
+                             h3 / qmf;
+            wx = W * px * px;
+            wy = W * py * py;
+            wz = W * pz * pz;
+            const double tauGamma = tauMass / gamma;
+            const double aHx = tauGamma * Hx, aHy = tauGamma * Hy, aHz = tauGamma * Hz;
+            const double uu = px + py * aHz - pz * aHy, vv = py + pz * aHx - px * aHz,
+                         ww = pz + px * aHy - py * aHx;
+            const double bxx = aHx * aHx, byy = aHy * aHy, bzz = aHz * aHz, bxy = aHx * aHy,
+                         byz = aHy * aHz, bxz = aHz * aHx;
+            const double delta = 1.0 / (1 + bxx + byy + bzz);
+            u[0] = tauMass * Ex + ((1 + bxx) * uu + (bxy + aHz) * vv + (bxz - aHy) * ww) * delta;
+            u[1] = tauMass * Ey + ((bxy - aHz) * uu + (1 + byy) * vv + (byz + aHx) * ww) * delta;
+            u[2] = tauMass * Ez + ((bxz + aHy) * uu + (byz - aHx) * vv + (1 + bzz) * ww) * delta;
+            const double tg = tau / sqrt(1 + u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+            const double x2 = x1 + u[0] * tg, y2 = y1 + u[1] * tg, z2 = z1 + u[2] * tg;
+            int i2 = DBL_TO_INT(x2 / h1), j2 = DBL_TO_INT(y2 / h2), k2 = DBL_TO_INT(z2 / h3);
+            int di = abs(i2 - i), dj = abs(j2 - j), dk = abs(k2 - k);
+            if (fabs(x2 - x1) < 1e-10 * h1) { di = 0; i2 = i; }
+            if (fabs(y2 - y1) < 1e-10 * h2) { dj = 0; j2 = j; }
+            if (fabs(z2 - z1) < 1e-10 * h3) { dk = 0; k2 = k; }
+            const double rp = rhof;
+            if (di + dj + dk == 0) {
+                deposit_seg(v, i, j, k, x1, x2, y1, y2, z1, z2, rp);
+            } else if (di + dj + dk == 1) {
+                const double alpha = dk * ((k + k2 + 1) * 0.5 * h3 - z1) / (z2 - z1 + ((1 - dk) << 16))
+                                   + dj * ((j + j2 + 1) * 0.5 * h2 - y1) / (y2 - y1 + ((1 - dj) << 16))
+                                   + di * ((i + i2 + 1) * 0.5 * h1 - x1) / (x2 - x1 + ((1 - di) << 16));
+                const double xC = x1 + alpha * (x2 - x1), yC = y1 + alpha * (y2 - y1),
+                             zC = z1 + alpha * (z2 - z1);
+                deposit_seg(v, i, j, k, x1, xC, y1, yC, z1, zC, rp);
+                deposit_seg(v, i2, j2, k2, xC, x2, yC, y2, zC, z2, rp);
+            } else {
+                deposit_multi(v, i, j, k, i2, j2, k2, x1, y1, z1, x2, y2, z2, rp);
+            }
+            r[0] = x2;
+            r[1] = y2;
+            r[2] = z2;
+            }
+        }
+        if (!walls(c, r, u)) {
+            m.qm[p] = 0.0f;
+            m.w[p] = 0.0f;
+        } else {
+            m.x[p] = r[0];
+            m.y[p] = r[1];
+            m.z[p] = r[2];
+            m.ux[p] = u[0];
+            m.uy[p] = u[1];
+            m.uz[p] = u[2];
+        }
+    }
+reduce:
+    if (mode != 0) return;
+    sw[0][threadIdx.x] = wx;
+    sw[1][threadIdx.x] = wy;
+    sw[2][threadIdx.x] = wz;
+    __syncthreads();
+    for (int s = TPB / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            sw[0][threadIdx.x] += sw[0][threadIdx.x + s];
+            sw[1][threadIdx.x] += sw[1][threadIdx.x + s];
+            sw[2][threadIdx.x] += sw[2][threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        atomicAdd(Wacc + 0, sw[0][0]);
+        atomicAdd(Wacc + 1, sw[1][0]);
+        atomicAdd(Wacc + 2, sw[2][0]);
+    }
+}
+
+int launch_push64(hipStream_t s, const Markers64 &m, long n, const Fields64 &f, const Grid &g,
+                  const PushConsts &c, double *Wacc, int mode, unsigned *err)
+{
+    if (n <= 0) return MPIC_OK;
+    F64View v;
+    for (int a = 0; a < 3; ++a) {
+        v.E[a] = f.E[a];
+        v.H[a] = f.H[a];
+        v.J[a] = f.J[a];
+    }
+    v.g = g;
+    k_push64<<<(unsigned)((n + TPB - 1) / TPB), TPB, 0, s>>>(m, n, v, c, Wacc, mode, err);
+    MPIC_CHECK(hipGetLastError());
+    return MPIC_OK;
+}
+
+/* CIC charge density of the live markers, source/core/plasma_rho.c:160-173. */
+__global__ void k_rho64(Markers64 m, long n, double *rho, Grid g)
+{
+    long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n || m.qm[p] == 0.0f) return;
+    int i, j, k;
+    double sx, sy, sz;
+    sx = m.x[p] / g.h[0]; i = DBL_TO_INT(sx); sx -= i;
+    sy = m.y[p] / g.h[1]; j = DBL_TO_INT(sy); sy -= j;
+    sz = m.z[p] / g.h[2]; k = DBL_TO_INT(sz); sz -= k;
+    const double val = m.w[p];
+    unsafeAtomicAdd(rho + gidx(g, i, j, k), (1 - sx) * ((1 - sy) * ((1 - sz) * val)));
+    unsafeAtomicAdd(rho + gidx(g, i + 1, j, k), sx * ((1 - sy) * ((1 - sz) * val)));
+    unsafeAtomicAdd(rho + gidx(g, i, j + 1, k), (1 - sx) * (sy * ((1 - sz) * val)));
+    unsafeAtomicAdd(rho + gidx(g, i + 1, j + 1, k), sx * (sy * ((1 - sz) * val)));
+    unsafeAtomicAdd(rho + gidx(g, i, j, k + 1), (1 - sx) * ((1 - sy) * (sz * val)));
+    unsafeAtomicAdd(rho + gidx(g, i + 1, j, k + 1), sx * ((1 - sy) * (sz * val)));
+    unsafeAtomicAdd(rho + gidx(g, i, j + 1, k + 1), (1 - sx) * (sy * (sz * val)));
+    unsafeAtomicAdd(rho + gidx(g, i + 1, j + 1, k + 1), sx * (sy * (sz * val)));
+}
+
+int launch_rho64(hipStream_t s, const Markers64 &m, long n, double *rho, const Grid &g)
+{
+    if (n <= 0) return MPIC_OK;
+    k_rho64<<<(unsigned)((n + TPB - 1) / TPB), TPB, 0, s>>>(m, n, rho, g);
+    MPIC_CHECK(hipGetLastError());
+    return MPIC_OK;
+}
+
+/* Linear cell index of floor(x/h) for every marker (-1 for deleted ones). */
+__global__ void k_cells64(Markers64 m, long n, Grid g, int32_t *cell)
+{
+    long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    if (m.qm[p] == 0.0f) {
+        cell[p] = -1;
+        return;
+    }
+    int i = DBL_TO_INT(m.x[p] / g.h[0]), j = DBL_TO_INT(m.y[p] / g.h[1]),
+        k = DBL_TO_INT(m.z[p] / g.h[2]);
+    cell[p] = (i * g.n[1] + j) * g.n[2] + k;
+}
+
+int launch_cells64(hipStream_t s, const Markers64 &m, long n, const Grid &g, int32_t *cell)
+{
+    if (n <= 0) return MPIC_OK;
+    k_cells64<<<(unsigned)((n + TPB - 1) / TPB), TPB, 0, s>>>(m, n, g, cell);
+    MPIC_CHECK(hipGetLastError());
+    return MPIC_OK;
+}

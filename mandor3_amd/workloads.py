@@ -1,0 +1,136 @@
+"""Synthetic initial states of the benchmark configurations.
+
+Laser-foil (256 x 128 x 128 cells at dx = lambda/20, Mur walls, a0 = 3
+Gaussian beam of waist 3 lambda and 10-period intensity FWHM injected on the
+x = 20 plane, fully ionised 20 n_c foil 0.5 lambda thick at x = 128 cells,
+27 markers per cell per species, electrons Maxwellian vth = 0.01 c, cold
+protons) and the 32^3 periodic two-stream box.  Code units as the
+reference: lengths in lambda, time in lambda/c, momenta gamma*v/c, critical
+density rho = 4 pi^2 (source/misc_units.c:96-115).
+"""
+import math
+
+import numpy as np
+
+from . import BC_OPEN, BC_PERIODIC, MARKER_DTYPE
+
+RHO_CRIT = 4 * math.pi ** 2
+QM_PROTON = 1.0 / 1836.0
+
+
+def courant_tau(h, frac=0.45):
+    """tau = frac * tau_max, the negative-tau convention of
+    source/setup/tag_mesh.c:59-60."""
+    return frac / math.sqrt(sum(1.0 / (v * v) for v in h))
+
+
+def gaussian_map(n, h, plane, a0, waist, focus):
+    """Incident-field map of a y-polarised paraxial Gaussian beam for the
+    one-plane soft source (E map: Ey^ at x = (I-1)h driving Hz on plane I;
+    H map: -Hz^ at x = (I-1/2)h driving Ey on plane I-1), j, k in [5, n-5]."""
+    I = plane
+    k0 = 2 * math.pi
+    w0 = waist
+    xf, yf, zf = focus
+    zr = math.pi * w0 * w0
+    js = np.arange(5, n[1] - 5 + 1)
+    ks = np.arange(5, n[2] - 5 + 1)
+    jj, kk = np.meshgrid(js, ks, indexing="ij")
+
+    def beam(x, y, z):
+        xi = x - xf
+        w = w0 * math.sqrt(1 + (xi / zr) ** 2)
+        invR = xi / (xi * xi + zr * zr)
+        gouy = math.atan2(xi, zr)
+        r2 = (y - yf) ** 2 + (z - zf) ** 2
+        amp = (w0 / w) * np.exp(-r2 / (w * w))
+        th = -k0 * r2 * invR / 2 + gouy - k0 * (x - I * h[0])
+        return amp * np.cos(th), -amp * np.sin(th)
+
+    mapE = np.zeros(jj.shape + (4,))
+    mapH = np.zeros(jj.shape + (4,))
+    c, s = beam((I - 1) * h[0], (jj - 0.5) * h[1], kk * h[2])
+    mapE[..., 0], mapE[..., 1] = c, s
+    c, s = beam((I - 0.5) * h[0], (jj - 0.5) * h[1], kk * h[2])
+    mapH[..., 2], mapH[..., 3] = -c, -s
+    return np.ascontiguousarray(np.stack([mapE.reshape(-1, 4), mapH.reshape(-1, 4)])).reshape(-1)
+
+
+def laser_params(a0, plane, h, focus_x, fwhm_periods, peak_time):
+    """(A, T0, delay, dt, phi0, forward, plane) of the soft source; dt from
+    the intensity FWHM as source/scpic/em_mirror.c:305-308 does."""
+    dt = fwhm_periods * math.sqrt(2.0 / math.log(2))
+    return (2 * math.pi * a0, peak_time, focus_x - plane * h[0], dt, 0.0, 1, plane)
+
+
+def _quiet_cells(cells, sub, h, rng):
+    """Markers at the sub-cell centres of every listed cell (the placement
+    of source/setup/tag_plasma.c:92-110)."""
+    sx, sy, sz = sub
+    off = np.stack(np.meshgrid((np.arange(sx) + 0.5) / sx, (np.arange(sy) + 0.5) / sy,
+                               (np.arange(sz) + 0.5) / sz, indexing="ij"), -1).reshape(-1, 3)
+    ppc = off.shape[0]
+    pos = (cells[:, None, :] + off[None, :, :]).reshape(-1, 3) * np.asarray(h)
+    return pos, ppc
+
+
+def laser_foil(scale=1.0, seed=12345, ppc_sub=(3, 3, 3)):
+    """The 256x128x128 laser-foil state.  scale < 1 shrinks the transverse
+    box (tests); markers are generated for the whole foil slab."""
+    n = (256, max(16, int(round(128 * scale))), max(16, int(round(128 * scale))))
+    h = (0.05, 0.05, 0.05)
+    L = tuple(n[a] * h[a] for a in range(3))
+    tau = courant_tau(h)
+    rng = np.random.default_rng(seed)
+    i0, i1 = 123, 133                      # 10 cells = 0.5 lambda centred at 128
+    ii, jj, kk = np.meshgrid(np.arange(i0, i1), np.arange(n[1]), np.arange(n[2]), indexing="ij")
+    cells = np.stack([ii.ravel(), jj.ravel(), kk.ravel()], -1).astype(np.float64)
+    pos, ppc = _quiet_cells(cells, ppc_sub, h, rng)
+    N = pos.shape[0]
+    M = np.zeros(2 * N, dtype=MARKER_DTYPE)
+    w = 20.0 * RHO_CRIT / ppc
+    for s, (q, qm, vth) in enumerate(((-w, -1.0, 0.01), (w, QM_PROTON, 0.0))):
+        sl = slice(s * N, (s + 1) * N)
+        M["x"][sl], M["y"][sl], M["z"][sl] = pos[:, 0], pos[:, 1], pos[:, 2]
+        if vth > 0:
+            u = rng.normal(0.0, vth, size=(N, 3))
+            M["vx"][sl], M["vy"][sl], M["vz"][sl] = u[:, 0], u[:, 1], u[:, 2]
+        M["rho"][sl] = q
+        M["qDivM"][sl] = qm
+    M["id"] = np.arange(1, 2 * N + 1)
+    plane = 20
+    focus_x = 128 * h[0]
+    lp = laser_params(3.0, plane, h, focus_x, 10.0, focus_x - plane * h[0] + 2.0)
+    lmap = gaussian_map(n, h, plane, 3.0, 3.0, (focus_x, L[1] / 2, L[2] / 2))
+    return dict(name="laser-foil 256x128x128 h=lambda/20, 20 n_c foil 0.5 lambda, "
+                     "27 ppc x 2 species, a0=3 w0=3 lambda, Mur",
+                n=n, h=h, L=L, tau=tau, bc=(BC_OPEN,) * 6, markers=M,
+                laser=(lp, lmap), ppc=2 * ppc, plasma_cells=cells.shape[0])
+
+
+def two_stream_box(n=32, seed=7):
+    """32^3 periodic box, h = 0.05, electrons (vth 0.05 + beams +-0.2 c
+    along x) and cold ions, 8 markers per cell each, n_e = n_i = 1 n_c."""
+    h = (0.05, 0.05, 0.05)
+    nn = (n, n, n)
+    L = tuple(nn[a] * h[a] for a in range(3))
+    rng = np.random.default_rng(seed)
+    ii, jj, kk = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    cells = np.stack([ii.ravel(), jj.ravel(), kk.ravel()], -1).astype(np.float64)
+    N = cells.shape[0] * 8
+    M = np.zeros(2 * N, dtype=MARKER_DTYPE)
+    w = RHO_CRIT / 8
+    for s, (q, qm) in enumerate(((-w, -1.0), (w, QM_PROTON))):
+        sl = slice(s * N, (s + 1) * N)
+        pos = (np.repeat(cells, 8, axis=0) + rng.random((N, 3))) * 0.05
+        M["x"][sl], M["y"][sl], M["z"][sl] = pos[:, 0], pos[:, 1], pos[:, 2]
+        if s == 0:
+            u = rng.normal(0.0, 0.05, size=(N, 3))
+            u[:, 0] += np.where(np.arange(N) % 2 == 0, 0.2, -0.2)
+            M["vx"][sl], M["vy"][sl], M["vz"][sl] = u[:, 0], u[:, 1], u[:, 2]
+        M["rho"][sl] = q
+        M["qDivM"][sl] = qm
+    M["id"] = np.arange(1, 2 * N + 1)
+    return dict(name="two-stream 32^3 periodic h=0.05, 8 ppc x 2 species",
+                n=nn, h=h, L=L, tau=courant_tau(h), bc=(BC_PERIODIC,) * 6, markers=M,
+                laser=None, ppc=16, plasma_cells=cells.shape[0])

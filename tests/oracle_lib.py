@@ -99,6 +99,9 @@ class Oracle:
         M = np.ascontiguousarray(M, dtype=MARKER_DTYPE)
         assert lib().opic_set_markers(self.s, _ptr(M), M.size, 64) == 0
 
+    def marker_count(self):
+        return int(lib().opic_marker_count(self.s))
+
     def markers(self):
         n = lib().opic_marker_count(self.s)
         M = np.zeros(n, dtype=MARKER_DTYPE)
