@@ -32,6 +32,8 @@ def parse():
     p.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
     p.add_argument("--workload", default="laser-foil", choices=["laser-foil", "two-stream"])
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--untiled", action="store_true",
+                   help="unsorted one-thread-per-marker push (J atomics straight to HBM)")
     p.add_argument("--cpu-seconds", type=float, default=12.0)
     return p.parse_args()
 
@@ -87,6 +89,8 @@ def main():
         print("bench.py: this build runs on one GPU only (--gpus %d requested); "
               "multi-GPU sharding is design-only" % a.gpus, file=sys.stderr)
         sys.exit(2)
+    if a.untiled:
+        os.environ["MPIC_UNTILED"] = "1"
     import mandor3_amd as ma
     from mandor3_amd import workloads
     if ma.device_count() < 1:
@@ -130,7 +134,8 @@ def main():
         "dtype": "f64" if a.precision == "fp64" else "f32+f64",
         "data": "synthetic",
         "config": {"workload": wl["name"], "grid": list(wl["n"]), "markers": int(live),
-                   "precision": a.precision, "cell_updates_per_s": ncells * a.steps / wall,
+                   "precision": a.precision, "tiled": not a.untiled,
+                   "cell_updates_per_s": ncells * a.steps / wall,
                    "push_ms_per_step": avg_push_ms, "field_ms_per_step": field_ms / a.steps},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},

@@ -40,47 +40,53 @@ __device__ static inline bool box_decode(const BoxSet &bs, long t, int &i, int &
 }
 
 /* Cap H^{n+1/2} = H + c3*(dEy/dz) - c2*(dEz/dy)..., source/core/em_caps.c:418-439. */
-__global__ void k_cap_H(Fields64 f, Grid g, BoxSet bs, double c1, double c2, double c3)
+template <class T>
+__global__ void k_cap_H(FieldsT<T> f, Grid g, BoxSet bs, T c1, T c2, T c3)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     int i, j, k;
     if (!box_decode(bs, t, i, j, k)) return;
     long o = gidx(g, i, j, k);
     long oi = gidx(g, i - 1, j, k), oj = gidx(g, i, j - 1, k), ok = gidx(g, i, j, k - 1);
-    const double *Ex = f.E[0], *Ey = f.E[1], *Ez = f.E[2];
+    const T *Ex = f.E[0], *Ey = f.E[1], *Ez = f.E[2];
     f.Hc[0][o] = f.H[0][o] + c3 * (Ey[o] - Ey[ok]) - c2 * (Ez[o] - Ez[oj]);
     f.Hc[1][o] = f.H[1][o] + c1 * (Ez[o] - Ez[oi]) - c3 * (Ex[o] - Ex[ok]);
     f.Hc[2][o] = f.H[2][o] + c2 * (Ex[o] - Ex[oj]) - c1 * (Ey[o] - Ey[oi]);
 }
 
 /* Cap E^{n+1} from the cap H, source/core/em_caps.c:580-599. */
-__global__ void k_cap_E(Fields64 f, Grid g, BoxSet bs, double c1, double c2, double c3)
+template <class T>
+__global__ void k_cap_E(FieldsT<T> f, Grid g, BoxSet bs, T c1, T c2, T c3)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     int i, j, k;
     if (!box_decode(bs, t, i, j, k)) return;
     long o = gidx(g, i, j, k);
     long oi = gidx(g, i + 1, j, k), oj = gidx(g, i, j + 1, k), ok = gidx(g, i, j, k + 1);
-    const double *Hx = f.Hc[0], *Hy = f.Hc[1], *Hz = f.Hc[2];
+    const T *Hx = f.Hc[0], *Hy = f.Hc[1], *Hz = f.Hc[2];
     f.Ec[0][o] = f.E[0][o] - c3 * (Hy[ok] - Hy[o]) + c2 * (Hz[oj] - Hz[o]);
     f.Ec[1][o] = f.E[1][o] - c1 * (Hz[oi] - Hz[o]) + c3 * (Hx[ok] - Hx[o]);
     f.Ec[2][o] = f.E[2][o] - c2 * (Hx[oj] - Hx[o]) + c1 * (Hy[oi] - Hy[o]);
 }
 
-int launch_cap_H(hipStream_t s, const Fields64 &f, const Grid &g, const BoxSet &bs)
+template <class T>
+int launch_cap_H(hipStream_t s, const FieldsT<T> &f, const Grid &g, const BoxSet &bs)
 {
     long n = bs.off[bs.n];
     if (n <= 0) return MPIC_OK;
-    k_cap_H<<<nblk(n), TPB, 0, s>>>(f, g, bs, g.tau / g.h[0], g.tau / g.h[1], g.tau / g.h[2]);
+    k_cap_H<T><<<nblk(n), TPB, 0, s>>>(f, g, bs, (T)(g.tau / g.h[0]), (T)(g.tau / g.h[1]),
+                                        (T)(g.tau / g.h[2]));
     MPIC_CHECK(hipGetLastError());
     return MPIC_OK;
 }
 
-int launch_cap_E(hipStream_t s, const Fields64 &f, const Grid &g, const BoxSet &bs)
+template <class T>
+int launch_cap_E(hipStream_t s, const FieldsT<T> &f, const Grid &g, const BoxSet &bs)
 {
     long n = bs.off[bs.n];
     if (n <= 0) return MPIC_OK;
-    k_cap_E<<<nblk(n), TPB, 0, s>>>(f, g, bs, g.tau / g.h[0], g.tau / g.h[1], g.tau / g.h[2]);
+    k_cap_E<T><<<nblk(n), TPB, 0, s>>>(f, g, bs, (T)(g.tau / g.h[0]), (T)(g.tau / g.h[1]),
+                                        (T)(g.tau / g.h[2]));
     MPIC_CHECK(hipGetLastError());
     return MPIC_OK;
 }
@@ -88,8 +94,11 @@ int launch_cap_E(hipStream_t s, const Fields64 &f, const Grid &g, const BoxSet &
 /* Walls of one face: periodic, mirror and first-order Mur
  * (source/core/emCap_periodic.c:12-68, emCap_mirror.c:17-72,
  * emCap_Mur.c:55-175). */
+template <class T>
 __global__ void k_face(Grid g, FaceOp op)
 {
+    T *X[3] = {(T *)op.X[0], (T *)op.X[1], (T *)op.X[2]};
+    const T alpha = (T)op.alpha;
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     int nq = op.qhi - op.qlo + 1;
     long tot = (long)(op.phi - op.plo + 1) * nq;
@@ -106,55 +115,55 @@ __global__ void k_face(Grid g, FaceOp op)
     case FACE_PERIODIC_H: {
         long o0 = AT(0), oN = AT(N), o1 = AT(1), oN1 = AT(N + 1);
         for (int c = 0; c < 3; ++c) {
-            op.X[c][o0] = op.X[c][oN];
-            op.X[c][oN1] = op.X[c][o1];
+            X[c][o0] = X[c][oN];
+            X[c][oN1] = X[c][o1];
         }
         break;
     }
     case FACE_PERIODIC_E: {
         long o0 = AT(0), oN = AT(N), o1 = AT(1), oN1 = AT(N + 1);
         for (int c = 0; c < 3; ++c) {
-            op.X[c][oN] = op.X[c][o0];
-            op.X[c][oN1] = op.X[c][o1];
+            X[c][oN] = X[c][o0];
+            X[c][oN1] = X[c][o1];
         }
         break;
     }
     case FACE_WALL_H: {
         long opl = AT(pl), oin = AT(pl + 1 - top), oex = AT(pl + top);
-        if (op.zeroN) op.X[ea][opl] = 0;
-        op.X[ep][oex] = op.X[ep][oin];
-        op.X[eq][oex] = op.X[eq][oin];
+        if (op.zeroN) X[ea][opl] = 0;
+        X[ep][oex] = X[ep][oin];
+        X[eq][oex] = X[eq][oin];
         break;
     }
     case FACE_MIRROR_E: {
         long opl = AT(pl), oin = AT(pl + 1 - top), oex = AT(pl + top);
-        op.X[ep][opl] = 0;
-        op.X[eq][opl] = 0;
-        op.X[ea][oex] = op.X[ea][oin];
+        X[ep][opl] = 0;
+        X[eq][opl] = 0;
+        X[ea][oex] = X[ea][oin];
         break;
     }
     case FACE_MUR_E: {
         long opl = AT(pl), os = AT(pl + 1 - 2 * top), oin = AT(pl + 1 - top), oex = AT(pl + top);
-        double *b = op.buf + 4 * t;
-        op.X[ea][oex] = op.X[ea][oin];
-        double vp = b[2] + op.alpha * (op.X[ep][os] - b[0]);
-        double vq = b[3] + op.alpha * (op.X[eq][os] - b[1]);
-        op.X[ep][opl] = vp;
-        op.X[eq][opl] = vq;
+        T *b = (T *)op.buf + 4 * t;
+        X[ea][oex] = X[ea][oin];
+        T vp = b[2] + alpha * (X[ep][os] - b[0]);
+        T vq = b[3] + alpha * (X[eq][os] - b[1]);
+        X[ep][opl] = vp;
+        X[eq][opl] = vq;
         b[0] = vp;
         b[1] = vq;
-        b[2] = op.X[ep][os];
-        b[3] = op.X[eq][os];
+        b[2] = X[ep][os];
+        b[3] = X[eq][os];
         break;
     }
     case FACE_MUR_INIT: {
         long opl = AT(pl), os = AT(pl + 1 - 2 * top), oin = AT(pl + 1 - top), oex = AT(pl + top);
-        double *b = op.buf + 4 * t;
-        b[0] = op.X[ep][opl];
-        b[1] = op.X[eq][opl];
-        b[2] = op.X[ep][os];
-        b[3] = op.X[eq][os];
-        op.X[ea][oex] = op.X[ea][oin];
+        T *b = (T *)op.buf + 4 * t;
+        b[0] = X[ep][opl];
+        b[1] = X[eq][opl];
+        b[2] = X[ep][os];
+        b[3] = X[eq][os];
+        X[ea][oex] = X[ea][oin];
         break;
     }
     }
@@ -165,7 +174,10 @@ int launch_face(hipStream_t s, const Grid &g, const FaceOp &op)
 {
     long n = (long)(op.phi - op.plo + 1) * (op.qhi - op.qlo + 1);
     if (n <= 0) return MPIC_OK;
-    k_face<<<nblk(n), TPB, 0, s>>>(g, op);
+    if (op.fp32)
+        k_face<float><<<nblk(n), TPB, 0, s>>>(g, op);
+    else
+        k_face<double><<<nblk(n), TPB, 0, s>>>(g, op);
     MPIC_CHECK(hipGetLastError());
     return MPIC_OK;
 }
@@ -185,8 +197,8 @@ __device__ static inline int in_boxes(const BoxSet &bs, int i, int j, int k)
  * cap layer: H = (H + Hc)/2 after the first half step (cap_flushHalfH,
  * em_caps.c:730-749) or H = Hc after the second (cap_flushH, :754-770).
  * Nodes [0, n+1]^3; interior and flush regions are disjoint. */
-__global__ void k_h_update(Fields64 f, Grid g, BoxSet fl, int full, double c1, double c2,
-                           double c3)
+template <class T>
+__global__ void k_h_update(FieldsT<T> f, Grid g, BoxSet fl, int full, T c1, T c2, T c3)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const int NX = g.n[0] + 2, NY = g.n[1] + 2, NZ = g.n[2] + 2;
@@ -197,8 +209,8 @@ __global__ void k_h_update(Fields64 f, Grid g, BoxSet fl, int full, double c1, d
     long o = gidx(g, i, j, k);
     if (i >= 3 && i <= g.n[0] - 2 && j >= 3 && j <= g.n[1] - 2 && k >= 3 && k <= g.n[2] - 2) {
         long oi = gidx(g, i - 1, j, k), oj = gidx(g, i, j - 1, k), ok = gidx(g, i, j, k - 1);
-        const double *Ex = f.E[0], *Ey = f.E[1], *Ez = f.E[2];
-        const double ex = Ex[o], ey = Ey[o], ez = Ez[o];
+        const T *Ex = f.E[0], *Ey = f.E[1], *Ez = f.E[2];
+        const T ex = Ex[o], ey = Ey[o], ez = Ez[o];
         f.H[0][o] += c3 * (ey - Ey[ok]) - c2 * (ez - Ez[oj]);
         f.H[1][o] += c1 * (ez - Ez[oi]) - c3 * (ex - Ex[ok]);
         f.H[2][o] += c2 * (ex - Ex[oj]) - c1 * (ey - Ey[oi]);
@@ -208,26 +220,29 @@ __global__ void k_h_update(Fields64 f, Grid g, BoxSet fl, int full, double c1, d
             f.H[1][o] = f.Hc[1][o];
             f.H[2][o] = f.Hc[2][o];
         } else {
-            f.H[0][o] = 0.5 * (f.H[0][o] + f.Hc[0][o]);
-            f.H[1][o] = 0.5 * (f.H[1][o] + f.Hc[1][o]);
-            f.H[2][o] = 0.5 * (f.H[2][o] + f.Hc[2][o]);
+            f.H[0][o] = (T)0.5 * (f.H[0][o] + f.Hc[0][o]);
+            f.H[1][o] = (T)0.5 * (f.H[1][o] + f.Hc[1][o]);
+            f.H[2][o] = (T)0.5 * (f.H[2][o] + f.Hc[2][o]);
         }
     }
 }
 
-int launch_h_update(hipStream_t s, const Fields64 &f, const Grid &g, const BoxSet &flush, int full)
+template <class T>
+int launch_h_update(hipStream_t s, const FieldsT<T> &f, const Grid &g, const BoxSet &flush,
+                    int full)
 {
     long n = (long)(g.n[0] + 2) * (g.n[1] + 2) * (g.n[2] + 2);
-    k_h_update<<<nblk(n), TPB, 0, s>>>(f, g, flush, full, 0.5 * g.tau / g.h[0],
-                                        0.5 * g.tau / g.h[1], 0.5 * g.tau / g.h[2]);
+    k_h_update<T><<<nblk(n), TPB, 0, s>>>(f, g, flush, full, (T)(0.5 * g.tau / g.h[0]),
+                                           (T)(0.5 * g.tau / g.h[1]), (T)(0.5 * g.tau / g.h[2]));
     MPIC_CHECK(hipGetLastError());
     return MPIC_OK;
 }
 
 /* Interior E step (source/core/em.c:134-161) and/or the flush of the cap
  * layer E = Ec (cap_flushE, em_caps.c:775-794). */
-__global__ void k_e_update(Fields64 f, Grid g, BoxSet fl, int interior, int do_flush,
-                           double c1, double c2, double c3)
+template <class T>
+__global__ void k_e_update(FieldsT<T> f, Grid g, BoxSet fl, int interior, int do_flush, T c1,
+                           T c2, T c3)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const int NX = g.n[0] + 2, NY = g.n[1] + 2, NZ = g.n[2] + 2;
@@ -239,8 +254,8 @@ __global__ void k_e_update(Fields64 f, Grid g, BoxSet fl, int interior, int do_f
     if (i >= 2 && i <= g.n[0] - 2 && j >= 2 && j <= g.n[1] - 2 && k >= 2 && k <= g.n[2] - 2) {
         if (!interior) return;
         long oi = gidx(g, i + 1, j, k), oj = gidx(g, i, j + 1, k), ok = gidx(g, i, j, k + 1);
-        const double *Hx = f.H[0], *Hy = f.H[1], *Hz = f.H[2];
-        const double hx = Hx[o], hy = Hy[o], hz = Hz[o];
+        const T *Hx = f.H[0], *Hy = f.H[1], *Hz = f.H[2];
+        const T hx = Hx[o], hy = Hy[o], hz = Hz[o];
         f.E[0][o] -= c3 * (Hy[ok] - hy) - c2 * (Hz[oj] - hz);
         f.E[1][o] -= c1 * (Hz[oi] - hz) - c3 * (Hx[ok] - hx);
         f.E[2][o] -= c2 * (Hx[oj] - hx) - c1 * (Hy[oi] - hy);
@@ -251,18 +266,20 @@ __global__ void k_e_update(Fields64 f, Grid g, BoxSet fl, int interior, int do_f
     }
 }
 
-int launch_e_update(hipStream_t s, const Fields64 &f, const Grid &g, const BoxSet &flush,
+template <class T>
+int launch_e_update(hipStream_t s, const FieldsT<T> &f, const Grid &g, const BoxSet &flush,
                     int interior, int do_flush)
 {
     long n = (long)(g.n[0] + 2) * (g.n[1] + 2) * (g.n[2] + 2);
-    k_e_update<<<nblk(n), TPB, 0, s>>>(f, g, flush, interior, do_flush, g.tau / g.h[0],
-                                        g.tau / g.h[1], g.tau / g.h[2]);
+    k_e_update<T><<<nblk(n), TPB, 0, s>>>(f, g, flush, interior, do_flush, (T)(g.tau / g.h[0]),
+                                           (T)(g.tau / g.h[1]), (T)(g.tau / g.h[2]));
     MPIC_CHECK(hipGetLastError());
     return MPIC_OK;
 }
 
 /* E -= tau*J over nodes [-1, n+1]^3, source/core/em.c:166-184. */
-__global__ void k_e_tauJ(Fields64 f, Grid g)
+template <class T>
+__global__ void k_e_tauJ(FieldsT<T> f, Grid g)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const int NX = g.n[0] + 3, NY = g.n[1] + 3, NZ = g.n[2] + 3;
@@ -272,15 +289,17 @@ __global__ void k_e_tauJ(Fields64 f, Grid g)
     int j = (int)(r % NY) - 1, i = (int)(r / NY) - 1;
     long o = gidx(g, i, j, k);
     const double tau = g.tau;
-    f.E[0][o] -= tau * f.J[0][o];
-    f.E[1][o] -= tau * f.J[1][o];
-    f.E[2][o] -= tau * f.J[2][o];
+    /* FP32 fields: the FP64 J sum is applied in FP64 and rounded once */
+    f.E[0][o] = (T)(f.E[0][o] - tau * f.J[0][o]);
+    f.E[1][o] = (T)(f.E[1][o] - tau * f.J[1][o]);
+    f.E[2][o] = (T)(f.E[2][o] - tau * f.J[2][o]);
 }
 
-int launch_e_tauJ(hipStream_t s, const Fields64 &f, const Grid &g)
+template <class T>
+int launch_e_tauJ(hipStream_t s, const FieldsT<T> &f, const Grid &g)
 {
     long n = (long)(g.n[0] + 3) * (g.n[1] + 3) * (g.n[2] + 3);
-    k_e_tauJ<<<nblk(n), TPB, 0, s>>>(f, g);
+    k_e_tauJ<T><<<nblk(n), TPB, 0, s>>>(f, g);
     MPIC_CHECK(hipGetLastError());
     return MPIC_OK;
 }
@@ -288,9 +307,10 @@ int launch_e_tauJ(hipStream_t s, const Fields64 &f, const Grid &g)
 /* One-plane laser soft source, source/scpic/em_mirror.c:667-693.  The time
  * factors (envelope, cos, sin of the carrier) are evaluated in FP64 on the
  * host once per step and passed in. */
-__global__ void k_laser(double *Fy, double *Fz, Grid g, const double *map, int pos, int JMIN,
-                        int KMIN, int KMAX, int jmin, int jmax, int kmin, int kmax, double A,
-                        double cs, double sn)
+template <class T>
+__global__ void k_laser(T *Fy, T *Fz, Grid g, const double *map, int pos, int JMIN, int KMIN,
+                        int KMAX, int jmin, int jmax, int kmin, int kmax, double A, double cs,
+                        double sn)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     int nk = kmax - kmin + 1;
@@ -299,25 +319,26 @@ __global__ void k_laser(double *Fy, double *Fz, Grid g, const double *map, int p
     long cnt = (long)(j - JMIN) * (KMAX - KMIN + 1) + k - KMIN;
     const double *w = map + 4 * cnt;
     long o = gidx(g, pos, j, k);
-    Fy[o] += A * (w[2] * cs + w[3] * sn);
-    Fz[o] += A * (w[0] * cs + w[1] * sn);
+    Fy[o] = (T)(Fy[o] + A * (w[2] * cs + w[3] * sn));
+    Fz[o] = (T)(Fz[o] + A * (w[0] * cs + w[1] * sn));
 }
 
-int launch_laser(hipStream_t s, double *Fy, double *Fz, const Grid &g, const double *map, int pos,
-                 int JMIN, int KMIN, int KMAX, int jmin, int jmax, int kmin, int kmax, double A,
-                 double cs, double sn)
+template <class T>
+int launch_laser(hipStream_t s, T *Fy, T *Fz, const Grid &g, const double *map, int pos, int JMIN,
+                 int KMIN, int KMAX, int jmin, int jmax, int kmin, int kmax, double A, double cs,
+                 double sn)
 {
     long n = (long)(jmax - jmin + 1) * (kmax - kmin + 1);
     if (jmax < jmin || kmax < kmin) return MPIC_OK;
-    k_laser<<<nblk(n), TPB, 0, s>>>(Fy, Fz, g, map, pos, JMIN, KMIN, KMAX, jmin, jmax, kmin,
-                                     kmax, A, cs, sn);
+    k_laser<T><<<nblk(n), TPB, 0, s>>>(Fy, Fz, g, map, pos, JMIN, KMIN, KMAX, jmin, jmax, kmin,
+                                        kmax, A, cs, sn);
     MPIC_CHECK(hipGetLastError());
     return MPIC_OK;
 }
 
 /* J boundary fold of one face b (order x-min, x-max, y-min, y-max, z-min,
  * z-max), source/core/plasma_currentWalls.c:132-227. */
-__global__ void k_jfold(Fields64 f, Grid g, int b, int bc)
+__global__ void k_jfold(JPtrs f, Grid g, int b, int bc)
 {
     const int ea = b >> 1, top = b & 1;
     const int ep = (ea == 0) ? 1 : 0, eq = (ea == 2) ? 1 : 2;
@@ -361,7 +382,7 @@ __global__ void k_jfold(Fields64 f, Grid g, int b, int bc)
 #undef AT
 }
 
-int launch_jfold(hipStream_t s, const Fields64 &f, const Grid &g, int b, int bc)
+int launch_jfold(hipStream_t s, const JPtrs &f, const Grid &g, int b, int bc)
 {
     const int ea = b >> 1;
     const int ep = (ea == 0) ? 1 : 0, eq = (ea == 2) ? 1 : 2;
@@ -374,7 +395,8 @@ int launch_jfold(hipStream_t s, const Fields64 &f, const Grid &g, int b, int bc)
 
 /* Field energy sums, source/core/em.c:226-252 (FP64; block partial sums
  * combined with FP64 atomics, so the order differs from the CPU loop). */
-__global__ void k_energy(Fields64 f, Grid g, double *acc)
+template <class T>
+__global__ void k_energy(FieldsT<T> f, Grid g, double *acc)
 {
     __shared__ double se[TPB], sb[TPB];
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -406,12 +428,13 @@ __global__ void k_energy(Fields64 f, Grid g, double *acc)
     }
 }
 
-int launch_energy(hipStream_t s, const Fields64 &f, const Grid &g, double *acc)
+template <class T>
+int launch_energy(hipStream_t s, const FieldsT<T> &f, const Grid &g, double *acc)
 {
     long tot = (long)g.n[0] * g.n[1] * g.n[2];
     long nb = (tot + TPB - 1) / TPB;
     if (nb > 2048) nb = 2048;
-    k_energy<<<(unsigned)nb, TPB, 0, s>>>(f, g, acc);
+    k_energy<T><<<(unsigned)nb, TPB, 0, s>>>(f, g, acc);
     MPIC_CHECK(hipGetLastError());
     return MPIC_OK;
 }
@@ -477,7 +500,8 @@ int launch_rho_fold(hipStream_t s, double *rho, const Grid &g, int b, int bc)
 /* Gauss / continuity residuals over nodes [0, n]^3,
  * source/core/plasma_gaussTest.c:110-149.  acc = {max r_J, max r_E} as
  * unsigned bit patterns of non-negative doubles (ordered like the values). */
-__global__ void k_gauss(Fields64 f, const double *rho1, const double *rho2, Grid g,
+template <class T>
+__global__ void k_gauss(FieldsT<T> f, const double *rho1, const double *rho2, Grid g,
                         unsigned long long *acc)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -491,20 +515,38 @@ __global__ void k_gauss(Fields64 f, const double *rho1, const double *rho2, Grid
     double t0 = (rho2[o] - rho1[o]) / g.tau + (f.J[0][gidx(g, i + 1, j, k)] - f.J[0][o]) / h1 +
                 (f.J[1][gidx(g, i, j + 1, k)] - f.J[1][o]) / h2 +
                 (f.J[2][gidx(g, i, j, k + 1)] - f.J[2][o]) / h3;
-    double t1 = (f.E[0][gidx(g, i + 1, j, k)] - f.E[0][o]) / h1 +
-                (f.E[1][gidx(g, i, j + 1, k)] - f.E[1][o]) / h2 +
-                (f.E[2][gidx(g, i, j, k + 1)] - f.E[2][o]) / h3 - rho2[o];
+    double t1 = ((double)f.E[0][gidx(g, i + 1, j, k)] - (double)f.E[0][o]) / h1 +
+                ((double)f.E[1][gidx(g, i, j + 1, k)] - (double)f.E[1][o]) / h2 +
+                ((double)f.E[2][gidx(g, i, j, k + 1)] - (double)f.E[2][o]) / h3 - rho2[o];
     t0 = fabs(t0);
     t1 = fabs(t1);
     atomicMax(acc + 0, (unsigned long long)__double_as_longlong(t0));
     atomicMax(acc + 1, (unsigned long long)__double_as_longlong(t1));
 }
 
-int launch_gauss(hipStream_t s, const Fields64 &f, const double *rho1, const double *rho2,
+template <class T>
+int launch_gauss(hipStream_t s, const FieldsT<T> &f, const double *rho1, const double *rho2,
                  const Grid &g, double *acc)
 {
     long n = (long)(g.n[0] + 1) * (g.n[1] + 1) * (g.n[2] + 1);
-    k_gauss<<<nblk(n), TPB, 0, s>>>(f, rho1, rho2, g, (unsigned long long *)acc);
+    k_gauss<T><<<nblk(n), TPB, 0, s>>>(f, rho1, rho2, g, (unsigned long long *)acc);
     MPIC_CHECK(hipGetLastError());
     return MPIC_OK;
 }
+
+/* explicit instantiations for the two field precisions */
+#define INST(T)                                                                               \
+    template int launch_cap_H<T>(hipStream_t, const FieldsT<T> &, const Grid &, const BoxSet &); \
+    template int launch_cap_E<T>(hipStream_t, const FieldsT<T> &, const Grid &, const BoxSet &); \
+    template int launch_h_update<T>(hipStream_t, const FieldsT<T> &, const Grid &,                \
+                                    const BoxSet &, int);                                       \
+    template int launch_e_update<T>(hipStream_t, const FieldsT<T> &, const Grid &,                \
+                                    const BoxSet &, int, int);                                  \
+    template int launch_e_tauJ<T>(hipStream_t, const FieldsT<T> &, const Grid &);                \
+    template int launch_laser<T>(hipStream_t, T *, T *, const Grid &, const double *, int, int,  \
+                                 int, int, int, int, int, int, double, double, double);         \
+    template int launch_energy<T>(hipStream_t, const FieldsT<T> &, const Grid &, double *);      \
+    template int launch_gauss<T>(hipStream_t, const FieldsT<T> &, const double *,                \
+                                 const double *, const Grid &, double *);
+INST(double)
+INST(float)

@@ -47,8 +47,18 @@ struct BoxSet {
     long off[7];            /* prefix sums of the box volumes */
 };
 
-struct Fields64 {
-    double *E[3], *H[3], *J[3], *Hc[3], *Ec[3];
+/* Field arrays of one precision; J is always accumulated in FP64. */
+template <class T>
+struct FieldsT {
+    T      *E[3], *H[3];
+    double *J[3];
+    T      *Hc[3], *Ec[3];
+};
+typedef FieldsT<double> Fields64;
+typedef FieldsT<float>  Fields32;
+
+struct JPtrs {
+    double *J[3];
 };
 
 struct Markers64 {
@@ -68,11 +78,20 @@ enum FaceKind {
 };
 
 struct FaceOp {
-    int     kind, ea, top, N, zeroN;
+    int     kind, ea, top, N, zeroN, fp32;
     int     plo, phi, qlo, qhi;  /* transverse extents along ep = (ea+1)%3, eq = (ea+2)%3 */
     double  alpha;
-    double *buf;
-    double *X[3];
+    void   *buf;                 /* Mur buffer, field precision */
+    void   *X[3];                /* field arrays, field precision */
+};
+
+/* FP32 split-precision markers (see mpic_push32.hip). */
+struct Markers32 {
+    int32_t *cell;           /* (species << 29) | tile-major cell key, -1 = deleted */
+    float   *ox, *oy, *oz;   /* offset in the cell, cell units */
+    float   *ux, *uy, *uz;
+    float   *w;
+    int32_t *id;
 };
 
 struct PushConsts {
@@ -94,21 +113,23 @@ struct PushConsts {
 
 void mpic_set_error(const char *fmt, ...);
 
-/* Launchers (mpic_fdtd.hip, mpic_push64.hip). */
-int launch_cap_H(hipStream_t s, const Fields64 &f, const Grid &g, const BoxSet &bs);
-int launch_cap_E(hipStream_t s, const Fields64 &f, const Grid &g, const BoxSet &bs);
+/* Launchers (mpic_fdtd.hip, mpic_push64.hip, mpic_push32.hip). */
+template <class T> int launch_cap_H(hipStream_t s, const FieldsT<T> &f, const Grid &g, const BoxSet &bs);
+template <class T> int launch_cap_E(hipStream_t s, const FieldsT<T> &f, const Grid &g, const BoxSet &bs);
 int launch_face(hipStream_t s, const Grid &g, const FaceOp &op);
-int launch_h_update(hipStream_t s, const Fields64 &f, const Grid &g,
-                    const BoxSet &flush, int full);
-int launch_e_update(hipStream_t s, const Fields64 &f, const Grid &g,
-                    const BoxSet &flush, int interior, int do_flush);
-int launch_e_tauJ(hipStream_t s, const Fields64 &f, const Grid &g);
-int launch_laser(hipStream_t s, double *Fy, double *Fz, const Grid &g,
-                 const double *map, int pos, int JMIN, int KMIN, int KMAX,
-                 int jmin, int jmax, int kmin, int kmax, double A, double cs,
-                 double sn);
-int launch_jfold(hipStream_t s, const Fields64 &f, const Grid &g, int b, int bc);
-int launch_energy(hipStream_t s, const Fields64 &f, const Grid &g, double *acc);
+template <class T>
+int launch_h_update(hipStream_t s, const FieldsT<T> &f, const Grid &g, const BoxSet &flush,
+                    int full);
+template <class T>
+int launch_e_update(hipStream_t s, const FieldsT<T> &f, const Grid &g, const BoxSet &flush,
+                    int interior, int do_flush);
+template <class T> int launch_e_tauJ(hipStream_t s, const FieldsT<T> &f, const Grid &g);
+template <class T>
+int launch_laser(hipStream_t s, T *Fy, T *Fz, const Grid &g, const double *map, int pos,
+                 int JMIN, int KMIN, int KMAX, int jmin, int jmax, int kmin, int kmax, double A,
+                 double cs, double sn);
+int launch_jfold(hipStream_t s, const JPtrs &f, const Grid &g, int b, int bc);
+template <class T> int launch_energy(hipStream_t s, const FieldsT<T> &f, const Grid &g, double *acc);
 int launch_zero(hipStream_t s, double *p, long n);
 
 int launch_push64(hipStream_t s, const Markers64 &m, long n, const Fields64 &f,
@@ -116,8 +137,9 @@ int launch_push64(hipStream_t s, const Markers64 &m, long n, const Fields64 &f,
 int launch_rho64(hipStream_t s, const Markers64 &m, long n, double *rho,
                  const Grid &g);
 int launch_rho_fold(hipStream_t s, double *rho, const Grid &g, int b, int bc);
-int launch_gauss(hipStream_t s, const Fields64 &f, const double *rho1,
-                 const double *rho2, const Grid &g, double *acc);
+template <class T>
+int launch_gauss(hipStream_t s, const FieldsT<T> &f, const double *rho1, const double *rho2,
+                 const Grid &g, double *acc);
 int launch_cells64(hipStream_t s, const Markers64 &m, long n, const Grid &g,
                    int32_t *cell);
 

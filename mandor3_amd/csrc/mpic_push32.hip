@@ -1,0 +1,573 @@
+/*
+ * mpic_push32.hip -- FP32 split-precision marker push and current deposit.
+ *
+ * Marker layout (SoA, 32 B read + 28 B written per push):
+ *   cell  int32  (species << 29) | tile-major cell key   (-1: deleted)
+ *   ox oy oz     FP32 offset inside the cell, in cell units, [0, 1)
+ *   ux uy uz     FP32 momentum gamma*v/c
+ *   w            FP32 charge density of the marker (the reference's rho)
+ *   id   int32   reference marker id (moved by the sort only)
+ * Position = (cell index + offset) * h: the integer part carries the
+ * position exactly, so FP32 resolves 6e-8 of a cell anywhere in the box.
+ *
+ * Per marker: trilinear gather of FP32 E/H (source/misc_PIC.h:69-126),
+ * Boris rotation in FP32 (source/core/plasma.c:132-171), straight move in
+ * cell units, charge-conserving deposit of the trajectory split at cell
+ * faces (the Villasenor-Buneman form of source/core/plasma_currentKernel.c:
+ * 20-124 in cell units), walls (source/core/plasma_walls.c:27-104).
+ *
+ * Deposit: markers are sorted by cell, so the lanes of a wavefront mostly
+ * share one start cell.  The segment inside the start cell (every marker
+ * has one) is summed across each run of equal cells with a segmented
+ * wavefront reduction, then added to the tile's FP32 J block in LDS once
+ * per run; later segments (cell crossings) go to LDS with atomics.  The
+ * block is flushed to the FP64 J in HBM once per tile.
+ *
+ * Roofline: HBM-bound.  Algorithmic bytes per marker-step 32 B read + 28 B
+ * written (id is not touched), plus per occupied cell 24 B of FP32 E/H reads
+ * and 48 B of FP64 J read-modify-write.
+ */
+#include "mpic_internal.h"
+#include "mpic_tiles.h"
+
+#define TPB 256
+#define TILE 4
+#define TJN (TILE + 3)         /* J nodes per axis of a tile's LDS block */
+#define TFN (TILE + 2)         /* E/H nodes per axis staged for a tile's gathers */
+#define TFN3 (TFN * TFN * TFN)
+#define SPECIES_SHIFT 29
+#define KEY_MASK ((1 << SPECIES_SHIFT) - 1)
+
+struct F32View {
+    const float *E[3], *H[3];
+    double *J[3];
+    Grid g;
+};
+
+/* 12 J values of one straight segment inside a cell, cell units:
+ * start (x1,y1,z1), end (x2,y2,z2) relative to the cell's origin. */
+struct Seg12 {
+    float v[12];
+};
+
+__device__ static inline void seg_values(float x1, float y1, float z1, float x2, float y2,
+                                         float z2, float wx, float wy, float wz, Seg12 &s)
+{
+    const float dx = x2 - x1, dy = y2 - y1, dz = z2 - z1;
+    const float tx = 0.5f * (x1 + x2), ty = 0.5f * (y1 + y2), tz = 0.5f * (z1 + z2);
+    const float yz = dy * dz * (1.0f / 12), xz = dx * dz * (1.0f / 12), xy = dx * dy * (1.0f / 12);
+    const float ax = wx * dx, ay = wy * dy, az = wz * dz;
+    s.v[0] = ax * ((1 - ty) * (1 - tz) + yz);
+    s.v[1] = ax * (ty * (1 - tz) - yz);
+    s.v[2] = ax * ((1 - ty) * tz - yz);
+    s.v[3] = ax * (ty * tz + yz);
+    s.v[4] = ay * ((1 - tx) * (1 - tz) + xz);
+    s.v[5] = ay * (tx * (1 - tz) - xz);
+    s.v[6] = ay * ((1 - tx) * tz - xz);
+    s.v[7] = ay * (tx * tz + xz);
+    s.v[8] = az * ((1 - tx) * (1 - ty) + xy);
+    s.v[9] = az * (tx * (1 - ty) - xy);
+    s.v[10] = az * ((1 - tx) * ty - xy);
+    s.v[11] = az * (tx * ty + xy);
+}
+
+#define LJ(lj, a, b, c, comp) ((lj) + 3 * (((a) * TJN + (b)) * TJN + (c)) + (comp))
+
+/* Adds a segment of local cell (a,b,c) (LDS block coordinates) to LDS. */
+__device__ static inline void seg_to_lds(float *lj, int a, int b, int c, const Seg12 &s)
+{
+    atomicAdd(LJ(lj, a + 1, b, c, 0), s.v[0]);
+    atomicAdd(LJ(lj, a + 1, b + 1, c, 0), s.v[1]);
+    atomicAdd(LJ(lj, a + 1, b, c + 1, 0), s.v[2]);
+    atomicAdd(LJ(lj, a + 1, b + 1, c + 1, 0), s.v[3]);
+    atomicAdd(LJ(lj, a, b + 1, c, 1), s.v[4]);
+    atomicAdd(LJ(lj, a + 1, b + 1, c, 1), s.v[5]);
+    atomicAdd(LJ(lj, a, b + 1, c + 1, 1), s.v[6]);
+    atomicAdd(LJ(lj, a + 1, b + 1, c + 1, 1), s.v[7]);
+    atomicAdd(LJ(lj, a, b, c + 1, 2), s.v[8]);
+    atomicAdd(LJ(lj, a + 1, b, c + 1, 2), s.v[9]);
+    atomicAdd(LJ(lj, a, b + 1, c + 1, 2), s.v[10]);
+    atomicAdd(LJ(lj, a + 1, b + 1, c + 1, 2), s.v[11]);
+}
+
+/* Same, straight to the FP64 J in HBM (segment outside the LDS block). */
+__device__ static inline void seg_to_hbm(const F32View &v, int i, int j, int k, const Seg12 &s)
+{
+    const Grid &g = v.g;
+    unsafeAtomicAdd(v.J[0] + gidx(g, i + 1, j, k), (double)s.v[0]);
+    unsafeAtomicAdd(v.J[0] + gidx(g, i + 1, j + 1, k), (double)s.v[1]);
+    unsafeAtomicAdd(v.J[0] + gidx(g, i + 1, j, k + 1), (double)s.v[2]);
+    unsafeAtomicAdd(v.J[0] + gidx(g, i + 1, j + 1, k + 1), (double)s.v[3]);
+    unsafeAtomicAdd(v.J[1] + gidx(g, i, j + 1, k), (double)s.v[4]);
+    unsafeAtomicAdd(v.J[1] + gidx(g, i + 1, j + 1, k), (double)s.v[5]);
+    unsafeAtomicAdd(v.J[1] + gidx(g, i, j + 1, k + 1), (double)s.v[6]);
+    unsafeAtomicAdd(v.J[1] + gidx(g, i + 1, j + 1, k + 1), (double)s.v[7]);
+    unsafeAtomicAdd(v.J[2] + gidx(g, i, j, k + 1), (double)s.v[8]);
+    unsafeAtomicAdd(v.J[2] + gidx(g, i + 1, j, k + 1), (double)s.v[9]);
+    unsafeAtomicAdd(v.J[2] + gidx(g, i, j + 1, k + 1), (double)s.v[10]);
+    unsafeAtomicAdd(v.J[2] + gidx(g, i + 1, j + 1, k + 1), (double)s.v[11]);
+}
+
+__device__ static inline void seg_add(const F32View &v, float *lj, int o0, int o1, int o2, int i,
+                                      int j, int k, const Seg12 &s)
+{
+    const int a = i - o0, b = j - o1, c = k - o2;
+    if (a < 0 || b < 0 || c < 0 || a > TJN - 2 || b > TJN - 2 || c > TJN - 2)
+        seg_to_hbm(v, i, j, k, s);
+    else
+        seg_to_lds(lj, a, b, c, s);
+}
+
+struct Push32Consts {
+    float tau, dtn[3];       /* tau, tau/h */
+    float cj[3];             /* h/tau: J of a segment per unit w and displacement */
+    double vol;              /* h1*h2*h3 (energy) */
+    float qm[8];             /* q/m per species */
+    int bc[6];
+};
+
+/* Wall of axis a on cell index c / offset o (cell units); returns 0 when
+ * an open wall deletes the marker.  Reference order: mins then maxes. */
+__device__ static inline int wall_lo(int bc, int &c, float &o, float &u, int N)
+{
+    if (bc == MPIC_BC_PERIODIC) {
+        if (c < 0) c += N;                              /* x <= 0: x += L */
+    } else if (bc == MPIC_BC_MIRROR) {
+        if (c < 0 || (c == 0 && o == 0.0f)) {           /* x <= 0: x = -x, u = -u */
+            const float x = -((float)(c) + o);          /* in (0, 1] */
+            if (x >= 1.0f) { c = 1; o = 0.0f; } else { c = 0; o = x; }
+            u = -u;
+        }
+    } else if (bc == MPIC_BC_OPEN) {
+        if (c < 1 || (c == 1 && o == 0.0f)) return 0;   /* x <= h: deleted */
+    }
+    return 1;
+}
+
+__device__ static inline int wall_hi(int bc, int &c, float &o, float &u, int N)
+{
+    if (bc == MPIC_BC_PERIODIC) {
+        if (c >= N) c -= N;                             /* x >= L: x -= L */
+    } else if (bc == MPIC_BC_MIRROR) {
+        if (c >= N) {                                   /* x >= L: x = 2L - x */
+            /* reflected position relative to cell N-1: 1 - o - (c - N) */
+            float on = 1.0f - o - (float)(c - N);
+            if (on >= 1.0f) on = 0.99999994f;           /* x == L stays at the wall */
+            if (on < 0.0f) on = 0.0f;
+            c = N - 1;
+            o = on;
+            u = -u;
+        }
+    } else if (bc == MPIC_BC_OPEN) {
+        if (c >= N - 1) return 0;                       /* x >= L - h: deleted */
+    }
+    return 1;
+}
+
+__device__ static inline void cswap(float &a, int &ia, float &b, int &ib)
+{
+    if (b < a) {
+        const float t = a; a = b; b = t;
+        const int u = ia; ia = ib; ib = u;
+    }
+}
+
+/* Marker state held in registers between the prefetch and the push. */
+struct MarkerRegs {
+    int   word;
+    float ox, oy, oz, ux, uy, uz, w;
+};
+
+__device__ static inline void load_marker(const Markers32 &m, long p, bool in, MarkerRegs &r)
+{
+    r.word = -1;
+    if (!in) return;
+    r.word = m.cell[p];
+    r.ox = m.ox[p];
+    r.oy = m.oy[p];
+    r.oz = m.oz[p];
+    r.ux = m.ux[p];
+    r.uy = m.uy[p];
+    r.uz = m.uz[p];
+    r.w = m.w[p];
+}
+
+/* Trilinear gather from a tile's staged field block (TFN^3 nodes, node
+ * (a,b,c) relative to the tile's first cell). */
+__device__ static inline float lgather(const float *F, int a, float sx, int b, float sy, int c,
+                                       float sz)
+{
+    const float sx1 = 1.0f - sx, sy1 = 1.0f - sy, sz1 = 1.0f - sz;
+    const int o = (a * TFN + b) * TFN + c;
+    const int dj = TFN, di = TFN * TFN;
+    return sz1 * (sy1 * (sx1 * F[o] + sx * F[o + di]) + sy * (sx1 * F[o + dj] + sx * F[o + di + dj])) +
+           sz * (sy1 * (sx1 * F[o + 1] + sx * F[o + di + 1]) +
+                 sy * (sx1 * F[o + dj + 1] + sx * F[o + di + dj + 1]));
+}
+
+/* Push of one marker of tile-local cell (li,lj,lk) (cell (ci,cj,ck)):
+ * gather from the staged fields, Boris, move, deposit of every segment but
+ * the first into the tile's LDS J (or HBM beyond it), walls.  Writes the
+ * marker back and returns its new cell word (-1: deleted) and the first
+ * segment, which lies inside the start cell. */
+__device__ static inline int push32_marker(const Markers32 &m, long p, const MarkerRegs &r, int li,
+                                           int lj_, int lk, int ci, int cj, int ck,
+                                           const float *lf, float *lj, int o0, int o1, int o2,
+                                           const F32View &v, const Push32Consts &c,
+                                           const TileView &tv, Seg12 &first, double &w0,
+                                           double &w1, double &w2)
+{
+    const Grid &g = v.g;
+    const int sp = (int)((unsigned)r.word >> SPECIES_SHIFT);
+    const float ox = r.ox, oy = r.oy, oz = r.oz;
+    float ux = r.ux, uy = r.uy, uz = r.uz;
+    const float w = r.w, qm = c.qm[sp];
+    /* gather: E_x at (i-1/2, j, k), E_y at (i, j-1/2, k), ... */
+    const int hx = ox >= 0.5f, hy = oy >= 0.5f, hz = oz >= 0.5f;
+    const float sxh = ox + 0.5f - (float)hx, syh = oy + 0.5f - (float)hy, szh = oz + 0.5f - (float)hz;
+    const int ah = li + hx, bh = lj_ + hy, ch = lk + hz;
+    const float Ex = lgather(lf + 0 * TFN3, ah, sxh, lj_, oy, lk, oz),
+                Ey = lgather(lf + 1 * TFN3, li, ox, bh, syh, lk, oz),
+                Ez = lgather(lf + 2 * TFN3, li, ox, lj_, oy, ch, szh),
+                Hx = lgather(lf + 3 * TFN3, li, ox, bh, syh, ch, szh),
+                Hy = lgather(lf + 4 * TFN3, ah, sxh, lj_, oy, ch, szh),
+                Hz = lgather(lf + 5 * TFN3, ah, sxh, bh, syh, lk, oz);
+    const float tm = 0.5f * c.tau * qm;
+    const float px = ux + tm * Ex, py = uy + tm * Ey, pz = uz + tm * Ez;
+    const float p2 = px * px + py * py + pz * pz;
+    const float gam = sqrtf(1.0f + p2);
+    /* (gamma - 1)/p^2 = 1/(gamma + 1): no cancellation for slow markers */
+    const float W = w / ((gam + 1.0f) * qm);
+    w0 += (double)(W * px * px);
+    w1 += (double)(W * py * py);
+    w2 += (double)(W * pz * pz);
+    const float tg = tm / gam;
+    const float ax = tg * Hx, ay = tg * Hy, az = tg * Hz;
+    const float uu = px + py * az - pz * ay, vv = py + pz * ax - px * az, ww = pz + px * ay - py * ax;
+    const float bxx = ax * ax, byy = ay * ay, bzz = az * az, bxy = ax * ay, byz = ay * az,
+                bxz = az * ax;
+    const float del = 1.0f / (1.0f + bxx + byy + bzz);
+    ux = tm * Ex + ((1 + bxx) * uu + (bxy + az) * vv + (bxz - ay) * ww) * del;
+    uy = tm * Ey + ((bxy - az) * uu + (1 + byy) * vv + (byz + ax) * ww) * del;
+    uz = tm * Ez + ((bxz + ay) * uu + (byz - ax) * vv + (1 + bzz) * ww) * del;
+    const float ig = 1.0f / sqrtf(1.0f + ux * ux + uy * uy + uz * uz);
+    /* move in cell units from (ox,oy,oz) of cell (ci,cj,ck) */
+    const float x2 = ox + ux * c.dtn[0] * ig, y2 = oy + uy * c.dtn[1] * ig, z2 = oz + uz * c.dtn[2] * ig;
+    const int fx = (int)floorf(x2), fy = (int)floorf(y2), fz = (int)floorf(z2);
+    const float wx = w * c.cj[0], wy = w * c.cj[1], wz = w * c.cj[2];
+    if ((fx | fy | fz) == 0) {
+        seg_values(ox, oy, oz, x2, y2, z2, wx, wy, wz, first);
+    } else {
+        /* crossing fractions along the path, ascending (axes that are not
+         * crossed get 2 and sort last); a sorting network on scalars keeps
+         * everything in registers */
+        float a0 = fx ? ((fx > 0 ? 1.0f : 0.0f) - ox) / (x2 - ox) : 2.0f;
+        float a1 = fy ? ((fy > 0 ? 1.0f : 0.0f) - oy) / (y2 - oy) : 2.0f;
+        float a2 = fz ? ((fz > 0 ? 1.0f : 0.0f) - oz) / (z2 - oz) : 2.0f;
+        int x0 = 0, x1 = 1, x2i = 2;
+        cswap(a0, x0, a1, x1);
+        cswap(a1, x1, a2, x2i);
+        cswap(a0, x0, a1, x1);
+        const int nc = (fx != 0) + (fy != 0) + (fz != 0);
+        float px0 = ox, py0 = oy, pz0 = oz;
+        int sx = 0, sy = 0, sz = 0;
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            if (s > nc) break;
+            float qx = x2, qy = y2, qz = z2;
+            int axs = -1;
+            if (s < nc) {
+                const float a = (s == 0) ? a0 : (s == 1 ? a1 : a2);
+                axs = (s == 0) ? x0 : (s == 1 ? x1 : x2i);
+                qx = ox + a * (x2 - ox);
+                qy = oy + a * (y2 - oy);
+                qz = oz + a * (z2 - oz);
+                /* the crossing point lies on the face exactly */
+                if (axs == 0) qx = (fx > 0) ? 1.0f : 0.0f;
+                if (axs == 1) qy = (fy > 0) ? 1.0f : 0.0f;
+                if (axs == 2) qz = (fz > 0) ? 1.0f : 0.0f;
+            }
+            Seg12 sg;
+            seg_values(px0 - sx, py0 - sy, pz0 - sz, qx - sx, qy - sy, qz - sz, wx, wy, wz, sg);
+            if (s == 0)
+                first = sg;
+            else
+                seg_add(v, lj, o0, o1, o2, ci + sx, cj + sy, ck + sz, sg);
+            if (axs == 0) sx += fx;
+            if (axs == 1) sy += fy;
+            if (axs == 2) sz += fz;
+            px0 = qx;
+            py0 = qy;
+            pz0 = qz;
+        }
+    }
+    /* new cell and offset, then the walls */
+    int ni = ci + fx, nj = cj + fy, nk = ck + fz;
+    float nx = x2 - (float)fx, ny = y2 - (float)fy, nz = z2 - (float)fz;
+    if (nx >= 1.0f) { nx = 0.0f; ++ni; }
+    if (ny >= 1.0f) { ny = 0.0f; ++nj; }
+    if (nz >= 1.0f) { nz = 0.0f; ++nk; }
+    const int alive = wall_lo(c.bc[0], ni, nx, ux, g.n[0]) && wall_lo(c.bc[1], nj, ny, uy, g.n[1]) &&
+                      wall_lo(c.bc[2], nk, nz, uz, g.n[2]) && wall_hi(c.bc[3], ni, nx, ux, g.n[0]) &&
+                      wall_hi(c.bc[4], nj, ny, uy, g.n[1]) && wall_hi(c.bc[5], nk, nz, uz, g.n[2]);
+    if (!alive) return -1;
+    m.ox[p] = nx;
+    m.oy[p] = ny;
+    m.oz[p] = nz;
+    m.ux[p] = ux;
+    m.uy[p] = uy;
+    m.uz[p] = uz;
+    return (sp << SPECIES_SHIFT) | cell_key_ijk(tv, g, ni, nj, nk);
+}
+
+/* Segmented sum of the 12 first-segment values over the caller's run of
+ * equal start cells (lanes [start, end) of the wavefront): afterwards the
+ * run's head lane holds the run total. */
+__device__ static inline void run_sum(Seg12 &f, int end)
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int q = 0; q < 12; ++q) {
+        float x = f.v[q];
+        for (int d = 1; d < 64; d <<= 1) {
+            const float y = __shfl_down(x, d, 64);
+            if (lane + d < end) x += y;
+        }
+        f.v[q] = x;
+    }
+}
+
+__device__ static inline void key_to_ijk(const TileView &tv, int key, int &i, int &j, int &k)
+{
+    int t0, t1, t2;
+    tile_coords(tv, key >> 6, t0, t1, t2);
+    const int loc = key & 63;
+    i = TILE * t0 + (loc >> 4);
+    j = TILE * t1 + ((loc >> 2) & 3);
+    k = TILE * t2 + (loc & 3);
+}
+
+/* Persistent tiled push, one 4x4x4-cell tile per workgroup iteration: the
+ * tile's E/H nodes are staged in LDS, its J is accumulated in LDS and
+ * flushed once, and the next batch of markers is loaded before the current
+ * one is pushed. */
+__global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(4))) k_push32_tiled(Markers32 m, F32View v, Push32Consts c,
+                                                      TileView tv, double *Wacc)
+{
+    __shared__ float lj[TJN * TJN * TJN * 3];
+    __shared__ float lf[6 * TFN3];
+    __shared__ int s_tile;
+    const Grid &g = v.g;
+    const int xcd = blockIdx.x % MPIC_XCDS;
+    const int lane = threadIdx.x & 63;
+    double w0 = 0, w1 = 0, w2 = 0;
+    for (;;) {
+        if (threadIdx.x == 0) s_tile = tile_next(tv, xcd);
+        __syncthreads();
+        const int t = s_tile;
+        if (t < 0) break;
+        int t0, t1, t2;
+        tile_coords(tv, t, t0, t1, t2);
+        const int c0 = TILE * t0, c1 = TILE * t1, c2 = TILE * t2;
+        const int o0 = c0 - 1, o1 = c1 - 1, o2 = c2 - 1;
+        for (int q = threadIdx.x; q < TJN * TJN * TJN * 3; q += TPB) lj[q] = 0.0f;
+        /* nodes [c, c + TFN) per axis; those past the last ghost layer (a
+         * partial tile at the high edge) are never read by a gather */
+        for (int q = threadIdx.x; q < 6 * TFN3; q += TPB) {
+            const int comp = q / TFN3, rr = q - comp * TFN3;
+            const int a = rr / (TFN * TFN), b = (rr / TFN) % TFN, cc = rr % TFN;
+            const int i = c0 + a, j = c1 + b, k = c2 + cc;
+            const float *F = comp < 3 ? v.E[comp] : v.H[comp - 3];
+            lf[q] = (i <= g.n[0] + 2 && j <= g.n[1] + 2 && k <= g.n[2] + 2) ? F[gidx(g, i, j, k)] : 0.0f;
+        }
+        const long beg = tv.off[(long)t * TILE_CELLS], end = tv.off[(long)(t + 1) * TILE_CELLS];
+        MarkerRegs cur;
+        load_marker(m, beg + threadIdx.x, beg + threadIdx.x < end, cur);
+        __syncthreads();
+        for (long base = beg; base < end; base += TPB) {
+            const long p = base + threadIdx.x;
+            MarkerRegs nxt;
+            load_marker(m, p + TPB, p + TPB < end, nxt);
+            int rkey = -1, newword = -1, ci = 0, cj = 0, ck = 0;
+            Seg12 first;
+            for (int q = 0; q < 12; ++q) first.v[q] = 0.0f;
+            if (cur.word >= 0) {
+                rkey = cur.word & KEY_MASK;
+                const int loc = rkey & (TILE_CELLS - 1);
+                const int li = loc >> 4, lj_ = (loc >> 2) & 3, lk = loc & 3;
+                ci = c0 + li;
+                cj = c1 + lj_;
+                ck = c2 + lk;
+                newword = push32_marker(m, p, cur, li, lj_, lk, ci, cj, ck, lf, lj, o0, o1, o2, v, c,
+                                        tv, first, w0, w1, w2);
+                m.cell[p] = newword;
+            }
+            int start, len;
+            bool head;
+            wave_run(rkey, start, len, head);
+            run_sum(first, start + len);
+            if (head && rkey >= 0) seg_add(v, lj, o0, o1, o2, ci, cj, ck, first);
+            wave_run_count(tv.count, newword < 0 ? -1 : (newword & KEY_MASK));
+            cur = nxt;
+        }
+        __syncthreads();
+        for (int q = threadIdx.x; q < TJN * TJN * TJN * 3; q += TPB) {
+            const float val = lj[q];
+            if (val == 0.0f) continue;
+            const int comp = q % 3, node = q / 3;
+            const int cc = node % TJN, bb = (node / TJN) % TJN, aa = node / (TJN * TJN);
+            const int i = o0 + aa, j = o1 + bb, k = o2 + cc;
+            if (i < -2 || j < -2 || k < -2 || i > g.n[0] + 2 || j > g.n[1] + 2 || k > g.n[2] + 2)
+                continue;
+            unsafeAtomicAdd(v.J[comp] + gidx(g, i, j, k), (double)val);
+        }
+        __syncthreads();
+    }
+    for (int d = 32; d > 0; d >>= 1) {
+        w0 += __shfl_down(w0, d, 64);
+        w1 += __shfl_down(w1, d, 64);
+        w2 += __shfl_down(w2, d, 64);
+    }
+    if (lane == 0) {
+        atomicAdd(Wacc + 0, w0 * c.vol);
+        atomicAdd(Wacc + 1, w1 * c.vol);
+        atomicAdd(Wacc + 2, w2 * c.vol);
+    }
+}
+
+/* Workgroups of the persistent push that fit on one CU at once. */
+int push32_blocks_per_cu(void)
+{
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_push32_tiled, TPB, 0) != hipSuccess || nb < 1)
+        nb = 1;
+    return nb;
+}
+
+int launch_push32(hipStream_t s, const Markers32 &m, const Fields32 &f, const Grid &g,
+                  const PushConsts &pc, const float *qm, const TileView &tv, double *Wacc,
+                  int nblocks)
+{
+    F32View v;
+    for (int a = 0; a < 3; ++a) {
+        v.E[a] = f.E[a];
+        v.H[a] = f.H[a];
+        v.J[a] = f.J[a];
+    }
+    v.g = g;
+    Push32Consts c;
+    c.tau = (float)g.tau;
+    for (int a = 0; a < 3; ++a) {
+        c.dtn[a] = (float)(g.tau / g.h[a]);
+        c.cj[a] = (float)(g.h[a] / g.tau);
+    }
+    c.vol = g.h[0] * g.h[1] * g.h[2];
+    for (int q = 0; q < 8; ++q) c.qm[q] = qm[q];
+    for (int b = 0; b < 6; ++b) c.bc[b] = pc.bc[b];
+    k_push32_tiled<<<nblocks, TPB, 0, s>>>(m, v, c, tv, Wacc);
+    MPIC_CHECK(hipGetLastError());
+    return MPIC_OK;
+}
+
+/* Keys of the loaded markers (the cell word already holds them): counts. */
+__global__ void k_keys32(Markers32 m, long n, TileView tv)
+{
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int word = (p < n) ? m.cell[p] : -1;
+    wave_run_count(tv.count, word < 0 ? -1 : (word & KEY_MASK));
+}
+
+int launch_keys32(hipStream_t s, const Markers32 &m, long n, const TileView &tv)
+{
+    if (n <= 0) return MPIC_OK;
+    k_keys32<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(m, n, tv);
+    MPIC_CHECK(hipGetLastError());
+    return MPIC_OK;
+}
+
+__global__ void k_scatter32(Markers32 src, Markers32 dst, long nbound, TileView tv, int parity)
+{
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long n = tv.nmark[parity];
+    const int word = (p < n && p < nbound) ? src.cell[p] : -1;
+    const int key = word < 0 ? -1 : (word & KEY_MASK);
+    int start, len;
+    bool head;
+    wave_run(key, start, len, head);
+    int base = 0;
+    if (head && key >= 0) base = atomicAdd(tv.cursor + key, len);
+    base = __shfl(base, start, 64);
+    if (key < 0) return;
+    const long d = base + ((int)(threadIdx.x & 63) - start);
+    dst.cell[d] = word;
+    dst.ox[d] = src.ox[p];
+    dst.oy[d] = src.oy[p];
+    dst.oz[d] = src.oz[p];
+    dst.ux[d] = src.ux[p];
+    dst.uy[d] = src.uy[p];
+    dst.uz[d] = src.uz[p];
+    dst.w[d] = src.w[p];
+    dst.id[d] = src.id[p];
+}
+
+int launch_scatter32(hipStream_t s, const Markers32 &src, const Markers32 &dst, long nbound,
+                     const TileView &tv, int parity)
+{
+    if (nbound <= 0) return MPIC_OK;
+    k_scatter32<<<(unsigned)((nbound + 255) / 256), 256, 0, s>>>(src, dst, nbound, tv, parity);
+    MPIC_CHECK(hipGetLastError());
+    return MPIC_OK;
+}
+
+/* CIC charge density of the split-precision markers, FP64. */
+__global__ void k_rho32(Markers32 m, long n, double *rho, Grid g, TileView tv)
+{
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int word = m.cell[p];
+    if (word < 0) return;
+    int i, j, k;
+    key_to_ijk(tv, word & KEY_MASK, i, j, k);
+    const double sx = m.ox[p], sy = m.oy[p], sz = m.oz[p], val = m.w[p];
+    unsafeAtomicAdd(rho + gidx(g, i, j, k), (1 - sx) * ((1 - sy) * ((1 - sz) * val)));
+    unsafeAtomicAdd(rho + gidx(g, i + 1, j, k), sx * ((1 - sy) * ((1 - sz) * val)));
+    unsafeAtomicAdd(rho + gidx(g, i, j + 1, k), (1 - sx) * (sy * ((1 - sz) * val)));
+    unsafeAtomicAdd(rho + gidx(g, i + 1, j + 1, k), sx * (sy * ((1 - sz) * val)));
+    unsafeAtomicAdd(rho + gidx(g, i, j, k + 1), (1 - sx) * ((1 - sy) * (sz * val)));
+    unsafeAtomicAdd(rho + gidx(g, i + 1, j, k + 1), sx * ((1 - sy) * (sz * val)));
+    unsafeAtomicAdd(rho + gidx(g, i, j + 1, k + 1), (1 - sx) * (sy * (sz * val)));
+    unsafeAtomicAdd(rho + gidx(g, i + 1, j + 1, k + 1), sx * (sy * (sz * val)));
+}
+
+int launch_rho32(hipStream_t s, const Markers32 &m, long n, double *rho, const Grid &g,
+                 const TileView &tv)
+{
+    if (n <= 0) return MPIC_OK;
+    k_rho32<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(m, n, rho, g, tv);
+    MPIC_CHECK(hipGetLastError());
+    return MPIC_OK;
+}
+
+/* Natural cell index (i*n1 + j)*n2 + k of every marker (-1: deleted). */
+__global__ void k_cells32(Markers32 m, long n, Grid g, TileView tv, int32_t *cell)
+{
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int word = m.cell[p];
+    if (word < 0) {
+        cell[p] = -1;
+        return;
+    }
+    int i, j, k;
+    key_to_ijk(tv, word & KEY_MASK, i, j, k);
+    cell[p] = (i * g.n[1] + j) * g.n[2] + k;
+}
+
+int launch_cells32(hipStream_t s, const Markers32 &m, long n, const Grid &g, const TileView &tv,
+                   int32_t *cell)
+{
+    if (n <= 0) return MPIC_OK;
+    k_cells32<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(m, n, g, tv, cell);
+    MPIC_CHECK(hipGetLastError());
+    return MPIC_OK;
+}

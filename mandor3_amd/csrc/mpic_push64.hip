@@ -16,6 +16,7 @@
  * markers per cell.
  */
 #include "mpic_internal.h"
+#include "mpic_tiles.h"
 
 #define TPB 256
 
@@ -40,16 +41,69 @@ __device__ static inline double gather(const double *F, const Grid &g, int i, do
            F[gidx(g, i + 1, j + 1, k + 1)] * sx * (sy * sz);
 }
 
-__device__ static inline void jadd(double *p, double v)
-{
-    unsafeAtomicAdd(p, v);
-}
+/* Deposit targets.  GlobalSink adds straight to J in HBM with FP64
+ * atomics; TileSink (tiled kernel) adds to a 7x7x7-node J block in LDS that
+ * covers a 4x4x4-cell tile and its one-cell rim, and falls back to HBM for a
+ * segment outside it. */
+struct GlobalSink {
+    double *J[3];
+    Grid g;
+    __device__ void seg(int i, int j, int k, const double vx[4], const double vy[4],
+                        const double vz[4]) const
+    {
+        unsafeAtomicAdd(J[0] + gidx(g, i + 1, j, k), vx[0]);
+        unsafeAtomicAdd(J[0] + gidx(g, i + 1, j + 1, k), vx[1]);
+        unsafeAtomicAdd(J[0] + gidx(g, i + 1, j, k + 1), vx[2]);
+        unsafeAtomicAdd(J[0] + gidx(g, i + 1, j + 1, k + 1), vx[3]);
+        unsafeAtomicAdd(J[1] + gidx(g, i, j + 1, k), vy[0]);
+        unsafeAtomicAdd(J[1] + gidx(g, i + 1, j + 1, k), vy[1]);
+        unsafeAtomicAdd(J[1] + gidx(g, i, j + 1, k + 1), vy[2]);
+        unsafeAtomicAdd(J[1] + gidx(g, i + 1, j + 1, k + 1), vy[3]);
+        unsafeAtomicAdd(J[2] + gidx(g, i, j, k + 1), vz[0]);
+        unsafeAtomicAdd(J[2] + gidx(g, i + 1, j, k + 1), vz[1]);
+        unsafeAtomicAdd(J[2] + gidx(g, i, j + 1, k + 1), vz[2]);
+        unsafeAtomicAdd(J[2] + gidx(g, i + 1, j + 1, k + 1), vz[3]);
+    }
+};
+
+#define TILE 4
+#define TJN (TILE + 3)             /* LDS J nodes per axis: tile + rim */
+
+struct TileSink {
+    GlobalSink gs;
+    double *lj;                    /* LDS: [TJN^3][3] */
+    int o0, o1, o2;                /* node of lj[0] */
+    __device__ void seg(int i, int j, int k, const double vx[4], const double vy[4],
+                        const double vz[4]) const
+    {
+        const int a = i - o0, b = j - o1, c = k - o2;
+        if (a < 0 || b < 0 || c < 0 || a > TJN - 2 || b > TJN - 2 || c > TJN - 2) {
+            gs.seg(i, j, k, vx, vy, vz);
+            return;
+        }
+#define L(aa, bb, cc, comp) (lj + 3 * (((aa) * TJN + (bb)) * TJN + (cc)) + (comp))
+        atomicAdd(L(a + 1, b, c, 0), vx[0]);
+        atomicAdd(L(a + 1, b + 1, c, 0), vx[1]);
+        atomicAdd(L(a + 1, b, c + 1, 0), vx[2]);
+        atomicAdd(L(a + 1, b + 1, c + 1, 0), vx[3]);
+        atomicAdd(L(a, b + 1, c, 1), vy[0]);
+        atomicAdd(L(a + 1, b + 1, c, 1), vy[1]);
+        atomicAdd(L(a, b + 1, c + 1, 1), vy[2]);
+        atomicAdd(L(a + 1, b + 1, c + 1, 1), vy[3]);
+        atomicAdd(L(a, b, c + 1, 2), vz[0]);
+        atomicAdd(L(a + 1, b, c + 1, 2), vz[1]);
+        atomicAdd(L(a, b + 1, c + 1, 2), vz[2]);
+        atomicAdd(L(a + 1, b + 1, c + 1, 2), vz[3]);
+#undef L
+    }
+};
 
 /* Deposit of one straight segment inside cell (i,j,k). */
-__device__ static void deposit_seg(const F64View &v, int i, int j, int k, double x1, double x2,
-                                   double y1, double y2, double z1, double z2, double rp)
+template <class S>
+__device__ static void deposit_seg(const S &v, const Grid &g, int i, int j, int k, double x1,
+                                   double x2, double y1, double y2, double z1, double z2,
+                                   double rp)
 {
-    const Grid &g = v.g;
     const double h1 = g.h[0], h2 = g.h[1], h3 = g.h[2], tau = g.tau;
     const double dX = x2 - x1, dY = y2 - y1, dZ = z2 - z1;
     const double tX = 0.5 * (x1 + x2) - i * h1;
@@ -57,28 +111,24 @@ __device__ static void deposit_seg(const F64View &v, int i, int j, int k, double
     const double tZ = 0.5 * (z1 + z2) - k * h3;
     const double yz = dY * dZ / 12, xz = dX * dZ / 12, xy = dX * dY / 12;
     const double ax = rp * dX / (tau * h2 * h3);
-    jadd(v.J[0] + gidx(g, i + 1, j, k), ax * ((h2 - tY) * (h3 - tZ) + yz));
-    jadd(v.J[0] + gidx(g, i + 1, j + 1, k), ax * (tY * (h3 - tZ) - yz));
-    jadd(v.J[0] + gidx(g, i + 1, j, k + 1), ax * ((h2 - tY) * tZ - yz));
-    jadd(v.J[0] + gidx(g, i + 1, j + 1, k + 1), ax * (tY * tZ + yz));
     const double ay = rp * dY / (tau * h1 * h3);
-    jadd(v.J[1] + gidx(g, i, j + 1, k), ay * ((h1 - tX) * (h3 - tZ) + xz));
-    jadd(v.J[1] + gidx(g, i + 1, j + 1, k), ay * (tX * (h3 - tZ) - xz));
-    jadd(v.J[1] + gidx(g, i, j + 1, k + 1), ay * ((h1 - tX) * tZ - xz));
-    jadd(v.J[1] + gidx(g, i + 1, j + 1, k + 1), ay * (tX * tZ + xz));
     const double az = rp * dZ / (tau * h1 * h2);
-    jadd(v.J[2] + gidx(g, i, j, k + 1), az * ((h1 - tX) * (h2 - tY) + xy));
-    jadd(v.J[2] + gidx(g, i + 1, j, k + 1), az * (tX * (h2 - tY) - xy));
-    jadd(v.J[2] + gidx(g, i, j + 1, k + 1), az * ((h1 - tX) * tY - xy));
-    jadd(v.J[2] + gidx(g, i + 1, j + 1, k + 1), az * (tX * tY + xy));
+    const double vx[4] = {ax * ((h2 - tY) * (h3 - tZ) + yz), ax * (tY * (h3 - tZ) - yz),
+                          ax * ((h2 - tY) * tZ - yz), ax * (tY * tZ + yz)};
+    const double vy[4] = {ay * ((h1 - tX) * (h3 - tZ) + xz), ay * (tX * (h3 - tZ) - xz),
+                          ay * ((h1 - tX) * tZ - xz), ay * (tX * tZ + xz)};
+    const double vz[4] = {az * ((h1 - tX) * (h2 - tY) + xy), az * (tX * (h2 - tY) - xy),
+                          az * ((h1 - tX) * tY - xy), az * (tX * tY + xy)};
+    v.seg(i, j, k, vx, vy, vz);
 }
 
 /* Two or three face crossings, source/core/plasma_currentKernel.c:156-279. */
-__device__ static void deposit_multi(const F64View &v, int i1, int j1, int k1, int i2, int j2,
-                                     int k2, double x1, double y1, double z1, double x2,
+template <class S>
+__device__ static void deposit_multi(const S &v, const Grid &g, int i1, int j1, int k1, int i2,
+                                     int j2, int k2, double x1, double y1, double z1, double x2,
                                      double y2, double z2, double rp)
 {
-    const double h1 = v.g.h[0], h2 = v.g.h[1], h3 = v.g.h[2];
+    const double h1 = g.h[0], h2 = g.h[1], h3 = g.h[2];
     const int di = abs(i2 - i1), dj = abs(j2 - j1), dk = abs(k2 - k1);
     if (di + dj + dk == 2) {
         int i = i1, j = j1, k = k1;
@@ -99,15 +149,15 @@ __device__ static void deposit_multi(const F64View &v, int i1, int j1, int k1, i
         const double xA = x1 + k1s * (x2 - x1), yA = y1 + k1s * (y2 - y1), zA = z1 + k1s * (z2 - z1);
         const double xB = x1 + k2s * (x2 - x1), yB = y1 + k2s * (y2 - y1), zB = z1 + k2s * (z2 - z1);
         const int *fst = st ? dI1 : dI0, *snd = st ? dI0 : dI1;
-        deposit_seg(v, i, j, k, x1, xA, y1, yA, z1, zA, rp);
+        deposit_seg(v, g, i, j, k, x1, xA, y1, yA, z1, zA, rp);
         i += fst[0];
         j += fst[1];
         k += fst[2];
-        deposit_seg(v, i, j, k, xA, xB, yA, yB, zA, zB, rp);
+        deposit_seg(v, g, i, j, k, xA, xB, yA, yB, zA, zB, rp);
         i += snd[0];
         j += snd[1];
         k += snd[2];
-        deposit_seg(v, i, j, k, xB, x2, yB, y2, zB, z2, rp);
+        deposit_seg(v, g, i, j, k, xB, x2, yB, y2, zB, z2, rp);
         return;
     }
     const double xC = (i1 + i2 + 1) * 0.5 * h1, yC = (j1 + j2 + 1) * 0.5 * h2,
@@ -130,14 +180,14 @@ __device__ static void deposit_multi(const F64View &v, int i1, int j1, int k1, i
     if (a1 > a3) { int t = s1; s1 = s3; s3 = t; double d = a1; a1 = a3; a3 = d; }
     if (a2 > a3) { int t = s2; s2 = s3; s3 = t; double d = a2; a2 = a3; a3 = d; }
     int ii[3] = {i1, j1, k1};
-    deposit_seg(v, ii[0], ii[1], ii[2], x1, rc[s1][0], y1, rc[s1][1], z1, rc[s1][2], rp);
+    deposit_seg(v, g, ii[0], ii[1], ii[2], x1, rc[s1][0], y1, rc[s1][1], z1, rc[s1][2], rp);
     ii[s1] += dI[s1];
-    deposit_seg(v, ii[0], ii[1], ii[2], rc[s1][0], rc[s2][0], rc[s1][1], rc[s2][1], rc[s1][2],
+    deposit_seg(v, g, ii[0], ii[1], ii[2], rc[s1][0], rc[s2][0], rc[s1][1], rc[s2][1], rc[s1][2],
                 rc[s2][2], rp);
     ii[s2] += dI[s2];
-    deposit_seg(v, ii[0], ii[1], ii[2], rc[s2][0], rc[s3][0], rc[s2][1], rc[s3][1], rc[s2][2],
+    deposit_seg(v, g, ii[0], ii[1], ii[2], rc[s2][0], rc[s3][0], rc[s2][1], rc[s3][1], rc[s2][2],
                 rc[s3][2], rp);
-    deposit_seg(v, i2, j2, k2, rc[s3][0], x2, rc[s3][1], y2, rc[s3][2], z2, rp);
+    deposit_seg(v, g, i2, j2, k2, rc[s3][0], x2, rc[s3][1], y2, rc[s3][2], z2, rp);
 }
 
 /* Particle walls in the reference order xMin yMin zMin xMax yMax zMax.
@@ -162,103 +212,81 @@ __device__ static inline int walls(const PushConsts &c, double *r, double *u)
     return 1;
 }
 
-/* mode 0: full push; mode 1: walls only (plasma_init).  A marker whose
- * cell lies outside [-1, n] (a state the reference never produces) is
- * deleted and flagged in *err instead of touching memory off the grid. */
-__global__ void k_push64(Markers64 m, long n, F64View v, PushConsts c, double *Wacc, int mode,
-                         unsigned *err)
+/* Gather, Boris and deposit of one live marker; returns 0 for a marker off
+ * the grid (deleted and flagged), else applies the walls and returns
+ * whether the marker survives.  r/u are in/out. */
+template <class S>
+__device__ static int push_marker(const S &sink, const F64View &v, const PushConsts &c, double r[3],
+                                  double u[3], float rhof, float qmf, double w3[3], unsigned *err)
+{
+    const Grid &g = v.g;
+    const double h1 = g.h[0], h2 = g.h[1], h3 = g.h[2], tau = g.tau;
+    const double x1 = r[0], y1 = r[1], z1 = r[2];
+    if (!(x1 / h1 > -1.0 && x1 / h1 < g.n[0] + 1.0 && y1 / h2 > -1.0 &&
+          y1 / h2 < g.n[1] + 1.0 && z1 / h3 > -1.0 && z1 / h3 < g.n[2] + 1.0)) {
+        atomicOr(err, 1u);
+        return 0;
+    }
+    double sx, sy, sz, sx_, sy_, sz_;
+    int i, j, k, i_, j_, k_;
+    sx = x1 / h1; sx_ = sx + 0.5; i = DBL_TO_INT(sx); i_ = DBL_TO_INT(sx_); sx -= i; sx_ -= i_;
+    sy = y1 / h2; sy_ = sy + 0.5; j = DBL_TO_INT(sy); j_ = DBL_TO_INT(sy_); sy -= j; sy_ -= j_;
+    sz = z1 / h3; sz_ = sz + 0.5; k = DBL_TO_INT(sz); k_ = DBL_TO_INT(sz_); sz -= k; sz_ -= k_;
+    const double Ex = gather(v.E[0], g, i_, sx_, j, sy, k, sz),
+                 Ey = gather(v.E[1], g, i, sx, j_, sy_, k, sz),
+                 Ez = gather(v.E[2], g, i, sx, j, sy, k_, sz_),
+                 Hx = gather(v.H[0], g, i, sx, j_, sy_, k_, sz_),
+                 Hy = gather(v.H[1], g, i_, sx_, j, sy, k_, sz_),
+                 Hz = gather(v.H[2], g, i_, sx_, j_, sy_, k, sz);
+    const double tauMass = 0.5 * tau * qmf, px = u[0] + tauMass * Ex, py = u[1] + tauMass * Ey,
+                 pz = u[2] + tauMass * Ez, gamma = sqrt(1 + px * px + py * py + pz * pz);
+    const double W = (gamma - 1.0) * rhof / (px * px + py * py + pz * pz + 1e-19) * h1 * h2 * h3 / qmf;
+    w3[0] += W * px * px;
+    w3[1] += W * py * py;
+    w3[2] += W * pz * pz;
+    const double tauGamma = tauMass / gamma;
+    const double aHx = tauGamma * Hx, aHy = tauGamma * Hy, aHz = tauGamma * Hz;
+    const double uu = px + py * aHz - pz * aHy, vv = py + pz * aHx - px * aHz,
+                 ww = pz + px * aHy - py * aHx;
+    const double bxx = aHx * aHx, byy = aHy * aHy, bzz = aHz * aHz, bxy = aHx * aHy,
+                 byz = aHy * aHz, bxz = aHz * aHx;
+    const double delta = 1.0 / (1 + bxx + byy + bzz);
+    u[0] = tauMass * Ex + ((1 + bxx) * uu + (bxy + aHz) * vv + (bxz - aHy) * ww) * delta;
+    u[1] = tauMass * Ey + ((bxy - aHz) * uu + (1 + byy) * vv + (byz + aHx) * ww) * delta;
+    u[2] = tauMass * Ez + ((bxz + aHy) * uu + (byz - aHx) * vv + (1 + bzz) * ww) * delta;
+    const double tg = tau / sqrt(1 + u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+    const double x2 = x1 + u[0] * tg, y2 = y1 + u[1] * tg, z2 = z1 + u[2] * tg;
+    int i2 = DBL_TO_INT(x2 / h1), j2 = DBL_TO_INT(y2 / h2), k2 = DBL_TO_INT(z2 / h3);
+    int di = abs(i2 - i), dj = abs(j2 - j), dk = abs(k2 - k);
+    if (fabs(x2 - x1) < 1e-10 * h1) { di = 0; i2 = i; }
+    if (fabs(y2 - y1) < 1e-10 * h2) { dj = 0; j2 = j; }
+    if (fabs(z2 - z1) < 1e-10 * h3) { dk = 0; k2 = k; }
+    const double rp = rhof;
+    if (di + dj + dk == 0) {
+        deposit_seg(sink, g, i, j, k, x1, x2, y1, y2, z1, z2, rp);
+    } else if (di + dj + dk == 1) {
+        const double alpha = dk * ((k + k2 + 1) * 0.5 * h3 - z1) / (z2 - z1 + ((1 - dk) << 16))
+                           + dj * ((j + j2 + 1) * 0.5 * h2 - y1) / (y2 - y1 + ((1 - dj) << 16))
+                           + di * ((i + i2 + 1) * 0.5 * h1 - x1) / (x2 - x1 + ((1 - di) << 16));
+        const double xC = x1 + alpha * (x2 - x1), yC = y1 + alpha * (y2 - y1),
+                     zC = z1 + alpha * (z2 - z1);
+        deposit_seg(sink, g, i, j, k, x1, xC, y1, yC, z1, zC, rp);
+        deposit_seg(sink, g, i2, j2, k2, xC, x2, yC, y2, zC, z2, rp);
+    } else {
+        deposit_multi(sink, g, i, j, k, i2, j2, k2, x1, y1, z1, x2, y2, z2, rp);
+    }
+    r[0] = x2;
+    r[1] = y2;
+    r[2] = z2;
+    return walls(c, r, u);
+}
+
+__device__ static void block_sum3(double w3[3], double *Wacc)
 {
     __shared__ double sw[3][TPB];
-    long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    double wx = 0, wy = 0, wz = 0;
-    if (p < n && m.qm[p] != 0.0f) {
-        double r[3] = {m.x[p], m.y[p], m.z[p]};
-        double u[3] = {m.ux[p], m.uy[p], m.uz[p]};
-        if (mode == 0) {
-            const Grid &g = v.g;
-            const double h1 = g.h[0], h2 = g.h[1], h3 = g.h[2], tau = g.tau;
-            const double x1 = r[0], y1 = r[1], z1 = r[2];
-            const float rhof = m.w[p], qmf = m.qm[p];
-            double sx, sy, sz, sx_, sy_, sz_;
-            int i, j, k, i_, j_, k_;
-            sx = x1 / h1; sx_ = sx + 0.5; i = DBL_TO_INT(sx); i_ = DBL_TO_INT(sx_); sx -= i; sx_ -= i_;
-            sy = y1 / h2; sy_ = sy + 0.5; j = DBL_TO_INT(sy); j_ = DBL_TO_INT(sy_); sy -= j; sy_ -= j_;
-            sz = z1 / h3; sz_ = sz + 0.5; k = DBL_TO_INT(sz); k_ = DBL_TO_INT(sz_); sz -= k; sz_ -= k_;
-            if (!(x1 / h1 > -1.0 && x1 / h1 < g.n[0] + 1.0 && y1 / h2 > -1.0 &&
-                  y1 / h2 < g.n[1] + 1.0 && z1 / h3 > -1.0 && z1 / h3 < g.n[2] + 1.0)) {
-                atomicOr(err, 1u);
-                m.qm[p] = 0.0f;
-                m.w[p] = 0.0f;
-                goto reduce;
-            }
-            {
-            const double Ex = gather(v.E[0], g, i_, sx_, j, sy, k, sz),
-                         Ey = gather(v.E[1], g, i, sx, j_, sy_, k, sz),
-                         Ez = gather(v.E[2], g, i, sx, j, sy, k_, sz_),
-                         Hx = gather(v.H[0], g, i, sx, j_, sy_, k_, sz_),
-                         Hy = gather(v.H[1], g, i_, sx_, j, sy, k_, sz_),
-                         Hz = gather(v.H[2], g, i_, sx_, j_, sy_, k, sz);
-            const double tauMass = 0.5 * tau * qmf, px = u[0] + tauMass * Ex,
-                         py = u[1] + tauMass * Ey, pz = u[2] + tauMass * Ez,
-                         gamma = sqrt(1 + px * px + py * py + pz * pz);
-            const double W = (gamma - 1.0) * rhof / (px * px + py * py + pz * pz + 1e-19) * h1 * h2 *
-                             h3 / qmf;
-            wx = W * px * px;
-            wy = W * py * py;
-            wz = W * pz * pz;
-            const double tauGamma = tauMass / gamma;
-            const double aHx = tauGamma * Hx, aHy = tauGamma * Hy, aHz = tauGamma * Hz;
-            const double uu = px + py * aHz - pz * aHy, vv = py + pz * aHx - px * aHz,
-                         ww = pz + px * aHy - py * aHx;
-            const double bxx = aHx * aHx, byy = aHy * aHy, bzz = aHz * aHz, bxy = aHx * aHy,
-                         byz = aHy * aHz, bxz = aHz * aHx;
-            const double delta = 1.0 / (1 + bxx + byy + bzz);
-            u[0] = tauMass * Ex + ((1 + bxx) * uu + (bxy + aHz) * vv + (bxz - aHy) * ww) * delta;
-            u[1] = tauMass * Ey + ((bxy - aHz) * uu + (1 + byy) * vv + (byz + aHx) * ww) * delta;
-            u[2] = tauMass * Ez + ((bxz + aHy) * uu + (byz - aHx) * vv + (1 + bzz) * ww) * delta;
-            const double tg = tau / sqrt(1 + u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
-            const double x2 = x1 + u[0] * tg, y2 = y1 + u[1] * tg, z2 = z1 + u[2] * tg;
-            int i2 = DBL_TO_INT(x2 / h1), j2 = DBL_TO_INT(y2 / h2), k2 = DBL_TO_INT(z2 / h3);
-            int di = abs(i2 - i), dj = abs(j2 - j), dk = abs(k2 - k);
-            if (fabs(x2 - x1) < 1e-10 * h1) { di = 0; i2 = i; }
-            if (fabs(y2 - y1) < 1e-10 * h2) { dj = 0; j2 = j; }
-            if (fabs(z2 - z1) < 1e-10 * h3) { dk = 0; k2 = k; }
-            const double rp = rhof;
-            if (di + dj + dk == 0) {
-                deposit_seg(v, i, j, k, x1, x2, y1, y2, z1, z2, rp);
-            } else if (di + dj + dk == 1) {
-                const double alpha = dk * ((k + k2 + 1) * 0.5 * h3 - z1) / (z2 - z1 + ((1 - dk) << 16))
-                                   + dj * ((j + j2 + 1) * 0.5 * h2 - y1) / (y2 - y1 + ((1 - dj) << 16))
-                                   + di * ((i + i2 + 1) * 0.5 * h1 - x1) / (x2 - x1 + ((1 - di) << 16));
-                const double xC = x1 + alpha * (x2 - x1), yC = y1 + alpha * (y2 - y1),
-                             zC = z1 + alpha * (z2 - z1);
-                deposit_seg(v, i, j, k, x1, xC, y1, yC, z1, zC, rp);
-                deposit_seg(v, i2, j2, k2, xC, x2, yC, y2, zC, z2, rp);
-            } else {
-                deposit_multi(v, i, j, k, i2, j2, k2, x1, y1, z1, x2, y2, z2, rp);
-            }
-            r[0] = x2;
-            r[1] = y2;
-            r[2] = z2;
-            }
-        }
-        if (!walls(c, r, u)) {
-            m.qm[p] = 0.0f;
-            m.w[p] = 0.0f;
-        } else {
-            m.x[p] = r[0];
-            m.y[p] = r[1];
-            m.z[p] = r[2];
-            m.ux[p] = u[0];
-            m.uy[p] = u[1];
-            m.uz[p] = u[2];
-        }
-    }
-reduce:
-    if (mode != 0) return;
-    sw[0][threadIdx.x] = wx;
-    sw[1][threadIdx.x] = wy;
-    sw[2][threadIdx.x] = wz;
+    sw[0][threadIdx.x] = w3[0];
+    sw[1][threadIdx.x] = w3[1];
+    sw[2][threadIdx.x] = w3[2];
     __syncthreads();
     for (int s = TPB / 2; s > 0; s >>= 1) {
         if ((int)threadIdx.x < s) {
@@ -275,6 +303,39 @@ reduce:
     }
 }
 
+/* Untiled push: one thread per marker, J straight to HBM.  mode 1 applies
+ * the walls only (plasma_init).  A marker whose cell lies outside [-1, n]
+ * (a state the reference never produces) is deleted and flagged in *err
+ * instead of touching memory off the grid. */
+__global__ void k_push64(Markers64 m, long n, F64View v, PushConsts c, double *Wacc, int mode,
+                         unsigned *err)
+{
+    long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double w3[3] = {0, 0, 0};
+    if (p < n && m.qm[p] != 0.0f) {
+        double r[3] = {m.x[p], m.y[p], m.z[p]};
+        double u[3] = {m.ux[p], m.uy[p], m.uz[p]};
+        GlobalSink gs;
+        for (int a = 0; a < 3; ++a) gs.J[a] = v.J[a];
+        gs.g = v.g;
+        int alive = (mode == 0) ? push_marker(gs, v, c, r, u, m.w[p], m.qm[p], w3, err)
+                                : walls(c, r, u);
+        if (!alive) {
+            m.qm[p] = 0.0f;
+            m.w[p] = 0.0f;
+        } else {
+            m.x[p] = r[0];
+            m.y[p] = r[1];
+            m.z[p] = r[2];
+            m.ux[p] = u[0];
+            m.uy[p] = u[1];
+            m.uz[p] = u[2];
+        }
+    }
+    if (mode != 0) return;
+    block_sum3(w3, Wacc);
+}
+
 int launch_push64(hipStream_t s, const Markers64 &m, long n, const Fields64 &f, const Grid &g,
                   const PushConsts &c, double *Wacc, int mode, unsigned *err)
 {
@@ -287,6 +348,103 @@ int launch_push64(hipStream_t s, const Markers64 &m, long n, const Fields64 &f, 
     }
     v.g = g;
     k_push64<<<(unsigned)((n + TPB - 1) / TPB), TPB, 0, s>>>(m, n, v, c, Wacc, mode, err);
+    MPIC_CHECK(hipGetLastError());
+    return MPIC_OK;
+}
+
+/* Tiled push: persistent workgroups pull 4x4x4-cell tiles (markers sorted
+ * by tile-major cell key) from per-XCD queues, accumulate the tile's J in
+ * LDS and flush it to HBM once; every marker leaves with its new cell key
+ * and the per-cell counts of the new positions for the next sort. */
+__global__ void __launch_bounds__(TPB) k_push64_tiled(Markers64 m, F64View v, PushConsts c,
+                                                      TileView tv, double *Wacc, unsigned *err)
+{
+    __shared__ double lj[TJN * TJN * TJN * 3];
+    __shared__ int s_tile;
+    const Grid &g = v.g;
+    const int xcd = blockIdx.x % MPIC_XCDS;
+    double w3[3] = {0, 0, 0};
+    GlobalSink gs;
+    for (int a = 0; a < 3; ++a) gs.J[a] = v.J[a];
+    gs.g = g;
+    for (;;) {
+        if (threadIdx.x == 0) s_tile = tile_next(tv, xcd);
+        __syncthreads();
+        const int t = s_tile;
+        if (t < 0) break;
+        int t0, t1, t2;
+        tile_coords(tv, t, t0, t1, t2);
+        TileSink ts;
+        ts.gs = gs;
+        ts.lj = lj;
+        ts.o0 = TILE * t0 - 1;
+        ts.o1 = TILE * t1 - 1;
+        ts.o2 = TILE * t2 - 1;
+        for (int q = threadIdx.x; q < TJN * TJN * TJN * 3; q += TPB) lj[q] = 0.0;
+        __syncthreads();
+        const long beg = tv.off[(long)t * TILE_CELLS], end = tv.off[(long)(t + 1) * TILE_CELLS];
+        for (long base = beg; base < end; base += TPB) {
+            const long p = base + threadIdx.x;
+            int key = -1;
+            if (p < end) {
+                double r[3] = {m.x[p], m.y[p], m.z[p]};
+                double u[3] = {m.ux[p], m.uy[p], m.uz[p]};
+                const float qm = m.qm[p];
+                int alive = (qm != 0.0f) && push_marker(ts, v, c, r, u, m.w[p], qm, w3, err);
+                if (!alive) {
+                    m.qm[p] = 0.0f;
+                    m.w[p] = 0.0f;
+                } else {
+                    m.x[p] = r[0];
+                    m.y[p] = r[1];
+                    m.z[p] = r[2];
+                    m.ux[p] = u[0];
+                    m.uy[p] = u[1];
+                    m.uz[p] = u[2];
+                    key = cell_key_of(tv, g, r);
+                }
+                tv.key[p] = key;
+            }
+            wave_run_count(tv.count, key);
+        }
+        __syncthreads();
+        /* flush the tile's J block: one HBM atomic per non-zero node value */
+        for (int q = threadIdx.x; q < TJN * TJN * TJN * 3; q += TPB) {
+            const double val = lj[q];
+            if (val == 0.0) continue;
+            const int comp = q % 3, node = q / 3;
+            const int cc = node % TJN, bb = (node / TJN) % TJN, aa = node / (TJN * TJN);
+            const int i = ts.o0 + aa, j = ts.o1 + bb, k = ts.o2 + cc;
+            if (i < -2 || j < -2 || k < -2 || i > g.n[0] + 2 || j > g.n[1] + 2 || k > g.n[2] + 2)
+                continue;
+            unsafeAtomicAdd(v.J[comp] + gidx(g, i, j, k), val);
+        }
+        __syncthreads();
+    }
+    block_sum3(w3, Wacc);
+}
+
+/* Workgroups of the persistent push that fit on one CU at once. */
+int push64_blocks_per_cu(void)
+{
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_push64_tiled, TPB, 0) != hipSuccess || nb < 1)
+        nb = 1;
+    return nb;
+}
+
+int launch_push64_tiled(hipStream_t s, const Markers64 &m, const Fields64 &f, const Grid &g,
+                        const PushConsts &c, const TileView &tv, double *Wacc, unsigned *err,
+                        int nblocks)
+{
+    F64View v;
+    for (int a = 0; a < 3; ++a) {
+        v.E[a] = f.E[a];
+        v.H[a] = f.H[a];
+        v.J[a] = f.J[a];
+    }
+    v.g = g;
+    k_push64_tiled<<<nblocks, TPB, 0, s>>>(m, v, c, tv, Wacc, err);
     MPIC_CHECK(hipGetLastError());
     return MPIC_OK;
 }

@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "mpic_internal.h"
+#include "mpic_tiles.h"
 
 static thread_local char g_err[512] = "";
 
@@ -39,6 +40,15 @@ extern "C" int mpic_device_count(void)
         if (rc_ != MPIC_OK) return rc_; \
     } while (0)
 
+/* Calls a field launcher with the run's field set (FP64 or FP32). */
+#define FCALL(fn, ...)                                                   \
+    do {                                                                 \
+        if (r->fp32)                                                     \
+            TRY(fn(r->stream, r->f32, __VA_ARGS__));                     \
+        else                                                             \
+            TRY(fn(r->stream, r->f, __VA_ARGS__));                       \
+    } while (0)
+
 struct Laser {
     double  A, T0, delay, dt, phi0;
     int     forward, plane;
@@ -56,11 +66,24 @@ struct mpic_run {
     Grid          g;
     int           dev;
     hipStream_t   stream;
-    Fields64      f;
-    double       *fblock;     /* all field arrays */
+    int           fp32;       /* MPIC_FP32_SPLIT run */
+    Fields64      f;          /* FP64 fields (plain form) */
+    Fields32      f32;        /* FP32 fields (split form); J is FP64 in both */
+    JPtrs         jp;
+    void         *fblock;     /* all field arrays */
+    Markers32     m32, m32buf[2];
+    float         qm[8];      /* q/m per species of the split form */
+    int           nspecies;
     double       *rho1, *rho2;
-    Markers64     m;
-    long          n, cap;
+    Markers64     m;          /* current marker order */
+    Markers64     mbuf[2];    /* double buffer of the per-step sort */
+    int           cur;        /* m == mbuf[cur] */
+    long          n, cap;     /* n: markers loaded (upper bound of the live count) */
+    int           tiled;      /* 1: sorted tiled push (default), 0: untiled push */
+    TileView      tv;
+    int          *blocksum;
+    int           parity;     /* tv.nmark[parity] = markers in the current order */
+    int           push_blocks;
     CapBoxes      cH, cE;
     BoxSet        updH, updE, flH, flE;
     std::vector<FaceOp> bcH, bcE;
@@ -147,7 +170,8 @@ static BoxSet make_set(const Box *bx)
     return s;
 }
 
-static FaceOp face(int kind, int ea, int top, const Box &ext, double *const X[3], int N)
+template <class T>
+static FaceOp face(int kind, int ea, int top, const Box &ext, T *const X[3], int N)
 {
     FaceOp op;
     memset(&op, 0, sizeof(op));
@@ -160,13 +184,15 @@ static FaceOp face(int kind, int ea, int top, const Box &ext, double *const X[3]
     op.phi = ext.hi[ep];
     op.qlo = ext.lo[eq];
     op.qhi = ext.hi[eq];
-    for (int c = 0; c < 3; ++c) op.X[c] = X[c];
+    for (int c = 0; c < 3; ++c) op.X[c] = (void *)X[c];
+    op.fp32 = (sizeof(T) == 4);
     return op;
 }
 
 /* Wall sequences of cap_cacheH / cap_cacheEH (source/core/em_caps.c:470-553,
  * 631-721): x, y, z; low side then high side. */
-static int build_wall_ops(mpic_run *r)
+template <class T>
+static int build_wall_ops_t(mpic_run *r, T *const Hc[3], T *const Ec[3])
 {
     const int *bc = r->cfg.bc;
     r->bcH.clear();
@@ -178,19 +204,19 @@ static int build_wall_ops(mpic_run *r)
             const Box &eH = r->cH.fl[(ea << 1) | top], &eE = r->cE.fl[(ea << 1) | top];
             if (b == MPIC_BC_PERIODIC) {
                 if (top) continue;
-                r->bcH.push_back(face(FACE_PERIODIC_H, ea, 0, eH, r->f.Hc, N));
-                r->bcE.push_back(face(FACE_PERIODIC_E, ea, 0, eE, r->f.Ec, N));
+                r->bcH.push_back(face(FACE_PERIODIC_H, ea, 0, eH, Hc, N));
+                r->bcE.push_back(face(FACE_PERIODIC_E, ea, 0, eE, Ec, N));
             } else if (b == MPIC_BC_MIRROR) {
-                FaceOp h = face(FACE_WALL_H, ea, top, eH, r->f.Hc, N);
+                FaceOp h = face(FACE_WALL_H, ea, top, eH, Hc, N);
                 h.zeroN = 1;
                 r->bcH.push_back(h);
-                r->bcE.push_back(face(FACE_MIRROR_E, ea, top, eE, r->f.Ec, N));
+                r->bcE.push_back(face(FACE_MIRROR_E, ea, top, eE, Ec, N));
             } else if (b == MPIC_BC_OPEN) {
-                r->bcH.push_back(face(FACE_WALL_H, ea, top, eH, r->f.Hc, N));
-                FaceOp e = face(FACE_MUR_E, ea, top, eE, r->f.Ec, N);
+                r->bcH.push_back(face(FACE_WALL_H, ea, top, eH, Hc, N));
+                FaceOp e = face(FACE_MUR_E, ea, top, eE, Ec, N);
                 const long nodes = (long)(e.phi - e.plo + 1) * (e.qhi - e.qlo + 1);
                 double *buf = nullptr;
-                MPIC_CHECK(hipMalloc(&buf, sizeof(double) * 4 * nodes));
+                MPIC_CHECK(hipMalloc(&buf, sizeof(T) * 4 * nodes));
                 r->murbuf.push_back(buf);
                 e.buf = buf;
                 e.alpha = (r->g.tau - r->g.h[ea]) / (r->g.tau + r->g.h[ea]);
@@ -204,18 +230,28 @@ static int build_wall_ops(mpic_run *r)
     return MPIC_OK;
 }
 
-/* ---------------- lifecycle ---------------- */
-static size_t field_bytes(const int n[3])
+static int build_wall_ops(mpic_run *r)
 {
-    return sizeof(double) * 15 * (size_t)(n[0] + 5) * (n[1] + 5) * (n[2] + 5);
+    return r->fp32 ? build_wall_ops_t<float>(r, r->f32.Hc, r->f32.Ec)
+                   : build_wall_ops_t<double>(r, r->f.Hc, r->f.Ec);
+}
+
+/* ---------------- lifecycle ---------------- */
+static size_t field_bytes(const int n[3], int fp32)
+{
+    const size_t cells = (size_t)(n[0] + 5) * (n[1] + 5) * (n[2] + 5);
+    /* E H Hc Ec in the field precision, J in FP64 */
+    return (fp32 ? sizeof(float) : sizeof(double)) * 12 * cells + sizeof(double) * 3 * cells;
 }
 
 extern "C" int mpic_footprint(const mpic_config_t *cfg, size_t *bytes)
 {
     if (!cfg || !bytes) return MPIC_ERR_ARG;
     const size_t cells = (size_t)(cfg->n[0] + 5) * (cfg->n[1] + 5) * (cfg->n[2] + 5);
-    *bytes = field_bytes(cfg->n) + 2 * sizeof(double) * cells +
-             (size_t)cfg->max_markers * (6 * sizeof(double) + 2 * sizeof(float) + 4);
+    const size_t tiles = (size_t)((cfg->n[0] + 3) / 4) * ((cfg->n[1] + 3) / 4) * ((cfg->n[2] + 3) / 4);
+    const size_t mk = (cfg->precision == MPIC_FP32_SPLIT) ? (4 + 7 * 4 + 4) : (6 * 8 + 2 * 4 + 4 + 4);
+    *bytes = field_bytes(cfg->n, cfg->precision == MPIC_FP32_SPLIT) + 2 * sizeof(double) * cells +
+             2 * (size_t)cfg->max_markers * mk + tiles * 64 * 3 * sizeof(int);
     return MPIC_OK;
 }
 
@@ -237,8 +273,8 @@ extern "C" int mpic_create(const mpic_config_t *cfg, mpic_run_t **out)
         mpic_set_error("mpic_create: Yee solver is unstable (Courant)");
         return MPIC_ERR_ARG;
     }
-    if (cfg->precision != MPIC_FP64) {
-        mpic_set_error("mpic_create: this library build supports MPIC_FP64 runs only");
+    if (cfg->precision != MPIC_FP64 && cfg->precision != MPIC_FP32_SPLIT) {
+        mpic_set_error("mpic_create: unknown precision %d", cfg->precision);
         return MPIC_ERR_ARG;
     }
     mpic_run *r = new mpic_run();
@@ -255,29 +291,98 @@ extern "C" int mpic_create(const mpic_config_t *cfg, mpic_run_t **out)
     g.GY = cfg->n[1] + 5;
     g.GZ = cfg->n[2] + 5;
     g.ncell = (long)(cfg->n[0] + 5) * g.GY * g.GZ;
-    MPIC_CHECK(hipMalloc(&r->fblock, field_bytes(cfg->n)));
+    r->fp32 = (cfg->precision == MPIC_FP32_SPLIT);
+    const size_t fb = field_bytes(cfg->n, r->fp32);
+    MPIC_CHECK(hipMalloc(&r->fblock, fb));
     /* NaN fill of E/H and the caps, as the reference's NaN-filled meshes;
      * J starts at zero. */
-    MPIC_CHECK(hipMemsetAsync(r->fblock, 0xff, field_bytes(cfg->n), r->stream));
-    double *p = r->fblock;
-    for (int c = 0; c < 3; ++c) r->f.E[c] = p + (0 + c) * g.ncell;
-    for (int c = 0; c < 3; ++c) r->f.H[c] = p + (3 + c) * g.ncell;
-    for (int c = 0; c < 3; ++c) r->f.J[c] = p + (6 + c) * g.ncell;
-    for (int c = 0; c < 3; ++c) r->f.Hc[c] = p + (9 + c) * g.ncell;
-    for (int c = 0; c < 3; ++c) r->f.Ec[c] = p + (12 + c) * g.ncell;
-    MPIC_CHECK(hipMemsetAsync(r->f.J[0], 0, sizeof(double) * 3 * g.ncell, r->stream));
+    MPIC_CHECK(hipMemsetAsync(r->fblock, 0xff, fb, r->stream));
+    double *jb = (double *)r->fblock;
+    if (r->fp32) {
+        float *p = (float *)r->fblock;
+        for (int c = 0; c < 3; ++c) r->f32.E[c] = p + (0 + c) * g.ncell;
+        for (int c = 0; c < 3; ++c) r->f32.H[c] = p + (3 + c) * g.ncell;
+        for (int c = 0; c < 3; ++c) r->f32.Hc[c] = p + (6 + c) * g.ncell;
+        for (int c = 0; c < 3; ++c) r->f32.Ec[c] = p + (9 + c) * g.ncell;
+        jb = (double *)(p + 12 * g.ncell);
+        for (int c = 0; c < 3; ++c) r->f32.J[c] = jb + c * g.ncell;
+    } else {
+        double *p = (double *)r->fblock;
+        for (int c = 0; c < 3; ++c) r->f.E[c] = p + (0 + c) * g.ncell;
+        for (int c = 0; c < 3; ++c) r->f.H[c] = p + (3 + c) * g.ncell;
+        for (int c = 0; c < 3; ++c) r->f.Hc[c] = p + (6 + c) * g.ncell;
+        for (int c = 0; c < 3; ++c) r->f.Ec[c] = p + (9 + c) * g.ncell;
+        jb = p + 12 * g.ncell;
+        for (int c = 0; c < 3; ++c) r->f.J[c] = jb + c * g.ncell;
+    }
+    for (int c = 0; c < 3; ++c) r->jp.J[c] = jb + c * g.ncell;
+    MPIC_CHECK(hipMemsetAsync(jb, 0, sizeof(double) * 3 * g.ncell, r->stream));
     MPIC_CHECK(hipMalloc(&r->rho1, sizeof(double) * g.ncell));
     MPIC_CHECK(hipMalloc(&r->rho2, sizeof(double) * g.ncell));
     r->cap = cfg->max_markers > 0 ? cfg->max_markers : 1;
-    MPIC_CHECK(hipMalloc(&r->m.x, sizeof(double) * r->cap));
-    MPIC_CHECK(hipMalloc(&r->m.y, sizeof(double) * r->cap));
-    MPIC_CHECK(hipMalloc(&r->m.z, sizeof(double) * r->cap));
-    MPIC_CHECK(hipMalloc(&r->m.ux, sizeof(double) * r->cap));
-    MPIC_CHECK(hipMalloc(&r->m.uy, sizeof(double) * r->cap));
-    MPIC_CHECK(hipMalloc(&r->m.uz, sizeof(double) * r->cap));
-    MPIC_CHECK(hipMalloc(&r->m.w, sizeof(float) * r->cap));
-    MPIC_CHECK(hipMalloc(&r->m.qm, sizeof(float) * r->cap));
-    MPIC_CHECK(hipMalloc(&r->m.id, sizeof(int32_t) * r->cap));
+    if (r->cap >= (1L << 31)) {
+        mpic_set_error("mpic_create: capacity %ld exceeds 2^31 - 1 markers", r->cap);
+        delete r;
+        return MPIC_ERR_ARG;
+    }
+    for (int b = 0; b < 2 && r->fp32; ++b) {
+        Markers32 &mb = r->m32buf[b];
+        MPIC_CHECK(hipMalloc(&mb.cell, sizeof(int32_t) * r->cap));
+        float **fs[7] = {&mb.ox, &mb.oy, &mb.oz, &mb.ux, &mb.uy, &mb.uz, &mb.w};
+        for (int q = 0; q < 7; ++q) MPIC_CHECK(hipMalloc(fs[q], sizeof(float) * r->cap));
+        MPIC_CHECK(hipMalloc(&mb.id, sizeof(int32_t) * r->cap));
+    }
+    r->m32 = r->m32buf[0];
+    for (int b = 0; b < 2 && !r->fp32; ++b) {
+        Markers64 &mb = r->mbuf[b];
+        MPIC_CHECK(hipMalloc(&mb.x, sizeof(double) * r->cap));
+        MPIC_CHECK(hipMalloc(&mb.y, sizeof(double) * r->cap));
+        MPIC_CHECK(hipMalloc(&mb.z, sizeof(double) * r->cap));
+        MPIC_CHECK(hipMalloc(&mb.ux, sizeof(double) * r->cap));
+        MPIC_CHECK(hipMalloc(&mb.uy, sizeof(double) * r->cap));
+        MPIC_CHECK(hipMalloc(&mb.uz, sizeof(double) * r->cap));
+        MPIC_CHECK(hipMalloc(&mb.w, sizeof(float) * r->cap));
+        MPIC_CHECK(hipMalloc(&mb.qm, sizeof(float) * r->cap));
+        MPIC_CHECK(hipMalloc(&mb.id, sizeof(int32_t) * r->cap));
+    }
+    r->cur = 0;
+    r->m = r->mbuf[0];
+    {
+        const char *e = getenv("MPIC_UNTILED");
+        r->tiled = r->fp32 || !(e && e[0] == '1');
+    }
+    TileView &tv = r->tv;
+    tv.ntiles = 1;
+    for (int a = 0; a < 3; ++a) {
+        tv.nt[a] = (cfg->n[a] + 3) / 4;
+        tv.ntiles *= tv.nt[a];
+    }
+    tv.ncellp = (long)tv.ntiles * TILE_CELLS;
+    MPIC_CHECK(hipMalloc(&tv.off, sizeof(int) * (tv.ncellp + 1)));
+    MPIC_CHECK(hipMalloc(&tv.cursor, sizeof(int) * tv.ncellp));
+    MPIC_CHECK(hipMalloc(&tv.count, sizeof(int) * tv.ncellp));
+    MPIC_CHECK(hipMemsetAsync(tv.count, 0, sizeof(int) * tv.ncellp, r->stream));
+    MPIC_CHECK(hipMemsetAsync(tv.off, 0, sizeof(int) * (tv.ncellp + 1), r->stream));
+    MPIC_CHECK(hipMalloc(&tv.key, sizeof(int) * (r->fp32 ? 1 : r->cap)));
+    MPIC_CHECK(hipMalloc(&tv.active, sizeof(int) * tv.ntiles));
+    MPIC_CHECK(hipMalloc(&tv.nactive, sizeof(int) * (4 + MPIC_XCDS)));
+    tv.queue = tv.nactive + 1;
+    tv.nmark = tv.nactive + 1 + MPIC_XCDS;
+    MPIC_CHECK(hipMemsetAsync(tv.nactive, 0, sizeof(int) * (4 + MPIC_XCDS), r->stream));
+    /* scan scratch: block sums (2 per block + totals) and 2 ints per tile */
+    MPIC_CHECK(hipMalloc(&r->blocksum,
+                         sizeof(int) * (2 * ((tv.ntiles + 1023) / 1024 + 1) + 2 * (long)tv.ntiles)));
+    r->parity = 0;
+    {
+        int ncu = 256;
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, r->dev) == hipSuccess) ncu = prop.multiProcessorCount;
+        /* exactly the resident workgroups of the persistent push: extra ones
+         * only arrive once the tile queues are drained */
+        r->push_blocks = ncu * (r->fp32 ? push32_blocks_per_cu() : push64_blocks_per_cu());
+        const char *e = getenv("MPIC_PUSH_BLOCKS");
+        if (e && atoi(e) > 0) r->push_blocks = atoi(e);
+    }
     r->n = 0;
     MPIC_CHECK(hipMalloc(&r->dacc, sizeof(double) * 8));
     MPIC_CHECK(hipHostMalloc(&r->hacc, sizeof(double) * 8, hipHostMallocDefault));
@@ -319,9 +424,17 @@ extern "C" int mpic_destroy(mpic_run_t *r)
     hipFree(r->fblock);
     hipFree(r->rho1);
     hipFree(r->rho2);
-    hipFree(r->m.x); hipFree(r->m.y); hipFree(r->m.z);
-    hipFree(r->m.ux); hipFree(r->m.uy); hipFree(r->m.uz);
-    hipFree(r->m.w); hipFree(r->m.qm); hipFree(r->m.id);
+    for (int b = 0; b < 2; ++b) {
+        Markers64 &mb = r->mbuf[b];
+        hipFree(mb.x); hipFree(mb.y); hipFree(mb.z);
+        hipFree(mb.ux); hipFree(mb.uy); hipFree(mb.uz);
+        hipFree(mb.w); hipFree(mb.qm); hipFree(mb.id);
+        Markers32 &m3 = r->m32buf[b];
+        hipFree(m3.cell); hipFree(m3.ox); hipFree(m3.oy); hipFree(m3.oz);
+        hipFree(m3.ux); hipFree(m3.uy); hipFree(m3.uz); hipFree(m3.w); hipFree(m3.id);
+    }
+    hipFree(r->tv.off); hipFree(r->tv.cursor); hipFree(r->tv.count); hipFree(r->tv.key);
+    hipFree(r->tv.active); hipFree(r->tv.nactive); hipFree(r->blocksum);
     hipFree(r->dacc);
     hipFree(r->derr);
     if (r->hacc) hipHostFree(r->hacc);
@@ -337,32 +450,34 @@ extern "C" int mpic_destroy(mpic_run_t *r)
 /* ---------------- host <-> device data ---------------- */
 static int nodes_of(const Grid &g) { return (g.n[0] + 1) * (g.n[1] + 1) * (g.n[2] + 1); }
 
-static int scatter_block(mpic_run *r, const double *src, double *const dst[3])
+template <class T>
+static int scatter_block(mpic_run *r, const double *src, T *const dst[3])
 {
-    /* node block [0,n]^3, interleaved xyz -> padded SoA */
+    /* node block [0,n]^3, interleaved xyz -> padded SoA (ghosts keep NaN) */
     const Grid &g = r->g;
-    std::vector<double> tmp((size_t)g.ncell);
+    std::vector<T> tmp((size_t)g.ncell);
     for (int c = 0; c < 3; ++c) {
-        MPIC_CHECK(hipMemcpy(tmp.data(), dst[c], sizeof(double) * g.ncell, hipMemcpyDeviceToHost));
+        MPIC_CHECK(hipMemcpy(tmp.data(), dst[c], sizeof(T) * g.ncell, hipMemcpyDeviceToHost));
         long q = 0;
         for (int i = 0; i <= g.n[0]; ++i)
             for (int j = 0; j <= g.n[1]; ++j)
-                for (int k = 0; k <= g.n[2]; ++k, ++q) tmp[gidx(g, i, j, k)] = src[3 * q + c];
-        MPIC_CHECK(hipMemcpy(dst[c], tmp.data(), sizeof(double) * g.ncell, hipMemcpyHostToDevice));
+                for (int k = 0; k <= g.n[2]; ++k, ++q) tmp[gidx(g, i, j, k)] = (T)src[3 * q + c];
+        MPIC_CHECK(hipMemcpy(dst[c], tmp.data(), sizeof(T) * g.ncell, hipMemcpyHostToDevice));
     }
     return MPIC_OK;
 }
 
-static int gather_block(mpic_run *r, double *const src[3], double *out)
+template <class T>
+static int gather_block(mpic_run *r, T *const src[3], double *out)
 {
     const Grid &g = r->g;
-    std::vector<double> tmp((size_t)g.ncell);
+    std::vector<T> tmp((size_t)g.ncell);
     for (int c = 0; c < 3; ++c) {
-        MPIC_CHECK(hipMemcpy(tmp.data(), src[c], sizeof(double) * g.ncell, hipMemcpyDeviceToHost));
+        MPIC_CHECK(hipMemcpy(tmp.data(), src[c], sizeof(T) * g.ncell, hipMemcpyDeviceToHost));
         long q = 0;
         for (int i = 0; i <= g.n[0]; ++i)
             for (int j = 0; j <= g.n[1]; ++j)
-                for (int k = 0; k <= g.n[2]; ++k, ++q) out[3 * q + c] = tmp[gidx(g, i, j, k)];
+                for (int k = 0; k <= g.n[2]; ++k, ++q) out[3 * q + c] = (double)tmp[gidx(g, i, j, k)];
     }
     return MPIC_OK;
 }
@@ -372,8 +487,13 @@ extern "C" int mpic_set_fields(mpic_run_t *r, const double *E, const double *H)
     if (!r || !E || !H) return MPIC_ERR_ARG;
     MPIC_CHECK(hipSetDevice(r->dev));
     MPIC_CHECK(hipStreamSynchronize(r->stream));
-    TRY(scatter_block(r, E, r->f.E));
-    TRY(scatter_block(r, H, r->f.H));
+    if (r->fp32) {
+        TRY(scatter_block(r, E, r->f32.E));
+        TRY(scatter_block(r, H, r->f32.H));
+    } else {
+        TRY(scatter_block(r, E, r->f.E));
+        TRY(scatter_block(r, H, r->f.H));
+    }
     return MPIC_OK;
 }
 
@@ -382,8 +502,13 @@ extern "C" int mpic_get_fields(mpic_run_t *r, double *E, double *H)
     if (!r) return MPIC_ERR_ARG;
     MPIC_CHECK(hipSetDevice(r->dev));
     MPIC_CHECK(hipStreamSynchronize(r->stream));
-    if (E) TRY(gather_block(r, r->f.E, E));
-    if (H) TRY(gather_block(r, r->f.H, H));
+    if (r->fp32) {
+        if (E) TRY(gather_block(r, r->f32.E, E));
+        if (H) TRY(gather_block(r, r->f32.H, H));
+    } else {
+        if (E) TRY(gather_block(r, r->f.E, E));
+        if (H) TRY(gather_block(r, r->f.H, H));
+    }
     return MPIC_OK;
 }
 
@@ -395,9 +520,23 @@ extern "C" int mpic_get_J(mpic_run_t *r, double *J)
     const Grid &g = r->g;
     std::vector<double> tmp((size_t)g.ncell);
     for (int c = 0; c < 3; ++c) {
-        MPIC_CHECK(hipMemcpy(tmp.data(), r->f.J[c], sizeof(double) * g.ncell, hipMemcpyDeviceToHost));
+        MPIC_CHECK(hipMemcpy(tmp.data(), r->jp.J[c], sizeof(double) * g.ncell, hipMemcpyDeviceToHost));
         for (long q = 0; q < g.ncell; ++q) J[3 * q + c] = tmp[q];
     }
+    return MPIC_OK;
+}
+
+/* Markers in the current device order (the sort drops deleted ones). */
+static int live_n(mpic_run *r, long *n)
+{
+    if (!r->tiled || !r->inited) {
+        *n = r->n;
+        return MPIC_OK;
+    }
+    int v = 0;
+    MPIC_CHECK(hipStreamSynchronize(r->stream));
+    MPIC_CHECK(hipMemcpy(&v, r->tv.nmark + r->parity, sizeof(int), hipMemcpyDeviceToHost));
+    *n = v;
     return MPIC_OK;
 }
 
@@ -406,6 +545,136 @@ struct RecView {              /* marker_t, source/type_marker.h:12-25 */
     float  rho, qDivM;
     int32_t id;
 };
+
+/* Host form of the walls for absolute FP64 positions (plasma_init applies
+ * them to the loaded markers, source/core/plasma.c:409-414). */
+static int host_walls(const mpic_run *r, double x[3], double u[3])
+{
+    for (int b = 0; b < 6; ++b) {
+        const int ax = b % 3, top = b / 3, bc = r->pc.bc[b];
+        const double bnd = top ? r->pc.shell2[ax] : r->pc.shell1[ax];
+        if (bc == MPIC_BC_PERIODIC) {
+            if (top ? (x[ax] >= bnd) : (x[ax] <= bnd)) x[ax] += top ? -r->pc.L[ax] : r->pc.L[ax];
+        } else if (bc == MPIC_BC_MIRROR) {
+            if (top ? (x[ax] >= bnd) : (x[ax] <= bnd)) {
+                x[ax] = 2 * bnd - x[ax];
+                u[ax] *= -1;
+            }
+        } else if (bc == MPIC_BC_OPEN) {
+            const double lim = top ? bnd - r->pc.h[ax] : bnd + r->pc.h[ax];
+            if (top ? (x[ax] >= lim) : (x[ax] <= lim)) return 0;
+        }
+    }
+    return 1;
+}
+
+static int host_key(const mpic_run *r, int i, int j, int k)
+{
+    const int *n = r->g.n;
+    i = i < 0 ? 0 : (i >= n[0] ? n[0] - 1 : i);
+    j = j < 0 ? 0 : (j >= n[1] ? n[1] - 1 : j);
+    k = k < 0 ? 0 : (k >= n[2] ? n[2] - 1 : k);
+    const int t = ((i >> 2) * r->tv.nt[1] + (j >> 2)) * r->tv.nt[2] + (k >> 2);
+    return t * 64 + (((i & 3) * 4 + (j & 3)) * 4 + (k & 3));
+}
+
+/* Split-precision load: walls in FP64, then cell index + FP32 offset. */
+static int load_split(mpic_run *r, std::vector<double> *d, std::vector<float> &w,
+                      std::vector<float> &qm, std::vector<int32_t> &id, long n)
+{
+    std::vector<int32_t> cell;
+    std::vector<float> f[7];
+    cell.reserve(n);
+    for (int c = 0; c < 7; ++c) f[c].reserve(n);
+    std::vector<int32_t> ids;
+    ids.reserve(n);
+    for (long q = 0; q < n; ++q) {
+        double x[3] = {d[0][q], d[1][q], d[2][q]}, u[3] = {d[3][q], d[4][q], d[5][q]};
+        if (!host_walls(r, x, u)) continue;
+        int sp = -1;
+        for (int s2 = 0; s2 < r->nspecies; ++s2)
+            if (r->qm[s2] == qm[q]) sp = s2;
+        if (sp < 0) {
+            if (r->nspecies >= 4) {
+                mpic_set_error("split form holds at most 4 species (q/m values)");
+                return MPIC_ERR_ARG;
+            }
+            sp = r->nspecies++;
+            r->qm[sp] = qm[q];
+        }
+        int ic[3];
+        float of[3];
+        for (int a = 0; a < 3; ++a) {
+            const double sgm = x[a] / r->g.h[a];
+            ic[a] = (int)(sgm + 128.0) - 128;
+            double o = sgm - ic[a];
+            float fo = (float)o;
+            if (fo >= 1.0f) fo = 0.99999994f;
+            if (fo < 0.0f) fo = 0.0f;
+            of[a] = fo;
+            if (ic[a] < 0 || ic[a] >= r->g.n[a]) {
+                mpic_set_error("marker %ld lies outside the box", q);
+                return MPIC_ERR_ARG;
+            }
+        }
+        cell.push_back((sp << 29) | host_key(r, ic[0], ic[1], ic[2]));
+        f[0].push_back(of[0]); f[1].push_back(of[1]); f[2].push_back(of[2]);
+        f[3].push_back((float)u[0]); f[4].push_back((float)u[1]); f[5].push_back((float)u[2]);
+        f[6].push_back(w[q]);
+        ids.push_back(id[q]);
+    }
+    const long m = (long)cell.size();
+    Markers32 &M = r->m32;
+    MPIC_CHECK(hipMemcpy(M.cell, cell.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice));
+    float *dst[7] = {M.ox, M.oy, M.oz, M.ux, M.uy, M.uz, M.w};
+    for (int c = 0; c < 7; ++c)
+        MPIC_CHECK(hipMemcpy(dst[c], f[c].data(), sizeof(float) * m, hipMemcpyHostToDevice));
+    MPIC_CHECK(hipMemcpy(M.id, ids.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice));
+    r->n = m;
+    return MPIC_OK;
+}
+
+static int save_split(mpic_run *r, void *rec, long *n, int recsize, long N)
+{
+    Markers32 &M = r->m32;
+    std::vector<int32_t> cell(N), id(N);
+    std::vector<float> f[7];
+    float *src[7] = {M.ox, M.oy, M.oz, M.ux, M.uy, M.uz, M.w};
+    MPIC_CHECK(hipMemcpy(cell.data(), M.cell, sizeof(int32_t) * N, hipMemcpyDeviceToHost));
+    MPIC_CHECK(hipMemcpy(id.data(), M.id, sizeof(int32_t) * N, hipMemcpyDeviceToHost));
+    for (int c = 0; c < 7; ++c) {
+        f[c].resize(N);
+        MPIC_CHECK(hipMemcpy(f[c].data(), src[c], sizeof(float) * N, hipMemcpyDeviceToHost));
+    }
+    char *dst = (char *)rec;
+    long o = 0;
+    for (long q = 0; q < N; ++q) {
+        if (cell[q] < 0) continue;
+        if (o >= *n) {
+            mpic_set_error("mpic_markers_save: output holds %ld markers, more are live", *n);
+            return MPIC_ERR_CAPACITY;
+        }
+        const int key = cell[q] & ((1 << 29) - 1), sp = (int)((unsigned)cell[q] >> 29);
+        const int t = key >> 6, loc = key & 63;
+        const int t2 = t % r->tv.nt[2], t1 = (t / r->tv.nt[2]) % r->tv.nt[1],
+                  t0 = t / (r->tv.nt[2] * r->tv.nt[1]);
+        const int ic[3] = {4 * t0 + (loc >> 4), 4 * t1 + ((loc >> 2) & 3), 4 * t2 + (loc & 3)};
+        RecView v;
+        memset(&v, 0, sizeof(v));
+        v.x = (ic[0] + (double)f[0][q]) * r->g.h[0];
+        v.y = (ic[1] + (double)f[1][q]) * r->g.h[1];
+        v.z = (ic[2] + (double)f[2][q]) * r->g.h[2];
+        v.vx = f[3][q]; v.vy = f[4][q]; v.vz = f[5][q];
+        v.rho = f[6][q];
+        v.qDivM = r->qm[sp];
+        v.id = id[q];
+        memset(dst + o * recsize, 0, recsize);
+        memcpy(dst + o * recsize, &v, recsize == 64 ? 60 : 56);
+        ++o;
+    }
+    *n = o;
+    return MPIC_OK;
+}
 
 extern "C" int mpic_markers_load(mpic_run_t *r, const void *rec, long n, int recsize)
 {
@@ -434,6 +703,7 @@ extern "C" int mpic_markers_load(mpic_run_t *r, const void *rec, long n, int rec
         qm[q] = v.qDivM;
         id[q] = (recsize == 64) ? v.id : (int32_t)(q + 1);
     }
+    if (r->fp32) return load_split(r, d, w, qm, id, n);
     double *dst[6] = {r->m.x, r->m.y, r->m.z, r->m.ux, r->m.uy, r->m.uz};
     for (int c = 0; c < 6; ++c)
         MPIC_CHECK(hipMemcpy(dst[c], d[c].data(), sizeof(double) * n, hipMemcpyHostToDevice));
@@ -449,7 +719,9 @@ extern "C" int mpic_markers_save(mpic_run_t *r, void *rec, long *n, int recsize)
     if (!r || !n || (recsize != 56 && recsize != 64)) return MPIC_ERR_ARG;
     MPIC_CHECK(hipSetDevice(r->dev));
     MPIC_CHECK(hipStreamSynchronize(r->stream));
-    const long N = r->n;
+    long N = 0;
+    TRY(live_n(r, &N));
+    if (r->fp32) return save_split(r, rec, n, recsize, N);
     std::vector<double> d[6];
     std::vector<float> w(N), qm(N);
     std::vector<int32_t> id(N);
@@ -488,12 +760,16 @@ extern "C" long mpic_marker_count(mpic_run_t *r)
 {
     if (!r) return -1;
     if (hipSetDevice(r->dev) != hipSuccess) return -1;
-    std::vector<float> qm(r->n);
+    long n = 0;
+    if (live_n(r, &n) != MPIC_OK) return -1;
+    if (r->tiled && r->inited) return n;          /* the sort keeps live markers only */
+    if (r->fp32) return n;
+    std::vector<float> qm(n);
     if (hipStreamSynchronize(r->stream) != hipSuccess) return -1;
-    if (hipMemcpy(qm.data(), r->m.qm, sizeof(float) * r->n, hipMemcpyDeviceToHost) != hipSuccess)
+    if (n && hipMemcpy(qm.data(), r->m.qm, sizeof(float) * n, hipMemcpyDeviceToHost) != hipSuccess)
         return -1;
     long c = 0;
-    for (long q = 0; q < r->n; ++q) c += (qm[q] != 0.0f);
+    for (long q = 0; q < n; ++q) c += (qm[q] != 0.0f);
     return c;
 }
 
@@ -532,6 +808,31 @@ static int walls_on(mpic_run *r, std::vector<FaceOp> &ops)
     return MPIC_OK;
 }
 
+template <class T>
+static int init_ghosts(mpic_run *r, T *const E[3], T *const H[3]);
+
+/* Counting sort of the current markers by cell key: with keys == 1 the
+ * keys are computed here (loaded or untouched markers), else they come
+ * from the push that just ran. */
+static int sort_pass(mpic_run *r, int keys)
+{
+    const int other = r->cur ^ 1;
+    if (r->fp32) {
+        if (keys) TRY(launch_keys32(r->stream, r->m32, r->n, r->tv));
+        TRY(launch_tile_scan(r->stream, r->tv, r->blocksum, r->parity ^ 1));
+        TRY(launch_scatter32(r->stream, r->m32, r->m32buf[other], r->n, r->tv, r->parity));
+        r->m32 = r->m32buf[other];
+    } else {
+        if (keys) TRY(launch_keys64(r->stream, r->m, r->n, r->g, r->tv));
+        TRY(launch_tile_scan(r->stream, r->tv, r->blocksum, r->parity ^ 1));
+        TRY(launch_scatter64(r->stream, r->m, r->mbuf[other], r->n, r->tv, r->parity));
+        r->m = r->mbuf[other];
+    }
+    r->cur = other;
+    r->parity ^= 1;
+    return MPIC_OK;
+}
+
 extern "C" int mpic_init(mpic_run_t *r)
 {
     if (!r) return MPIC_ERR_ARG;
@@ -542,8 +843,35 @@ extern "C" int mpic_init(mpic_run_t *r)
     MPIC_CHECK(hipSetDevice(r->dev));
     const Grid &g = r->g;
     const int *bc = r->cfg.bc;
-    /* cap_resetGhosts on the main meshes (source/core/em_caps.c:266-308);
-     * the main meshes carry one ghost layer: transverse extent [-1, n+1]. */
+    if (r->fp32)
+        TRY(init_ghosts(r, r->f32.E, r->f32.H));
+    else
+        TRY(init_ghosts(r, r->f.E, r->f.H));
+    /* plasma_init: walls applied once to every loaded marker
+     * (source/core/plasma.c:409-414). */
+    if (!r->fp32) TRY(launch_push64(r->stream, r->m, r->n, r->f, g, r->pc, r->dacc, 1, r->derr));
+    if (r->tiled) {
+        /* first sort of the loaded markers by tile-major cell key */
+        const int nn = (int)r->n;
+        MPIC_CHECK(hipMemcpyAsync(r->tv.nmark + r->parity, &nn, sizeof(int), hipMemcpyHostToDevice,
+                                  r->stream));
+        TRY(sort_pass(r, 1));
+    }
+    MPIC_CHECK(hipStreamSynchronize(r->stream));
+    r->inited = 1;
+    r->stage = 0;
+    return MPIC_OK;
+}
+
+/* cap_resetGhosts on the main meshes (source/core/em_caps.c:266-308; the
+ * main meshes carry one ghost layer: transverse extent [-1, n+1]) and the
+ * Mur buffers from the main E (capBC_Mur_init, source/core/emCap_Mur.c:
+ * 92-132). */
+template <class T>
+static int init_ghosts(mpic_run *r, T *const E[3], T *const H[3])
+{
+    const Grid &g = r->g;
+    const int *bc = r->cfg.bc;
     for (int ax = 0; ax < 3; ++ax) {
         Box full;
         for (int a = 0; a < 3; ++a) {
@@ -551,35 +879,28 @@ extern "C" int mpic_init(mpic_run_t *r)
             full.hi[a] = g.n[a] + 1;
         }
         if (bc[ax] == MPIC_BC_PERIODIC) {
-            TRY(launch_face(r->stream, g, face(FACE_PERIODIC_H, ax, 0, full, r->f.H, g.n[ax])));
-            TRY(launch_face(r->stream, g, face(FACE_PERIODIC_E, ax, 0, full, r->f.E, g.n[ax])));
+            TRY(launch_face(r->stream, g, face(FACE_PERIODIC_H, ax, 0, full, H, g.n[ax])));
+            TRY(launch_face(r->stream, g, face(FACE_PERIODIC_E, ax, 0, full, E, g.n[ax])));
         } else if (bc[ax] == MPIC_BC_MIRROR) {
-            FaceOp h = face(FACE_WALL_H, ax, 0, full, r->f.H, g.n[ax]);
+            FaceOp h = face(FACE_WALL_H, ax, 0, full, H, g.n[ax]);
             h.zeroN = 1;
             TRY(launch_face(r->stream, g, h));
-            TRY(launch_face(r->stream, g, face(FACE_MIRROR_E, ax, 0, full, r->f.E, g.n[ax])));
+            TRY(launch_face(r->stream, g, face(FACE_MIRROR_E, ax, 0, full, E, g.n[ax])));
         }
         if (bc[3 + ax] == MPIC_BC_MIRROR) {
-            FaceOp h = face(FACE_WALL_H, ax, 1, full, r->f.H, g.n[ax]);
+            FaceOp h = face(FACE_WALL_H, ax, 1, full, H, g.n[ax]);
             h.zeroN = 1;
             TRY(launch_face(r->stream, g, h));
-            TRY(launch_face(r->stream, g, face(FACE_MIRROR_E, ax, 1, full, r->f.E, g.n[ax])));
+            TRY(launch_face(r->stream, g, face(FACE_MIRROR_E, ax, 1, full, E, g.n[ax])));
         }
     }
-    /* Mur buffers from the main E (capBC_Mur_init, source/core/emCap_Mur.c:92-132). */
     for (FaceOp &op : r->bcE) {
         if (op.kind != FACE_MUR_E) continue;
         FaceOp ini = op;
         ini.kind = FACE_MUR_INIT;
-        for (int c = 0; c < 3; ++c) ini.X[c] = r->f.E[c];
+        for (int c = 0; c < 3; ++c) ini.X[c] = (void *)E[c];
         TRY(launch_face(r->stream, g, ini));
     }
-    /* plasma_init: walls applied once to every loaded marker
-     * (source/core/plasma.c:409-414). */
-    TRY(launch_push64(r->stream, r->m, r->n, r->f, g, r->pc, r->dacc, 1, r->derr));
-    MPIC_CHECK(hipStreamSynchronize(r->stream));
-    r->inited = 1;
-    r->stage = 0;
     return MPIC_OK;
 }
 
@@ -596,7 +917,8 @@ static int check_stage(mpic_run *r, int want)
     return MPIC_OK;
 }
 
-static int laser_H(mpic_run *r, double *Fy, double *Fz, const Box &tu)
+template <class T>
+static int laser_H(mpic_run *r, T *Fy, T *Fz, const Box &tu)
 {
     const Grid &g = r->g;
     for (const Laser &l : r->lasers) {
@@ -615,7 +937,8 @@ static int laser_H(mpic_run *r, double *Fy, double *Fz, const Box &tu)
     return MPIC_OK;
 }
 
-static int laser_E(mpic_run *r, const Box &tu)
+template <class T>
+static int laser_E(mpic_run *r, T *Ey, T *Ez, const Box &tu)
 {
     const Grid &g = r->g;
     for (const Laser &l : r->lasers) {
@@ -628,7 +951,7 @@ static int laser_E(mpic_run *r, const Box &tu)
         const int jmin = (int)(fmax(tu.lo[1], JMIN) + 0.1), jmax = (int)(fmin(tu.hi[1], JMAX) + 0.1);
         const int kmin = (int)(fmax(tu.lo[2], KMIN) + 0.1), kmax = (int)(fmin(tu.hi[2], KMAX) + 0.1);
         const double phase = 2 * M_PI * t + l.phi0;
-        TRY(launch_laser(r->stream, r->f.E[1], r->f.E[2], g, l.dmap + 4 * l.nodes, I, JMIN, KMIN,
+        TRY(launch_laser(r->stream, Ey, Ez, g, l.dmap + 4 * l.nodes, I, JMIN, KMIN,
                          KMAX, jmin, jmax, kmin, kmax, ampl * g.tau / g.h[0], cos(phase),
                          sin(phase)));
     }
@@ -642,12 +965,12 @@ extern "C" int mpic_em_h_half(mpic_run_t *r)
     MPIC_CHECK(hipSetDevice(r->dev));
     /* cap_cacheEH: cap H, walls on cap H, cap E, walls on cap E.  Sources
      * never reach the cap layer (checked in mpic_add_laser). */
-    TRY(launch_cap_H(r->stream, r->f, r->g, r->updH));
+    FCALL(launch_cap_H, r->g, r->updH);
     TRY(walls_on(r, r->bcH));
-    TRY(launch_cap_E(r->stream, r->f, r->g, r->updE));
+    FCALL(launch_cap_E, r->g, r->updE);
     TRY(walls_on(r, r->bcE));
     /* em_HSmallStep + cap_flushHalfH */
-    TRY(launch_h_update(r->stream, r->f, r->g, r->flH, 0));
+    FCALL(launch_h_update, r->g, r->flH, 0);
     r->stage = 1;
     return MPIC_OK;
 }
@@ -657,7 +980,7 @@ extern "C" int mpic_em_energy(mpic_run_t *r, double *W_E, double *W_M)
     if (!r) return MPIC_ERR_ARG;
     MPIC_CHECK(hipSetDevice(r->dev));
     MPIC_CHECK(hipMemsetAsync(r->dacc, 0, 2 * sizeof(double), r->stream));
-    TRY(launch_energy(r->stream, r->f, r->g, r->dacc));
+    FCALL(launch_energy, r->g, r->dacc);
     MPIC_CHECK(hipMemcpyAsync(r->hacc, r->dacc, 2 * sizeof(double), hipMemcpyDeviceToHost, r->stream));
     MPIC_CHECK(hipStreamSynchronize(r->stream));
     const double *h = r->g.h;
@@ -671,11 +994,21 @@ extern "C" int mpic_push_deposit(mpic_run_t *r)
     if (!r) return MPIC_ERR_ARG;
     TRY(check_stage(r, 1));
     MPIC_CHECK(hipSetDevice(r->dev));
-    TRY(launch_zero(r->stream, r->f.J[0], 3 * r->g.ncell));
+    TRY(launch_zero(r->stream, r->jp.J[0], 3 * r->g.ncell));
     MPIC_CHECK(hipMemsetAsync(r->dacc + 2, 0, 3 * sizeof(double), r->stream));
     if (r->timed) MPIC_CHECK(hipEventRecord(r->ev[1], r->stream));
-    TRY(launch_push64(r->stream, r->m, r->n, r->f, r->g, r->pc, r->dacc + 2, 0, r->derr));
+    if (r->fp32) {
+        TRY(launch_push32(r->stream, r->m32, r->f32, r->g, r->pc, r->qm, r->tv, r->dacc + 2,
+                          r->push_blocks));
+    } else if (r->tiled) {
+        TRY(launch_push64_tiled(r->stream, r->m, r->f, r->g, r->pc, r->tv, r->dacc + 2, r->derr,
+                                r->push_blocks));
+    } else {
+        TRY(launch_push64(r->stream, r->m, r->n, r->f, r->g, r->pc, r->dacc + 2, 0, r->derr));
+    }
     if (r->timed) MPIC_CHECK(hipEventRecord(r->ev[2], r->stream));
+    /* per-step marker sort by cell (keys written by the push) */
+    if (r->tiled) TRY(sort_pass(r, 0));
     return MPIC_OK;
 }
 
@@ -697,9 +1030,12 @@ extern "C" int mpic_em_h_step(mpic_run_t *r)
     if (!r) return MPIC_ERR_ARG;
     TRY(check_stage(r, 1));
     MPIC_CHECK(hipSetDevice(r->dev));
-    TRY(launch_h_update(r->stream, r->f, r->g, r->flH, 1));
+    FCALL(launch_h_update, r->g, r->flH, 1);
     Box inner = {{3, 3, 3}, {r->g.n[0] - 2, r->g.n[1] - 2, r->g.n[2] - 2}};
-    TRY(laser_H(r, r->f.H[1], r->f.H[2], inner));
+    if (r->fp32)
+        TRY(laser_H(r, r->f32.H[1], r->f32.H[2], inner));
+    else
+        TRY(laser_H(r, r->f.H[1], r->f.H[2], inner));
     r->stage = 2;
     return MPIC_OK;
 }
@@ -711,7 +1047,7 @@ extern "C" int mpic_em_e_start(mpic_run_t *r)
     MPIC_CHECK(hipSetDevice(r->dev));
     /* interior E step; the cap flush (which em_EStep_finish does) touches
      * disjoint nodes and reads only Ec, so it is done in the same pass */
-    TRY(launch_e_update(r->stream, r->f, r->g, r->flE, 1, 1));
+    FCALL(launch_e_update, r->g, r->flE, 1, 1);
     r->stage = 3;
     return MPIC_OK;
 }
@@ -722,7 +1058,7 @@ extern "C" int mpic_j_finish(mpic_run_t *r)
     MPIC_CHECK(hipSetDevice(r->dev));
     const int *bc = r->cfg.bc;
     const int BCv[6] = {bc[0], bc[3], bc[1], bc[4], bc[2], bc[5]};
-    for (int b = 0; b < 6; ++b) TRY(launch_jfold(r->stream, r->f, r->g, b, BCv[b]));
+    for (int b = 0; b < 6; ++b) TRY(launch_jfold(r->stream, r->jp, r->g, b, BCv[b]));
     return MPIC_OK;
 }
 
@@ -732,8 +1068,11 @@ extern "C" int mpic_em_e_finish(mpic_run_t *r)
     TRY(check_stage(r, 3));
     MPIC_CHECK(hipSetDevice(r->dev));
     Box dom = {{-1, -1, -1}, {r->g.n[0] + 1, r->g.n[1] + 1, r->g.n[2] + 1}};
-    TRY(laser_E(r, dom));
-    TRY(launch_e_tauJ(r->stream, r->f, r->g));
+    if (r->fp32)
+        TRY(laser_E(r, r->f32.E[1], r->f32.E[2], dom));
+    else
+        TRY(laser_E(r, r->f.E[1], r->f.E[2], dom));
+    FCALL(launch_e_tauJ, r->g);
     r->time = r->time + r->g.tau;          /* parameter_setTime (Time + tau) */
     r->stage = 0;
     return MPIC_OK;
@@ -827,8 +1166,13 @@ extern "C" int mpic_last_timing(mpic_run_t *r, double *push_ms, double *field_ms
 /* ---------------- rho, Gauss residuals, cells ---------------- */
 static int rho_into(mpic_run *r, double *rho)
 {
+    long n = 0;
+    TRY(live_n(r, &n));
     TRY(launch_zero(r->stream, rho, r->g.ncell));
-    TRY(launch_rho64(r->stream, r->m, r->n, rho, r->g));
+    if (r->fp32)
+        TRY(launch_rho32(r->stream, r->m32, n, rho, r->g, r->tv));
+    else
+        TRY(launch_rho64(r->stream, r->m, n, rho, r->g));
     const int *bc = r->cfg.bc;
     const int BCv[6] = {bc[0], bc[3], bc[1], bc[4], bc[2], bc[5]};
     for (int b = 0; b < 6; ++b) TRY(launch_rho_fold(r->stream, rho, r->g, b, BCv[b]));
@@ -859,7 +1203,7 @@ extern "C" int mpic_gauss_after(mpic_run_t *r, double *res)
     MPIC_CHECK(hipSetDevice(r->dev));
     TRY(rho_into(r, r->rho2));
     MPIC_CHECK(hipMemsetAsync(r->dacc + 5, 0, 2 * sizeof(double), r->stream));
-    TRY(launch_gauss(r->stream, r->f, r->rho1, r->rho2, r->g, r->dacc + 5));
+    FCALL(launch_gauss, r->rho1, r->rho2, r->g, r->dacc + 5);
     MPIC_CHECK(hipMemcpyAsync(r->hacc + 5, r->dacc + 5, 2 * sizeof(double), hipMemcpyDeviceToHost,
                               r->stream));
     MPIC_CHECK(hipStreamSynchronize(r->stream));
@@ -872,16 +1216,21 @@ extern "C" int mpic_marker_cells(mpic_run_t *r, int32_t *cells, long *n)
 {
     if (!r || !cells || !n) return MPIC_ERR_ARG;
     MPIC_CHECK(hipSetDevice(r->dev));
+    long N = 0;
+    TRY(live_n(r, &N));
     int32_t *d = nullptr;
-    MPIC_CHECK(hipMallocAsync((void **)&d, sizeof(int32_t) * (r->n ? r->n : 1), r->stream));
-    TRY(launch_cells64(r->stream, r->m, r->n, r->g, d));
-    std::vector<int32_t> all(r->n);
-    MPIC_CHECK(hipMemcpyAsync(all.data(), d, sizeof(int32_t) * r->n, hipMemcpyDeviceToHost,
+    MPIC_CHECK(hipMallocAsync((void **)&d, sizeof(int32_t) * (N ? N : 1), r->stream));
+    if (r->fp32)
+        TRY(launch_cells32(r->stream, r->m32, N, r->g, r->tv, d));
+    else
+        TRY(launch_cells64(r->stream, r->m, N, r->g, d));
+    std::vector<int32_t> all(N);
+    MPIC_CHECK(hipMemcpyAsync(all.data(), d, sizeof(int32_t) * N, hipMemcpyDeviceToHost,
                               r->stream));
     MPIC_CHECK(hipStreamSynchronize(r->stream));
     MPIC_CHECK(hipFreeAsync(d, r->stream));
     long o = 0;
-    for (long q = 0; q < r->n; ++q) {
+    for (long q = 0; q < N; ++q) {
         if (all[q] < 0) continue;
         if (o >= *n) return MPIC_ERR_CAPACITY;
         cells[o++] = all[q];
@@ -892,8 +1241,37 @@ extern "C" int mpic_marker_cells(mpic_run_t *r, int32_t *cells, long *n)
 
 extern "C" int mpic_sort(mpic_run_t *r, int32_t *counts)
 {
-    (void)counts;
     if (!r) return MPIC_ERR_ARG;
-    mpic_set_error("mpic_sort: not available in this build");
-    return MPIC_ERR_STATE;
+    if (!r->inited) {
+        mpic_set_error("mpic_sort: run is not initialised");
+        return MPIC_ERR_STATE;
+    }
+    MPIC_CHECK(hipSetDevice(r->dev));
+    if (!r->tiled) {
+        long n = r->n;
+        const int nn = (int)n;
+        MPIC_CHECK(hipMemcpyAsync(r->tv.nmark + r->parity, &nn, sizeof(int), hipMemcpyHostToDevice,
+                                  r->stream));
+    }
+    TRY(sort_pass(r, 1));
+    if (!r->tiled) {
+        /* the untiled push keeps working on the compacted array */
+        long n = 0;
+        int v = 0;
+        MPIC_CHECK(hipStreamSynchronize(r->stream));
+        MPIC_CHECK(hipMemcpy(&v, r->tv.nmark + r->parity, sizeof(int), hipMemcpyDeviceToHost));
+        n = v;
+        r->n = n;
+    }
+    if (counts) {
+        const long nc = (long)r->g.n[0] * r->g.n[1] * r->g.n[2];
+        int32_t *d = nullptr;
+        MPIC_CHECK(hipMallocAsync((void **)&d, sizeof(int32_t) * nc, r->stream));
+        TRY(launch_counts_natural(r->stream, r->tv, r->g, d));
+        MPIC_CHECK(hipMemcpyAsync(counts, d, sizeof(int32_t) * nc, hipMemcpyDeviceToHost,
+                                  r->stream));
+        MPIC_CHECK(hipStreamSynchronize(r->stream));
+        MPIC_CHECK(hipFreeAsync(d, r->stream));
+    }
+    return MPIC_OK;
 }

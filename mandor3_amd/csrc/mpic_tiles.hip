@@ -1,0 +1,214 @@
+/*
+ * mpic_tiles.hip -- per-step counting sort of the markers by tile-major
+ * cell key.
+ *
+ * Per step: the push kernel writes each marker's new key and bumps
+ * count[key] once per run of equal keys in a wavefront; the scan turns the
+ * counts into per-key offsets, builds the list of non-empty tiles and resets
+ * the tile queues; the scatter moves every live marker to its slot
+ * (cursor[key] reserved once per run), dropping deleted markers.
+ *
+ * Roofline (HBM-bound): scan ~ 3 x 4 B per cell key; scatter reads key (4 B)
+ * and the marker and writes the marker: FP64 plain form 4 + 60 + 60 B per
+ * marker.
+ */
+#include "mpic_tiles.h"
+
+#define SB 1024                 /* tiles per scan block */
+
+/* Keys of an unsorted array (after loading), with per-run counting. */
+__global__ void k_keys64(Markers64 m, long n, Grid g, TileView tv)
+{
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    int key = -1;
+    if (p < n && m.qm[p] != 0.0f) {
+        const double r[3] = {m.x[p], m.y[p], m.z[p]};
+        key = cell_key_of(tv, g, r);
+    }
+    if (p < n) tv.key[p] = key;
+    wave_run_count(tv.count, key);
+}
+
+int launch_keys64(hipStream_t s, const Markers64 &m, long n, const Grid &g, const TileView &tv)
+{
+    if (n <= 0) return MPIC_OK;
+    k_keys64<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(m, n, g, tv);
+    MPIC_CHECK(hipGetLastError());
+    return MPIC_OK;
+}
+
+/* Block-local exclusive scan of (markers per tile, tile non-empty). */
+__global__ void k_scan_a(TileView tv, int *blocksum, int *tileloc)
+{
+    __shared__ int sc[SB], sa[SB];
+    const int t = blockIdx.x * SB + threadIdx.x;
+    int c = 0;
+    if (t < tv.ntiles) {
+        const int4 *q = (const int4 *)(tv.count + (long)t * TILE_CELLS);
+        for (int v = 0; v < TILE_CELLS / 4; ++v) {
+            const int4 x = q[v];
+            c += x.x + x.y + x.z + x.w;
+        }
+    }
+    sc[threadIdx.x] = c;
+    sa[threadIdx.x] = (c > 0);
+    __syncthreads();
+    for (int d = 1; d < SB; d <<= 1) {          /* Hillis-Steele inclusive */
+        int a = 0, b = 0;
+        if ((int)threadIdx.x >= d) {
+            a = sc[threadIdx.x - d];
+            b = sa[threadIdx.x - d];
+        }
+        __syncthreads();
+        sc[threadIdx.x] += a;
+        sa[threadIdx.x] += b;
+        __syncthreads();
+    }
+    if (t < tv.ntiles) {
+        tileloc[2 * t] = sc[threadIdx.x] - c;            /* block-local exclusive */
+        tileloc[2 * t + 1] = sa[threadIdx.x] - (c > 0);
+    }
+    if (threadIdx.x == SB - 1) {
+        blocksum[2 * blockIdx.x] = sc[SB - 1];
+        blocksum[2 * blockIdx.x + 1] = sa[SB - 1];
+    }
+}
+
+/* Exclusive scan of the block sums (one block, loops for any count). */
+__global__ void k_scan_b(int *blocksum, int nb)
+{
+    __shared__ int sc[SB], sa[SB];
+    __shared__ int carry[2];
+    if (threadIdx.x == 0) carry[0] = carry[1] = 0;
+    __syncthreads();
+    for (int b0 = 0; b0 < nb; b0 += SB) {
+        const int b = b0 + threadIdx.x;
+        const int c = (b < nb) ? blocksum[2 * b] : 0, a = (b < nb) ? blocksum[2 * b + 1] : 0;
+        sc[threadIdx.x] = c;
+        sa[threadIdx.x] = a;
+        __syncthreads();
+        for (int d = 1; d < SB; d <<= 1) {
+            int x = 0, y = 0;
+            if ((int)threadIdx.x >= d) {
+                x = sc[threadIdx.x - d];
+                y = sa[threadIdx.x - d];
+            }
+            __syncthreads();
+            sc[threadIdx.x] += x;
+            sa[threadIdx.x] += y;
+            __syncthreads();
+        }
+        if (b < nb) {
+            blocksum[2 * b] = carry[0] + sc[threadIdx.x] - c;
+            blocksum[2 * b + 1] = carry[1] + sa[threadIdx.x] - a;
+        }
+        __syncthreads();
+        if (threadIdx.x == SB - 1) {
+            carry[0] += sc[SB - 1];
+            carry[1] += sa[SB - 1];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        blocksum[2 * nb] = carry[0];
+        blocksum[2 * nb + 1] = carry[1];
+    }
+}
+
+/* Per-key offsets and cursors, the active-tile list, counters reset: one
+ * wavefront per tile, one lane per cell key (coalesced). */
+__global__ void k_scan_c(TileView tv, const int *blocksum, const int *tileloc, int nb,
+                         int parity_next)
+{
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = (int)(q / TILE_CELLS), lane = threadIdx.x & 63;
+    if (t >= tv.ntiles) return;                      /* whole waves exit together */
+    const int c = tv.count[q];
+    int incl = c;
+    for (int d = 1; d < 64; d <<= 1) {
+        const int v = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += v;
+    }
+    const int tb = t / SB;
+    const int o = blocksum[2 * tb] + tileloc[2 * t] + incl - c;
+    tv.off[q] = o;
+    tv.cursor[q] = o;
+    tv.count[q] = 0;
+    const int tot = __shfl(incl, 63, 64);
+    if (lane == 0 && tot > 0) tv.active[blocksum[2 * tb + 1] + tileloc[2 * t + 1]] = t;
+    if (q == 0) {
+        tv.off[tv.ncellp] = blocksum[2 * nb];
+        *tv.nactive = blocksum[2 * nb + 1];
+        tv.nmark[parity_next] = blocksum[2 * nb];
+        for (int x = 0; x < MPIC_XCDS; ++x) tv.queue[x] = 0;
+    }
+}
+
+int launch_tile_scan(hipStream_t s, const TileView &tv, int *blocksum, int parity_next)
+{
+    const int nb = (tv.ntiles + SB - 1) / SB;
+    int *tileloc = blocksum + 2 * (nb + 1);
+    k_scan_a<<<nb, SB, 0, s>>>(tv, blocksum, tileloc);
+    MPIC_CHECK(hipGetLastError());
+    k_scan_b<<<1, SB, 0, s>>>(blocksum, nb);
+    MPIC_CHECK(hipGetLastError());
+    k_scan_c<<<(unsigned)((tv.ncellp + 255) / 256), 256, 0, s>>>(tv, blocksum, tileloc, nb,
+                                                                   parity_next);
+    MPIC_CHECK(hipGetLastError());
+    return MPIC_OK;
+}
+
+/* Moves each live marker of src to its slot in dst. */
+__global__ void k_scatter64(Markers64 src, Markers64 dst, long nbound, TileView tv, int parity)
+{
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long n = tv.nmark[parity];
+    const int key = (p < n && p < nbound) ? tv.key[p] : -1;
+    int start, len;
+    bool head;
+    wave_run(key, start, len, head);
+    int base = 0;
+    if (head && key >= 0) base = atomicAdd(tv.cursor + key, len);
+    base = __shfl(base, start, 64);
+    if (key < 0) return;
+    const long d = base + ((int)(threadIdx.x & 63) - start);
+    dst.x[d] = src.x[p];
+    dst.y[d] = src.y[p];
+    dst.z[d] = src.z[p];
+    dst.ux[d] = src.ux[p];
+    dst.uy[d] = src.uy[p];
+    dst.uz[d] = src.uz[p];
+    dst.w[d] = src.w[p];
+    dst.qm[d] = src.qm[p];
+    dst.id[d] = src.id[p];
+}
+
+int launch_scatter64(hipStream_t s, const Markers64 &src, const Markers64 &dst, long nbound,
+                     const TileView &tv, int parity)
+{
+    if (nbound <= 0) return MPIC_OK;
+    k_scatter64<<<(unsigned)((nbound + 255) / 256), 256, 0, s>>>(src, dst, nbound, tv, parity);
+    MPIC_CHECK(hipGetLastError());
+    return MPIC_OK;
+}
+
+/* Per-cell marker counts in natural (i, j, k) order from the offsets. */
+__global__ void k_counts_natural(TileView tv, Grid g, int32_t *out)
+{
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long nc = (long)g.n[0] * g.n[1] * g.n[2];
+    if (c >= nc) return;
+    const int k = (int)(c % g.n[2]);
+    const long r = c / g.n[2];
+    const int j = (int)(r % g.n[1]), i = (int)(r / g.n[1]);
+    const int key = cell_key_ijk(tv, g, i, j, k);
+    out[c] = tv.off[key + 1] - tv.off[key];
+}
+
+int launch_counts_natural(hipStream_t s, const TileView &tv, const Grid &g, int32_t *out)
+{
+    const long nc = (long)g.n[0] * g.n[1] * g.n[2];
+    k_counts_natural<<<(unsigned)((nc + 255) / 256), 256, 0, s>>>(tv, g, out);
+    MPIC_CHECK(hipGetLastError());
+    return MPIC_OK;
+}

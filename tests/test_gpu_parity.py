@@ -139,13 +139,18 @@ def test_gauss_residuals(case):
         g.stage("em_e_finish")
         rg = g.gauss_after()
         _, ro = o.step(gauss=True)
-        # charge conservation and Gauss's law hold to round-off on both
-        # paths ("of order 1e-14" relative, tests/05_slab/README:10-12 of the
-        # reference); the residuals themselves are round-off noise, so they
-        # are bounded, not compared digit by digit
+        # Both residuals are round-off in a closed box ("of order 1e-14"
+        # relative, tests/05_slab/README:10-12 of the reference): there they
+        # are bounded, not compared digit by digit.  Open walls delete
+        # markers without a current and the mirror image currents are not
+        # charge-conserving at the wall, so there the residuals are physical
+        # and both paths must report the same values.
         scale = np.abs(fx["M0"]["rho"]).max() / o.tau
-        assert rg[0] < 1e-12 * scale and ro[0] < 1e-12 * scale, (rg, ro)
-        assert rg[1] < 1e-12 * scale and ro[1] < 1e-12 * scale, (rg, ro)
+        for q in range(2):
+            if ro[q] < 1e-12 * scale:
+                assert rg[q] < 1e-12 * scale, (q, rg, ro)
+            else:
+                assert abs(rg[q] - ro[q]) <= 1e-9 * ro[q], (q, rg, ro)
 
 
 def test_stage_order_enforced():
