@@ -118,8 +118,7 @@ int  mpic_em_e_finish(mpic_run_t *run);                         /* em_EStep_fini
  * WTy, WTz} as handed to wDensity_addPoint (source/core/diag_WDensity.c:136).
  * With diag == NULL nothing is copied back and the step stays asynchronous. */
 int  mpic_step(mpic_run_t *run, double *diag);
-/* n steps without host synchronisation (captured into a HIP graph once the
- * launch topology is stable). */
+/* n steps enqueued on the run's stream without host synchronisation. */
 int  mpic_advance(mpic_run_t *run, int nsteps);
 int  mpic_sync(mpic_run_t *run);
 double mpic_time(mpic_run_t *run);

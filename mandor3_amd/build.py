@@ -24,7 +24,9 @@ COMMON = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",
           "-I" + os.path.join(ROOT, "include"), "-I" + CSRC,
           "-Wno-unused-result"]
 # per-file contraction mode: the FP64 plain path must not fuse mul+add
-FP_MODE = {"default": ["-ffp-contract=off"]}
+# (bit-exact with the reference); the FP32 split push may
+FP_MODE = {"default": ["-ffp-contract=off"],
+           "mpic_push32.hip": ["-ffp-contract=fast"]}
 
 
 def sources():

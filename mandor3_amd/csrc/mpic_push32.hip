@@ -18,10 +18,12 @@
  *
  * Deposit: markers are sorted by cell, so the lanes of a wavefront mostly
  * share one start cell.  The segment inside the start cell (every marker
- * has one) is summed across each run of equal cells with a segmented
- * wavefront reduction, then added to the tile's FP32 J block in LDS once
- * per run; later segments (cell crossings) go to LDS with atomics.  The
- * block is flushed to the FP64 J in HBM once per tile.
+ * has one) is summed across each run of equal cells inside each 16-lane
+ * row with a segmented DPP scan (VALU only), then added to the tile's FP32 J
+ * block in LDS once per run and row; later segments (cell crossings) go to
+ * LDS with atomics.  The block is
+ * flushed to the FP64 J in HBM once per tile.  The tile's E/H nodes are
+ * staged in LDS, so gathers never leave the CU.
  *
  * Roofline: HBM-bound.  Algorithmic bytes per marker-step 32 B read + 28 B
  * written (id is not touched), plus per occupied cell 24 B of FP32 E/H reads
@@ -31,8 +33,8 @@
 #include "mpic_tiles.h"
 
 #define TPB 256
+#define TW 64                  /* the tile push: one wavefront per workgroup */
 #define TILE 4
-#define TJN (TILE + 3)         /* J nodes per axis of a tile's LDS block */
 #define TFN (TILE + 2)         /* E/H nodes per axis staged for a tile's gathers */
 #define TFN3 (TFN * TFN * TFN)
 #define SPECIES_SHIFT 29
@@ -122,7 +124,7 @@ struct Push32Consts {
     float tau, dtn[3];       /* tau, tau/h */
     float cj[3];             /* h/tau: J of a segment per unit w and displacement */
     double vol;              /* h1*h2*h3 (energy) */
-    float qm[8];             /* q/m per species */
+    float qm[4];             /* q/m per species (species is 2 bits of the cell word) */
     int bc[6];
 };
 
@@ -164,6 +166,17 @@ __device__ static inline int wall_hi(int bc, int &c, float &o, float &u, int N)
     return 1;
 }
 
+/* Point at fraction a of the move (ox,oy,oz) -> (x2,y2,z2), placed exactly
+ * on the crossed face of axis ax. */
+__device__ static inline void crossing_point(float ox, float oy, float oz, float x2, float y2,
+                                             float z2, int fx, int fy, int fz, float a, int ax,
+                                             float &qx, float &qy, float &qz)
+{
+    qx = (ax == 0) ? (fx > 0 ? 1.0f : 0.0f) : ox + a * (x2 - ox);
+    qy = (ax == 1) ? (fy > 0 ? 1.0f : 0.0f) : oy + a * (y2 - oy);
+    qz = (ax == 2) ? (fz > 0 ? 1.0f : 0.0f) : oz + a * (z2 - oz);
+}
+
 __device__ static inline void cswap(float &a, int &ia, float &b, int &ib)
 {
     if (b < a) {
@@ -172,24 +185,42 @@ __device__ static inline void cswap(float &a, int &ia, float &b, int &ib)
     }
 }
 
+/* Hardware reciprocal and reciprocal square root (1 ulp): the split form's
+ * error budget is set by FP32 storage, not by these. */
+__device__ static inline float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ static inline float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
+
 /* Marker state held in registers between the prefetch and the push. */
 struct MarkerRegs {
     int   word;
     float ox, oy, oz, ux, uy, uz, w;
 };
 
-__device__ static inline void load_marker(const Markers32 &m, long p, bool in, MarkerRegs &r)
+/* Element p of a 4-byte marker array as base + 32-bit byte offset, so the
+ * access uses a scalar base and a 32-bit vector offset (runs hold fewer
+ * than 2^30 markers). */
+template <class T>
+__device__ static inline T &at(T *base, unsigned p)
 {
-    r.word = -1;
-    if (!in) return;
-    r.word = m.cell[p];
-    r.ox = m.ox[p];
-    r.oy = m.oy[p];
-    r.oz = m.oz[p];
-    r.ux = m.ux[p];
-    r.uy = m.uy[p];
-    r.uz = m.uz[p];
-    r.w = m.w[p];
+    return *(T *)((char *)base + (p << 2));
+}
+
+/* Loads marker min(p, last): unconditional, so that every path through the
+ * push loop issues the same memory operations and the compiler's wait
+ * counts stay exact (a skippable load makes it wait for later loads too).
+ * Lanes past the tile's end carry a duplicate, masked by the caller. */
+__device__ static inline void load_marker(const Markers32 &m, unsigned p, unsigned last,
+                                          MarkerRegs &r)
+{
+    p = p < last ? p : last;
+    r.word = at(m.cell, p);
+    r.ox = at(m.ox, p);
+    r.oy = at(m.oy, p);
+    r.oz = at(m.oz, p);
+    r.ux = at(m.ux, p);
+    r.uy = at(m.uy, p);
+    r.uz = at(m.uz, p);
+    r.w = at(m.w, p);
 }
 
 /* Trilinear gather from a tile's staged field block (TFN^3 nodes, node
@@ -207,21 +238,26 @@ __device__ static inline float lgather(const float *F, int a, float sx, int b, f
 
 /* Push of one marker of tile-local cell (li,lj,lk) (cell (ci,cj,ck)):
  * gather from the staged fields, Boris, move, deposit of every segment but
- * the first into the tile's LDS J (or HBM beyond it), walls.  Writes the
- * marker back and returns its new cell word (-1: deleted) and the first
- * segment, which lies inside the start cell. */
-__device__ static inline int push32_marker(const Markers32 &m, long p, const MarkerRegs &r, int li,
+ * the first into the tile's LDS J (or HBM beyond it), walls.  Returns the
+ * new cell word (-1: deleted), the new offsets and momenta in 'out' and the
+ * first segment, which lies inside the start cell. */
+__device__ static inline int push32_marker(MarkerRegs &out, const MarkerRegs &r, int li,
                                            int lj_, int lk, int ci, int cj, int ck,
-                                           const float *lf, float *lj, int o0, int o1, int o2,
+                                           const float *lf, const float *lqm, float *lj, int o0,
+                                           int o1, int o2,
                                            const F32View &v, const Push32Consts &c,
-                                           const TileView &tv, Seg12 &first, double &w0,
-                                           double &w1, double &w2)
+                                           const TileView &tv, Seg12 &first, float &w0,
+                                           float &w1, float &w2)
 {
     const Grid &g = v.g;
     const int sp = (int)((unsigned)r.word >> SPECIES_SHIFT);
     const float ox = r.ox, oy = r.oy, oz = r.oz;
     float ux = r.ux, uy = r.uy, uz = r.uz;
-    const float w = r.w, qm = c.qm[sp];
+    /* q/m from the workgroup's LDS copy: an indexed read of the argument
+     * block would be a vector memory load, and its wait would also wait for
+     * the prefetched markers */
+    const float qm = lqm[sp];
+    const float w = r.w;
     /* gather: E_x at (i-1/2, j, k), E_y at (i, j-1/2, k), ... */
     const int hx = ox >= 0.5f, hy = oy >= 0.5f, hz = oz >= 0.5f;
     const float sxh = ox + 0.5f - (float)hx, syh = oy + 0.5f - (float)hy, szh = oz + 0.5f - (float)hz;
@@ -235,70 +271,66 @@ __device__ static inline int push32_marker(const Markers32 &m, long p, const Mar
     const float tm = 0.5f * c.tau * qm;
     const float px = ux + tm * Ex, py = uy + tm * Ey, pz = uz + tm * Ez;
     const float p2 = px * px + py * py + pz * pz;
-    const float gam = sqrtf(1.0f + p2);
+    const float ig0 = rsq(1.0f + p2);
+    const float gam = (1.0f + p2) * ig0;
     /* (gamma - 1)/p^2 = 1/(gamma + 1): no cancellation for slow markers */
-    const float W = w / ((gam + 1.0f) * qm);
-    w0 += (double)(W * px * px);
-    w1 += (double)(W * py * py);
-    w2 += (double)(W * pz * pz);
-    const float tg = tm / gam;
+    const float W = w * rcp((gam + 1.0f) * qm);
+    w0 += W * px * px;
+    w1 += W * py * py;
+    w2 += W * pz * pz;
+    const float tg = tm * ig0;
     const float ax = tg * Hx, ay = tg * Hy, az = tg * Hz;
     const float uu = px + py * az - pz * ay, vv = py + pz * ax - px * az, ww = pz + px * ay - py * ax;
     const float bxx = ax * ax, byy = ay * ay, bzz = az * az, bxy = ax * ay, byz = ay * az,
                 bxz = az * ax;
-    const float del = 1.0f / (1.0f + bxx + byy + bzz);
+    const float del = rcp(1.0f + bxx + byy + bzz);
     ux = tm * Ex + ((1 + bxx) * uu + (bxy + az) * vv + (bxz - ay) * ww) * del;
     uy = tm * Ey + ((bxy - az) * uu + (1 + byy) * vv + (byz + ax) * ww) * del;
     uz = tm * Ez + ((bxz + ay) * uu + (byz - ax) * vv + (1 + bzz) * ww) * del;
-    const float ig = 1.0f / sqrtf(1.0f + ux * ux + uy * uy + uz * uz);
+    const float ig = rsq(1.0f + ux * ux + uy * uy + uz * uz);
     /* move in cell units from (ox,oy,oz) of cell (ci,cj,ck) */
     const float x2 = ox + ux * c.dtn[0] * ig, y2 = oy + uy * c.dtn[1] * ig, z2 = oz + uz * c.dtn[2] * ig;
     const int fx = (int)floorf(x2), fy = (int)floorf(y2), fz = (int)floorf(z2);
     const float wx = w * c.cj[0], wy = w * c.cj[1], wz = w * c.cj[2];
-    if ((fx | fy | fz) == 0) {
-        seg_values(ox, oy, oz, x2, y2, z2, wx, wy, wz, first);
-    } else {
-        /* crossing fractions along the path, ascending (axes that are not
-         * crossed get 2 and sort last); a sorting network on scalars keeps
-         * everything in registers */
-        float a0 = fx ? ((fx > 0 ? 1.0f : 0.0f) - ox) / (x2 - ox) : 2.0f;
-        float a1 = fy ? ((fy > 0 ? 1.0f : 0.0f) - oy) / (y2 - oy) : 2.0f;
-        float a2 = fz ? ((fz > 0 ? 1.0f : 0.0f) - oz) / (z2 - oz) : 2.0f;
-        int x0 = 0, x1 = 1, x2i = 2;
+    /* Segments: from the start to the first face crossing (or the end),
+     * then one per crossed face.  Crossing fractions ascend; an axis that is
+     * not crossed gets 2 and sorts last.  Each axis is crossed at most once
+     * per step, so a crossing point lies on face 0 or 1 of the start cell's
+     * frame on that axis. */
+    const int nc = (fx != 0) + (fy != 0) + (fz != 0);
+    float a0 = 2.0f, a1 = 2.0f, a2 = 2.0f;
+    int x0 = 0, x1 = 1, x2i = 2;
+    float qx = x2, qy = y2, qz = z2;
+    if (nc) {
+        if (fx) a0 = ((fx > 0 ? 1.0f : 0.0f) - ox) * rcp(x2 - ox);
+        if (fy) a1 = ((fy > 0 ? 1.0f : 0.0f) - oy) * rcp(y2 - oy);
+        if (fz) a2 = ((fz > 0 ? 1.0f : 0.0f) - oz) * rcp(z2 - oz);
         cswap(a0, x0, a1, x1);
         cswap(a1, x1, a2, x2i);
         cswap(a0, x0, a1, x1);
-        const int nc = (fx != 0) + (fy != 0) + (fz != 0);
-        float px0 = ox, py0 = oy, pz0 = oz;
-        int sx = 0, sy = 0, sz = 0;
-#pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            if (s > nc) break;
-            float qx = x2, qy = y2, qz = z2;
-            int axs = -1;
-            if (s < nc) {
-                const float a = (s == 0) ? a0 : (s == 1 ? a1 : a2);
-                axs = (s == 0) ? x0 : (s == 1 ? x1 : x2i);
-                qx = ox + a * (x2 - ox);
-                qy = oy + a * (y2 - oy);
-                qz = oz + a * (z2 - oz);
-                /* the crossing point lies on the face exactly */
-                if (axs == 0) qx = (fx > 0) ? 1.0f : 0.0f;
-                if (axs == 1) qy = (fy > 0) ? 1.0f : 0.0f;
-                if (axs == 2) qz = (fz > 0) ? 1.0f : 0.0f;
-            }
+        crossing_point(ox, oy, oz, x2, y2, z2, fx, fy, fz, a0, x0, qx, qy, qz);
+    }
+    seg_values(ox, oy, oz, qx, qy, qz, wx, wy, wz, first);
+    if (nc) {
+        int sx = 0, sy = 0, sz = 0, axs = x0;
+#pragma unroll 1
+        for (int s = 1; s <= nc; ++s) {
+            sx += (axs == 0) ? fx : 0;
+            sy += (axs == 1) ? fy : 0;
+            sz += (axs == 2) ? fz : 0;
+            a0 = a1;
+            x0 = x1;
+            a1 = a2;
+            x1 = x2i;
+            float ex = x2, ey = y2, ez = z2;
+            if (s < nc) crossing_point(ox, oy, oz, x2, y2, z2, fx, fy, fz, a0, x0, ex, ey, ez);
             Seg12 sg;
-            seg_values(px0 - sx, py0 - sy, pz0 - sz, qx - sx, qy - sy, qz - sz, wx, wy, wz, sg);
-            if (s == 0)
-                first = sg;
-            else
-                seg_add(v, lj, o0, o1, o2, ci + sx, cj + sy, ck + sz, sg);
-            if (axs == 0) sx += fx;
-            if (axs == 1) sy += fy;
-            if (axs == 2) sz += fz;
-            px0 = qx;
-            py0 = qy;
-            pz0 = qz;
+            seg_values(qx - sx, qy - sy, qz - sz, ex - sx, ey - sy, ez - sz, wx, wy, wz, sg);
+            seg_add(v, lj, o0, o1, o2, ci + sx, cj + sy, ck + sz, sg);
+            qx = ex;
+            qy = ey;
+            qz = ez;
+            axs = x0;
         }
     }
     /* new cell and offset, then the walls */
@@ -311,28 +343,46 @@ __device__ static inline int push32_marker(const Markers32 &m, long p, const Mar
                       wall_lo(c.bc[2], nk, nz, uz, g.n[2]) && wall_hi(c.bc[3], ni, nx, ux, g.n[0]) &&
                       wall_hi(c.bc[4], nj, ny, uy, g.n[1]) && wall_hi(c.bc[5], nk, nz, uz, g.n[2]);
     if (!alive) return -1;
-    m.ox[p] = nx;
-    m.oy[p] = ny;
-    m.oz[p] = nz;
-    m.ux[p] = ux;
-    m.uy[p] = uy;
-    m.uz[p] = uz;
+    out.ox = nx;
+    out.oy = ny;
+    out.oz = nz;
+    out.ux = ux;
+    out.uy = uy;
+    out.uz = uz;
     return (sp << SPECIES_SHIFT) | cell_key_ijk(tv, g, ni, nj, nk);
 }
 
-/* Segmented sum of the 12 first-segment values over the caller's run of
- * equal start cells (lanes [start, end) of the wavefront): afterwards the
- * run's head lane holds the run total. */
-__device__ static inline void run_sum(Seg12 &f, int end)
+/* DPP move of a float across lanes: lanes whose source lies outside the
+ * pattern (or rows outside row_mask) read 0. */
+template <int CTRL, int ROWS>
+__device__ static inline float dpp0(float x)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, ROWS, 0xF, false));
+}
+
+#define DPP_ROW_SHR(d) (0x110 + (d))
+
+/* Segmented inclusive prefix sum of the 12 first-segment values over runs
+ * of equal start cells inside each 16-lane row; 'start' is the first lane of
+ * the caller's run.  Afterwards the last lane of each run in a row holds
+ * that row's part of the run total.  A step adds only when its source lane
+ * belongs to the caller's run (sources outside the row read 0). */
+__device__ static inline void row_scan(Seg12 &f, int start)
 {
     const int lane = threadIdx.x & 63;
+    const bool c1 = lane - 1 >= start, c2 = lane - 2 >= start, c4 = lane - 4 >= start,
+               c8 = lane - 8 >= start;
 #pragma unroll
     for (int q = 0; q < 12; ++q) {
-        float x = f.v[q];
-        for (int d = 1; d < 64; d <<= 1) {
-            const float y = __shfl_down(x, d, 64);
-            if (lane + d < end) x += y;
-        }
+        float x = f.v[q], y;
+        y = dpp0<DPP_ROW_SHR(1), 0xF>(x);
+        if (c1) x += y;
+        y = dpp0<DPP_ROW_SHR(2), 0xF>(x);
+        if (c2) x += y;
+        y = dpp0<DPP_ROW_SHR(4), 0xF>(x);
+        if (c4) x += y;
+        y = dpp0<DPP_ROW_SHR(8), 0xF>(x);
+        if (c8) x += y;
         f.v[q] = x;
     }
 }
@@ -347,106 +397,200 @@ __device__ static inline void key_to_ijk(const TileView &tv, int key, int &i, in
     k = TILE * t2 + (loc & 3);
 }
 
-/* Persistent tiled push, one 4x4x4-cell tile per workgroup iteration: the
- * tile's E/H nodes are staged in LDS, its J is accumulated in LDS and
- * flushed once, and the next batch of markers is loaded before the current
- * one is pushed. */
-__global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(4))) k_push32_tiled(Markers32 m, F32View v, Push32Consts c,
-                                                      TileView tv, double *Wacc)
+/* Results of one batch, written back at the start of the next batch (after
+ * its prefetch is issued) so that no wait for the prefetch also waits for
+ * fresh stores. */
+struct Pending {
+    MarkerRegs r;
+    unsigned   p;
+    int        word;      /* new cell word, -1 deleted */
+    bool       valid;     /* lane pushed a marker */
+    int        ckey, clen;/* count run head: key (-1 none), length */
+};
+
+__device__ static inline void write_back(const Markers32 &m, const TileView &tv, const Pending &q)
 {
-    __shared__ float lj[TJN * TJN * TJN * 3];
+    if (q.valid) {
+        at(m.cell, q.p) = q.word;
+        if (q.word >= 0) {
+            at(m.ox, q.p) = q.r.ox;
+            at(m.oy, q.p) = q.r.oy;
+            at(m.oz, q.p) = q.r.oz;
+            at(m.ux, q.p) = q.r.ux;
+            at(m.uy, q.p) = q.r.uy;
+            at(m.uz, q.p) = q.r.uz;
+        }
+    }
+    if (q.ckey >= 0) atomicAdd(tv.count + q.ckey, q.clen);
+}
+
+/* Tiled push: one wavefront per active tile (4x4x4 cells), taken slot by
+ * slot with stride = grid size; with at least as many wavefronts as active
+ * tiles each takes one and the hardware dispatcher balances uneven tiles,
+ * while the other wavefronts of the CU hide each tile's start-up latency.
+ * The tile's E/H nodes are staged in LDS, its J is accumulated in an FP32
+ * LDS block and stored, without atomics, as the tile's block in 'tj'; the
+ * next batch of markers is loaded before the current one is pushed. */
+__global__ void __launch_bounds__(TW) __attribute__((amdgpu_waves_per_eu(4)))
+k_push32_tiles(Markers32 m, F32View v, Push32Consts c, TileView tv, float *tj, double *Wacc)
+{
+    __shared__ float lj[TJB];
     __shared__ float lf[6 * TFN3];
-    __shared__ int s_tile;
+    __shared__ float lqm[4];
+    if (threadIdx.x < 4) lqm[threadIdx.x] = c.qm[threadIdx.x];   /* ordered by the loop's barrier */
     const Grid &g = v.g;
-    const int xcd = blockIdx.x % MPIC_XCDS;
     const int lane = threadIdx.x & 63;
-    double w0 = 0, w1 = 0, w2 = 0;
-    for (;;) {
-        if (threadIdx.x == 0) s_tile = tile_next(tv, xcd);
-        __syncthreads();
-        const int t = s_tile;
-        if (t < 0) break;
+    const int nact = *tv.nactive;
+    float w0 = 0, w1 = 0, w2 = 0;         /* per-thread kinetic energy partials */
+    for (int sl = blockIdx.x; sl < nact; sl += gridDim.x) {
+        const int t = tv.active[sl];
         int t0, t1, t2;
         tile_coords(tv, t, t0, t1, t2);
         const int c0 = TILE * t0, c1 = TILE * t1, c2 = TILE * t2;
         const int o0 = c0 - 1, o1 = c1 - 1, o2 = c2 - 1;
-        for (int q = threadIdx.x; q < TJN * TJN * TJN * 3; q += TPB) lj[q] = 0.0f;
+        __syncthreads();                  /* previous tile's LDS reads are done */
+        for (int q = threadIdx.x; q < TJB; q += TW) lj[q] = 0.0f;
         /* nodes [c, c + TFN) per axis; those past the last ghost layer (a
          * partial tile at the high edge) are never read by a gather */
-        for (int q = threadIdx.x; q < 6 * TFN3; q += TPB) {
+        for (int q = threadIdx.x; q < 6 * TFN3; q += TW) {
             const int comp = q / TFN3, rr = q - comp * TFN3;
             const int a = rr / (TFN * TFN), b = (rr / TFN) % TFN, cc = rr % TFN;
             const int i = c0 + a, j = c1 + b, k = c2 + cc;
             const float *F = comp < 3 ? v.E[comp] : v.H[comp - 3];
             lf[q] = (i <= g.n[0] + 2 && j <= g.n[1] + 2 && k <= g.n[2] + 2) ? F[gidx(g, i, j, k)] : 0.0f;
         }
-        const long beg = tv.off[(long)t * TILE_CELLS], end = tv.off[(long)(t + 1) * TILE_CELLS];
+        const unsigned beg = tv.off[t * TILE_CELLS], end = tv.off[(t + 1) * TILE_CELLS];
         MarkerRegs cur;
-        load_marker(m, beg + threadIdx.x, beg + threadIdx.x < end, cur);
+        load_marker(m, beg + threadIdx.x, end - 1, cur);
+        /* everything outstanding completes here, so that the loop's waits
+         * only ever concern the prefetch of the batch in flight */
+        __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
-        for (long base = beg; base < end; base += TPB) {
-            const long p = base + threadIdx.x;
+        Pending pend;
+        pend.valid = false;
+        pend.ckey = -1;
+        for (unsigned base = beg; base < end; base += TW) {
+            const unsigned p = base + threadIdx.x;
             MarkerRegs nxt;
-            load_marker(m, p + TPB, p + TPB < end, nxt);
+            load_marker(m, p + TW, end - 1, nxt);
+            write_back(m, tv, pend);
             int rkey = -1, newword = -1, ci = 0, cj = 0, ck = 0;
             Seg12 first;
             for (int q = 0; q < 12; ++q) first.v[q] = 0.0f;
-            if (cur.word >= 0) {
+            const bool in = p < end && cur.word >= 0;
+            if (in) {
                 rkey = cur.word & KEY_MASK;
                 const int loc = rkey & (TILE_CELLS - 1);
                 const int li = loc >> 4, lj_ = (loc >> 2) & 3, lk = loc & 3;
                 ci = c0 + li;
                 cj = c1 + lj_;
                 ck = c2 + lk;
-                newword = push32_marker(m, p, cur, li, lj_, lk, ci, cj, ck, lf, lj, o0, o1, o2, v, c,
-                                        tv, first, w0, w1, w2);
-                m.cell[p] = newword;
+                newword = push32_marker(pend.r, cur, li, lj_, lk, ci, cj, ck, lf, lqm, lj, o0, o1, o2,
+                                        v, c, tv, first, w0, w1, w2);
             }
+            pend.valid = in;
+            pend.p = p;
+            pend.word = newword;
             int start, len;
             bool head;
             wave_run(rkey, start, len, head);
-            run_sum(first, start + len);
-            if (head && rkey >= 0) seg_add(v, lj, o0, o1, o2, ci, cj, ck, first);
-            wave_run_count(tv.count, newword < 0 ? -1 : (newword & KEY_MASK));
+            row_scan(first, start);
+            /* last lane of the run inside its row: adds the row's part */
+            const int ln = threadIdx.x & 63;
+            const bool tail = ln == start + len - 1 || ((ln & 15) == 15 && ln < start + len - 1);
+            if (tail && rkey >= 0) seg_add(v, lj, o0, o1, o2, ci, cj, ck, first);
+            /* count[new key] += run length, once per run of equal new keys */
+            const int nkey = newword < 0 ? -1 : (newword & KEY_MASK);
+            wave_run(nkey, start, len, head);
+            pend.ckey = (head && nkey >= 0) ? nkey : -1;
+            pend.clen = len;
             cur = nxt;
         }
+        write_back(m, tv, pend);
         __syncthreads();
-        for (int q = threadIdx.x; q < TJN * TJN * TJN * 3; q += TPB) {
-            const float val = lj[q];
-            if (val == 0.0f) continue;
-            const int comp = q % 3, node = q / 3;
-            const int cc = node % TJN, bb = (node / TJN) % TJN, aa = node / (TJN * TJN);
-            const int i = o0 + aa, j = o1 + bb, k = o2 + cc;
-            if (i < -2 || j < -2 || k < -2 || i > g.n[0] + 2 || j > g.n[1] + 2 || k > g.n[2] + 2)
-                continue;
-            unsafeAtomicAdd(v.J[comp] + gidx(g, i, j, k), (double)val);
-        }
-        __syncthreads();
+        /* the tile's J block, stored whole (coalesced, no atomics) */
+        float *dst = tj + (long)sl * TJB;
+        for (int q = threadIdx.x; q < TJB; q += TW) dst[q] = lj[q];
     }
+    double d0 = w0, d1 = w1, d2 = w2;     /* wave sums in FP64 */
     for (int d = 32; d > 0; d >>= 1) {
-        w0 += __shfl_down(w0, d, 64);
-        w1 += __shfl_down(w1, d, 64);
-        w2 += __shfl_down(w2, d, 64);
+        d0 += __shfl_down(d0, d, 64);
+        d1 += __shfl_down(d1, d, 64);
+        d2 += __shfl_down(d2, d, 64);
     }
-    if (lane == 0) {
-        atomicAdd(Wacc + 0, w0 * c.vol);
-        atomicAdd(Wacc + 1, w1 * c.vol);
-        atomicAdd(Wacc + 2, w2 * c.vol);
+    if (lane == 0 && (d0 != 0.0 || d1 != 0.0 || d2 != 0.0)) {
+        atomicAdd(Wacc + 0, d0 * c.vol);
+        atomicAdd(Wacc + 1, d1 * c.vol);
+        atomicAdd(Wacc + 2, d2 * c.vol);
     }
 }
 
-/* Workgroups of the persistent push that fit on one CU at once. */
-int push32_blocks_per_cu(void)
+/* Range of tiles u (along one axis) whose J block [4u-1, 4u+5] holds node i. */
+__device__ static inline void covering(int i, int nt, int &lo, int &hi)
 {
-    int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_push32_tiled, TPB, 0) != hipSuccess || nb < 1)
-        nb = 1;
-    return nb;
+    lo = (i - 2 + 1024) / 4 - 256;          /* ceil((i - 5) / 4) = floor((i - 2) / 4), i >= -1022 */
+    hi = (i + 1 + 1024) / 4 - 256;          /* floor((i + 1) / 4) */
+    lo = lo < 0 ? 0 : lo;
+    hi = hi > nt - 1 ? nt - 1 : hi;
+}
+
+/* J += sum of the tile blocks covering each node, in FP64.  Every node of
+ * an active tile's block is written once, by the first active tile (in tile
+ * order) whose block holds it; the sum runs over the covering tiles in tile
+ * order, so the result does not depend on scheduling. */
+__global__ void __launch_bounds__(TPB) k_jgather32(F32View v, TileView tv, const float *tj)
+{
+    __shared__ int nslot[27];            /* slots of the 3x3x3 tiles around t, -1 empty */
+    const Grid &g = v.g;
+    const int nact = *tv.nactive;
+    for (int sl = blockIdx.x; sl < nact; sl += gridDim.x) {
+        const int t = tv.active[sl];
+        int t0, t1, t2;
+        tile_coords(tv, t, t0, t1, t2);
+        __syncthreads();
+        if (threadIdx.x < 27) {
+            const int ux = t0 - 1 + threadIdx.x / 9, uy = t1 - 1 + (threadIdx.x / 3) % 3,
+                      uz = t2 - 1 + threadIdx.x % 3;
+            const bool in = ux >= 0 && uy >= 0 && uz >= 0 && ux < tv.nt[0] && uy < tv.nt[1] &&
+                            uz < tv.nt[2];
+            nslot[threadIdx.x] = in ? tv.slot[(ux * tv.nt[1] + uy) * tv.nt[2] + uz] : -1;
+        }
+        __syncthreads();
+        for (int q = threadIdx.x; q < TJB; q += TPB) {
+            const int comp = q % 3, node = q / 3;
+            const int cc = node % TJN, bb = (node / TJN) % TJN, aa = node / (TJN * TJN);
+            const int i = TILE * t0 - 1 + aa, j = TILE * t1 - 1 + bb, k = TILE * t2 - 1 + cc;
+            if (i < -2 || j < -2 || k < -2 || i > g.n[0] + 2 || j > g.n[1] + 2 || k > g.n[2] + 2)
+                continue;
+            int xl, xh, yl, yh, zl, zh;
+            covering(i, tv.nt[0], xl, xh);
+            covering(j, tv.nt[1], yl, yh);
+            covering(k, tv.nt[2], zl, zh);
+            /* owner: the first covering tile (in tile order) that is active */
+            int own = -1;
+            for (int ux = xl; ux <= xh && own < 0; ++ux)
+                for (int uy = yl; uy <= yh && own < 0; ++uy)
+                    for (int uz = zl; uz <= zh && own < 0; ++uz)
+                        if (nslot[((ux - t0 + 1) * 3 + (uy - t1 + 1)) * 3 + (uz - t2 + 1)] >= 0)
+                            own = (ux * tv.nt[1] + uy) * tv.nt[2] + uz;
+            if (own != t) continue;
+            double sum = 0.0;
+            for (int ux = xl; ux <= xh; ++ux)
+                for (int uy = yl; uy <= yh; ++uy)
+                    for (int uz = zl; uz <= zh; ++uz) {
+                        const int su = nslot[((ux - t0 + 1) * 3 + (uy - t1 + 1)) * 3 + (uz - t2 + 1)];
+                        if (su < 0) continue;
+                        const int a = i - (TILE * ux - 1), b = j - (TILE * uy - 1), cz = k - (TILE * uz - 1);
+                        sum += (double)tj[(long)su * TJB + ((a * TJN + b) * TJN + cz) * 3 + comp];
+                    }
+            v.J[comp][gidx(g, i, j, k)] += sum;
+        }
+    }
 }
 
 int launch_push32(hipStream_t s, const Markers32 &m, const Fields32 &f, const Grid &g,
-                  const PushConsts &pc, const float *qm, const TileView &tv, double *Wacc,
-                  int nblocks)
+                  const PushConsts &pc, const float *qm, const TileView &tv, float *tj,
+                  double *Wacc, int nblocks)
 {
     F32View v;
     for (int a = 0; a < 3; ++a) {
@@ -462,9 +606,11 @@ int launch_push32(hipStream_t s, const Markers32 &m, const Fields32 &f, const Gr
         c.cj[a] = (float)(g.h[a] / g.tau);
     }
     c.vol = g.h[0] * g.h[1] * g.h[2];
-    for (int q = 0; q < 8; ++q) c.qm[q] = qm[q];
+    for (int q = 0; q < 4; ++q) c.qm[q] = qm[q];
     for (int b = 0; b < 6; ++b) c.bc[b] = pc.bc[b];
-    k_push32_tiled<<<nblocks, TPB, 0, s>>>(m, v, c, tv, Wacc);
+    k_push32_tiles<<<nblocks, TW, 0, s>>>(m, v, c, tv, tj, Wacc);
+    MPIC_CHECK(hipGetLastError());
+    k_jgather32<<<nblocks, TPB, 0, s>>>(v, tv, tj);
     MPIC_CHECK(hipGetLastError());
     return MPIC_OK;
 }

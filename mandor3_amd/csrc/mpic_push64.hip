@@ -67,7 +67,7 @@ struct GlobalSink {
 };
 
 #define TILE 4
-#define TJN (TILE + 3)             /* LDS J nodes per axis: tile + rim */
+/* TJN (mpic_tiles.h): LDS J nodes per axis, the tile plus its rim */
 
 struct TileSink {
     GlobalSink gs;

@@ -83,7 +83,9 @@ struct mpic_run {
     TileView      tv;
     int          *blocksum;
     int           parity;     /* tv.nmark[parity] = markers in the current order */
-    int           push_blocks;
+    int           push_blocks;  /* FP64: resident workgroups of the persistent push */
+    int           tile_blocks;  /* FP32: workgroups of the tile push (one per active tile) */
+    float        *tj;           /* FP32: per-tile J blocks, [ntiles][TJB] */
     CapBoxes      cH, cE;
     BoxSet        updH, updE, flH, flE;
     std::vector<FaceOp> bcH, bcE;
@@ -251,7 +253,9 @@ extern "C" int mpic_footprint(const mpic_config_t *cfg, size_t *bytes)
     const size_t tiles = (size_t)((cfg->n[0] + 3) / 4) * ((cfg->n[1] + 3) / 4) * ((cfg->n[2] + 3) / 4);
     const size_t mk = (cfg->precision == MPIC_FP32_SPLIT) ? (4 + 7 * 4 + 4) : (6 * 8 + 2 * 4 + 4 + 4);
     *bytes = field_bytes(cfg->n, cfg->precision == MPIC_FP32_SPLIT) + 2 * sizeof(double) * cells +
-             2 * (size_t)cfg->max_markers * mk + tiles * 64 * 3 * sizeof(int);
+             2 * (size_t)cfg->max_markers * mk + tiles * 64 * 3 * sizeof(int) +
+             tiles * 4 * sizeof(int) +
+             (cfg->precision == MPIC_FP32_SPLIT ? tiles * TJB * sizeof(float) : 0);
     return MPIC_OK;
 }
 
@@ -365,6 +369,8 @@ extern "C" int mpic_create(const mpic_config_t *cfg, mpic_run_t **out)
     MPIC_CHECK(hipMemsetAsync(tv.off, 0, sizeof(int) * (tv.ncellp + 1), r->stream));
     MPIC_CHECK(hipMalloc(&tv.key, sizeof(int) * (r->fp32 ? 1 : r->cap)));
     MPIC_CHECK(hipMalloc(&tv.active, sizeof(int) * tv.ntiles));
+    MPIC_CHECK(hipMalloc(&tv.slot, sizeof(int) * tv.ntiles));
+    if (r->fp32) MPIC_CHECK(hipMalloc(&r->tj, sizeof(float) * TJB * (size_t)tv.ntiles));
     MPIC_CHECK(hipMalloc(&tv.nactive, sizeof(int) * (4 + MPIC_XCDS)));
     tv.queue = tv.nactive + 1;
     tv.nmark = tv.nactive + 1 + MPIC_XCDS;
@@ -379,9 +385,12 @@ extern "C" int mpic_create(const mpic_config_t *cfg, mpic_run_t **out)
         if (hipGetDeviceProperties(&prop, r->dev) == hipSuccess) ncu = prop.multiProcessorCount;
         /* exactly the resident workgroups of the persistent push: extra ones
          * only arrive once the tile queues are drained */
-        r->push_blocks = ncu * (r->fp32 ? push32_blocks_per_cu() : push64_blocks_per_cu());
+        r->push_blocks = ncu * push64_blocks_per_cu();
+        /* enough workgroups for one active tile each in most runs; more
+         * active tiles are taken in turn (stride = grid size) */
+        r->tile_blocks = 16 * ncu < tv.ntiles ? 16 * ncu : tv.ntiles;
         const char *e = getenv("MPIC_PUSH_BLOCKS");
-        if (e && atoi(e) > 0) r->push_blocks = atoi(e);
+        if (e && atoi(e) > 0) r->push_blocks = r->tile_blocks = atoi(e);
     }
     r->n = 0;
     MPIC_CHECK(hipMalloc(&r->dacc, sizeof(double) * 8));
@@ -434,7 +443,8 @@ extern "C" int mpic_destroy(mpic_run_t *r)
         hipFree(m3.ux); hipFree(m3.uy); hipFree(m3.uz); hipFree(m3.w); hipFree(m3.id);
     }
     hipFree(r->tv.off); hipFree(r->tv.cursor); hipFree(r->tv.count); hipFree(r->tv.key);
-    hipFree(r->tv.active); hipFree(r->tv.nactive); hipFree(r->blocksum);
+    hipFree(r->tv.active); hipFree(r->tv.slot); hipFree(r->tv.nactive); hipFree(r->blocksum);
+    hipFree(r->tj);
     hipFree(r->dacc);
     hipFree(r->derr);
     if (r->hacc) hipHostFree(r->hacc);
@@ -998,8 +1008,8 @@ extern "C" int mpic_push_deposit(mpic_run_t *r)
     MPIC_CHECK(hipMemsetAsync(r->dacc + 2, 0, 3 * sizeof(double), r->stream));
     if (r->timed) MPIC_CHECK(hipEventRecord(r->ev[1], r->stream));
     if (r->fp32) {
-        TRY(launch_push32(r->stream, r->m32, r->f32, r->g, r->pc, r->qm, r->tv, r->dacc + 2,
-                          r->push_blocks));
+        TRY(launch_push32(r->stream, r->m32, r->f32, r->g, r->pc, r->qm, r->tv, r->tj, r->dacc + 2,
+                          r->tile_blocks));
     } else if (r->tiled) {
         TRY(launch_push64_tiled(r->stream, r->m, r->f, r->g, r->pc, r->tv, r->dacc + 2, r->derr,
                                 r->push_blocks));

@@ -24,6 +24,7 @@ struct TileView {
     int *count;              /* [ncellp] markers per key after the push (zeroed by the scan) */
     int *key;                /* [capacity] new key of every marker, -1 = deleted */
     int *active;             /* [ntiles] tiles holding markers, in tile order */
+    int *slot;               /* [ntiles] position of a tile in 'active', -1 if empty */
     int *nactive;            /* [1] */
     int *queue;              /* [MPIC_XCDS] work counters */
     int *nmark;              /* [2] markers in the current / next order */
@@ -106,12 +107,15 @@ int launch_push64_tiled(hipStream_t s, const Markers64 &m, const Fields64 &f, co
                         const PushConsts &c, const TileView &tv, double *Wacc, unsigned *err,
                         int nblocks);
 int push64_blocks_per_cu(void);
-int push32_blocks_per_cu(void);
 int launch_counts_natural(hipStream_t s, const TileView &tv, const Grid &g, int32_t *out);
 
+/* J block of one tile: nodes [4t-1, 4t+5] per axis, xyz interleaved. */
+#define TJN 7
+#define TJB (TJN * TJN * TJN * 3)
+
 int launch_push32(hipStream_t s, const Markers32 &m, const Fields32 &f, const Grid &g,
-                  const PushConsts &pc, const float *qm, const TileView &tv, double *Wacc,
-                  int nblocks);
+                  const PushConsts &pc, const float *qm, const TileView &tv, float *tj,
+                  double *Wacc, int nblocks);
 int launch_keys32(hipStream_t s, const Markers32 &m, long n, const TileView &tv);
 int launch_scatter32(hipStream_t s, const Markers32 &src, const Markers32 &dst, long nbound,
                      const TileView &tv, int parity);

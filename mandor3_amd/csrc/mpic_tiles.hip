@@ -135,7 +135,11 @@ __global__ void k_scan_c(TileView tv, const int *blocksum, const int *tileloc, i
     tv.cursor[q] = o;
     tv.count[q] = 0;
     const int tot = __shfl(incl, 63, 64);
-    if (lane == 0 && tot > 0) tv.active[blocksum[2 * tb + 1] + tileloc[2 * t + 1]] = t;
+    if (lane == 0) {
+        const int a = blocksum[2 * tb + 1] + tileloc[2 * t + 1];
+        if (tot > 0) tv.active[a] = t;
+        tv.slot[t] = tot > 0 ? a : -1;
+    }
     if (q == 0) {
         tv.off[tv.ncellp] = blocksum[2 * nb];
         *tv.nactive = blocksum[2 * nb + 1];

@@ -32,6 +32,11 @@
  *     by recomputation from the pinned fields.
  *   - rho / Gauss residuals: against the reference MC_GAUSS_TEST build's
  *     gauss_test.dat, and the discrete continuity identity (~1e-14).
+ *   - independently of the fixtures: continuity at round-off for single
+ *     markers crossing 0-3 faces (deposit), p linear in t under uniform E
+ *     and the exact 2 atan|t| rotation under uniform H (Boris), and the
+ *     exact discrete standing-wave solution of the Yee leapfrog (field
+ *     update order of the main loop).
  */
 #include <math.h>
 #include <stdlib.h>
