@@ -1,14 +1,16 @@
 """Benchmark of the PIC time step on one MI355X.
 
-    python bench.py [--steps K] [--warmup W] [--precision fp64|fp32]
+    python bench.py [--steps K] [--warmup W] [--precision fp32|fp64|both]
                     [--workload laser-foil|two-stream] [--no-cpu]
 
 A step is one full main-loop iteration (H half step, push + charge-conserving
-deposit, H step, E step, J fold, laser source, Mur/periodic walls) over the
-whole synthetic state, resident in HBM before the timed region.  Prints one
-JSON line: marker-pushes/s (live markers x steps / wall), the roofline of the
-push+deposit kernel from HIP events on the run's stream, and the CPU oracle
-timed on a bounded sample of the same workload on this host.
+deposit, per-step sort by cell, H step, E step, J fold, laser source, Mur /
+periodic walls) over the whole synthetic state, resident in HBM before the
+timed region; the K steps are enqueued without host synchronisation.  Prints
+one JSON line: marker-pushes/s (live markers x steps / wall) of the FP32
+split form with the FP64 plain form beside it, the roofline of the push
+kernel from HIP events on the run's stream, and the CPU oracle timed on a
+bounded sample of the same workload on this host.
 """
 import argparse
 import json
@@ -29,22 +31,60 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=100)
     p.add_argument("--warmup", type=int, default=10)
-    p.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
+    p.add_argument("--precision", default="both", choices=["fp32", "fp64", "both"],
+                   help="both: the FP32 split form gives 'value', the FP64 plain form "
+                        "is reported beside it")
     p.add_argument("--workload", default="laser-foil", choices=["laser-foil", "two-stream"])
     p.add_argument("--no-cpu", action="store_true")
-    p.add_argument("--untiled", action="store_true",
-                   help="unsorted one-thread-per-marker push (J atomics straight to HBM)")
     p.add_argument("--cpu-seconds", type=float, default=12.0)
     return p.parse_args()
 
 
 def push_bytes(n_markers, plasma_cells, precision):
-    """Algorithmic HBM bytes of one push+deposit launch: SoA marker read
-    (x y z ux uy uz as FP64 + w qm as FP32 = 56 B) and write (48 B), plus,
-    per cell holding markers, E/H read (48 B) and J read-modify-write (48 B)."""
+    """Algorithmic HBM bytes of one launch of the push kernel.
+
+    FP64 plain form: marker read 56 B (x y z ux uy uz FP64, w qm FP32) and
+    write 48 B; per cell holding markers E/H read 48 B and the J
+    read-modify-write 48 B (the kernel flushes J itself).
+    FP32 split form: marker read 32 B (cell word, 3 offsets, 3 momenta, w)
+    and write 28 B; per cell holding markers E/H read 24 B.  Its J (48 B per
+    node, FP64 read-modify-write) is summed by the separate gather kernel."""
     if precision == "fp64":
         return n_markers * (56 + 48) + plasma_cells * (48 + 48)
-    return n_markers * (32 + 28) + plasma_cells * (24 + 48)
+    return n_markers * (32 + 28) + plasma_cells * 24
+
+
+def run_gpu(ma, wl, precision, steps, warmup):
+    prec = ma.FP64 if precision == "fp64" else ma.FP32_SPLIT
+    M = wl["markers"]
+    r = ma.Run(wl["n"], wl["h"], wl["L"], wl["tau"], wl["bc"], 0.0, precision=prec,
+               max_markers=M.size)
+    z = np.zeros(3 * int(np.prod([v + 1 for v in wl["n"]])))
+    r.set_fields(z, z)
+    r.load_markers(M)
+    if wl["laser"] is not None:
+        r.add_laser(*wl["laser"])
+    r.init()
+    live = r.marker_count()
+    r.advance(warmup)
+    r.sync()
+    t0 = time.perf_counter()
+    r.advance(steps)
+    r.sync()
+    wall = time.perf_counter() - t0
+    push_ms, dep_ms, rest_ms = r.timing()
+    live_end = r.marker_count()
+    r.close()
+    avg_push = push_ms / steps
+    alg = push_bytes(live, wl["plasma_cells"], precision)
+    return {"pushes_per_s": 0.5 * (live + live_end) * steps / wall,
+            "ms_per_step": 1e3 * wall / steps,
+            "cell_updates_per_s": int(np.prod(wl["n"])) * steps / wall,
+            "markers": int(live),
+            "push_kernel_ms": avg_push, "deposit_rest_ms": dep_ms / steps,
+            "rest_of_step_ms": rest_ms / steps,
+            "push_kernel_bytes": alg,
+            "push_kernel_GBps": alg / (avg_push * 1e-3) / 1e9}
 
 
 def cpu_baseline(wl, seconds):
@@ -89,58 +129,47 @@ def main():
         print("bench.py: this build runs on one GPU only (--gpus %d requested); "
               "multi-GPU sharding is design-only" % a.gpus, file=sys.stderr)
         sys.exit(2)
-    if a.untiled:
-        os.environ["MPIC_UNTILED"] = "1"
     import mandor3_amd as ma
     from mandor3_amd import workloads
     if ma.device_count() < 1:
         raise SystemExit("no HIP device visible")
     wl = workloads.laser_foil() if a.workload == "laser-foil" else workloads.two_stream_box()
-    prec = ma.FP64 if a.precision == "fp64" else ma.FP32_SPLIT
-    M = wl["markers"]
-    r = ma.Run(wl["n"], wl["h"], wl["L"], wl["tau"], wl["bc"], 0.0, precision=prec,
-               max_markers=M.size)
-    z = np.zeros(3 * int(np.prod([v + 1 for v in wl["n"]])))
-    r.set_fields(z, z)
-    r.load_markers(M)
-    if wl["laser"] is not None:
-        r.add_laser(*wl["laser"])
-    r.init()
-    live = r.marker_count()
-    r.advance(a.warmup)
-    r.sync()
-    t0 = time.perf_counter()
-    r.advance(a.steps)
-    r.sync()
-    wall = time.perf_counter() - t0
-    push_ms, field_ms = r.timing()
-    live_end = r.marker_count()
-    pushes = 0.5 * (live + live_end) * a.steps
-    ncells = int(np.prod(wl["n"]))
-    avg_push_ms = push_ms / a.steps
-    alg = push_bytes(live, wl["plasma_cells"], a.precision)
-    achieved = alg / (avg_push_ms * 1e-3) / 1e9
+    main_prec = "fp64" if a.precision == "fp64" else "fp32"
+    res = {main_prec: run_gpu(ma, wl, main_prec, a.steps, a.warmup)}
+    if a.precision == "both":
+        res["fp64"] = run_gpu(ma, wl, "fp64", a.steps, a.warmup)
+    m = res[main_prec]
+    kernel = "k_push32_tiles" if main_prec == "fp32" else "k_push64_tiled"
     out = {
         "metric": "marker-pushes/s incl. deposit+FDTD, FP32 vs FP64 on 1 GPU; p50/p99 run latency",
-        "value": pushes / wall,
+        "value": m["pushes_per_s"],
         "unit": "marker-pushes/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": 1e3 * wall / a.steps,
+        "ms_per_step": m["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if a.precision == "fp64" else "f32+f64",
+        "dtype": "f32" if main_prec == "fp32" else "f64",
         "data": "synthetic",
-        "config": {"workload": wl["name"], "grid": list(wl["n"]), "markers": int(live),
-                   "precision": a.precision, "tiled": not a.untiled,
-                   "cell_updates_per_s": ncells * a.steps / wall,
-                   "push_ms_per_step": avg_push_ms, "field_ms_per_step": field_ms / a.steps},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "config": {"workload": wl["name"], "grid": list(wl["n"]), "markers": m["markers"],
+                   "precision": "fp32 split (int32 cell + FP32 offset; FP64 J sum)"
+                   if main_prec == "fp32" else "fp64 plain",
+                   "cell_updates_per_s": m["cell_updates_per_s"],
+                   "push_kernel_ms": m["push_kernel_ms"],
+                   "deposit_gather_ms": m["deposit_rest_ms"],
+                   "rest_of_step_ms": m["rest_of_step_ms"]},
+        "roofline": {"bound": "hbm", "kernel": kernel, "achieved": m["push_kernel_GBps"],
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": m["push_kernel_GBps"] / HBM_PEAK_GBS, "traffic": None},
     }
-    r.close()
+    if "fp64" in res and main_prec != "fp64":
+        f = res["fp64"]
+        out["fp64_plain"] = {"value": f["pushes_per_s"], "ms_per_step": f["ms_per_step"],
+                             "push_kernel_ms": f["push_kernel_ms"],
+                             "push_kernel_GBps": f["push_kernel_GBps"]}
+        out["fp32_over_fp64"] = m["pushes_per_s"] / f["pushes_per_s"]
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(wl, a.cpu_seconds)
     print(json.dumps(out))

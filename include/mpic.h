@@ -131,16 +131,19 @@ int  mpic_rho(mpic_run_t *run, double *rho);
 int  mpic_gauss_before(mpic_run_t *run);
 int  mpic_gauss_after(mpic_run_t *run, double *res);
 
-/* Per-step marker sort by cell (tile-bucketed, stable); drops deleted
- * markers.  counts (may be NULL): per-cell live-marker counts, n0*n1*n2. */
+/* Per-step marker sort by cell (tile-major cell keys; the order of markers
+ * inside one cell is unspecified); drops deleted markers.  counts (may be
+ * NULL): per-cell live-marker counts, n0*n1*n2. */
 int  mpic_sort(mpic_run_t *run, int32_t *counts);
 /* Cell index of every live marker (device order) for parity checks:
  * (i*n1 + j)*n2 + k of floor(x/h). */
 int  mpic_marker_cells(mpic_run_t *run, int32_t *cells, long *n);
 
-/* Kernel timing of the last mpic_advance (HIP events on the run's stream):
- * milliseconds spent in push+deposit and in the field update. */
-int  mpic_last_timing(mpic_run_t *run, double *push_ms, double *field_ms);
+/* Kernel timing of the last mpic_advance, from HIP events on the run's
+ * stream (waits for those steps to finish): milliseconds, summed over the
+ * steps, in the push kernel, in the rest of the deposit (the FP32 form's J
+ * gather; 0 for FP64) and in everything else of the step. */
+int  mpic_last_timing(mpic_run_t *run, double *push_ms, double *deposit_ms, double *rest_ms);
 
 #ifdef __cplusplus
 }

@@ -77,7 +77,7 @@ def lib():
             "mpic_gauss_after": (I, [P, P]),
             "mpic_sort": (I, [P, P]),
             "mpic_marker_cells": (I, [P, P, ctypes.POINTER(LG)]),
-            "mpic_last_timing": (I, [P, P, P]),
+            "mpic_last_timing": (I, [P, P, P, P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -203,9 +203,11 @@ class Run:
         _check(lib().mpic_sync(self.handle), "mpic_sync")
 
     def timing(self):
-        a, b = np.zeros(1), np.zeros(1)
-        _check(lib().mpic_last_timing(self.handle, _ptr(a), _ptr(b)), "mpic_last_timing")
-        return float(a[0]), float(b[0])
+        """(push kernel, rest of the deposit, rest of the step) in ms summed
+        over the steps of the last advance()."""
+        a, b, c = np.zeros(1), np.zeros(1), np.zeros(1)
+        _check(lib().mpic_last_timing(self.handle, _ptr(a), _ptr(b), _ptr(c)), "mpic_last_timing")
+        return float(a[0]), float(b[0]), float(c[0])
 
     @property
     def time(self):

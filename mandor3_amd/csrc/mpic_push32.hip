@@ -610,6 +610,19 @@ int launch_push32(hipStream_t s, const Markers32 &m, const Fields32 &f, const Gr
     for (int b = 0; b < 6; ++b) c.bc[b] = pc.bc[b];
     k_push32_tiles<<<nblocks, TW, 0, s>>>(m, v, c, tv, tj, Wacc);
     MPIC_CHECK(hipGetLastError());
+    return MPIC_OK;
+}
+
+int launch_jgather32(hipStream_t s, const Fields32 &f, const Grid &g, const TileView &tv,
+                     const float *tj, int nblocks)
+{
+    F32View v;
+    for (int a = 0; a < 3; ++a) {
+        v.E[a] = f.E[a];
+        v.H[a] = f.H[a];
+        v.J[a] = f.J[a];
+    }
+    v.g = g;
     k_jgather32<<<nblocks, TPB, 0, s>>>(v, tv, tj);
     MPIC_CHECK(hipGetLastError());
     return MPIC_OK;

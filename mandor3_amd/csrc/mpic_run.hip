@@ -97,10 +97,11 @@ struct mpic_run {
     unsigned     *derr;       /* device error flags (bit 0: marker off the grid) */
     int           stage;
     int           inited;
-    int           timed;      /* record push-kernel events (mpic_advance) */
+    int           timed;      /* >= 0: index of the timed step of mpic_advance */
     double        time;
-    hipEvent_t    ev[4];
-    double        push_ms, field_ms;
+    std::vector<hipEvent_t> evpool;  /* 5 events per timed step */
+    int           ev_steps;   /* timed steps of the last mpic_advance */
+
 };
 
 /* ---------------- boundary-cap geometry (single rank) ---------------- */
@@ -397,7 +398,7 @@ extern "C" int mpic_create(const mpic_config_t *cfg, mpic_run_t **out)
     MPIC_CHECK(hipHostMalloc(&r->hacc, sizeof(double) * 8, hipHostMallocDefault));
     MPIC_CHECK(hipMalloc(&r->derr, sizeof(unsigned)));
     MPIC_CHECK(hipMemsetAsync(r->derr, 0, sizeof(unsigned), r->stream));
-    for (int e = 0; e < 4; ++e) MPIC_CHECK(hipEventCreate(&r->ev[e]));
+    r->timed = -1;
     cap_geometry(cfg->n, cfg->bc, kUpdH, kFlH, r->cH);
     cap_geometry(cfg->n, cfg->bc, kUpdE, kFlE, r->cE);
     r->updH = make_set(r->cH.upd_s);
@@ -450,8 +451,7 @@ extern "C" int mpic_destroy(mpic_run_t *r)
     if (r->hacc) hipHostFree(r->hacc);
     for (double *b : r->murbuf) hipFree(b);
     for (Laser &l : r->lasers) hipFree(l.dmap);
-    for (int e = 0; e < 4; ++e)
-        if (r->ev[e]) hipEventDestroy(r->ev[e]);
+    for (hipEvent_t e : r->evpool) hipEventDestroy(e);
     if (r->stream) hipStreamDestroy(r->stream);
     delete r;
     return MPIC_OK;
@@ -999,6 +999,13 @@ extern "C" int mpic_em_energy(mpic_run_t *r, double *W_E, double *W_M)
     return MPIC_OK;
 }
 
+/* Event k of the timed step r->timed. */
+static int record(mpic_run *r, int k)
+{
+    MPIC_CHECK(hipEventRecord(r->evpool[5 * r->timed + k], r->stream));
+    return MPIC_OK;
+}
+
 extern "C" int mpic_push_deposit(mpic_run_t *r)
 {
     if (!r) return MPIC_ERR_ARG;
@@ -1006,17 +1013,22 @@ extern "C" int mpic_push_deposit(mpic_run_t *r)
     MPIC_CHECK(hipSetDevice(r->dev));
     TRY(launch_zero(r->stream, r->jp.J[0], 3 * r->g.ncell));
     MPIC_CHECK(hipMemsetAsync(r->dacc + 2, 0, 3 * sizeof(double), r->stream));
-    if (r->timed) MPIC_CHECK(hipEventRecord(r->ev[1], r->stream));
+    if (r->timed >= 0) TRY(record(r, 1));
     if (r->fp32) {
         TRY(launch_push32(r->stream, r->m32, r->f32, r->g, r->pc, r->qm, r->tv, r->tj, r->dacc + 2,
                           r->tile_blocks));
+        if (r->timed >= 0) TRY(record(r, 2));
+        TRY(launch_jgather32(r->stream, r->f32, r->g, r->tv, r->tj, r->tile_blocks));
     } else if (r->tiled) {
         TRY(launch_push64_tiled(r->stream, r->m, r->f, r->g, r->pc, r->tv, r->dacc + 2, r->derr,
                                 r->push_blocks));
     } else {
         TRY(launch_push64(r->stream, r->m, r->n, r->f, r->g, r->pc, r->dacc + 2, 0, r->derr));
     }
-    if (r->timed) MPIC_CHECK(hipEventRecord(r->ev[2], r->stream));
+    if (r->timed >= 0) {
+        if (!r->fp32) TRY(record(r, 2));
+        TRY(record(r, 3));
+    }
     /* per-step marker sort by cell (keys written by the push) */
     if (r->tiled) TRY(sort_pass(r, 0));
     return MPIC_OK;
@@ -1088,26 +1100,34 @@ extern "C" int mpic_em_e_finish(mpic_run_t *r)
     return MPIC_OK;
 }
 
-static int step_async(mpic_run *r, int timed)
+/* One step; with idx >= 0 its 5 events (start, push kernel start/end,
+ * deposit end, step end) are recorded into the pool. */
+static int step_body(mpic_run *r)
 {
-    r->timed = timed;
-    if (timed) MPIC_CHECK(hipEventRecord(r->ev[0], r->stream));
+    if (r->timed >= 0) TRY(record(r, 0));
     TRY(mpic_em_h_half(r));
     TRY(mpic_push_deposit(r));
-    r->timed = 0;
     TRY(mpic_em_h_step(r));
     TRY(mpic_em_e_start(r));
     TRY(mpic_j_finish(r));
     TRY(mpic_em_e_finish(r));
-    if (timed) MPIC_CHECK(hipEventRecord(r->ev[3], r->stream));
+    if (r->timed >= 0) TRY(record(r, 4));
     return MPIC_OK;
+}
+
+static int step_async(mpic_run *r, int idx)
+{
+    r->timed = idx;
+    const int rc = step_body(r);
+    r->timed = -1;
+    return rc;
 }
 
 extern "C" int mpic_step(mpic_run_t *r, double *diag)
 {
     if (!r) return MPIC_ERR_ARG;
     MPIC_CHECK(hipSetDevice(r->dev));
-    if (!diag) return step_async(r, 0);
+    if (!diag) return step_async(r, -1);
     const double t0 = r->time;
     double WE, WM;
     TRY(mpic_em_h_half(r));
@@ -1128,19 +1148,13 @@ extern "C" int mpic_advance(mpic_run_t *r, int nsteps)
 {
     if (!r || nsteps < 0) return MPIC_ERR_ARG;
     MPIC_CHECK(hipSetDevice(r->dev));
-    double push = 0, field = 0;
-    for (int s = 0; s < nsteps; ++s) {
-        TRY(step_async(r, 1));
-        MPIC_CHECK(hipEventSynchronize(r->ev[3]));
-        float a = 0, b = 0, c = 0;
-        MPIC_CHECK(hipEventElapsedTime(&a, r->ev[0], r->ev[1]));
-        MPIC_CHECK(hipEventElapsedTime(&b, r->ev[1], r->ev[2]));
-        MPIC_CHECK(hipEventElapsedTime(&c, r->ev[2], r->ev[3]));
-        push += b;          /* the push+deposit kernel alone */
-        field += a + c;     /* everything else of the step */
+    while ((int)r->evpool.size() < 5 * nsteps) {
+        hipEvent_t e;
+        MPIC_CHECK(hipEventCreate(&e));
+        r->evpool.push_back(e);
     }
-    r->push_ms = push;
-    r->field_ms = field;
+    for (int s = 0; s < nsteps; ++s) TRY(step_async(r, s));
+    r->ev_steps = nsteps;
     return MPIC_OK;
 }
 
@@ -1165,11 +1179,29 @@ extern "C" int mpic_sync(mpic_run_t *r)
 
 extern "C" double mpic_time(mpic_run_t *r) { return r ? r->time : NAN; }
 
-extern "C" int mpic_last_timing(mpic_run_t *r, double *push_ms, double *field_ms)
+extern "C" int mpic_last_timing(mpic_run_t *r, double *push_ms, double *deposit_ms,
+                                double *rest_ms)
 {
     if (!r) return MPIC_ERR_ARG;
-    if (push_ms) *push_ms = r->push_ms;
-    if (field_ms) *field_ms = r->field_ms;
+    double t[3] = {0, 0, 0};
+    if (r->ev_steps > 0) {
+        MPIC_CHECK(hipSetDevice(r->dev));
+        MPIC_CHECK(hipEventSynchronize(r->evpool[5 * r->ev_steps - 1]));
+        for (int s = 0; s < r->ev_steps; ++s) {
+            const hipEvent_t *e = &r->evpool[5 * s];
+            float a, b, c, d;
+            MPIC_CHECK(hipEventElapsedTime(&a, e[0], e[1]));
+            MPIC_CHECK(hipEventElapsedTime(&b, e[1], e[2]));
+            MPIC_CHECK(hipEventElapsedTime(&c, e[2], e[3]));
+            MPIC_CHECK(hipEventElapsedTime(&d, e[3], e[4]));
+            t[0] += b;
+            t[1] += c;
+            t[2] += a + d;
+        }
+    }
+    if (push_ms) *push_ms = t[0];
+    if (deposit_ms) *deposit_ms = t[1];
+    if (rest_ms) *rest_ms = t[2];
     return MPIC_OK;
 }
 

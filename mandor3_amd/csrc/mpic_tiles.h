@@ -116,6 +116,8 @@ int launch_counts_natural(hipStream_t s, const TileView &tv, const Grid &g, int3
 int launch_push32(hipStream_t s, const Markers32 &m, const Fields32 &f, const Grid &g,
                   const PushConsts &pc, const float *qm, const TileView &tv, float *tj,
                   double *Wacc, int nblocks);
+int launch_jgather32(hipStream_t s, const Fields32 &f, const Grid &g, const TileView &tv,
+                     const float *tj, int nblocks);
 int launch_keys32(hipStream_t s, const Markers32 &m, long n, const TileView &tv);
 int launch_scatter32(hipStream_t s, const Markers32 &src, const Markers32 &dst, long nbound,
                      const TileView &tv, int parity);
