@@ -65,7 +65,36 @@ def build(verbose=False):
         sys.stderr.write(r.stdout.decode(errors="replace"))
         raise RuntimeError("link failed")
     os.replace(tmp, LIB)
+    build_host()
     return LIB
+
+
+HOST = os.path.join(PKG, "host")
+HOST_LIB = os.path.join(PKG, "libmandor_host.so")
+CORE_EXE = os.path.join(PKG, "mandor_core")
+SETUP_EXE = os.path.join(PKG, "mandor_setup")
+CC = os.environ.get("CC", "gcc")
+CFLAGS = ["-O2", "-std=gnu99", "-Wall", "-Wextra", "-I" + os.path.join(ROOT, "include"),
+          "-I" + HOST]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout.decode(errors="replace"))
+        raise RuntimeError("failed: " + " ".join(cmd))
+
+
+def build_host():
+    """The C host side: libmandor_host.so (formats, RANMAR), mandor_setup
+    and mandor_core (linked to libmpic.so, found next to it)."""
+    src = os.path.join(HOST, "mandor_host.c")
+    obj = os.path.join(OUT, "mandor_host.o")
+    _run([CC] + CFLAGS + ["-fPIC", "-c", src, "-o", obj])
+    _run([CC, "-shared", obj, "-lm", "-o", HOST_LIB])
+    _run([CC] + CFLAGS + [os.path.join(HOST, "mandor_setup.c"), obj, "-lm", "-o", SETUP_EXE])
+    _run([CC] + CFLAGS + [os.path.join(HOST, "mandor_core.c"), obj, "-L" + PKG, "-lmpic",
+                          "-Wl,-rpath,$ORIGIN", "-lm", "-o", CORE_EXE])
 
 
 if __name__ == "__main__":

@@ -1,0 +1,601 @@
+/*
+ * mandor_host.c -- configuration and binData record formats, energy
+ * history, stop flag, laser source files and RANMAR for the C drivers.
+ * Each function cites the reference file:line whose format it reads or
+ * writes (cvarin/mandor3).
+ */
+#define _GNU_SOURCE
+#include "mandor_host.h"
+
+#include <errno.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdlib.h>
+#include <string.h>
+#include <sys/stat.h>
+
+static char g_err[512];
+
+static int fail(const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return -1;
+}
+
+const char *mh_error(void) { return g_err; }
+
+/* ---- '@' parameter files (source/misc_cfgReader.c:20-111) -------------- */
+
+static mh_tag_t *new_tag(mh_cfg_t *c, const char *name)
+{
+    mh_tag_t *t = (mh_tag_t *)realloc(c->tags, sizeof(mh_tag_t) * (c->ntags + 1));
+    if (!t) return NULL;
+    c->tags = t;
+    t = c->tags + c->ntags++;
+    memset(t, 0, sizeof(*t));
+    snprintf(t->name, sizeof(t->name), "%s", name);
+    return t;
+}
+
+static int push_value(char ***list, int *n, const char *word)
+{
+    char **l = (char **)realloc(*list, sizeof(char *) * (*n + 1));
+    if (!l) return -1;
+    *list = l;
+    l[*n] = strdup(word);
+    if (!l[*n]) return -1;
+    ++*n;
+    return 0;
+}
+
+int mh_cfg_load(const char *path, mh_cfg_t *cfg)
+{
+    memset(cfg, 0, sizeof(*cfg));
+    FILE *fp = fopen(path, "rt");
+    if (!fp) return fail("cannot open '%s': %s", path, strerror(errno));
+    mh_tag_t *cur = new_tag(cfg, "");
+    char line[4096];
+    while (cur && fgets(line, sizeof(line), fp)) {
+        if (line[0] == '[') {
+            char *e = strchr(line, ']');
+            if (!e) continue;
+            *e = 0;
+            cur = new_tag(cfg, line + 1);
+        } else if (line[0] == '@' || line[0] == '>') {
+            char word[512];
+            if (sscanf(line + 1, "%511s", word) != 1) {
+                fclose(fp);
+                return fail("'%s': empty parameter line", path);
+            }
+            if (line[0] == '@') {
+                if (push_value(&cur->req, &cur->nreq, word)) break;
+            } else if (push_value(&cur->opt, &cur->nopt, word)) {
+                break;
+            }
+        }
+    }
+    fclose(fp);
+    if (!cur) return fail("out of memory");
+    return 0;
+}
+
+void mh_cfg_free(mh_cfg_t *cfg)
+{
+    for (int t = 0; t < cfg->ntags; ++t) {
+        for (int q = 0; q < cfg->tags[t].nreq; ++q) free(cfg->tags[t].req[q]);
+        for (int q = 0; q < cfg->tags[t].nopt; ++q) free(cfg->tags[t].opt[q]);
+        free(cfg->tags[t].req);
+        free(cfg->tags[t].opt);
+    }
+    free(cfg->tags);
+    memset(cfg, 0, sizeof(*cfg));
+}
+
+int mh_tag_int(const mh_tag_t *t, int idx, int *v)
+{
+    if (idx < 0 || idx >= t->nreq) return fail("[%s]: parameter %d missing", t->name, idx + 1);
+    char *end;
+    long x = strtol(t->req[idx], &end, 10);
+    if (*end) return fail("[%s]: parameter %d '%s' is not an integer", t->name, idx + 1, t->req[idx]);
+    *v = (int)x;
+    return 0;
+}
+
+int mh_tag_double(const mh_tag_t *t, int idx, double *v)
+{
+    if (idx < 0 || idx >= t->nreq) return fail("[%s]: parameter %d missing", t->name, idx + 1);
+    char *end;
+    double x = strtod(t->req[idx], &end);
+    if (*end) return fail("[%s]: parameter %d '%s' is not a number", t->name, idx + 1, t->req[idx]);
+    *v = x;
+    return 0;
+}
+
+/* run_mandor.cfg, source/core/main.c:205-228. */
+int mh_runcfg_load(const char *path, mh_runcfg_t *rc)
+{
+    mh_cfg_t c;
+    if (mh_cfg_load(path, &c)) return -1;
+    int v[6], rcode = 0;
+    for (int q = 0; q < 6 && !rcode; ++q) rcode = mh_tag_int(&c.tags[0], q, v + q);
+    mh_cfg_free(&c);
+    if (rcode) return -1;
+    rc->total_steps = v[0];
+    rc->chpoint_full = v[1];
+    rc->chpoint_tecplot = v[2];
+    rc->continue_run = (v[3] == 1);
+    rc->chpoint_stop = v[4];
+    rc->chpoint_spectr = v[5];
+    if (v[0] == 0 || v[1] == 0 || v[2] == 0 || v[4] == 0 || v[5] == 0)
+        return fail("'%s' contains zero step(s)", path);
+    return 0;
+}
+
+/* ---- domain.bin, source/misc_parameters.c:98-131 ----------------------- */
+
+int mh_domain_load(const char *path, mh_domain_t *d)
+{
+    FILE *fp = fopen(path, "rb");
+    if (!fp) return fail("cannot open '%s'", path);
+    double v[7];
+    int i[12];
+    const int ok = fread(v, sizeof(double), 7, fp) == 7 && fread(i, sizeof(int), 12, fp) == 12;
+    fclose(fp);
+    if (!ok) return fail("'%s': short file", path);
+    for (int a = 0; a < 3; ++a) {
+        d->h[a] = v[a];
+        d->L[a] = v[3 + a];
+        d->dmn_min[a] = i[a];
+        d->dmn_max[a] = i[3 + a];
+        d->bc_min[a] = i[6 + a];
+        d->bc_max[a] = i[9 + a];
+    }
+    d->tau = v[6];
+    return 0;
+}
+
+int mh_domain_save(const char *path, const mh_domain_t *d)
+{
+    FILE *fp = fopen(path, "wb");
+    if (!fp) return fail("cannot create '%s'", path);
+    const double v[7] = {d->h[0], d->h[1], d->h[2], d->L[0], d->L[1], d->L[2], d->tau};
+    int i[12];
+    for (int a = 0; a < 3; ++a) {
+        i[a] = d->dmn_min[a];
+        i[3 + a] = d->dmn_max[a];
+        i[6 + a] = d->bc_min[a];
+        i[9 + a] = d->bc_max[a];
+    }
+    const int ok = fwrite(v, sizeof(double), 7, fp) == 7 && fwrite(i, sizeof(int), 12, fp) == 12;
+    return (fclose(fp) == 0 && ok) ? 0 : fail("write error on '%s'", path);
+}
+
+/* ---- units.bin, source/misc_units.c:39-118 ----------------------------- */
+
+#define CGS_C 2.9979e+10
+#define CGS_E 4.8032e-10
+#define CGS_M 9.1094e-28
+
+void mh_units_from_lambda(double lambda, mh_units_t *u)
+{
+    const double pi = M_PI;
+    u->lambda = lambda;
+    u->v = CGS_C;
+    u->t = lambda / u->v;
+    u->omega = 2 * pi / u->t;
+    u->A = CGS_M * CGS_C * u->omega / CGS_E;
+    u->E = u->A / (2 * pi);
+    u->n = CGS_M * u->omega * u->omega / (16 * pi * pi * pi * CGS_E * CGS_E);
+    u->rho = CGS_E * u->n;
+    u->micron = 1.0e-4 / lambda;
+    u->femtosecond = 1.0e-15 / u->t;
+    u->wpe = 2 * CGS_E * sqrt(pi * u->n / CGS_M) * u->t;
+    u->Tpe = 2 * pi / u->wpe;
+    u->ncr = u->omega * u->omega * CGS_M / (4 * pi * CGS_E * CGS_E);
+    u->S = -1;
+    u->debye = -1;
+}
+
+double mh_rho_crit(const mh_units_t *u) { return CGS_E * u->ncr / u->rho; }
+
+int mh_units_load(const char *path, mh_units_t *u)
+{
+    FILE *fp = fopen(path, "rb");
+    if (!fp) return fail("cannot open '%s'", path);
+    double v[15];
+    const int ok = fread(v, sizeof(double), 15, fp) == 15;
+    fclose(fp);
+    if (!ok) return fail("'%s': short file", path);
+    u->lambda = v[0]; u->v = v[1]; u->t = v[2]; u->omega = v[3]; u->E = v[4];
+    u->S = v[5]; u->A = v[6]; u->n = v[7]; u->rho = v[8]; u->ncr = v[9];
+    u->debye = v[10]; u->micron = v[11]; u->femtosecond = v[12]; u->wpe = v[13]; u->Tpe = v[14];
+    return 0;
+}
+
+int mh_units_save(const char *path, const mh_units_t *u)
+{
+    FILE *fp = fopen(path, "wb");
+    if (!fp) return fail("cannot create '%s'", path);
+    const double v[15] = {u->lambda, u->v, u->t, u->omega, u->E, u->S, u->A, u->n,
+                          u->rho, u->ncr, u->debye, u->micron, u->femtosecond, u->wpe, u->Tpe};
+    const int ok = fwrite(v, sizeof(double), 15, fp) == 15;
+    return (fclose(fp) == 0 && ok) ? 0 : fail("write error on '%s'", path);
+}
+
+/* ---- mesh records, source/type_mesh.c:285-357 -------------------------- */
+
+/* The reference's mesh_t as laid out on disk on x86-64 (88 bytes). */
+typedef struct {
+    int32_t imin, imax, jmin, jmax, kmin, kmax;
+    int32_t type, width_z, width_yz, size;
+    uint64_t storage, origin;
+    char     name[30];
+    char     pad[2];
+} disk_mesh_t;
+
+int mh_mesh_load(const char *path, const int n[3], double *block)
+{
+    FILE *fp = fopen(path, "rb");
+    if (!fp) return fail("cannot open '%s'", path);
+    int32_t r[6];
+    disk_mesh_t m;
+    if (fread(r, sizeof(int32_t), 6, fp) != 6 || fread(&m, sizeof(m), 1, fp) != 1) {
+        fclose(fp);
+        return fail("'%s': short header", path);
+    }
+    if (m.type != 1) {
+        fclose(fp);
+        return fail("'%s': not a vector mesh (type %d)", path, m.type);
+    }
+    const long span = r[5] - r[2] + 1;      /* kmax - kmin + 1 */
+    double *line = (double *)malloc(sizeof(double) * 3 * (span > 0 ? span : 1));
+    if (!line) {
+        fclose(fp);
+        return fail("out of memory");
+    }
+    int rc = 0;
+    for (int i = r[0]; i <= r[3] && !rc; ++i)
+        for (int j = r[1]; j <= r[4] && !rc; ++j) {
+            if (fread(line, sizeof(double) * 3, span, fp) != (size_t)span) {
+                rc = fail("'%s': short data", path);
+                break;
+            }
+            if (i < 0 || i > n[0] || j < 0 || j > n[1]) continue;
+            for (int k = r[2]; k <= r[5]; ++k) {
+                if (k < 0 || k > n[2]) continue;
+                double *dst = block + 3 * (((long)i * (n[1] + 1) + j) * (n[2] + 1) + k);
+                memcpy(dst, line + 3 * (k - r[2]), sizeof(double) * 3);
+            }
+        }
+    free(line);
+    fclose(fp);
+    return rc;
+}
+
+int mh_mesh_save(const char *path, const int n[3], const double *block, const int lo[3],
+                 const int hi[3], const char *name)
+{
+    FILE *fp = fopen(path, "wb");
+    if (!fp) return fail("cannot create '%s'", path);
+    const int32_t r[6] = {lo[0], lo[1], lo[2], hi[0], hi[1], hi[2]};
+    /* the core.out E/H meshes span [min - 1, max + 1] (source/core/main.c:109-115) */
+    disk_mesh_t m;
+    memset(&m, 0, sizeof(m));
+    m.imin = -1; m.imax = n[0] + 1;
+    m.jmin = -1; m.jmax = n[1] + 1;
+    m.kmin = -1; m.kmax = n[2] + 1;
+    m.type = 1;
+    m.width_z = n[2] + 3;
+    m.width_yz = (n[1] + 3) * (n[2] + 3);
+    m.size = (int32_t)((long)(n[0] + 3) * m.width_yz * 24);
+    snprintf(m.name, sizeof(m.name), "%s", name);
+    int ok = fwrite(r, sizeof(int32_t), 6, fp) == 6 && fwrite(&m, sizeof(m), 1, fp) == 1;
+    const long span = hi[2] - lo[2] + 1;
+    for (int i = lo[0]; i <= hi[0] && ok; ++i)
+        for (int j = lo[1]; j <= hi[1] && ok; ++j) {
+            const double *src = block + 3 * (((long)i * (n[1] + 1) + j) * (n[2] + 1) + lo[2]);
+            ok = fwrite(src, sizeof(double) * 3, span, fp) == (size_t)span;
+        }
+    return (fclose(fp) == 0 && ok) ? 0 : fail("write error on '%s'", path);
+}
+
+/* ---- marker records, source/core/plasma_IO.c:273-287 ------------------- */
+
+int mh_markers_load(const char *path, void **rec, long *n, int *recsize)
+{
+    FILE *fp = fopen(path, "rb");
+    if (!fp) return fail("cannot open '%s'", path);
+    int64_t N = 0;
+    struct stat st;
+    if (fread(&N, sizeof(N), 1, fp) != 1 || fstat(fileno(fp), &st) || N < 0) {
+        fclose(fp);
+        return fail("'%s': bad header", path);
+    }
+    const long bytes = (long)st.st_size - 8;
+    int rs = 64;
+    if (N > 0) {
+        if (bytes % N || (bytes / N != 56 && bytes / N != 64)) {
+            fclose(fp);
+            return fail("'%s': %ld bytes for %ld markers", path, bytes, (long)N);
+        }
+        rs = (int)(bytes / N);
+    }
+    void *buf = malloc(N > 0 ? (size_t)bytes : 1);
+    if (!buf) {
+        fclose(fp);
+        return fail("out of memory (%ld bytes)", bytes);
+    }
+    if (N > 0 && fread(buf, rs, N, fp) != (size_t)N) {
+        free(buf);
+        fclose(fp);
+        return fail("'%s': short data", path);
+    }
+    fclose(fp);
+    *rec = buf;
+    *n = (long)N;
+    *recsize = rs;
+    return 0;
+}
+
+int mh_markers_save(const char *path, const void *rec, long n, int recsize)
+{
+    FILE *fp = fopen(path, "wb");
+    if (!fp) return fail("cannot create '%s'", path);
+    const int64_t N = n;
+    int ok = fwrite(&N, sizeof(N), 1, fp) == 1;
+    if (ok && n > 0) ok = fwrite(rec, recsize, n, fp) == (size_t)n;
+    return (fclose(fp) == 0 && ok) ? 0 : fail("write error on '%s'", path);
+}
+
+/* ---- checkpoint descriptors, source/IO_sys.c:20-124, IO_fileMap.c ----- */
+
+static int read_first_ints(const char *path, int *v, int nv)
+{
+    mh_cfg_t c;
+    if (mh_cfg_load(path, &c)) return -1;
+    int rc = 0;
+    for (int q = 0; q < nv && !rc; ++q) rc = mh_tag_int(&c.tags[0], q, v + q);
+    mh_cfg_free(&c);
+    return rc;
+}
+
+int mh_sys_records(const char *bindir)
+{
+    char p[1024];
+    snprintf(p, sizeof(p), "%s/sys.N", bindir);
+    FILE *fp = fopen(p, "rt");
+    if (!fp) return -1;
+    fclose(fp);
+    int v;
+    return read_first_ints(p, &v, 1) ? -1 : v;
+}
+
+int mh_sys_info(const char *bindir, int rec, double *time, int *cpus, int *fmap)
+{
+    char p[1024];
+    snprintf(p, sizeof(p), "%s/sys_%06d.txt", bindir, rec);
+    mh_cfg_t c;
+    if (mh_cfg_load(p, &c)) return -1;
+    int rc = mh_tag_double(&c.tags[0], 0, time);
+    if (!rc) rc = mh_tag_int(&c.tags[0], 1, cpus);
+    if (!rc) rc = mh_tag_int(&c.tags[0], 2, fmap);
+    mh_cfg_free(&c);
+    return rc;
+}
+
+static int write_text(const char *path, const char *text)
+{
+    FILE *fp = fopen(path, "wt");
+    if (!fp) return fail("cannot create '%s'", path);
+    const int ok = fputs(text, fp) >= 0;
+    return (fclose(fp) == 0 && ok) ? 0 : fail("write error on '%s'", path);
+}
+
+int mh_sys_write(const char *bindir, int rec, double time, const int n[3], int setup_mode)
+{
+    char p[1024], t[512];
+    /* one file map per record: the whole domain on rank 0 */
+    snprintf(p, sizeof(p), "%s/fileMap_%06d.txt", bindir, rec);
+    snprintf(t, sizeof(t), "1 cpus total.\n0: (0 0 0) - (%d %d %d)\n", n[0], n[1], n[2]);
+    if (write_text(p, t)) return -1;
+    snprintf(p, sizeof(p), "%s/fileMap.N", bindir);
+    snprintf(t, sizeof(t), "@ %d filemaps total is written.\n", rec + 1);
+    if (write_text(p, t)) return -1;
+    if (setup_mode) {
+        snprintf(p, sizeof(p), "%s/fileMap_start.txt", bindir);
+        snprintf(t, sizeof(t),
+                 "@ %d filemaps are generated by setup - keep them on the start of simulation.\n",
+                 rec + 1);
+        if (write_text(p, t)) return -1;
+    }
+    snprintf(p, sizeof(p), "%s/sys_%06d.txt", bindir, rec);
+    snprintf(t, sizeof(t),
+             "@ %.14e    time.\n@ %d       cpus total.\n@ %d       file map id.\n"
+             "That is checkpoint (record of the full system state) #%d.\n",
+             time, 1, rec, rec);
+    if (write_text(p, t)) return -1;
+    snprintf(p, sizeof(p), "%s/sys.N", bindir);
+    snprintf(t, sizeof(t), "@ %d\t  system check-point records written.\n", rec + 1);
+    return write_text(p, t);
+}
+
+/* ---- diag.dat, source/core/diag_WDensity.c:81-105 ---------------------- */
+
+int mh_diag_append(const char *path, const double *pts, long npts, double tau, int fresh)
+{
+    FILE *fp = fresh ? NULL : fopen(path, "rt");
+    const int newfile = !fp;
+    if (fp) fclose(fp);
+    fp = fopen(path, newfile ? "wt" : "at");
+    if (!fp) return fail("cannot open '%s'", path);
+    if (newfile)
+        fprintf(fp, "variables = t, W_E, W_M, W_T_x, W_T_y, W_T_z, W_p_e_r_p, W_k_i_n, W_E_M, W_`S\n"
+                    "zone f=point\n");
+    for (long q = 0; q < npts; ++q) {
+        const double *d = pts + 6 * q;
+        if (d[0] < 0.5 * tau) continue;
+        fprintf(fp, "%e %e %e %e %e %e %e %e %e %e\n", d[0], d[1], d[2], d[3], d[4], d[5],
+                d[4] + d[5], d[3] + d[4] + d[5], d[1] + d[2], d[1] + d[2] + d[3] + d[4] + d[5]);
+    }
+    return fclose(fp) == 0 ? 0 : fail("write error on '%s'", path);
+}
+
+/* ---- tmp/stop.flag, source/core/main.c:121-160,231-236 ----------------- */
+
+int mh_stopflag_read(const char *path, int *force, int *limit)
+{
+    FILE *fp = fopen(path, "rt");
+    if (!fp) return fail("cannot open '%s'", path);
+    const int k = fscanf(fp, "%d %d", force, limit);
+    fclose(fp);
+    return k == 2 ? 0 : fail("'%s': two integers expected", path);
+}
+
+int mh_stopflag_default(const char *path)
+{
+    FILE *fp = fopen(path, "rt");
+    if (fp) {
+        fclose(fp);
+        return 0;
+    }
+    char t[64];
+    snprintf(t, sizeof(t), "0\n%d\n", 23 * 60 * 60 + 30 * 60);
+    return write_text(path, t);
+}
+
+/* ---- laser source files, source/scpic/em_mirror.c:291-485 -------------- */
+
+#define SOURCE_PLANE 5          /* OFFSET / SOURCE_X_OFFSET of em_mirror.c */
+
+int mh_laser_load(const char *path, const mh_domain_t *d, const mh_units_t *u, mh_laser_t *l)
+{
+    memset(l, 0, sizeof(*l));
+    FILE *fp = fopen(path, "rb");
+    if (!fp) return fail("cannot open '%s'", path);
+    char line[4096];
+    double v[10];
+    int nv = 0, dashes = 0;
+    /* header: title, dashes, 10 '@' lines, dashes; the maps follow */
+    while (dashes < 2 && fgets(line, sizeof(line), fp)) {
+        if (!strncmp(line, "----", 4)) {
+            ++dashes;
+        } else if (line[0] == '@' && nv < 10) {
+            if (sscanf(line + 1, "%lf", v + nv) != 1) break;
+            ++nv;
+        }
+    }
+    if (dashes < 2 || nv != 10) {
+        fclose(fp);
+        return fail("'%s': bad header (%d values)", path, nv);
+    }
+    double A = v[0];
+    if (A > 0)   /* intensity [W/cm^2] -> e E/(m omega c) */
+        A = u->lambda * CGS_E * sqrt(8 * M_PI * A * 1e7 / CGS_C) / (CGS_M * CGS_C * CGS_C);
+    else
+        A *= -2 * M_PI;
+    const double xf = v[2] * 1e-4 / u->lambda;
+    l->A = A;
+    l->dt = v[5] * 1e-15 / u->t * sqrt(2.0 / log(2.0));
+    l->T0 = v[6] * 1e-15 / u->t;
+    l->phi0 = v[7];
+    l->forward = v[9] > 0;
+    l->plane = SOURCE_PLANE;
+    l->delay = l->forward ? xf - SOURCE_PLANE * d->h[0]
+                          : d->dmn_max[0] - SOURCE_PLANE * d->h[0] - xf;
+    l->nodes = (long)(d->dmn_max[1] + 1 - 2 * SOURCE_PLANE) * (d->dmn_max[2] + 1 - 2 * SOURCE_PLANE);
+    if (l->nodes <= 0) {
+        fclose(fp);
+        return fail("'%s': domain too small for a laser source", path);
+    }
+    l->map = (double *)malloc(sizeof(double) * 8 * l->nodes);
+    if (!l->map) {
+        fclose(fp);
+        return fail("out of memory");
+    }
+    const size_t want = 8 * (size_t)l->nodes;
+    const size_t got = fread(l->map, sizeof(double), want, fp);
+    fclose(fp);
+    if (got != want) {
+        mh_laser_free(l);
+        return fail("'%s': %zu of %zu map values", path, got, want);
+    }
+    return 0;
+}
+
+int mh_laser_save(const char *path, const mh_domain_t *d, const mh_units_t *u,
+                  double a, double fnum, const double focus_um[3], double dt_fs, double T0_fs,
+                  double phi0, int forward, const double *map)
+{
+    (void)u;
+    FILE *fp = fopen(path, "wb");
+    if (!fp) return fail("cannot create '%s'", path);
+    fprintf(fp, "Mirror focused laser beam #%d\n", 0);
+    fprintf(fp, "---------------------------------------\n");
+    fprintf(fp, "@ %.15le   Poynting vector at the best focus: positive => I [W/cm^2] / negative => e*E/(m*omega*c).\n", a);
+    fprintf(fp, "@ %.15le   Mirror f-number.\n", fnum);
+    fprintf(fp, "@ %.15le   Focus X coordinate [micron].\n", focus_um[0]);
+    fprintf(fp, "@ %.15le   Focus Y coordinate [micron].\n", focus_um[1]);
+    fprintf(fp, "@ %.15le   Focus Z coordinate [micron].\n", focus_um[2]);
+    fprintf(fp, "@ %.15le   Pulse duration, [fs] (intensity FWHM), Gaussian envelope.\n", dt_fs);
+    fprintf(fp, "@ %.15le   Pulse to focus arrival time [fs].\n", T0_fs);
+    fprintf(fp, "@ %.15le   Phase shift.\n", phi0);
+    fprintf(fp, "@ %.15le   Ratio between incident pulse and mirror radii (-1: plane wave).\n", -1.0);
+    fprintf(fp, "@ %d       Direction of propagation (positive or negative).\n", forward ? 1 : -1);
+    fprintf(fp, "---------------------------------------\n");
+    const long nodes = (long)(d->dmn_max[1] + 1 - 2 * SOURCE_PLANE) * (d->dmn_max[2] + 1 - 2 * SOURCE_PLANE);
+    const int ok = fwrite(map, sizeof(double), 8 * nodes, fp) == (size_t)(8 * nodes);
+    return (fclose(fp) == 0 && ok) ? 0 : fail("write error on '%s'", path);
+}
+
+void mh_laser_free(mh_laser_t *l)
+{
+    free(l->map);
+    l->map = NULL;
+}
+
+/* ---- RANMAR, source/misc_ranmar.c:78-146 ------------------------------ */
+
+int mh_ranmar_init(mh_ranmar_t *r, int ij, int kl)
+{
+    if (ij < 0 || ij > 31328 || kl < 0 || kl > 30081)
+        return fail("RANMAR seeds out of range (0 <= ij <= 31328, 0 <= kl <= 30081)");
+    int i = (ij / 177) % 177 + 2, j = ij % 177 + 2, k = (kl / 169) % 178 + 1, l = kl % 169;
+    for (int ii = 0; ii < 97; ++ii) {
+        double s = 0.0, t = 0.5;
+        for (int jj = 0; jj < 24; ++jj) {
+            const int m = (((i * j) % 179) * k) % 179;
+            i = j;
+            j = k;
+            k = m;
+            l = (53 * l + 1) % 169;
+            if ((l * m) % 64 >= 32) s += t;
+            t *= 0.5;
+        }
+        r->u[ii] = s;
+    }
+    r->c = 362436.0 / 16777216.0;
+    r->cd = 7654321.0 / 16777216.0;
+    r->cm = 16777213.0 / 16777216.0;
+    r->i97 = 96;
+    r->j97 = 32;
+    return 0;
+}
+
+void mh_ranmar(mh_ranmar_t *r, double *out, int len)
+{
+    for (int q = 0; q < len; ++q) {
+        double uni = r->u[r->i97] - r->u[r->j97];
+        if (uni < 0.0) uni += 1.0;
+        r->u[r->i97] = uni;
+        if (--r->i97 < 0) r->i97 = 96;
+        if (--r->j97 < 0) r->j97 = 96;
+        r->c -= r->cd;
+        if (r->c < 0.0) r->c += r->cm;
+        uni -= r->c;
+        if (uni < 0.0) uni += 1.0;
+        out[q] = uni;
+    }
+}
