@@ -118,8 +118,23 @@ int  mpic_em_e_finish(mpic_run_t *run);                         /* em_EStep_fini
  * WTy, WTz} as handed to wDensity_addPoint (source/core/diag_WDensity.c:136).
  * With diag == NULL nothing is copied back and the step stays asynchronous. */
 int  mpic_step(mpic_run_t *run, double *diag);
-/* n steps enqueued on the run's stream without host synchronisation. */
+/* n steps enqueued on the run's stream without host synchronisation; with
+ * graph replay on, as replays of the captured step. */
 int  mpic_advance(mpic_run_t *run, int nsteps);
+/* Graph replay (no counterpart in the reference: its loop calls the stages
+ * one by one): on = 1 captures one step per marker-buffer parity into HIP
+ * graphs at the next advance and replays them; results are those of the
+ * stage calls.  Kernel timing (mpic_last_timing) is recorded only with
+ * replay off. */
+int  mpic_set_graphs(mpic_run_t *run, int on);
+/* An ensemble of runs advanced together, step by step across the runs,
+ * each replaying its captured step on its own stream (no counterpart in the
+ * reference, which runs one simulation per job). */
+int  mpic_batch_advance(mpic_run_t *const *runs, int nruns, int nsteps);
+/* Picks the workgroup count of the FP32 push and J gather by timing
+ * 'steps' real steps per candidate (the steps advance the run; the choice
+ * does not change results).  *chosen receives the count. */
+int  mpic_autotune(mpic_run_t *run, int steps, int *chosen);
 int  mpic_sync(mpic_run_t *run);
 double mpic_time(mpic_run_t *run);
 

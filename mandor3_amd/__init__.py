@@ -70,6 +70,9 @@ def lib():
             "mpic_em_e_finish": (I, [P]),
             "mpic_step": (I, [P, P]),
             "mpic_advance": (I, [P, I]),
+            "mpic_set_graphs": (I, [P, I]),
+            "mpic_batch_advance": (I, [P, I, I]),
+            "mpic_autotune": (I, [P, I, P]),
             "mpic_sync": (I, [P]),
             "mpic_time": (D, [P]),
             "mpic_rho": (I, [P, P]),
@@ -95,6 +98,13 @@ def _check(rc, what):
     if rc != 0:
         msg = lib().mpic_last_error()
         raise MpicError("%s failed (%d): %s" % (what, rc, msg.decode() if msg else ""))
+
+
+def batch_advance(runs, nsteps):
+    """Advances an ensemble of runs together, step by step across the runs,
+    each replaying its captured step on its own stream."""
+    arr = (ctypes.c_void_p * len(runs))(*[r.handle.value for r in runs])
+    _check(lib().mpic_batch_advance(arr, len(runs), int(nsteps)), "mpic_batch_advance")
 
 
 def device_count():
@@ -198,6 +208,17 @@ class Run:
 
     def advance(self, nsteps):
         _check(lib().mpic_advance(self.handle, int(nsteps)), "mpic_advance")
+
+    def set_graphs(self, on=True):
+        """Replay captured HIP graphs of the step in advance()."""
+        _check(lib().mpic_set_graphs(self.handle, 1 if on else 0), "mpic_set_graphs")
+
+    def autotune(self, steps=3):
+        """Times `steps` real steps per candidate workgroup count of the FP32
+        push and keeps the fastest; returns the count."""
+        c = np.zeros(1, dtype=np.int32)
+        _check(lib().mpic_autotune(self.handle, int(steps), _ptr(c)), "mpic_autotune")
+        return int(c[0])
 
     def sync(self):
         _check(lib().mpic_sync(self.handle), "mpic_sync")

@@ -309,12 +309,15 @@ int launch_e_tauJ(hipStream_t s, const FieldsT<T> &f, const Grid &g)
  * host once per step and passed in. */
 template <class T>
 __global__ void k_laser(T *Fy, T *Fz, Grid g, const double *map, int pos, int JMIN, int KMIN,
-                        int KMAX, int jmin, int jmax, int kmin, int kmax, double A, double cs,
-                        double sn)
+                        int KMAX, int jmin, int jmax, int kmin, int kmax, const double *tab,
+                        const int *step, int cap)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     int nk = kmax - kmin + 1;
     if (t >= (long)(jmax - jmin + 1) * nk) return;
+    /* {A, cos, sin} of this step, evaluated on the host */
+    const double *e = tab + 3 * (*step % cap);
+    const double A = e[0], cs = e[1], sn = e[2];
     int j = jmin + (int)(t / nk), k = kmin + (int)(t % nk);
     long cnt = (long)(j - JMIN) * (KMAX - KMIN + 1) + k - KMIN;
     const double *w = map + 4 * cnt;
@@ -325,13 +328,26 @@ __global__ void k_laser(T *Fy, T *Fz, Grid g, const double *map, int pos, int JM
 
 template <class T>
 int launch_laser(hipStream_t s, T *Fy, T *Fz, const Grid &g, const double *map, int pos, int JMIN,
-                 int KMIN, int KMAX, int jmin, int jmax, int kmin, int kmax, double A, double cs,
-                 double sn)
+                 int KMIN, int KMAX, int jmin, int jmax, int kmin, int kmax, const double *tab,
+                 const int *step, int cap)
 {
     long n = (long)(jmax - jmin + 1) * (kmax - kmin + 1);
     if (jmax < jmin || kmax < kmin) return MPIC_OK;
     k_laser<T><<<nblk(n), TPB, 0, s>>>(Fy, Fz, g, map, pos, JMIN, KMIN, KMAX, jmin, jmax, kmin,
-                                        kmax, A, cs, sn);
+                                        kmax, tab, step, cap);
+    MPIC_CHECK(hipGetLastError());
+    return MPIC_OK;
+}
+
+/* The run's step counter, advanced once per step on the stream. */
+__global__ void k_tick(int *step)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *step += 1;
+}
+
+int launch_tick(hipStream_t s, int *step)
+{
+    k_tick<<<1, 64, 0, s>>>(step);
     MPIC_CHECK(hipGetLastError());
     return MPIC_OK;
 }
@@ -544,7 +560,8 @@ int launch_gauss(hipStream_t s, const FieldsT<T> &f, const double *rho1, const d
                                     const BoxSet &, int, int);                                  \
     template int launch_e_tauJ<T>(hipStream_t, const FieldsT<T> &, const Grid &);                \
     template int launch_laser<T>(hipStream_t, T *, T *, const Grid &, const double *, int, int,  \
-                                 int, int, int, int, int, int, double, double, double);         \
+                                 int, int, int, int, int, int, const double *, const int *,     \
+                                 int);                                                          \
     template int launch_energy<T>(hipStream_t, const FieldsT<T> &, const Grid &, double *);      \
     template int launch_gauss<T>(hipStream_t, const FieldsT<T> &, const double *,                \
                                  const double *, const Grid &, double *);

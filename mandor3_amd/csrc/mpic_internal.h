@@ -126,8 +126,9 @@ int launch_e_update(hipStream_t s, const FieldsT<T> &f, const Grid &g, const Box
 template <class T> int launch_e_tauJ(hipStream_t s, const FieldsT<T> &f, const Grid &g);
 template <class T>
 int launch_laser(hipStream_t s, T *Fy, T *Fz, const Grid &g, const double *map, int pos,
-                 int JMIN, int KMIN, int KMAX, int jmin, int jmax, int kmin, int kmax, double A,
-                 double cs, double sn);
+                 int JMIN, int KMIN, int KMAX, int jmin, int jmax, int kmin, int kmax,
+                 const double *tab, const int *step, int cap);
+int launch_tick(hipStream_t s, int *step);
 int launch_jfold(hipStream_t s, const JPtrs &f, const Grid &g, int b, int bc);
 template <class T> int launch_energy(hipStream_t s, const FieldsT<T> &f, const Grid &g, double *acc);
 int launch_zero(hipStream_t s, double *p, long n);

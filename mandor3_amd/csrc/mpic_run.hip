@@ -49,11 +49,16 @@ extern "C" int mpic_device_count(void)
             TRY(fn(r->stream, r->f, __VA_ARGS__));                       \
     } while (0)
 
+/* Steps covered by one fill of the laser tables. */
+#define LASER_CAP 1024
+
 struct Laser {
     double  A, T0, delay, dt, phi0;
     int     forward, plane;
     double *dmap;             /* device: E map then H map */
     long    nodes;
+    double *dtab;             /* device: per step {A, cos, sin}, H table then E table */
+    double *htab;             /* pinned staging of dtab */
 };
 
 struct CapBoxes {             /* one field's six caps */
@@ -85,6 +90,7 @@ struct mpic_run {
     int           parity;     /* tv.nmark[parity] = markers in the current order */
     int           push_blocks;  /* FP64: resident workgroups of the persistent push */
     int           tile_blocks;  /* FP32: workgroups of the tile push (one per active tile) */
+    int           ncu;          /* compute units of the device */
     float        *tj;           /* FP32: per-tile J blocks, [ntiles][TJB] */
     CapBoxes      cH, cE;
     BoxSet        updH, updE, flH, flE;
@@ -100,6 +106,13 @@ struct mpic_run {
     int           timed;      /* >= 0: index of the timed step of mpic_advance */
     double        time;
     std::vector<hipEvent_t> evpool;  /* 5 events per timed step */
+    int          *dstep;      /* device step counter (indexes the laser tables) */
+    long          step;       /* host copy of the step counter */
+    long          tab_lo, tab_hi;  /* steps the laser tables hold */
+    int           use_graphs; /* mpic_advance replays captured steps */
+    int           graphs_ok;  /* exec[] match the current launch topology */
+    hipGraphExec_t exec[2];   /* one step from each of the two marker-buffer parities */
+    int           exec_parity;     /* parity at which exec[0] was captured */
     int           ev_steps;   /* timed steps of the last mpic_advance */
 
 };
@@ -390,6 +403,7 @@ extern "C" int mpic_create(const mpic_config_t *cfg, mpic_run_t **out)
         /* enough workgroups for one active tile each in most runs; more
          * active tiles are taken in turn (stride = grid size) */
         r->tile_blocks = 16 * ncu < tv.ntiles ? 16 * ncu : tv.ntiles;
+        r->ncu = ncu;
         const char *e = getenv("MPIC_PUSH_BLOCKS");
         if (e && atoi(e) > 0) r->push_blocks = r->tile_blocks = atoi(e);
     }
@@ -399,6 +413,8 @@ extern "C" int mpic_create(const mpic_config_t *cfg, mpic_run_t **out)
     MPIC_CHECK(hipMalloc(&r->derr, sizeof(unsigned)));
     MPIC_CHECK(hipMemsetAsync(r->derr, 0, sizeof(unsigned), r->stream));
     r->timed = -1;
+    MPIC_CHECK(hipMalloc(&r->dstep, sizeof(int)));
+    MPIC_CHECK(hipMemsetAsync(r->dstep, 0, sizeof(int), r->stream));
     cap_geometry(cfg->n, cfg->bc, kUpdH, kFlH, r->cH);
     cap_geometry(cfg->n, cfg->bc, kUpdE, kFlE, r->cE);
     r->updH = make_set(r->cH.upd_s);
@@ -452,6 +468,13 @@ extern "C" int mpic_destroy(mpic_run_t *r)
     for (double *b : r->murbuf) hipFree(b);
     for (Laser &l : r->lasers) hipFree(l.dmap);
     for (hipEvent_t e : r->evpool) hipEventDestroy(e);
+    for (Laser &l : r->lasers) {
+        hipFree(l.dtab);
+        hipHostFree(l.htab);
+    }
+    hipFree(r->dstep);
+    for (int k = 0; k < 2; ++k)
+        if (r->exec[k]) hipGraphExecDestroy(r->exec[k]);
     if (r->stream) hipStreamDestroy(r->stream);
     delete r;
     return MPIC_OK;
@@ -692,6 +715,7 @@ extern "C" int mpic_markers_load(mpic_run_t *r, const void *rec, long n, int rec
         mpic_set_error("mpic_markers_load: bad arguments");
         return MPIC_ERR_ARG;
     }
+    r->graphs_ok = 0;
     if (n > r->cap) {
         mpic_set_error("mpic_markers_load: %ld markers exceed the capacity %ld", n, r->cap);
         return MPIC_ERR_CAPACITY;
@@ -807,7 +831,11 @@ extern "C" int mpic_add_laser(mpic_run_t *r, double A, double T0, double delay, 
     MPIC_CHECK(hipSetDevice(r->dev));
     MPIC_CHECK(hipMalloc(&l.dmap, sizeof(double) * 8 * l.nodes));
     MPIC_CHECK(hipMemcpy(l.dmap, map, sizeof(double) * 8 * l.nodes, hipMemcpyHostToDevice));
+    MPIC_CHECK(hipMalloc(&l.dtab, sizeof(double) * 6 * LASER_CAP));
+    MPIC_CHECK(hipHostMalloc(&l.htab, sizeof(double) * 6 * LASER_CAP, hipHostMallocDefault));
     r->lasers.push_back(l);
+    r->tab_hi = r->tab_lo = 0;      /* refill before the next step */
+    r->graphs_ok = 0;
     return MPIC_OK;
 }
 
@@ -824,6 +852,18 @@ static int init_ghosts(mpic_run *r, T *const E[3], T *const H[3]);
 /* Counting sort of the current markers by cell key: with keys == 1 the
  * keys are computed here (loaded or untouched markers), else they come
  * from the push that just ran. */
+/* The sorted array becomes the current one. */
+static void swap_markers(mpic_run *r)
+{
+    const int other = r->cur ^ 1;
+    if (r->fp32)
+        r->m32 = r->m32buf[other];
+    else
+        r->m = r->mbuf[other];
+    r->cur = other;
+    r->parity ^= 1;
+}
+
 static int sort_pass(mpic_run *r, int keys)
 {
     const int other = r->cur ^ 1;
@@ -831,15 +871,12 @@ static int sort_pass(mpic_run *r, int keys)
         if (keys) TRY(launch_keys32(r->stream, r->m32, r->n, r->tv));
         TRY(launch_tile_scan(r->stream, r->tv, r->blocksum, r->parity ^ 1));
         TRY(launch_scatter32(r->stream, r->m32, r->m32buf[other], r->n, r->tv, r->parity));
-        r->m32 = r->m32buf[other];
     } else {
         if (keys) TRY(launch_keys64(r->stream, r->m, r->n, r->g, r->tv));
         TRY(launch_tile_scan(r->stream, r->tv, r->blocksum, r->parity ^ 1));
         TRY(launch_scatter64(r->stream, r->m, r->mbuf[other], r->n, r->tv, r->parity));
-        r->m = r->mbuf[other];
     }
-    r->cur = other;
-    r->parity ^= 1;
+    swap_markers(r);
     return MPIC_OK;
 }
 
@@ -927,22 +964,56 @@ static int check_stage(mpic_run *r, int want)
     return MPIC_OK;
 }
 
+/* The laser tables for steps [r->step, r->step + n): amplitude x tau/h,
+ * cos and sin of the phase of each step, for the H source (time t) and the
+ * E source (t + tau/2), with t advanced exactly as the run's clock
+ * (source/scpic/em_mirror.c:667-717).  Refilled LASER_CAP steps at a time
+ * on the run's stream. */
+static int laser_tables(mpic_run *r, long n)
+{
+    if (r->lasers.empty() || (r->step >= r->tab_lo && r->step + n <= r->tab_hi)) return MPIC_OK;
+    if (n > LASER_CAP) {
+        mpic_set_error("laser tables hold %d steps", LASER_CAP);
+        return MPIC_ERR_ARG;
+    }
+    const Grid &g = r->g;
+    MPIC_CHECK(hipStreamSynchronize(r->stream));     /* the staging buffers are reused */
+    for (Laser &l : r->lasers) {
+        double time = r->time;
+        for (long q = 0; q < LASER_CAP; ++q) {
+            const long idx = (r->step + q) % LASER_CAP;
+            for (int half = 0; half < 2; ++half) {
+                const double t = (half ? time + 0.5 * g.tau : time) - l.T0 + l.delay;
+                const double ampl = exp(-pow(2 * t / l.dt, 2));
+                const double phase = 2 * M_PI * t + l.phi0;
+                double *e = l.htab + 3 * (half * LASER_CAP + idx);
+                e[0] = ampl * g.tau / g.h[0];
+                e[1] = cos(phase);
+                e[2] = sin(phase);
+            }
+            time = time + g.tau;
+        }
+        MPIC_CHECK(hipMemcpyAsync(l.dtab, l.htab, sizeof(double) * 6 * LASER_CAP,
+                                  hipMemcpyHostToDevice, r->stream));
+    }
+    r->tab_lo = r->step;
+    r->tab_hi = r->step + LASER_CAP;
+    return MPIC_OK;
+}
+
 template <class T>
 static int laser_H(mpic_run *r, T *Fy, T *Fz, const Box &tu)
 {
     const Grid &g = r->g;
     for (const Laser &l : r->lasers) {
-        const double t = r->time - l.T0 + l.delay;
-        const double ampl = exp(-pow(2 * t / l.dt, 2));
         const int I = l.forward ? l.plane : g.n[0] - l.plane + 1;
         if (I < tu.lo[0] || I > tu.hi[0]) continue;
         const int JMIN = (int)(fmax(-1, 5) + 0.1), JMAX = (int)(fmin(g.n[1] + 1, g.n[1] - 5) + 0.1);
         const int KMIN = (int)(fmax(-1, 5) + 0.1), KMAX = (int)(fmin(g.n[2] + 1, g.n[2] - 5) + 0.1);
         const int jmin = (int)(fmax(tu.lo[1], JMIN) + 0.1), jmax = (int)(fmin(tu.hi[1], JMAX) + 0.1);
         const int kmin = (int)(fmax(tu.lo[2], KMIN) + 0.1), kmax = (int)(fmin(tu.hi[2], KMAX) + 0.1);
-        const double phase = 2 * M_PI * t + l.phi0;
         TRY(launch_laser(r->stream, Fy, Fz, g, l.dmap, I, JMIN, KMIN, KMAX, jmin, jmax, kmin, kmax,
-                         ampl * g.tau / g.h[0], cos(phase), sin(phase)));
+                         l.dtab, r->dstep, LASER_CAP));
     }
     return MPIC_OK;
 }
@@ -952,18 +1023,14 @@ static int laser_E(mpic_run *r, T *Ey, T *Ez, const Box &tu)
 {
     const Grid &g = r->g;
     for (const Laser &l : r->lasers) {
-        const double t = r->time + 0.5 * g.tau - l.T0 + l.delay;
-        const double ampl = exp(-pow(2 * t / l.dt, 2));
         const int I = l.forward ? l.plane - 1 : g.n[0] - l.plane + 1;
         if (I < tu.lo[0] || I > tu.hi[0]) continue;
         const int JMIN = (int)(fmax(-1, 5) + 0.1), JMAX = (int)(fmin(g.n[1] + 1, g.n[1] - 5) + 0.1);
         const int KMIN = (int)(fmax(-1, 5) + 0.1), KMAX = (int)(fmin(g.n[2] + 1, g.n[2] - 5) + 0.1);
         const int jmin = (int)(fmax(tu.lo[1], JMIN) + 0.1), jmax = (int)(fmin(tu.hi[1], JMAX) + 0.1);
         const int kmin = (int)(fmax(tu.lo[2], KMIN) + 0.1), kmax = (int)(fmin(tu.hi[2], KMAX) + 0.1);
-        const double phase = 2 * M_PI * t + l.phi0;
-        TRY(launch_laser(r->stream, Ey, Ez, g, l.dmap + 4 * l.nodes, I, JMIN, KMIN,
-                         KMAX, jmin, jmax, kmin, kmax, ampl * g.tau / g.h[0], cos(phase),
-                         sin(phase)));
+        TRY(launch_laser(r->stream, Ey, Ez, g, l.dmap + 4 * l.nodes, I, JMIN, KMIN, KMAX, jmin, jmax,
+                         kmin, kmax, l.dtab + 3 * LASER_CAP, r->dstep, LASER_CAP));
     }
     return MPIC_OK;
 }
@@ -973,6 +1040,7 @@ extern "C" int mpic_em_h_half(mpic_run_t *r)
     if (!r) return MPIC_ERR_ARG;
     TRY(check_stage(r, 0));
     MPIC_CHECK(hipSetDevice(r->dev));
+    TRY(laser_tables(r, 1));
     /* cap_cacheEH: cap H, walls on cap H, cap E, walls on cap E.  Sources
      * never reach the cap layer (checked in mpic_add_laser). */
     FCALL(launch_cap_H, r->g, r->updH);
@@ -1012,7 +1080,7 @@ extern "C" int mpic_push_deposit(mpic_run_t *r)
     TRY(check_stage(r, 1));
     MPIC_CHECK(hipSetDevice(r->dev));
     TRY(launch_zero(r->stream, r->jp.J[0], 3 * r->g.ncell));
-    MPIC_CHECK(hipMemsetAsync(r->dacc + 2, 0, 3 * sizeof(double), r->stream));
+    TRY(launch_zero(r->stream, r->dacc + 2, 3));   /* a kernel: the step may be captured */
     if (r->timed >= 0) TRY(record(r, 1));
     if (r->fp32) {
         TRY(launch_push32(r->stream, r->m32, r->f32, r->g, r->pc, r->qm, r->tv, r->tj, r->dacc + 2,
@@ -1095,7 +1163,9 @@ extern "C" int mpic_em_e_finish(mpic_run_t *r)
     else
         TRY(laser_E(r, r->f.E[1], r->f.E[2], dom));
     FCALL(launch_e_tauJ, r->g);
+    TRY(launch_tick(r->stream, r->dstep));
     r->time = r->time + r->g.tau;          /* parameter_setTime (Time + tau) */
+    ++r->step;
     r->stage = 0;
     return MPIC_OK;
 }
@@ -1144,10 +1214,9 @@ extern "C" int mpic_step(mpic_run_t *r, double *diag)
     return MPIC_OK;
 }
 
-extern "C" int mpic_advance(mpic_run_t *r, int nsteps)
+/* n steps with a timed event record per step (kernel timing). */
+static int advance_timed(mpic_run *r, int nsteps)
 {
-    if (!r || nsteps < 0) return MPIC_ERR_ARG;
-    MPIC_CHECK(hipSetDevice(r->dev));
     while ((int)r->evpool.size() < 5 * nsteps) {
         hipEvent_t e;
         MPIC_CHECK(hipEventCreate(&e));
@@ -1155,6 +1224,161 @@ extern "C" int mpic_advance(mpic_run_t *r, int nsteps)
     }
     for (int s = 0; s < nsteps; ++s) TRY(step_async(r, s));
     r->ev_steps = nsteps;
+    return MPIC_OK;
+}
+
+/* Host bookkeeping of a step that a replayed graph performs on the device. */
+static void host_step(mpic_run *r)
+{
+    if (r->tiled) swap_markers(r);
+    r->time = r->time + r->g.tau;
+    ++r->step;
+    r->stage = 0;
+}
+
+/* Captures one step from each marker-buffer parity into exec[0..1]: the
+ * kernel arguments of a step differ only by which buffer holds the sorted
+ * markers, so two graphs replay every step.  The host state is restored
+ * afterwards (capturing executes nothing). */
+static int capture(mpic_run *r)
+{
+    if (!r->tiled || !r->inited || r->stage != 0) {
+        mpic_set_error("graph capture needs an initialised, tiled run between steps");
+        return MPIC_ERR_STATE;
+    }
+    TRY(laser_tables(r, 2));
+    for (int k = 0; k < 2; ++k)
+        if (r->exec[k]) {
+            hipGraphExecDestroy(r->exec[k]);
+            r->exec[k] = nullptr;
+        }
+    const double time = r->time;
+    const long step = r->step;
+    const int cur = r->cur, parity = r->parity;
+    const Markers32 m32 = r->m32;
+    const Markers64 m = r->m;
+    r->exec_parity = r->parity;
+    int rc = MPIC_OK;
+    for (int k = 0; k < 2 && rc == MPIC_OK; ++k) {
+        hipGraph_t gr = nullptr;
+        if (hipStreamBeginCapture(r->stream, hipStreamCaptureModeRelaxed) != hipSuccess) {
+            mpic_set_error("hipStreamBeginCapture failed");
+            rc = MPIC_ERR_DEVICE;
+            break;
+        }
+        rc = step_async(r, -1);
+        const hipError_t e = hipStreamEndCapture(r->stream, &gr);
+        if (rc == MPIC_OK && e != hipSuccess) {
+            mpic_set_error("hipStreamEndCapture: %s", hipGetErrorString(e));
+            rc = MPIC_ERR_DEVICE;
+        }
+        if (rc == MPIC_OK && hipGraphInstantiate(&r->exec[k], gr, nullptr, nullptr, 0) != hipSuccess) {
+            mpic_set_error("hipGraphInstantiate failed");
+            rc = MPIC_ERR_DEVICE;
+        }
+        if (gr) hipGraphDestroy(gr);
+    }
+    r->time = time;
+    r->step = step;
+    r->cur = cur;
+    r->parity = parity;
+    r->m32 = m32;
+    r->m = m;
+    r->stage = 0;
+    r->graphs_ok = rc == MPIC_OK;
+    return rc;
+}
+
+static int replay(mpic_run *r)
+{
+    const int k = (r->parity == r->exec_parity) ? 0 : 1;
+    MPIC_CHECK(hipGraphLaunch(r->exec[k], r->stream));
+    host_step(r);
+    return MPIC_OK;
+}
+
+extern "C" int mpic_set_graphs(mpic_run_t *r, int on)
+{
+    if (!r) return MPIC_ERR_ARG;
+    if (on && !r->tiled) {
+        mpic_set_error("graph replay needs the sorted (tiled) push");
+        return MPIC_ERR_ARG;
+    }
+    r->use_graphs = on != 0;
+    return MPIC_OK;
+}
+
+extern "C" int mpic_advance(mpic_run_t *r, int nsteps)
+{
+    if (!r || nsteps < 0) return MPIC_ERR_ARG;
+    MPIC_CHECK(hipSetDevice(r->dev));
+    if (!r->use_graphs) return advance_timed(r, nsteps);
+    mpic_run_t *one[1] = {r};
+    return mpic_batch_advance(one, 1, nsteps);
+}
+
+extern "C" int mpic_batch_advance(mpic_run_t *const *runs, int nruns, int nsteps)
+{
+    if (!runs || nruns < 0 || nsteps < 0) return MPIC_ERR_ARG;
+    for (int q = 0; q < nruns; ++q) {
+        mpic_run *r = runs[q];
+        if (!r || !r->inited || !r->tiled) {
+            mpic_set_error("mpic_batch_advance: run %d is not an initialised tiled run", q);
+            return MPIC_ERR_ARG;
+        }
+        r->ev_steps = 0;
+    }
+    for (long left = nsteps; left > 0;) {
+        const long chunk = left < LASER_CAP - 2 ? left : LASER_CAP - 2;
+        for (int q = 0; q < nruns; ++q) {
+            mpic_run *r = runs[q];
+            MPIC_CHECK(hipSetDevice(r->dev));
+            TRY(laser_tables(r, chunk + 2));
+            if (!r->graphs_ok) TRY(capture(r));
+        }
+        /* step by step across the runs, each on its own stream */
+        int dev = -1;
+        for (long s = 0; s < chunk; ++s)
+            for (int q = 0; q < nruns; ++q) {
+                mpic_run *r = runs[q];
+                if (r->dev != dev) {
+                    MPIC_CHECK(hipSetDevice(r->dev));
+                    dev = r->dev;
+                }
+                TRY(replay(r));
+            }
+        left -= chunk;
+    }
+    return MPIC_OK;
+}
+
+extern "C" int mpic_autotune(mpic_run_t *r, int steps, int *chosen)
+{
+    if (!r || steps < 1) return MPIC_ERR_ARG;
+    MPIC_CHECK(hipSetDevice(r->dev));
+    if (!r->fp32 || !r->tiled) {
+        if (chosen) *chosen = r->push_blocks;
+        return MPIC_OK;
+    }
+    int best = r->tile_blocks;
+    double best_ms = 1e300;
+    for (int mult = 4; mult <= 64; mult *= 2) {
+        int nb = mult * r->ncu;
+        if (nb > r->tv.ntiles) nb = r->tv.ntiles;
+        r->tile_blocks = nb;
+        TRY(advance_timed(r, steps));
+        double p, d, rest;
+        TRY(mpic_last_timing(r, &p, &d, &rest));
+        if (p + d < best_ms) {
+            best_ms = p + d;
+            best = nb;
+        }
+        if (nb == r->tv.ntiles) break;
+    }
+    r->tile_blocks = best;
+    r->graphs_ok = 0;
+    r->ev_steps = 0;
+    if (chosen) *chosen = best;
     return MPIC_OK;
 }
 

@@ -134,3 +134,62 @@ def two_stream_box(n=32, seed=7):
     return dict(name="two-stream 32^3 periodic h=0.05, 8 ppc x 2 species",
                 n=nn, h=h, L=L, tau=courant_tau(h), bc=(BC_PERIODIC,) * 6, markers=M,
                 laser=None, ppc=16, plasma_cells=cells.shape[0])
+
+
+# markers per cell per species -> in-cell sub-lattice
+_SUB = {4: (2, 2, 1), 8: (2, 2, 2), 16: (4, 2, 2), 27: (3, 3, 3), 32: (4, 4, 2)}
+
+
+def foil_request(n, ppc, a0, density, thickness, seed=0, steps=100):
+    """One laser-foil scan point: grid n (h = lambda/20, Mur walls), a fully
+    ionised foil of `density` n_c and `thickness` lambda centred at 5/8 of
+    the box along x, electrons (vth 0.01 c) and cold protons at `ppc`
+    markers per cell each, a y-polarised Gaussian beam of amplitude a0
+    focused on the foil's front face and injected near the low-x wall."""
+    n = tuple(int(v) for v in n)
+    h = (0.05, 0.05, 0.05)
+    L = tuple(n[a] * h[a] for a in range(3))
+    rng = np.random.default_rng(seed)
+    xc = int(round(0.625 * n[0]))
+    half = max(1, int(round(thickness / h[0] / 2)))
+    i0, i1 = max(xc - half, 8), min(xc + half, n[0] - 3)
+    ii, jj, kk = np.meshgrid(np.arange(i0, i1), np.arange(2, n[1] - 2), np.arange(2, n[2] - 2),
+                             indexing="ij")
+    cells = np.stack([ii.ravel(), jj.ravel(), kk.ravel()], -1).astype(np.float64)
+    pos, p = _quiet_cells(cells, _SUB[ppc], h, rng)
+    N = pos.shape[0]
+    M = np.zeros(2 * N, dtype=MARKER_DTYPE)
+    w = density * RHO_CRIT / p
+    for s, (q, qm, vth) in enumerate(((-w, -1.0, 0.01), (w, QM_PROTON, 0.0))):
+        sl = slice(s * N, (s + 1) * N)
+        M["x"][sl], M["y"][sl], M["z"][sl] = pos[:, 0], pos[:, 1], pos[:, 2]
+        if vth > 0:
+            u = rng.normal(0.0, vth, size=(N, 3))
+            M["vx"][sl], M["vy"][sl], M["vz"][sl] = u[:, 0], u[:, 1], u[:, 2]
+        M["rho"][sl] = q
+        M["qDivM"][sl] = qm
+    M["id"] = np.arange(1, 2 * N + 1)
+    plane = max(6, n[0] // 8)
+    focus_x = i0 * h[0]
+    waist = 0.25 * min(L[1], L[2])
+    lp = laser_params(a0, plane, h, focus_x, 3.0, focus_x - plane * h[0] + 1.5)
+    lmap = gaussian_map(n, h, plane, a0, waist, (focus_x, L[1] / 2, L[2] / 2))
+    return dict(name="foil %dx%dx%d ppc %d a0 %.2f n %.1f d %.2f" % (n + (ppc, a0, density, thickness)),
+                n=n, h=h, L=L, tau=courant_tau(h), bc=(BC_OPEN,) * 6, markers=M,
+                laser=(lp, lmap), ppc=2 * p, plasma_cells=cells.shape[0], steps=int(steps))
+
+
+def scan_request(rng, large_fraction=0.0, steps=(500, 3000)):
+    """Parameters of one ensemble / serving request: grid uniformly from
+    {32^3, 64x32x32, 64^3, 128x64x64}, markers per cell from {4, 8, 16, 32},
+    a0 log-uniform in [0.5, 5], density uniform in [5, 50] n_c, thickness
+    uniform in [0.2, 1.0] lambda, steps uniform in `steps`; with probability
+    large_fraction a 256x128x128 run at 16 markers per cell."""
+    grids = [(32, 32, 32), (64, 32, 32), (64, 64, 64), (128, 64, 64)]
+    if rng.random() < large_fraction:
+        n, ppc = (256, 128, 128), 16
+    else:
+        n, ppc = grids[rng.integers(len(grids))], int(rng.choice([4, 8, 16, 32]))
+    return dict(n=n, ppc=ppc, a0=float(np.exp(rng.uniform(np.log(0.5), np.log(5.0)))),
+                density=float(rng.uniform(5, 50)), thickness=float(rng.uniform(0.2, 1.0)),
+                seed=int(rng.integers(1 << 30)), steps=int(rng.integers(steps[0], steps[1] + 1)))

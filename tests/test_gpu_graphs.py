@@ -1,0 +1,111 @@
+"""Graph replay, ensembles and the autotuner against the stage-by-stage
+path (which the parity tests pin to the oracle).
+
+  - replayed graphs in vacuum with the laser are bit-exact with the oracle
+    (the per-step laser table is evaluated with the same host arithmetic);
+  - replay with markers equals stage calls within the J summation order;
+  - runs of different sizes advanced together in one ensemble give the
+    results they give alone;
+  - the autotuner's choice of workgroup count leaves results unchanged.
+"""
+import numpy as np
+import pytest
+
+import mandor3_amd as ma
+from oracle_lib import load_fixture
+from test_gpu_parity import by_id, cells, oracle, random_fields, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def make(fx, precision, markers=True, E=None, H=None, graphs=False):
+    r = ma.Run(fx["n"], fx["h"], fx["L"], fx["tau"][0], fx["bc"], 0.0, precision=precision,
+               max_markers=max(fx["M0"].size, 1))
+    r.set_fields(fx["E0"] if E is None else E, fx["H0"] if H is None else H)
+    if markers:
+        r.load_markers(fx["M0"])
+    if "laser_params" in fx:
+        r.add_laser(fx["laser_params"], fx["laser_map"])
+    r.init()
+    if graphs:
+        r.set_graphs(True)
+    return r
+
+
+def test_replay_vacuum_laser_bit_exact():
+    fx = load_fixture("laser_mur")
+    E0, H0 = random_fields(fx, 11)
+    g = make(fx, ma.FP64, markers=False, E=E0, H=H0, graphs=True)
+    o = oracle(fx, markers=False, E=E0, H=H0)
+    for chunk in (1, 4, 7):
+        g.advance(chunk)
+        for _ in range(chunk):
+            o.step()
+        Eg, Hg = g.fields()
+        Eo, Ho = o.fields()
+        assert np.array_equal(Eg, Eo) and np.array_equal(Hg, Ho), chunk
+        assert g.time == o.time
+
+
+@pytest.mark.parametrize("prec", [ma.FP64, ma.FP32_SPLIT])
+@pytest.mark.parametrize("case", ["laser_mur", "mixed"])
+def test_replay_equals_stage_calls(case, prec):
+    fx = load_fixture(case)
+    a = make(fx, prec)
+    b = make(fx, prec, graphs=True)
+    for _ in range(13):
+        a.step(diag=False)
+    b.advance(5)
+    b.advance(8)
+    a.sync()
+    b.sync()
+    tol = 1e-12 if prec == ma.FP64 else 2e-5
+    Ea, Ha = a.fields()
+    Eb, Hb = b.fields()
+    assert rel_l2(Eb, Ea) <= tol and rel_l2(Hb, Ha) <= tol
+    Ma, Mb = by_id(a.markers()), by_id(b.markers())
+    assert np.array_equal(Ma["id"], Mb["id"])
+    if prec == ma.FP64:
+        assert np.array_equal(cells(Ma, fx), cells(Mb, fx))
+    assert np.max(np.abs(Ma["x"] - Mb["x"])) <= (1e-12 if prec == ma.FP64 else 1e-5) * fx["h"][0]
+    assert a.time == b.time
+
+
+@pytest.mark.parametrize("prec", [ma.FP64, ma.FP32_SPLIT])
+def test_ensemble_equals_alone(prec):
+    fxs = [load_fixture(c) for c in ("periodic", "laser_mur", "mixed")]
+    alone = [make(fx, prec) for fx in fxs]
+    mix = [make(fx, prec) for fx in fxs]
+    for r in alone:
+        for _ in range(9):
+            r.step(diag=False)
+        r.sync()
+    ma.batch_advance(mix, 9)
+    tol = 1e-12 if prec == ma.FP64 else 2e-5
+    for fx, a, b in zip(fxs, alone, mix):
+        b.sync()
+        Ea, Ha = a.fields()
+        Eb, Hb = b.fields()
+        assert rel_l2(Eb, Ea) <= tol and rel_l2(Hb, Ha) <= tol
+        Ma, Mb = by_id(a.markers()), by_id(b.markers())
+        assert np.array_equal(Ma["id"], Mb["id"])
+        assert b.time == a.time
+
+
+def test_autotune_is_result_neutral():
+    fx = load_fixture("laser_mur")
+    a = make(fx, ma.FP32_SPLIT)
+    b = make(fx, ma.FP32_SPLIT)
+    chosen = b.autotune(steps=2)
+    assert chosen > 0
+    done = int(round(b.time / fx["tau"][0]))
+    for _ in range(done + 4):
+        a.step(diag=False)
+    b.set_graphs(True)
+    b.advance(4)
+    a.sync()
+    b.sync()
+    Ea, Ha = a.fields()
+    Eb, Hb = b.fields()
+    assert rel_l2(Eb, Ea) <= 2e-5 and rel_l2(Hb, Ha) <= 2e-5
+    assert a.time == pytest.approx(b.time, rel=1e-14)
