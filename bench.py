@@ -34,7 +34,13 @@ def parse():
     p.add_argument("--precision", default="both", choices=["fp32", "fp64", "both"],
                    help="both: the FP32 split form gives 'value', the FP64 plain form "
                         "is reported beside it")
-    p.add_argument("--workload", default="laser-foil", choices=["laser-foil", "two-stream"])
+    p.add_argument("--workload", default="laser-foil",
+                   choices=["laser-foil", "two-stream", "ensemble", "serve"],
+                   help="ensemble: 64 laser-foil scan runs advanced together (graph replay); "
+                        "serve: Poisson stream of scan requests through the serving loop")
+    p.add_argument("--runs", type=int, default=64, help="ensemble size")
+    p.add_argument("--requests", type=int, default=48, help="serving: number of requests")
+    p.add_argument("--rate", type=float, default=4.0, help="serving: mean arrivals per second")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=12.0)
     return p.parse_args()
@@ -123,8 +129,78 @@ def cpu_baseline(wl, seconds):
                       "oracle, %.1f s, %s" % (steps, nm, el, cpu)}
 
 
+def run_ensemble(ma, workloads, precision, runs, steps, warmup, seed=64):
+    """The ensemble configuration: `runs` scan points drawn from `seed`
+    (grids 32^3 .. 128x64x64, 4-32 markers per cell, a0 0.5-5, 5-50 n_c,
+    0.2-1 lambda), each on its own stream, advanced together by replaying
+    their captured steps."""
+    rng = np.random.default_rng(seed)
+    prec = ma.FP64 if precision == "fp64" else ma.FP32_SPLIT
+    from mandor3_amd import serve
+    rs, markers = [], 0
+    for _ in range(runs):
+        st = workloads.foil_request(**workloads.scan_request(rng, 0.0, (steps, steps)))
+        r = serve.start_run(st, prec, graphs=True)
+        rs.append(r)
+        markers += st["markers"].size
+    ma.batch_advance(rs, warmup)
+    for r in rs:
+        r.sync()
+    t0 = time.perf_counter()
+    ma.batch_advance(rs, steps)
+    for r in rs:
+        r.sync()
+    wall = time.perf_counter() - t0
+    for r in rs:
+        r.close()
+    return {"pushes_per_s": markers * steps / wall, "ms_per_step": 1e3 * wall / steps,
+            "markers": int(markers), "runs": runs}
+
+
+def main_ensemble(a):
+    import mandor3_amd as ma
+    from mandor3_amd import workloads
+    res = {p: run_ensemble(ma, workloads, p, a.runs, a.steps, a.warmup) for p in ("fp32", "fp64")}
+    m = res["fp32"]
+    print(json.dumps({
+        "metric": "marker-pushes/s incl. deposit+FDTD, FP32 vs FP64 on 1 GPU; p50/p99 run latency",
+        "value": m["pushes_per_s"], "unit": "marker-pushes/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": m["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "ensemble of %d laser-foil scan runs (grids 32^3..128x64x64, "
+                               "4-32 ppc, a0 0.5-5, 5-50 n_c, 0.2-1 lambda), graph replay, "
+                               "one stream per run" % a.runs,
+                   "markers": m["markers"]},
+        "fp64_plain": {"value": res["fp64"]["pushes_per_s"], "ms_per_step": res["fp64"]["ms_per_step"]},
+        "fp32_over_fp64": m["pushes_per_s"] / res["fp64"]["pushes_per_s"]}))
+
+
+def main_serve(a):
+    from mandor3_amd import serve
+    import mandor3_amd as ma
+    out = {}
+    for p in ("fp32", "fp64"):
+        srv = serve.Server(ma.FP32_SPLIT if p == "fp32" else ma.FP64, 240.0, 25)
+        for req in serve.poisson_requests(a.requests, a.rate, 2024, 0.1, (a.steps, 6 * a.steps)):
+            srv.submit(req)
+        out[p] = srv.serve()
+    m = out["fp32"]
+    print(json.dumps({
+        "metric": "marker-pushes/s incl. deposit+FDTD, FP32 vs FP64 on 1 GPU; p50/p99 run latency",
+        "value": m["marker_pushes_per_s"], "unit": "marker-pushes/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": 0, "ms_per_step": None, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "serving: Poisson %.1f req/s, %d laser-foil scan requests "
+                               "(10%% at 256x128x128 16 ppc), 240 GB budget" % (a.rate, a.requests)},
+        "p50_s": m["p50_s"], "p99_s": m["p99_s"], "fp64_plain": out["fp64"]}))
+
+
 def main():
     a = parse()
+    if a.gpus == 1 and a.workload == "ensemble":
+        return main_ensemble(a)
+    if a.gpus == 1 and a.workload == "serve":
+        return main_serve(a)
     if a.gpus != 1:
         print("bench.py: this build runs on one GPU only (--gpus %d requested); "
               "multi-GPU sharding is design-only" % a.gpus, file=sys.stderr)

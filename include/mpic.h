@@ -100,6 +100,12 @@ int  mpic_add_laser(mpic_run_t *run, double A, double T0, double delay,
                     double dt, double phi0, int forward, int plane,
                     const double *map);
 
+/* Hard Gaussian spot source (source/core/em_gaussSpot.c:42-156): p = the 11
+ * values of .gaussSpot.cfg in file order {ampl, frequency [omega_0], x, y, z,
+ * width, duration, front offset, plateau, Ey flag, Ez flag}; Ey, Ez on the
+ * plane x/h + 1/2 are set at the end of every E step. */
+int  mpic_add_gauss_spot(mpic_run_t *run, const double *p);
+
 /* em_init + plasma_init (caps, Mur buffers, ghost nodes, walls applied to
  * the loaded markers). */
 int  mpic_init(mpic_run_t *run);

@@ -59,6 +59,7 @@ def lib():
             "mpic_markers_save": (I, [P, P, ctypes.POINTER(LG), I]),
             "mpic_marker_count": (LG, [P]),
             "mpic_add_laser": (I, [P, D, D, D, D, D, I, I, P]),
+            "mpic_add_gauss_spot": (I, [P, P]),
             "mpic_init": (I, [P]),
             "mpic_em_h_half": (I, [P]),
             "mpic_em_energy": (I, [P, P, P]),
@@ -208,6 +209,13 @@ class Run:
 
     def advance(self, nsteps):
         _check(lib().mpic_advance(self.handle, int(nsteps)), "mpic_advance")
+
+    def add_gauss_spot(self, params):
+        """Hard Gaussian spot: the 11 values of .gaussSpot.cfg in file order."""
+        p = np.ascontiguousarray(params, dtype=np.float64)
+        if p.size != 11:
+            raise ValueError("11 spot parameters expected")
+        _check(lib().mpic_add_gauss_spot(self.handle, _ptr(p)), "mpic_add_gauss_spot")
 
     def set_graphs(self, on=True):
         """Replay captured HIP graphs of the step in advance()."""

@@ -95,11 +95,10 @@ int launch_cap_E(hipStream_t s, const FieldsT<T> &f, const Grid &g, const BoxSet
  * (source/core/emCap_periodic.c:12-68, emCap_mirror.c:17-72,
  * emCap_Mur.c:55-175). */
 template <class T>
-__global__ void k_face(Grid g, FaceOp op)
+__device__ static inline void face_apply(const Grid &g, const FaceOp &op, long t)
 {
     T *X[3] = {(T *)op.X[0], (T *)op.X[1], (T *)op.X[2]};
     const T alpha = (T)op.alpha;
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     int nq = op.qhi - op.qlo + 1;
     long tot = (long)(op.phi - op.plo + 1) * nq;
     if (t >= tot) return;
@@ -170,6 +169,19 @@ __global__ void k_face(Grid g, FaceOp op)
 #undef AT
 }
 
+template <class T>
+__global__ void k_face(Grid g, FaceOp op)
+{
+    face_apply<T>(g, op, (long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+/* The two walls of one axis in one launch (blockIdx.y picks the wall). */
+template <class T>
+__global__ void k_face2(Grid g, FaceOp a, FaceOp b)
+{
+    face_apply<T>(g, blockIdx.y ? b : a, (long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
 int launch_face(hipStream_t s, const Grid &g, const FaceOp &op)
 {
     long n = (long)(op.phi - op.plo + 1) * (op.qhi - op.qlo + 1);
@@ -178,6 +190,24 @@ int launch_face(hipStream_t s, const Grid &g, const FaceOp &op)
         k_face<float><<<nblk(n), TPB, 0, s>>>(g, op);
     else
         k_face<double><<<nblk(n), TPB, 0, s>>>(g, op);
+    MPIC_CHECK(hipGetLastError());
+    return MPIC_OK;
+}
+
+/* a and b: the min and max wall of one axis with at least 8 cells, so that
+ * the node layers they touch ([0, 1] and [N-1, N+1]) are disjoint and one
+ * launch gives the result of the two in sequence. */
+int launch_face_pair(hipStream_t s, const Grid &g, const FaceOp &a, const FaceOp &b)
+{
+    const long na = (long)(a.phi - a.plo + 1) * (a.qhi - a.qlo + 1);
+    const long nb = (long)(b.phi - b.plo + 1) * (b.qhi - b.qlo + 1);
+    const long n = na > nb ? na : nb;
+    if (n <= 0) return MPIC_OK;
+    const dim3 grid((unsigned)nblk(n), 2);
+    if (a.fp32)
+        k_face2<float><<<grid, TPB, 0, s>>>(g, a, b);
+    else
+        k_face2<double><<<grid, TPB, 0, s>>>(g, a, b);
     MPIC_CHECK(hipGetLastError());
     return MPIC_OK;
 }
@@ -339,6 +369,33 @@ int launch_laser(hipStream_t s, T *Fy, T *Fz, const Grid &g, const double *map, 
     return MPIC_OK;
 }
 
+/* Hard Gaussian spot (source/core/em_gaussSpot.c:129-156): Ey, Ez on plane
+ * i over j, k in [-1, n+1] set to the step's amplitudes times the spatial
+ * profiles (both evaluated on the host). */
+template <class T>
+__global__ void k_spot(T *Ey, T *Ez, Grid g, int i, const double *py, const double *pz,
+                       const double *tab, const int *step, int cap)
+{
+    const int nk = g.n[2] + 3;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)(g.n[1] + 3) * nk) return;
+    const int j = -1 + (int)(t / nk), k = -1 + (int)(t % nk);
+    const double *e = tab + 2 * (*step % cap);
+    const long o = gidx(g, i, j, k);
+    Ey[o] = (T)(e[0] * py[t]);
+    Ez[o] = (T)(e[1] * pz[t]);
+}
+
+template <class T>
+int launch_spot(hipStream_t s, T *Ey, T *Ez, const Grid &g, int i, const double *py,
+                const double *pz, const double *tab, const int *step, int cap)
+{
+    const long n = (long)(g.n[1] + 3) * (g.n[2] + 3);
+    k_spot<T><<<nblk(n), TPB, 0, s>>>(Ey, Ez, g, i, py, pz, tab, step, cap);
+    MPIC_CHECK(hipGetLastError());
+    return MPIC_OK;
+}
+
 /* The run's step counter, advanced once per step on the stream. */
 __global__ void k_tick(int *step)
 {
@@ -354,9 +411,10 @@ int launch_tick(hipStream_t s, int *step)
 
 /* J boundary fold of one face b (order x-min, x-max, y-min, y-max, z-min,
  * z-max), source/core/plasma_currentWalls.c:132-227. */
-__global__ void k_jfold(JPtrs f, Grid g, int b, int bc)
+__global__ void k_jfold(JPtrs f, Grid g, int ea, int bc0, int bc1, int top0)
 {
-    const int ea = b >> 1, top = b & 1;
+    /* wall top0 + blockIdx.y of the axis: 0 min, 1 max */
+    const int top = top0 + (int)blockIdx.y, bc = top ? bc1 : bc0;
     const int ep = (ea == 0) ? 1 : 0, eq = (ea == 2) ? 1 : 2;
     const int np = g.n[ep] + 5, nq = g.n[eq] + 5;
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -398,13 +456,23 @@ __global__ void k_jfold(JPtrs f, Grid g, int b, int bc)
 #undef AT
 }
 
-int launch_jfold(hipStream_t s, const JPtrs &f, const Grid &g, int b, int bc)
+/* The J fold of both walls of axis ea (min first).  A periodic axis is one
+ * fold; two non-periodic walls of an axis with at least 8 cells touch the
+ * disjoint layers [-2, 2] and [N-2, N+2] and share one launch. */
+int launch_jfold(hipStream_t s, const JPtrs &f, const Grid &g, int ea, int bc0, int bc1)
 {
-    const int ea = b >> 1;
     const int ep = (ea == 0) ? 1 : 0, eq = (ea == 2) ? 1 : 2;
-    if (bc == MPIC_BC_PERIODIC && (b & 1)) return MPIC_OK;
-    long n = (long)(g.n[ep] + 5) * (g.n[eq] + 5);
-    k_jfold<<<nblk(n), TPB, 0, s>>>(f, g, b, bc);
+    const long n = (long)(g.n[ep] + 5) * (g.n[eq] + 5);
+    const unsigned nb = (unsigned)nblk(n);
+    if (bc0 == MPIC_BC_PERIODIC) {
+        k_jfold<<<dim3(nb, 1), TPB, 0, s>>>(f, g, ea, bc0, bc1, 0);
+    } else if (g.n[ea] >= 8) {
+        k_jfold<<<dim3(nb, 2), TPB, 0, s>>>(f, g, ea, bc0, bc1, 0);
+    } else {
+        k_jfold<<<dim3(nb, 1), TPB, 0, s>>>(f, g, ea, bc0, bc1, 0);
+        MPIC_CHECK(hipGetLastError());
+        k_jfold<<<dim3(nb, 1), TPB, 0, s>>>(f, g, ea, bc0, bc1, 1);
+    }
     MPIC_CHECK(hipGetLastError());
     return MPIC_OK;
 }
@@ -562,6 +630,8 @@ int launch_gauss(hipStream_t s, const FieldsT<T> &f, const double *rho1, const d
     template int launch_laser<T>(hipStream_t, T *, T *, const Grid &, const double *, int, int,  \
                                  int, int, int, int, int, int, const double *, const int *,     \
                                  int);                                                          \
+    template int launch_spot<T>(hipStream_t, T *, T *, const Grid &, int, const double *,        \
+                                const double *, const double *, const int *, int);              \
     template int launch_energy<T>(hipStream_t, const FieldsT<T> &, const Grid &, double *);      \
     template int launch_gauss<T>(hipStream_t, const FieldsT<T> &, const double *,                \
                                  const double *, const Grid &, double *);

@@ -117,6 +117,7 @@ void mpic_set_error(const char *fmt, ...);
 template <class T> int launch_cap_H(hipStream_t s, const FieldsT<T> &f, const Grid &g, const BoxSet &bs);
 template <class T> int launch_cap_E(hipStream_t s, const FieldsT<T> &f, const Grid &g, const BoxSet &bs);
 int launch_face(hipStream_t s, const Grid &g, const FaceOp &op);
+int launch_face_pair(hipStream_t s, const Grid &g, const FaceOp &a, const FaceOp &b);
 template <class T>
 int launch_h_update(hipStream_t s, const FieldsT<T> &f, const Grid &g, const BoxSet &flush,
                     int full);
@@ -129,7 +130,10 @@ int launch_laser(hipStream_t s, T *Fy, T *Fz, const Grid &g, const double *map, 
                  int JMIN, int KMIN, int KMAX, int jmin, int jmax, int kmin, int kmax,
                  const double *tab, const int *step, int cap);
 int launch_tick(hipStream_t s, int *step);
-int launch_jfold(hipStream_t s, const JPtrs &f, const Grid &g, int b, int bc);
+template <class T>
+int launch_spot(hipStream_t s, T *Ey, T *Ez, const Grid &g, int i, const double *py,
+                const double *pz, const double *tab, const int *step, int cap);
+int launch_jfold(hipStream_t s, const JPtrs &f, const Grid &g, int ea, int bc0, int bc1);
 template <class T> int launch_energy(hipStream_t s, const FieldsT<T> &f, const Grid &g, double *acc);
 int launch_zero(hipStream_t s, double *p, long n);
 

@@ -61,6 +61,15 @@ struct Laser {
     double *htab;             /* pinned staging of dtab */
 };
 
+/* Hard Gaussian spot source (source/core/em_gaussSpot.c:20-156). */
+struct Spot {
+    int     on, i;
+    double  omega, front, duration, plateau, Ey, Ez;
+    double *dprof;            /* device: profiles of Ey then Ez, (n1+3) x (n2+3) */
+    double *dtab;             /* device: per step {Ey amplitude, Ez amplitude} */
+    double *htab;             /* pinned staging of dtab */
+};
+
 struct CapBoxes {             /* one field's six caps */
     Box upd[6], fl[6];        /* original boxes */
     Box upd_s[6], fl_s[6];    /* after overlap removal */
@@ -107,6 +116,7 @@ struct mpic_run {
     double        time;
     std::vector<hipEvent_t> evpool;  /* 5 events per timed step */
     int          *dstep;      /* device step counter (indexes the laser tables) */
+    Spot          spot;
     long          step;       /* host copy of the step counter */
     long          tab_lo, tab_hi;  /* steps the laser tables hold */
     int           use_graphs; /* mpic_advance replays captured steps */
@@ -473,6 +483,9 @@ extern "C" int mpic_destroy(mpic_run_t *r)
         hipHostFree(l.htab);
     }
     hipFree(r->dstep);
+    hipFree(r->spot.dprof);
+    hipFree(r->spot.dtab);
+    if (r->spot.htab) hipHostFree(r->spot.htab);
     for (int k = 0; k < 2; ++k)
         if (r->exec[k]) hipGraphExecDestroy(r->exec[k]);
     if (r->stream) hipStreamDestroy(r->stream);
@@ -839,19 +852,78 @@ extern "C" int mpic_add_laser(mpic_run_t *r, double A, double T0, double delay, 
     return MPIC_OK;
 }
 
+/* gaussSpot_init (source/core/em_gaussSpot.c:42-120): p = the 11 values of
+ * .gaussSpot.cfg in file order.  The spatial profiles are evaluated here,
+ * on the host, as gaussSpot_forceE does per step. */
+extern "C" int mpic_add_gauss_spot(mpic_run_t *r, const double *p)
+{
+    if (!r || !p) return MPIC_ERR_ARG;
+    if (r->spot.on) {
+        mpic_set_error("mpic_add_gauss_spot: one spot per run");
+        return MPIC_ERR_STATE;
+    }
+    const Grid &g = r->g;
+    Spot &sp = r->spot;
+    const double ampl = p[0], x = p[2], y = p[3], z = p[4], width = p[5];
+    sp.omega = 2 * M_PI * p[1];
+    sp.duration = p[6];
+    sp.front = p[7];
+    sp.plateau = p[8];
+    const int ey = (int)p[9], ez = (int)p[10];
+    sp.Ey = ((ey > 0) - (ey < 0)) * ampl;
+    sp.Ez = ((ez > 0) - (ez < 0)) * ampl;
+    sp.i = (int)(x / g.h[0] + 0.5);
+    if (!(width > 0) || !(sp.duration > 0)) {
+        mpic_set_error("mpic_add_gauss_spot: width and duration must be positive");
+        return MPIC_ERR_ARG;
+    }
+    const int nj = g.n[1] + 3, nk = g.n[2] + 3;
+    std::vector<double> prof(2 * (size_t)nj * nk);
+    for (int j = -1; j <= g.n[1] + 1; ++j) {
+        const double dy_ = ((j - 0.5) * g.h[1] - y) / width, dy = (j * g.h[1] - y) / width;
+        for (int k = -1; k <= g.n[2] + 1; ++k) {
+            const double dz = (k * g.h[2] - z) / width, dz_ = ((k - 0.5) * g.h[2] - z) / width;
+            const size_t o = (size_t)(j + 1) * nk + (k + 1);
+            prof[o] = exp(-dy_ * dy_ - dz * dz);
+            prof[(size_t)nj * nk + o] = exp(-dy * dy - dz_ * dz_);
+        }
+    }
+    MPIC_CHECK(hipSetDevice(r->dev));
+    MPIC_CHECK(hipMalloc(&sp.dprof, sizeof(double) * prof.size()));
+    MPIC_CHECK(hipMemcpy(sp.dprof, prof.data(), sizeof(double) * prof.size(), hipMemcpyHostToDevice));
+    MPIC_CHECK(hipMalloc(&sp.dtab, sizeof(double) * 2 * LASER_CAP));
+    MPIC_CHECK(hipHostMalloc(&sp.htab, sizeof(double) * 2 * LASER_CAP, hipHostMallocDefault));
+    /* outside the E mesh [-1, n+1] the reference skips the spot */
+    sp.on = sp.i >= -1 && sp.i <= g.n[0] + 1;
+    r->tab_hi = r->tab_lo = 0;
+    r->graphs_ok = 0;
+    return MPIC_OK;
+}
+
 /* ---------------- stages ---------------- */
+/* The walls in list order; the min and max wall of one axis with at least
+ * 8 cells (adjacent in the list, disjoint node layers) share a launch. */
 static int walls_on(mpic_run *r, std::vector<FaceOp> &ops)
 {
-    for (FaceOp &op : ops) TRY(launch_face(r->stream, r->g, op));
+    for (size_t q = 0; q < ops.size(); ++q) {
+        const FaceOp &a = ops[q];
+        if (q + 1 < ops.size()) {
+            const FaceOp &b = ops[q + 1];
+            if (a.ea == b.ea && a.top == 0 && b.top == 1 && a.kind == b.kind &&
+                a.kind != FACE_PERIODIC_H && a.kind != FACE_PERIODIC_E && a.N >= 8) {
+                TRY(launch_face_pair(r->stream, r->g, a, b));
+                ++q;
+                continue;
+            }
+        }
+        TRY(launch_face(r->stream, r->g, a));
+    }
     return MPIC_OK;
 }
 
 template <class T>
 static int init_ghosts(mpic_run *r, T *const E[3], T *const H[3]);
 
-/* Counting sort of the current markers by cell key: with keys == 1 the
- * keys are computed here (loaded or untouched markers), else they come
- * from the push that just ran. */
 /* The sorted array becomes the current one. */
 static void swap_markers(mpic_run *r)
 {
@@ -864,6 +936,9 @@ static void swap_markers(mpic_run *r)
     r->parity ^= 1;
 }
 
+/* Counting sort of the current markers by cell key: with keys == 1 the
+ * keys are computed here (loaded or untouched markers), else they come
+ * from the push that just ran. */
 static int sort_pass(mpic_run *r, int keys)
 {
     const int other = r->cur ^ 1;
@@ -971,7 +1046,8 @@ static int check_stage(mpic_run *r, int want)
  * on the run's stream. */
 static int laser_tables(mpic_run *r, long n)
 {
-    if (r->lasers.empty() || (r->step >= r->tab_lo && r->step + n <= r->tab_hi)) return MPIC_OK;
+    if ((r->lasers.empty() && !r->spot.on) || (r->step >= r->tab_lo && r->step + n <= r->tab_hi))
+        return MPIC_OK;
     if (n > LASER_CAP) {
         mpic_set_error("laser tables hold %d steps", LASER_CAP);
         return MPIC_ERR_ARG;
@@ -994,6 +1070,30 @@ static int laser_tables(mpic_run *r, long n)
             time = time + g.tau;
         }
         MPIC_CHECK(hipMemcpyAsync(l.dtab, l.htab, sizeof(double) * 6 * LASER_CAP,
+                                  hipMemcpyHostToDevice, r->stream));
+    }
+    if (r->spot.on) {
+        /* gaussSpot_forceE at Time + tau: the time envelope and oscillation
+         * (source/core/em_gaussSpot.c:133-147) */
+        const Spot &sp = r->spot;
+        double time = r->time;
+        for (long q = 0; q < LASER_CAP; ++q) {
+            const double t = time + g.tau;
+            double tf = 1;
+            if (t < sp.front) {
+                const double l = (t - sp.front) / sp.duration;
+                tf = exp(-l * l);
+            }
+            if (t > sp.front + sp.plateau) {
+                const double l = (t - sp.front - sp.plateau) / sp.duration;
+                tf = exp(-l * l);
+            }
+            double *e = sp.htab + 2 * ((r->step + q) % LASER_CAP);
+            e[0] = sp.Ey * sin(sp.omega * t) * tf;
+            e[1] = sp.Ez * cos(sp.omega * t) * tf;
+            time = time + g.tau;
+        }
+        MPIC_CHECK(hipMemcpyAsync(sp.dtab, sp.htab, sizeof(double) * 2 * LASER_CAP,
                                   hipMemcpyHostToDevice, r->stream));
     }
     r->tab_lo = r->step;
@@ -1147,8 +1247,7 @@ extern "C" int mpic_j_finish(mpic_run_t *r)
     if (!r) return MPIC_ERR_ARG;
     MPIC_CHECK(hipSetDevice(r->dev));
     const int *bc = r->cfg.bc;
-    const int BCv[6] = {bc[0], bc[3], bc[1], bc[4], bc[2], bc[5]};
-    for (int b = 0; b < 6; ++b) TRY(launch_jfold(r->stream, r->jp, r->g, b, BCv[b]));
+    for (int ea = 0; ea < 3; ++ea) TRY(launch_jfold(r->stream, r->jp, r->g, ea, bc[ea], bc[3 + ea]));
     return MPIC_OK;
 }
 
@@ -1163,6 +1262,16 @@ extern "C" int mpic_em_e_finish(mpic_run_t *r)
     else
         TRY(laser_E(r, r->f.E[1], r->f.E[2], dom));
     FCALL(launch_e_tauJ, r->g);
+    if (r->spot.on) {
+        const Spot &sp = r->spot;
+        const long np = (long)(r->g.n[1] + 3) * (r->g.n[2] + 3);
+        if (r->fp32)
+            TRY(launch_spot(r->stream, r->f32.E[1], r->f32.E[2], r->g, sp.i, sp.dprof, sp.dprof + np,
+                            sp.dtab, r->dstep, LASER_CAP));
+        else
+            TRY(launch_spot(r->stream, r->f.E[1], r->f.E[2], r->g, sp.i, sp.dprof, sp.dprof + np,
+                            sp.dtab, r->dstep, LASER_CAP));
+    }
     TRY(launch_tick(r->stream, r->dstep));
     r->time = r->time + r->g.tau;          /* parameter_setTime (Time + tau) */
     ++r->step;

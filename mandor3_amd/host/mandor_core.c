@@ -4,10 +4,11 @@
  *   mandor_core [--fp32] [--device N]     (run in the directory of a setup)
  *
  * Reads run_mandor.cfg, binData/domain.bin, binData/units.bin, the start
- * record (record 0, or the last one with the continue flag) and the laser
- * sources in .EM_sources/laser, then runs the time step through the C ABI
- * of include/mpic.h, writing checkpoints, the energy history and honouring
- * tmp/stop.flag as source/core/main.c:195-490 does for one rank.
+ * record (record 0, or the last one with the continue flag), the laser
+ * sources in .EM_sources/laser and the hard spot of .gaussSpot.cfg, then
+ * runs the time step through the C ABI of include/mpic.h, writing
+ * checkpoints, the energy history and honouring tmp/stop.flag as
+ * source/core/main.c:195-490 does for one rank.
  * Tecplot and spectral dumps are not produced (a warning says so).
  */
 #define _GNU_SOURCE
@@ -172,6 +173,16 @@ int main(int argc, char **argv)
         GPU(mpic_add_laser(s.run, l.A, l.T0, l.delay, l.dt, l.phi0, l.forward, l.plane, l.map),
             "mpic_add_laser");
         mh_laser_free(&l);
+    }
+    FILE *spot = fopen(".gaussSpot.cfg", "rt");       /* source/core/em_gaussSpot.c:42-55 */
+    if (spot) {
+        fclose(spot);
+        mh_cfg_t sc;
+        HOST(mh_cfg_load(".gaussSpot.cfg", &sc), ".gaussSpot.cfg");
+        double p[11];
+        for (int q = 0; q < 11; ++q) HOST(mh_tag_double(&sc.tags[0], q, p + q), ".gaussSpot.cfg");
+        mh_cfg_free(&sc);
+        GPU(mpic_add_gauss_spot(s.run, p), "mpic_add_gauss_spot");
     }
     GPU(mpic_init(s.run), "mpic_init");
     printf("mandor_core: %s form, %d x %d x %d cells, %ld markers, start record %d, t = %.6f\n",

@@ -14,6 +14,7 @@
  *   boundary caps              source/core/em_caps.c:68-809
  *   periodic / mirror / Mur    source/core/emCap_{periodic,mirror,Mur}.c
  *   laser soft source          source/scpic/em_mirror.c:667-717
+ *   Gaussian hard spot         source/core/em_gaussSpot.c:42-156
  *   energies                   source/core/em.c:227-252, plasma.c:140-143
  *   CIC rho + Gauss residuals  source/core/plasma_rho.c:23-173,
  *                              source/core/plasma_gaussTest.c:110-149
@@ -127,6 +128,12 @@ typedef struct {           /* mirror_t subset used by core.out */
     double *mapE, *mapH;
 } laser_t;
 
+/* Hard Gaussian spot source, source/core/em_gaussSpot.c:20-30. */
+typedef struct {
+    int    on, i;
+    double ampl, omega, x, y, z, width, duration, front, plateau, Ey, Ez;
+} spot_t;
+
 struct opic_s {
     int    n[3];
     double h[3], L[3], tau, time;
@@ -140,6 +147,7 @@ struct opic_s {
     double Wx, Wy, Wz;
     int    nlaser;
     laser_t laser[4];
+    spot_t spot;
     int    inited;
 };
 
@@ -709,6 +717,59 @@ static void em_EStep_start(opic_t *s)
             }
 }
 
+/* gaussSpot_forceE, source/core/em_gaussSpot.c:129-156: sets Ey, Ez on the
+ * plane i over the whole E mesh to the Gaussian profile times the
+ * oscillation and the time envelope (hard source). */
+static void spot_force(opic_t *s, double t)
+{
+    const spot_t *g = &s->spot;
+    if (!g->on || g->i < -1 || g->i > s->n[0] + 1) return;
+    double tf = 1;
+    if (t < g->front) {
+        const double l = (t - g->front) / g->duration;
+        tf = exp(-l * l);
+    }
+    if (t > g->front + g->plateau) {
+        const double l = (t - g->front - g->plateau) / g->duration;
+        tf = exp(-l * l);
+    }
+    for (int j = -1; j <= s->n[1] + 1; j++) {
+        const double dy_ = ((j - 0.5) * s->h[1] - g->y) / g->width,
+                     dy = (j * s->h[1] - g->y) / g->width;
+        for (int k = -1; k <= s->n[2] + 1; k++) {
+            const double dz = (k * s->h[2] - g->z) / g->width,
+                         dz_ = ((k - 0.5) * s->h[2] - g->z) / g->width;
+            double *e = AT(&s->E, g->i, j, k);
+            e[1] = g->Ey * sin(g->omega * t) * tf * exp(-dy_ * dy_ - dz * dz);
+            e[2] = g->Ez * cos(g->omega * t) * tf * exp(-dy * dy - dz_ * dz_);
+        }
+    }
+}
+
+int opic_add_gauss_spot(opic_t *s, const double p[11])
+{
+    /* gaussSpot_init, source/core/em_gaussSpot.c:42-120: the values of
+     * .gaussSpot.cfg in file order */
+    spot_t *g = &s->spot;
+    g->ampl = p[0];
+    g->omega = 2 * M_PI * p[1];
+    g->x = p[2];
+    g->y = p[3];
+    g->z = p[4];
+    g->width = p[5];
+    g->duration = p[6];
+    g->front = p[7];
+    g->plateau = p[8];
+    const int ey = (int)p[9], ez = (int)p[10];
+    g->Ey = (ey > 0) - (ey < 0);
+    g->Ez = (ez > 0) - (ez < 0);
+    g->i = (int)(g->x / s->h[0] + 0.5);
+    g->Ey *= g->ampl;
+    g->Ez *= g->ampl;
+    g->on = 1;
+    return 0;
+}
+
 /* em_EStep_finish, source/core/em.c:188-221 */
 static void em_EStep_finish(opic_t *s)
 {
@@ -724,6 +785,7 @@ static void em_EStep_finish(opic_t *s)
                 e[1] -= s->tau * jj[1];
                 e[2] -= s->tau * jj[2];
             }
+    spot_force(s, s->time + s->tau);                /* source/core/em.c:216 */
 }
 
 /* em_energy, source/core/em.c:226-252 */

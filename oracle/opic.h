@@ -50,6 +50,11 @@ int     opic_add_laser(opic_t *s, double A, double T0, double delay,
                        double dt, double phi0, int forward, int plane,
                        const double *map);
 
+/* Hard Gaussian spot source: the 11 values of .gaussSpot.cfg in file order
+ * {ampl, frequency [omega_0], x, y, z, width, duration, front offset,
+ * plateau, Ey flag, Ez flag} (source/core/em_gaussSpot.c:42-55). */
+int     opic_add_gauss_spot(opic_t *s, const double p[11]);
+
 int     opic_init(opic_t *s);              /* em_init + plasma_init */
 /* One main-loop iteration, source/core/main.c:353-476.
  * diag (may be NULL) receives {Time, W_E, W_M, WTx, WTy, WTz} as passed to

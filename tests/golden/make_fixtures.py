@@ -92,6 +92,15 @@ CASES = {
                       box=((0.8, 0.4, 0.4), (0.1, 0.3, 0.3)))],
         laser=dict(a0=1.5, waist=0.25, focus=(0.7, 0.4, 0.4), dt_fs=4.0,
                    T0_fs=2.0, phi0=0.3)),
+    # Mur box with the hard Gaussian spot source (source/core/em_gaussSpot.c:
+    # 129-156) on the plane i = 8, both polarisations, and electrons.
+    "spot": dict(
+        n=(16, 12, 12), L=(0.8, 0.6, 0.6), bc=(2, 2, 2, 2, 2, 2),
+        species=[dict(sub=(1, 1, 1, 1), rho=-0.5, qm=-1.0,
+                      u=(0, 0, 0), du=(0.4, 0.4, 0.4))],
+        laser=None,
+        spot=dict(ampl=2.0, frequency=1.0, x=0.4, y=0.31, z=0.28, width=0.1,
+                  duration=0.12, front=0.15, plateau=0.05, Ey=1, Ez=-1)),
     # x periodic, y mirror, z Mur: every wall kind and their edges.
     "mixed": dict(
         n=(10, 12, 9), L=(0.5, 0.6, 0.405), bc=(0, 1, 2, 0, 1, 2),
@@ -226,6 +235,15 @@ def make_case(name, cfg, workroot):
             [-2 * math.pi * -lz["a0"], T0, xf - 5 * d7[0], dt,
              lz["phi0"], 1.0, 5.0])
         out["laser_map"] = lmap.reshape(-1)
+    sp = cfg.get("spot")
+    if sp is not None:
+        # .gaussSpot.cfg in code units, read by gaussSpot_init
+        # (source/core/em_gaussSpot.c:42-55)
+        keys = ("ampl", "frequency", "x", "y", "z", "width", "duration", "front", "plateau")
+        lines = ["@ %.17e   %s" % (sp[k], k) for k in keys]
+        lines += ["@ %d   Ey" % sp["Ey"], "@ %d   Ez" % sp["Ez"]]
+        open(os.path.join(wd, ".gaussSpot.cfg"), "w").write("\n".join(lines) + "\n")
+        out["spot_params"] = np.array([sp[k] for k in keys] + [sp["Ey"], sp["Ez"]], dtype=float)
     # setup.out seeds rand() from the clock (source/misc_partition.c:255):
     # the Gauss build runs on a copy of the same record 0.
     wg = os.path.join(workroot, name + "_gauss")

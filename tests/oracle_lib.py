@@ -44,6 +44,7 @@ def lib():
         L.opic_marker_count.argtypes = [P]
         L.opic_get_markers.argtypes = [P, P, I]
         L.opic_add_laser.argtypes = [P, D, D, D, D, D, I, I, P]
+        L.opic_add_gauss_spot.argtypes = [P, P]
         L.opic_init.argtypes = [P]
         L.opic_step.argtypes = [P, P, P]
         L.opic_time.restype = D
@@ -114,6 +115,11 @@ class Oracle:
         assert lib().opic_add_laser(self.s, A, T0, delay, dt, phi0, int(fwd),
                                     int(plane), _ptr(lmap)) == 0
 
+    def add_gauss_spot(self, params):
+        p = np.ascontiguousarray(params, dtype=np.float64)
+        assert p.size == 11
+        assert lib().opic_add_gauss_spot(self.s, _ptr(p)) == 0
+
     def init(self):
         rc = lib().opic_init(self.s)
         assert rc == 0, rc
@@ -159,5 +165,7 @@ def oracle_from_fixture(fx):
     o.set_markers(fx["M0"])
     if "laser_params" in fx:
         o.add_laser(fx["laser_params"], fx["laser_map"])
+    if "spot_params" in fx:
+        o.add_gauss_spot(fx["spot_params"])
     o.init()
     return o

@@ -19,7 +19,7 @@ from oracle_lib import MARKER_DTYPE, Oracle, load_fixture, oracle_from_fixture
 
 pytestmark = pytest.mark.gpu
 
-CASES = ["periodic", "laser_mur", "mixed"]
+CASES = ["periodic", "laser_mur", "mixed", "spot"]
 
 
 def run32(fx, markers=True, E=None, H=None):
@@ -30,6 +30,8 @@ def run32(fx, markers=True, E=None, H=None):
         r.load_markers(fx["M0"])
     if "laser_params" in fx:
         r.add_laser(fx["laser_params"], fx["laser_map"])
+    if "spot_params" in fx:
+        r.add_gauss_spot(fx["spot_params"])
     r.init()
     return r
 
@@ -62,6 +64,8 @@ def test_fp32_vacuum_fields(case):
     o.set_markers(np.zeros(0, MARKER_DTYPE))
     if "laser_params" in fx:
         o.add_laser(fx["laser_params"], fx["laser_map"])
+    if "spot_params" in fx:
+        o.add_gauss_spot(fx["spot_params"])
     o.init()
     worst = 0.0
     for _ in range(12):

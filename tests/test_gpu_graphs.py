@@ -26,6 +26,8 @@ def make(fx, precision, markers=True, E=None, H=None, graphs=False):
         r.load_markers(fx["M0"])
     if "laser_params" in fx:
         r.add_laser(fx["laser_params"], fx["laser_map"])
+    if "spot_params" in fx:
+        r.add_gauss_spot(fx["spot_params"])
     r.init()
     if graphs:
         r.set_graphs(True)
@@ -48,7 +50,7 @@ def test_replay_vacuum_laser_bit_exact():
 
 
 @pytest.mark.parametrize("prec", [ma.FP64, ma.FP32_SPLIT])
-@pytest.mark.parametrize("case", ["laser_mur", "mixed"])
+@pytest.mark.parametrize("case", ["laser_mur", "mixed", "spot"])
 def test_replay_equals_stage_calls(case, prec):
     fx = load_fixture(case)
     a = make(fx, prec)

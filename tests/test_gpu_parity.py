@@ -15,7 +15,7 @@ from oracle_lib import MARKER_DTYPE, Oracle, load_fixture, oracle_from_fixture
 
 pytestmark = pytest.mark.gpu
 
-CASES = ["periodic", "laser_mur", "mixed"]
+CASES = ["periodic", "laser_mur", "mixed", "spot"]
 
 
 def gpu_from_fixture(fx, markers=True, E=None, H=None):
@@ -26,6 +26,8 @@ def gpu_from_fixture(fx, markers=True, E=None, H=None):
         r.load_markers(fx["M0"])
     if "laser_params" in fx:
         r.add_laser(fx["laser_params"], fx["laser_map"])
+    if "spot_params" in fx:
+        r.add_gauss_spot(fx["spot_params"])
     r.init()
     return r
 
@@ -36,6 +38,8 @@ def oracle(fx, markers=True, E=None, H=None):
     o.set_markers(fx["M0"] if markers else np.zeros(0, MARKER_DTYPE))
     if "laser_params" in fx:
         o.add_laser(fx["laser_params"], fx["laser_map"])
+    if "spot_params" in fx:
+        o.add_gauss_spot(fx["spot_params"])
     o.init()
     return o
 

@@ -14,7 +14,7 @@ import pytest
 from oracle_lib import (field_digest, load_fixture, marker_digest,
                         oracle_from_fixture)
 
-CASES = ["periodic", "laser_mur", "mixed"]
+CASES = ["periodic", "laser_mur", "mixed", "spot"]
 
 
 @pytest.mark.parametrize("case", CASES)
