@@ -24,6 +24,17 @@
 
 static inline unsigned nblk(long n) { return (unsigned)((n + TPB - 1) / TPB); }
 
+/* Volume kernels: BX x BY threads along k and j, one i plane per
+ * blockIdx.z, so that no thread divides to find its node. */
+#define BX 64
+#define BY 4
+static inline dim3 grid3(const Grid &g, int extra, int)
+{
+    /* n + extra nodes per axis */
+    return dim3((unsigned)((g.n[2] + extra + BX - 1) / BX), (unsigned)((g.n[1] + extra + BY - 1) / BY),
+                (unsigned)(g.n[0] + extra));
+}
+
 __device__ static inline bool box_decode(const BoxSet &bs, long t, int &i, int &j, int &k)
 {
     if (t >= bs.off[bs.n]) return false;
@@ -230,12 +241,10 @@ __device__ static inline int in_boxes(const BoxSet &bs, int i, int j, int k)
 template <class T>
 __global__ void k_h_update(FieldsT<T> f, Grid g, BoxSet fl, int full, T c1, T c2, T c3)
 {
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int NX = g.n[0] + 2, NY = g.n[1] + 2, NZ = g.n[2] + 2;
-    if (t >= (long)NX * NY * NZ) return;
-    int k = (int)(t % NZ);
-    long r = t / NZ;
-    int j = (int)(r % NY), i = (int)(r / NY);
+    /* nodes [0, n+1]^3: k along x threads, j along y, one i plane per z */
+    const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int i = blockIdx.z;
+    if (k > g.n[2] + 1 || j > g.n[1] + 1) return;
     long o = gidx(g, i, j, k);
     if (i >= 3 && i <= g.n[0] - 2 && j >= 3 && j <= g.n[1] - 2 && k >= 3 && k <= g.n[2] - 2) {
         long oi = gidx(g, i - 1, j, k), oj = gidx(g, i, j - 1, k), ok = gidx(g, i, j, k - 1);
@@ -261,9 +270,9 @@ template <class T>
 int launch_h_update(hipStream_t s, const FieldsT<T> &f, const Grid &g, const BoxSet &flush,
                     int full)
 {
-    long n = (long)(g.n[0] + 2) * (g.n[1] + 2) * (g.n[2] + 2);
-    k_h_update<T><<<nblk(n), TPB, 0, s>>>(f, g, flush, full, (T)(0.5 * g.tau / g.h[0]),
-                                           (T)(0.5 * g.tau / g.h[1]), (T)(0.5 * g.tau / g.h[2]));
+    k_h_update<T><<<grid3(g, 2, 0), dim3(BX, BY), 0, s>>>(f, g, flush, full, (T)(0.5 * g.tau / g.h[0]),
+                                                        (T)(0.5 * g.tau / g.h[1]),
+                                                        (T)(0.5 * g.tau / g.h[2]));
     MPIC_CHECK(hipGetLastError());
     return MPIC_OK;
 }
@@ -274,12 +283,9 @@ template <class T>
 __global__ void k_e_update(FieldsT<T> f, Grid g, BoxSet fl, int interior, int do_flush, T c1,
                            T c2, T c3)
 {
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int NX = g.n[0] + 2, NY = g.n[1] + 2, NZ = g.n[2] + 2;
-    if (t >= (long)NX * NY * NZ) return;
-    int k = (int)(t % NZ);
-    long r = t / NZ;
-    int j = (int)(r % NY), i = (int)(r / NY);
+    const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int i = blockIdx.z;
+    if (k > g.n[2] + 1 || j > g.n[1] + 1) return;
     long o = gidx(g, i, j, k);
     if (i >= 2 && i <= g.n[0] - 2 && j >= 2 && j <= g.n[1] - 2 && k >= 2 && k <= g.n[2] - 2) {
         if (!interior) return;
@@ -300,9 +306,9 @@ template <class T>
 int launch_e_update(hipStream_t s, const FieldsT<T> &f, const Grid &g, const BoxSet &flush,
                     int interior, int do_flush)
 {
-    long n = (long)(g.n[0] + 2) * (g.n[1] + 2) * (g.n[2] + 2);
-    k_e_update<T><<<nblk(n), TPB, 0, s>>>(f, g, flush, interior, do_flush, (T)(g.tau / g.h[0]),
-                                           (T)(g.tau / g.h[1]), (T)(g.tau / g.h[2]));
+    k_e_update<T><<<grid3(g, 2, 0), dim3(BX, BY), 0, s>>>(f, g, flush, interior, do_flush,
+                                                        (T)(g.tau / g.h[0]), (T)(g.tau / g.h[1]),
+                                                        (T)(g.tau / g.h[2]));
     MPIC_CHECK(hipGetLastError());
     return MPIC_OK;
 }
@@ -311,12 +317,10 @@ int launch_e_update(hipStream_t s, const FieldsT<T> &f, const Grid &g, const Box
 template <class T>
 __global__ void k_e_tauJ(FieldsT<T> f, Grid g)
 {
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int NX = g.n[0] + 3, NY = g.n[1] + 3, NZ = g.n[2] + 3;
-    if (t >= (long)NX * NY * NZ) return;
-    int k = (int)(t % NZ) - 1;
-    long r = t / NZ;
-    int j = (int)(r % NY) - 1, i = (int)(r / NY) - 1;
+    /* nodes [-1, n+1]^3 */
+    const int k = (int)(blockIdx.x * blockDim.x + threadIdx.x) - 1;
+    const int j = (int)(blockIdx.y * blockDim.y + threadIdx.y) - 1, i = (int)blockIdx.z - 1;
+    if (k > g.n[2] + 1 || j > g.n[1] + 1) return;
     long o = gidx(g, i, j, k);
     const double tau = g.tau;
     /* FP32 fields: the FP64 J sum is applied in FP64 and rounded once */
@@ -328,8 +332,7 @@ __global__ void k_e_tauJ(FieldsT<T> f, Grid g)
 template <class T>
 int launch_e_tauJ(hipStream_t s, const FieldsT<T> &f, const Grid &g)
 {
-    long n = (long)(g.n[0] + 3) * (g.n[1] + 3) * (g.n[2] + 3);
-    k_e_tauJ<T><<<nblk(n), TPB, 0, s>>>(f, g);
+    k_e_tauJ<T><<<grid3(g, 3, 0), dim3(BX, BY), 0, s>>>(f, g);
     MPIC_CHECK(hipGetLastError());
     return MPIC_OK;
 }
