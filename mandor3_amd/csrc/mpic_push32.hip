@@ -73,23 +73,26 @@ __device__ static inline void seg_values(float x1, float y1, float z1, float x2,
     s.v[11] = az * (tx * ty + xy);
 }
 
-#define LJ(lj, a, b, c, comp) ((lj) + 3 * (((a) * TJN + (b)) * TJN + (c)) + (comp))
+/* Offset of node (+da,+db,+dc), component comp, from a node of the LDS J block. */
+#define LJO(da, db, dc, comp) (3 * (((da) * TJN + (db)) * TJN + (dc)) + (comp))
 
-/* Adds a segment of local cell (a,b,c) (LDS block coordinates) to LDS. */
+/* Adds a segment of local cell (a,b,c) (LDS block coordinates) to LDS:
+ * one address, the twelve nodes as constant offsets of it. */
 __device__ static inline void seg_to_lds(float *lj, int a, int b, int c, const Seg12 &s)
 {
-    atomicAdd(LJ(lj, a + 1, b, c, 0), s.v[0]);
-    atomicAdd(LJ(lj, a + 1, b + 1, c, 0), s.v[1]);
-    atomicAdd(LJ(lj, a + 1, b, c + 1, 0), s.v[2]);
-    atomicAdd(LJ(lj, a + 1, b + 1, c + 1, 0), s.v[3]);
-    atomicAdd(LJ(lj, a, b + 1, c, 1), s.v[4]);
-    atomicAdd(LJ(lj, a + 1, b + 1, c, 1), s.v[5]);
-    atomicAdd(LJ(lj, a, b + 1, c + 1, 1), s.v[6]);
-    atomicAdd(LJ(lj, a + 1, b + 1, c + 1, 1), s.v[7]);
-    atomicAdd(LJ(lj, a, b, c + 1, 2), s.v[8]);
-    atomicAdd(LJ(lj, a + 1, b, c + 1, 2), s.v[9]);
-    atomicAdd(LJ(lj, a, b + 1, c + 1, 2), s.v[10]);
-    atomicAdd(LJ(lj, a + 1, b + 1, c + 1, 2), s.v[11]);
+    float *p = lj + __mul24(__mul24(__mul24(a, TJN) + b, TJN) + c, 3);
+    atomicAdd(p + LJO(1, 0, 0, 0), s.v[0]);
+    atomicAdd(p + LJO(1, 1, 0, 0), s.v[1]);
+    atomicAdd(p + LJO(1, 0, 1, 0), s.v[2]);
+    atomicAdd(p + LJO(1, 1, 1, 0), s.v[3]);
+    atomicAdd(p + LJO(0, 1, 0, 1), s.v[4]);
+    atomicAdd(p + LJO(1, 1, 0, 1), s.v[5]);
+    atomicAdd(p + LJO(0, 1, 1, 1), s.v[6]);
+    atomicAdd(p + LJO(1, 1, 1, 1), s.v[7]);
+    atomicAdd(p + LJO(0, 0, 1, 2), s.v[8]);
+    atomicAdd(p + LJO(1, 0, 1, 2), s.v[9]);
+    atomicAdd(p + LJO(0, 1, 1, 2), s.v[10]);
+    atomicAdd(p + LJO(1, 1, 1, 2), s.v[11]);
 }
 
 /* Same, straight to the FP64 J in HBM (segment outside the LDS block). */
@@ -236,6 +239,15 @@ __device__ static inline float lgather(const float *F, int a, float sx, int b, f
                  sy * (sx1 * F[o + dj + 1] + sx * F[o + di + dj + 1]));
 }
 
+/* The tile being pushed: its index, its neighbour strides along i and j
+ * (in tiles) and whether a wall lies within reach of its markers (a tile
+ * one tile or more away from every face is out of reach: markers move less
+ * than a cell per step). */
+struct TilePos {
+    int  t, s0, s1;
+    bool edge;
+};
+
 /* Push of one marker of tile-local cell (li,lj,lk) (cell (ci,cj,ck)):
  * gather from the staged fields, Boris, move, deposit of every segment but
  * the first into the tile's LDS J (or HBM beyond it), walls.  Returns the
@@ -246,7 +258,8 @@ __device__ static inline int push32_marker(MarkerRegs &out, const MarkerRegs &r,
                                            const float *lf, const float *lqm, float *lj, int o0,
                                            int o1, int o2,
                                            const F32View &v, const Push32Consts &c,
-                                           const TileView &tv, Seg12 &first, float &w0,
+                                           const TileView &tv, const TilePos &tp,
+                                           Seg12 &first, float &w0,
                                            float &w1, float &w2)
 {
     const Grid &g = v.g;
@@ -339,17 +352,28 @@ __device__ static inline int push32_marker(MarkerRegs &out, const MarkerRegs &r,
     if (nx >= 1.0f) { nx = 0.0f; ++ni; }
     if (ny >= 1.0f) { ny = 0.0f; ++nj; }
     if (nz >= 1.0f) { nz = 0.0f; ++nk; }
-    const int alive = wall_lo(c.bc[0], ni, nx, ux, g.n[0]) && wall_lo(c.bc[1], nj, ny, uy, g.n[1]) &&
-                      wall_lo(c.bc[2], nk, nz, uz, g.n[2]) && wall_hi(c.bc[3], ni, nx, ux, g.n[0]) &&
-                      wall_hi(c.bc[4], nj, ny, uy, g.n[1]) && wall_hi(c.bc[5], nk, nz, uz, g.n[2]);
-    if (!alive) return -1;
+    int key;
+    if (tp.edge) {
+        const int alive =
+            wall_lo(c.bc[0], ni, nx, ux, g.n[0]) && wall_lo(c.bc[1], nj, ny, uy, g.n[1]) &&
+            wall_lo(c.bc[2], nk, nz, uz, g.n[2]) && wall_hi(c.bc[3], ni, nx, ux, g.n[0]) &&
+            wall_hi(c.bc[4], nj, ny, uy, g.n[1]) && wall_hi(c.bc[5], nk, nz, uz, g.n[2]);
+        if (!alive) return -1;
+        key = cell_key_ijk(tv, g, ni, nj, nk);
+    } else {
+        /* no wall within reach: the new cell is within one cell of the
+         * tile, so its key follows from the tile's neighbour strides */
+        const int ui = li + ni - ci, uj = lj_ + nj - cj, uk = lk + nk - ck;
+        const int nt = tp.t + __mul24(ui >> 2, tp.s0) + __mul24(uj >> 2, tp.s1) + (uk >> 2);
+        key = nt * TILE_CELLS + (((ui & 3) * TILE + (uj & 3)) * TILE + (uk & 3));
+    }
     out.ox = nx;
     out.oy = ny;
     out.oz = nz;
     out.ux = ux;
     out.uy = uy;
     out.uz = uz;
-    return (sp << SPECIES_SHIFT) | cell_key_ijk(tv, g, ni, nj, nk);
+    return (sp << SPECIES_SHIFT) | key;
 }
 
 /* DPP move of a float across lanes: lanes whose source lies outside the
@@ -448,6 +472,12 @@ k_push32_tiles(Markers32 m, F32View v, Push32Consts c, TileView tv, float *tj, d
         tile_coords(tv, t, t0, t1, t2);
         const int c0 = TILE * t0, c1 = TILE * t1, c2 = TILE * t2;
         const int o0 = c0 - 1, o1 = c1 - 1, o2 = c2 - 1;
+        TilePos tp;
+        tp.t = t;
+        tp.s0 = tv.nt[1] * tv.nt[2];
+        tp.s1 = tv.nt[2];
+        tp.edge = !(c0 >= TILE && c0 + 2 * TILE <= g.n[0] && c1 >= TILE && c1 + 2 * TILE <= g.n[1] &&
+                    c2 >= TILE && c2 + 2 * TILE <= g.n[2]);
         __syncthreads();                  /* previous tile's LDS reads are done */
         for (int q = threadIdx.x; q < TJB; q += TW) lj[q] = 0.0f;
         /* nodes [c, c + TFN) per axis; those past the last ghost layer (a
@@ -486,7 +516,7 @@ k_push32_tiles(Markers32 m, F32View v, Push32Consts c, TileView tv, float *tj, d
                 cj = c1 + lj_;
                 ck = c2 + lk;
                 newword = push32_marker(pend.r, cur, li, lj_, lk, ci, cj, ck, lf, lqm, lj, o0, o1, o2,
-                                        v, c, tv, first, w0, w1, w2);
+                                        v, c, tv, tp, first, w0, w1, w2);
             }
             pend.valid = in;
             pend.p = p;

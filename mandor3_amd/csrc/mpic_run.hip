@@ -305,6 +305,17 @@ extern "C" int mpic_create(const mpic_config_t *cfg, mpic_run_t **out)
         mpic_set_error("mpic_create: unknown precision %d", cfg->precision);
         return MPIC_ERR_ARG;
     }
+    if (cfg->precision == MPIC_FP32_SPLIT) {
+        /* the split form's cell word keeps 29 bits for the cell and its
+         * kernels address marker arrays with 32-bit byte offsets */
+        const long cells = (long)((cfg->n[0] + 3) / 4) * ((cfg->n[1] + 3) / 4) *
+                           ((cfg->n[2] + 3) / 4) * 64;
+        if (cells > (1L << 29) || cfg->max_markers >= (1L << 30)) {
+            mpic_set_error("mpic_create: the FP32 split form holds at most 2^29 cells (in 4^3 "
+                           "tiles) and 2^30 - 1 markers per run");
+            return MPIC_ERR_ARG;
+        }
+    }
     mpic_run *r = new mpic_run();
     r->cfg = *cfg;
     r->dev = cfg->device;
