@@ -22,7 +22,7 @@ COMMON = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",
           "-munsafe-fp-atomics",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-I" + os.path.join(ROOT, "include"), "-I" + CSRC,
-          "-Wno-unused-result"]
+          "-Wno-unused-result", "-Wno-unused-value"]
 # per-file contraction mode: the FP64 plain path must not fuse mul+add
 # (bit-exact with the reference); the FP32 split push may
 FP_MODE = {"default": ["-ffp-contract=off"],

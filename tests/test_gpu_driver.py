@@ -217,4 +217,6 @@ def test_core_mur_laser_matches_oracle(tmp_path, host):  # noqa: F811
         o.step()
     rec = records.record(wd, 1, n)
     compare(rec, o, n, d["h"])
-    assert np.abs(rec["E"]).max() > 1.0          # the laser is in the box
+    # the laser is in the box: the plasma alone (|rho| = 3, u ~ 0.02) leaves
+    # fields of order 1e-5 after 20 steps, the pulse's leading edge ~0.5
+    assert np.abs(rec["E"]).max() > 1e-2

@@ -47,7 +47,8 @@ def test_results_same_alone_and_in_the_mix(prec, tol):
 def test_admission_respects_budget():
     reqs = requests(5, steps=(10, 20), seed=9)
     states = [workloads.foil_request(**r.params) for r in reqs]
-    biggest = max(serve.footprint(s, ma.FP32_SPLIT) for s in states)
+    sizes = [serve.footprint(s, ma.FP32_SPLIT) for s in states]
+    biggest = max(sizes)
     budget = 1.5 * biggest / serve.GB
     srv = serve.Server(ma.FP32_SPLIT, budget_gb=budget, slice_steps=5)
     for r in reqs:
@@ -56,7 +57,7 @@ def test_admission_respects_budget():
     assert rep["requests"] == 5 and rep["rejected"] == 0
     assert rep["peak_resident_gb"] <= budget + 1e-9
     # an oversize request is answered with an error instead of waiting
-    big = serve.Request(99, dict(reqs[0].params), 0.0)
+    big = serve.Request(99, dict(reqs[int(np.argmax(sizes))].params), 0.0)
     srv2 = serve.Server(ma.FP32_SPLIT, budget_gb=0.5 * biggest / serve.GB)
     srv2.submit(big)
     rep2 = srv2.serve()
