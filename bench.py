@@ -43,7 +43,24 @@ def parse():
     p.add_argument("--rate", type=float, default=4.0, help="serving: mean arrivals per second")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=12.0)
+    p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                   help="first run a plain device copy of MIB MiB three times: the reference "
+                        "point that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings")
     return p.parse_args()
+
+
+def copy_calibration(mib):
+    """A plain copy of known size (read mib MiB, write mib MiB) as the
+    reference dispatch of a counter run."""
+    import torch
+    x = torch.ones(mib * (1 << 20) // 4, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print("copy calibration: %d MiB read + %d MiB written per dispatch" % (mib, mib),
+          file=sys.stderr, flush=True)
+    del x, y
 
 
 def push_bytes(n_markers, plasma_cells, precision):
@@ -143,6 +160,8 @@ def run_ensemble(ma, workloads, precision, runs, steps, warmup, seed=64):
         r = serve.start_run(st, prec, graphs=True)
         rs.append(r)
         markers += st["markers"].size
+    print("ensemble %s: %d runs, %d markers set up" % (precision, runs, markers), file=sys.stderr,
+          flush=True)
     ma.batch_advance(rs, warmup)
     for r in rs:
         r.sync()
@@ -184,6 +203,7 @@ def main_serve(a):
         for req in serve.poisson_requests(a.requests, a.rate, 2024, 0.1, (a.steps, 6 * a.steps)):
             srv.submit(req)
         out[p] = srv.serve()
+        print("serve %s: %d requests done" % (p, out[p]["requests"]), file=sys.stderr, flush=True)
     m = out["fp32"]
     print(json.dumps({
         "metric": "marker-pushes/s incl. deposit+FDTD, FP32 vs FP64 on 1 GPU; p50/p99 run latency",
@@ -209,6 +229,8 @@ def main():
     from mandor3_amd import workloads
     if ma.device_count() < 1:
         raise SystemExit("no HIP device visible")
+    if a.copy_calibration > 0:
+        copy_calibration(a.copy_calibration)
     wl = workloads.laser_foil() if a.workload == "laser-foil" else workloads.two_stream_box()
     main_prec = "fp64" if a.precision == "fp64" else "fp32"
     res = {main_prec: run_gpu(ma, wl, main_prec, a.steps, a.warmup)}

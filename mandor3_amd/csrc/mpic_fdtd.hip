@@ -24,15 +24,30 @@
 
 static inline unsigned nblk(long n) { return (unsigned)((n + TPB - 1) / TPB); }
 
-/* Volume kernels: BX x BY threads along k and j, one i plane per
- * blockIdx.z, so that no thread divides to find its node. */
-#define BX 64
-#define BY 4
-static inline dim3 grid3(const Grid &g, int extra, int)
+/* Volume kernels over n + extra nodes per axis: a run of workgroups covers
+ * one i plane, the flat thread index runs over the plane's (j, k) nodes, so
+ * rows of any length fill whole wavefronts and a node costs one 32-bit
+ * division.  Planes are dealt to XCDs in contiguous ranges, so the planes
+ * an update reads beside its own are mostly in the same L2. */
+static inline unsigned grid_planes(const Grid &g, int extra)
 {
-    /* n + extra nodes per axis */
-    return dim3((unsigned)((g.n[2] + extra + BX - 1) / BX), (unsigned)((g.n[1] + extra + BY - 1) / BY),
-                (unsigned)(g.n[0] + extra));
+    const long plane = (long)(g.n[1] + extra) * (g.n[2] + extra);
+    return (unsigned)((plane + TPB - 1) / TPB) * (unsigned)(g.n[0] + extra);
+}
+
+__device__ static inline bool plane_node(const Grid &g, int extra, int &i, int &j, int &k)
+{
+    const unsigned nz = (unsigned)(g.n[2] + extra), plane = (unsigned)(g.n[1] + extra) * nz;
+    const unsigned bpp = (plane + TPB - 1) / TPB;            /* workgroups per plane */
+    const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
+    const unsigned ip = lb / bpp;
+    const unsigned t = (lb - ip * bpp) * TPB + threadIdx.x;
+    if (t >= plane) return false;
+    const unsigned jj = t / nz;
+    i = (int)ip;
+    j = (int)jj;
+    k = (int)(t - jj * nz);
+    return true;
 }
 
 __device__ static inline bool box_decode(const BoxSet &bs, long t, int &i, int &j, int &k)
@@ -241,13 +256,12 @@ __device__ static inline int in_boxes(const BoxSet &bs, int i, int j, int k)
 template <class T>
 __global__ void k_h_update(FieldsT<T> f, Grid g, BoxSet fl, int full, T c1, T c2, T c3)
 {
-    /* nodes [0, n+1]^3: k along x threads, j along y, one i plane per z */
-    const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y;
-    const int i = blockIdx.z;
-    if (k > g.n[2] + 1 || j > g.n[1] + 1) return;
+    /* nodes [0, n+1]^3: one i plane per blockIdx.y */
+    int i, j, k;
+    if (!plane_node(g, 2, i, j, k)) return;
     long o = gidx(g, i, j, k);
     if (i >= 3 && i <= g.n[0] - 2 && j >= 3 && j <= g.n[1] - 2 && k >= 3 && k <= g.n[2] - 2) {
-        long oi = gidx(g, i - 1, j, k), oj = gidx(g, i, j - 1, k), ok = gidx(g, i, j, k - 1);
+        const long oi = o - (long)g.GY * g.GZ, oj = o - g.GZ, ok = o - 1;
         const T *Ex = f.E[0], *Ey = f.E[1], *Ez = f.E[2];
         const T ex = Ex[o], ey = Ey[o], ez = Ez[o];
         f.H[0][o] += c3 * (ey - Ey[ok]) - c2 * (ez - Ez[oj]);
@@ -270,7 +284,7 @@ template <class T>
 int launch_h_update(hipStream_t s, const FieldsT<T> &f, const Grid &g, const BoxSet &flush,
                     int full)
 {
-    k_h_update<T><<<grid3(g, 2, 0), dim3(BX, BY), 0, s>>>(f, g, flush, full, (T)(0.5 * g.tau / g.h[0]),
+    k_h_update<T><<<grid_planes(g, 2), TPB, 0, s>>>(f, g, flush, full, (T)(0.5 * g.tau / g.h[0]),
                                                         (T)(0.5 * g.tau / g.h[1]),
                                                         (T)(0.5 * g.tau / g.h[2]));
     MPIC_CHECK(hipGetLastError());
@@ -283,13 +297,12 @@ template <class T>
 __global__ void k_e_update(FieldsT<T> f, Grid g, BoxSet fl, int interior, int do_flush, T c1,
                            T c2, T c3)
 {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y;
-    const int i = blockIdx.z;
-    if (k > g.n[2] + 1 || j > g.n[1] + 1) return;
+    int i, j, k;
+    if (!plane_node(g, 2, i, j, k)) return;
     long o = gidx(g, i, j, k);
     if (i >= 2 && i <= g.n[0] - 2 && j >= 2 && j <= g.n[1] - 2 && k >= 2 && k <= g.n[2] - 2) {
         if (!interior) return;
-        long oi = gidx(g, i + 1, j, k), oj = gidx(g, i, j + 1, k), ok = gidx(g, i, j, k + 1);
+        const long oi = o + (long)g.GY * g.GZ, oj = o + g.GZ, ok = o + 1;
         const T *Hx = f.H[0], *Hy = f.H[1], *Hz = f.H[2];
         const T hx = Hx[o], hy = Hy[o], hz = Hz[o];
         f.E[0][o] -= c3 * (Hy[ok] - hy) - c2 * (Hz[oj] - hz);
@@ -306,7 +319,7 @@ template <class T>
 int launch_e_update(hipStream_t s, const FieldsT<T> &f, const Grid &g, const BoxSet &flush,
                     int interior, int do_flush)
 {
-    k_e_update<T><<<grid3(g, 2, 0), dim3(BX, BY), 0, s>>>(f, g, flush, interior, do_flush,
+    k_e_update<T><<<grid_planes(g, 2), TPB, 0, s>>>(f, g, flush, interior, do_flush,
                                                         (T)(g.tau / g.h[0]), (T)(g.tau / g.h[1]),
                                                         (T)(g.tau / g.h[2]));
     MPIC_CHECK(hipGetLastError());
@@ -318,9 +331,11 @@ template <class T>
 __global__ void k_e_tauJ(FieldsT<T> f, Grid g)
 {
     /* nodes [-1, n+1]^3 */
-    const int k = (int)(blockIdx.x * blockDim.x + threadIdx.x) - 1;
-    const int j = (int)(blockIdx.y * blockDim.y + threadIdx.y) - 1, i = (int)blockIdx.z - 1;
-    if (k > g.n[2] + 1 || j > g.n[1] + 1) return;
+    int i, j, k;
+    if (!plane_node(g, 3, i, j, k)) return;
+    --i;
+    --j;
+    --k;
     long o = gidx(g, i, j, k);
     const double tau = g.tau;
     /* FP32 fields: the FP64 J sum is applied in FP64 and rounded once */
@@ -332,7 +347,7 @@ __global__ void k_e_tauJ(FieldsT<T> f, Grid g)
 template <class T>
 int launch_e_tauJ(hipStream_t s, const FieldsT<T> &f, const Grid &g)
 {
-    k_e_tauJ<T><<<grid3(g, 3, 0), dim3(BX, BY), 0, s>>>(f, g);
+    k_e_tauJ<T><<<grid_planes(g, 3), TPB, 0, s>>>(f, g);
     MPIC_CHECK(hipGetLastError());
     return MPIC_OK;
 }

@@ -22,6 +22,16 @@
 #include "mpic.h"
 
 #define MPIC_GHOST 2
+#define MPIC_XCDS 8          /* MI355X: 8 XCDs, workgroups dispatched round-robin */
+
+/* Workgroup L of 'total' as the hardware places it (XCD L % 8, its
+ * (L / 8)-th there) mapped to a logical index such that every XCD takes one
+ * contiguous range: neighbouring work then shares the XCD's L2. */
+__device__ static inline unsigned xcd_block(unsigned L, unsigned total)
+{
+    const unsigned x = L % MPIC_XCDS, q = total / MPIC_XCDS, r = total % MPIC_XCDS;
+    return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + L / MPIC_XCDS;
+}
 
 struct Grid {
     int    n[3];

@@ -13,7 +13,6 @@
 #include "mpic_internal.h"
 
 #define TILE_CELLS 64
-#define MPIC_XCDS 8          /* MI355X: 8 XCDs, workgroups dispatched round-robin */
 
 struct TileView {
     int  nt[3];              /* tiles per axis */
