@@ -50,17 +50,28 @@ def parse():
 
 
 def copy_calibration(mib):
-    """A plain copy of known size (read mib MiB, write mib MiB) as the
-    reference dispatch of a counter run."""
-    import torch
-    x = torch.ones(mib * (1 << 20) // 4, dtype=torch.float32, device="cuda")
-    y = torch.empty_like(x)
+    """A plain device-to-device copy of known size (read mib MiB, write mib
+    MiB; the runtime's blit kernel) as the reference dispatch of a counter
+    run.  It goes through the HIP runtime that libmpic.so itself is linked
+    against, so that one runtime serves the whole process."""
+    import ctypes
+    import mandor3_amd as ma
+    ma.lib()                                   # libamdhip64 is loaded with libmpic.so
+    hip = ctypes.CDLL("libamdhip64.so.7")
+    n = ctypes.c_size_t(mib << 20)
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(src), n) or hip.hipMalloc(ctypes.byref(dst), n):
+        raise SystemExit("copy calibration: hipMalloc failed")
+    ok = hip.hipMemset(src, 0, n) == 0
     for _ in range(3):
-        y.copy_(x)
-    torch.cuda.synchronize()
+        ok = ok and hip.hipMemcpy(dst, src, n, 3) == 0      # hipMemcpyDeviceToDevice
+    ok = ok and hip.hipDeviceSynchronize() == 0
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    if not ok:
+        raise SystemExit("copy calibration: HIP call failed")
     print("copy calibration: %d MiB read + %d MiB written per dispatch" % (mib, mib),
           file=sys.stderr, flush=True)
-    del x, y
 
 
 def push_bytes(n_markers, plasma_cells, precision):
