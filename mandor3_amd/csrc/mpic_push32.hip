@@ -448,13 +448,13 @@ __device__ static inline void write_back(const Markers32 &m, const TileView &tv,
     if (q.ckey >= 0) atomicAdd(tv.count + q.ckey, q.clen);
 }
 
-/* Tiled push: one wavefront per active tile (4x4x4 cells), taken slot by
- * slot with stride = grid size; with at least as many wavefronts as active
- * tiles each takes one and the hardware dispatcher balances uneven tiles,
- * while the other wavefronts of the CU hide each tile's start-up latency.
- * The tile's E/H nodes are staged in LDS, its J is accumulated in an FP32
- * LDS block and stored, without atomics, as the tile's block in 'tj'; the
- * next batch of markers is loaded before the current one is pushed. */
+/* Tiled push: one wavefront per work item, an even share of one active
+ * tile's (4x4x4 cells) markers, so that tiles of very different occupancy
+ * still keep every wavefront slot busy to the end.  A workgroup starts with
+ * item blockIdx.x and claims the next from a queue while it pushes.  The
+ * tile's E/H nodes are staged in LDS, the item's J is accumulated in an
+ * FP32 LDS block and stored, without atomics, as the item's block in 'tj';
+ * the next batch of markers is loaded before the current one is pushed. */
 __global__ void __launch_bounds__(TW) __attribute__((amdgpu_waves_per_eu(4)))
 k_push32_tiles(Markers32 m, F32View v, Push32Consts c, TileView tv, float *tj, double *Wacc)
 {
@@ -464,10 +464,13 @@ k_push32_tiles(Markers32 m, F32View v, Push32Consts c, TileView tv, float *tj, d
     if (threadIdx.x < 4) lqm[threadIdx.x] = c.qm[threadIdx.x];   /* ordered by the loop's barrier */
     const Grid &g = v.g;
     const int lane = threadIdx.x & 63;
-    const int nact = *tv.nactive;
+    const int nitems = *tv.nitems;
     float w0 = 0, w1 = 0, w2 = 0;         /* per-thread kinetic energy partials */
-    for (int sl = blockIdx.x; sl < nact; sl += gridDim.x) {
+    int it = blockIdx.x;                  /* first item: the workgroup's own; then the queue */
+    while (it < nitems) {
+        const int sl = tv.item[it];
         const int t = tv.active[sl];
+        const int u = it - tv.istart[sl], nu = tv.istart[sl + 1] - tv.istart[sl];
         int t0, t1, t2;
         tile_coords(tv, t, t0, t1, t2);
         const int c0 = TILE * t0, c1 = TILE * t1, c2 = TILE * t2;
@@ -489,13 +492,19 @@ k_push32_tiles(Markers32 m, F32View v, Push32Consts c, TileView tv, float *tj, d
             const float *F = comp < 3 ? v.E[comp] : v.H[comp - 3];
             lf[q] = (i <= g.n[0] + 2 && j <= g.n[1] + 2 && k <= g.n[2] + 2) ? F[gidx(g, i, j, k)] : 0.0f;
         }
-        const unsigned beg = tv.off[t * TILE_CELLS], end = tv.off[(t + 1) * TILE_CELLS];
+        /* item u of nu: an even share of the tile's markers */
+        const unsigned tb = tv.off[t * TILE_CELLS], cnt = tv.off[(t + 1) * TILE_CELLS] - tb;
+        const unsigned beg = tb + (unsigned)((unsigned long long)u * cnt / nu),
+                       end = tb + (unsigned)((unsigned long long)(u + 1) * cnt / nu);
         MarkerRegs cur;
         load_marker(m, beg + threadIdx.x, end - 1, cur);
         /* everything outstanding completes here, so that the loop's waits
          * only ever concern the prefetch of the batch in flight */
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
+        /* the next item, claimed now so that its latency hides behind this one */
+        int claim = 0;
+        if (lane == 0) claim = atomicAdd(tv.queue, 1);
         Pending pend;
         pend.valid = false;
         pend.ckey = -1;
@@ -539,8 +548,9 @@ k_push32_tiles(Markers32 m, F32View v, Push32Consts c, TileView tv, float *tj, d
         write_back(m, tv, pend);
         __syncthreads();
         /* the tile's J block, stored whole (coalesced, no atomics) */
-        float *dst = tj + (long)sl * TJB;
+        float *dst = tj + (long)it * TJB;
         for (int q = threadIdx.x; q < TJB; q += TW) dst[q] = lj[q];
+        it = (int)gridDim.x + __shfl(claim, 0, 64);
     }
     double d0 = w0, d1 = w1, d2 = w2;     /* wave sums in FP64 */
     for (int d = 32; d > 0; d >>= 1) {
@@ -564,13 +574,15 @@ __device__ static inline void covering(int i, int nt, int &lo, int &hi)
     hi = hi > nt - 1 ? nt - 1 : hi;
 }
 
-/* J += sum of the tile blocks covering each node, in FP64.  Every node of
- * an active tile's block is written once, by the first active tile (in tile
- * order) whose block holds it; the sum runs over the covering tiles in tile
- * order, so the result does not depend on scheduling. */
+/* J += sum of the item blocks of the tiles covering each node, in FP64.
+ * Every node of an active tile's block is written once, by the first active
+ * tile (in tile order) whose block holds it; the sum runs over the covering
+ * tiles in tile order and over each tile's items in order, so the result
+ * does not depend on scheduling. */
 __global__ void __launch_bounds__(TPB) k_jgather32(F32View v, TileView tv, const float *tj)
 {
     __shared__ int nslot[27];            /* slots of the 3x3x3 tiles around t, -1 empty */
+    __shared__ int nfirst[27], nitem[27];/* their first item and item count */
     const Grid &g = v.g;
     const int nact = *tv.nactive;
     for (int sl = blockIdx.x; sl < nact; sl += gridDim.x) {
@@ -583,7 +595,10 @@ __global__ void __launch_bounds__(TPB) k_jgather32(F32View v, TileView tv, const
                       uz = t2 - 1 + threadIdx.x % 3;
             const bool in = ux >= 0 && uy >= 0 && uz >= 0 && ux < tv.nt[0] && uy < tv.nt[1] &&
                             uz < tv.nt[2];
-            nslot[threadIdx.x] = in ? tv.slot[(ux * tv.nt[1] + uy) * tv.nt[2] + uz] : -1;
+            const int su = in ? tv.slot[(ux * tv.nt[1] + uy) * tv.nt[2] + uz] : -1;
+            nslot[threadIdx.x] = su;
+            nfirst[threadIdx.x] = su >= 0 ? tv.istart[su] : 0;
+            nitem[threadIdx.x] = su >= 0 ? tv.istart[su + 1] - tv.istart[su] : 0;
         }
         __syncthreads();
         for (int q = threadIdx.x; q < TJB; q += TPB) {
@@ -608,10 +623,10 @@ __global__ void __launch_bounds__(TPB) k_jgather32(F32View v, TileView tv, const
             for (int ux = xl; ux <= xh; ++ux)
                 for (int uy = yl; uy <= yh; ++uy)
                     for (int uz = zl; uz <= zh; ++uz) {
-                        const int su = nslot[((ux - t0 + 1) * 3 + (uy - t1 + 1)) * 3 + (uz - t2 + 1)];
-                        if (su < 0) continue;
+                        const int nb = ((ux - t0 + 1) * 3 + (uy - t1 + 1)) * 3 + (uz - t2 + 1);
                         const int a = i - (TILE * ux - 1), b = j - (TILE * uy - 1), cz = k - (TILE * uz - 1);
-                        sum += (double)tj[(long)su * TJB + ((a * TJN + b) * TJN + cz) * 3 + comp];
+                        const float *blk = tj + (long)nfirst[nb] * TJB + ((a * TJN + b) * TJN + cz) * 3 + comp;
+                        for (int w = 0; w < nitem[nb]; ++w) sum += (double)blk[(long)w * TJB];
                     }
             v.J[comp][gidx(g, i, j, k)] += sum;
         }

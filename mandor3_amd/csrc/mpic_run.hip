@@ -98,7 +98,8 @@ struct mpic_run {
     int          *blocksum;
     int           parity;     /* tv.nmark[parity] = markers in the current order */
     int           push_blocks;  /* FP64: resident workgroups of the persistent push */
-    int           tile_blocks;  /* FP32: workgroups of the tile push (one per active tile) */
+    int           tile_blocks;  /* FP32: workgroups of the tile push (work items taken in turn) */
+    long          items_cap;    /* work items (and FP32 J blocks) the buffers hold */
     int           ncu;          /* compute units of the device */
     float        *tj;           /* FP32: per-tile J blocks, [ntiles][TJB] */
     CapBoxes      cH, cE;
@@ -270,16 +271,33 @@ static size_t field_bytes(const int n[3], int fp32)
     return (fp32 ? sizeof(float) : sizeof(double)) * 12 * cells + sizeof(double) * 3 * cells;
 }
 
+/* Markers per work item of the FP32 push (the tile's markers are split
+ * evenly into items of at most this many); MPIC_ITEM_MARKERS overrides. */
+static int push32_item_markers(void)
+{
+    const char *e = getenv("MPIC_ITEM_MARKERS");
+    const int v = e ? atoi(e) : 0;
+    return v >= 64 ? v : 1024;
+}
+
+/* Work items of a run: at most one per tile plus one per chm markers. */
+static long items_capacity(int ntiles, long cap, int chm)
+{
+    return (long)ntiles + (chm > 0 ? cap / chm : 0) + 1;
+}
+
 extern "C" int mpic_footprint(const mpic_config_t *cfg, size_t *bytes)
 {
     if (!cfg || !bytes) return MPIC_ERR_ARG;
     const size_t cells = (size_t)(cfg->n[0] + 5) * (cfg->n[1] + 5) * (cfg->n[2] + 5);
     const size_t tiles = (size_t)((cfg->n[0] + 3) / 4) * ((cfg->n[1] + 3) / 4) * ((cfg->n[2] + 3) / 4);
     const size_t mk = (cfg->precision == MPIC_FP32_SPLIT) ? (4 + 7 * 4 + 4) : (6 * 8 + 2 * 4 + 4 + 4);
-    *bytes = field_bytes(cfg->n, cfg->precision == MPIC_FP32_SPLIT) + 2 * sizeof(double) * cells +
-             2 * (size_t)cfg->max_markers * mk + tiles * 64 * 3 * sizeof(int) +
-             tiles * 4 * sizeof(int) +
-             (cfg->precision == MPIC_FP32_SPLIT ? tiles * TJB * sizeof(float) : 0);
+    const int fp32 = cfg->precision == MPIC_FP32_SPLIT;
+    const long cap = cfg->max_markers > 0 ? cfg->max_markers : 1;
+    const size_t items = (size_t)items_capacity((int)tiles, cap, fp32 ? push32_item_markers() : 0);
+    *bytes = field_bytes(cfg->n, fp32) + 2 * sizeof(double) * cells +
+             2 * (size_t)cap * mk + tiles * 64 * 3 * sizeof(int) + tiles * 6 * sizeof(int) +
+             items * sizeof(int) + (fp32 ? items * TJB * sizeof(float) : 0);
     return MPIC_OK;
 }
 
@@ -405,14 +423,19 @@ extern "C" int mpic_create(const mpic_config_t *cfg, mpic_run_t **out)
     MPIC_CHECK(hipMalloc(&tv.key, sizeof(int) * (r->fp32 ? 1 : r->cap)));
     MPIC_CHECK(hipMalloc(&tv.active, sizeof(int) * tv.ntiles));
     MPIC_CHECK(hipMalloc(&tv.slot, sizeof(int) * tv.ntiles));
-    if (r->fp32) MPIC_CHECK(hipMalloc(&r->tj, sizeof(float) * TJB * (size_t)tv.ntiles));
+    tv.chm = r->fp32 ? push32_item_markers() : 0;
+    r->items_cap = items_capacity(tv.ntiles, r->cap, tv.chm);
+    MPIC_CHECK(hipMalloc(&tv.item, sizeof(int) * r->items_cap));
+    MPIC_CHECK(hipMalloc(&tv.istart, sizeof(int) * ((size_t)tv.ntiles + 1)));
+    if (r->fp32) MPIC_CHECK(hipMalloc(&r->tj, sizeof(float) * TJB * (size_t)r->items_cap));
     MPIC_CHECK(hipMalloc(&tv.nactive, sizeof(int) * (4 + MPIC_XCDS)));
     tv.queue = tv.nactive + 1;
     tv.nmark = tv.nactive + 1 + MPIC_XCDS;
+    tv.nitems = tv.nactive + 3 + MPIC_XCDS;
     MPIC_CHECK(hipMemsetAsync(tv.nactive, 0, sizeof(int) * (4 + MPIC_XCDS), r->stream));
-    /* scan scratch: block sums (2 per block + totals) and 2 ints per tile */
+    /* scan scratch: block sums (3 per block + totals) and 3 ints per tile */
     MPIC_CHECK(hipMalloc(&r->blocksum,
-                         sizeof(int) * (2 * ((tv.ntiles + 1023) / 1024 + 1) + 2 * (long)tv.ntiles)));
+                         sizeof(int) * (3 * ((tv.ntiles + 1023) / 1024 + 1) + 3 * (long)tv.ntiles)));
     r->parity = 0;
     {
         int ncu = 256;
@@ -421,9 +444,9 @@ extern "C" int mpic_create(const mpic_config_t *cfg, mpic_run_t **out)
         /* exactly the resident workgroups of the persistent push: extra ones
          * only arrive once the tile queues are drained */
         r->push_blocks = ncu * push64_blocks_per_cu();
-        /* enough workgroups for one active tile each in most runs; more
-         * active tiles are taken in turn (stride = grid size) */
-        r->tile_blocks = 16 * ncu < tv.ntiles ? 16 * ncu : tv.ntiles;
+        /* the resident wavefronts of the tile push (16 per CU); work items
+         * beyond them are taken from a queue as wavefronts finish */
+        r->tile_blocks = 16 * ncu < r->items_cap ? 16 * ncu : (int)r->items_cap;
         r->ncu = ncu;
         const char *e = getenv("MPIC_PUSH_BLOCKS");
         if (e && atoi(e) > 0) r->push_blocks = r->tile_blocks = atoi(e);
@@ -482,6 +505,7 @@ extern "C" int mpic_destroy(mpic_run_t *r)
     }
     hipFree(r->tv.off); hipFree(r->tv.cursor); hipFree(r->tv.count); hipFree(r->tv.key);
     hipFree(r->tv.active); hipFree(r->tv.slot); hipFree(r->tv.nactive); hipFree(r->blocksum);
+    hipFree(r->tv.item); hipFree(r->tv.istart);
     hipFree(r->tj);
     hipFree(r->dacc);
     hipFree(r->derr);
@@ -1484,7 +1508,7 @@ extern "C" int mpic_autotune(mpic_run_t *r, int steps, int *chosen)
     double best_ms = 1e300;
     for (int mult = 4; mult <= 64; mult *= 2) {
         int nb = mult * r->ncu;
-        if (nb > r->tv.ntiles) nb = r->tv.ntiles;
+        if (nb > r->items_cap) nb = (int)r->items_cap;
         r->tile_blocks = nb;
         TRY(advance_timed(r, steps));
         double p, d, rest;
@@ -1493,7 +1517,7 @@ extern "C" int mpic_autotune(mpic_run_t *r, int steps, int *chosen)
             best_ms = p + d;
             best = nb;
         }
-        if (nb == r->tv.ntiles) break;
+        if (nb == r->items_cap) break;
     }
     r->tile_blocks = best;
     r->graphs_ok = 0;

@@ -27,7 +27,20 @@ struct TileView {
     int *nactive;            /* [1] */
     int *queue;              /* [MPIC_XCDS] work counters */
     int *nmark;              /* [2] markers in the current / next order */
+    /* Work items of the FP32 push: an active tile's markers split evenly
+     * into ceil(count / chm) items of at most chm markers (chm = 0: one
+     * item per active tile). */
+    int  chm;
+    int *item;               /* [item capacity] active slot of each work item */
+    int *istart;             /* [ntiles + 1] first item of each active slot, [nactive] = items */
+    int *nitems;             /* [1] */
 };
+
+/* Work items of a tile holding c markers. */
+__host__ __device__ static inline int tile_items(int c, int chm)
+{
+    return c <= 0 ? 0 : (chm > 0 ? (c + chm - 1) / chm : 1);
+}
 
 __device__ static inline void tile_coords(const TileView &tv, int t, int &t0, int &t1, int &t2)
 {

@@ -37,10 +37,12 @@ int launch_keys64(hipStream_t s, const Markers64 &m, long n, const Grid &g, cons
     return MPIC_OK;
 }
 
-/* Block-local exclusive scan of (markers per tile, tile non-empty). */
+/* Block-local exclusive scans of (markers per tile, tile non-empty, work
+ * items of the tile), three ints per tile and per block. */
+#define NSCAN 3
 __global__ void k_scan_a(TileView tv, int *blocksum, int *tileloc)
 {
-    __shared__ int sc[SB], sa[SB];
+    __shared__ int sv[NSCAN][SB];
     const int t = blockIdx.x * SB + threadIdx.x;
     int c = 0;
     if (t < tv.ntiles) {
@@ -50,69 +52,55 @@ __global__ void k_scan_a(TileView tv, int *blocksum, int *tileloc)
             c += x.x + x.y + x.z + x.w;
         }
     }
-    sc[threadIdx.x] = c;
-    sa[threadIdx.x] = (c > 0);
+    const int own[NSCAN] = {c, c > 0, tile_items(c, tv.chm)};
+    for (int a = 0; a < NSCAN; ++a) sv[a][threadIdx.x] = own[a];
     __syncthreads();
     for (int d = 1; d < SB; d <<= 1) {          /* Hillis-Steele inclusive */
-        int a = 0, b = 0;
-        if ((int)threadIdx.x >= d) {
-            a = sc[threadIdx.x - d];
-            b = sa[threadIdx.x - d];
-        }
+        int x[NSCAN] = {0, 0, 0};
+        if ((int)threadIdx.x >= d)
+            for (int a = 0; a < NSCAN; ++a) x[a] = sv[a][threadIdx.x - d];
         __syncthreads();
-        sc[threadIdx.x] += a;
-        sa[threadIdx.x] += b;
+        for (int a = 0; a < NSCAN; ++a) sv[a][threadIdx.x] += x[a];
         __syncthreads();
     }
-    if (t < tv.ntiles) {
-        tileloc[2 * t] = sc[threadIdx.x] - c;            /* block-local exclusive */
-        tileloc[2 * t + 1] = sa[threadIdx.x] - (c > 0);
-    }
-    if (threadIdx.x == SB - 1) {
-        blocksum[2 * blockIdx.x] = sc[SB - 1];
-        blocksum[2 * blockIdx.x + 1] = sa[SB - 1];
-    }
+    if (t < tv.ntiles)
+        for (int a = 0; a < NSCAN; ++a) tileloc[NSCAN * t + a] = sv[a][threadIdx.x] - own[a];
+    if (threadIdx.x == SB - 1)
+        for (int a = 0; a < NSCAN; ++a) blocksum[NSCAN * blockIdx.x + a] = sv[a][SB - 1];
 }
 
 /* Exclusive scan of the block sums (one block, loops for any count). */
 __global__ void k_scan_b(int *blocksum, int nb)
 {
-    __shared__ int sc[SB], sa[SB];
-    __shared__ int carry[2];
-    if (threadIdx.x == 0) carry[0] = carry[1] = 0;
+    __shared__ int sv[NSCAN][SB];
+    __shared__ int carry[NSCAN];
+    if (threadIdx.x < NSCAN) carry[threadIdx.x] = 0;
     __syncthreads();
     for (int b0 = 0; b0 < nb; b0 += SB) {
         const int b = b0 + threadIdx.x;
-        const int c = (b < nb) ? blocksum[2 * b] : 0, a = (b < nb) ? blocksum[2 * b + 1] : 0;
-        sc[threadIdx.x] = c;
-        sa[threadIdx.x] = a;
+        int own[NSCAN];
+        for (int a = 0; a < NSCAN; ++a) {
+            own[a] = (b < nb) ? blocksum[NSCAN * b + a] : 0;
+            sv[a][threadIdx.x] = own[a];
+        }
         __syncthreads();
         for (int d = 1; d < SB; d <<= 1) {
-            int x = 0, y = 0;
-            if ((int)threadIdx.x >= d) {
-                x = sc[threadIdx.x - d];
-                y = sa[threadIdx.x - d];
-            }
+            int x[NSCAN] = {0, 0, 0};
+            if ((int)threadIdx.x >= d)
+                for (int a = 0; a < NSCAN; ++a) x[a] = sv[a][threadIdx.x - d];
             __syncthreads();
-            sc[threadIdx.x] += x;
-            sa[threadIdx.x] += y;
+            for (int a = 0; a < NSCAN; ++a) sv[a][threadIdx.x] += x[a];
             __syncthreads();
         }
-        if (b < nb) {
-            blocksum[2 * b] = carry[0] + sc[threadIdx.x] - c;
-            blocksum[2 * b + 1] = carry[1] + sa[threadIdx.x] - a;
-        }
+        if (b < nb)
+            for (int a = 0; a < NSCAN; ++a)
+                blocksum[NSCAN * b + a] = carry[a] + sv[a][threadIdx.x] - own[a];
         __syncthreads();
-        if (threadIdx.x == SB - 1) {
-            carry[0] += sc[SB - 1];
-            carry[1] += sa[SB - 1];
-        }
+        if (threadIdx.x == SB - 1)
+            for (int a = 0; a < NSCAN; ++a) carry[a] += sv[a][SB - 1];
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        blocksum[2 * nb] = carry[0];
-        blocksum[2 * nb + 1] = carry[1];
-    }
+    if (threadIdx.x < NSCAN) blocksum[NSCAN * nb + threadIdx.x] = carry[threadIdx.x];
 }
 
 /* Per-key offsets and cursors, the active-tile list, counters reset: one
@@ -130,20 +118,28 @@ __global__ void k_scan_c(TileView tv, const int *blocksum, const int *tileloc, i
         if (lane >= d) incl += v;
     }
     const int tb = t / SB;
-    const int o = blocksum[2 * tb] + tileloc[2 * t] + incl - c;
+    const int o = blocksum[NSCAN * tb] + tileloc[NSCAN * t] + incl - c;
     tv.off[q] = o;
     tv.cursor[q] = o;
     tv.count[q] = 0;
     const int tot = __shfl(incl, 63, 64);
     if (lane == 0) {
-        const int a = blocksum[2 * tb + 1] + tileloc[2 * t + 1];
-        if (tot > 0) tv.active[a] = t;
+        const int a = blocksum[NSCAN * tb + 1] + tileloc[NSCAN * t + 1];
+        if (tot > 0) {
+            const int is = blocksum[NSCAN * tb + 2] + tileloc[NSCAN * t + 2];
+            tv.active[a] = t;
+            tv.istart[a] = is;
+            const int ni = tile_items(tot, tv.chm);
+            for (int u = 0; u < ni; ++u) tv.item[is + u] = a;
+        }
         tv.slot[t] = tot > 0 ? a : -1;
     }
     if (q == 0) {
-        tv.off[tv.ncellp] = blocksum[2 * nb];
-        *tv.nactive = blocksum[2 * nb + 1];
-        tv.nmark[parity_next] = blocksum[2 * nb];
+        tv.off[tv.ncellp] = blocksum[NSCAN * nb];
+        *tv.nactive = blocksum[NSCAN * nb + 1];
+        tv.istart[blocksum[NSCAN * nb + 1]] = blocksum[NSCAN * nb + 2];
+        *tv.nitems = blocksum[NSCAN * nb + 2];
+        tv.nmark[parity_next] = blocksum[NSCAN * nb];
         for (int x = 0; x < MPIC_XCDS; ++x) tv.queue[x] = 0;
     }
 }
@@ -151,7 +147,7 @@ __global__ void k_scan_c(TileView tv, const int *blocksum, const int *tileloc, i
 int launch_tile_scan(hipStream_t s, const TileView &tv, int *blocksum, int parity_next)
 {
     const int nb = (tv.ntiles + SB - 1) / SB;
-    int *tileloc = blocksum + 2 * (nb + 1);
+    int *tileloc = blocksum + NSCAN * (nb + 1);
     k_scan_a<<<nb, SB, 0, s>>>(tv, blocksum, tileloc);
     MPIC_CHECK(hipGetLastError());
     k_scan_b<<<1, SB, 0, s>>>(blocksum, nb);
