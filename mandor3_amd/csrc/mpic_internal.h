@@ -96,11 +96,14 @@ struct FaceOp {
 };
 
 /* FP32 split-precision markers (see mpic_push32.hip). */
+/* FP32 split markers: two 16-byte records per marker, each moved by one
+ * 128-bit load or store, and the id:
+ *   pos = (cell word as float bits, ox, oy, oz): cell word = (species << 29)
+ *         | tile-major cell key, -1 = deleted; offsets in cell units
+ *   mom = (ux, uy, uz, w) */
 struct Markers32 {
-    int32_t *cell;           /* (species << 29) | tile-major cell key, -1 = deleted */
-    float   *ox, *oy, *oz;   /* offset in the cell, cell units */
-    float   *ux, *uy, *uz;
-    float   *w;
+    float4  *pos;
+    float4  *mom;
     int32_t *id;
 };
 

@@ -1,12 +1,13 @@
 /*
  * mpic_push32.hip -- FP32 split-precision marker push and current deposit.
  *
- * Marker layout (SoA, 32 B read + 28 B written per push):
- *   cell  int32  (species << 29) | tile-major cell key   (-1: deleted)
- *   ox oy oz     FP32 offset inside the cell, in cell units, [0, 1)
- *   ux uy uz     FP32 momentum gamma*v/c
- *   w            FP32 charge density of the marker (the reference's rho)
- *   id   int32   reference marker id (moved by the sort only)
+ * Marker layout, two 16-byte records per marker (32 B read + 32 B written
+ * per push, each record one 128-bit access) and the id:
+ *   pos   cell word (int32 bits: (species << 29) | tile-major cell key,
+ *         -1 deleted), ox oy oz FP32 offset inside the cell, cell units
+ *   mom   ux uy uz FP32 momentum gamma*v/c, w FP32 charge density of the
+ *         marker (the reference's rho)
+ *   id    int32 reference marker id (moved by the sort only)
  * Position = (cell index + offset) * h: the integer part carries the
  * position exactly, so FP32 resolves 6e-8 of a cell anywhere in the box.
  *
@@ -25,7 +26,7 @@
  * flushed to the FP64 J in HBM once per tile.  The tile's E/H nodes are
  * staged in LDS, so gathers never leave the CU.
  *
- * Roofline: HBM-bound.  Algorithmic bytes per marker-step 32 B read + 28 B
+ * Roofline: HBM-bound.  Algorithmic bytes per marker-step 32 B read + 32 B
  * written (id is not touched), plus per occupied cell 24 B of FP32 E/H reads
  * and 48 B of FP64 J read-modify-write.
  */
@@ -37,6 +38,7 @@
 #define TILE 4
 #define TFN (TILE + 2)         /* E/H nodes per axis staged for a tile's gathers */
 #define TFN3 (TFN * TFN * TFN)
+#define NSTG ((TFN3 + TW - 1) / TW)  /* staged nodes per lane and component */
 #define SPECIES_SHIFT 29
 #define KEY_MASK ((1 << SPECIES_SHIFT) - 1)
 
@@ -216,14 +218,15 @@ __device__ static inline void load_marker(const Markers32 &m, unsigned p, unsign
                                           MarkerRegs &r)
 {
     p = p < last ? p : last;
-    r.word = at(m.cell, p);
-    r.ox = at(m.ox, p);
-    r.oy = at(m.oy, p);
-    r.oz = at(m.oz, p);
-    r.ux = at(m.ux, p);
-    r.uy = at(m.uy, p);
-    r.uz = at(m.uz, p);
-    r.w = at(m.w, p);
+    const float4 a = m.pos[p], b = m.mom[p];
+    r.word = __float_as_int(a.x);
+    r.ox = a.y;
+    r.oy = a.z;
+    r.oz = a.w;
+    r.ux = b.x;
+    r.uy = b.y;
+    r.uz = b.z;
+    r.w = b.w;
 }
 
 /* Trilinear gather from a tile's staged field block (TFN^3 nodes, node
@@ -373,6 +376,7 @@ __device__ static inline int push32_marker(MarkerRegs &out, const MarkerRegs &r,
     out.ux = ux;
     out.uy = uy;
     out.uz = uz;
+    out.w = w;
     return (sp << SPECIES_SHIFT) | key;
 }
 
@@ -435,23 +439,16 @@ struct Pending {
 __device__ static inline void write_back(const Markers32 &m, const TileView &tv, const Pending &q)
 {
     if (q.valid) {
-        at(m.cell, q.p) = q.word;
-        if (q.word >= 0) {
-            at(m.ox, q.p) = q.r.ox;
-            at(m.oy, q.p) = q.r.oy;
-            at(m.oz, q.p) = q.r.oz;
-            at(m.ux, q.p) = q.r.ux;
-            at(m.uy, q.p) = q.r.uy;
-            at(m.uz, q.p) = q.r.uz;
-        }
+        m.pos[q.p] = make_float4(__int_as_float(q.word), q.r.ox, q.r.oy, q.r.oz);
+        if (q.word >= 0) m.mom[q.p] = make_float4(q.r.ux, q.r.uy, q.r.uz, q.r.w);
     }
     if (q.ckey >= 0) atomicAdd(tv.count + q.ckey, q.clen);
 }
 
 /* Tiled push: one wavefront per work item, an even share of one active
  * tile's (4x4x4 cells) markers, so that tiles of very different occupancy
- * still keep every wavefront slot busy to the end.  A workgroup starts with
- * item blockIdx.x and claims the next from a queue while it pushes.  The
+ * still keep the wavefront slots busy to the end; items are taken with
+ * stride = grid size (a shared queue would serialise on its counter).  The
  * tile's E/H nodes are staged in LDS, the item's J is accumulated in an
  * FP32 LDS block and stored, without atomics, as the item's block in 'tj';
  * the next batch of markers is loaded before the current one is pushed. */
@@ -466,8 +463,7 @@ k_push32_tiles(Markers32 m, F32View v, Push32Consts c, TileView tv, float *tj, d
     const int lane = threadIdx.x & 63;
     const int nitems = *tv.nitems;
     float w0 = 0, w1 = 0, w2 = 0;         /* per-thread kinetic energy partials */
-    int it = blockIdx.x;                  /* first item: the workgroup's own; then the queue */
-    while (it < nitems) {
+    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
         const int sl = tv.item[it];
         const int t = tv.active[sl];
         const int u = it - tv.istart[sl], nu = tv.istart[sl + 1] - tv.istart[sl];
@@ -484,13 +480,35 @@ k_push32_tiles(Markers32 m, F32View v, Push32Consts c, TileView tv, float *tj, d
         __syncthreads();                  /* previous tile's LDS reads are done */
         for (int q = threadIdx.x; q < TJB; q += TW) lj[q] = 0.0f;
         /* nodes [c, c + TFN) per axis; those past the last ghost layer (a
-         * partial tile at the high edge) are never read by a gather */
-        for (int q = threadIdx.x; q < 6 * TFN3; q += TW) {
-            const int comp = q / TFN3, rr = q - comp * TFN3;
-            const int a = rr / (TFN * TFN), b = (rr / TFN) % TFN, cc = rr % TFN;
-            const int i = c0 + a, j = c1 + b, k = c2 + cc;
-            const float *F = comp < 3 ? v.E[comp] : v.H[comp - 3];
-            lf[q] = (i <= g.n[0] + 2 && j <= g.n[1] + 2 && k <= g.n[2] + 2) ? F[gidx(g, i, j, k)] : 0.0f;
+         * partial tile at the high edge) are never read by a gather and are
+         * staged as 0.  Every load of a lane is issued before the first LDS
+         * store, so staging costs one memory latency, not one per node. */
+        {
+            const long base = gidx(g, c0, c1, c2);
+            unsigned offs[NSTG];
+            bool ok[NSTG];
+#pragma unroll
+            for (int r = 0; r < NSTG; ++r) {
+                const int rr = lane + TW * r;
+                const int a = rr / (TFN * TFN), b = (rr / TFN) % TFN, cc = rr % TFN;
+                ok[r] = rr < TFN3 && c0 + a <= g.n[0] + 2 && c1 + b <= g.n[1] + 2 &&
+                        c2 + cc <= g.n[2] + 2;
+                offs[r] = ok[r] ? (unsigned)((a * g.GY + b) * g.GZ + cc) : 0u;
+            }
+            float st[6][NSTG];
+#pragma unroll
+            for (int comp = 0; comp < 6; ++comp) {
+                const float *F = (comp < 3 ? v.E[comp] : v.H[comp - 3]) + base;
+#pragma unroll
+                for (int r = 0; r < NSTG; ++r) st[comp][r] = at(F, offs[r]);
+            }
+#pragma unroll
+            for (int comp = 0; comp < 6; ++comp)
+#pragma unroll
+                for (int r = 0; r < NSTG; ++r) {
+                    const int rr = lane + TW * r;
+                    if (rr < TFN3) lf[comp * TFN3 + rr] = ok[r] ? st[comp][r] : 0.0f;
+                }
         }
         /* item u of nu: an even share of the tile's markers */
         const unsigned tb = tv.off[t * TILE_CELLS], cnt = tv.off[(t + 1) * TILE_CELLS] - tb;
@@ -502,9 +520,6 @@ k_push32_tiles(Markers32 m, F32View v, Push32Consts c, TileView tv, float *tj, d
          * only ever concern the prefetch of the batch in flight */
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
-        /* the next item, claimed now so that its latency hides behind this one */
-        int claim = 0;
-        if (lane == 0) claim = atomicAdd(tv.queue, 1);
         Pending pend;
         pend.valid = false;
         pend.ckey = -1;
@@ -550,7 +565,6 @@ k_push32_tiles(Markers32 m, F32View v, Push32Consts c, TileView tv, float *tj, d
         /* the tile's J block, stored whole (coalesced, no atomics) */
         float *dst = tj + (long)it * TJB;
         for (int q = threadIdx.x; q < TJB; q += TW) dst[q] = lj[q];
-        it = (int)gridDim.x + __shfl(claim, 0, 64);
     }
     double d0 = w0, d1 = w1, d2 = w2;     /* wave sums in FP64 */
     for (int d = 32; d > 0; d >>= 1) {
@@ -677,7 +691,7 @@ int launch_jgather32(hipStream_t s, const Fields32 &f, const Grid &g, const Tile
 __global__ void k_keys32(Markers32 m, long n, TileView tv)
 {
     const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int word = (p < n) ? m.cell[p] : -1;
+    const int word = (p < n) ? ((const int *)m.pos)[4 * p] : -1;
     wave_run_count(tv.count, word < 0 ? -1 : (word & KEY_MASK));
 }
 
@@ -693,7 +707,9 @@ __global__ void k_scatter32(Markers32 src, Markers32 dst, long nbound, TileView 
 {
     const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long n = tv.nmark[parity];
-    const int word = (p < n && p < nbound) ? src.cell[p] : -1;
+    float4 a = make_float4(__int_as_float(-1), 0.0f, 0.0f, 0.0f);
+    if (p < n && p < nbound) a = src.pos[p];
+    const int word = __float_as_int(a.x);
     const int key = word < 0 ? -1 : (word & KEY_MASK);
     int start, len;
     bool head;
@@ -703,14 +719,8 @@ __global__ void k_scatter32(Markers32 src, Markers32 dst, long nbound, TileView 
     base = __shfl(base, start, 64);
     if (key < 0) return;
     const long d = base + ((int)(threadIdx.x & 63) - start);
-    dst.cell[d] = word;
-    dst.ox[d] = src.ox[p];
-    dst.oy[d] = src.oy[p];
-    dst.oz[d] = src.oz[p];
-    dst.ux[d] = src.ux[p];
-    dst.uy[d] = src.uy[p];
-    dst.uz[d] = src.uz[p];
-    dst.w[d] = src.w[p];
+    dst.pos[d] = a;
+    dst.mom[d] = src.mom[p];
     dst.id[d] = src.id[p];
 }
 
@@ -728,11 +738,12 @@ __global__ void k_rho32(Markers32 m, long n, double *rho, Grid g, TileView tv)
 {
     const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
-    const int word = m.cell[p];
+    const float4 a = m.pos[p];
+    const int word = __float_as_int(a.x);
     if (word < 0) return;
     int i, j, k;
     key_to_ijk(tv, word & KEY_MASK, i, j, k);
-    const double sx = m.ox[p], sy = m.oy[p], sz = m.oz[p], val = m.w[p];
+    const double sx = a.y, sy = a.z, sz = a.w, val = m.mom[p].w;
     unsafeAtomicAdd(rho + gidx(g, i, j, k), (1 - sx) * ((1 - sy) * ((1 - sz) * val)));
     unsafeAtomicAdd(rho + gidx(g, i + 1, j, k), sx * ((1 - sy) * ((1 - sz) * val)));
     unsafeAtomicAdd(rho + gidx(g, i, j + 1, k), (1 - sx) * (sy * ((1 - sz) * val)));
@@ -757,7 +768,7 @@ __global__ void k_cells32(Markers32 m, long n, Grid g, TileView tv, int32_t *cel
 {
     const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
-    const int word = m.cell[p];
+    const int word = ((const int *)m.pos)[4 * p];
     if (word < 0) {
         cell[p] = -1;
         return;

@@ -271,13 +271,15 @@ static size_t field_bytes(const int n[3], int fp32)
     return (fp32 ? sizeof(float) : sizeof(double)) * 12 * cells + sizeof(double) * 3 * cells;
 }
 
-/* Markers per work item of the FP32 push (the tile's markers are split
- * evenly into items of at most this many); MPIC_ITEM_MARKERS overrides. */
+/* Markers per work item of the FP32 push (a tile's markers are split
+ * evenly into items of at most this many; 0 = one item per tile, the
+ * default: on the laser-foil workload splitting measured slower at every
+ * item size from 512 to 1728).  MPIC_ITEM_MARKERS overrides. */
 static int push32_item_markers(void)
 {
     const char *e = getenv("MPIC_ITEM_MARKERS");
     const int v = e ? atoi(e) : 0;
-    return v >= 64 ? v : 1024;
+    return v >= 64 ? v : 0;
 }
 
 /* Work items of a run: at most one per tile plus one per chm markers. */
@@ -324,13 +326,12 @@ extern "C" int mpic_create(const mpic_config_t *cfg, mpic_run_t **out)
         return MPIC_ERR_ARG;
     }
     if (cfg->precision == MPIC_FP32_SPLIT) {
-        /* the split form's cell word keeps 29 bits for the cell and its
-         * kernels address marker arrays with 32-bit byte offsets */
+        /* the split form's cell word keeps 29 bits for the cell */
         const long cells = (long)((cfg->n[0] + 3) / 4) * ((cfg->n[1] + 3) / 4) *
                            ((cfg->n[2] + 3) / 4) * 64;
-        if (cells > (1L << 29) || cfg->max_markers >= (1L << 30)) {
+        if (cells > (1L << 29)) {
             mpic_set_error("mpic_create: the FP32 split form holds at most 2^29 cells (in 4^3 "
-                           "tiles) and 2^30 - 1 markers per run");
+                           "tiles)");
             return MPIC_ERR_ARG;
         }
     }
@@ -384,9 +385,8 @@ extern "C" int mpic_create(const mpic_config_t *cfg, mpic_run_t **out)
     }
     for (int b = 0; b < 2 && r->fp32; ++b) {
         Markers32 &mb = r->m32buf[b];
-        MPIC_CHECK(hipMalloc(&mb.cell, sizeof(int32_t) * r->cap));
-        float **fs[7] = {&mb.ox, &mb.oy, &mb.oz, &mb.ux, &mb.uy, &mb.uz, &mb.w};
-        for (int q = 0; q < 7; ++q) MPIC_CHECK(hipMalloc(fs[q], sizeof(float) * r->cap));
+        MPIC_CHECK(hipMalloc(&mb.pos, sizeof(float4) * r->cap));
+        MPIC_CHECK(hipMalloc(&mb.mom, sizeof(float4) * r->cap));
         MPIC_CHECK(hipMalloc(&mb.id, sizeof(int32_t) * r->cap));
     }
     r->m32 = r->m32buf[0];
@@ -500,8 +500,7 @@ extern "C" int mpic_destroy(mpic_run_t *r)
         hipFree(mb.ux); hipFree(mb.uy); hipFree(mb.uz);
         hipFree(mb.w); hipFree(mb.qm); hipFree(mb.id);
         Markers32 &m3 = r->m32buf[b];
-        hipFree(m3.cell); hipFree(m3.ox); hipFree(m3.oy); hipFree(m3.oz);
-        hipFree(m3.ux); hipFree(m3.uy); hipFree(m3.uz); hipFree(m3.w); hipFree(m3.id);
+        hipFree(m3.pos); hipFree(m3.mom); hipFree(m3.id);
     }
     hipFree(r->tv.off); hipFree(r->tv.cursor); hipFree(r->tv.count); hipFree(r->tv.key);
     hipFree(r->tv.active); hipFree(r->tv.slot); hipFree(r->tv.nactive); hipFree(r->blocksum);
@@ -663,10 +662,9 @@ static int host_key(const mpic_run *r, int i, int j, int k)
 static int load_split(mpic_run *r, std::vector<double> *d, std::vector<float> &w,
                       std::vector<float> &qm, std::vector<int32_t> &id, long n)
 {
-    std::vector<int32_t> cell;
-    std::vector<float> f[7];
-    cell.reserve(n);
-    for (int c = 0; c < 7; ++c) f[c].reserve(n);
+    std::vector<float> pos, mom;          /* 4 floats per marker each */
+    pos.reserve(4 * n);
+    mom.reserve(4 * n);
     std::vector<int32_t> ids;
     ids.reserve(n);
     for (long q = 0; q < n; ++q) {
@@ -698,18 +696,18 @@ static int load_split(mpic_run *r, std::vector<double> *d, std::vector<float> &w
                 return MPIC_ERR_ARG;
             }
         }
-        cell.push_back((sp << 29) | host_key(r, ic[0], ic[1], ic[2]));
-        f[0].push_back(of[0]); f[1].push_back(of[1]); f[2].push_back(of[2]);
-        f[3].push_back((float)u[0]); f[4].push_back((float)u[1]); f[5].push_back((float)u[2]);
-        f[6].push_back(w[q]);
+        const int32_t word = (sp << 29) | host_key(r, ic[0], ic[1], ic[2]);
+        float wf;
+        memcpy(&wf, &word, sizeof(wf));
+        pos.push_back(wf); pos.push_back(of[0]); pos.push_back(of[1]); pos.push_back(of[2]);
+        mom.push_back((float)u[0]); mom.push_back((float)u[1]); mom.push_back((float)u[2]);
+        mom.push_back(w[q]);
         ids.push_back(id[q]);
     }
-    const long m = (long)cell.size();
+    const long m = (long)ids.size();
     Markers32 &M = r->m32;
-    MPIC_CHECK(hipMemcpy(M.cell, cell.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice));
-    float *dst[7] = {M.ox, M.oy, M.oz, M.ux, M.uy, M.uz, M.w};
-    for (int c = 0; c < 7; ++c)
-        MPIC_CHECK(hipMemcpy(dst[c], f[c].data(), sizeof(float) * m, hipMemcpyHostToDevice));
+    MPIC_CHECK(hipMemcpy(M.pos, pos.data(), sizeof(float4) * m, hipMemcpyHostToDevice));
+    MPIC_CHECK(hipMemcpy(M.mom, mom.data(), sizeof(float4) * m, hipMemcpyHostToDevice));
     MPIC_CHECK(hipMemcpy(M.id, ids.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice));
     r->n = m;
     return MPIC_OK;
@@ -718,35 +716,33 @@ static int load_split(mpic_run *r, std::vector<double> *d, std::vector<float> &w
 static int save_split(mpic_run *r, void *rec, long *n, int recsize, long N)
 {
     Markers32 &M = r->m32;
-    std::vector<int32_t> cell(N), id(N);
-    std::vector<float> f[7];
-    float *src[7] = {M.ox, M.oy, M.oz, M.ux, M.uy, M.uz, M.w};
-    MPIC_CHECK(hipMemcpy(cell.data(), M.cell, sizeof(int32_t) * N, hipMemcpyDeviceToHost));
+    std::vector<int32_t> id(N);
+    std::vector<float> pos(4 * N), mom(4 * N);
+    MPIC_CHECK(hipMemcpy(pos.data(), M.pos, sizeof(float4) * N, hipMemcpyDeviceToHost));
+    MPIC_CHECK(hipMemcpy(mom.data(), M.mom, sizeof(float4) * N, hipMemcpyDeviceToHost));
     MPIC_CHECK(hipMemcpy(id.data(), M.id, sizeof(int32_t) * N, hipMemcpyDeviceToHost));
-    for (int c = 0; c < 7; ++c) {
-        f[c].resize(N);
-        MPIC_CHECK(hipMemcpy(f[c].data(), src[c], sizeof(float) * N, hipMemcpyDeviceToHost));
-    }
     char *dst = (char *)rec;
     long o = 0;
     for (long q = 0; q < N; ++q) {
-        if (cell[q] < 0) continue;
+        int32_t word;
+        memcpy(&word, &pos[4 * q], sizeof(word));
+        if (word < 0) continue;
         if (o >= *n) {
             mpic_set_error("mpic_markers_save: output holds %ld markers, more are live", *n);
             return MPIC_ERR_CAPACITY;
         }
-        const int key = cell[q] & ((1 << 29) - 1), sp = (int)((unsigned)cell[q] >> 29);
+        const int key = word & ((1 << 29) - 1), sp = (int)((unsigned)word >> 29);
         const int t = key >> 6, loc = key & 63;
         const int t2 = t % r->tv.nt[2], t1 = (t / r->tv.nt[2]) % r->tv.nt[1],
                   t0 = t / (r->tv.nt[2] * r->tv.nt[1]);
         const int ic[3] = {4 * t0 + (loc >> 4), 4 * t1 + ((loc >> 2) & 3), 4 * t2 + (loc & 3)};
         RecView v;
         memset(&v, 0, sizeof(v));
-        v.x = (ic[0] + (double)f[0][q]) * r->g.h[0];
-        v.y = (ic[1] + (double)f[1][q]) * r->g.h[1];
-        v.z = (ic[2] + (double)f[2][q]) * r->g.h[2];
-        v.vx = f[3][q]; v.vy = f[4][q]; v.vz = f[5][q];
-        v.rho = f[6][q];
+        v.x = (ic[0] + (double)pos[4 * q + 1]) * r->g.h[0];
+        v.y = (ic[1] + (double)pos[4 * q + 2]) * r->g.h[1];
+        v.z = (ic[2] + (double)pos[4 * q + 3]) * r->g.h[2];
+        v.vx = mom[4 * q]; v.vy = mom[4 * q + 1]; v.vz = mom[4 * q + 2];
+        v.rho = mom[4 * q + 3];
         v.qDivM = r->qm[sp];
         v.id = id[q];
         memset(dst + o * recsize, 0, recsize);
