@@ -24,29 +24,59 @@
 
 static inline unsigned nblk(long n) { return (unsigned)((n + TPB - 1) / TPB); }
 
-/* Volume kernels over n + extra nodes per axis: a run of workgroups covers
- * one i plane, the flat thread index runs over the plane's (j, k) nodes, so
- * rows of any length fill whole wavefronts and a node costs one 32-bit
- * division.  Planes are dealt to XCDs in contiguous ranges, so the planes
- * an update reads beside its own are mostly in the same L2. */
-static inline unsigned grid_planes(const Grid &g, int extra)
+/* Volume kernels, vector form: a thread takes V consecutive nodes of a row
+ * (k + 2 = 0 mod V) and moves each field as one 16-byte access (V = 4 FP32
+ * or 2 FP64 nodes); the row pitch is a multiple of 4 nodes, so every such
+ * access is aligned.  Per node the arithmetic is the scalar update's.  A
+ * run of workgroups covers one i plane (threads over the plane's rows and
+ * groups); planes are dealt to XCDs in contiguous ranges, so the planes an
+ * update reads beside its own are mostly in the same L2. */
+template <class T> struct Vec;
+template <> struct Vec<float> {
+    typedef float v __attribute__((ext_vector_type(4)));
+    enum { N = 4 };
+};
+template <> struct Vec<double> {
+    typedef double v __attribute__((ext_vector_type(2)));
+    enum { N = 2 };
+};
+
+/* Groups of V along k covering nodes [klo, khi], rows j in [jlo, jhi]. */
+struct Span {
+    int ilo, jlo, nj, glo, ng;      /* first plane, first row, rows, first group, groups */
+};
+
+template <int V>
+static inline Span span_of(int ilo, int jlo, int jhi, int klo, int khi)
 {
-    const long plane = (long)(g.n[1] + extra) * (g.n[2] + extra);
-    return (unsigned)((plane + TPB - 1) / TPB) * (unsigned)(g.n[0] + extra);
+    Span sp;
+    sp.ilo = ilo;
+    sp.jlo = jlo;
+    sp.nj = jhi - jlo + 1;
+    sp.glo = (klo + MPIC_GHOST) / V;
+    sp.ng = (khi + MPIC_GHOST) / V - sp.glo + 1;
+    return sp;
 }
 
-__device__ static inline bool plane_node(const Grid &g, int extra, int &i, int &j, int &k)
+static inline unsigned grid_span(const Span &sp, int planes)
 {
-    const unsigned nz = (unsigned)(g.n[2] + extra), plane = (unsigned)(g.n[1] + extra) * nz;
-    const unsigned bpp = (plane + TPB - 1) / TPB;            /* workgroups per plane */
+    const long per = (long)sp.nj * sp.ng;
+    return (unsigned)((per + TPB - 1) / TPB) * (unsigned)planes;
+}
+
+/* Plane i, row j and first node k0 of this thread's group. */
+__device__ static inline bool span_group(const Span &sp, int V, int &i, int &j, int &k0)
+{
+    const unsigned per = (unsigned)sp.nj * (unsigned)sp.ng;
+    const unsigned bpp = (per + TPB - 1) / TPB;              /* workgroups per plane */
     const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
     const unsigned ip = lb / bpp;
     const unsigned t = (lb - ip * bpp) * TPB + threadIdx.x;
-    if (t >= plane) return false;
-    const unsigned jj = t / nz;
-    i = (int)ip;
-    j = (int)jj;
-    k = (int)(t - jj * nz);
+    if (t >= per) return false;
+    const unsigned jj = t / (unsigned)sp.ng;
+    i = sp.ilo + (int)ip;
+    j = sp.jlo + (int)jj;
+    k0 = (sp.glo + (int)(t - jj * (unsigned)sp.ng)) * V - MPIC_GHOST;
     return true;
 }
 
@@ -254,39 +284,56 @@ __device__ static inline int in_boxes(const BoxSet &bs, int i, int j, int k)
  * em_caps.c:730-749) or H = Hc after the second (cap_flushH, :754-770).
  * Nodes [0, n+1]^3; interior and flush regions are disjoint. */
 template <class T>
-__global__ void k_h_update(FieldsT<T> f, Grid g, BoxSet fl, int full, T c1, T c2, T c3)
+__global__ void __launch_bounds__(TPB) k_h_update(FieldsT<T> f, Grid g, BoxSet fl, int full, Span sp,
+                                                  T c1, T c2, T c3)
 {
-    /* nodes [0, n+1]^3: one i plane per blockIdx.y */
-    int i, j, k;
-    if (!plane_node(g, 2, i, j, k)) return;
-    long o = gidx(g, i, j, k);
-    if (i >= 3 && i <= g.n[0] - 2 && j >= 3 && j <= g.n[1] - 2 && k >= 3 && k <= g.n[2] - 2) {
-        const long oi = o - (long)g.GY * g.GZ, oj = o - g.GZ, ok = o - 1;
-        const T *Ex = f.E[0], *Ey = f.E[1], *Ez = f.E[2];
-        const T ex = Ex[o], ey = Ey[o], ez = Ez[o];
-        f.H[0][o] += c3 * (ey - Ey[ok]) - c2 * (ez - Ez[oj]);
-        f.H[1][o] += c1 * (ez - Ez[oi]) - c3 * (ex - Ex[ok]);
-        f.H[2][o] += c2 * (ex - Ex[oj]) - c1 * (ey - Ey[oi]);
-    } else if (in_boxes(fl, i, j, k)) {
-        if (full) {
-            f.H[0][o] = f.Hc[0][o];
-            f.H[1][o] = f.Hc[1][o];
-            f.H[2][o] = f.Hc[2][o];
-        } else {
-            f.H[0][o] = (T)0.5 * (f.H[0][o] + f.Hc[0][o]);
-            f.H[1][o] = (T)0.5 * (f.H[1][o] + f.Hc[1][o]);
-            f.H[2][o] = (T)0.5 * (f.H[2][o] + f.Hc[2][o]);
+    typedef typename Vec<T>::v VT;
+    constexpr int V = Vec<T>::N;
+    int i, j, k0;
+    if (!span_group(sp, V, i, j, k0)) return;
+    const long o = gidx(g, i, j, k0);
+    const long si = (long)g.GY * g.GZ, sj = g.GZ;
+    const T *Ex = f.E[0], *Ey = f.E[1], *Ez = f.E[2];
+    const VT ex = *(const VT *)(Ex + o), ey = *(const VT *)(Ey + o), ez = *(const VT *)(Ez + o);
+    const VT exj = *(const VT *)(Ex + o - sj), ezj = *(const VT *)(Ez + o - sj);
+    const VT eyi = *(const VT *)(Ey + o - si), ezi = *(const VT *)(Ez + o - si);
+    const T exk0 = Ex[o - 1], eyk0 = Ey[o - 1];          /* k - 1 of the first node */
+    VT hx = *(const VT *)(f.H[0] + o), hy = *(const VT *)(f.H[1] + o), hz = *(const VT *)(f.H[2] + o);
+    const bool ij_in = i >= 3 && i <= g.n[0] - 2 && j >= 3 && j <= g.n[1] - 2;
+#pragma unroll
+    for (int e = 0; e < V; ++e) {
+        const int k = k0 + e;
+        const T exk = e ? ex[e - 1] : exk0, eyk = e ? ey[e - 1] : eyk0;
+        if (ij_in && k >= 3 && k <= g.n[2] - 2) {
+            hx[e] += c3 * (ey[e] - eyk) - c2 * (ez[e] - ezj[e]);
+            hy[e] += c1 * (ez[e] - ezi[e]) - c3 * (ex[e] - exk);
+            hz[e] += c2 * (ex[e] - exj[e]) - c1 * (ey[e] - eyi[e]);
+        } else if (k >= 0 && k <= g.n[2] + 1 && in_boxes(fl, i, j, k)) {
+            if (full) {
+                hx[e] = f.Hc[0][o + e];
+                hy[e] = f.Hc[1][o + e];
+                hz[e] = f.Hc[2][o + e];
+            } else {
+                hx[e] = (T)0.5 * (hx[e] + f.Hc[0][o + e]);
+                hy[e] = (T)0.5 * (hy[e] + f.Hc[1][o + e]);
+                hz[e] = (T)0.5 * (hz[e] + f.Hc[2][o + e]);
+            }
         }
     }
+    *(VT *)(f.H[0] + o) = hx;
+    *(VT *)(f.H[1] + o) = hy;
+    *(VT *)(f.H[2] + o) = hz;
 }
 
 template <class T>
 int launch_h_update(hipStream_t s, const FieldsT<T> &f, const Grid &g, const BoxSet &flush,
                     int full)
 {
-    k_h_update<T><<<grid_planes(g, 2), TPB, 0, s>>>(f, g, flush, full, (T)(0.5 * g.tau / g.h[0]),
-                                                        (T)(0.5 * g.tau / g.h[1]),
-                                                        (T)(0.5 * g.tau / g.h[2]));
+    const Span sp = span_of<Vec<T>::N>(0, 0, g.n[1] + 1, 0, g.n[2] + 1);
+    k_h_update<T><<<grid_span(sp, g.n[0] + 2), TPB, 0, s>>>(f, g, flush, full, sp,
+                                                            (T)(0.5 * g.tau / g.h[0]),
+                                                            (T)(0.5 * g.tau / g.h[1]),
+                                                            (T)(0.5 * g.tau / g.h[2]));
     MPIC_CHECK(hipGetLastError());
     return MPIC_OK;
 }
@@ -294,60 +341,85 @@ int launch_h_update(hipStream_t s, const FieldsT<T> &f, const Grid &g, const Box
 /* Interior E step (source/core/em.c:134-161) and/or the flush of the cap
  * layer E = Ec (cap_flushE, em_caps.c:775-794). */
 template <class T>
-__global__ void k_e_update(FieldsT<T> f, Grid g, BoxSet fl, int interior, int do_flush, T c1,
-                           T c2, T c3)
+__global__ void __launch_bounds__(TPB) k_e_update(FieldsT<T> f, Grid g, BoxSet fl, int interior,
+                                                  int do_flush, Span sp, T c1, T c2, T c3)
 {
-    int i, j, k;
-    if (!plane_node(g, 2, i, j, k)) return;
-    long o = gidx(g, i, j, k);
-    if (i >= 2 && i <= g.n[0] - 2 && j >= 2 && j <= g.n[1] - 2 && k >= 2 && k <= g.n[2] - 2) {
-        if (!interior) return;
-        const long oi = o + (long)g.GY * g.GZ, oj = o + g.GZ, ok = o + 1;
-        const T *Hx = f.H[0], *Hy = f.H[1], *Hz = f.H[2];
-        const T hx = Hx[o], hy = Hy[o], hz = Hz[o];
-        f.E[0][o] -= c3 * (Hy[ok] - hy) - c2 * (Hz[oj] - hz);
-        f.E[1][o] -= c1 * (Hz[oi] - hz) - c3 * (Hx[ok] - hx);
-        f.E[2][o] -= c2 * (Hx[oj] - hx) - c1 * (Hy[oi] - hy);
-    } else if (do_flush && in_boxes(fl, i, j, k)) {
-        f.E[0][o] = f.Ec[0][o];
-        f.E[1][o] = f.Ec[1][o];
-        f.E[2][o] = f.Ec[2][o];
+    typedef typename Vec<T>::v VT;
+    constexpr int V = Vec<T>::N;
+    int i, j, k0;
+    if (!span_group(sp, V, i, j, k0)) return;
+    const long o = gidx(g, i, j, k0);
+    const long si = (long)g.GY * g.GZ, sj = g.GZ;
+    const T *Hx = f.H[0], *Hy = f.H[1], *Hz = f.H[2];
+    const VT hx = *(const VT *)(Hx + o), hy = *(const VT *)(Hy + o), hz = *(const VT *)(Hz + o);
+    const VT hxj = *(const VT *)(Hx + o + sj), hzj = *(const VT *)(Hz + o + sj);
+    const VT hyi = *(const VT *)(Hy + o + si), hzi = *(const VT *)(Hz + o + si);
+    const T hxkV = Hx[o + V], hykV = Hy[o + V];          /* k + 1 of the last node */
+    VT ex = *(const VT *)(f.E[0] + o), ey = *(const VT *)(f.E[1] + o), ez = *(const VT *)(f.E[2] + o);
+    const bool ij_in = i >= 2 && i <= g.n[0] - 2 && j >= 2 && j <= g.n[1] - 2;
+#pragma unroll
+    for (int e = 0; e < V; ++e) {
+        const int k = k0 + e;
+        const T hxk = e < V - 1 ? hx[e + 1] : hxkV, hyk = e < V - 1 ? hy[e + 1] : hykV;
+        if (ij_in && k >= 2 && k <= g.n[2] - 2) {
+            if (interior) {
+                ex[e] -= c3 * (hyk - hy[e]) - c2 * (hzj[e] - hz[e]);
+                ey[e] -= c1 * (hzi[e] - hz[e]) - c3 * (hxk - hx[e]);
+                ez[e] -= c2 * (hxj[e] - hx[e]) - c1 * (hyi[e] - hy[e]);
+            }
+        } else if (do_flush && k >= 0 && k <= g.n[2] + 1 && in_boxes(fl, i, j, k)) {
+            ex[e] = f.Ec[0][o + e];
+            ey[e] = f.Ec[1][o + e];
+            ez[e] = f.Ec[2][o + e];
+        }
     }
+    *(VT *)(f.E[0] + o) = ex;
+    *(VT *)(f.E[1] + o) = ey;
+    *(VT *)(f.E[2] + o) = ez;
 }
 
 template <class T>
 int launch_e_update(hipStream_t s, const FieldsT<T> &f, const Grid &g, const BoxSet &flush,
                     int interior, int do_flush)
 {
-    k_e_update<T><<<grid_planes(g, 2), TPB, 0, s>>>(f, g, flush, interior, do_flush,
-                                                        (T)(g.tau / g.h[0]), (T)(g.tau / g.h[1]),
-                                                        (T)(g.tau / g.h[2]));
+    const Span sp = span_of<Vec<T>::N>(0, 0, g.n[1] + 1, 0, g.n[2] + 1);
+    k_e_update<T><<<grid_span(sp, g.n[0] + 2), TPB, 0, s>>>(f, g, flush, interior, do_flush, sp,
+                                                            (T)(g.tau / g.h[0]), (T)(g.tau / g.h[1]),
+                                                            (T)(g.tau / g.h[2]));
     MPIC_CHECK(hipGetLastError());
     return MPIC_OK;
 }
 
-/* E -= tau*J over nodes [-1, n+1]^3, source/core/em.c:166-184. */
+/* E -= tau*J over nodes [-1, n+1]^3, source/core/em.c:166-184.  FP32
+ * fields: the FP64 J sum is applied in FP64 and rounded once. */
 template <class T>
-__global__ void k_e_tauJ(FieldsT<T> f, Grid g)
+__global__ void __launch_bounds__(TPB) k_e_tauJ(FieldsT<T> f, Grid g, Span sp)
 {
-    /* nodes [-1, n+1]^3 */
-    int i, j, k;
-    if (!plane_node(g, 3, i, j, k)) return;
-    --i;
-    --j;
-    --k;
-    long o = gidx(g, i, j, k);
+    typedef typename Vec<T>::v VT;
+    typedef double VD __attribute__((ext_vector_type(Vec<T>::N)));
+    constexpr int V = Vec<T>::N;
+    int i, j, k0;
+    if (!span_group(sp, V, i, j, k0)) return;
+    const long o = gidx(g, i, j, k0);
     const double tau = g.tau;
-    /* FP32 fields: the FP64 J sum is applied in FP64 and rounded once */
-    f.E[0][o] = (T)(f.E[0][o] - tau * f.J[0][o]);
-    f.E[1][o] = (T)(f.E[1][o] - tau * f.J[1][o]);
-    f.E[2][o] = (T)(f.E[2][o] - tau * f.J[2][o]);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        VT e = *(const VT *)(f.E[c] + o);
+        const VD jv = *(const VD *)(f.J[c] + o);
+#pragma unroll
+        for (int q = 0; q < V; ++q) {
+            const int k = k0 + q;
+            if (k >= -1 && k <= g.n[2] + 1) e[q] = (T)(e[q] - tau * jv[q]);
+        }
+        *(VT *)(f.E[c] + o) = e;
+    }
 }
 
 template <class T>
 int launch_e_tauJ(hipStream_t s, const FieldsT<T> &f, const Grid &g)
 {
-    k_e_tauJ<T><<<grid_planes(g, 3), TPB, 0, s>>>(f, g);
+    const Span sp = span_of<Vec<T>::N>(-1, -1, g.n[1] + 1, -1, g.n[2] + 1);
+    k_e_tauJ<T><<<grid_span(sp, g.n[0] + 3), TPB, 0, s>>>(f, g, sp);
     MPIC_CHECK(hipGetLastError());
     return MPIC_OK;
 }

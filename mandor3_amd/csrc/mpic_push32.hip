@@ -592,11 +592,14 @@ __device__ static inline void covering(int i, int nt, int &lo, int &hi)
  * Every node of an active tile's block is written once, by the first active
  * tile (in tile order) whose block holds it; the sum runs over the covering
  * tiles in tile order and over each tile's items in order, so the result
- * does not depend on scheduling. */
+ * does not depend on scheduling.  One thread per node and all three
+ * components: a node lies in at most 2 blocks per axis, and the first item
+ * of each of the (at most 8) covering tiles is loaded before any is added,
+ * so the gather waits for one memory latency per node, not one per block. */
+#define TJN3 (TJN * TJN * TJN)
 __global__ void __launch_bounds__(TPB) k_jgather32(F32View v, TileView tv, const float *tj)
 {
-    __shared__ int nslot[27];            /* slots of the 3x3x3 tiles around t, -1 empty */
-    __shared__ int nfirst[27], nitem[27];/* their first item and item count */
+    __shared__ int nfirst[27], nitem[27];    /* items of the 3x3x3 tiles around t (0: none) */
     const Grid &g = v.g;
     const int nact = *tv.nactive;
     for (int sl = blockIdx.x; sl < nact; sl += gridDim.x) {
@@ -610,13 +613,11 @@ __global__ void __launch_bounds__(TPB) k_jgather32(F32View v, TileView tv, const
             const bool in = ux >= 0 && uy >= 0 && uz >= 0 && ux < tv.nt[0] && uy < tv.nt[1] &&
                             uz < tv.nt[2];
             const int su = in ? tv.slot[(ux * tv.nt[1] + uy) * tv.nt[2] + uz] : -1;
-            nslot[threadIdx.x] = su;
             nfirst[threadIdx.x] = su >= 0 ? tv.istart[su] : 0;
             nitem[threadIdx.x] = su >= 0 ? tv.istart[su + 1] - tv.istart[su] : 0;
         }
         __syncthreads();
-        for (int q = threadIdx.x; q < TJB; q += TPB) {
-            const int comp = q % 3, node = q / 3;
+        for (int node = threadIdx.x; node < TJN3; node += TPB) {
             const int cc = node % TJN, bb = (node / TJN) % TJN, aa = node / (TJN * TJN);
             const int i = TILE * t0 - 1 + aa, j = TILE * t1 - 1 + bb, k = TILE * t2 - 1 + cc;
             if (i < -2 || j < -2 || k < -2 || i > g.n[0] + 2 || j > g.n[1] + 2 || k > g.n[2] + 2)
@@ -625,24 +626,49 @@ __global__ void __launch_bounds__(TPB) k_jgather32(F32View v, TileView tv, const
             covering(i, tv.nt[0], xl, xh);
             covering(j, tv.nt[1], yl, yh);
             covering(k, tv.nt[2], zl, zh);
-            /* owner: the first covering tile (in tile order) that is active */
+            /* the covering tiles (2 per axis at most), in tile order */
+            int nb[8], off[8];
+            bool use[8];
             int own = -1;
-            for (int ux = xl; ux <= xh && own < 0; ++ux)
-                for (int uy = yl; uy <= yh && own < 0; ++uy)
-                    for (int uz = zl; uz <= zh && own < 0; ++uz)
-                        if (nslot[((ux - t0 + 1) * 3 + (uy - t1 + 1)) * 3 + (uz - t2 + 1)] >= 0)
-                            own = (ux * tv.nt[1] + uy) * tv.nt[2] + uz;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int ux = xl + (q >> 2), uy = yl + ((q >> 1) & 1), uz = zl + (q & 1);
+                const bool inr = ux <= xh && uy <= yh && uz <= zh;
+                const int b = inr ? ((ux - t0 + 1) * 3 + (uy - t1 + 1)) * 3 + (uz - t2 + 1) : 13;
+                use[q] = inr && nitem[b] > 0;
+                nb[q] = b;
+                const int a = i - (TILE * ux - 1), bq = j - (TILE * uy - 1), cz = k - (TILE * uz - 1);
+                off[q] = use[q] ? ((a * TJN + bq) * TJN + cz) * 3 : 0;
+                if (use[q] && own < 0) own = (ux * tv.nt[1] + uy) * tv.nt[2] + uz;
+            }
             if (own != t) continue;
-            double sum = 0.0;
-            for (int ux = xl; ux <= xh; ++ux)
-                for (int uy = yl; uy <= yh; ++uy)
-                    for (int uz = zl; uz <= zh; ++uz) {
-                        const int nb = ((ux - t0 + 1) * 3 + (uy - t1 + 1)) * 3 + (uz - t2 + 1);
-                        const int a = i - (TILE * ux - 1), b = j - (TILE * uy - 1), cz = k - (TILE * uz - 1);
-                        const float *blk = tj + (long)nfirst[nb] * TJB + ((a * TJN + b) * TJN + cz) * 3 + comp;
-                        for (int w = 0; w < nitem[nb]; ++w) sum += (double)blk[(long)w * TJB];
-                    }
-            v.J[comp][gidx(g, i, j, k)] += sum;
+            float f[8][3];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const float *blk = tj + (long)(use[q] ? nfirst[nb[q]] : 0) * TJB + off[q];
+                f[q][0] = blk[0];
+                f[q][1] = blk[1];
+                f[q][2] = blk[2];
+            }
+            double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                if (!use[q]) continue;
+                s0 += (double)f[q][0];
+                s1 += (double)f[q][1];
+                s2 += (double)f[q][2];
+                const int n_it = nitem[nb[q]];
+                for (int w = 1; w < n_it; ++w) {          /* tiles split into several items */
+                    const float *blk = tj + (long)(nfirst[nb[q]] + w) * TJB + off[q];
+                    s0 += (double)blk[0];
+                    s1 += (double)blk[1];
+                    s2 += (double)blk[2];
+                }
+            }
+            const long o = gidx(g, i, j, k);
+            v.J[0][o] += s0;
+            v.J[1][o] += s1;
+            v.J[2][o] += s2;
         }
     }
 }

@@ -264,9 +264,14 @@ static int build_wall_ops(mpic_run *r)
 }
 
 /* ---------------- lifecycle ---------------- */
+/* Row pitch along k: n + 5 nodes rounded up to a multiple of 4, so that
+ * every row starts 16-byte aligned (the volume kernels move 16 bytes per
+ * access). */
+static int row_pitch(int n2) { return (n2 + 5 + 3) & ~3; }
+
 static size_t field_bytes(const int n[3], int fp32)
 {
-    const size_t cells = (size_t)(n[0] + 5) * (n[1] + 5) * (n[2] + 5);
+    const size_t cells = (size_t)(n[0] + 5) * (n[1] + 5) * row_pitch(n[2]);
     /* E H Hc Ec in the field precision, J in FP64 */
     return (fp32 ? sizeof(float) : sizeof(double)) * 12 * cells + sizeof(double) * 3 * cells;
 }
@@ -291,7 +296,7 @@ static long items_capacity(int ntiles, long cap, int chm)
 extern "C" int mpic_footprint(const mpic_config_t *cfg, size_t *bytes)
 {
     if (!cfg || !bytes) return MPIC_ERR_ARG;
-    const size_t cells = (size_t)(cfg->n[0] + 5) * (cfg->n[1] + 5) * (cfg->n[2] + 5);
+    const size_t cells = (size_t)(cfg->n[0] + 5) * (cfg->n[1] + 5) * row_pitch(cfg->n[2]);
     const size_t tiles = (size_t)((cfg->n[0] + 3) / 4) * ((cfg->n[1] + 3) / 4) * ((cfg->n[2] + 3) / 4);
     const size_t mk = (cfg->precision == MPIC_FP32_SPLIT) ? (4 + 7 * 4 + 4) : (6 * 8 + 2 * 4 + 4 + 4);
     const int fp32 = cfg->precision == MPIC_FP32_SPLIT;
@@ -347,7 +352,7 @@ extern "C" int mpic_create(const mpic_config_t *cfg, mpic_run_t **out)
     }
     g.tau = cfg->tau;
     g.GY = cfg->n[1] + 5;
-    g.GZ = cfg->n[2] + 5;
+    g.GZ = row_pitch(cfg->n[2]);
     g.ncell = (long)(cfg->n[0] + 5) * g.GY * g.GZ;
     r->fp32 = (cfg->precision == MPIC_FP32_SPLIT);
     const size_t fb = field_bytes(cfg->n, r->fp32);
@@ -601,7 +606,10 @@ extern "C" int mpic_get_J(mpic_run_t *r, double *J)
     std::vector<double> tmp((size_t)g.ncell);
     for (int c = 0; c < 3; ++c) {
         MPIC_CHECK(hipMemcpy(tmp.data(), r->jp.J[c], sizeof(double) * g.ncell, hipMemcpyDeviceToHost));
-        for (long q = 0; q < g.ncell; ++q) J[3 * q + c] = tmp[q];
+        long q = 0;
+        for (int i = -2; i <= g.n[0] + 2; ++i)
+            for (int j = -2; j <= g.n[1] + 2; ++j)
+                for (int k = -2; k <= g.n[2] + 2; ++k, ++q) J[3 * q + c] = tmp[gidx(g, i, j, k)];
     }
     return MPIC_OK;
 }
@@ -1590,9 +1598,15 @@ extern "C" int mpic_rho(mpic_run_t *r, double *rho)
     if (!r || !rho) return MPIC_ERR_ARG;
     MPIC_CHECK(hipSetDevice(r->dev));
     TRY(rho_into(r, r->rho2));
-    MPIC_CHECK(hipMemcpyAsync(rho, r->rho2, sizeof(double) * r->g.ncell, hipMemcpyDeviceToHost,
+    const Grid &g = r->g;
+    std::vector<double> tmp((size_t)g.ncell);
+    MPIC_CHECK(hipMemcpyAsync(tmp.data(), r->rho2, sizeof(double) * g.ncell, hipMemcpyDeviceToHost,
                               r->stream));
     MPIC_CHECK(hipStreamSynchronize(r->stream));
+    long q = 0;
+    for (int i = -2; i <= g.n[0] + 2; ++i)
+        for (int j = -2; j <= g.n[1] + 2; ++j)
+            for (int k = -2; k <= g.n[2] + 2; ++k, ++q) rho[q] = tmp[gidx(g, i, j, k)];
     return MPIC_OK;
 }
 
