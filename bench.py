@@ -80,12 +80,28 @@ def push_bytes(n_markers, plasma_cells, precision):
     FP64 plain form: marker read 56 B (x y z ux uy uz FP64, w qm FP32) and
     write 48 B; per cell holding markers E/H read 48 B and the J
     read-modify-write 48 B (the kernel flushes J itself).
-    FP32 split form: marker read 32 B (cell word, 3 offsets, 3 momenta, w)
-    and write 28 B; per cell holding markers E/H read 24 B.  Its J (48 B per
-    node, FP64 read-modify-write) is summed by the separate gather kernel."""
+    FP32 split form: marker read 32 B and write 32 B (two 16-byte records:
+    cell word + 3 offsets, 3 momenta + w); per cell holding markers E/H read
+    24 B.  Its J (48 B per node, FP64 read-modify-write) is summed by the
+    separate gather kernel."""
     if precision == "fp64":
         return n_markers * (56 + 48) + plasma_cells * (48 + 48)
-    return n_markers * (32 + 28) + plasma_cells * 24
+    return n_markers * (32 + 32) + plasma_cells * 24
+
+
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` on `workload` from the committed
+    counter summary (rocprofv3 FETCH_SIZE and WRITE_SIZE runs, each scaled
+    by the same counter's reading for a plain copy of known size), or None
+    when no summary covers this kernel and workload."""
+    path = os.path.join(ROOT, "profiles", "traffic_%s.json" % kernel)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        rec = json.load(f)
+    if rec.get("workload") != workload:
+        return None
+    return rec.get("bytes_per_launch")
 
 
 def run_gpu(ma, wl, precision, steps, warmup):
@@ -271,7 +287,8 @@ def main():
                    "rest_of_step_ms": m["rest_of_step_ms"]},
         "roofline": {"bound": "hbm", "kernel": kernel, "achieved": m["push_kernel_GBps"],
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": m["push_kernel_GBps"] / HBM_PEAK_GBS, "traffic": None},
+                     "frac": m["push_kernel_GBps"] / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(kernel, wl["name"])},
     }
     if "fp64" in res and main_prec != "fp64":
         f = res["fp64"]
