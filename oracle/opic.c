@@ -134,6 +134,21 @@ typedef struct {
     double ampl, omega, x, y, z, width, duration, front, plateau, Ey, Ez;
 } spot_t;
 
+/* One TFSF face (tfsfFace_t, source/core/em_TFSF.c:64-76): the played
+ * frames (nframes x volume, ep outer / eq inner) replace the file stream. */
+typedef struct {
+    int     comp, ea, ep, eq;
+    int     lo[3], hi[3];
+    long    vol;
+    double *data;          /* NULL: face opened or absent */
+} tface_t;
+
+/* TFSF playback state (source/core/em_TFSF.c:124-133). */
+typedef struct {
+    int     on, mode, last, frame;
+    tface_t E[12], H[12];
+} tfsf_t;
+
 struct opic_s {
     int    n[3];
     double h[3], L[3], tau, time;
@@ -148,6 +163,7 @@ struct opic_s {
     int    nlaser;
     laser_t laser[4];
     spot_t spot;
+    tfsf_t tfsf;
     int    inited;
 };
 
@@ -187,6 +203,10 @@ void opic_destroy(opic_t *s)
     for (int l = 0; l < s->nlaser; ++l) {
         free(s->laser[l].mapE);
         free(s->laser[l].mapH);
+    }
+    for (int f = 0; f < 12; ++f) {
+        free(s->tfsf.E[f].data);
+        free(s->tfsf.H[f].data);
     }
     free(s->pl);
     free(s);
@@ -692,11 +712,122 @@ static void flush_full(mesh_t *dst, cap_t *caps)
     }
 }
 
+/* TFSF_addSource / TFSF_subSource (source/core/em_TFSF.c:666-772): the
+ * cached frame of every active face, in face order, added (sign +1) or
+ * subtracted (sign -1) on the face's nodes inside the region tu. */
+static void tfsf_apply(mesh_t *F, const tface_t *faces, long frame, double sign, const box3 *tu)
+{
+    for (int f = 0; f < 12; ++f) {
+        const tface_t *fc = faces + f;
+        if (!fc->data) continue;
+        const int ea = fc->ea, ep = fc->ep, eq = fc->eq;
+        if (fc->lo[ea] > tu->hi[ea] || fc->hi[ea] < tu->lo[ea]) continue;
+        const int p1 = fc->lo[ep] < tu->lo[ep] ? tu->lo[ep] : fc->lo[ep];
+        const int p2 = fc->hi[ep] > tu->hi[ep] ? tu->hi[ep] : fc->hi[ep];
+        const int q1 = fc->lo[eq] < tu->lo[eq] ? tu->lo[eq] : fc->lo[eq];
+        const int q2 = fc->hi[eq] > tu->hi[eq] ? tu->hi[eq] : fc->hi[eq];
+        const long dQ = fc->hi[eq] - fc->lo[eq] + 1;
+        const double *buf = fc->data + frame * fc->vol;
+        int r[3];
+        r[ea] = fc->lo[ea];
+        for (int p = p1; p <= p2; ++p)
+            for (int q = q1; q <= q2; ++q) {
+                r[ep] = p;
+                r[eq] = q;
+                const double v = buf[(q - fc->lo[eq]) + (p - fc->lo[ep]) * dQ];
+                double *e = AT(F, r[0], r[1], r[2]) + fc->comp;
+                if (sign > 0)
+                    *e += v;
+                else
+                    *e -= v;
+            }
+    }
+}
+
+/* The interface is live while the frame counter F satisfies F + 1 < last - 1:
+ * at the start of every step cap_cacheEH calls TFSF_postEStep on the cap
+ * meshes (source/core/em_caps.c:607), which switches the interface off once
+ * F + 1 >= last - 1 (source/core/em_TFSF.c:838-841) before any source of that
+ * step is applied. */
+static int tfsf_live(const tfsf_t *t) { return t->on && t->frame + 1 < t->last - 1; }
+
+/* TFSF_postHStep on the inner mesh (source/core/em_TFSF.c:812-825, called
+ * from em_HStep, source/core/em.c:125): H faces play frame F (forward) or
+ * last - 1 - F (backward), always added. */
+static void tfsf_H(opic_t *s, const box3 *inner)
+{
+    const tfsf_t *t = &s->tfsf;
+    if (!tfsf_live(t)) return;
+    const long fr = t->mode == 0 ? t->frame : t->last - 1 - t->frame;
+    tfsf_apply(&s->H, t->H, fr, 1.0, inner);
+}
+
+/* TFSF_postEStep on the inner mesh (source/core/em_TFSF.c:832-850, called
+ * from em_EStep_finish, source/core/em.c:200): E faces play frame F + 1
+ * (forward, added) or last - 1 - F (backward, subtracted). */
+static void tfsf_E(opic_t *s, const box3 *inner)
+{
+    const tfsf_t *t = &s->tfsf;
+    if (!tfsf_live(t)) return;
+    if (t->mode == 0)
+        tfsf_apply(&s->E, t->E, t->frame + 1, 1.0, inner);
+    else
+        tfsf_apply(&s->E, t->E, t->last - 1 - t->frame, -1.0, inner);
+}
+
+int opic_add_tfsf(opic_t *s, const int reg[6], int mode, int nframes,
+                  const double *const dataE[12], const double *const dataH[12])
+{
+    /* TFSF_init, source/core/em_TFSF.c:481-545 (all axes active) */
+    tfsf_t *t = &s->tfsf;
+    if (t->on || (mode != 0 && mode != 2) || nframes < 1) return -1;
+    const int side[2][3] = {{reg[0], reg[2], reg[4]}, {reg[1], reg[3], reg[5]}};
+    for (int ea = 0; ea < 3; ++ea) {
+        const int ep = (ea + 1) % 3, eq = (ea + 2) % 3;
+        for (int top = 0; top < 2; ++top)
+            for (int comp = 0; comp < 2; ++comp) {
+                const int f = ea * 4 + top * 2 + comp;
+                tface_t *fe = t->E + f, *fh = t->H + f;
+                for (int a = 0; a < 3; ++a) {
+                    fe->lo[a] = side[0][a];
+                    fe->hi[a] = side[1][a];
+                }
+                fe->lo[ea] = fe->hi[ea] = side[top][ea];
+                fe->comp = (ea + 1 + comp) % 3;
+                fe->lo[fe->comp] += 1;               /* E nodes of that component */
+                fe->ea = ea;
+                fe->ep = ep;
+                fe->eq = eq;
+                *fh = *fe;
+                fh->lo[ea] += top;                   /* H nodes of the face */
+                fh->hi[ea] += top;
+                fh->comp = 3 - fe->comp - ea;
+                fe->vol = (long)(fe->hi[ep] - fe->lo[ep] + 1) * (fe->hi[eq] - fe->lo[eq] + 1);
+                fh->vol = fe->vol;
+                fe->data = fh->data = NULL;
+                if (dataE && dataE[f]) {
+                    fe->data = (double *)malloc(sizeof(double) * fe->vol * (size_t)nframes);
+                    memcpy(fe->data, dataE[f], sizeof(double) * fe->vol * (size_t)nframes);
+                }
+                if (dataH && dataH[f]) {
+                    fh->data = (double *)malloc(sizeof(double) * fh->vol * (size_t)nframes);
+                    memcpy(fh->data, dataH[f], sizeof(double) * fh->vol * (size_t)nframes);
+                }
+            }
+    }
+    t->mode = mode;
+    t->last = nframes;
+    t->frame = (int)(s->time / s->tau + 0.1);        /* TFSF_turnPlay, :466 */
+    t->on = 1;
+    return 0;
+}
+
 static void em_HStep(opic_t *s)
 {
     em_HSmallStep(s);
     flush_full(&s->H, s->capH);                     /* cap_flushH */
     box3 inner = box_make(3, 3, 3, s->n[0] - 2, s->n[1] - 2, s->n[2] - 2);
+    tfsf_H(s, &inner);
     add_H_sources(s, &s->H, s->time, &inner);
 }
 
@@ -774,6 +905,8 @@ int opic_add_gauss_spot(opic_t *s, const double p[11])
 static void em_EStep_finish(opic_t *s)
 {
     flush_full(&s->E, s->capE);                     /* cap_flushE */
+    box3 innerE = box_make(2, 2, 2, s->n[0] - 2, s->n[1] - 2, s->n[2] - 2);
+    tfsf_E(s, &innerE);
     box3 dom = box_make(-1, -1, -1, s->n[0] + 1, s->n[1] + 1, s->n[2] + 1);
     add_E_sources(s, &s->E, s->time, &dom);
     for (int i = -1; i <= s->n[0] + 1; i++)
@@ -786,6 +919,7 @@ static void em_EStep_finish(opic_t *s)
                 e[2] -= s->tau * jj[2];
             }
     spot_force(s, s->time + s->tau);                /* source/core/em.c:216 */
+    if (s->tfsf.on) ++s->tfsf.frame;                 /* TFSF_completeFrame, :857-860 */
 }
 
 /* em_energy, source/core/em.c:226-252 */

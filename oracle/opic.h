@@ -55,6 +55,16 @@ int     opic_add_laser(opic_t *s, double A, double T0, double delay,
  * plateau, Ey flag, Ez flag} (source/core/em_gaussSpot.c:42-55). */
 int     opic_add_gauss_spot(opic_t *s, const double p[11]);
 
+/* TFSF interface playback (source/core/em_TFSF.c): reg = TFSF.cfg's box
+ * {imin, imax, jmin, jmax, kmin, kmax}, mode 0 playForward / 2 playBackward,
+ * nframes = the recorded frame count (lastFrameTime.bin).  dataE[f] holds the
+ * frames played onto E face f (the recording of the H face of slot f),
+ * dataH[f] those played onto H face f (the recording of E face f), each
+ * nframes x face nodes doubles in file order; NULL = face opened.  Slot f =
+ * axis*4 + top*2 + comp, as mf_face (source/core/em_TFSF.c:79). */
+int     opic_add_tfsf(opic_t *s, const int reg[6], int mode, int nframes,
+                      const double *const dataE[12], const double *const dataH[12]);
+
 int     opic_init(opic_t *s);              /* em_init + plasma_init */
 /* One main-loop iteration, source/core/main.c:353-476.
  * diag (may be NULL) receives {Time, W_E, W_M, WTx, WTy, WTz} as passed to

@@ -101,6 +101,24 @@ CASES = {
         laser=None,
         spot=dict(ampl=2.0, frequency=1.0, x=0.4, y=0.31, z=0.28, width=0.1,
                   duration=0.12, front=0.15, plateau=0.05, Ey=1, Ez=-1)),
+    # Mur box with a TFSF interface played forward from synthetic face
+    # recordings (source/core/em_TFSF.c); 15 frames, so the interface
+    # switches itself off after 13 of the 20 steps.
+    "tfsf": dict(
+        n=(24, 20, 18), L=(1.2, 1.0, 0.9), bc=(2, 2, 2, 2, 2, 2),
+        species=[dict(sub=(1, 1, 1, 1), rho=-0.5, qm=-1.0,
+                      u=(0, 0, 0), du=(0.4, 0.4, 0.4))],
+        laser=None,
+        tfsf=dict(reg=(7, 16, 6, 13, 6, 11), mode=0, frames=15, opened=(0, 0, 0, 0, 0, 0),
+                  seed=101)),
+    # The same box played backward (sources of E subtracted), z faces opened.
+    "tfsf_back": dict(
+        n=(24, 20, 18), L=(1.2, 1.0, 0.9), bc=(2, 2, 2, 2, 2, 2),
+        species=[dict(sub=(1, 1, 1, 1), rho=-0.5, qm=-1.0,
+                      u=(0, 0, 0), du=(0.4, 0.4, 0.4))],
+        laser=None,
+        tfsf=dict(reg=(7, 16, 6, 13, 6, 11), mode=2, frames=40, opened=(0, 0, 0, 0, 1, 1),
+                  seed=202)),
     # x periodic, y mirror, z Mur: every wall kind and their edges.
     "mixed": dict(
         n=(10, 12, 9), L=(0.5, 0.6, 0.405), bc=(0, 1, 2, 0, 1, 2),
@@ -149,6 +167,91 @@ def laser_map(n, h, tau, lz):
     # Hy[I] += -c1*Ez^ (zero for y polarisation), Hz[I] += +c1*Ey^
     return np.ascontiguousarray(np.stack([mapE.reshape(-1, 4),
                                           mapH.reshape(-1, 4)]))
+
+
+# Face file names in slot order axis*4 + top*2 + comp (tfsfE / tfsfH of
+# source/core/em_TFSF.c:82-123).
+TFSF_E_NAMES = ("Ey_xMin", "Ez_xMin", "Ey_xMax", "Ez_xMax", "Ez_yMin", "Ex_yMin",
+                "Ez_yMax", "Ex_yMax", "Ex_zMin", "Ey_zMin", "Ex_zMax", "Ey_zMax")
+TFSF_H_NAMES = ("Hz_xMin", "Hy_xMin", "Hz_xMax", "Hy_xMax", "Hx_yMin", "Hz_yMin",
+                "Hx_yMax", "Hz_yMax", "Hy_zMin", "Hx_zMin", "Hy_zMax", "Hx_zMax")
+
+
+def tfsf_faces(reg):
+    """Node boxes (lo, hi) of the 12 E faces and the 12 H faces of an
+    interface box reg = (imin, imax, jmin, jmax, kmin, kmax), following
+    TFSF_init (source/core/em_TFSF.c:500-537)."""
+    side = ((reg[0], reg[2], reg[4]), (reg[1], reg[3], reg[5]))
+    fe, fh = [], []
+    for ea in range(3):
+        for top in range(2):
+            for comp in range(2):
+                lo, hi = list(side[0]), list(side[1])
+                lo[ea] = hi[ea] = side[top][ea]
+                c = (ea + 1 + comp) % 3
+                lo[c] += 1
+                fe.append((tuple(lo), tuple(hi)))
+                lo[ea] += top
+                hi[ea] += top
+                fh.append((tuple(lo), tuple(hi)))
+    return fe, fh
+
+
+def box_volume(b):
+    lo, hi = b
+    return int(np.prod([hi[a] - lo[a] + 1 for a in range(3)]))
+
+
+def write_tfsf(wd, cfg, h, tau):
+    """Synthetic recordings in the play layout (reg_t header of 11 int32,
+    then frames x face nodes doubles, source/core/em_TFSF.c:630-659) and the
+    control files core.out reads in play mode (:400-470).  Returns the
+    arrays handed to the oracle and the GPU."""
+    t = cfg["tfsf"]
+    reg, nf = t["reg"], t["frames"]
+    rng = np.random.default_rng(t["seed"])
+    fe, fh = tfsf_faces(reg)
+    d = os.path.join(wd, ".EM_sources/TFSF")
+    dataE, dataH = [], []
+    opened = t["opened"]                    # xMin xMax yMin yMax zMin zMax
+    for f in range(12):
+        ea, top = f // 4, (f // 2) % 2
+        vol = box_volume(fe[f])
+        # a smooth face pattern times a random amplitude per frame
+        a = rng.standard_normal(nf)[:, None] * 0.05
+        b = rng.standard_normal(vol)[None, :] * 0.5 + 1.0
+        e_src = (a * b).reshape(-1)                         # played onto E face f
+        a = rng.standard_normal(nf)[:, None] * 0.05
+        b = rng.standard_normal(vol)[None, :] * 0.5 + 1.0
+        h_src = (a * b).reshape(-1)                         # played onto H face f
+        off = opened[2 * ea + top]
+        dataE.append(None if off else e_src)
+        dataH.append(None if off else h_src)
+        for name, box, arr in ((TFSF_H_NAMES[f], fe[f], e_src), (TFSF_E_NAMES[f], fh[f], h_src)):
+            hdr = np.zeros(11, "<i4")
+            hdr[0:3], hdr[3:6] = box[0], box[1]
+            with open(os.path.join(d, name + "_99999.bin"), "wb") as fp:
+                fp.write(hdr.tobytes())
+                fp.write(arr.astype("<f8").tobytes())
+    # every value line carries a comment: cfg_readDouble's "@ %le %*[^\\n] "
+    # (source/misc_cfgReader.c:62) would otherwise swallow the next line
+    lines = ["@ %d    bounding box" % v for v in reg] + ["@ %d    mode" % t["mode"]]
+    open(os.path.join(d, "TFSF.cfg"), "w").write("\n".join(lines) + "\n")
+    lines = ["@ %d    interface size" % (reg[2 * a + 1] - reg[2 * a]) for a in range(3)]
+    lines += ["@ %.17e    time step" % tau] + ["@ %.17e    spatial step" % v for v in h]
+    open(os.path.join(d, "consistencyCheck.dat"), "w").write("\n".join(lines) + "\n")
+    np.array([nf], "<i4").tofile(os.path.join(d, "lastFrameTime.bin"))
+    if any(opened):
+        lines = ["@ %d    face opened" % v for v in opened]
+        open(os.path.join(d, "openFaces.cfg"), "w").write("\n".join(lines) + "\n")
+    vols = np.array([box_volume(b) for b in fe], dtype=np.int64)
+    return dict(tfsf_reg=np.array(reg, dtype=np.int32), tfsf_mode=np.array([t["mode"]]),
+                tfsf_frames=np.array([nf]), tfsf_vol=vols,
+                tfsf_on=np.array([0 if x is None else 1 for x in dataE], dtype=np.int32),
+                tfsf_E=np.concatenate([x if x is not None else np.zeros(nf * v)
+                                       for x, v in zip(dataE, vols)]),
+                tfsf_H=np.concatenate([x if x is not None else np.zeros(nf * v)
+                                       for x, v in zip(dataH, vols)]))
 
 
 def read_mesh(path, n):
@@ -244,6 +347,8 @@ def make_case(name, cfg, workroot):
         lines += ["@ %d   Ey" % sp["Ey"], "@ %d   Ez" % sp["Ez"]]
         open(os.path.join(wd, ".gaussSpot.cfg"), "w").write("\n".join(lines) + "\n")
         out["spot_params"] = np.array([sp[k] for k in keys] + [sp["Ey"], sp["Ez"]], dtype=float)
+    if cfg.get("tfsf") is not None:
+        out.update(write_tfsf(wd, cfg, d7[:3], d7[6]))
     # setup.out seeds rand() from the clock (source/misc_partition.c:255):
     # the Gauss build runs on a copy of the same record 0.
     wg = os.path.join(workroot, name + "_gauss")

@@ -45,6 +45,7 @@ def lib():
         L.opic_get_markers.argtypes = [P, P, I]
         L.opic_add_laser.argtypes = [P, D, D, D, D, D, I, I, P]
         L.opic_add_gauss_spot.argtypes = [P, P]
+        L.opic_add_tfsf.argtypes = [P, P, I, I, P, P]
         L.opic_init.argtypes = [P]
         L.opic_step.argtypes = [P, P, P]
         L.opic_time.restype = D
@@ -120,6 +121,17 @@ class Oracle:
         assert p.size == 11
         assert lib().opic_add_gauss_spot(self.s, _ptr(p)) == 0
 
+    def add_tfsf(self, reg, mode, nframes, dataE, dataH):
+        """dataE / dataH: 12 arrays (or None for an opened face) of the frames
+        played onto E / H face slot f."""
+        self._tfsf = [np.ascontiguousarray(x, dtype=np.float64) if x is not None else None
+                      for x in list(dataE) + list(dataH)]
+        arr = (ctypes.c_void_p * 12)
+        pe = arr(*[x.ctypes.data if x is not None else None for x in self._tfsf[:12]])
+        ph = arr(*[x.ctypes.data if x is not None else None for x in self._tfsf[12:]])
+        r = np.ascontiguousarray(reg, dtype=np.int32)
+        assert lib().opic_add_tfsf(self.s, _ptr(r), int(mode), int(nframes), pe, ph) == 0
+
     def init(self):
         rc = lib().opic_init(self.s)
         assert rc == 0, rc
@@ -159,6 +171,18 @@ def load_fixture(name):
     return dict(np.load(path, allow_pickle=False))
 
 
+def tfsf_arrays(fx):
+    """(reg, mode, nframes, dataE, dataH) of a fixture's TFSF recordings,
+    split per face slot; opened faces are None."""
+    nf = int(fx["tfsf_frames"][0])
+    vols = fx["tfsf_vol"]
+    offs = np.concatenate([[0], np.cumsum(vols * nf)])
+    on = fx["tfsf_on"]
+    dataE = [fx["tfsf_E"][offs[f]:offs[f + 1]] if on[f] else None for f in range(12)]
+    dataH = [fx["tfsf_H"][offs[f]:offs[f + 1]] if on[f] else None for f in range(12)]
+    return fx["tfsf_reg"], int(fx["tfsf_mode"][0]), nf, dataE, dataH
+
+
 def oracle_from_fixture(fx):
     o = Oracle(fx["n"], fx["h"], fx["L"], fx["tau"][0], fx["bc"], 0.0)
     o.set_fields(fx["E0"], fx["H0"])
@@ -167,5 +191,7 @@ def oracle_from_fixture(fx):
         o.add_laser(fx["laser_params"], fx["laser_map"])
     if "spot_params" in fx:
         o.add_gauss_spot(fx["spot_params"])
+    if "tfsf_reg" in fx:
+        o.add_tfsf(*tfsf_arrays(fx))
     o.init()
     return o

@@ -14,7 +14,7 @@ import pytest
 from oracle_lib import (field_digest, load_fixture, marker_digest,
                         oracle_from_fixture)
 
-CASES = ["periodic", "laser_mur", "mixed", "spot"]
+CASES = ["periodic", "laser_mur", "mixed", "spot", "tfsf", "tfsf_back"]
 
 
 @pytest.mark.parametrize("case", CASES)
@@ -50,3 +50,43 @@ def test_diagnostics_match_reference_files(case):
     np.testing.assert_allclose(rows, ref, rtol=6e-7, atol=0)
     g = fx["gauss"][:, 1:]
     np.testing.assert_allclose(np.array(gs), g, rtol=6e-7, atol=0)
+
+
+def _tfsf_first_mismatch(fx, dataE, dataH, mode=None, nframes=None):
+    """First record (1-based) whose E or H digest differs from the
+    reference's when the oracle plays the given face data, or 0."""
+    from oracle_lib import Oracle, tfsf_arrays
+    reg, m, nf, _, _ = tfsf_arrays(fx)
+    o = Oracle(fx["n"], fx["h"], fx["L"], fx["tau"][0], fx["bc"], 0.0)
+    o.set_fields(fx["E0"], fx["H0"])
+    o.set_markers(fx["M0"])
+    if dataE is not None:
+        o.add_tfsf(reg, m if mode is None else mode, nf if nframes is None else nframes,
+                   dataE, dataH)
+    o.init()
+    for r in range(1, len(fx["digest_E"]) + 1):
+        o.step()
+        E, H = o.fields()
+        if field_digest(E) != fx["digest_E"][r - 1] or field_digest(H) != fx["digest_H"][r - 1]:
+            return r
+    return 0
+
+
+@pytest.mark.parametrize("case", ["tfsf", "tfsf_back"])
+def test_tfsf_fixture_pins_the_interface(case):
+    """The TFSF fixtures are sensitive to each plausible mistake: no
+    interface, E and H recordings swapped, the play direction reversed, and
+    an interface that stays live one frame longer (forward case: the cap
+    call that switches it off at the start of a step)."""
+    from oracle_lib import tfsf_arrays
+    fx = load_fixture(case)
+    _, mode, nf, dataE, dataH = tfsf_arrays(fx)
+    assert _tfsf_first_mismatch(fx, dataE, dataH) == 0
+    assert _tfsf_first_mismatch(fx, None, None) == 1
+    assert _tfsf_first_mismatch(fx, dataH, dataE) == 1
+    assert _tfsf_first_mismatch(fx, dataE, dataH, mode=2 - mode) == 1
+    if case == "tfsf":
+        # one frame more keeps the interface live through step 14 (F = 13)
+        ext = [None if x is None else np.concatenate([x, x[-x.size // nf:]]) for x in dataE]
+        exth = [None if x is None else np.concatenate([x, x[-x.size // nf:]]) for x in dataH]
+        assert _tfsf_first_mismatch(fx, ext, exth, nframes=nf + 1) == 14
