@@ -106,6 +106,20 @@ int  mpic_add_laser(mpic_run_t *run, double A, double T0, double delay,
  * plane x/h + 1/2 are set at the end of every E step. */
 int  mpic_add_gauss_spot(mpic_run_t *run, const double *p);
 
+/* TFSF interface playback, replacing TFSF_init/TFSF_turnPlay and the
+ * post-step source terms (source/core/em_TFSF.c:400-545, 666-860) for one
+ * GPU: reg = the interface box of TFSF.cfg {imin, imax, jmin, jmax, kmin,
+ * kmax}; mode 0 playForward or 2 playBackward; nframes = the recorded frame
+ * count (lastFrameTime.bin).  dataE[f] are the frames played onto E face f
+ * (the recording of the H face of the same slot), dataH[f] those played onto
+ * H face f (the recording of E face f), each nframes x face nodes doubles in
+ * the file's order (tangential axis ep outer, eq inner); NULL = face opened
+ * (openFaces.cfg).  Slot f = axis*4 + top*2 + comp (mf_face, :79).  The
+ * recording is copied to device memory; faces must keep 5 cells from the
+ * domain edges.  Add before mpic_init. */
+int  mpic_add_tfsf(mpic_run_t *run, const int reg[6], int mode, int nframes,
+                   const double *const dataE[12], const double *const dataH[12]);
+
 /* em_init + plasma_init (caps, Mur buffers, ghost nodes, walls applied to
  * the loaded markers). */
 int  mpic_init(mpic_run_t *run);

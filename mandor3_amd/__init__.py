@@ -60,6 +60,7 @@ def lib():
             "mpic_marker_count": (LG, [P]),
             "mpic_add_laser": (I, [P, D, D, D, D, D, I, I, P]),
             "mpic_add_gauss_spot": (I, [P, P]),
+            "mpic_add_tfsf": (I, [P, P, I, I, P, P]),
             "mpic_init": (I, [P]),
             "mpic_em_h_half": (I, [P]),
             "mpic_em_energy": (I, [P, P, P]),
@@ -216,6 +217,19 @@ class Run:
         if p.size != 11:
             raise ValueError("11 spot parameters expected")
         _check(lib().mpic_add_gauss_spot(self.handle, _ptr(p)), "mpic_add_gauss_spot")
+
+    def add_tfsf(self, reg, mode, nframes, dataE, dataH):
+        """TFSF interface playback: reg = (imin, imax, jmin, jmax, kmin, kmax),
+        mode 0 forward / 2 backward, dataE / dataH = 12 arrays each (None for
+        an opened face) of the frames played onto E / H face slot f."""
+        self._tfsf = [np.ascontiguousarray(x, dtype=np.float64) if x is not None else None
+                      for x in list(dataE) + list(dataH)]
+        arr = ctypes.c_void_p * 12
+        pe = arr(*[x.ctypes.data if x is not None else None for x in self._tfsf[:12]])
+        ph = arr(*[x.ctypes.data if x is not None else None for x in self._tfsf[12:]])
+        r = np.ascontiguousarray(reg, dtype=np.int32)
+        _check(lib().mpic_add_tfsf(self.handle, _ptr(r), int(mode), int(nframes), pe, ph),
+               "mpic_add_tfsf")
 
     def set_graphs(self, on=True):
         """Replay captured HIP graphs of the step in advance()."""

@@ -486,6 +486,67 @@ int launch_spot(hipStream_t s, T *Ey, T *Ez, const Grid &g, int i, const double 
     return MPIC_OK;
 }
 
+/* TFSF_addSource / TFSF_subSource (source/core/em_TFSF.c:666-772) for all
+ * twelve faces of one kind in one launch: blockIdx.y is the face, a thread
+ * one node of it.  Faces of the same component meet along the interface's
+ * edges, and the reference adds them one face after the other, so a node
+ * shared by several faces is updated by the thread of the first of them,
+ * which adds every face's value in face order (bit-exact in FP64).  The
+ * frame played this step comes from the host-filled table (slot 0: H
+ * faces, 1: E faces; -1: interface off). */
+__device__ static inline bool tfsf_has(const TfsfFace &f, const int r[3])
+{
+    return f.off >= 0 && r[0] >= f.lo[0] && r[0] <= f.hi[0] && r[1] >= f.lo[1] &&
+           r[1] <= f.hi[1] && r[2] >= f.lo[2] && r[2] <= f.hi[2];
+}
+
+template <class T>
+__global__ void k_tfsf(T *F0, T *F1, T *F2, Grid g, TfsfSet fs, const double *data,
+                       const int *tab, const int *step, int cap, int slot, double sign)
+{
+    const int f = blockIdx.y;
+    const TfsfFace &fc = fs.f[f];
+    if (fc.off < 0) return;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= fc.vol) return;
+    const int frame = tab[2 * (*step % cap) + slot];
+    if (frame < 0) return;
+    int r[3];
+    r[fc.ea] = fc.lo[fc.ea];
+    r[fc.ep] = fc.lo[fc.ep] + (int)(t / fc.dQ);
+    r[fc.eq] = fc.lo[fc.eq] + (int)(t % fc.dQ);
+    for (int a = 0; a < 3; ++a)
+        if (r[a] < fs.lo_in || r[a] > g.n[a] - 2) return;
+    for (int q = 0; q < f; ++q)
+        if (fs.f[q].comp == fc.comp && tfsf_has(fs.f[q], r)) return;   /* an earlier face owns it */
+    T *F = fc.comp == 0 ? F0 : (fc.comp == 1 ? F1 : F2);
+    const long o = gidx(g, r[0], r[1], r[2]);
+    double v = F[o];
+    for (int q = f; q < 12; ++q) {
+        const TfsfFace &h = fs.f[q];
+        if (h.comp != fc.comp || !tfsf_has(h, r)) continue;
+        const double d = data[h.off + (long)frame * h.vol + (long)(r[h.ep] - h.lo[h.ep]) * h.dQ +
+                              (r[h.eq] - h.lo[h.eq])];
+        if (sign > 0)
+            v += d;
+        else
+            v -= d;
+    }
+    F[o] = (T)v;
+}
+
+template <class T>
+int launch_tfsf(hipStream_t s, T *const F[3], const Grid &g, const TfsfSet &fs, long maxvol,
+                const double *data, const int *tab, const int *step, int cap, int slot,
+                double sign)
+{
+    if (maxvol <= 0) return MPIC_OK;
+    k_tfsf<T><<<dim3(nblk(maxvol), 12), TPB, 0, s>>>(F[0], F[1], F[2], g, fs, data, tab, step,
+                                                     cap, slot, sign);
+    MPIC_CHECK(hipGetLastError());
+    return MPIC_OK;
+}
+
 /* The run's step counter, advanced once per step on the stream. */
 __global__ void k_tick(int *step)
 {
@@ -723,6 +784,8 @@ int launch_gauss(hipStream_t s, const FieldsT<T> &f, const double *rho1, const d
     template int launch_spot<T>(hipStream_t, T *, T *, const Grid &, int, const double *,        \
                                 const double *, const double *, const int *, int);              \
     template int launch_energy<T>(hipStream_t, const FieldsT<T> &, const Grid &, double *);      \
+    template int launch_tfsf<T>(hipStream_t, T *const[3], const Grid &, const TfsfSet &, long,   \
+                                const double *, const int *, const int *, int, int, double);   \
     template int launch_gauss<T>(hipStream_t, const FieldsT<T> &, const double *,                \
                                  const double *, const Grid &, double *);
 INST(double)

@@ -67,6 +67,21 @@ struct FieldsT {
 typedef FieldsT<double> Fields64;
 typedef FieldsT<float>  Fields32;
 
+/* TFSF interface faces of one kind (E or H), source/core/em_TFSF.c:64-76:
+ * the field component, the normal axis ea and tangential axes ep (outer) /
+ * eq (inner) of the played frames, the node box, and the offset of the
+ * face's frames in the device data (off < 0: face opened or absent). */
+struct TfsfFace {
+    int  comp, ea, ep, eq;
+    int  lo[3], hi[3];
+    int  dQ;
+    long vol, off;
+};
+struct TfsfSet {
+    TfsfFace f[12];
+    int      lo_in;               /* sources act on nodes [lo_in, n-2] per axis */
+};
+
 struct JPtrs {
     double *J[3];
 };
@@ -143,6 +158,10 @@ int launch_laser(hipStream_t s, T *Fy, T *Fz, const Grid &g, const double *map, 
                  int JMIN, int KMIN, int KMAX, int jmin, int jmax, int kmin, int kmax,
                  const double *tab, const int *step, int cap);
 int launch_tick(hipStream_t s, int *step);
+template <class T>
+int launch_tfsf(hipStream_t s, T *const F[3], const Grid &g, const TfsfSet &fs, long maxvol,
+                const double *data, const int *tab, const int *step, int cap, int slot,
+                double sign);
 template <class T>
 int launch_spot(hipStream_t s, T *Ey, T *Ez, const Grid &g, int i, const double *py,
                 const double *pz, const double *tab, const int *step, int cap);

@@ -70,6 +70,18 @@ struct Spot {
     double *htab;             /* pinned staging of dtab */
 };
 
+/* TFSF interface playback: the whole recording lives in HBM; the frame of
+ * each step comes from a host-filled table like the laser's. */
+struct Tfsf {
+    int     on, mode, last, frame0;
+    long    step0;                 /* run step at which the frame counter was frame0 */
+    TfsfSet fe, fh;                /* E faces (inner nodes from 2), H faces (from 3) */
+    long    maxvol;
+    double *ddata;
+    int    *dtab;                  /* device: per step {H frame, E frame}, -1 off */
+    int    *htab;                  /* pinned staging of dtab */
+};
+
 struct CapBoxes {             /* one field's six caps */
     Box upd[6], fl[6];        /* original boxes */
     Box upd_s[6], fl_s[6];    /* after overlap removal */
@@ -118,6 +130,7 @@ struct mpic_run {
     std::vector<hipEvent_t> evpool;  /* 5 events per timed step */
     int          *dstep;      /* device step counter (indexes the laser tables) */
     Spot          spot;
+    Tfsf          tfsf;
     long          step;       /* host copy of the step counter */
     long          tab_lo, tab_hi;  /* steps the laser tables hold */
     int           use_graphs; /* mpic_advance replays captured steps */
@@ -523,6 +536,9 @@ extern "C" int mpic_destroy(mpic_run_t *r)
     }
     hipFree(r->dstep);
     hipFree(r->spot.dprof);
+    hipFree(r->tfsf.ddata);
+    hipFree(r->tfsf.dtab);
+    if (r->tfsf.htab) hipHostFree(r->tfsf.htab);
     hipFree(r->spot.dtab);
     if (r->spot.htab) hipHostFree(r->spot.htab);
     for (int k = 0; k < 2; ++k)
@@ -939,6 +955,95 @@ extern "C" int mpic_add_gauss_spot(mpic_run_t *r, const double *p)
     return MPIC_OK;
 }
 
+extern "C" int mpic_add_tfsf(mpic_run_t *r, const int reg[6], int mode, int nframes,
+                             const double *const dataE[12], const double *const dataH[12])
+{
+    if (!r || !reg || !dataE || !dataH) return MPIC_ERR_ARG;
+    if (r->inited) {
+        mpic_set_error("mpic_add_tfsf: add sources before mpic_init");
+        return MPIC_ERR_STATE;
+    }
+    if (r->tfsf.on) {
+        mpic_set_error("mpic_add_tfsf: one interface per run");
+        return MPIC_ERR_STATE;
+    }
+    if ((mode != 0 && mode != 2) || nframes < 1) {
+        mpic_set_error("mpic_add_tfsf: mode must be 0 (playForward) or 2 (playBackward), "
+                       "frames >= 1");
+        return MPIC_ERR_ARG;
+    }
+    const Grid &g = r->g;
+    Tfsf &tf = r->tfsf;
+    /* face geometry, TFSF_init (source/core/em_TFSF.c:500-537) */
+    const int side[2][3] = {{reg[0], reg[2], reg[4]}, {reg[1], reg[3], reg[5]}};
+    long total = 0;
+    tf.maxvol = 0;
+    for (int ea = 0; ea < 3; ++ea) {
+        const int ep = (ea + 1) % 3, eq = (ea + 2) % 3;
+        for (int top = 0; top < 2; ++top)
+            for (int comp = 0; comp < 2; ++comp) {
+                const int f = ea * 4 + top * 2 + comp;
+                TfsfFace &fe = tf.fe.f[f], &fh = tf.fh.f[f];
+                for (int a = 0; a < 3; ++a) {
+                    fe.lo[a] = side[0][a];
+                    fe.hi[a] = side[1][a];
+                }
+                fe.lo[ea] = fe.hi[ea] = side[top][ea];
+                fe.comp = (ea + 1 + comp) % 3;
+                fe.lo[fe.comp] += 1;
+                fe.ea = ea;
+                fe.ep = ep;
+                fe.eq = eq;
+                fh = fe;
+                fh.lo[ea] += top;
+                fh.hi[ea] += top;
+                fh.comp = 3 - fe.comp - ea;
+                fe.dQ = fh.dQ = fe.hi[eq] - fe.lo[eq] + 1;
+                fe.vol = fh.vol = (long)(fe.hi[ep] - fe.lo[ep] + 1) * fe.dQ;
+                /* TFSF_turnPlay: faces must keep 5 cells from the domain
+                 * edges (source/core/em_TFSF.c:402-414) */
+                for (int a = 0; a < 3; ++a)
+                    if (fe.lo[a] < 5 || fe.hi[a] > g.n[a] - 5 || fh.lo[a] < 5 || fh.hi[a] > g.n[a] - 5) {
+                        mpic_set_error("TFSF interface is too close to a boundary (less than 5 cell)");
+                        return MPIC_ERR_ARG;
+                    }
+                if (fe.vol <= 0) {
+                    mpic_set_error("mpic_add_tfsf: empty interface box");
+                    return MPIC_ERR_ARG;
+                }
+                fe.off = dataE[f] ? total : -1;
+                total += dataE[f] ? fe.vol * nframes : 0;
+                fh.off = dataH[f] ? total : -1;
+                total += dataH[f] ? fh.vol * nframes : 0;
+                if (fe.vol > tf.maxvol) tf.maxvol = fe.vol;
+            }
+    }
+    tf.fe.lo_in = 2;                     /* innerRegE, source/core/em.c:197-198 */
+    tf.fh.lo_in = 3;                     /* innerRegH, source/core/em.c:122-123 */
+    MPIC_CHECK(hipSetDevice(r->dev));
+    if (total > 0) {
+        MPIC_CHECK(hipMalloc(&tf.ddata, sizeof(double) * total));
+        for (int f = 0; f < 12; ++f) {
+            if (dataE[f])
+                MPIC_CHECK(hipMemcpy(tf.ddata + tf.fe.f[f].off, dataE[f],
+                                     sizeof(double) * tf.fe.f[f].vol * nframes, hipMemcpyHostToDevice));
+            if (dataH[f])
+                MPIC_CHECK(hipMemcpy(tf.ddata + tf.fh.f[f].off, dataH[f],
+                                     sizeof(double) * tf.fh.f[f].vol * nframes, hipMemcpyHostToDevice));
+        }
+    }
+    MPIC_CHECK(hipMalloc(&tf.dtab, sizeof(int) * 2 * LASER_CAP));
+    MPIC_CHECK(hipHostMalloc(&tf.htab, sizeof(int) * 2 * LASER_CAP, hipHostMallocDefault));
+    tf.mode = mode;
+    tf.last = nframes;
+    tf.frame0 = (int)(r->time / g.tau + 0.1);   /* TFSF_turnPlay, :466 */
+    tf.step0 = r->step;
+    tf.on = 1;
+    r->tab_hi = r->tab_lo = 0;
+    r->graphs_ok = 0;
+    return MPIC_OK;
+}
+
 /* ---------------- stages ---------------- */
 /* The walls in list order; the min and max wall of one axis with at least
  * 8 cells (adjacent in the list, disjoint node layers) share a launch. */
@@ -1085,7 +1190,8 @@ static int check_stage(mpic_run *r, int want)
  * on the run's stream. */
 static int laser_tables(mpic_run *r, long n)
 {
-    if ((r->lasers.empty() && !r->spot.on) || (r->step >= r->tab_lo && r->step + n <= r->tab_hi))
+    if ((r->lasers.empty() && !r->spot.on && !r->tfsf.on) ||
+        (r->step >= r->tab_lo && r->step + n <= r->tab_hi))
         return MPIC_OK;
     if (n > LASER_CAP) {
         mpic_set_error("laser tables hold %d steps", LASER_CAP);
@@ -1133,6 +1239,23 @@ static int laser_tables(mpic_run *r, long n)
             time = time + g.tau;
         }
         MPIC_CHECK(hipMemcpyAsync(sp.dtab, sp.htab, sizeof(double) * 2 * LASER_CAP,
+                                  hipMemcpyHostToDevice, r->stream));
+    }
+    if (r->tfsf.on) {
+        /* TFSF_cacheFrame / post-steps (source/core/em_TFSF.c:780-850): at
+         * frame counter F the interface is live while F + 1 < last - 1 (the
+         * caps' TFSF_postEStep at the start of a step switches it off,
+         * source/core/em_caps.c:607); H faces play F (forward) or
+         * last - 1 - F, E faces F + 1 or last - 1 - F. */
+        const Tfsf &tf = r->tfsf;
+        for (long q = 0; q < LASER_CAP; ++q) {
+            const long F = tf.frame0 + (r->step + q - tf.step0);
+            const bool live = F >= 0 && F + 1 < tf.last - 1;   /* F < 0: before the recording */
+            int *e = tf.htab + 2 * ((r->step + q) % LASER_CAP);
+            e[0] = live ? (int)(tf.mode == 0 ? F : tf.last - 1 - F) : -1;
+            e[1] = live ? (int)(tf.mode == 0 ? F + 1 : tf.last - 1 - F) : -1;
+        }
+        MPIC_CHECK(hipMemcpyAsync(tf.dtab, tf.htab, sizeof(int) * 2 * LASER_CAP,
                                   hipMemcpyHostToDevice, r->stream));
     }
     r->tab_lo = r->step;
@@ -1260,6 +1383,15 @@ extern "C" int mpic_em_h_step(mpic_run_t *r)
     TRY(check_stage(r, 1));
     MPIC_CHECK(hipSetDevice(r->dev));
     FCALL(launch_h_update, r->g, r->flH, 1);
+    if (r->tfsf.on) {                    /* TFSF_postHStep, source/core/em.c:125 */
+        const Tfsf &tf = r->tfsf;
+        if (r->fp32)
+            TRY(launch_tfsf(r->stream, r->f32.H, r->g, tf.fh, tf.maxvol, tf.ddata, tf.dtab,
+                            r->dstep, LASER_CAP, 0, 1.0));
+        else
+            TRY(launch_tfsf(r->stream, r->f.H, r->g, tf.fh, tf.maxvol, tf.ddata, tf.dtab,
+                            r->dstep, LASER_CAP, 0, 1.0));
+    }
     Box inner = {{3, 3, 3}, {r->g.n[0] - 2, r->g.n[1] - 2, r->g.n[2] - 2}};
     if (r->fp32)
         TRY(laser_H(r, r->f32.H[1], r->f32.H[2], inner));
@@ -1295,6 +1427,16 @@ extern "C" int mpic_em_e_finish(mpic_run_t *r)
     if (!r) return MPIC_ERR_ARG;
     TRY(check_stage(r, 3));
     MPIC_CHECK(hipSetDevice(r->dev));
+    if (r->tfsf.on) {                    /* TFSF_postEStep, source/core/em.c:200 */
+        const Tfsf &tf = r->tfsf;
+        const double sign = tf.mode == 0 ? 1.0 : -1.0;
+        if (r->fp32)
+            TRY(launch_tfsf(r->stream, r->f32.E, r->g, tf.fe, tf.maxvol, tf.ddata, tf.dtab,
+                            r->dstep, LASER_CAP, 1, sign));
+        else
+            TRY(launch_tfsf(r->stream, r->f.E, r->g, tf.fe, tf.maxvol, tf.ddata, tf.dtab,
+                            r->dstep, LASER_CAP, 1, sign));
+    }
     Box dom = {{-1, -1, -1}, {r->g.n[0] + 1, r->g.n[1] + 1, r->g.n[2] + 1}};
     if (r->fp32)
         TRY(laser_E(r, r->f32.E[1], r->f32.E[2], dom));

@@ -15,11 +15,11 @@ import numpy as np
 import pytest
 
 import mandor3_amd as ma
-from oracle_lib import MARKER_DTYPE, Oracle, load_fixture, oracle_from_fixture
+from oracle_lib import MARKER_DTYPE, Oracle, load_fixture, oracle_from_fixture, tfsf_arrays
 
 pytestmark = pytest.mark.gpu
 
-CASES = ["periodic", "laser_mur", "mixed", "spot"]
+CASES = ["periodic", "laser_mur", "mixed", "spot", "tfsf", "tfsf_back"]
 
 
 def run32(fx, markers=True, E=None, H=None):
@@ -32,6 +32,8 @@ def run32(fx, markers=True, E=None, H=None):
         r.add_laser(fx["laser_params"], fx["laser_map"])
     if "spot_params" in fx:
         r.add_gauss_spot(fx["spot_params"])
+    if "tfsf_reg" in fx:
+        r.add_tfsf(*tfsf_arrays(fx))
     r.init()
     return r
 
@@ -66,6 +68,8 @@ def test_fp32_vacuum_fields(case):
         o.add_laser(fx["laser_params"], fx["laser_map"])
     if "spot_params" in fx:
         o.add_gauss_spot(fx["spot_params"])
+    if "tfsf_reg" in fx:
+        o.add_tfsf(*tfsf_arrays(fx))
     o.init()
     worst = 0.0
     for _ in range(12):

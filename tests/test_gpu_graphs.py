@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 
 import mandor3_amd as ma
-from oracle_lib import load_fixture
+from oracle_lib import load_fixture, tfsf_arrays
 from test_gpu_parity import by_id, cells, oracle, random_fields, rel_l2
 
 pytestmark = pytest.mark.gpu
@@ -28,6 +28,8 @@ def make(fx, precision, markers=True, E=None, H=None, graphs=False):
         r.add_laser(fx["laser_params"], fx["laser_map"])
     if "spot_params" in fx:
         r.add_gauss_spot(fx["spot_params"])
+    if "tfsf_reg" in fx:
+        r.add_tfsf(*tfsf_arrays(fx))
     r.init()
     if graphs:
         r.set_graphs(True)
@@ -50,7 +52,7 @@ def test_replay_vacuum_laser_bit_exact():
 
 
 @pytest.mark.parametrize("prec", [ma.FP64, ma.FP32_SPLIT])
-@pytest.mark.parametrize("case", ["laser_mur", "mixed", "spot"])
+@pytest.mark.parametrize("case", ["laser_mur", "mixed", "spot", "tfsf"])
 def test_replay_equals_stage_calls(case, prec):
     fx = load_fixture(case)
     a = make(fx, prec)

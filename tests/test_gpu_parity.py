@@ -11,11 +11,11 @@ import numpy as np
 import pytest
 
 import mandor3_amd as ma
-from oracle_lib import MARKER_DTYPE, Oracle, load_fixture, oracle_from_fixture
+from oracle_lib import MARKER_DTYPE, Oracle, load_fixture, oracle_from_fixture, tfsf_arrays
 
 pytestmark = pytest.mark.gpu
 
-CASES = ["periodic", "laser_mur", "mixed", "spot"]
+CASES = ["periodic", "laser_mur", "mixed", "spot", "tfsf", "tfsf_back"]
 
 
 def gpu_from_fixture(fx, markers=True, E=None, H=None):
@@ -28,6 +28,8 @@ def gpu_from_fixture(fx, markers=True, E=None, H=None):
         r.add_laser(fx["laser_params"], fx["laser_map"])
     if "spot_params" in fx:
         r.add_gauss_spot(fx["spot_params"])
+    if "tfsf_reg" in fx:
+        r.add_tfsf(*tfsf_arrays(fx))
     r.init()
     return r
 
@@ -40,6 +42,8 @@ def oracle(fx, markers=True, E=None, H=None):
         o.add_laser(fx["laser_params"], fx["laser_map"])
     if "spot_params" in fx:
         o.add_gauss_spot(fx["spot_params"])
+    if "tfsf_reg" in fx:
+        o.add_tfsf(*tfsf_arrays(fx))
     o.init()
     return o
 
