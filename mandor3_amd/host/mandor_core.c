@@ -5,7 +5,8 @@
  *
  * Reads run_mandor.cfg, binData/domain.bin, binData/units.bin, the start
  * record (record 0, or the last one with the continue flag), the laser
- * sources in .EM_sources/laser and the hard spot of .gaussSpot.cfg, then
+ * sources in .EM_sources/laser, the TFSF recordings in .EM_sources/TFSF
+ * (play modes) and the hard spot of .gaussSpot.cfg, then
  * runs the time step through the C ABI of include/mpic.h, writing
  * checkpoints, the energy history and honouring tmp/stop.flag as
  * source/core/main.c:195-490 does for one rank.
@@ -184,6 +185,16 @@ int main(int argc, char **argv)
         mh_cfg_free(&sc);
         GPU(mpic_add_gauss_spot(s.run, p), "mpic_add_gauss_spot");
     }
+    mh_tfsf_t tfsf;                                    /* source/core/em_TFSF.c:474-545 */
+    HOST(mh_tfsf_load(".EM_sources/TFSF", &dom, &tfsf), ".EM_sources/TFSF");
+    if (tfsf.mode == 0 || tfsf.mode == 2)
+        GPU(mpic_add_tfsf(s.run, tfsf.reg, tfsf.mode, tfsf.nframes,
+                          (const double *const *)tfsf.dataE, (const double *const *)tfsf.dataH),
+            "mpic_add_tfsf");
+    else if (tfsf.mode == 1)
+        fprintf(stderr, "mandor_core: TFSF record mode is not produced by this driver; "
+                        "the interface stays off\n");
+    mh_tfsf_free(&tfsf);
     GPU(mpic_init(s.run), "mpic_init");
     printf("mandor_core: %s form, %d x %d x %d cells, %ld markers, start record %d, t = %.6f\n",
            precision == MPIC_FP64 ? "FP64 plain" : "FP32 split", s.n[0], s.n[1], s.n[2], nm, point,

@@ -599,3 +599,120 @@ void mh_ranmar(mh_ranmar_t *r, double *out, int len)
         out[q] = uni;
     }
 }
+
+/* ---- TFSF recordings, source/core/em_TFSF.c ---------------------------- */
+
+/* Face file names by slot axis*4 + top*2 + comp (tfsfE / tfsfH,
+ * source/core/em_TFSF.c:82-123): the interface's file-name contract. */
+static const char *const TFSF_E_NAME[12] = {
+    "Ey_xMin", "Ez_xMin", "Ey_xMax", "Ez_xMax", "Ez_yMin", "Ex_yMin",
+    "Ez_yMax", "Ex_yMax", "Ex_zMin", "Ey_zMin", "Ex_zMax", "Ey_zMax"};
+static const char *const TFSF_H_NAME[12] = {
+    "Hz_xMin", "Hy_xMin", "Hz_xMax", "Hy_xMax", "Hx_yMin", "Hz_yMin",
+    "Hx_yMax", "Hz_yMax", "Hy_zMin", "Hx_zMin", "Hy_zMax", "Hx_zMax"};
+
+static int read_frames(const char *path, long count, double **out)
+{
+    FILE *fp = fopen(path, "rb");
+    if (!fp) return fail("cannot open '%s'", path);
+    double *d = (double *)malloc(sizeof(double) * (size_t)count);
+    int ok = d && fseek(fp, 11 * sizeof(int32_t), SEEK_SET) == 0 &&
+             fread(d, sizeof(double), (size_t)count, fp) == (size_t)count;
+    fclose(fp);
+    if (!ok) {
+        free(d);
+        return fail("'%s': fewer frames than lastFrameTime.bin promises", path);
+    }
+    *out = d;
+    return 0;
+}
+
+void mh_tfsf_free(mh_tfsf_t *t)
+{
+    for (int f = 0; f < 12; ++f) {
+        free(t->dataE[f]);
+        free(t->dataH[f]);
+        t->dataE[f] = t->dataH[f] = NULL;
+    }
+}
+
+int mh_tfsf_load(const char *dir, const mh_domain_t *d, mh_tfsf_t *t)
+{
+    memset(t, 0, sizeof(*t));
+    t->mode = -1;
+    char p[1024];
+    snprintf(p, sizeof(p), "%s/TFSF.cfg", dir);
+    FILE *probe = fopen(p, "rt");
+    if (!probe) return 0;                                    /* no interface */
+    fclose(probe);
+    int v[7];
+    if (read_first_ints(p, v, 7)) return fail("'%s': 7 integers expected", p);
+    memcpy(t->reg, v, sizeof(t->reg));
+    t->mode = v[6];
+    if (t->mode == 1) return 0;                              /* record: not played */
+    if (t->mode != 0 && t->mode != 2) return fail("'%s': unknown mode %d", p, t->mode);
+    snprintf(p, sizeof(p), "%s/TFSF_defrag.map", dir);     /* TFSF_init, :477-478 */
+    probe = fopen(p, "rt");
+    if (probe) {
+        fclose(probe);
+        return fail("TFSF input data is not compiled (use 'TFSFdefrag')");
+    }
+    /* consistencyCheck.dat: sizes, tau, h (TFSF_turnPlay, :432-453) */
+    snprintf(p, sizeof(p), "%s/consistencyCheck.dat", dir);
+    mh_cfg_t c;
+    if (mh_cfg_load(p, &c)) return fail("there are no recorded data ('%s')", p);
+    int sz[3] = {0, 0, 0};
+    double dt = 0, dh[3] = {0, 0, 0};
+    int rc = 0;
+    for (int a = 0; a < 3 && !rc; ++a) rc = mh_tag_int(&c.tags[0], a, sz + a);
+    if (!rc) rc = mh_tag_double(&c.tags[0], 3, &dt);
+    for (int a = 0; a < 3 && !rc; ++a) rc = mh_tag_double(&c.tags[0], 4 + a, dh + a);
+    mh_cfg_free(&c);
+    if (rc) return fail("'%s': malformed", p);
+    for (int a = 0; a < 3; ++a)
+        if (sz[a] != t->reg[2 * a + 1] - t->reg[2 * a])
+            return fail("inconsistent TFSF size along axis '%c'", 'X' + a);
+    if (!(fabs(1 - dt / d->tau) < 1e-11)) return fail("inconsistent TFSF time step");
+    for (int a = 0; a < 3; ++a)
+        if (!(fabs(1 - dh[a] / d->h[a]) < 1e-11))
+            return fail("inconsistent TFSF spatial step along %c", 'X' + a);
+    snprintf(p, sizeof(p), "%s/lastFrameTime.bin", dir);
+    FILE *fp = fopen(p, "rb");
+    int32_t nf = 0;
+    const int okf = fp && fread(&nf, sizeof(nf), 1, fp) == 1;
+    if (fp) fclose(fp);
+    if (!okf || nf < 1) return fail("'%s': no frame count", p);
+    t->nframes = nf;
+    /* opened faces (TFSF_openFaces, :339-371): xMin xMax yMin yMax zMin zMax */
+    int opened[6] = {0, 0, 0, 0, 0, 0};
+    snprintf(p, sizeof(p), "%s/openFaces.cfg", dir);
+    probe = fopen(p, "rt");
+    if (probe) {
+        fclose(probe);
+        if (read_first_ints(p, opened, 6)) return fail("'%s': 6 integers expected", p);
+    }
+    /* face boxes as TFSF_init (:500-537); E and H faces of a slot share the
+     * tangential extents, so one node count serves both recordings */
+    for (int ea = 0; ea < 3; ++ea) {
+        const int ep = (ea + 1) % 3, eq = (ea + 2) % 3;
+        for (int top = 0; top < 2; ++top)
+            for (int comp = 0; comp < 2; ++comp) {
+                const int f = ea * 4 + top * 2 + comp, cE = (ea + 1 + comp) % 3;
+                int lo[3] = {t->reg[0], t->reg[2], t->reg[4]}, hi[3] = {t->reg[1], t->reg[3], t->reg[5]};
+                lo[cE] += 1;
+                t->vol[f] = (long)(hi[ep] - lo[ep] + 1) * (hi[eq] - lo[eq] + 1);
+                if (opened[2 * ea + top]) continue;
+                snprintf(p, sizeof(p), "%s/%s_99999.bin", dir, TFSF_H_NAME[f]);
+                if (read_frames(p, t->vol[f] * nf, t->dataE + f)) {
+                    mh_tfsf_free(t);
+                    return -1;
+                }
+                snprintf(p, sizeof(p), "%s/%s_99999.bin", dir, TFSF_E_NAME[f]);
+                if (read_frames(p, t->vol[f] * nf, t->dataH + f)) {
+                    mh_tfsf_free(t);
+                    return -1;
+                }
+            }
+    }
+    return 0;
+}

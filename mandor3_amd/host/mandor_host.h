@@ -118,6 +118,24 @@ int  mh_laser_save(const char *path, const mh_domain_t *d, const mh_units_t *u,
                    double T0_fs, double phi0, int forward, const double *map);
 void mh_laser_free(mh_laser_t *l);
 
+/* .EM_sources/TFSF in play mode (source/core/em_TFSF.c:151-180 TFSF.cfg,
+ * :339-371 openFaces.cfg, :400-470 consistencyCheck.dat, lastFrameTime.bin
+ * and the defragmented recordings <face>_99999.bin: a reg_t header of 11
+ * int32, then frames x face nodes doubles).  mode -1: no TFSF.cfg (no
+ * interface); 1 (record) is reported, not loaded.  dataE[f] / dataH[f] are
+ * the frames played onto E / H face slot f (the recordings of the H / E
+ * face of that slot), NULL for opened faces: the arguments of
+ * mpic_add_tfsf. */
+typedef struct {
+    int     reg[6];           /* imin imax jmin jmax kmin kmax */
+    int     mode;             /* 0 playForward, 1 record, 2 playBackward, -1 none */
+    int     nframes;
+    long    vol[12];          /* nodes per face */
+    double *dataE[12], *dataH[12];
+} mh_tfsf_t;
+int  mh_tfsf_load(const char *dir, const mh_domain_t *d, mh_tfsf_t *t);
+void mh_tfsf_free(mh_tfsf_t *t);
+
 /* ---- RANMAR (source/misc_ranmar.c:78-146) ------------------------------ */
 
 /* Marsaglia-Zaman-Tsang lagged Fibonacci generator (97, 33) combined with

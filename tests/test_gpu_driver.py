@@ -220,3 +220,31 @@ def test_core_mur_laser_matches_oracle(tmp_path, host):  # noqa: F811
     # the laser is in the box: the plasma alone (|rho| = 3, u ~ 0.02) leaves
     # fields of order 1e-5 after 20 steps, the pulse's leading edge ~0.5
     assert np.abs(rec["E"]).max() > 1e-2
+
+
+def test_core_tfsf_play_matches_oracle(tmp_path):
+    """mandor_core plays a TFSF recording from .EM_sources/TFSF (forward,
+    15 frames, so the interface switches off within the 20 steps) and its
+    records match the oracle playing the same frames."""
+    import tfsf_files
+    wd = str(tmp_path / "run")
+    setup_dir(wd, MUR_DECK.split("[laser]")[0])
+    open(os.path.join(wd, "run_mandor.cfg"), "w").write(cfg(20, 20))
+    n = (24, 16, 16)
+    d = records.domain(os.path.join(wd, "binData", "domain.bin"))
+    reg, nf = (7, 16, 6, 10, 6, 10), 15
+    dataE, dataH = tfsf_files.write_dir(os.path.join(wd, ".EM_sources", "TFSF"), reg, 0, nf,
+                                        d["h"], d["tau"], seed=31, scale=0.5)
+    r0 = records.record(wd, 0, n)
+    bc = list(d["bc_min"]) + list(d["bc_max"])
+    o = Oracle(n, d["h"], d["L"], d["tau"], bc, r0["time"])
+    o.set_fields(r0["E"], r0["H"])
+    o.set_markers(r0["M"])
+    o.add_tfsf(reg, 0, nf, dataE, dataH)
+    o.init()
+    core(wd)
+    for _ in range(20):
+        o.step()
+    rec = records.record(wd, 1, n)
+    compare(rec, o, n, d["h"])
+    assert np.abs(rec["H"]).max() > 1e-2          # the interface's fields are in the box

@@ -240,3 +240,60 @@ def test_run_cfg_rejects_zero(host, tmp_path):
     bad = tmp_path / "bad.cfg"
     bad.write_text("@ 20\n@ 0\n@ -100\n@ 0\n@ 5\n@ -100\n")
     assert host.mh_runcfg_load(str(bad).encode(), ctypes.byref(rc)) != 0
+
+
+class _Domain(ctypes.Structure):
+    _fields_ = [("h", ctypes.c_double * 3), ("L", ctypes.c_double * 3), ("tau", ctypes.c_double),
+                ("i", ctypes.c_int * 12)]
+
+
+class _Tfsf(ctypes.Structure):
+    _fields_ = [("reg", ctypes.c_int * 6), ("mode", ctypes.c_int), ("nframes", ctypes.c_int),
+                ("vol", ctypes.c_long * 12), ("dataE", ctypes.c_void_p * 12),
+                ("dataH", ctypes.c_void_p * 12)]
+
+
+def _tfsf_load(host, d, dom):
+    host.mh_tfsf_load.argtypes = [ctypes.c_char_p, ctypes.c_void_p, ctypes.c_void_p]
+    t = _Tfsf()
+    rc = host.mh_tfsf_load(d.encode(), ctypes.byref(dom), ctypes.byref(t))
+    return rc, t
+
+
+def test_tfsf_play_files(host, tmp_path):
+    """mh_tfsf_load reads the play-mode directory core.out reads: the
+    frames of every face slot (E faces get the H recordings and vice versa),
+    opened faces dropped, consistency checks enforced."""
+    import tfsf_files
+    dom = _Domain()
+    dom.h[:] = (0.05, 0.05, 0.05)
+    dom.tau = 0.0129903810567666
+    reg, nf = (7, 16, 6, 13, 6, 11), 9
+    d = str(tmp_path / "TFSF")
+    dataE, dataH = tfsf_files.write_dir(d, reg, 2, nf, dom.h[:], dom.tau, opened=(0, 1, 0, 0, 1, 0))
+    rc, t = _tfsf_load(host, d, dom)
+    assert rc == 0
+    assert tuple(t.reg) == reg and t.mode == 2 and t.nframes == nf
+    vols = tfsf_files.face_nodes(reg)
+    assert list(t.vol) == vols
+    for f in range(12):
+        if dataE[f] is None:
+            assert not t.dataE[f] and not t.dataH[f]
+            continue
+        e = np.ctypeslib.as_array((ctypes.c_double * (vols[f] * nf)).from_address(t.dataE[f]))
+        h = np.ctypeslib.as_array((ctypes.c_double * (vols[f] * nf)).from_address(t.dataH[f]))
+        assert np.array_equal(e, dataE[f]) and np.array_equal(h, dataH[f])
+    host.mh_tfsf_free(ctypes.byref(t))
+    # a different time step, an uncompiled recording and a missing frame
+    dom2 = _Domain()
+    dom2.h[:] = dom.h[:]
+    dom2.tau = dom.tau * (1 + 1e-9)
+    assert _tfsf_load(host, d, dom2)[0] != 0
+    open(os.path.join(d, "TFSF_defrag.map"), "w").write("x\n")
+    assert _tfsf_load(host, d, dom)[0] != 0
+    os.remove(os.path.join(d, "TFSF_defrag.map"))
+    np.array([nf + 1], "<i4").tofile(os.path.join(d, "lastFrameTime.bin"))
+    assert _tfsf_load(host, d, dom)[0] != 0
+    # no TFSF.cfg: no interface
+    rc, t = _tfsf_load(host, str(tmp_path / "none"), dom)
+    assert rc == 0 and t.mode == -1
