@@ -85,13 +85,15 @@ __device__ static inline bool box_decode(const BoxSet &bs, long t, int &i, int &
     if (t >= bs.off[bs.n]) return false;
     int b = 0;
     while (t >= bs.off[b + 1]) ++b;
-    long l = t - bs.off[b];
+    /* boxes hold fewer than 2^32 nodes: 32-bit division */
+    unsigned l = (unsigned)(t - bs.off[b]);
     const Box &B = bs.b[b];
-    int dz = B.hi[2] - B.lo[2] + 1, dy = B.hi[1] - B.lo[1] + 1;
-    k = B.lo[2] + (int)(l % dz);
-    l /= dz;
-    j = B.lo[1] + (int)(l % dy);
-    i = B.lo[0] + (int)(l / dy);
+    const unsigned dz = B.hi[2] - B.lo[2] + 1, dy = B.hi[1] - B.lo[1] + 1;
+    const unsigned l2 = l / dz;
+    k = B.lo[2] + (int)(l - l2 * dz);
+    const unsigned l3 = l2 / dy;
+    j = B.lo[1] + (int)(l2 - l3 * dy);
+    i = B.lo[0] + (int)l3;
     return true;
 }
 
@@ -155,10 +157,11 @@ __device__ static inline void face_apply(const Grid &g, const FaceOp &op, long t
 {
     T *X[3] = {(T *)op.X[0], (T *)op.X[1], (T *)op.X[2]};
     const T alpha = (T)op.alpha;
-    int nq = op.qhi - op.qlo + 1;
-    long tot = (long)(op.phi - op.plo + 1) * nq;
+    const unsigned nq = op.qhi - op.qlo + 1;
+    const long tot = (long)(op.phi - op.plo + 1) * nq;
     if (t >= tot) return;
-    int p = op.plo + (int)(t / nq), q = op.qlo + (int)(t % nq);
+    const unsigned pq = (unsigned)t / nq;
+    int p = op.plo + (int)pq, q = op.qlo + (int)((unsigned)t - pq * nq);
     int ea = op.ea, ep = (ea + 1) % 3, eq = (ea + 2) % 3;
     int r[3];
     r[ep] = p;
@@ -438,7 +441,8 @@ __global__ void k_laser(T *Fy, T *Fz, Grid g, const double *map, int pos, int JM
     /* {A, cos, sin} of this step, evaluated on the host */
     const double *e = tab + 3 * (*step % cap);
     const double A = e[0], cs = e[1], sn = e[2];
-    int j = jmin + (int)(t / nk), k = kmin + (int)(t % nk);
+    const unsigned jq = (unsigned)t / (unsigned)nk;
+    int j = jmin + (int)jq, k = kmin + (int)((unsigned)t - jq * (unsigned)nk);
     long cnt = (long)(j - JMIN) * (KMAX - KMIN + 1) + k - KMIN;
     const double *w = map + 4 * cnt;
     long o = gidx(g, pos, j, k);
@@ -469,7 +473,8 @@ __global__ void k_spot(T *Ey, T *Ez, Grid g, int i, const double *py, const doub
     const int nk = g.n[2] + 3;
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long)(g.n[1] + 3) * nk) return;
-    const int j = -1 + (int)(t / nk), k = -1 + (int)(t % nk);
+    const unsigned jq = (unsigned)t / (unsigned)nk;
+    const int j = -1 + (int)jq, k = -1 + (int)((unsigned)t - jq * (unsigned)nk);
     const double *e = tab + 2 * (*step % cap);
     const long o = gidx(g, i, j, k);
     Ey[o] = (T)(e[0] * py[t]);
@@ -513,8 +518,9 @@ __global__ void k_tfsf(T *F0, T *F1, T *F2, Grid g, TfsfSet fs, const double *da
     if (frame < 0) return;
     int r[3];
     r[fc.ea] = fc.lo[fc.ea];
-    r[fc.ep] = fc.lo[fc.ep] + (int)(t / fc.dQ);
-    r[fc.eq] = fc.lo[fc.eq] + (int)(t % fc.dQ);
+    const unsigned pq = (unsigned)t / (unsigned)fc.dQ;
+    r[fc.ep] = fc.lo[fc.ep] + (int)pq;
+    r[fc.eq] = fc.lo[fc.eq] + (int)((unsigned)t - pq * (unsigned)fc.dQ);
     for (int a = 0; a < 3; ++a)
         if (r[a] < fs.lo_in || r[a] > g.n[a] - 2) return;
     for (int q = 0; q < f; ++q)
@@ -571,8 +577,9 @@ __global__ void k_jfold(JPtrs f, Grid g, int ea, int bc0, int bc1, int top0)
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long)np * nq) return;
     int r[3];
-    r[ep] = -2 + (int)(t / nq);
-    r[eq] = -2 + (int)(t % nq);
+    const unsigned pq = (unsigned)t / (unsigned)nq;
+    r[ep] = -2 + (int)pq;
+    r[eq] = -2 + (int)((unsigned)t - pq * (unsigned)nq);
     const int N = g.n[ea];
 #define AT(a) (r[ea] = (a), gidx(g, r[0], r[1], r[2]))
     if (bc == MPIC_BC_PERIODIC) {
@@ -638,9 +645,9 @@ __global__ void k_energy(FieldsT<T> f, Grid g, double *acc)
     const long tot = (long)g.n[0] * g.n[1] * g.n[2];
     double we = 0, wb = 0;
     for (long c = t; c < tot; c += (long)gridDim.x * blockDim.x) {
-        int k = (int)(c % g.n[2]);
-        long r = c / g.n[2];
-        int j = (int)(r % g.n[1]), i = (int)(r / g.n[1]);
+        const unsigned cu = (unsigned)c, r = cu / (unsigned)g.n[2], r2 = r / (unsigned)g.n[1];
+        const int k = (int)(cu - r * (unsigned)g.n[2]), j = (int)(r - r2 * (unsigned)g.n[1]),
+                  i = (int)r2;
         double ex = f.E[0][gidx(g, i + 1, j, k)], ey = f.E[1][gidx(g, i, j + 1, k)],
                ez = f.E[2][gidx(g, i, j, k + 1)], hx = f.H[0][gidx(g, i, j + 1, k + 1)],
                hy = f.H[1][gidx(g, i + 1, j, k + 1)], hz = f.H[2][gidx(g, i + 1, j + 1, k)];
@@ -698,8 +705,9 @@ __global__ void k_rho_fold(double *rho, Grid g, int b, int bc)
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long)np * nq) return;
     int r[3];
-    r[ep] = -2 + (int)(t / nq);
-    r[eq] = -2 + (int)(t % nq);
+    const unsigned pq = (unsigned)t / (unsigned)nq;
+    r[ep] = -2 + (int)pq;
+    r[eq] = -2 + (int)((unsigned)t - pq * (unsigned)nq);
     const int N = g.n[ea];
 #define AT(a) (r[ea] = (a), gidx(g, r[0], r[1], r[2]))
     if (bc == MPIC_BC_PERIODIC) {
@@ -742,9 +750,8 @@ __global__ void k_gauss(FieldsT<T> f, const double *rho1, const double *rho2, Gr
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const int NX = g.n[0] + 1, NY = g.n[1] + 1, NZ = g.n[2] + 1;
     if (t >= (long)NX * NY * NZ) return;
-    int k = (int)(t % NZ);
-    long r = t / NZ;
-    int j = (int)(r % NY), i = (int)(r / NY);
+    const unsigned tu = (unsigned)t, r = tu / (unsigned)NZ, r2 = r / (unsigned)NY;
+    const int k = (int)(tu - r * (unsigned)NZ), j = (int)(r - r2 * (unsigned)NY), i = (int)r2;
     long o = gidx(g, i, j, k);
     const double h1 = g.h[0], h2 = g.h[1], h3 = g.h[2];
     double t0 = (rho2[o] - rho1[o]) / g.tau + (f.J[0][gidx(g, i + 1, j, k)] - f.J[0][o]) / h1 +
