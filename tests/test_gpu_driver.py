@@ -154,10 +154,11 @@ MUR_DECK = """[units]
 @ 2 ny
 @ 1 nz
 @ 1 layers
-> xmin
-> 0.8
-> xmax
-> 0.95
+@ boxes  a slab 0.8 <= x <= 0.95
+> 0.875 0.4 0.4  centre
+> 0.075 0 0  half-axis
+> 0 1 0  half-axis
+> 0 0 1  half-axis
 [DF:uniform]
 @ -3.0 rho
 @ -1.0 q/M

@@ -97,11 +97,12 @@ DECK = """[units]
 @ 1        ny
 @ 1        nz
 @ 1        layers
-> random
-> xmin
-> 0.1
-> xmax
-> 0.3
+> random   in-cell positions from RANMAR
+@ boxes    a slab 0.1 <= x <= 0.3
+> 0.2 0.175 0.25   centre
+> 0.1 0 0          half-axis
+> 0 1 0            half-axis
+> 0 0 1            half-axis
 [DF:thermal]
 @ 1.0      rho [n_c]
 @ 0.000544662   q/M
@@ -136,10 +137,11 @@ def test_setup_record_zero(host, tmp_path):
     assert tuple(reg) == (0, 0, 0) + n and not E.any()
     M = records.markers(b / "plasma_000000_000.bin")
     ne = 8 * 7 * 10 * 4
-    ni = 4 * 7 * 10              # slab of cells with centres in [0.1, 0.3) micron
+    ni = 4 * 7 * 10              # random points of the cells 2..5 lie in the slab
     assert M.size == ne + ni
     assert np.array_equal(M["id"], np.arange(1, M.size + 1))
-    e, i = M[:ne], M[ne:]
+    # plasma objects are written newest first (source/setup/plasma.c:93-124)
+    i, e = M[:ni], M[ni:]
     # charge per marker: rho * rho_crit / ppc, rho_crit = 4 pi^2 up to rounding
     assert np.allclose(e["rho"], -4 * math.pi ** 2 / 4, rtol=1e-6)
     assert np.allclose(i["rho"], 4 * math.pi ** 2, rtol=1e-6)
@@ -152,7 +154,7 @@ def test_setup_record_zero(host, tmp_path):
     # ions: random in-cell positions inside the slab
     assert i["x"].min() >= 0.1 - 1e-12 and i["x"].max() < 0.3
     # uniform momenta: <u> + (r - 1/2) du with r the RANMAR stream, 3 draws
-    # per marker in file order
+    # per marker in creation order
     r = ranmar_stream(host, 1802, 9373, 3 * ne).reshape(ne, 3)
     assert np.array_equal(e["vx"], 0.1 + (r[:, 0] - 0.5) * 0.2)
     assert np.array_equal(e["vz"], -0.05 + (r[:, 2] - 0.5) * 0.2)

@@ -1,0 +1,261 @@
+"""mandor_setup against the reference setup.out, on the CPU.
+
+Record 0 of every deck in tests/golden/setup_fixtures.npz (written by
+tests/golden/make_setup_fixtures.py from the reference build) is reproduced
+bit for bit: domain.bin, units.bin, every marker field but the padding word
+in the reference's order (plasma objects newest first, box / plane / sphere
+/ cylinder filters, foils, two-stream beams, gradients), and the values of
+.gaussSpot.cfg, TFSF.cfg, filter.cfg and openFaces.cfg.  The decks of the
+time-step fixtures (tests/golden/<case>.npz) are reproduced too, momenta
+aside: the reference draws [DF:uniform] momenta with rand() seeded from the
+wall clock (source/misc_partition.c:255).
+
+[DF:Maxwell] does not run in the reference built with gcc or clang:
+tag_plasma.c:109 stores each pattern place with the unsequenced
+'cell[p].vx = p++', which both compilers evaluate increment first, so the
+places stay NaN and tag_maxwell.c:162 indexes its pattern with them.  Its
+quiet start is pinned here by what the mathematics fixes instead: the
+Hammersley digit reversals (source/setup/setup_denavit.c:24-53) of known
+value, the inverse of the cumulative distribution of exp(-x^2)
+(source/setup/setup_distrMapper.c) against erfinv, mirrored pairs, the
+charge normalisation and the tuned thermal energy (tag_maxwell.c:84-167).
+"""
+import math
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+from scipy.special import erf, erfinv
+
+import records
+from test_host import SETUP, host  # noqa: F401  (fixture)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = os.path.join(HERE, "golden", "setup_fixtures.npz")
+FIELDS = ("x", "y", "z", "vx", "vy", "vz", "rho", "qDivM", "id")
+CONTROL = {"gaussSpot": ".gaussSpot.cfg", "TFSF": ".EM_sources/TFSF/TFSF.cfg",
+           "filter": ".EM_sources/TFSF/filter.cfg", "openFaces": ".EM_sources/TFSF/openFaces.cfg"}
+
+
+def run_setup(wd, deck, *args, ok=True):
+    os.makedirs(wd, exist_ok=True)
+    with open(os.path.join(wd, "case.input"), "w") as f:
+        f.write(deck)
+    r = subprocess.run([SETUP, "case.input"] + list(args), cwd=wd, stdout=subprocess.PIPE,
+                       stderr=subprocess.STDOUT)
+    out = r.stdout.decode()
+    if ok:
+        assert r.returncode == 0, out
+    else:
+        assert r.returncode != 0, out
+    return out
+
+
+def at_values(path):
+    return [ln[1:].split()[0] for ln in open(path).read().splitlines() if ln.startswith("@")]
+
+
+def same_bits(a, b):
+    a, b = np.ascontiguousarray(a), np.ascontiguousarray(b)
+    return a.dtype == b.dtype and a.shape == b.shape and a.tobytes() == b.tobytes()
+
+
+@pytest.mark.parametrize("case", ["shapes", "foil", "two_stream", "sources"])
+def test_record0_matches_reference(host, tmp_path, case):
+    fx = np.load(GOLD)
+    wd = str(tmp_path)
+    run_setup(wd, str(fx[case + "/deck"]))
+    b = os.path.join(wd, "binData")
+    assert open(os.path.join(b, "domain.bin"), "rb").read() == fx[case + "/domain"].tobytes()
+    assert open(os.path.join(b, "units.bin"), "rb").read() == fx[case + "/units"].tobytes()
+    M = records.markers(os.path.join(b, "plasma_000000_000.bin"))
+    assert M.size == fx[case + "/M/x"].size > 0
+    for k in FIELDS:
+        assert same_bits(M[k], fx["%s/M/%s" % (case, k)]), (case, k)
+    for k, p in CONTROL.items():
+        key = "%s/%s" % (case, k)
+        path = os.path.join(wd, p)
+        if key in fx:
+            assert at_values(path) == list(fx[key]), (case, k)
+        else:
+            assert not os.path.exists(path), (case, k)
+
+
+@pytest.mark.parametrize("case", ["periodic", "laser_mur", "spot", "tfsf", "mixed"])
+def test_timestep_decks_match_reference_record0(host, tmp_path, case):
+    sys.path.insert(0, os.path.join(HERE, "golden"))
+    import make_fixtures
+    cfg = make_fixtures.CASES[case]
+    run_setup(str(tmp_path), make_fixtures.deck(cfg["n"], cfg["L"], cfg["bc"], cfg["species"]))
+    M = records.markers(os.path.join(str(tmp_path), "binData", "plasma_000000_000.bin"))
+    M0 = np.load(os.path.join(HERE, "golden", case + ".npz"))["M0"]
+    for k in ("x", "y", "z", "rho", "qDivM", "id"):
+        assert same_bits(M[k], M0[k]), (case, k)
+    # momenta: <u> + (r - 1/2) du with r in [0, 1), per species
+    for sp in cfg["species"]:
+        ppc = int(np.prod(sp["sub"]))
+        sel = M["qDivM"] == np.float32(sp["qm"])
+        sel &= np.isclose(M["rho"], sp["rho"] * 4 * math.pi ** 2 / ppc, rtol=1e-5)
+        assert sel.any()
+        for a, k in enumerate(("vx", "vy", "vz")):
+            d = M[k][sel] - sp["u"][a]
+            assert np.all(np.abs(d) <= 0.5 * sp["du"][a] + 1e-15)
+
+
+BOX = """[units]
+@ 1.0   lambda
+[mesh]
+@ 8     imax
+@ 6     jmax
+@ 5     kmax
+@ 0.4   Lx
+@ 0.3   Ly
+@ 0.25  Lz
+@ -0.45 tau
+[boundaryConditions]
+@ 0 xMin
+@ 0 xMax
+@ 0 yMin
+@ 0 yMax
+@ 0 zMin
+@ 0 zMax
+"""
+
+
+def maxwell_deck(layers, tune, uniform_weight, w2=0.5, qm=-1.0, u=(0.05, 0.03, 0.04)):
+    return BOX + ("[plasma]\n@ 1 nx\n@ 1 ny\n@ 1 nz\n@ %d layers\n[DF:Maxwell]\n@ %r w2\n@ %r q/M\n"
+                  "@ %r ux\n@ %r uy\n@ %r uz\n@ %d tune\n@ %d uniform weight\n"
+                  % (layers, w2, qm, u[0], u[1], u[2], tune, uniform_weight))
+
+
+def radical_inverse(i, r):
+    x, w = 0.0, 1.0 / r
+    while i:
+        x += (i % r) * w
+        i //= r
+        w /= r
+    return x
+
+
+def cell_pattern(wd, ppc):
+    M = records.markers(os.path.join(wd, "binData", "plasma_000000_000.bin"))
+    assert M.size == 8 * 6 * 5 * ppc
+    return M.reshape(-1, ppc)
+
+
+def test_maxwell_quiet_start_uniform_spacing(host, tmp_path):
+    """Weighted form: velocities on the Hammersley set scaled to the wings
+    [0, 1.6] x [-1.6, 1.6]^2, weights exp(-|v|^2) normalised to the charge
+    density omega_pe^2 / (q/M) in units of omega_0^2/(4 pi^2)."""
+    wd = str(tmp_path)
+    u = (0.05, 0.03, 0.04)
+    run_setup(wd, maxwell_deck(8, 0, 0, u=u))
+    P = cell_pattern(wd, 8)
+    for k in ("vx", "vy", "vz", "rho"):                      # every cell the same pattern
+        assert np.all(P[k] == P[k][0])
+    c = P[0]
+    # digit reversals of 1..4: radix 2 (van der Corput), 3 and 5
+    np.testing.assert_array_equal([radical_inverse(i, 2) for i in range(1, 5)], [0.5, 0.25, 0.75, 0.125])
+    np.testing.assert_allclose([radical_inverse(i, 3) for i in range(1, 5)], [1 / 3, 2 / 3, 1 / 9, 4 / 9],
+                               rtol=1e-15)
+    phi2 = np.array([0.5, 0.25, 0.75, 0.125])
+    phi3 = np.array([1 / 3, 2 / 3, 1 / 9, 4 / 9])
+    phi5 = np.array([0.2, 0.4, 0.6, 0.8])
+    vx, vy, vz = 1.6 * phi2, 3.2 * (phi3 - 0.5), 3.2 * (phi5 - 0.5)
+    np.testing.assert_allclose(c["vx"][0::2], vx * u[0], rtol=1e-14)
+    np.testing.assert_allclose(c["vy"][0::2], vy * u[1], rtol=1e-14)
+    np.testing.assert_allclose(c["vz"][0::2], vz * u[2], rtol=1e-14)
+    for k in ("vx", "vy", "vz"):                            # mirrored partners
+        np.testing.assert_array_equal(c[k][1::2], -c[k][0::2])
+    w = np.exp(-(vx * vx + vy * vy + vz * vz))
+    rho_cell = 0.5 * 4 * math.pi ** 2 / -1.0
+    np.testing.assert_allclose(c["rho"][0::2], rho_cell * w / (2 * w.sum()), rtol=2e-7)
+    np.testing.assert_array_equal(c["rho"][1::2], c["rho"][0::2])
+    assert np.all(c["qDivM"] == np.float32(-1.0))
+
+
+def test_maxwell_quiet_start_uniform_weight_and_tuned_energy(host, tmp_path):
+    """Equal weights with velocities at the quantiles of exp(-x^2) (one-sided
+    for x), then scaled per axis so that sum rho v^2 = u_th^2 rho / 2."""
+    wd = str(tmp_path)
+    u = (0.05, 0.03, 0.04)
+    ppc = 16
+    run_setup(wd, maxwell_deck(ppc, 0, 1, u=u))
+    c = cell_pattern(wd, ppc)[0]
+    k = np.arange(1, ppc // 2 + 1)
+    e6 = erf(6.0)
+    qx = erfinv(np.array([radical_inverse(i, 2) for i in k]) * e6)
+    qy = erfinv((2 * np.array([radical_inverse(i, 3) for i in k]) - 1) * e6)
+    qz = erfinv((2 * np.array([radical_inverse(i, 5) for i in k]) - 1) * e6)
+    np.testing.assert_allclose(c["vx"][0::2], qx * u[0], atol=1e-7 * u[0])
+    np.testing.assert_allclose(c["vy"][0::2], qy * u[1], atol=1e-7 * u[1])
+    np.testing.assert_allclose(c["vz"][0::2], qz * u[2], atol=1e-7 * u[2])
+    rho_cell = 0.5 * 4 * math.pi ** 2 / -1.0
+    np.testing.assert_allclose(c["rho"], rho_cell / ppc, rtol=1e-7)
+
+    wd2 = str(tmp_path / "tuned")
+    run_setup(wd2, maxwell_deck(ppc, 1, 1, u=u))
+    t = cell_pattern(wd2, ppc)[0]
+    for a, key in enumerate(("vx", "vy", "vz")):
+        W = np.sum(t[key] ** 2 * t["rho"].astype(np.float64))
+        assert W == pytest.approx(u[a] ** 2 * rho_cell * 0.5, rel=1e-6)
+        ratio = t[key] / c[key]
+        np.testing.assert_allclose(ratio, ratio[0], rtol=1e-12)
+
+
+def test_maxwell_needs_even_layers_and_a_lattice(host, tmp_path):
+    assert "even number" in run_setup(str(tmp_path / "a"), maxwell_deck(3, 0, 0), ok=False)
+    two_stream = ("[DF:two-stream]\n@ 0.05 g\n@ -1 q/M\n@ 4 Nx\n@ 3 Ny\n@ 2 Nz\n@ 1 mx\n@ 0 my\n@ 0 mz\n"
+                  "[DF:Maxwell]\n@ 0.5 w2\n@ -1 q/M\n@ 0.05 ux\n@ 0.05 uy\n@ 0.05 uz\n@ 0 t\n@ 0 w\n")
+    out = run_setup(str(tmp_path / "b"), BOX + two_stream, ok=False)
+    assert "[plasma]" in out
+
+
+def test_deck_errors(host, tmp_path):
+    out = run_setup(str(tmp_path / "a"), BOX.replace("[mesh]", "[meshes]"), ok=False)
+    assert "[units], [mesh] and [boundaryConditions]" in out
+    out = run_setup(str(tmp_path / "b"), BOX + "[EMWave]\n@ 1 x\n", ok=False)
+    assert "EMWave" in out
+    out = run_setup(str(tmp_path / "c"), BOX.replace("@ -0.45 tau", "@ -1.5 tau"), ok=False)
+    assert "Courant" in out
+    out = run_setup(str(tmp_path / "d"), BOX.replace("@ 0 xMax", "@ 2 xMax"), ok=False)
+    assert "periodic" in out
+
+
+def test_estimate_writes_nothing(host, tmp_path):
+    wd = str(tmp_path)
+    out = run_setup(wd, maxwell_deck(8, 0, 0), "estimate")
+    assert "1920 markers" in out
+    assert not os.path.exists(os.path.join(wd, "binData"))
+
+
+def test_focused_laser_tag(host, tmp_path):
+    """[focused laser] takes the reference's ten parameters and writes the
+    same source as [laser] with the waist 2 F lambda / pi of a Gaussian beam
+    focused at f-number F."""
+    deck = BOX.replace("@ 8     imax", "@ 24    imax").replace("@ 6     jmax", "@ 16    jmax").replace(
+        "@ 5     kmax", "@ 16    kmax").replace("@ 0.4   Lx", "@ 1.2   Lx").replace(
+        "@ 0.3   Ly", "@ 0.8   Ly").replace("@ 0.25  Lz", "@ 0.8   Lz")
+    F = 2.5
+    a = str(tmp_path / "a")
+    run_setup(a, deck + "[focused laser]\n@ -1.5 a0\n@ %r F\n@ 0.7 xf\n@ 0.4 yf\n@ 0.4 zf\n@ 4.0 dt\n"
+              "@ 3.0 T0\n@ 0.2 phi0\n@ 10 ratio\n@ 1 direction\n" % F)
+    b = str(tmp_path / "b")
+    run_setup(b, deck + "[laser]\n@ 1.5 a0\n@ %r waist\n@ 0.7 xf\n@ 0.4 yf\n@ 0.4 zf\n@ 4.0 dt\n"
+              "@ 3.0 T0\n@ 0.2 phi0\n" % (2 * F / math.pi))
+
+    def body(wd):
+        raw = open(os.path.join(wd, ".EM_sources/laser/mirror_00"), "rb").read()
+        head, _, rest = raw.partition(b"---------------------------------------\n")
+        hdr, _, data = rest.partition(b"---------------------------------------\n")
+        return [ln.split()[1] for ln in hdr.decode().splitlines()], data
+
+    ha, da = body(a)
+    hb, db = body(b)
+    assert da == db and len(da) == 8 * 8 * (17 - 10) * (17 - 10)
+    assert float(ha[0]) == -1.5 and float(ha[1]) == F and ha[2:] == hb[2:]
+    out = run_setup(str(tmp_path / "c"), deck + "[focused laser]\n@ -1.5 a0\n@ 2 F\n@ 0.7 xf\n@ 0.4 yf\n"
+                    "@ 0.4 zf\n@ 4.0 dt\n@ 3.0 T0\n@ 0.2 phi0\n@ 10 ratio\n@ -1 direction\n", ok=False)
+    assert "+x" in out
