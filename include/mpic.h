@@ -160,6 +160,19 @@ double mpic_time(mpic_run_t *run);
 
 /* CIC charge density with boundary fold (plasma_rho), nodes [-2,n+2]. */
 int  mpic_rho(mpic_run_t *run, double *rho);
+/* Tecplot payloads computed on the device (tecIO_saveVec with
+ * tecIO_interpE / tecIO_interpH, source/IO_tecplot.c:131-214): what = 0 E,
+ * 1 H, 2 J; out: nodes [0,n] (k fastest), 3 interleaved doubles each.  E and
+ * J are the mean of the two values along each component's axis, H the mean
+ * of the four around the node. */
+int  mpic_node_fields(mpic_run_t *run, int what, double *out);
+/* Probes (probe_allocate / probe_postData, source/core/diag_probe.c:171-383):
+ * the nodes ijk[3p..3p+2], each in [0, n-1]; a sample writes
+ * {Ex, Ey, Ez, Hx, Hy, Hz} per probe, interpolated as mpic_node_fields does,
+ * from the fields as they are when it is called (the reference samples
+ * after em_HHalfStep). */
+int  mpic_set_probes(mpic_run_t *run, const int *ijk, int nprobes);
+int  mpic_probe_sample(mpic_run_t *run, double *out);
 /* gauss_before / gauss_after residuals {max|drho/dt + div J|, max|div E - rho|}:
  * call mpic_gauss_before() before mpic_push_deposit and mpic_gauss_after()
  * after mpic_em_e_finish. */

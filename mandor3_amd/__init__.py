@@ -78,6 +78,9 @@ def lib():
             "mpic_sync": (I, [P]),
             "mpic_time": (D, [P]),
             "mpic_rho": (I, [P, P]),
+            "mpic_node_fields": (I, [P, I, P]),
+            "mpic_set_probes": (I, [P, P, I]),
+            "mpic_probe_sample": (I, [P, P]),
             "mpic_gauss_before": (I, [P]),
             "mpic_gauss_after": (I, [P, P]),
             "mpic_sort": (I, [P, P]),
@@ -260,6 +263,23 @@ class Run:
         r = np.empty(int(np.prod([v + 5 for v in self.n])))
         _check(lib().mpic_rho(self.handle, _ptr(r)), "mpic_rho")
         return r
+
+    def node_fields(self, what):
+        """Node values of E (0), H (1) or J (2) on the nodes [0, n], as the
+        Tecplot records hold them: shape ((n0+1)(n1+1)(n2+1), 3)."""
+        out = np.empty(3 * self.nodes)
+        _check(lib().mpic_node_fields(self.handle, int(what), _ptr(out)), "mpic_node_fields")
+        return out.reshape(-1, 3)
+
+    def set_probes(self, ijk):
+        ijk = np.ascontiguousarray(ijk, dtype=np.int32).reshape(-1, 3)
+        _check(lib().mpic_set_probes(self.handle, _ptr(ijk), ijk.shape[0]), "mpic_set_probes")
+        self._nprobes = ijk.shape[0]
+
+    def probe_sample(self):
+        out = np.empty(6 * getattr(self, "_nprobes", 0))
+        _check(lib().mpic_probe_sample(self.handle, _ptr(out)), "mpic_probe_sample")
+        return out.reshape(-1, 6)
 
     def gauss_before(self):
         _check(lib().mpic_gauss_before(self.handle), "mpic_gauss_before")

@@ -776,6 +776,90 @@ int launch_gauss(hipStream_t s, const FieldsT<T> &f, const double *rho1, const d
     return MPIC_OK;
 }
 
+/* Node values of the Tecplot records and the probes (source/IO_tecplot.c:
+ * 131-153, source/core/diag_probe.c:361-383), in FP64: kind 0 (E, J) is the
+ * mean of the two values along the component's own axis, kind 1 (H) the mean
+ * of the four around the node in the plane normal to it, summed in the
+ * reference's order. */
+template <class T>
+__device__ static inline void node_values(T *const F[3], const Grid &g, int kind, int i, int j,
+                                          int k, double v[3])
+{
+    if (kind == 0) {
+        v[0] = 0.5 * ((double)F[0][gidx(g, i, j, k)] + (double)F[0][gidx(g, i + 1, j, k)]);
+        v[1] = 0.5 * ((double)F[1][gidx(g, i, j, k)] + (double)F[1][gidx(g, i, j + 1, k)]);
+        v[2] = 0.5 * ((double)F[2][gidx(g, i, j, k)] + (double)F[2][gidx(g, i, j, k + 1)]);
+    } else {
+        v[0] = 0.25 * ((double)F[0][gidx(g, i, j, k)] + (double)F[0][gidx(g, i, j + 1, k)] +
+                       (double)F[0][gidx(g, i, j, k + 1)] + (double)F[0][gidx(g, i, j + 1, k + 1)]);
+        v[1] = 0.25 * ((double)F[1][gidx(g, i, j, k)] + (double)F[1][gidx(g, i + 1, j, k)] +
+                       (double)F[1][gidx(g, i, j, k + 1)] + (double)F[1][gidx(g, i + 1, j, k + 1)]);
+        v[2] = 0.25 * ((double)F[2][gidx(g, i, j, k)] + (double)F[2][gidx(g, i + 1, j, k)] +
+                       (double)F[2][gidx(g, i, j + 1, k)] + (double)F[2][gidx(g, i + 1, j + 1, k)]);
+    }
+}
+
+template <class T>
+struct Ptr3 {
+    T *p[3];
+};
+
+/* nodes [0, n] (k fastest), three interleaved doubles per node */
+template <class T>
+__global__ void k_node_fields(Ptr3<T> F, Grid g, int kind, double *out)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned nz = (unsigned)g.n[2] + 1, ny = (unsigned)g.n[1] + 1;
+    if (t >= (long)(g.n[0] + 1) * ny * nz) return;
+    const unsigned r = (unsigned)t / nz, i = r / ny;
+    const int k = (int)((unsigned)t - r * nz), j = (int)(r - i * ny);
+    double v[3];
+    node_values(F.p, g, kind, (int)i, j, k, v);
+    out[3 * t] = v[0];
+    out[3 * t + 1] = v[1];
+    out[3 * t + 2] = v[2];
+}
+
+template <class T>
+int launch_node_fields(hipStream_t s, T *const F[3], const Grid &g, int kind, double *out)
+{
+    Ptr3<T> P = {{F[0], F[1], F[2]}};
+    const long n = (long)(g.n[0] + 1) * (g.n[1] + 1) * (g.n[2] + 1);
+    k_node_fields<T><<<nblk(n), TPB, 0, s>>>(P, g, kind, out);
+    MPIC_CHECK(hipGetLastError());
+    return MPIC_OK;
+}
+
+/* probe p at node ijk[3p..3p+2] (checked on the host to lie in [0, n-1]):
+ * out[6p..6p+5] = Ex Ey Ez Hx Hy Hz */
+template <class T>
+__global__ void k_probes(Ptr3<T> E, Ptr3<T> H, Grid g, const int *ijk, int np, double *out)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    const int i = ijk[3 * p], j = ijk[3 * p + 1], k = ijk[3 * p + 2];
+    double v[3];
+    node_values(E.p, g, 0, i, j, k, v);
+    out[6 * p] = v[0];
+    out[6 * p + 1] = v[1];
+    out[6 * p + 2] = v[2];
+    node_values(H.p, g, 1, i, j, k, v);
+    out[6 * p + 3] = v[0];
+    out[6 * p + 4] = v[1];
+    out[6 * p + 5] = v[2];
+}
+
+template <class T>
+int launch_probes(hipStream_t s, T *const E[3], T *const H[3], const Grid &g, const int *ijk,
+                  int np, double *out)
+{
+    if (np <= 0) return MPIC_OK;
+    Ptr3<T> Pe = {{E[0], E[1], E[2]}}, Ph = {{H[0], H[1], H[2]}};
+    k_probes<T><<<(np + TPB - 1) / TPB, TPB, 0, s>>>(Pe, Ph, g, ijk, np, out);
+    MPIC_CHECK(hipGetLastError());
+    return MPIC_OK;
+}
+
 /* explicit instantiations for the two field precisions */
 #define INST(T)                                                                               \
     template int launch_cap_H<T>(hipStream_t, const FieldsT<T> &, const Grid &, const BoxSet &); \
@@ -794,6 +878,9 @@ int launch_gauss(hipStream_t s, const FieldsT<T> &f, const double *rho1, const d
     template int launch_tfsf<T>(hipStream_t, T *const[3], const Grid &, const TfsfSet &, long,   \
                                 const double *, const int *, const int *, int, int, double);   \
     template int launch_gauss<T>(hipStream_t, const FieldsT<T> &, const double *,                \
-                                 const double *, const Grid &, double *);
+                                 const double *, const Grid &, double *);                       \
+    template int launch_node_fields<T>(hipStream_t, T *const[3], const Grid &, int, double *);   \
+    template int launch_probes<T>(hipStream_t, T *const[3], T *const[3], const Grid &,           \
+                                  const int *, int, double *);
 INST(double)
 INST(float)

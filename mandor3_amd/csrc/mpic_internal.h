@@ -166,6 +166,11 @@ template <class T>
 int launch_spot(hipStream_t s, T *Ey, T *Ez, const Grid &g, int i, const double *py,
                 const double *pz, const double *tab, const int *step, int cap);
 int launch_jfold(hipStream_t s, const JPtrs &f, const Grid &g, int ea, int bc0, int bc1);
+template <class T>
+int launch_node_fields(hipStream_t s, T *const F[3], const Grid &g, int kind, double *out);
+template <class T>
+int launch_probes(hipStream_t s, T *const E[3], T *const H[3], const Grid &g, const int *ijk,
+                  int np, double *out);
 template <class T> int launch_energy(hipStream_t s, const FieldsT<T> &f, const Grid &g, double *acc);
 int launch_zero(hipStream_t s, double *p, long n);
 

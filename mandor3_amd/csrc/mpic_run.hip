@@ -138,7 +138,10 @@ struct mpic_run {
     hipGraphExec_t exec[2];   /* one step from each of the two marker-buffer parities */
     int           exec_parity;     /* parity at which exec[0] was captured */
     int           ev_steps;   /* timed steps of the last mpic_advance */
-
+    double       *dnode;      /* node values of the Tecplot records, 3 (n+1)^3 doubles */
+    int          *dprobe;     /* probe nodes, 3 ints each */
+    double       *dsample;    /* probe samples, 6 doubles each */
+    int           nprobes;
 };
 
 /* ---------------- boundary-cap geometry (single rank) ---------------- */
@@ -535,6 +538,9 @@ extern "C" int mpic_destroy(mpic_run_t *r)
         hipHostFree(l.htab);
     }
     hipFree(r->dstep);
+    hipFree(r->dnode);
+    hipFree(r->dprobe);
+    hipFree(r->dsample);
     hipFree(r->spot.dprof);
     hipFree(r->tfsf.ddata);
     hipFree(r->tfsf.dtab);
@@ -627,6 +633,72 @@ extern "C" int mpic_get_J(mpic_run_t *r, double *J)
             for (int j = -2; j <= g.n[1] + 2; ++j)
                 for (int k = -2; k <= g.n[2] + 2; ++k, ++q) J[3 * q + c] = tmp[gidx(g, i, j, k)];
     }
+    return MPIC_OK;
+}
+
+/* tecIO_saveFields / tecIO_saveCurrents payloads (source/IO_tecplot.c:
+ * 131-266): the node values of E, H or J, computed on the device. */
+extern "C" int mpic_node_fields(mpic_run_t *r, int what, double *out)
+{
+    if (!r || !out || what < 0 || what > 2) return MPIC_ERR_ARG;
+    MPIC_CHECK(hipSetDevice(r->dev));
+    const Grid &g = r->g;
+    const long nodes = (long)(g.n[0] + 1) * (g.n[1] + 1) * (g.n[2] + 1);
+    if (!r->dnode) MPIC_CHECK(hipMalloc(&r->dnode, sizeof(double) * 3 * nodes));
+    if (what == 2) {
+        TRY(launch_node_fields<double>(r->stream, r->jp.J, g, 0, r->dnode));
+    } else if (r->fp32) {
+        TRY(launch_node_fields<float>(r->stream, what ? r->f32.H : r->f32.E, g, what, r->dnode));
+    } else {
+        TRY(launch_node_fields<double>(r->stream, what ? r->f.H : r->f.E, g, what, r->dnode));
+    }
+    MPIC_CHECK(hipMemcpyAsync(out, r->dnode, sizeof(double) * 3 * nodes, hipMemcpyDeviceToHost,
+                              r->stream));
+    MPIC_CHECK(hipStreamSynchronize(r->stream));
+    return MPIC_OK;
+}
+
+extern "C" int mpic_set_probes(mpic_run_t *r, const int *ijk, int nprobes)
+{
+    if (!r || nprobes < 0 || (nprobes && !ijk)) return MPIC_ERR_ARG;
+    for (int p = 0; p < nprobes; ++p)
+        for (int a = 0; a < 3; ++a)
+            if (ijk[3 * p + a] < 0 || ijk[3 * p + a] >= r->g.n[a]) {
+                mpic_set_error("mpic_set_probes: probe %d node (%d, %d, %d) outside [0, n-1]", p,
+                               ijk[3 * p], ijk[3 * p + 1], ijk[3 * p + 2]);
+                return MPIC_ERR_ARG;
+            }
+    MPIC_CHECK(hipSetDevice(r->dev));
+    MPIC_CHECK(hipStreamSynchronize(r->stream));
+    hipFree(r->dprobe);
+    hipFree(r->dsample);
+    r->dprobe = nullptr;
+    r->dsample = nullptr;
+    r->nprobes = 0;
+    if (!nprobes) return MPIC_OK;
+    MPIC_CHECK(hipMalloc(&r->dprobe, sizeof(int) * 3 * nprobes));
+    MPIC_CHECK(hipMalloc(&r->dsample, sizeof(double) * 6 * nprobes));
+    MPIC_CHECK(hipMemcpy(r->dprobe, ijk, sizeof(int) * 3 * nprobes, hipMemcpyHostToDevice));
+    r->nprobes = nprobes;
+    return MPIC_OK;
+}
+
+/* probe_postData (source/core/diag_probe.c:361-383): the fields at the
+ * probe nodes now (the reference samples right after em_HHalfStep). */
+extern "C" int mpic_probe_sample(mpic_run_t *r, double *out)
+{
+    if (!r || (r->nprobes && !out)) return MPIC_ERR_ARG;
+    if (!r->nprobes) return MPIC_OK;
+    MPIC_CHECK(hipSetDevice(r->dev));
+    if (r->fp32)
+        TRY(launch_probes<float>(r->stream, r->f32.E, r->f32.H, r->g, r->dprobe, r->nprobes,
+                                 r->dsample));
+    else
+        TRY(launch_probes<double>(r->stream, r->f.E, r->f.H, r->g, r->dprobe, r->nprobes,
+                                  r->dsample));
+    MPIC_CHECK(hipMemcpyAsync(out, r->dsample, sizeof(double) * 6 * r->nprobes,
+                              hipMemcpyDeviceToHost, r->stream));
+    MPIC_CHECK(hipStreamSynchronize(r->stream));
     return MPIC_OK;
 }
 

@@ -8,11 +8,14 @@
  * sources in .EM_sources/laser, the TFSF recordings in .EM_sources/TFSF
  * (play modes) and the hard spot of .gaussSpot.cfg, then
  * runs the time step through the C ABI of include/mpic.h, writing
- * checkpoints, the energy history and honouring tmp/stop.flag as
- * source/core/main.c:195-490 does for one rank.
- * Tecplot and spectral dumps are not produced (a warning says so).
+ * checkpoints, Tecplot records (every chPoint_tecplot steps: E and H after
+ * the push, J and the charge density at the end of the step), probe samples
+ * (run_probes.cfg, every step after the first H half step) and the energy
+ * history, and honouring tmp/stop.flag, as source/core/main.c:195-490 does
+ * for one rank.  Spectral dumps are not produced (a warning says so).
  */
 #define _GNU_SOURCE
+#include <dirent.h>
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -52,11 +55,20 @@ typedef struct {
     void       *rec;
     long        cap;
     int         recsize;
+    int         fmap;         /* next file map id (source/IO_fileMap.c:40-68) */
+    int         tec;          /* next Tecplot record */
+    double     *node, *rho, *rho0;
+    int         nprobes;
+    int        *probe;        /* nodes, 3 ints each */
+    FILE      **pf;           /* binData/probe_%d_%d_%d.bin */
+    double     *sample;
 } state_t;
 
 /* plasma_save + sysIO_save (source/core/main.c:444-462). */
 static void checkpoint(state_t *s, int rec, double time)
 {
+    const int fmap = mh_filemap_save("binData", s->fmap++, s->n, 0);
+    if (fmap < 0) die("file map", mh_error());
     char p[256];
     long n = s->cap;
     GPU(mpic_markers_save(s->run, s->rec, &n, s->recsize), "mpic_markers_save");
@@ -65,10 +77,158 @@ static void checkpoint(state_t *s, int rec, double time)
     GPU(mpic_get_fields(s->run, s->E, s->H), "mpic_get_fields");
     const int lo[3] = {0, 0, 0};
     snprintf(p, sizeof(p), "binData/sys_E_%06d(%03d).bin", rec, 0);
-    HOST(mh_mesh_save(p, s->n, s->E, lo, s->n, "E"), "E record");
+    HOST(mh_mesh_save(p, s->n, s->E, lo, s->n, "vec_E"), "E record");
     snprintf(p, sizeof(p), "binData/sys_H_%06d(%03d).bin", rec, 0);
-    HOST(mh_mesh_save(p, s->n, s->H, lo, s->n, "H"), "H record");
-    HOST(mh_sys_write("binData", rec, time, s->n, 0), "checkpoint descriptor");
+    HOST(mh_mesh_save(p, s->n, s->H, lo, s->n, "vec_H"), "H record");
+    HOST(mh_sys_write("binData", rec, time, s->n, fmap), "checkpoint descriptor");
+}
+
+/* tecIO_saveFields (source/IO_tecplot.c:218-245): node values of E and H
+ * as they are after the push, file map, descriptor. */
+static void tec_fields(state_t *s, double time)
+{
+    char p[256];
+    const int lo[3] = {0, 0, 0};
+    const int fmap = mh_filemap_save("binData", s->fmap++, s->n, 0);
+    if (fmap < 0) die("file map", mh_error());
+    GPU(mpic_node_fields(s->run, 0, s->node), "mpic_node_fields");
+    snprintf(p, sizeof(p), "binData/tec_E_%06d(%03d).bin", s->tec, 0);
+    HOST(mh_mesh_save_ex(p, s->n, s->node, 3, lo, s->n, 1, "vec_E"), "tec_E");
+    GPU(mpic_node_fields(s->run, 1, s->node), "mpic_node_fields");
+    snprintf(p, sizeof(p), "binData/tec_H_%06d(%03d).bin", s->tec, 0);
+    HOST(mh_mesh_save_ex(p, s->n, s->node, 3, lo, s->n, 1, "vec_H"), "tec_H");
+    HOST(mh_tec_write("binData", s->tec, time, s->n, fmap), "tec descriptor");
+}
+
+/* tecIO_saveCurrents (:251-264): node values of J after the fold and the
+ * charge density taken after the push, then tecplot.N. */
+static void tec_currents(state_t *s)
+{
+    char p[256];
+    const int lo[3] = {0, 0, 0};
+    GPU(mpic_node_fields(s->run, 2, s->node), "mpic_node_fields");
+    snprintf(p, sizeof(p), "binData/tec_J_%06d(%03d).bin", s->tec, 0);
+    HOST(mh_mesh_save_ex(p, s->n, s->node, 3, lo, s->n, 2, "vec_J"), "tec_J");
+    for (int i = 0, q = 0; i <= s->n[0]; ++i)
+        for (int j = 0; j <= s->n[1]; ++j)
+            for (int k = 0; k <= s->n[2]; ++k, ++q)
+                s->rho0[q] = s->rho[(((long)i + 2) * (s->n[1] + 5) + j + 2) * (s->n[2] + 5) + k + 2];
+    snprintf(p, sizeof(p), "binData/tec_rho_%06d(%03d).bin", s->tec, 0);
+    HOST(mh_mesh_save_ex(p, s->n, s->rho0, 1, lo, s->n, 2, "scal_rho"), "tec_rho");
+    ++s->tec;
+    HOST(mh_tec_count("binData", s->tec), "tecplot.N");
+}
+
+/* probe_allocate / probe_touch (source/core/diag_probe.c:47-100,171-357):
+ * old probe files removed unless continuing, globals and locations written,
+ * one file per probe node opened (header on creation, append otherwise). */
+static void probes_open(state_t *s, const mh_domain_t *d, const mh_units_t *u, int cont)
+{
+    DIR *dir = cont ? NULL : opendir("binData");
+    if (dir) {
+        struct dirent *e;
+        char p[512];
+        while ((e = readdir(dir))) {
+            const size_t len = strlen(e->d_name);
+            if (!strncmp(e->d_name, "probe_", 6) && len > 4 && !strcmp(e->d_name + len - 4, ".bin")) {
+                snprintf(p, sizeof(p), "binData/%s", e->d_name);
+                remove(p);
+            }
+        }
+        closedir(dir);
+    }
+    HOST(mh_probes_parse("run_probes.cfg", d, u, &s->probe, &s->nprobes), "run_probes.cfg");
+    for (int p = 0; p < s->nprobes; ++p)
+        for (int a = 0; a < 3; ++a)
+            if (s->probe[3 * p + a] < 0 || s->probe[3 * p + a] >= s->n[a])
+                die("run_probes.cfg", "a probe lies outside the domain");
+    const double mic = u->micron;
+    FILE *fp = fopen("output/probe_locations.dat", "wt");
+    if (fp) {
+        fprintf(fp, "variables = \"x [<greek>m</greek>m]\", \"y [<greek>m</greek>m]\", "
+                    "\"z [<greek>m</greek>m]\"\nzone t=\"domain\", f= point\n");
+        for (int i = 0; i < 2; ++i)
+            for (int j = 0; j < 2; ++j)
+                for (int k = 0; k < 2; ++k)
+                    fprintf(fp, "%e %e %e\n", (d->dmn_min[0] * i + d->dmn_max[0] * (1 - i)) * d->h[0] / mic,
+                            (d->dmn_min[1] * j + d->dmn_max[1] * (1 - j)) * d->h[1] / mic,
+                            (d->dmn_min[2] * k + d->dmn_max[2] * (1 - k)) * d->h[2] / mic);
+        fprintf(fp, "zone t=\"probes\", f= point\n");
+        for (int p = 0; p < s->nprobes; ++p)
+            fprintf(fp, "%e %e %e\n", s->probe[3 * p] * d->h[0] / mic, s->probe[3 * p + 1] * d->h[1] / mic,
+                    s->probe[3 * p + 2] * d->h[2] / mic);
+        fclose(fp);
+    }
+    fp = fopen("binData/probe_globals.bin", "wb");
+    if (!fp) die("binData/probe_globals.bin", "cannot create");
+    const double glob[10] = {d->tau, d->h[0], d->h[1], d->h[2],
+                             d->dmn_min[0] * d->h[0], d->dmn_min[1] * d->h[1], d->dmn_min[2] * d->h[2],
+                             d->dmn_max[0] * d->h[0], d->dmn_max[1] * d->h[1], d->dmn_max[2] * d->h[2]};
+    if (fwrite(glob, sizeof(double), 10, fp) != 10) die("binData/probe_globals.bin", "write error");
+    fclose(fp);
+    if (!s->nprobes) return;
+    s->pf = (FILE **)calloc((size_t)s->nprobes, sizeof(FILE *));
+    s->sample = (double *)malloc(sizeof(double) * 6 * (size_t)s->nprobes);
+    if (!s->pf || !s->sample) die("probes", "out of memory");
+    for (int p = 0; p < s->nprobes; ++p) {
+        char name[256];
+        const int *q = s->probe + 3 * p;
+        snprintf(name, sizeof(name), "binData/probe_%d_%d_%d.bin", q[0], q[1], q[2]);
+        FILE *t = fopen(name, "rb");
+        if (t) {
+            fclose(t);
+            s->pf[p] = fopen(name, "ab");
+        } else {
+            s->pf[p] = fopen(name, "wb");
+            if (s->pf[p]) {
+                struct {
+                    char    magic[8];
+                    int32_t i, j, k;
+                } h;
+                memset(&h, 0, sizeof(h));
+                memcpy(h.magic, "probes", 6);
+                h.i = q[0];
+                h.j = q[1];
+                h.k = q[2];
+                if (fwrite(&h, sizeof(h), 1, s->pf[p]) != 1) die(name, "write error");
+            }
+        }
+        if (!s->pf[p]) die(name, "cannot open");
+    }
+    GPU(mpic_set_probes(s->run, s->probe, s->nprobes), "mpic_set_probes");
+}
+
+/* One step through the stage entry points, with the probe samples after
+ * the first H half step and a Tecplot record when 'tec' is set
+ * (source/core/main.c:365-435). */
+static void step_stages(state_t *s, double *pt, int tec)
+{
+    const double t0 = mpic_time(s->run);
+    double WE, WM;
+    GPU(mpic_em_h_half(s->run), "mpic_em_h_half");
+    GPU(mpic_em_energy(s->run, &WE, &WM), "mpic_em_energy");
+    if (s->nprobes) {
+        GPU(mpic_probe_sample(s->run, s->sample), "mpic_probe_sample");
+        for (int p = 0; p < s->nprobes; ++p) {
+            double rec[7] = {t0};
+            memcpy(rec + 1, s->sample + 6 * p, sizeof(double) * 6);
+            if (fwrite(rec, sizeof(double), 7, s->pf[p]) != 7) die("probe file", "write error");
+        }
+    }
+    GPU(mpic_push_deposit(s->run), "mpic_push_deposit");
+    GPU(mpic_temperature(s->run, pt + 3, pt + 4, pt + 5), "mpic_temperature");
+    if (tec) {
+        GPU(mpic_rho(s->run, s->rho), "mpic_rho");
+        tec_fields(s, t0);
+    }
+    GPU(mpic_em_h_step(s->run), "mpic_em_h_step");
+    GPU(mpic_em_e_start(s->run), "mpic_em_e_start");
+    GPU(mpic_j_finish(s->run), "mpic_j_finish");
+    GPU(mpic_em_e_finish(s->run), "mpic_em_e_finish");
+    if (tec) tec_currents(s);
+    pt[0] = t0;
+    pt[1] = WE;
+    pt[2] = WM;
 }
 
 int main(int argc, char **argv)
@@ -87,8 +247,8 @@ int main(int argc, char **argv)
     mh_runcfg_t rc;
     HOST(mh_runcfg_load("run_mandor.cfg", &rc), "run_mandor.cfg");
     HOST(mh_stopflag_default("tmp/stop.flag"), "tmp/stop.flag");
-    if (rc.chpoint_tecplot > 0 || rc.chpoint_spectr > 0)
-        fprintf(stderr, "mandor_core: tecplot / spectral dumps are not produced by this driver\n");
+    if (rc.chpoint_spectr > 0)
+        fprintf(stderr, "mandor_core: spectral dumps are not produced by this driver\n");
 
     mh_domain_t dom;
     mh_units_t units;
@@ -116,7 +276,15 @@ int main(int argc, char **argv)
     s.nodes = (long)(s.n[0] + 1) * (s.n[1] + 1) * (s.n[2] + 1);
     s.E = (double *)calloc(3 * s.nodes, sizeof(double));
     s.H = (double *)calloc(3 * s.nodes, sizeof(double));
-    if (!s.E || !s.H) die("fields", "out of memory");
+    s.node = (double *)calloc(3 * s.nodes, sizeof(double));
+    s.rho0 = (double *)calloc(s.nodes, sizeof(double));
+    s.rho = (double *)calloc((size_t)(s.n[0] + 5) * (s.n[1] + 5) * (s.n[2] + 5), sizeof(double));
+    if (!s.E || !s.H || !s.node || !s.rho0 || !s.rho) die("fields", "out of memory");
+    /* file map and Tecplot record counters (source/core/main.c:245-257) */
+    s.fmap = mh_filemap_next("binData", cont);
+    if (s.fmap < 0) die(cont ? "binData/fileMap.N" : "binData/fileMap_start.txt", "missing or unreadable");
+    s.tec = cont ? mh_tec_records("binData") : 1;
+    if (s.tec < 1) s.tec = 1;
 
     /* every rank's piece of the record (file map of the record) */
     void *rec = NULL;
@@ -196,6 +364,7 @@ int main(int argc, char **argv)
                         "the interface stays off\n");
     mh_tfsf_free(&tfsf);
     GPU(mpic_init(s.run), "mpic_init");
+    probes_open(&s, &dom, &units, cont);
     printf("mandor_core: %s form, %d x %d x %d cells, %ld markers, start record %d, t = %.6f\n",
            precision == MPIC_FP64 ? "FP64 plain" : "FP32 split", s.n[0], s.n[1], s.n[2], nm, point,
            time);
@@ -221,7 +390,11 @@ int main(int argc, char **argv)
             first_check = 0;
             last_check = tn;
         }
-        GPU(mpic_step(s.run, pts + 6 * npts), "mpic_step");
+        const int tec = rc.chpoint_tecplot > 0 && t % rc.chpoint_tecplot == 0;
+        if (tec || s.nprobes)
+            step_stages(&s, pts + 6 * npts, tec);
+        else
+            GPU(mpic_step(s.run, pts + 6 * npts), "mpic_step");
         if (!isfinite(pts[6 * npts + 1] + pts[6 * npts + 2] + pts[6 * npts + 3] +
                       pts[6 * npts + 4] + pts[6 * npts + 5]))
             die("energy diagnostic", "nan detected");
@@ -243,7 +416,14 @@ int main(int argc, char **argv)
     if (saved_at != time) checkpoint(&s, next++, time);
     HOST(mh_diag_append("binData/diag.dat", pts, npts, dom.tau, fresh), "diag.dat");
     remove("tmp/stop.flag");
+    for (int p = 0; p < s.nprobes; ++p) fclose(s.pf[p]);
     mpic_destroy(s.run);
+    free(s.pf);
+    free(s.sample);
+    free(s.probe);
+    free(s.node);
+    free(s.rho);
+    free(s.rho0);
     free(pts);
     free(rec);
     free(s.E);

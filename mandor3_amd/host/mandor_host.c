@@ -275,31 +275,39 @@ int mh_mesh_load(const char *path, const int n[3], double *block)
     return rc;
 }
 
-int mh_mesh_save(const char *path, const int n[3], const double *block, const int lo[3],
-                 const int hi[3], const char *name)
+int mh_mesh_save_ex(const char *path, const int n[3], const double *block, int ncomp,
+                    const int lo[3], const int hi[3], int ghost, const char *name)
 {
+    if (ncomp != 1 && ncomp != 3) return fail("'%s': %d components per node", path, ncomp);
     FILE *fp = fopen(path, "wb");
     if (!fp) return fail("cannot create '%s'", path);
     const int32_t r[6] = {lo[0], lo[1], lo[2], hi[0], hi[1], hi[2]};
-    /* the core.out E/H meshes span [min - 1, max + 1] (source/core/main.c:109-115) */
+    /* the mesh_t of the saved mesh: storage [-ghost, n + ghost] per axis */
     disk_mesh_t m;
     memset(&m, 0, sizeof(m));
-    m.imin = -1; m.imax = n[0] + 1;
-    m.jmin = -1; m.jmax = n[1] + 1;
-    m.kmin = -1; m.kmax = n[2] + 1;
-    m.type = 1;
-    m.width_z = n[2] + 3;
-    m.width_yz = (n[1] + 3) * (n[2] + 3);
-    m.size = (int32_t)((long)(n[0] + 3) * m.width_yz * 24);
+    m.imin = -ghost; m.imax = n[0] + ghost;
+    m.jmin = -ghost; m.jmax = n[1] + ghost;
+    m.kmin = -ghost; m.kmax = n[2] + ghost;
+    m.type = ncomp == 3;
+    m.width_z = n[2] + 1 + 2 * ghost;
+    m.width_yz = (n[1] + 1 + 2 * ghost) * m.width_z;
+    m.size = (int32_t)((long)(n[0] + 1 + 2 * ghost) * m.width_yz * 8 * ncomp);
     snprintf(m.name, sizeof(m.name), "%s", name);
     int ok = fwrite(r, sizeof(int32_t), 6, fp) == 6 && fwrite(&m, sizeof(m), 1, fp) == 1;
     const long span = hi[2] - lo[2] + 1;
     for (int i = lo[0]; i <= hi[0] && ok; ++i)
         for (int j = lo[1]; j <= hi[1] && ok; ++j) {
-            const double *src = block + 3 * (((long)i * (n[1] + 1) + j) * (n[2] + 1) + lo[2]);
-            ok = fwrite(src, sizeof(double) * 3, span, fp) == (size_t)span;
+            const double *src = block + ncomp * (((long)i * (n[1] + 1) + j) * (n[2] + 1) + lo[2]);
+            ok = fwrite(src, sizeof(double) * ncomp, span, fp) == (size_t)span;
         }
     return (fclose(fp) == 0 && ok) ? 0 : fail("write error on '%s'", path);
+}
+
+int mh_mesh_save(const char *path, const int n[3], const double *block, const int lo[3],
+                 const int hi[3], const char *name)
+{
+    /* the core.out E/H meshes span [min - 1, max + 1] (source/core/main.c:109-115) */
+    return mh_mesh_save_ex(path, n, block, 3, lo, hi, 1, name);
 }
 
 /* ---- marker records, source/core/plasma_IO.c:273-287 ------------------- */
@@ -394,32 +402,230 @@ static int write_text(const char *path, const char *text)
     return (fclose(fp) == 0 && ok) ? 0 : fail("write error on '%s'", path);
 }
 
-int mh_sys_write(const char *bindir, int rec, double time, const int n[3], int setup_mode)
+int mh_filemap_next(const char *bindir, int cont)
+{
+    char p[1024];
+    snprintf(p, sizeof(p), "%s/%s", bindir, cont ? "fileMap.N" : "fileMap_start.txt");
+    FILE *fp = fopen(p, "rt");
+    if (!fp) return -1;
+    int v = -1;
+    if (fscanf(fp, " @ %d", &v) != 1) v = -1;
+    fclose(fp);
+    return v;
+}
+
+int mh_filemap_save(const char *bindir, int id, const int n[3], int setup_mode)
 {
     char p[1024], t[512];
-    /* one file map per record: the whole domain on rank 0 */
-    snprintf(p, sizeof(p), "%s/fileMap_%06d.txt", bindir, rec);
+    snprintf(p, sizeof(p), "%s/fileMap_%06d.txt", bindir, id);
     snprintf(t, sizeof(t), "1 cpus total.\n0: (0 0 0) - (%d %d %d)\n", n[0], n[1], n[2]);
-    if (write_text(p, t)) return -1;
-    snprintf(p, sizeof(p), "%s/fileMap.N", bindir);
-    snprintf(t, sizeof(t), "@ %d filemaps total is written.\n", rec + 1);
     if (write_text(p, t)) return -1;
     if (setup_mode) {
         snprintf(p, sizeof(p), "%s/fileMap_start.txt", bindir);
         snprintf(t, sizeof(t),
                  "@ %d filemaps are generated by setup - keep them on the start of simulation.\n",
-                 rec + 1);
+                 id + 1);
         if (write_text(p, t)) return -1;
     }
+    snprintf(p, sizeof(p), "%s/fileMap.N", bindir);
+    snprintf(t, sizeof(t), "@ %d filemaps total is written.\n", id + 1);
+    if (write_text(p, t)) return -1;
+    return id;
+}
+
+/* partition_save of one rank (source/misc_partition.c:422-435,
+ * source/misc_partitionPack.c:35-76): the packed size, then the domain box,
+ * no split axis, 1 cpu, 1 tree node; the root record is depth 0, cpus
+ * [0, 0], no split, next record at 18.  The reference leaves the children
+ * offset of a leaf unset; it is written as 0 here. */
+static int prtn_save(const char *path, const int n[3])
+{
+    const int32_t v[19] = {18, 0, 0, 0, n[0], n[1], n[2], 0, 0, 0, 1, 1, 0, 0, 0, 0, 0, 0, 18};
+    FILE *fp = fopen(path, "wb");
+    if (!fp) return fail("cannot create '%s'", path);
+    const int ok = fwrite(v, sizeof(int32_t), 19, fp) == 19;
+    return (fclose(fp) == 0 && ok) ? 0 : fail("write error on '%s'", path);
+}
+
+int mh_sys_write(const char *bindir, int rec, double time, const int n[3], int fmap)
+{
+    char p[1024], t[512];
+    snprintf(p, sizeof(p), "%s/sys_prtn_%06d.bin", bindir, rec);
+    if (prtn_save(p, n)) return -1;
     snprintf(p, sizeof(p), "%s/sys_%06d.txt", bindir, rec);
     snprintf(t, sizeof(t),
              "@ %.14e    time.\n@ %d       cpus total.\n@ %d       file map id.\n"
              "That is checkpoint (record of the full system state) #%d.\n",
-             time, 1, rec, rec);
+             time, 1, fmap, rec);
     if (write_text(p, t)) return -1;
     snprintf(p, sizeof(p), "%s/sys.N", bindir);
     snprintf(t, sizeof(t), "@ %d\t  system check-point records written.\n", rec + 1);
     return write_text(p, t);
+}
+
+int mh_tec_write(const char *bindir, int rec, double time, const int n[3], int fmap)
+{
+    char p[1024];
+    snprintf(p, sizeof(p), "%s/tec_%06d.txt", bindir, rec);
+    FILE *fp = fopen(p, "wb");
+    if (!fp) return fail("cannot create '%s'", p);
+    const int32_t cf[2] = {1, fmap};
+    int ok = fwrite(&time, sizeof(double), 1, fp) == 1 && fwrite(cf, sizeof(int32_t), 2, fp) == 2;
+    if (fclose(fp) != 0 || !ok) return fail("write error on '%s'", p);
+    snprintf(p, sizeof(p), "%s/tec_prtn_%06d.bin", bindir, rec);
+    return prtn_save(p, n);
+}
+
+int mh_tec_count(const char *bindir, int count)
+{
+    char p[1024], t[128];
+    snprintf(p, sizeof(p), "%s/tecplot.N", bindir);
+    snprintf(t, sizeof(t), "@ %d  tecplots' records written.\n", count);
+    return write_text(p, t);
+}
+
+int mh_tec_records(const char *bindir)
+{
+    char p[1024];
+    snprintf(p, sizeof(p), "%s/tecplot.N", bindir);
+    FILE *fp = fopen(p, "rt");
+    if (!fp) return -1;
+    int v = -1;
+    if (fscanf(fp, " @ %d", &v) != 1) v = -1;
+    fclose(fp);
+    return v;
+}
+
+/* run_probes.cfg (source/core/diag_probe.c:155-330) */
+static int probe_add(int **ijk, int *n, int *cap, const mh_domain_t *d, const mh_units_t *u,
+                     double x, double y, double z)
+{
+    if (*n == *cap) {
+        *cap = *cap ? 2 * *cap : 64;
+        int *g = (int *)realloc(*ijk, sizeof(int) * 3 * (size_t)*cap);
+        if (!g) return fail("out of memory");
+        *ijk = g;
+    }
+    const double r[3] = {x, y, z};
+    for (int a = 0; a < 3; ++a)
+        (*ijk)[3 * *n + a] = (int)(r[a] * u->micron / d->h[a] + 128.0) - 128;
+    ++*n;
+    return 0;
+}
+
+int mh_probes_parse(const char *path, const mh_domain_t *d, const mh_units_t *u, int **ijk, int *n)
+{
+    *ijk = NULL;
+    *n = 0;
+    FILE *fp = fopen(path, "rt");
+    if (!fp) return 0;                       /* no file: no probes */
+    /* the '@' words and '>' values in file order */
+    char line[4096];
+    char kind[4096];
+    double *v = NULL;
+    int nv = 0, vcap = 0, cap = 0, rc = 0;
+    char words[64][16];
+    int nw = 0, wat[64];
+    while (fgets(line, sizeof(line), fp) && !rc) {
+        const char *s = line;
+        while (*s == ' ' || *s == '\t') ++s;
+        if (*s == '@') {
+            if (sscanf(s + 1, "%4095s", kind) != 1) continue;
+            if (nw == 64) {
+                rc = fail("'%s': too many probe groups", path);
+                break;
+            }
+            snprintf(words[nw], sizeof(words[nw]), "%s", kind);
+            wat[nw++] = nv;
+            if (!strcmp(kind, "end")) break;
+        } else if (*s == '>') {
+            if (nv == vcap) {
+                vcap = vcap ? 2 * vcap : 64;
+                double *g = (double *)realloc(v, sizeof(double) * vcap);
+                if (!g) {
+                    rc = fail("out of memory");
+                    break;
+                }
+                v = g;
+            }
+            if (sscanf(s + 1, "%lf", v + nv) != 1) {
+                rc = fail("'%s': bad probe value '%s'", path, s);
+                break;
+            }
+            ++nv;
+        }
+    }
+    fclose(fp);
+    for (int w = 0; w < nw && !rc; ++w) {
+        const double *a = v + wat[w];
+        const int have = (w + 1 < nw ? wat[w + 1] : nv) - wat[w];
+        const char *k = words[w];
+        if (!strcmp(k, "end")) break;
+        if (!strcmp(k, "point")) {
+            if (have < 3) rc = fail("'%s': a point needs 3 values", path);
+            else rc = probe_add(ijk, n, &cap, d, u, a[0], a[1], a[2]);
+        } else if (!strcmp(k, "line")) {
+            if (have < 7) {
+                rc = fail("'%s': a line needs 7 values", path);
+                break;
+            }
+            const int N = 2 + abs((int)a[6]);
+            double x = a[0], y = a[1], z = a[2];
+            const double dx = (a[3] - x) / (N - 1), dy = (a[4] - y) / (N - 1), dz = (a[5] - z) / (N - 1);
+            for (int i = 0; i < N && !rc; ++i, x += dx, y += dy, z += dz)
+                rc = probe_add(ijk, n, &cap, d, u, x, y, z);
+        } else if (!strcmp(k, "rectangle")) {
+            if (have < 11) {
+                rc = fail("'%s': a rectangle needs 11 values", path);
+                break;
+            }
+            const int Np = 2 + abs((int)a[9]), Nq = 2 + abs((int)a[10]);
+            const double px = (a[3] - a[0]) / (Np - 1), py = (a[4] - a[1]) / (Np - 1),
+                         pz = (a[5] - a[2]) / (Np - 1), qx = (a[6] - a[0]) / (Nq - 1),
+                         qy = (a[7] - a[1]) / (Nq - 1), qz = (a[8] - a[2]) / (Nq - 1);
+            for (int p = 0; p < Np && !rc; ++p)
+                for (int q = 0; q < Nq && !rc; ++q)
+                    rc = probe_add(ijk, n, &cap, d, u, a[0] + p * px + q * qx, a[1] + p * py + q * qy,
+                                   a[2] + p * pz + q * qz);
+        } else if (!strcmp(k, "volume")) {
+            if (have < 15) {
+                rc = fail("'%s': a volume needs 15 values", path);
+                break;
+            }
+            const int Np = 2 + abs((int)a[12]), Nq = 2 + abs((int)a[13]), Nr = 2 + abs((int)a[14]);
+            const double px = (a[3] - a[0]) / (Np - 1), py = (a[4] - a[1]) / (Np - 1),
+                         pz = (a[5] - a[2]) / (Np - 1), qx = (a[6] - a[0]) / (Nq - 1),
+                         qy = (a[7] - a[1]) / (Nq - 1), qz = (a[8] - a[2]) / (Nq - 1),
+                         rx = (a[9] - a[0]) / (Nr - 1), ry = (a[10] - a[1]) / (Nr - 1),
+                         rz = (a[11] - a[2]) / (Nr - 1);
+            for (int p = 0; p < Np && !rc; ++p)
+                for (int q = 0; q < Nq && !rc; ++q)
+                    for (int r = 0; r < Nr && !rc; ++r)
+                        rc = probe_add(ijk, n, &cap, d, u, a[0] + p * px + q * qx + r * rx,
+                                       a[1] + p * py + q * qy + r * ry, a[2] + p * pz + q * qz + r * rz);
+        } else {
+            rc = fail("'%s': bad probe group '%s' (point, line, rectangle, volume or end)", path, k);
+        }
+    }
+    free(v);
+    if (rc) {
+        free(*ijk);
+        *ijk = NULL;
+        *n = 0;
+        return rc;
+    }
+    /* repeated nodes: the later one replaced by the last probe, which is
+     * tested in its place (diag_probe.c:302-308) */
+    for (int p1 = 0; p1 < *n; ++p1)
+        for (int p2 = p1 + 1; p2 < *n; ++p2) {
+            int *A = *ijk + 3 * p1, *B = *ijk + 3 * p2;
+            if (A[0] == B[0] && A[1] == B[1] && A[2] == B[2]) {
+                --*n;
+                memcpy(B, *ijk + 3 * *n, sizeof(int) * 3);
+                --p2;
+            }
+        }
+    return 0;
 }
 
 /* ---- diag.dat, source/core/diag_WDensity.c:81-105 ---------------------- */

@@ -79,6 +79,11 @@ int  mh_units_save(const char *path, const mh_units_t *u);
 int mh_mesh_load(const char *path, const int n[3], double *block);
 int mh_mesh_save(const char *path, const int n[3], const double *block, const int lo[3],
                  const int hi[3], const char *name);
+/* The same for a vector (ncomp 3) or scalar (ncomp 1) block [0,n] whose
+ * mesh_t spans [-ghost, n+ghost] (core.out E/H: 1; J, rho and every setup
+ * mesh: 2, source/core/main.c:109-115, source/setup/main.c:76-80). */
+int mh_mesh_save_ex(const char *path, const int n[3], const double *block, int ncomp,
+                    const int lo[3], const int hi[3], int ghost, const char *name);
 
 /* plasma_%06d_%03d.bin (source/core/plasma_IO.c:273-287): int64 N, then N
  * marker_t records (56 B, or 64 B with the tracer id).  load returns the
@@ -86,12 +91,29 @@ int mh_mesh_save(const char *path, const int n[3], const double *block, const in
 int mh_markers_load(const char *path, void **rec, long *n, int *recsize);
 int mh_markers_save(const char *path, const void *rec, long n, int recsize);
 
-/* sys_%06d.txt, sys.N, fileMap_%06d.txt, fileMap.N, fileMap_start.txt
- * (source/IO_sys.c:99-124, source/IO_fileMap.c:130-160): one rank owning
- * the whole domain [0, n]. */
+/* Record descriptors of one rank owning the whole domain [0, n].
+ * File maps (source/IO_fileMap.c:40-160): checkpoints and Tecplot records
+ * draw their ids from one counter; mh_filemap_next gives the first id of a
+ * run (fileMap_start.txt, or fileMap.N when continuing; -1 if absent) and
+ * mh_filemap_save writes fileMap_%06d.txt, fileMap.N and, for setup,
+ * fileMap_start.txt.  mh_sys_write: sys_%06d.txt, sys.N and sys_prtn_%06d.bin
+ * (source/IO_sys.c:99-124); mh_tec_write: tec_%06d.txt {double time, int
+ * cpus, int file map} and tec_prtn_%06d.bin (source/IO_tecplot.c:225-245);
+ * mh_tec_count / mh_tec_records write / read tecplot.N (:259-263). */
 int mh_sys_records(const char *bindir);                 /* sys.N, -1 if absent */
 int mh_sys_info(const char *bindir, int rec, double *time, int *cpus, int *fmap);
-int mh_sys_write(const char *bindir, int rec, double time, const int n[3], int setup_mode);
+int mh_filemap_next(const char *bindir, int cont);
+int mh_filemap_save(const char *bindir, int id, const int n[3], int setup_mode);
+int mh_sys_write(const char *bindir, int rec, double time, const int n[3], int fmap);
+int mh_tec_write(const char *bindir, int rec, double time, const int n[3], int fmap);
+int mh_tec_count(const char *bindir, int count);
+int mh_tec_records(const char *bindir);
+
+/* run_probes.cfg (source/core/diag_probe.c:171-330): '@ point', '@ line',
+ * '@ rectangle', '@ volume' groups of '>' values [micron] up to '@ end',
+ * turned into nodes (floor(x/h) with the +128 trick) without repeats.
+ * *ijk is malloc'd (3 ints per probe); no file means no probes. */
+int mh_probes_parse(const char *path, const mh_domain_t *d, const mh_units_t *u, int **ijk, int *n);
 
 /* Energy history (source/core/diag_WDensity.c:30-158): points {t, W_E, W_M,
  * WTx, WTy, WTz} merged into binData/diag.dat (Tecplot ASCII, 10 columns,

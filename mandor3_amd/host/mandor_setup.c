@@ -1183,6 +1183,67 @@ static void tag_laser(cur_t *c)
     write_laser(c, p, v[1] * U.micron);
 }
 
+/* ---- record 0 ----------------------------------------------------------- */
+
+/* plasma_rho (source/setup/plasma.c:129-144): CIC charge density of the
+ * markers in record order on the nodes [-2, n+2] (k fastest), each marker's
+ * eight shares added x fastest, then z (plasmaRho_add with the staged
+ * mf_PIC_distribute of source/misc_PIC.h:164-201), then the wall folds in
+ * the order xMin xMax yMin yMax zMin zMax (plasmaRho_applyBoundary,
+ * source/core/plasma_rho.c:18-96). */
+static void cic_rho(const rec64_t *M, long nm, double *rho)
+{
+    const long W[3] = {(long)(NC[1] + 5) * (NC[2] + 5), NC[2] + 5, 1};
+#define RHO(i, j, k) rho[((long)(i) + 2) * W[0] + ((long)(j) + 2) * W[1] + (k) + 2]
+    for (long q = 0; q < nm; ++q) {
+        const double r[3] = {M[q].x, M[q].y, M[q].z};
+        int c[3];
+        double s[3];
+        for (int a = 0; a < 3; ++a) {
+            s[a] = r[a] / D.h[a];
+            c[a] = (int)(s[a] + 128.0) - 128;
+            s[a] -= c[a];
+        }
+        const double v = M[q].rho;
+        for (int dk = 0; dk < 2; ++dk) {
+            const double ez = dk ? s[2] * v : (1 - s[2]) * v;
+            for (int dj = 0; dj < 2; ++dj) {
+                const double ey = dj ? s[1] * ez : (1 - s[1]) * ez;
+                RHO(c[0], c[1] + dj, c[2] + dk) += (1 - s[0]) * ey;
+                RHO(c[0] + 1, c[1] + dj, c[2] + dk) += s[0] * ey;
+            }
+        }
+    }
+    for (int b = 0; b < 6; ++b) {
+        const int ea = b >> 1, top = b & 1, ep = ea == 0 ? 1 : 0, eq = ea == 2 ? 1 : 2;
+        const int bc = top ? D.bc_max[ea] : D.bc_min[ea], N = NC[ea];
+        if (bc == 0 && top) continue;            /* periodic: folded once, at the min wall */
+        for (int p = -2; p <= NC[ep] + 2; ++p)
+            for (int q = -2; q <= NC[eq] + 2; ++q) {
+                long o[3];
+                o[ep] = p;
+                o[eq] = q;
+#define AT(a) (o[ea] = (a), &RHO(o[0], o[1], o[2]))
+                if (bc == 0) {
+                    for (int a = -2; a <= 2; ++a) {
+                        double *lo = AT(a), *hi = AT(a + N);
+                        *hi += *lo;
+                        *lo = *hi;
+                    }
+                } else {
+                    const int pl = top ? N : 0, so = top ? 1 : -1;
+                    *AT(pl - so) += *AT(pl + so);
+                    *AT(pl - 2 * so) += *AT(pl + 2 * so);
+                    *AT(pl) = 0;
+                    *AT(pl + so) = -*AT(pl - so);
+                    *AT(pl + 2 * so) = -*AT(pl - 2 * so);
+                }
+#undef AT
+            }
+    }
+#undef RHO
+}
+
 /* ---- main --------------------------------------------------------------- */
 
 static void clear_old_outputs(void)
@@ -1283,16 +1344,38 @@ int main(int argc, char **argv)
     HOST(mh_domain_save("binData/domain.bin", &D), "domain.bin");
     HOST(mh_units_save("binData/units.bin", &U), "units.bin");
     const long nodes = (long)(NC[0] + 1) * (NC[1] + 1) * (NC[2] + 1);
+    const long gnodes = (long)(NC[0] + 5) * (NC[1] + 5) * (NC[2] + 5);
     double *zero = (double *)calloc(3 * (size_t)nodes, sizeof(double));
-    if (!zero) die("fields", "out of memory");
+    double *rho = (double *)calloc((size_t)gnodes, sizeof(double));
+    double *rho0 = (double *)malloc(sizeof(double) * (size_t)nodes);
+    if (!zero || !rho || !rho0) die("fields", "out of memory");
+    cic_rho(M, nm, rho);
+    for (int i = 0, q = 0; i <= NC[0]; ++i)
+        for (int j = 0; j <= NC[1]; ++j)
+            for (int k = 0; k <= NC[2]; ++k, ++q)
+                rho0[q] = rho[(((long)i + 2) * (NC[1] + 5) + j + 2) * (NC[2] + 5) + k + 2];
     const int lo[3] = {0, 0, 0};
-    HOST(mh_mesh_save("binData/sys_E_000000(000).bin", NC, zero, lo, NC, "E"), "E record");
-    HOST(mh_mesh_save("binData/sys_H_000000(000).bin", NC, zero, lo, NC, "H"), "H record");
-    HOST(mh_markers_save("binData/plasma_000000_000.bin", M, nm, (int)sizeof(rec64_t)), "plasma record");
+    /* Tecplot record 0 (tecIO_saveFields / tecIO_saveCurrents): file map 0;
+     * the setup meshes span [-2, n+2] (source/setup/main.c:76-80) */
+    HOST(mh_filemap_save("binData", 0, NC, 1) < 0, "file map");
+    HOST(mh_mesh_save_ex("binData/tec_E_000000(000).bin", NC, zero, 3, lo, NC, 2, "setup_E"), "tec_E");
+    HOST(mh_mesh_save_ex("binData/tec_H_000000(000).bin", NC, zero, 3, lo, NC, 2, "setup_H"), "tec_H");
+    HOST(mh_tec_write("binData", 0, 0.0, NC, 0), "tec record descriptor");
+    HOST(mh_mesh_save_ex("binData/tec_J_000000(000).bin", NC, zero, 3, lo, NC, 2, "setup_J"), "tec_J");
+    HOST(mh_mesh_save_ex("binData/tec_rho_000000(000).bin", NC, rho0, 1, lo, NC, 2, "setup_rho"),
+         "tec_rho");
+    HOST(mh_tec_count("binData", 1), "tecplot.N");
+    /* checkpoint 0 (sysIO_save, plasma_save): file map 1 */
+    HOST(mh_filemap_save("binData", 1, NC, 1) < 0, "file map");
+    HOST(mh_mesh_save_ex("binData/sys_E_000000(000).bin", NC, zero, 3, lo, NC, 2, "setup_E"), "E record");
+    HOST(mh_mesh_save_ex("binData/sys_H_000000(000).bin", NC, zero, 3, lo, NC, 2, "setup_H"), "H record");
     HOST(mh_sys_write("binData", 0, 0.0, NC, 1), "record descriptor");
+    HOST(mh_markers_save("binData/plasma_000000_000.bin", M, nm, (int)sizeof(rec64_t)), "plasma record");
+    free(rho);
+    free(rho0);
     FILE *fp = fopen("binData/tracer.inf", "wt");      /* source/setup/main.c:245-250 */
     if (!fp) die("binData/tracer.inf", "cannot create");
-    fprintf(fp, "%le \ttau (time step)\n%d  \tnumber of trajectories", D.tau, 0);
+    fprintf(fp, "%le \t\xcf\x84 (time step)\n%d  \tnumber of trajectories", D.tau, 0);
     fclose(fp);
     free(zero);
     free(M);

@@ -11,7 +11,10 @@ create`` and keeps:
   * the marker records of plasma_000000_000.bin without the padding word
     (the reference leaves it uninitialised),
   * the '@' values of .gaussSpot.cfg, .EM_sources/TFSF/TFSF.cfg, filter.cfg
-    and openFaces.cfg where the deck writes them.
+    and openFaces.cfg where the deck writes them,
+  * the record descriptors (sys_000000.txt, sys.N, file maps, partition
+    files, tec_000000.txt, tecplot.N, tracer.inf) and the charge density of
+    the Tecplot record 0.
 
 [DF:uniform] momenta are drawn with rand() seeded from the wall clock
 (source/misc_partition.c:255), so the decks use it only with zero spread,
@@ -119,6 +122,11 @@ CONTROL = {"gaussSpot": ".gaussSpot.cfg", "TFSF": ".EM_sources/TFSF/TFSF.cfg",
            "filter": ".EM_sources/TFSF/filter.cfg", "openFaces": ".EM_sources/TFSF/openFaces.cfg"}
 
 
+RECORD_FILES = ("sys_000000.txt", "sys.N", "sys_prtn_000000.bin", "fileMap_000000.txt",
+                "fileMap_000001.txt", "fileMap.N", "fileMap_start.txt", "tec_000000.txt",
+                "tec_prtn_000000.bin", "tecplot.N", "tracer.inf")
+
+
 def at_values(path):
     """The first word of every '@' line of a control file."""
     return [ln[1:].split()[0] for ln in open(path).read().splitlines() if ln.startswith("@")]
@@ -150,6 +158,13 @@ def run_case(name, text, root):
         p = os.path.join(wd, p)
         if os.path.exists(p):
             out["%s/%s" % (name, k)] = np.array(at_values(p))
+    # record 0 descriptors and the Tecplot record 0 (rho payload: the mesh_t
+    # header holds pointers, so only the region and the data are kept)
+    for f in RECORD_FILES:
+        out["%s/file/%s" % (name, f)] = np.frombuffer(open(os.path.join(wd, "binData", f), "rb").read(),
+                                                      np.uint8)
+    raw = open(os.path.join(wd, "binData", "tec_rho_000000(000).bin"), "rb").read()
+    out[name + "/tec_rho"] = np.frombuffer(raw[24 + 88:], "<f8")
     return out
 
 

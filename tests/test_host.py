@@ -162,7 +162,7 @@ def test_setup_record_zero(host, tmp_path):
     assert abs(i["vx"].mean()) < 0.01 * 0.2
     assert 0.8 * 0.01 < i["vx"].std() < 1.2 * 0.01
     t, cpus, fmap = records.sys_info(b / "sys_000000.txt")
-    assert (t, cpus, fmap) == (0.0, 1, 0)
+    assert (t, cpus, fmap) == (0.0, 1, 1)          # file map 0 is the Tecplot record 0's
     assert "@ 1" in (b / "sys.N").read_text()
     assert (b / "fileMap_000000.txt").read_text().startswith("1 cpus total.")
 

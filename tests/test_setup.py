@@ -4,8 +4,10 @@ Record 0 of every deck in tests/golden/setup_fixtures.npz (written by
 tests/golden/make_setup_fixtures.py from the reference build) is reproduced
 bit for bit: domain.bin, units.bin, every marker field but the padding word
 in the reference's order (plasma objects newest first, box / plane / sphere
-/ cylinder filters, foils, two-stream beams, gradients), and the values of
-.gaussSpot.cfg, TFSF.cfg, filter.cfg and openFaces.cfg.  The decks of the
+/ cylinder filters, foils, two-stream beams, gradients), the values of
+.gaussSpot.cfg, TFSF.cfg, filter.cfg and openFaces.cfg, the record
+descriptors and file maps, and the Tecplot record 0 with its CIC charge
+density.  The decks of the
 time-step fixtures (tests/golden/<case>.npz) are reproduced too, momenta
 aside: the reference draws [DF:uniform] momenta with rand() seeded from the
 wall clock (source/misc_partition.c:255).
@@ -81,6 +83,24 @@ def test_record0_matches_reference(host, tmp_path, case):
             assert at_values(path) == list(fx[key]), (case, k)
         else:
             assert not os.path.exists(path), (case, k)
+    # record descriptors, file maps, partition files (the reference leaves
+    # the children offset of a leaf unset: int 17 of a partition file)
+    for key in fx.files:
+        if not key.startswith(case + "/file/"):
+            continue
+        f = key.split("/file/")[1]
+        mine = np.frombuffer(open(os.path.join(b, f), "rb").read(), np.uint8)
+        ref = fx[key]
+        if "prtn" in f:
+            mine, ref = mine.view("<i4").copy(), ref.view("<i4").copy()
+            mine[17] = ref[17] = 0
+        assert same_bits(mine, ref), (case, f)
+    # Tecplot record 0: E, H and J zero, rho the CIC density of the markers
+    raw = open(os.path.join(b, "tec_rho_000000(000).bin"), "rb").read()
+    assert same_bits(np.frombuffer(raw[24 + 88:], "<f8"), fx[case + "/tec_rho"])
+    for q in "EHJ":
+        raw = open(os.path.join(b, "tec_%s_000000(000).bin" % q), "rb").read()
+        assert not np.frombuffer(raw[24 + 88:], "<f8").any()
 
 
 @pytest.mark.parametrize("case", ["periodic", "laser_mur", "spot", "tfsf", "mixed"])
