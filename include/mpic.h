@@ -158,6 +158,21 @@ int  mpic_autotune(mpic_run_t *run, int steps, int *chosen);
 int  mpic_sync(mpic_run_t *run);
 double mpic_time(mpic_run_t *run);
 
+/* TFSF record mode (TFSF_turnRec / TFSF_preHStep / TFSF_writeMesh,
+ * source/core/em_TFSF.c:273-330, 573-624): at the start of every step (in
+ * mpic_em_h_half, before the boundary caps) the tangential E on the 12 E
+ * faces and H on the 12 H faces of the interface box reg = {imin, imax,
+ * jmin, jmax, kmin, kmax} are stored on the device, each value times
+ * (1-2 top)(1-2 comp) tau/h of the face's axis, for up to max_frames steps.
+ * Call before mpic_init; excludes mpic_add_tfsf on the same run. */
+int  mpic_tfsf_record(mpic_run_t *run, const int reg[6], int max_frames);
+/* The recording so far: *nframes; vol (may be NULL) the node count of face
+ * slot q (E faces q = 0..11, H faces 12..23, slot axis*4 + top*2 + comp);
+ * out (may be NULL) every slot's frames in slot order, each nframes x vol[q]
+ * doubles (the tangential index ep = (axis+1)%3 outer, eq = (axis+2)%3
+ * inner): the payloads of the reference's fragment files. */
+int  mpic_tfsf_recorded(mpic_run_t *run, int *nframes, long vol[24], double *out);
+
 /* CIC charge density with boundary fold (plasma_rho), nodes [-2,n+2]. */
 int  mpic_rho(mpic_run_t *run, double *rho);
 /* Tecplot payloads computed on the device (tecIO_saveVec with

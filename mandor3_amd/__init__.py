@@ -79,6 +79,8 @@ def lib():
             "mpic_time": (D, [P]),
             "mpic_rho": (I, [P, P]),
             "mpic_node_fields": (I, [P, I, P]),
+            "mpic_tfsf_record": (I, [P, P, I]),
+            "mpic_tfsf_recorded": (I, [P, P, P, P]),
             "mpic_set_probes": (I, [P, P, I]),
             "mpic_probe_sample": (I, [P, P]),
             "mpic_gauss_before": (I, [P]),
@@ -263,6 +265,23 @@ class Run:
         r = np.empty(int(np.prod([v + 5 for v in self.n])))
         _check(lib().mpic_rho(self.handle, _ptr(r)), "mpic_rho")
         return r
+
+    def tfsf_record(self, reg, max_frames):
+        r = np.ascontiguousarray(reg, dtype=np.int32)
+        _check(lib().mpic_tfsf_record(self.handle, _ptr(r), int(max_frames)), "mpic_tfsf_record")
+
+    def tfsf_recorded(self):
+        """(frames, [24 arrays of shape (frames, face nodes)]): E faces then H faces."""
+        nf = np.zeros(1, np.int32)
+        vol = np.zeros(24, np.int64)
+        _check(lib().mpic_tfsf_recorded(self.handle, _ptr(nf), _ptr(vol), None), "mpic_tfsf_recorded")
+        out = np.empty(int(nf[0] * vol.sum()))
+        _check(lib().mpic_tfsf_recorded(self.handle, _ptr(nf), _ptr(vol), _ptr(out)), "mpic_tfsf_recorded")
+        faces, at = [], 0
+        for v in vol:
+            faces.append(out[at:at + nf[0] * v].reshape(nf[0], v))
+            at += nf[0] * v
+        return int(nf[0]), faces
 
     def node_fields(self, what):
         """Node values of E (0), H (1) or J (2) on the nodes [0, n], as the

@@ -553,6 +553,49 @@ int launch_tfsf(hipStream_t s, T *const F[3], const Grid &g, const TfsfSet &fs, 
     return MPIC_OK;
 }
 
+template <class T>
+struct Ptr3 {
+    T *p[3];
+};
+
+/* TFSF recording (TFSF_preHStep / TFSF_writeMesh, source/core/em_TFSF.c:
+ * 573-624): at the start of a step, E on the 12 E faces and H on the 12 H
+ * faces (blockIdx.y = slot, E faces first), each value times its face's
+ * factor, into frame (*step - step0) of the face's frames (ep outer, eq
+ * inner).  Frames past 'cap' are not stored. */
+template <class T>
+__global__ void k_tfsf_record(Ptr3<T> E, Ptr3<T> H, Grid g, TfsfSet fe, TfsfSet fh,
+                              const double *factor, double *out, const int *step, int step0,
+                              int cap)
+{
+    const int slot = blockIdx.y, f = slot % 12;
+    const TfsfFace &fc = slot < 12 ? fe.f[f] : fh.f[f];
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= fc.vol) return;
+    const int frame = *step - step0;
+    if (frame < 0 || frame >= cap) return;
+    int r[3];
+    r[fc.ea] = fc.lo[fc.ea];
+    const unsigned pq = (unsigned)t / (unsigned)fc.dQ;
+    r[fc.ep] = fc.lo[fc.ep] + (int)pq;
+    r[fc.eq] = fc.lo[fc.eq] + (int)((unsigned)t - pq * (unsigned)fc.dQ);
+    const T *F = slot < 12 ? E.p[fc.comp] : H.p[fc.comp];
+    out[fc.off + (long)frame * fc.vol + t] = (double)F[gidx(g, r[0], r[1], r[2])] * factor[f];
+}
+
+template <class T>
+int launch_tfsf_record(hipStream_t s, T *const E[3], T *const H[3], const Grid &g,
+                       const TfsfSet &fe, const TfsfSet &fh, long maxvol, const double *factor,
+                       double *out, const int *step, int step0, int cap)
+{
+    if (maxvol <= 0) return MPIC_OK;
+    Ptr3<T> Pe = {{E[0], E[1], E[2]}}, Ph = {{H[0], H[1], H[2]}};
+    k_tfsf_record<T><<<dim3(nblk(maxvol), 24), TPB, 0, s>>>(Pe, Ph, g, fe, fh, factor, out, step,
+                                                            step0, cap);
+    MPIC_CHECK(hipGetLastError());
+    return MPIC_OK;
+}
+
 /* The run's step counter, advanced once per step on the stream. */
 __global__ void k_tick(int *step)
 {
@@ -799,11 +842,6 @@ __device__ static inline void node_values(T *const F[3], const Grid &g, int kind
     }
 }
 
-template <class T>
-struct Ptr3 {
-    T *p[3];
-};
-
 /* nodes [0, n] (k fastest), three interleaved doubles per node */
 template <class T>
 __global__ void k_node_fields(Ptr3<T> F, Grid g, int kind, double *out)
@@ -880,6 +918,9 @@ int launch_probes(hipStream_t s, T *const E[3], T *const H[3], const Grid &g, co
     template int launch_gauss<T>(hipStream_t, const FieldsT<T> &, const double *,                \
                                  const double *, const Grid &, double *);                       \
     template int launch_node_fields<T>(hipStream_t, T *const[3], const Grid &, int, double *);   \
+    template int launch_tfsf_record<T>(hipStream_t, T *const[3], T *const[3], const Grid &,      \
+                                       const TfsfSet &, const TfsfSet &, long, const double *,  \
+                                       double *, const int *, int, int);                        \
     template int launch_probes<T>(hipStream_t, T *const[3], T *const[3], const Grid &,           \
                                   const int *, int, double *);
 INST(double)

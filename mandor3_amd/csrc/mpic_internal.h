@@ -169,6 +169,10 @@ int launch_jfold(hipStream_t s, const JPtrs &f, const Grid &g, int ea, int bc0, 
 template <class T>
 int launch_node_fields(hipStream_t s, T *const F[3], const Grid &g, int kind, double *out);
 template <class T>
+int launch_tfsf_record(hipStream_t s, T *const E[3], T *const H[3], const Grid &g,
+                       const TfsfSet &fe, const TfsfSet &fh, long maxvol, const double *factor,
+                       double *out, const int *step, int step0, int cap);
+template <class T>
 int launch_probes(hipStream_t s, T *const E[3], T *const H[3], const Grid &g, const int *ijk,
                   int np, double *out);
 template <class T> int launch_energy(hipStream_t s, const FieldsT<T> &f, const Grid &g, double *acc);

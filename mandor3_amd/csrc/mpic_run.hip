@@ -82,6 +82,16 @@ struct Tfsf {
     int    *htab;                  /* pinned staging of dtab */
 };
 
+struct TfsfRec {
+    int     on;
+    long    step0;                 /* run step of frame 0 */
+    int     cap;                   /* frames the device buffer holds */
+    TfsfSet fe, fh;                /* faces; off = offset of the face's frames */
+    long    maxvol, total;         /* nodes of the largest face; doubles per frame */
+    double *dbuf;                  /* [slot][frame][node], E slots then H slots */
+    double *dfac;                  /* factor per slot f: (1-2 top)(1-2 comp) tau/h[axis] */
+};
+
 struct CapBoxes {             /* one field's six caps */
     Box upd[6], fl[6];        /* original boxes */
     Box upd_s[6], fl_s[6];    /* after overlap removal */
@@ -131,6 +141,7 @@ struct mpic_run {
     int          *dstep;      /* device step counter (indexes the laser tables) */
     Spot          spot;
     Tfsf          tfsf;
+    TfsfRec       trec;
     long          step;       /* host copy of the step counter */
     long          tab_lo, tab_hi;  /* steps the laser tables hold */
     int           use_graphs; /* mpic_advance replays captured steps */
@@ -543,6 +554,8 @@ extern "C" int mpic_destroy(mpic_run_t *r)
     hipFree(r->dsample);
     hipFree(r->spot.dprof);
     hipFree(r->tfsf.ddata);
+    hipFree(r->trec.dbuf);
+    hipFree(r->trec.dfac);
     hipFree(r->tfsf.dtab);
     if (r->tfsf.htab) hipHostFree(r->tfsf.htab);
     hipFree(r->spot.dtab);
@@ -1027,6 +1040,54 @@ extern "C" int mpic_add_gauss_spot(mpic_run_t *r, const double *p)
     return MPIC_OK;
 }
 
+/* Face geometry of the interface box reg = {imin, imax, jmin, jmax, kmin,
+ * kmax}, TFSF_init (source/core/em_TFSF.c:500-537): E face of slot
+ * f = axis*4 + top*2 + comp on the box side 'top' along 'axis', component
+ * (axis + 1 + comp) % 3 with its lower corner moved up by one node along the
+ * component; the H face of the slot is the E face moved by 'top' along the
+ * axis, component 3 - cE - axis.  Every face keeps 5 cells from the domain
+ * edges (TFSF_turnPlay / tag_TFSF.c:106-117). */
+static int tfsf_faces(const Grid &g, const int reg[6], TfsfSet &E, TfsfSet &H, long &maxvol)
+{
+    const int side[2][3] = {{reg[0], reg[2], reg[4]}, {reg[1], reg[3], reg[5]}};
+    maxvol = 0;
+    for (int ea = 0; ea < 3; ++ea) {
+        const int ep = (ea + 1) % 3, eq = (ea + 2) % 3;
+        for (int top = 0; top < 2; ++top)
+            for (int comp = 0; comp < 2; ++comp) {
+                const int f = ea * 4 + top * 2 + comp;
+                TfsfFace &fe = E.f[f], &fh = H.f[f];
+                for (int a = 0; a < 3; ++a) {
+                    fe.lo[a] = side[0][a];
+                    fe.hi[a] = side[1][a];
+                }
+                fe.lo[ea] = fe.hi[ea] = side[top][ea];
+                fe.comp = (ea + 1 + comp) % 3;
+                fe.lo[fe.comp] += 1;
+                fe.ea = ea;
+                fe.ep = ep;
+                fe.eq = eq;
+                fh = fe;
+                fh.lo[ea] += top;
+                fh.hi[ea] += top;
+                fh.comp = 3 - fe.comp - ea;
+                fe.dQ = fh.dQ = fe.hi[eq] - fe.lo[eq] + 1;
+                fe.vol = fh.vol = (long)(fe.hi[ep] - fe.lo[ep] + 1) * fe.dQ;
+                for (int a = 0; a < 3; ++a)
+                    if (fe.lo[a] < 5 || fe.hi[a] > g.n[a] - 5 || fh.lo[a] < 5 || fh.hi[a] > g.n[a] - 5) {
+                        mpic_set_error("TFSF interface is too close to a boundary (less than 5 cell)");
+                        return MPIC_ERR_ARG;
+                    }
+                if (fe.vol <= 0) {
+                    mpic_set_error("TFSF: empty interface box");
+                    return MPIC_ERR_ARG;
+                }
+                if (fe.vol > maxvol) maxvol = fe.vol;
+            }
+    }
+    return MPIC_OK;
+}
+
 extern "C" int mpic_add_tfsf(mpic_run_t *r, const int reg[6], int mode, int nframes,
                              const double *const dataE[12], const double *const dataH[12])
 {
@@ -1046,49 +1107,14 @@ extern "C" int mpic_add_tfsf(mpic_run_t *r, const int reg[6], int mode, int nfra
     }
     const Grid &g = r->g;
     Tfsf &tf = r->tfsf;
-    /* face geometry, TFSF_init (source/core/em_TFSF.c:500-537) */
-    const int side[2][3] = {{reg[0], reg[2], reg[4]}, {reg[1], reg[3], reg[5]}};
+    TRY(tfsf_faces(g, reg, tf.fe, tf.fh, tf.maxvol));
     long total = 0;
-    tf.maxvol = 0;
-    for (int ea = 0; ea < 3; ++ea) {
-        const int ep = (ea + 1) % 3, eq = (ea + 2) % 3;
-        for (int top = 0; top < 2; ++top)
-            for (int comp = 0; comp < 2; ++comp) {
-                const int f = ea * 4 + top * 2 + comp;
-                TfsfFace &fe = tf.fe.f[f], &fh = tf.fh.f[f];
-                for (int a = 0; a < 3; ++a) {
-                    fe.lo[a] = side[0][a];
-                    fe.hi[a] = side[1][a];
-                }
-                fe.lo[ea] = fe.hi[ea] = side[top][ea];
-                fe.comp = (ea + 1 + comp) % 3;
-                fe.lo[fe.comp] += 1;
-                fe.ea = ea;
-                fe.ep = ep;
-                fe.eq = eq;
-                fh = fe;
-                fh.lo[ea] += top;
-                fh.hi[ea] += top;
-                fh.comp = 3 - fe.comp - ea;
-                fe.dQ = fh.dQ = fe.hi[eq] - fe.lo[eq] + 1;
-                fe.vol = fh.vol = (long)(fe.hi[ep] - fe.lo[ep] + 1) * fe.dQ;
-                /* TFSF_turnPlay: faces must keep 5 cells from the domain
-                 * edges (source/core/em_TFSF.c:402-414) */
-                for (int a = 0; a < 3; ++a)
-                    if (fe.lo[a] < 5 || fe.hi[a] > g.n[a] - 5 || fh.lo[a] < 5 || fh.hi[a] > g.n[a] - 5) {
-                        mpic_set_error("TFSF interface is too close to a boundary (less than 5 cell)");
-                        return MPIC_ERR_ARG;
-                    }
-                if (fe.vol <= 0) {
-                    mpic_set_error("mpic_add_tfsf: empty interface box");
-                    return MPIC_ERR_ARG;
-                }
-                fe.off = dataE[f] ? total : -1;
-                total += dataE[f] ? fe.vol * nframes : 0;
-                fh.off = dataH[f] ? total : -1;
-                total += dataH[f] ? fh.vol * nframes : 0;
-                if (fe.vol > tf.maxvol) tf.maxvol = fe.vol;
-            }
+    for (int f = 0; f < 12; ++f) {
+        TfsfFace &fe = tf.fe.f[f], &fh = tf.fh.f[f];
+        fe.off = dataE[f] ? total : -1;
+        total += dataE[f] ? fe.vol * nframes : 0;
+        fh.off = dataH[f] ? total : -1;
+        total += dataH[f] ? fh.vol * nframes : 0;
     }
     tf.fe.lo_in = 2;                     /* innerRegE, source/core/em.c:197-198 */
     tf.fh.lo_in = 3;                     /* innerRegH, source/core/em.c:122-123 */
@@ -1113,6 +1139,72 @@ extern "C" int mpic_add_tfsf(mpic_run_t *r, const int reg[6], int mode, int nfra
     tf.on = 1;
     r->tab_hi = r->tab_lo = 0;
     r->graphs_ok = 0;
+    return MPIC_OK;
+}
+
+/* TFSF record mode (TFSF_turnRec, source/core/em_TFSF.c:273-330). */
+extern "C" int mpic_tfsf_record(mpic_run_t *r, const int reg[6], int max_frames)
+{
+    if (!r || !reg || max_frames < 1) return MPIC_ERR_ARG;
+    if (r->inited) {
+        mpic_set_error("mpic_tfsf_record: start the recording before mpic_init");
+        return MPIC_ERR_STATE;
+    }
+    if (r->tfsf.on || r->trec.on) {
+        mpic_set_error("mpic_tfsf_record: one interface per run");
+        return MPIC_ERR_STATE;
+    }
+    const Grid &g = r->g;
+    TfsfRec &tr = r->trec;
+    TRY(tfsf_faces(g, reg, tr.fe, tr.fh, tr.maxvol));
+    long off = 0;
+    for (int f = 0; f < 12; ++f) {
+        tr.fe.f[f].off = off;
+        off += tr.fe.f[f].vol * max_frames;
+    }
+    for (int f = 0; f < 12; ++f) {
+        tr.fh.f[f].off = off;
+        off += tr.fh.f[f].vol * max_frames;
+    }
+    double fac[12];
+    for (int f = 0; f < 12; ++f) {
+        const int ea = f >> 2, top = (f >> 1) & 1, comp = f & 1;
+        fac[f] = (1 - 2 * top) * (1 - 2 * comp) * (g.tau / g.h[ea]);
+    }
+    MPIC_CHECK(hipSetDevice(r->dev));
+    MPIC_CHECK(hipMalloc(&tr.dbuf, sizeof(double) * off));
+    MPIC_CHECK(hipMalloc(&tr.dfac, sizeof(double) * 12));
+    MPIC_CHECK(hipMemcpy(tr.dfac, fac, sizeof(fac), hipMemcpyHostToDevice));
+    tr.cap = max_frames;
+    tr.step0 = r->step;
+    tr.on = 1;
+    r->graphs_ok = 0;
+    return MPIC_OK;
+}
+
+extern "C" int mpic_tfsf_recorded(mpic_run_t *r, int *nframes, long vol[24], double *out)
+{
+    if (!r || !nframes) return MPIC_ERR_ARG;
+    const TfsfRec &tr = r->trec;
+    if (!tr.on) {
+        mpic_set_error("mpic_tfsf_recorded: no recording");
+        return MPIC_ERR_STATE;
+    }
+    long done = r->step - tr.step0 + (r->stage != 0);
+    if (done > tr.cap) done = tr.cap;
+    *nframes = (int)done;
+    if (vol)
+        for (int f = 0; f < 24; ++f) vol[f] = f < 12 ? tr.fe.f[f].vol : tr.fh.f[f - 12].vol;
+    if (!out) return MPIC_OK;
+    MPIC_CHECK(hipSetDevice(r->dev));
+    MPIC_CHECK(hipStreamSynchronize(r->stream));
+    long at = 0;
+    for (int q = 0; q < 24; ++q) {
+        const TfsfFace &fc = q < 12 ? tr.fe.f[q] : tr.fh.f[q - 12];
+        MPIC_CHECK(hipMemcpy(out + at, tr.dbuf + fc.off, sizeof(double) * fc.vol * done,
+                             hipMemcpyDeviceToHost));
+        at += fc.vol * done;
+    }
     return MPIC_OK;
 }
 
@@ -1375,6 +1467,15 @@ extern "C" int mpic_em_h_half(mpic_run_t *r)
     TRY(check_stage(r, 0));
     MPIC_CHECK(hipSetDevice(r->dev));
     TRY(laser_tables(r, 1));
+    if (r->trec.on) {                    /* TFSF_preHStep in record mode, source/core/em.c:92 */
+        const TfsfRec &tr = r->trec;
+        if (r->fp32)
+            TRY(launch_tfsf_record<float>(r->stream, r->f32.E, r->f32.H, r->g, tr.fe, tr.fh, tr.maxvol,
+                                          tr.dfac, tr.dbuf, r->dstep, (int)tr.step0, tr.cap));
+        else
+            TRY(launch_tfsf_record<double>(r->stream, r->f.E, r->f.H, r->g, tr.fe, tr.fh, tr.maxvol,
+                                           tr.dfac, tr.dbuf, r->dstep, (int)tr.step0, tr.cap));
+    }
     /* cap_cacheEH: cap H, walls on cap H, cap E, walls on cap E.  Sources
      * never reach the cap layer (checked in mpic_add_laser). */
     FCALL(launch_cap_H, r->g, r->updH);

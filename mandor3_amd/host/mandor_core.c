@@ -2,6 +2,7 @@
  * mandor_core.c -- the core.out main loop on one MI355X.
  *
  *   mandor_core [--fp32] [--device N]     (run in the directory of a setup)
+ *   mandor_core --tfsf-defrag             (TFSFdefrag: joins recorded TFSF fragments)
  *
  * Reads run_mandor.cfg, binData/domain.bin, binData/units.bin, the start
  * record (record 0, or the last one with the continue flag), the laser
@@ -12,7 +13,9 @@
  * the push, J and the charge density at the end of the step), probe samples
  * (run_probes.cfg, every step after the first H half step) and the energy
  * history, and honouring tmp/stop.flag, as source/core/main.c:195-490 does
- * for one rank.  Spectral dumps are not produced (a warning says so).
+ * for one rank.  A TFSF interface in record mode stores its faces on the
+ * device every step and writes the fragment files at the end of the run.
+ * Spectral dumps are not produced (a warning says so).
  */
 #define _GNU_SOURCE
 #include <dirent.h>
@@ -239,8 +242,11 @@ int main(int argc, char **argv)
             precision = MPIC_FP32_SPLIT;
         } else if (!strcmp(argv[a], "--device") && a + 1 < argc) {
             device = atoi(argv[++a]);
+        } else if (!strcmp(argv[a], "--tfsf-defrag")) {
+            HOST(mh_tfsf_defrag(".EM_sources/TFSF"), "TFSF defragmentation");
+            return 0;
         } else {
-            fprintf(stderr, "usage: %s [--fp32] [--device N]\n", argv[0]);
+            fprintf(stderr, "usage: %s [--fp32] [--device N] | --tfsf-defrag\n", argv[0]);
             return EXIT_FAILURE;
         }
     }
@@ -359,9 +365,13 @@ int main(int argc, char **argv)
         GPU(mpic_add_tfsf(s.run, tfsf.reg, tfsf.mode, tfsf.nframes,
                           (const double *const *)tfsf.dataE, (const double *const *)tfsf.dataH),
             "mpic_add_tfsf");
-    else if (tfsf.mode == 1)
-        fprintf(stderr, "mandor_core: TFSF record mode is not produced by this driver; "
-                        "the interface stays off\n");
+    const int recording = tfsf.mode == 1;
+    int rec_reg[6];
+    memcpy(rec_reg, tfsf.reg, sizeof(rec_reg));
+    if (recording) {                                   /* TFSF_turnRec, source/core/em_TFSF.c:273-330 */
+        GPU(mpic_tfsf_record(s.run, rec_reg, rc.total_steps > 0 ? rc.total_steps : 1), "mpic_tfsf_record");
+        HOST(mh_tfsf_record_begin(".EM_sources/TFSF", rec_reg, &dom), "TFSF record");
+    }
     mh_tfsf_free(&tfsf);
     GPU(mpic_init(s.run), "mpic_init");
     probes_open(&s, &dom, &units, cont);
@@ -416,6 +426,19 @@ int main(int argc, char **argv)
     if (saved_at != time) checkpoint(&s, next++, time);
     HOST(mh_diag_append("binData/diag.dat", pts, npts, dom.tau, fresh), "diag.dat");
     remove("tmp/stop.flag");
+    if (recording) {                                   /* TFSF_shutDown, :866-890 */
+        int nf = 0;
+        GPU(mpic_tfsf_recorded(s.run, &nf, NULL, NULL), "mpic_tfsf_recorded");
+        long vol[24], total = 0;
+        GPU(mpic_tfsf_recorded(s.run, &nf, vol, NULL), "mpic_tfsf_recorded");
+        for (int q = 0; q < 24; ++q) total += vol[q] * nf;
+        double *frames = (double *)malloc(sizeof(double) * (size_t)(total > 0 ? total : 1));
+        if (!frames) die("TFSF record", "out of memory");
+        GPU(mpic_tfsf_recorded(s.run, &nf, vol, frames), "mpic_tfsf_recorded");
+        HOST(mh_tfsf_record_save(".EM_sources/TFSF", rec_reg, nf, frames), "TFSF record");
+        printf("TFSF_shutDown: %d frames written.\n", nf);
+        free(frames);
+    }
     for (int p = 0; p < s.nprobes; ++p) fclose(s.pf[p]);
     mpic_destroy(s.run);
     free(s.pf);

@@ -8,6 +8,7 @@
 #include "mandor_host.h"
 
 #include <errno.h>
+#include <limits.h>
 #include <math.h>
 #include <stdarg.h>
 #include <stdlib.h>
@@ -831,6 +832,250 @@ static int read_frames(const char *path, long count, double **out)
     }
     *out = d;
     return 0;
+}
+
+/* Node box of face slot q of the interface reg (E faces 0..11, H faces
+ * 12..23), as TFSF_init lays them out (source/core/em_TFSF.c:500-537). */
+static void tfsf_face_box(const int reg[6], int q, int lo[3], int hi[3], int *ep, int *eq)
+{
+    const int f = q % 12, ea = f >> 2, top = (f >> 1) & 1, comp = f & 1;
+    const int side[2][3] = {{reg[0], reg[2], reg[4]}, {reg[1], reg[3], reg[5]}};
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = side[0][a];
+        hi[a] = side[1][a];
+    }
+    lo[ea] = hi[ea] = side[top][ea];
+    lo[(ea + 1 + comp) % 3] += 1;
+    if (q >= 12) {
+        lo[ea] += top;
+        hi[ea] += top;
+    }
+    *ep = (ea + 1) % 3;
+    *eq = (ea + 2) % 3;
+}
+
+static const char *tfsf_face_name(int q) { return q < 12 ? TFSF_E_NAME[q] : TFSF_H_NAME[q - 12]; }
+
+int mh_tfsf_record_begin(const char *dir, const int reg[6], const mh_domain_t *d)
+{
+    char p[1024], t[2048];
+    snprintf(p, sizeof(p), "%s/consistencyCheck.dat", dir);
+    snprintf(t, sizeof(t),
+             "@ %d\tInterface size in X direction (nodes)\n@ %d\tInterface size in Y direction (nodes)\n"
+             "@ %d\tInterface size in Z direction (nodes)\n@ %.12e\tTime step\n"
+             "@ %.12e\tSpatial step along X\n@ %.12e\tSpatial step along Y\n"
+             "@ %.12e\tSpatial step along Z\n",
+             reg[1] - reg[0], reg[3] - reg[2], reg[5] - reg[4], d->tau, d->h[0], d->h[1], d->h[2]);
+    if (write_text(p, t)) return -1;
+    /* TFSF_defragInit: appended, E faces then H faces */
+    snprintf(p, sizeof(p), "%s/TFSF_defrag.map", dir);
+    FILE *fp = fopen(p, "at");
+    if (!fp) return fail("cannot open '%s'", p);
+    for (int q = 0; q < 24; ++q) {
+        int lo[3], hi[3], ep, eq;
+        tfsf_face_box(reg, q, lo, hi, &ep, &eq);
+        fprintf(fp, "[%d %d] [%d %d] [%d %d]\nep = %d, eq = %d\n", lo[0], hi[0], lo[1], hi[1], lo[2], hi[2],
+                ep, eq);
+        fprintf(fp, "%s/%s_99999.bin\n%s/%s_000.bin\n---\n", dir, tfsf_face_name(q), dir, tfsf_face_name(q));
+    }
+    return fclose(fp) == 0 ? 0 : fail("write error on '%s'", p);
+}
+
+int mh_tfsf_record_save(const char *dir, const int reg[6], int nframes, const double *data)
+{
+    char p[1024];
+    long at = 0;
+    for (int q = 0; q < 24; ++q) {
+        int lo[3], hi[3], ep, eq;
+        tfsf_face_box(reg, q, lo, hi, &ep, &eq);
+        const long vol = (long)(hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1) * (hi[2] - lo[2] + 1);
+        /* reg_t of the fragment: box, cpu 0, mode 1 (record), no wrap */
+        const int32_t h[11] = {lo[0], lo[1], lo[2], hi[0], hi[1], hi[2], 0, 1, 0, 0, 0};
+        snprintf(p, sizeof(p), "%s/%s_000.bin", dir, tfsf_face_name(q));
+        FILE *fp = fopen(p, "wb");
+        if (!fp) return fail("cannot create '%s'", p);
+        int ok = fwrite(h, sizeof(int32_t), 11, fp) == 11 &&
+                 fwrite(data + at, sizeof(double), (size_t)(vol * nframes), fp) == (size_t)(vol * nframes);
+        if (fclose(fp) != 0 || !ok) return fail("write error on '%s'", p);
+        at += vol * nframes;
+    }
+    snprintf(p, sizeof(p), "%s/lastFrameTime.bin", dir);
+    FILE *fp = fopen(p, "wb");
+    if (!fp) return fail("cannot create '%s'", p);
+    const int32_t nf = nframes;
+    const int ok = fwrite(&nf, sizeof(nf), 1, fp) == 1;
+    return (fclose(fp) == 0 && ok) ? 0 : fail("write error on '%s'", p);
+}
+
+/* One job of TFSF_defrag.map: the face box, its unrolling, the joined file
+ * and its fragments; each frame of the joined file is its old content (or
+ * zero) plus every fragment's nodes inside the filter (TFSFdefrag,
+ * source/TFSFdefrag/main.c:50-198). */
+typedef struct {
+    int    lo[3], hi[3];
+    int    ep, eq;
+    char   joined[1024];
+    int    nfrag;
+    char **frag;
+} defrag_job_t;
+
+static int defrag_job(const defrag_job_t *j, const int flo[3], const int fhi[3], int skip, int *frames_out,
+                      int *frames_before)
+{
+    const long nodes = (long)(j->hi[0] - j->lo[0] + 1) * (j->hi[1] - j->lo[1] + 1) * (j->hi[2] - j->lo[2] + 1);
+    const int32_t hdr[11] = {j->lo[0], j->lo[1], j->lo[2], j->hi[0], j->hi[1], j->hi[2], 0, 0, 0, 0, 0};
+    FILE *out = fopen(j->joined, "r+b");
+    if (!out) out = fopen(j->joined, "w+b");
+    if (!out) return fail("cannot open '%s'", j->joined);
+    double *full = (double *)calloc((size_t)nodes, sizeof(double));
+    FILE **in = (FILE **)calloc((size_t)j->nfrag, sizeof(FILE *));
+    int32_t (*pr)[11] = (int32_t(*)[11])calloc((size_t)j->nfrag, sizeof(*pr));
+    int rc = (!full || !in || !pr) ? fail("out of memory") : 0;
+    long maxpiece = 1;
+    for (int f = 0; f < j->nfrag && !rc; ++f) {
+        in[f] = fopen(j->frag[f], "rb");
+        if (!in[f] || fread(pr[f], sizeof(int32_t), 11, in[f]) != 11) {
+            rc = fail("cannot read fragment '%s'", j->frag[f]);
+            break;
+        }
+        const long v = (long)(pr[f][3] - pr[f][0] + 1) * (pr[f][4] - pr[f][1] + 1) * (pr[f][5] - pr[f][2] + 1);
+        if (v > maxpiece) maxpiece = v;
+    }
+    double *piece = rc ? NULL : (double *)malloc(sizeof(double) * (size_t)maxpiece);
+    if (!rc && !piece) rc = fail("out of memory");
+    int frames = 0;
+    *frames_before = 0;
+    if (!rc) {
+        /* the header, then 'skip' + 1 zero frames when the file is shorter
+         * than that; the frames it held count as written */
+        fseek(out, 0L, SEEK_SET);
+        fwrite(hdr, sizeof(int32_t), 11, out);
+        fseek(out, 0L, SEEK_END);
+        const long tail = ftell(out);
+        if (tail < (long)sizeof(hdr) + (long)(skip * nodes * sizeof(double))) {
+            fseek(out, 0L, SEEK_SET);
+            fwrite(hdr, sizeof(int32_t), 11, out);
+            for (int k = 0; k <= skip; ++k) fwrite(full, sizeof(double), (size_t)nodes, out);
+        }
+        *frames_before = (int)((tail - (long)sizeof(hdr)) / (nodes * (long)sizeof(double)));
+        const int er = 3 - j->ep - j->eq;
+        for (;; ++frames) {
+            const long pos = (long)sizeof(hdr) + (long)(skip + frames) * nodes * (long)sizeof(double);
+            fseek(out, pos, SEEK_SET);
+            const size_t got = fread(full, sizeof(double), (size_t)nodes, out);
+            if (got == 0)
+                memset(full, 0, sizeof(double) * (size_t)nodes);
+            else if (got != (size_t)nodes) {
+                rc = fail("'%s': incomplete frame", j->joined);
+                break;
+            }
+            int any = 0;
+            for (int f = 0; f < j->nfrag; ++f) {
+                const int32_t *r = pr[f];
+                const long v = (long)(r[3] - r[0] + 1) * (r[4] - r[1] + 1) * (r[5] - r[2] + 1);
+                const size_t g = fread(piece, sizeof(double), (size_t)v, in[f]);
+                if (g == 0) continue;
+                if (g != (size_t)v) {
+                    rc = fail("'%s': incomplete fragment", j->frag[f]);
+                    break;
+                }
+                any = 1;
+                if (r[er] < flo[er] || r[er] > fhi[er]) continue;
+                const int ep = j->ep, eq = j->eq;
+                const int out_span = j->hi[eq] - j->lo[eq] + 1, in_span = r[3 + eq] - r[eq] + 1;
+                const int p0 = r[ep] < flo[ep] ? flo[ep] : r[ep], p1 = r[3 + ep] > fhi[ep] ? fhi[ep] : r[3 + ep];
+                const int q0 = r[eq] < flo[eq] ? flo[eq] : r[eq], q1 = r[3 + eq] > fhi[eq] ? fhi[eq] : r[3 + eq];
+                for (int pp = p0; pp <= p1; ++pp)
+                    for (int qq = q0; qq <= q1; ++qq)
+                        full[(qq - j->lo[eq]) + (long)out_span * (pp - j->lo[ep])] +=
+                            piece[(qq - r[eq]) + (long)in_span * (pp - r[ep])];
+            }
+            if (rc || !any) break;
+            fseek(out, pos, SEEK_SET);
+            if (fwrite(full, sizeof(double), (size_t)nodes, out) != (size_t)nodes) {
+                rc = fail("write error on '%s'", j->joined);
+                break;
+            }
+        }
+    }
+    for (int f = 0; f < j->nfrag; ++f)
+        if (in && in[f]) fclose(in[f]);
+    if (fclose(out) != 0 && !rc) rc = fail("write error on '%s'", j->joined);
+    if (!rc)
+        for (int f = 0; f < j->nfrag; ++f) remove(j->frag[f]);
+    free(full);
+    free(in);
+    free(pr);
+    free(piece);
+    *frames_out = frames;
+    return rc;
+}
+
+int mh_tfsf_defrag(const char *dir)
+{
+    char p[1024], line[2048];
+    /* filter.cfg: I0, I1 (along x) and the frame shift (tag_TFSF.c:170-176) */
+    int flo[3] = {INT_MIN, INT_MIN, INT_MIN}, fhi[3] = {INT_MAX, INT_MAX, INT_MAX}, skip = 0;
+    snprintf(p, sizeof(p), "%s/filter.cfg", dir);
+    int fv[3];
+    if (read_first_ints(p, fv, 3)) return fail("'%s': 3 integers expected", p);
+    flo[0] = fv[0];
+    fhi[0] = fv[1];
+    skip = fv[2];
+    if (skip < 0) return fail("unsupported frame shift (%d)", skip);
+    snprintf(p, sizeof(p), "%s/lastFrameTime.bin", dir);
+    int32_t written = 0;
+    FILE *fp = fopen(p, "rb");
+    if (!fp || fread(&written, sizeof(written), 1, fp) != 1) {
+        if (fp) fclose(fp);
+        return fail("'%s': no frame count", p);
+    }
+    fclose(fp);
+    snprintf(p, sizeof(p), "%s/TFSF_defrag.map", dir);
+    FILE *map = fopen(p, "rt");
+    if (!map) return 0;                                      /* nothing to join */
+    int rc = 0;
+    for (;;) {
+        defrag_job_t j;
+        memset(&j, 0, sizeof(j));
+        if (!fgets(line, sizeof(line), map)) break;
+        if (sscanf(line, "[%d %d] [%d %d] [%d %d]", &j.lo[0], &j.hi[0], &j.lo[1], &j.hi[1], &j.lo[2],
+                   &j.hi[2]) != 6)
+            break;
+        if (!fgets(line, sizeof(line), map) || sscanf(line, "ep = %d, eq = %d", &j.ep, &j.eq) != 2 ||
+            !fgets(line, sizeof(line), map) || sscanf(line, "%1023s", j.joined) != 1) {
+            rc = fail("'%s': malformed job", p);
+            break;
+        }
+        while (fgets(line, sizeof(line), map) && line[0] != '-') {
+            char name[1024];
+            if (sscanf(line, "%1023s", name) != 1) continue;
+            char **g = (char **)realloc(j.frag, sizeof(char *) * (size_t)(j.nfrag + 1));
+            if (!g) {
+                rc = fail("out of memory");
+                break;
+            }
+            j.frag = g;
+            j.frag[j.nfrag++] = strdup(name);
+        }
+        int frames = 0, before = 0;
+        if (!rc && j.nfrag) {
+            rc = defrag_job(&j, flo, fhi, skip, &frames, &before);
+            if (!rc && before > written) written = before;
+            if (!rc && frames + skip > written) written = frames + skip;
+        }
+        for (int f = 0; f < j.nfrag; ++f) free(j.frag[f]);
+        free(j.frag);
+        if (rc) break;
+    }
+    fclose(map);
+    if (rc) return rc;
+    remove(p);
+    snprintf(p, sizeof(p), "%s/lastFrameTime.bin", dir);
+    fp = fopen(p, "wb");
+    if (!fp) return fail("cannot create '%s'", p);
+    const int ok = fwrite(&written, sizeof(written), 1, fp) == 1;
+    return (fclose(fp) == 0 && ok) ? 0 : fail("write error on '%s'", p);
 }
 
 void mh_tfsf_free(mh_tfsf_t *t)

@@ -156,6 +156,19 @@ typedef struct {
     double *dataE[12], *dataH[12];
 } mh_tfsf_t;
 int  mh_tfsf_load(const char *dir, const mh_domain_t *d, mh_tfsf_t *t);
+/* Record mode (TFSF_turnRec / TFSF_shutDown, source/core/em_TFSF.c:273-330,
+ * 866-890): mh_tfsf_record_begin writes consistencyCheck.dat and appends
+ * the 24 jobs of TFSF_defrag.map (E faces, then H faces); mh_tfsf_record_save
+ * writes the one-rank fragments <face>_000.bin (reg_t header, then the
+ * frames: E faces 0..11 then H faces 0..11 in 'data', each frames x face
+ * nodes) and lastFrameTime.bin.  mh_tfsf_defrag does what TFSFdefrag
+ * (source/TFSFdefrag/main.c) does: joins the fragments of every map job into
+ * <face>_99999.bin (adding to what the file holds, with the I0/I1 filter and
+ * frame shift of filter.cfg), removes fragments and map, and updates
+ * lastFrameTime.bin. */
+int  mh_tfsf_record_begin(const char *dir, const int reg[6], const mh_domain_t *d);
+int  mh_tfsf_record_save(const char *dir, const int reg[6], int nframes, const double *data);
+int  mh_tfsf_defrag(const char *dir);
 void mh_tfsf_free(mh_tfsf_t *t);
 
 /* ---- RANMAR (source/misc_ranmar.c:78-146) ------------------------------ */

@@ -1,4 +1,4 @@
-# Builds the reference Mandor core.out / setup.out from the C sources where
+# Builds the reference Mandor core.out / setup.out / TFSFdefrag from the C sources where
 # they lie under /root/reference (read-only), into oracle/_ref/ only.
 # The reference needs MPI: it links against the MPICH that ships in this
 # image under /opt/conda (no stand-in headers or libraries are written).
@@ -40,7 +40,9 @@ SETUP_SRC := setup/main misc_units misc_cfgReader type_reg misc_parameters \
   profiler parr_meshes parr_regLists misc_socket core/plasma_VSP \
   setup/tag_gradient
 
-all: $(OUT)/core.out $(OUT)/setup.out $(OUT)/core_gauss.out
+DEFRAG_SRC := TFSFdefrag/main log misc_cfgReader type_reg
+
+all: $(OUT)/core.out $(OUT)/setup.out $(OUT)/core_gauss.out $(OUT)/TFSFdefrag.out
 
 $(OUT)/obj/%.o: $(SRC)/%.c
 	@mkdir -p $(dir $@)
@@ -57,6 +59,9 @@ $(OUT)/core_gauss.out: $(addprefix $(OUT)/objg/,$(addsuffix .o,$(CORE_SRC)))
 	$(MPICC) $^ -o $@ -lm
 
 $(OUT)/setup.out: $(addprefix $(OUT)/obj/,$(addsuffix .o,$(SETUP_SRC)))
+	$(MPICC) $^ -o $@ -lm
+
+$(OUT)/TFSFdefrag.out: $(addprefix $(OUT)/obj/,$(addsuffix .o,$(DEFRAG_SRC)))
 	$(MPICC) $^ -o $@ -lm
 
 .PHONY: all
