@@ -107,9 +107,13 @@ __global__ void k_cap_H(FieldsT<T> f, Grid g, BoxSet bs, T c1, T c2, T c3)
     long o = gidx(g, i, j, k);
     long oi = gidx(g, i - 1, j, k), oj = gidx(g, i, j - 1, k), ok = gidx(g, i, j, k - 1);
     const T *Ex = f.E[0], *Ey = f.E[1], *Ez = f.E[2];
-    f.Hc[0][o] = f.H[0][o] + c3 * (Ey[o] - Ey[ok]) - c2 * (Ez[o] - Ez[oj]);
-    f.Hc[1][o] = f.H[1][o] + c1 * (Ez[o] - Ez[oi]) - c3 * (Ex[o] - Ex[ok]);
-    f.Hc[2][o] = f.H[2][o] + c2 * (Ex[o] - Ex[oj]) - c1 * (Ey[o] - Ey[oi]);
+    /* all loads first (the stores go to Hc, which the loads never read) */
+    const T h0 = f.H[0][o], h1 = f.H[1][o], h2 = f.H[2][o];
+    const T ex = Ex[o], exj = Ex[oj], exk = Ex[ok], ey = Ey[o], eyi = Ey[oi], eyk = Ey[ok];
+    const T ez = Ez[o], ezi = Ez[oi], ezj = Ez[oj];
+    f.Hc[0][o] = h0 + c3 * (ey - eyk) - c2 * (ez - ezj);
+    f.Hc[1][o] = h1 + c1 * (ez - ezi) - c3 * (ex - exk);
+    f.Hc[2][o] = h2 + c2 * (ex - exj) - c1 * (ey - eyi);
 }
 
 /* Cap E^{n+1} from the cap H, source/core/em_caps.c:580-599. */
@@ -122,9 +126,13 @@ __global__ void k_cap_E(FieldsT<T> f, Grid g, BoxSet bs, T c1, T c2, T c3)
     long o = gidx(g, i, j, k);
     long oi = gidx(g, i + 1, j, k), oj = gidx(g, i, j + 1, k), ok = gidx(g, i, j, k + 1);
     const T *Hx = f.Hc[0], *Hy = f.Hc[1], *Hz = f.Hc[2];
-    f.Ec[0][o] = f.E[0][o] - c3 * (Hy[ok] - Hy[o]) + c2 * (Hz[oj] - Hz[o]);
-    f.Ec[1][o] = f.E[1][o] - c1 * (Hz[oi] - Hz[o]) + c3 * (Hx[ok] - Hx[o]);
-    f.Ec[2][o] = f.E[2][o] - c2 * (Hx[oj] - Hx[o]) + c1 * (Hy[oi] - Hy[o]);
+    /* all loads first (the stores go to Ec, which the loads never read) */
+    const T e0 = f.E[0][o], e1 = f.E[1][o], e2 = f.E[2][o];
+    const T hx = Hx[o], hxj = Hx[oj], hxk = Hx[ok], hy = Hy[o], hyi = Hy[oi], hyk = Hy[ok];
+    const T hz = Hz[o], hzi = Hz[oi], hzj = Hz[oj];
+    f.Ec[0][o] = e0 - c3 * (hyk - hy) + c2 * (hzj - hz);
+    f.Ec[1][o] = e1 - c1 * (hzi - hz) + c3 * (hxk - hx);
+    f.Ec[2][o] = e2 - c2 * (hxj - hx) + c1 * (hyi - hy);
 }
 
 template <class T>
@@ -169,59 +177,77 @@ __device__ static inline void face_apply(const Grid &g, const FaceOp &op, long t
 #define AT(a) (r[ea] = (a), gidx(g, r[0], r[1], r[2]))
     const int N = op.N, top = op.top;
     const int pl = top ? N : 0;
+    /* in every case all values are loaded before any is stored: the nodes a
+     * line reads and writes are distinct (per component), so this is the
+     * reference's sequence with one memory latency */
     switch (op.kind) {
     case FACE_PERIODIC_H: {
         long o0 = AT(0), oN = AT(N), o1 = AT(1), oN1 = AT(N + 1);
+        T vN[3], v1[3];
         for (int c = 0; c < 3; ++c) {
-            X[c][o0] = X[c][oN];
-            X[c][oN1] = X[c][o1];
+            vN[c] = X[c][oN];
+            v1[c] = X[c][o1];
+        }
+        for (int c = 0; c < 3; ++c) {
+            X[c][o0] = vN[c];
+            X[c][oN1] = v1[c];
         }
         break;
     }
     case FACE_PERIODIC_E: {
         long o0 = AT(0), oN = AT(N), o1 = AT(1), oN1 = AT(N + 1);
+        T v0[3], v1[3];
         for (int c = 0; c < 3; ++c) {
-            X[c][oN] = X[c][o0];
-            X[c][oN1] = X[c][o1];
+            v0[c] = X[c][o0];
+            v1[c] = X[c][o1];
+        }
+        for (int c = 0; c < 3; ++c) {
+            X[c][oN] = v0[c];
+            X[c][oN1] = v1[c];
         }
         break;
     }
     case FACE_WALL_H: {
         long opl = AT(pl), oin = AT(pl + 1 - top), oex = AT(pl + top);
+        const T vp = X[ep][oin], vq = X[eq][oin];
         if (op.zeroN) X[ea][opl] = 0;
-        X[ep][oex] = X[ep][oin];
-        X[eq][oex] = X[eq][oin];
+        X[ep][oex] = vp;
+        X[eq][oex] = vq;
         break;
     }
     case FACE_MIRROR_E: {
         long opl = AT(pl), oin = AT(pl + 1 - top), oex = AT(pl + top);
+        const T va = X[ea][oin];
         X[ep][opl] = 0;
         X[eq][opl] = 0;
-        X[ea][oex] = X[ea][oin];
+        X[ea][oex] = va;
         break;
     }
     case FACE_MUR_E: {
         long opl = AT(pl), os = AT(pl + 1 - 2 * top), oin = AT(pl + 1 - top), oex = AT(pl + top);
         T *b = (T *)op.buf + 4 * t;
-        X[ea][oex] = X[ea][oin];
-        T vp = b[2] + alpha * (X[ep][os] - b[0]);
-        T vq = b[3] + alpha * (X[eq][os] - b[1]);
+        const T va = X[ea][oin], sp = X[ep][os], sq = X[eq][os];
+        const T b0 = b[0], b1 = b[1], b2 = b[2], b3 = b[3];
+        const T vp = b2 + alpha * (sp - b0);
+        const T vq = b3 + alpha * (sq - b1);
+        X[ea][oex] = va;
         X[ep][opl] = vp;
         X[eq][opl] = vq;
         b[0] = vp;
         b[1] = vq;
-        b[2] = X[ep][os];
-        b[3] = X[eq][os];
+        b[2] = sp;
+        b[3] = sq;
         break;
     }
     case FACE_MUR_INIT: {
         long opl = AT(pl), os = AT(pl + 1 - 2 * top), oin = AT(pl + 1 - top), oex = AT(pl + top);
         T *b = (T *)op.buf + 4 * t;
-        b[0] = X[ep][opl];
-        b[1] = X[eq][opl];
-        b[2] = X[ep][os];
-        b[3] = X[eq][os];
-        X[ea][oex] = X[ea][oin];
+        const T pp = X[ep][opl], pq = X[eq][opl], sp = X[ep][os], sq = X[eq][os], va = X[ea][oin];
+        b[0] = pp;
+        b[1] = pq;
+        b[2] = sp;
+        b[3] = sq;
+        X[ea][oex] = va;
         break;
     }
     }
@@ -396,11 +422,13 @@ int launch_e_update(hipStream_t s, const FieldsT<T> &f, const Grid &g, const Box
 /* E -= tau*J over nodes [-1, n+1]^3, source/core/em.c:166-184.  FP32
  * fields: the FP64 J sum is applied in FP64 and rounded once. */
 template <class T>
-__global__ void __launch_bounds__(TPB) k_e_tauJ(FieldsT<T> f, Grid g, Span sp)
+__global__ void __launch_bounds__(TPB) k_e_tauJ(FieldsT<T> f, Grid g, Span sp, int *tick)
 {
     typedef typename Vec<T>::v VT;
     typedef double VD __attribute__((ext_vector_type(Vec<T>::N)));
     constexpr int V = Vec<T>::N;
+    /* the run's step counter, when this is the step's last kernel */
+    if (tick && blockIdx.x == 0 && threadIdx.x == 0) *tick += 1;
     int i, j, k0;
     if (!span_group(sp, V, i, j, k0)) return;
     const long o = gidx(g, i, j, k0);
@@ -419,10 +447,10 @@ __global__ void __launch_bounds__(TPB) k_e_tauJ(FieldsT<T> f, Grid g, Span sp)
 }
 
 template <class T>
-int launch_e_tauJ(hipStream_t s, const FieldsT<T> &f, const Grid &g)
+int launch_e_tauJ(hipStream_t s, const FieldsT<T> &f, const Grid &g, int *tick)
 {
     const Span sp = span_of<Vec<T>::N>(-1, -1, g.n[1] + 1, -1, g.n[2] + 1);
-    k_e_tauJ<T><<<grid_span(sp, g.n[0] + 3), TPB, 0, s>>>(f, g, sp);
+    k_e_tauJ<T><<<grid_span(sp, g.n[0] + 3), TPB, 0, s>>>(f, g, sp, tick);
     MPIC_CHECK(hipGetLastError());
     return MPIC_OK;
 }
@@ -625,34 +653,64 @@ __global__ void k_jfold(JPtrs f, Grid g, int ea, int bc0, int bc1, int top0)
     r[eq] = -2 + (int)((unsigned)t - pq * (unsigned)nq);
     const int N = g.n[ea];
 #define AT(a) (r[ea] = (a), gidx(g, r[0], r[1], r[2]))
-    if (bc == MPIC_BC_PERIODIC) {
+    /* every value is loaded before any is stored: the nodes of one line are
+     * distinct, so this is the reference's sequence with one memory latency */
+    if (bc == MPIC_BC_PERIODIC && N < 5) {          /* the layers overlap: in order */
         for (int a = -2; a <= 2; ++a) {
-            long lo = AT(a), hi = AT(a + N);
+            const long lo = AT(a), hi = AT(a + N);
             for (int c = 0; c < 3; ++c) {
                 f.J[c][hi] += f.J[c][lo];
                 f.J[c][lo] = f.J[c][hi];
             }
         }
+    } else if (bc == MPIC_BC_PERIODIC) {
+        long lo[5], hi[5];
+        double vl[3][5], vh[3][5];
+#pragma unroll
+        for (int a = 0; a < 5; ++a) {
+            lo[a] = AT(a - 2);
+            hi[a] = AT(a - 2 + N);
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int a = 0; a < 5; ++a) {
+                vl[c][a] = f.J[c][lo[a]];
+                vh[c][a] = f.J[c][hi[a]];
+            }
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int a = 0; a < 5; ++a) {
+                const double v = vh[c][a] + vl[c][a];
+                f.J[c][hi[a]] = v;
+                f.J[c][lo[a]] = v;
+            }
     } else {
         const int pl = top ? N : 0, so = top ? 1 : -1, in_ = 1 - top, ex_ = top;
         double *Ja = f.J[ea], *Jp = f.J[ep], *Jq = f.J[eq];
         long oin = AT(pl + in_), oex = AT(pl + ex_), om1 = AT(pl - so), op1 = AT(pl + so);
         long om2 = AT(pl - 2 * so), op2 = AT(pl + 2 * so), oin2 = AT(pl + in_ - so),
              oex2 = AT(pl + ex_ + so), o0 = AT(pl);
-        Ja[oin] -= Ja[oex];
-        Jp[om1] += Jp[op1];
-        Jq[om1] += Jq[op1];
-        Ja[oin2] -= Ja[oex2];
-        Jp[om2] += Jp[op2];
-        Jq[om2] += Jq[op2];
+        const double a_in = Ja[oin], a_ex = Ja[oex], a_in2 = Ja[oin2], a_ex2 = Ja[oex2];
+        const double p_m1 = Jp[om1], p_p1 = Jp[op1], p_m2 = Jp[om2], p_p2 = Jp[op2];
+        const double q_m1 = Jq[om1], q_p1 = Jq[op1], q_m2 = Jq[om2], q_p2 = Jq[op2];
+        const double A1 = a_in - a_ex, A2 = a_in2 - a_ex2;
+        const double P1 = p_m1 + p_p1, P2 = p_m2 + p_p2, Q1 = q_m1 + q_p1, Q2 = q_m2 + q_p2;
+        Ja[oin] = A1;
+        Ja[oex] = A1;
+        Ja[oin2] = A2;
+        Ja[oex2] = A2;
+        Jp[om1] = P1;
+        Jp[op1] = -P1;
+        Jq[om1] = Q1;
+        Jq[op1] = -Q1;
+        Jp[om2] = P2;
+        Jp[op2] = -P2;
+        Jq[om2] = Q2;
+        Jq[op2] = -Q2;
         Jp[o0] = 0;
         Jq[o0] = 0;
-        Ja[oex] = Ja[oin];
-        Jp[op1] = -Jp[om1];
-        Jq[op1] = -Jq[om1];
-        Ja[oex2] = Ja[oin2];
-        Jp[op2] = -Jp[om2];
-        Jq[op2] = -Jq[om2];
     }
 #undef AT
 }
@@ -735,6 +793,24 @@ int launch_zero(hipStream_t s, double *p, long n)
     long nb = (n + TPB - 1) / TPB;
     if (nb > 65536) nb = 65536;
     k_zero<<<(unsigned)nb, TPB, 0, s>>>(p, n);
+    MPIC_CHECK(hipGetLastError());
+    return MPIC_OK;
+}
+
+/* Two ranges cleared by one launch. */
+__global__ void k_zero2(double *p, long n, double *q, long m)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < m) q[t] = 0.0;
+    for (; t < n; t += (long)gridDim.x * blockDim.x) p[t] = 0.0;
+}
+
+int launch_zero2(hipStream_t s, double *p, long n, double *q, long m)
+{
+    long nb = ((n > m ? n : m) + TPB - 1) / TPB;
+    if (nb > 65536) nb = 65536;
+    if (nb < 1) nb = 1;
+    k_zero2<<<(unsigned)nb, TPB, 0, s>>>(p, n, q, m);
     MPIC_CHECK(hipGetLastError());
     return MPIC_OK;
 }
@@ -906,7 +982,7 @@ int launch_probes(hipStream_t s, T *const E[3], T *const H[3], const Grid &g, co
                                     const BoxSet &, int);                                       \
     template int launch_e_update<T>(hipStream_t, const FieldsT<T> &, const Grid &,                \
                                     const BoxSet &, int, int);                                  \
-    template int launch_e_tauJ<T>(hipStream_t, const FieldsT<T> &, const Grid &);                \
+    template int launch_e_tauJ<T>(hipStream_t, const FieldsT<T> &, const Grid &, int *);         \
     template int launch_laser<T>(hipStream_t, T *, T *, const Grid &, const double *, int, int,  \
                                  int, int, int, int, int, int, const double *, const int *,     \
                                  int);                                                          \

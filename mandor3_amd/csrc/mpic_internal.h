@@ -152,7 +152,7 @@ int launch_h_update(hipStream_t s, const FieldsT<T> &f, const Grid &g, const Box
 template <class T>
 int launch_e_update(hipStream_t s, const FieldsT<T> &f, const Grid &g, const BoxSet &flush,
                     int interior, int do_flush);
-template <class T> int launch_e_tauJ(hipStream_t s, const FieldsT<T> &f, const Grid &g);
+template <class T> int launch_e_tauJ(hipStream_t s, const FieldsT<T> &f, const Grid &g, int *tick);
 template <class T>
 int launch_laser(hipStream_t s, T *Fy, T *Fz, const Grid &g, const double *map, int pos,
                  int JMIN, int KMIN, int KMAX, int jmin, int jmax, int kmin, int kmax,
@@ -177,6 +177,7 @@ int launch_probes(hipStream_t s, T *const E[3], T *const H[3], const Grid &g, co
                   int np, double *out);
 template <class T> int launch_energy(hipStream_t s, const FieldsT<T> &f, const Grid &g, double *acc);
 int launch_zero(hipStream_t s, double *p, long n);
+int launch_zero2(hipStream_t s, double *p, long n, double *q, long m);
 
 int launch_push64(hipStream_t s, const Markers64 &m, long n, const Fields64 &f,
                   const Grid &g, const PushConsts &c, double *Wacc, int mode, unsigned *err);

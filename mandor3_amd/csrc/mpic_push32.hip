@@ -729,12 +729,18 @@ int launch_keys32(hipStream_t s, const Markers32 &m, long n, const TileView &tv)
     return MPIC_OK;
 }
 
+/* The scatter streams every marker once: nontemporal loads and stores keep
+ * it from displacing the field arrays in the caches. */
+typedef float v4f_nt __attribute__((ext_vector_type(4)));
 __global__ void k_scatter32(Markers32 src, Markers32 dst, long nbound, TileView tv, int parity)
 {
     const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long n = tv.nmark[parity];
     float4 a = make_float4(__int_as_float(-1), 0.0f, 0.0f, 0.0f);
-    if (p < n && p < nbound) a = src.pos[p];
+    if (p < n && p < nbound) {
+        const v4f_nt t = __builtin_nontemporal_load((const v4f_nt *)(src.pos + p));
+        a = make_float4(t.x, t.y, t.z, t.w);
+    }
     const int word = __float_as_int(a.x);
     const int key = word < 0 ? -1 : (word & KEY_MASK);
     int start, len;
@@ -745,9 +751,12 @@ __global__ void k_scatter32(Markers32 src, Markers32 dst, long nbound, TileView 
     base = __shfl(base, start, 64);
     if (key < 0) return;
     const long d = base + ((int)(threadIdx.x & 63) - start);
-    dst.pos[d] = a;
-    dst.mom[d] = src.mom[p];
-    dst.id[d] = src.id[p];
+    const v4f_nt mo = __builtin_nontemporal_load((const v4f_nt *)(src.mom + p));
+    const int idv = __builtin_nontemporal_load(src.id + p);
+    const v4f_nt ao = {a.x, a.y, a.z, a.w};
+    __builtin_nontemporal_store(ao, (v4f_nt *)(dst.pos + d));
+    __builtin_nontemporal_store(mo, (v4f_nt *)(dst.mom + d));
+    __builtin_nontemporal_store(idv, dst.id + d);
 }
 
 int launch_scatter32(hipStream_t s, const Markers32 &src, const Markers32 &dst, long nbound,

@@ -467,7 +467,8 @@ extern "C" int mpic_create(const mpic_config_t *cfg, mpic_run_t **out)
     MPIC_CHECK(hipMemsetAsync(tv.nactive, 0, sizeof(int) * (4 + MPIC_XCDS), r->stream));
     /* scan scratch: block sums (3 per block + totals) and 3 ints per tile */
     MPIC_CHECK(hipMalloc(&r->blocksum,
-                         sizeof(int) * (3 * ((tv.ntiles + 1023) / 1024 + 1) + 3 * (long)tv.ntiles)));
+                         sizeof(int) * (3 * ((tv.ntiles + MPIC_SCAN_BLOCK - 1) / MPIC_SCAN_BLOCK + 1) +
+                                       3 * (long)tv.ntiles)));
     r->parity = 0;
     {
         int ncu = 256;
@@ -1514,8 +1515,8 @@ extern "C" int mpic_push_deposit(mpic_run_t *r)
     if (!r) return MPIC_ERR_ARG;
     TRY(check_stage(r, 1));
     MPIC_CHECK(hipSetDevice(r->dev));
-    TRY(launch_zero(r->stream, r->jp.J[0], 3 * r->g.ncell));
-    TRY(launch_zero(r->stream, r->dacc + 2, 3));   /* a kernel: the step may be captured */
+    /* J and the temperature sums: a kernel, not a memset (the step may be captured) */
+    TRY(launch_zero2(r->stream, r->jp.J[0], 3 * r->g.ncell, r->dacc + 2, 3));
     if (r->timed >= 0) TRY(record(r, 1));
     if (r->fp32) {
         TRY(launch_push32(r->stream, r->m32, r->f32, r->g, r->pc, r->qm, r->tv, r->tj, r->dacc + 2,
@@ -1615,7 +1616,9 @@ extern "C" int mpic_em_e_finish(mpic_run_t *r)
         TRY(laser_E(r, r->f32.E[1], r->f32.E[2], dom));
     else
         TRY(laser_E(r, r->f.E[1], r->f.E[2], dom));
-    FCALL(launch_e_tauJ, r->g);
+    /* E -= tau J; the step counter advances in the same launch unless the
+     * spot source still has to read it */
+    FCALL(launch_e_tauJ, r->g, r->spot.on ? nullptr : r->dstep);
     if (r->spot.on) {
         const Spot &sp = r->spot;
         const long np = (long)(r->g.n[1] + 3) * (r->g.n[2] + 3);
@@ -1626,7 +1629,7 @@ extern "C" int mpic_em_e_finish(mpic_run_t *r)
             TRY(launch_spot(r->stream, r->f.E[1], r->f.E[2], r->g, sp.i, sp.dprof, sp.dprof + np,
                             sp.dtab, r->dstep, LASER_CAP));
     }
-    TRY(launch_tick(r->stream, r->dstep));
+    if (r->spot.on) TRY(launch_tick(r->stream, r->dstep));
     r->time = r->time + r->g.tau;          /* parameter_setTime (Time + tau) */
     ++r->step;
     r->stage = 0;

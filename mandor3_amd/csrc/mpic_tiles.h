@@ -13,6 +13,7 @@
 #include "mpic_internal.h"
 
 #define TILE_CELLS 64
+#define MPIC_SCAN_BLOCK 256     /* tiles per workgroup of the tile scan */
 
 struct TileView {
     int  nt[3];              /* tiles per axis */

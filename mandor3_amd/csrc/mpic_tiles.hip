@@ -14,7 +14,7 @@
  */
 #include "mpic_tiles.h"
 
-#define SB 1024                 /* tiles per scan block */
+#define SB MPIC_SCAN_BLOCK       /* tiles per scan block */
 
 /* Keys of an unsorted array (after loading), with per-run counting. */
 __global__ void k_keys64(Markers64 m, long n, Grid g, TileView tv)
@@ -40,10 +40,10 @@ int launch_keys64(hipStream_t s, const Markers64 &m, long n, const Grid &g, cons
 /* Block-local exclusive scans of (markers per tile, tile non-empty, work
  * items of the tile), three ints per tile and per block. */
 #define NSCAN 3
-__global__ void k_scan_a(TileView tv, int *blocksum, int *tileloc)
+__global__ void __launch_bounds__(SB) k_scan_a(TileView tv, int *blocksum, int *tileloc)
 {
-    __shared__ int sv[NSCAN][SB];
-    const int t = blockIdx.x * SB + threadIdx.x;
+    __shared__ int wsum[NSCAN][SB / 64];
+    const int t = blockIdx.x * SB + threadIdx.x, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     int c = 0;
     if (t < tv.ntiles) {
         const int4 *q = (const int4 *)(tv.count + (long)t * TILE_CELLS);
@@ -53,20 +53,23 @@ __global__ void k_scan_a(TileView tv, int *blocksum, int *tileloc)
         }
     }
     const int own[NSCAN] = {c, c > 0, tile_items(c, tv.chm)};
-    for (int a = 0; a < NSCAN; ++a) sv[a][threadIdx.x] = own[a];
-    __syncthreads();
-    for (int d = 1; d < SB; d <<= 1) {          /* Hillis-Steele inclusive */
-        int x[NSCAN] = {0, 0, 0};
-        if ((int)threadIdx.x >= d)
-            for (int a = 0; a < NSCAN; ++a) x[a] = sv[a][threadIdx.x - d];
-        __syncthreads();
-        for (int a = 0; a < NSCAN; ++a) sv[a][threadIdx.x] += x[a];
-        __syncthreads();
+    int inc[NSCAN];
+    for (int a = 0; a < NSCAN; ++a) {        /* wavefront inclusive scan */
+        int x = own[a];
+        for (int d = 1; d < 64; d <<= 1) {
+            const int y = __shfl_up(x, d, 64);
+            if (lane >= d) x += y;
+        }
+        inc[a] = x;
+        if (lane == 63) wsum[a][w] = x;
     }
+    __syncthreads();
+    for (int a = 0; a < NSCAN; ++a)
+        for (int q = 0; q < w; ++q) inc[a] += wsum[a][q];
     if (t < tv.ntiles)
-        for (int a = 0; a < NSCAN; ++a) tileloc[NSCAN * t + a] = sv[a][threadIdx.x] - own[a];
+        for (int a = 0; a < NSCAN; ++a) tileloc[NSCAN * t + a] = inc[a] - own[a];
     if (threadIdx.x == SB - 1)
-        for (int a = 0; a < NSCAN; ++a) blocksum[NSCAN * blockIdx.x + a] = sv[a][SB - 1];
+        for (int a = 0; a < NSCAN; ++a) blocksum[NSCAN * blockIdx.x + a] = inc[a];
 }
 
 /* Exclusive scan of the block sums (one block, loops for any count). */
@@ -172,15 +175,16 @@ __global__ void k_scatter64(Markers64 src, Markers64 dst, long nbound, TileView 
     base = __shfl(base, start, 64);
     if (key < 0) return;
     const long d = base + ((int)(threadIdx.x & 63) - start);
-    dst.x[d] = src.x[p];
-    dst.y[d] = src.y[p];
-    dst.z[d] = src.z[p];
-    dst.ux[d] = src.ux[p];
-    dst.uy[d] = src.uy[p];
-    dst.uz[d] = src.uz[p];
-    dst.w[d] = src.w[p];
-    dst.qm[d] = src.qm[p];
-    dst.id[d] = src.id[p];
+    /* every marker is streamed once: nontemporal, so the fields stay cached */
+    __builtin_nontemporal_store(__builtin_nontemporal_load(src.x + p), dst.x + d);
+    __builtin_nontemporal_store(__builtin_nontemporal_load(src.y + p), dst.y + d);
+    __builtin_nontemporal_store(__builtin_nontemporal_load(src.z + p), dst.z + d);
+    __builtin_nontemporal_store(__builtin_nontemporal_load(src.ux + p), dst.ux + d);
+    __builtin_nontemporal_store(__builtin_nontemporal_load(src.uy + p), dst.uy + d);
+    __builtin_nontemporal_store(__builtin_nontemporal_load(src.uz + p), dst.uz + d);
+    __builtin_nontemporal_store(__builtin_nontemporal_load(src.w + p), dst.w + d);
+    __builtin_nontemporal_store(__builtin_nontemporal_load(src.qm + p), dst.qm + d);
+    __builtin_nontemporal_store(__builtin_nontemporal_load(src.id + p), dst.id + d);
 }
 
 int launch_scatter64(hipStream_t s, const Markers64 &src, const Markers64 &dst, long nbound,
