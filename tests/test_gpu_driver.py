@@ -249,3 +249,85 @@ def test_core_tfsf_play_matches_oracle(tmp_path):
     rec = records.record(wd, 1, n)
     compare(rec, o, n, d["h"])
     assert np.abs(rec["H"]).max() > 1e-2          # the interface's fields are in the box
+
+
+TAG_DECK = MUR_DECK.split("[plasma]")[0] + """[DF:foil]
+@ 0.8    x0 [micron]
+@ 0.4    y0
+@ 0.4    z0
+@ 1      lx
+@ 0.3    ly
+@ 0      lz
+@ 0.15   width [micron]
+> 0.02   shift
+@ exponential  profile
+@ -2     n1 [n_c]
+@ -8     n2 [n_c]
+@ -1     q
+@ 1      M
+@ 2      nx
+@ 1      ny
+@ 2      nz
+[gradient]
+> 0.7 0 0   point 1
+> 0.9 0 0   point 2
+> 1.0 0.5 0 linear
+[plasma]
+@ 1      nx
+@ 1      ny
+@ 1      nz
+@ 4      layers
+@ boxes  a slab
+> 0.8 0.4 0.4   centre
+> 0.1 0 0       half-axis
+> 0 1 0         half-axis
+> 0 0 1         half-axis
+[DF:Maxwell]
+@ 0.005  omega_pe^2
+@ 5.4466e-4  q/M
+@ 0.001  ux
+@ 0.001  uy
+@ 0.001  uz
+@ 1      tune energy
+@ 1      uniform weight
+[gaussSpot]
+@ -1.0   a0
+@ 1.0    omega
+@ 0.6    X [micron]
+@ 0.4    Y
+@ 0.4    Z
+@ 0.2    width [micron]
+@ 3.0    duration [fs]
+@ 2.0    front offset [fs]
+@ 1      Ey
+@ 0      Ez
+"""
+
+
+def test_core_setup_tags_match_oracle(tmp_path, host):  # noqa: F811
+    """A deck of the reference's deterministic tags ([DF:foil] with an
+    exponential profile, [gradient], a [plasma] slab with a Maxwell quiet
+    start, a [gaussSpot] hard source) set up by mandor_setup and run by
+    mandor_core matches the oracle started from the same record 0."""
+    wd = str(tmp_path / "run")
+    setup_dir(wd, TAG_DECK)
+    open(os.path.join(wd, "run_mandor.cfg"), "w").write(cfg(20, 20))
+    n = (24, 16, 16)
+    r0 = records.record(wd, 0, n)
+    M = r0["M"]
+    assert (M["qDivM"] == np.float32(-1.0)).sum() > 0 and (M["qDivM"] == np.float32(5.4466e-4)).sum() > 0
+    spot = [float(ln.split()[1]) for ln in open(os.path.join(wd, ".gaussSpot.cfg")) if ln.startswith("@")]
+    assert len(spot) == 11
+    d = records.domain(os.path.join(wd, "binData", "domain.bin"))
+    bc = list(d["bc_min"]) + list(d["bc_max"])
+    o = Oracle(n, d["h"], d["L"], d["tau"], bc, r0["time"])
+    o.set_fields(r0["E"], r0["H"])
+    o.set_markers(M)
+    o.add_gauss_spot(spot)
+    o.init()
+    core(wd)
+    for _ in range(20):
+        o.step()
+    rec = records.record(wd, 1, n)
+    compare(rec, o, n, d["h"])
+    assert np.abs(rec["E"]).max() > 1e-2          # the spot's field is in the box
