@@ -165,11 +165,23 @@ __device__ static inline void face_apply(const Grid &g, const FaceOp &op, long t
 {
     T *X[3] = {(T *)op.X[0], (T *)op.X[1], (T *)op.X[2]};
     const T alpha = (T)op.alpha;
-    const unsigned nq = op.qhi - op.qlo + 1;
-    const long tot = (long)(op.phi - op.plo + 1) * nq;
+    const unsigned np = op.phi - op.plo + 1, nq = op.qhi - op.qlo + 1;
+    const long tot = (long)np * nq;
     if (t >= tot) return;
-    const unsigned pq = (unsigned)t / nq;
-    int p = op.plo + (int)pq, q = op.qlo + (int)((unsigned)t - pq * nq);
+    /* consecutive threads take consecutive nodes along the higher tangential
+     * axis (k for x and y walls, j for z walls), so that a wavefront's
+     * accesses are contiguous; the Mur buffer is indexed by t in the same
+     * order by every kind */
+    int p, q;
+    if (op.ea == 1) {
+        const unsigned qq = (unsigned)t / np;
+        p = op.plo + (int)((unsigned)t - qq * np);
+        q = op.qlo + (int)qq;
+    } else {
+        const unsigned pq = (unsigned)t / nq;
+        p = op.plo + (int)pq;
+        q = op.qlo + (int)((unsigned)t - pq * nq);
+    }
     int ea = op.ea, ep = (ea + 1) % 3, eq = (ea + 2) % 3;
     int r[3];
     r[ep] = p;

@@ -380,40 +380,45 @@ __device__ static inline int push32_marker(MarkerRegs &out, const MarkerRegs &r,
     return (sp << SPECIES_SHIFT) | key;
 }
 
-/* DPP move of a float across lanes: lanes whose source lies outside the
- * pattern (or rows outside row_mask) read 0. */
-template <int CTRL, int ROWS>
-__device__ static inline float dpp0(float x)
-{
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, ROWS, 0xF, false));
-}
-
-#define DPP_ROW_SHR(d) (0x110 + (d))
+/* One step of the row scan for all 12 values: v[q] += v[q] of lane - SH
+ * times the lane's 0/1 mask M (v_fmac_f32_dpp; a source outside the
+ * 16-lane row reads 0).  x + y*1 and x + y*0 round as x + y and x. */
+#define SCAN_STEP(SH, M)                                                                    \
+    "v_fmac_f32_dpp %0, %0, %" #M " row_shr:" #SH " row_mask:0xf bank_mask:0xf bound_ctrl:1\n" \
+    "v_fmac_f32_dpp %1, %1, %" #M " row_shr:" #SH " row_mask:0xf bank_mask:0xf bound_ctrl:1\n" \
+    "v_fmac_f32_dpp %2, %2, %" #M " row_shr:" #SH " row_mask:0xf bank_mask:0xf bound_ctrl:1\n" \
+    "v_fmac_f32_dpp %3, %3, %" #M " row_shr:" #SH " row_mask:0xf bank_mask:0xf bound_ctrl:1\n" \
+    "v_fmac_f32_dpp %4, %4, %" #M " row_shr:" #SH " row_mask:0xf bank_mask:0xf bound_ctrl:1\n" \
+    "v_fmac_f32_dpp %5, %5, %" #M " row_shr:" #SH " row_mask:0xf bank_mask:0xf bound_ctrl:1\n" \
+    "v_fmac_f32_dpp %6, %6, %" #M " row_shr:" #SH " row_mask:0xf bank_mask:0xf bound_ctrl:1\n" \
+    "v_fmac_f32_dpp %7, %7, %" #M " row_shr:" #SH " row_mask:0xf bank_mask:0xf bound_ctrl:1\n" \
+    "v_fmac_f32_dpp %8, %8, %" #M " row_shr:" #SH " row_mask:0xf bank_mask:0xf bound_ctrl:1\n" \
+    "v_fmac_f32_dpp %9, %9, %" #M " row_shr:" #SH " row_mask:0xf bank_mask:0xf bound_ctrl:1\n" \
+    "v_fmac_f32_dpp %10, %10, %" #M " row_shr:" #SH " row_mask:0xf bank_mask:0xf bound_ctrl:1\n" \
+    "v_fmac_f32_dpp %11, %11, %" #M " row_shr:" #SH " row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
 
 /* Segmented inclusive prefix sum of the 12 first-segment values over runs
  * of equal start cells inside each 16-lane row; 'start' is the first lane of
  * the caller's run.  Afterwards the last lane of each run in a row holds
  * that row's part of the run total.  A step adds only when its source lane
- * belongs to the caller's run (sources outside the row read 0). */
+ * belongs to the caller's run, by a 0/1 factor, so each step is one VALU
+ * instruction per value (a compiled conditional add is a DPP add plus a
+ * select).  All lanes of the wavefront execute it.  A DPP read of a VGPR
+ * needs 2 wait states after the VALU write of that VGPR: s_nop 1 covers the
+ * writes before the block, and inside it the 12 independent chains put 11
+ * instructions between a value's write and its next DPP read. */
 __device__ static inline void row_scan(Seg12 &f, int start)
 {
     const int lane = threadIdx.x & 63;
-    const bool c1 = lane - 1 >= start, c2 = lane - 2 >= start, c4 = lane - 4 >= start,
-               c8 = lane - 8 >= start;
-#pragma unroll
-    for (int q = 0; q < 12; ++q) {
-        float x = f.v[q], y;
-        y = dpp0<DPP_ROW_SHR(1), 0xF>(x);
-        if (c1) x += y;
-        y = dpp0<DPP_ROW_SHR(2), 0xF>(x);
-        if (c2) x += y;
-        y = dpp0<DPP_ROW_SHR(4), 0xF>(x);
-        if (c4) x += y;
-        y = dpp0<DPP_ROW_SHR(8), 0xF>(x);
-        if (c8) x += y;
-        f.v[q] = x;
-    }
+    const float m1 = lane - 1 >= start ? 1.0f : 0.0f, m2 = lane - 2 >= start ? 1.0f : 0.0f,
+                m4 = lane - 4 >= start ? 1.0f : 0.0f, m8 = lane - 8 >= start ? 1.0f : 0.0f;
+    asm volatile("s_nop 1\n" SCAN_STEP(1, 12) SCAN_STEP(2, 13) SCAN_STEP(4, 14) SCAN_STEP(8, 15)
+                 : "+v"(f.v[0]), "+v"(f.v[1]), "+v"(f.v[2]), "+v"(f.v[3]), "+v"(f.v[4]),
+                   "+v"(f.v[5]), "+v"(f.v[6]), "+v"(f.v[7]), "+v"(f.v[8]), "+v"(f.v[9]),
+                   "+v"(f.v[10]), "+v"(f.v[11])
+                 : "v"(m1), "v"(m2), "v"(m4), "v"(m8));
 }
+#undef SCAN_STEP
 
 __device__ static inline void key_to_ijk(const TileView &tv, int key, int &i, int &j, int &k)
 {
