@@ -516,7 +516,7 @@ k_push32_tiles(Markers32 m, F32View v, Push32Consts c, TileView tv, float *tj, d
                 }
         }
         /* item u of nu: an even share of the tile's markers */
-        const unsigned tb = tv.off[t * TILE_CELLS], cnt = tv.off[(t + 1) * TILE_CELLS] - tb;
+        const unsigned tb = tv.toff[t], cnt = tv.toff[t + 1] - tb;
         const unsigned beg = tb + (unsigned)((unsigned long long)u * cnt / nu),
                        end = tb + (unsigned)((unsigned long long)(u + 1) * cnt / nu);
         MarkerRegs cur;

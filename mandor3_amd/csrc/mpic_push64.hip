@@ -382,7 +382,7 @@ __global__ void __launch_bounds__(TPB) k_push64_tiled(Markers64 m, F64View v, Pu
         ts.o2 = TILE * t2 - 1;
         for (int q = threadIdx.x; q < TJN * TJN * TJN * 3; q += TPB) lj[q] = 0.0;
         __syncthreads();
-        const long beg = tv.off[(long)t * TILE_CELLS], end = tv.off[(long)(t + 1) * TILE_CELLS];
+        const long beg = tv.toff[t], end = tv.toff[t + 1];
         for (long base = beg; base < end; base += TPB) {
             const long p = base + threadIdx.x;
             int key = -1;

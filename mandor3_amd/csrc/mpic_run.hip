@@ -449,6 +449,8 @@ extern "C" int mpic_create(const mpic_config_t *cfg, mpic_run_t **out)
     tv.ncellp = (long)tv.ntiles * TILE_CELLS;
     MPIC_CHECK(hipMalloc(&tv.off, sizeof(int) * (tv.ncellp + 1)));
     MPIC_CHECK(hipMalloc(&tv.cursor, sizeof(int) * tv.ncellp));
+    MPIC_CHECK(hipMalloc(&tv.toff, sizeof(int) * ((size_t)tv.ntiles + 1)));
+    MPIC_CHECK(hipMemsetAsync(tv.toff, 0, sizeof(int) * ((size_t)tv.ntiles + 1), r->stream));
     MPIC_CHECK(hipMalloc(&tv.count, sizeof(int) * tv.ncellp));
     MPIC_CHECK(hipMemsetAsync(tv.count, 0, sizeof(int) * tv.ncellp, r->stream));
     MPIC_CHECK(hipMemsetAsync(tv.off, 0, sizeof(int) * (tv.ncellp + 1), r->stream));
@@ -535,7 +537,7 @@ extern "C" int mpic_destroy(mpic_run_t *r)
         Markers32 &m3 = r->m32buf[b];
         hipFree(m3.pos); hipFree(m3.mom); hipFree(m3.id);
     }
-    hipFree(r->tv.off); hipFree(r->tv.cursor); hipFree(r->tv.count); hipFree(r->tv.key);
+    hipFree(r->tv.off); hipFree(r->tv.toff); hipFree(r->tv.cursor); hipFree(r->tv.count); hipFree(r->tv.key);
     hipFree(r->tv.active); hipFree(r->tv.slot); hipFree(r->tv.nactive); hipFree(r->blocksum);
     hipFree(r->tv.item); hipFree(r->tv.istart);
     hipFree(r->tj);

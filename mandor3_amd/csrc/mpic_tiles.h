@@ -19,7 +19,9 @@ struct TileView {
     int  nt[3];              /* tiles per axis */
     int  ntiles;
     long ncellp;             /* ntiles * 64 */
-    int *off;                /* [ncellp + 1] first marker of every cell key */
+    int *off;                /* [ncellp + 1] first marker of every cell key of a non-empty
+                              * tile (entries of empty tiles are not maintained) */
+    int *toff;               /* [ntiles + 1] first marker of every tile */
     int *cursor;             /* [ncellp] scatter cursors */
     int *count;              /* [ncellp] markers per key after the push (zeroed by the scan) */
     int *key;                /* [capacity] new key of every marker, -1 = deleted */

@@ -106,27 +106,24 @@ __global__ void k_scan_b(int *blocksum, int nb)
     if (threadIdx.x < NSCAN) blocksum[NSCAN * nb + threadIdx.x] = carry[threadIdx.x];
 }
 
-/* Per-key offsets and cursors, the active-tile list, counters reset: one
- * wavefront per tile, one lane per cell key (coalesced). */
+/* Tile offsets, the active-tile list and, for every non-empty tile, the
+ * per-key offsets and cursors with its counters reset: one wavefront per
+ * tile, one lane per cell key (coalesced).  The tile's total comes from the
+ * tile scan, so the wavefront of an empty tile (its counters are all 0)
+ * writes two words and leaves: most tiles of a sparse plasma are empty. */
 __global__ void k_scan_c(TileView tv, const int *blocksum, const int *tileloc, int nb,
                          int parity_next)
 {
     const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const int t = (int)(q / TILE_CELLS), lane = threadIdx.x & 63;
     if (t >= tv.ntiles) return;                      /* whole waves exit together */
-    const int c = tv.count[q];
-    int incl = c;
-    for (int d = 1; d < 64; d <<= 1) {
-        const int v = __shfl_up(incl, d, 64);
-        if (lane >= d) incl += v;
-    }
-    const int tb = t / SB;
-    const int o = blocksum[NSCAN * tb] + tileloc[NSCAN * t] + incl - c;
-    tv.off[q] = o;
-    tv.cursor[q] = o;
-    tv.count[q] = 0;
-    const int tot = __shfl(incl, 63, 64);
+    const int tb = t / SB, tn = t + 1;
+    const int base = blocksum[NSCAN * tb] + tileloc[NSCAN * t];
+    const int next = tn < tv.ntiles ? blocksum[NSCAN * (tn / SB)] + tileloc[NSCAN * tn]
+                                    : blocksum[NSCAN * nb];
+    const int tot = next - base;
     if (lane == 0) {
+        tv.toff[t] = base;
         const int a = blocksum[NSCAN * tb + 1] + tileloc[NSCAN * t + 1];
         if (tot > 0) {
             const int is = blocksum[NSCAN * tb + 2] + tileloc[NSCAN * t + 2];
@@ -139,12 +136,24 @@ __global__ void k_scan_c(TileView tv, const int *blocksum, const int *tileloc, i
     }
     if (q == 0) {
         tv.off[tv.ncellp] = blocksum[NSCAN * nb];
+        tv.toff[tv.ntiles] = blocksum[NSCAN * nb];
         *tv.nactive = blocksum[NSCAN * nb + 1];
         tv.istart[blocksum[NSCAN * nb + 1]] = blocksum[NSCAN * nb + 2];
         *tv.nitems = blocksum[NSCAN * nb + 2];
         tv.nmark[parity_next] = blocksum[NSCAN * nb];
         for (int x = 0; x < MPIC_XCDS; ++x) tv.queue[x] = 0;
     }
+    if (tot == 0) return;                            /* wave-uniform */
+    const int c = tv.count[q];
+    int incl = c;
+    for (int d = 1; d < 64; d <<= 1) {
+        const int v = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += v;
+    }
+    const int o = base + incl - c;
+    tv.off[q] = o;
+    tv.cursor[q] = o;
+    tv.count[q] = 0;
 }
 
 int launch_tile_scan(hipStream_t s, const TileView &tv, int *blocksum, int parity_next)
@@ -205,8 +214,13 @@ __global__ void k_counts_natural(TileView tv, Grid g, int32_t *out)
     const int k = (int)(c % g.n[2]);
     const long r = c / g.n[2];
     const int j = (int)(r % g.n[1]), i = (int)(r / g.n[1]);
-    const int key = cell_key_ijk(tv, g, i, j, k);
-    out[c] = tv.off[key + 1] - tv.off[key];
+    const int key = cell_key_ijk(tv, g, i, j, k), t = key / TILE_CELLS;
+    if (tv.toff[t + 1] == tv.toff[t]) {
+        out[c] = 0;                       /* empty tile: its key offsets are not kept */
+        return;
+    }
+    const int end = (key % TILE_CELLS == TILE_CELLS - 1) ? tv.toff[t + 1] : tv.off[key + 1];
+    out[c] = end - tv.off[key];
 }
 
 int launch_counts_natural(hipStream_t s, const TileView &tv, const Grid &g, int32_t *out)
