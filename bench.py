@@ -1,7 +1,7 @@
 """Benchmark of the PIC time step on one MI355X.
 
     python bench.py [--steps K] [--warmup W] [--precision fp32|fp64|both]
-                    [--workload laser-foil|two-stream] [--no-cpu]
+                    [--workload laser-foil|two-stream|large|ensemble|serve] [--no-cpu]
 
 A step is one full main-loop iteration (H half step, push + charge-conserving
 deposit, per-step sort by cell, H step, E step, J fold, laser source, Mur /
@@ -35,8 +35,10 @@ def parse():
                    help="both: the FP32 split form gives 'value', the FP64 plain form "
                         "is reported beside it")
     p.add_argument("--workload", default="laser-foil",
-                   choices=["laser-foil", "two-stream", "ensemble", "serve"],
-                   help="ensemble: 64 laser-foil scan runs advanced together (graph replay); "
+                   choices=["laser-foil", "two-stream", "large", "ensemble", "serve"],
+                   help="large: the 512x256x256 full-box fill (1.05e9 markers, FP32 split "
+                        "form, streamed in by mpic_markers_append); "
+                        "ensemble: 64 laser-foil scan runs advanced together (graph replay); "
                         "serve: Poisson stream of scan requests through the serving loop")
     p.add_argument("--runs", type=int, default=64, help="ensemble size")
     p.add_argument("--requests", type=int, default=48, help="serving: number of requests")
@@ -242,12 +244,87 @@ def main_serve(a):
         "p50_s": m["p50_s"], "p99_s": m["p99_s"], "fp64_plain": out["fp64"]}))
 
 
+def main_large(a):
+    """The 512x256x256 single-GPU configuration at its full-box fill, FP32
+    split form only (the FP64 plain form would need 2 x 1.05e9 x 60 B of
+    double-buffered markers).  The records are built a few planes at a time
+    on a second thread while the previous piece is appended.  Besides the
+    rate it checks what holds at any size: the per-cell counts of the final
+    sort add up to the live markers, and markers are only ever removed (by
+    the open walls)."""
+    from concurrent.futures import ThreadPoolExecutor
+    import mandor3_amd as ma
+    from mandor3_amd import workloads
+    if ma.device_count() < 1:
+        raise SystemExit("no HIP device visible")
+    wl = workloads.large_foil()
+    r = ma.Run(wl["n"], wl["h"], wl["L"], wl["tau"], wl["bc"], 0.0, precision=ma.FP32_SPLIT,
+               max_markers=wl["n_markers"])
+    z = np.zeros(3 * int(np.prod([v + 1 for v in wl["n"]])))
+    r.set_fields(z, z)
+    del z
+    t0 = time.perf_counter()
+    loaded, pieces = 0, 0
+    gen = wl["chunks"]()
+    with ThreadPoolExecutor(1) as ex:
+        fut = ex.submit(next, gen, None)
+        while True:
+            M = fut.result()
+            if M is None:
+                break
+            fut = ex.submit(next, gen, None)       # the other buffer
+            r.append_markers(M)
+            loaded += M.size
+            pieces += 1
+            if pieces % 8 == 0:
+                print("large: %d of %d markers appended (%.0f s)" % (loaded, wl["n_markers"],
+                      time.perf_counter() - t0), file=sys.stderr, flush=True)
+    r.add_laser(*wl["laser"])
+    r.init()
+    live = r.marker_count()
+    print("large: %d live markers after init (%.0f s)" % (live, time.perf_counter() - t0),
+          file=sys.stderr, flush=True)
+    r.advance(a.warmup)
+    r.sync()
+    t1 = time.perf_counter()
+    r.advance(a.steps)
+    r.sync()
+    wall = time.perf_counter() - t1
+    push_ms, dep_ms, rest_ms = r.timing()
+    live_end = r.marker_count()
+    counts = r.sort(with_counts=True)
+    counts_ok = int(counts.sum()) == r.marker_count()
+    r.close()
+    avg_push = push_ms / a.steps
+    alg = push_bytes(live, wl["plasma_cells"], "fp32")
+    gbs = alg / (avg_push * 1e-3) / 1e9
+    print(json.dumps({
+        "metric": "marker-pushes/s incl. deposit+FDTD, FP32 vs FP64 on 1 GPU; p50/p99 run latency",
+        "value": 0.5 * (live + live_end) * a.steps / wall, "unit": "marker-pushes/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": 1e3 * wall / a.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": wl["name"], "grid": list(wl["n"]), "markers": int(live),
+                   "markers_end": int(live_end), "loaded": int(loaded), "pieces": pieces,
+                   "precision": "fp32 split (int32 cell + FP32 offset; FP64 J sum)",
+                   "cell_updates_per_s": int(np.prod(wl["n"])) * a.steps / wall,
+                   "push_kernel_ms": avg_push, "deposit_gather_ms": dep_ms / a.steps,
+                   "rest_of_step_ms": rest_ms / a.steps, "setup_s": t1 - t0},
+        "roofline": {"bound": "hbm", "kernel": "k_push32_tiles", "achieved": gbs,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+                     "traffic": None},
+        "checks": {"sort_counts_sum_to_live": counts_ok,
+                   "markers_only_removed": bool(live_end <= live)}}))
+
+
 def main():
     a = parse()
     if a.gpus == 1 and a.workload == "ensemble":
         return main_ensemble(a)
     if a.gpus == 1 and a.workload == "serve":
         return main_serve(a)
+    if a.gpus == 1 and a.workload == "large":
+        return main_large(a)
     if a.gpus != 1:
         print("bench.py: this build runs on one GPU only (--gpus %d requested); "
               "multi-GPU sharding is design-only" % a.gpus, file=sys.stderr)

@@ -87,6 +87,11 @@ int  mpic_get_J(mpic_run_t *run, double *J);
 /* plasma_load: records in file order.  plasma_save: live markers in device
  * order; *n in: capacity of 'rec', out: markers written. */
 int  mpic_markers_load(mpic_run_t *run, const void *rec, long n, int recsize);
+/* plasma_load's buffered read (source/core/plasma_IO.c:227-264: BUFFER_N
+ * records at a time, file after file): appends n records after the markers
+ * already loaded, so a record larger than host memory streams in pieces.
+ * Before mpic_init only; ids default to the position in the whole record. */
+int  mpic_markers_append(mpic_run_t *run, const void *rec, long n, int recsize);
 int  mpic_markers_save(mpic_run_t *run, void *rec, long *n, int recsize);
 long mpic_marker_count(mpic_run_t *run);
 

@@ -56,6 +56,7 @@ def lib():
             "mpic_get_fields": (I, [P, P, P]),
             "mpic_get_J": (I, [P, P]),
             "mpic_markers_load": (I, [P, P, LG, I]),
+            "mpic_markers_append": (I, [P, P, LG, I]),
             "mpic_markers_save": (I, [P, P, ctypes.POINTER(LG), I]),
             "mpic_marker_count": (LG, [P]),
             "mpic_add_laser": (I, [P, D, D, D, D, D, I, I, P]),
@@ -181,6 +182,13 @@ class Run:
         M = np.ascontiguousarray(M, dtype=MARKER_DTYPE)
         _check(lib().mpic_markers_load(self.handle, _ptr(M), M.size, 64),
                "mpic_markers_load")
+
+    def append_markers(self, M):
+        """Appends records after those loaded (before init), as plasma_load
+        reads a record buffer by buffer."""
+        M = np.ascontiguousarray(M, dtype=MARKER_DTYPE)
+        _check(lib().mpic_markers_append(self.handle, _ptr(M), M.size, 64),
+               "mpic_markers_append")
 
     def markers(self):
         cap = max(int(lib().mpic_marker_count(self.handle)), 1)

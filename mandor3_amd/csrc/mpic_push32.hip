@@ -596,8 +596,10 @@ __device__ static inline void covering(int i, int nt, int &lo, int &hi)
 /* J += sum of the item blocks of the tiles covering each node, in FP64.
  * Every node of an active tile's block is written once, by the first active
  * tile (in tile order) whose block holds it; the sum runs over the covering
- * tiles in tile order and over each tile's items in order, so the result
- * does not depend on scheduling.  One thread per node and all three
+ * tiles in tile order and over each tile's items in order, so the gather
+ * adds in a fixed order (what a block holds follows the device order of
+ * the tile's markers, which the sort's slot reservation by atomics does not
+ * fix from run to run).  One thread per node and all three
  * components: a node lies in at most 2 blocks per axis, and the first item
  * of each of the (at most 8) covering tiles is loaded before any is added,
  * so the gather waits for one memory latency per node, not one per block. */

@@ -367,6 +367,11 @@ extern "C" int mpic_create(const mpic_config_t *cfg, mpic_run_t **out)
             return MPIC_ERR_ARG;
         }
     }
+    /* marker offsets and counts are int32 */
+    if (cfg->max_markers >= (1L << 31)) {
+        mpic_set_error("mpic_create: capacity %ld exceeds 2^31 - 1 markers", cfg->max_markers);
+        return MPIC_ERR_ARG;
+    }
     mpic_run *r = new mpic_run();
     r->cfg = *cfg;
     r->dev = cfg->device;
@@ -410,11 +415,6 @@ extern "C" int mpic_create(const mpic_config_t *cfg, mpic_run_t **out)
     MPIC_CHECK(hipMalloc(&r->rho1, sizeof(double) * g.ncell));
     MPIC_CHECK(hipMalloc(&r->rho2, sizeof(double) * g.ncell));
     r->cap = cfg->max_markers > 0 ? cfg->max_markers : 1;
-    if (r->cap >= (1L << 31)) {
-        mpic_set_error("mpic_create: capacity %ld exceeds 2^31 - 1 markers", r->cap);
-        delete r;
-        return MPIC_ERR_ARG;
-    }
     for (int b = 0; b < 2 && r->fp32; ++b) {
         Markers32 &mb = r->m32buf[b];
         MPIC_CHECK(hipMalloc(&mb.pos, sizeof(float4) * r->cap));
@@ -771,8 +771,10 @@ static int host_key(const mpic_run *r, int i, int j, int k)
 }
 
 /* Split-precision load: walls in FP64, then cell index + FP32 offset. */
+/* Converts n records to the split form and stores the live ones from
+ * marker 'at' on. */
 static int load_split(mpic_run *r, std::vector<double> *d, std::vector<float> &w,
-                      std::vector<float> &qm, std::vector<int32_t> &id, long n)
+                      std::vector<float> &qm, std::vector<int32_t> &id, long n, long at)
 {
     std::vector<float> pos, mom;          /* 4 floats per marker each */
     pos.reserve(4 * n);
@@ -818,10 +820,12 @@ static int load_split(mpic_run *r, std::vector<double> *d, std::vector<float> &w
     }
     const long m = (long)ids.size();
     Markers32 &M = r->m32;
-    MPIC_CHECK(hipMemcpy(M.pos, pos.data(), sizeof(float4) * m, hipMemcpyHostToDevice));
-    MPIC_CHECK(hipMemcpy(M.mom, mom.data(), sizeof(float4) * m, hipMemcpyHostToDevice));
-    MPIC_CHECK(hipMemcpy(M.id, ids.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice));
-    r->n = m;
+    if (m > 0) {
+        MPIC_CHECK(hipMemcpy(M.pos + at, pos.data(), sizeof(float4) * m, hipMemcpyHostToDevice));
+        MPIC_CHECK(hipMemcpy(M.mom + at, mom.data(), sizeof(float4) * m, hipMemcpyHostToDevice));
+        MPIC_CHECK(hipMemcpy(M.id + at, ids.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice));
+    }
+    r->n = at + m;
     return MPIC_OK;
 }
 
@@ -865,15 +869,17 @@ static int save_split(mpic_run *r, void *rec, long *n, int recsize, long N)
     return MPIC_OK;
 }
 
-extern "C" int mpic_markers_load(mpic_run_t *r, const void *rec, long n, int recsize)
+/* Stores n records from marker 'at' on (at = 0: a fresh load). */
+static int load_records(mpic_run *r, const void *rec, long n, int recsize, long at,
+                        const char *who)
 {
     if (!r || (!rec && n) || n < 0 || (recsize != 56 && recsize != 64)) {
-        mpic_set_error("mpic_markers_load: bad arguments");
+        mpic_set_error("%s: bad arguments", who);
         return MPIC_ERR_ARG;
     }
     r->graphs_ok = 0;
-    if (n > r->cap) {
-        mpic_set_error("mpic_markers_load: %ld markers exceed the capacity %ld", n, r->cap);
+    if (n > r->cap - at) {
+        mpic_set_error("%s: %ld markers exceed the capacity %ld", who, at + n, r->cap);
         return MPIC_ERR_CAPACITY;
     }
     MPIC_CHECK(hipSetDevice(r->dev));
@@ -891,17 +897,32 @@ extern "C" int mpic_markers_load(mpic_run_t *r, const void *rec, long n, int rec
         d[3][q] = v.vx; d[4][q] = v.vy; d[5][q] = v.vz;
         w[q] = v.rho;
         qm[q] = v.qDivM;
-        id[q] = (recsize == 64) ? v.id : (int32_t)(q + 1);
+        id[q] = (recsize == 64) ? v.id : (int32_t)(at + q + 1);
     }
-    if (r->fp32) return load_split(r, d, w, qm, id, n);
+    if (r->fp32) return load_split(r, d, w, qm, id, n, at);
+    r->n = at + n;
+    if (n == 0) return MPIC_OK;
     double *dst[6] = {r->m.x, r->m.y, r->m.z, r->m.ux, r->m.uy, r->m.uz};
     for (int c = 0; c < 6; ++c)
-        MPIC_CHECK(hipMemcpy(dst[c], d[c].data(), sizeof(double) * n, hipMemcpyHostToDevice));
-    MPIC_CHECK(hipMemcpy(r->m.w, w.data(), sizeof(float) * n, hipMemcpyHostToDevice));
-    MPIC_CHECK(hipMemcpy(r->m.qm, qm.data(), sizeof(float) * n, hipMemcpyHostToDevice));
-    MPIC_CHECK(hipMemcpy(r->m.id, id.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
-    r->n = n;
+        MPIC_CHECK(hipMemcpy(dst[c] + at, d[c].data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    MPIC_CHECK(hipMemcpy(r->m.w + at, w.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+    MPIC_CHECK(hipMemcpy(r->m.qm + at, qm.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+    MPIC_CHECK(hipMemcpy(r->m.id + at, id.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
     return MPIC_OK;
+}
+
+extern "C" int mpic_markers_load(mpic_run_t *r, const void *rec, long n, int recsize)
+{
+    return load_records(r, rec, n, recsize, 0, "mpic_markers_load");
+}
+
+extern "C" int mpic_markers_append(mpic_run_t *r, const void *rec, long n, int recsize)
+{
+    if (r && r->inited) {
+        mpic_set_error("mpic_markers_append: markers are appended before mpic_init");
+        return MPIC_ERR_STATE;
+    }
+    return load_records(r, rec, n, recsize, r ? r->n : 0, "mpic_markers_append");
 }
 
 extern "C" int mpic_markers_save(mpic_run_t *r, void *rec, long *n, int recsize)

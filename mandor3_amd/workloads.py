@@ -108,6 +108,86 @@ def laser_foil(scale=1.0, seed=12345, ppc_sub=(3, 3, 3)):
                 laser=(lp, lmap), ppc=2 * ppc, plasma_cells=cells.shape[0])
 
 
+def large_foil(seed=4000, planes_per_chunk=8):
+    """The 512x256x256 single-GPU state, h = lambda/24, Mur walls: 16
+    markers per cell per species (4x2x2 sub-lattice) in every cell one cell
+    or more from the walls (510 x 254 x 254 cells, 1 052 949 120 markers:
+    the full-box fill used for throughput), weighted by the physical
+    profile along x: a 40 n_c foil 1 lambda thick from x = 256 cells, a
+    2 lambda exponential preplasma in front of it (scale length 0.5 lambda)
+    and a 0.1 n_c floor elsewhere; electrons Maxwellian vth = 0.01 c, cold
+    protons; a0 = 10, waist 4 lambda, 15-period beam injected on x = 20.
+    The markers are not built here: 'chunks' yields them a few i planes at
+    a time (about 1 GB of records each), in id order."""
+    n = (512, 256, 256)
+    h = (1.0 / 24,) * 3
+    L = tuple(n[a] * h[a] for a in range(3))
+    sub = _SUB[16]
+    ppc = sub[0] * sub[1] * sub[2]
+    i_lo, i_hi = 1, n[0] - 2                   # the open walls delete markers within a cell
+    jk = (n[1] - 2) * (n[2] - 2)
+    foil0, foil1 = 256, 256 + 24
+    pre0 = foil0 - 48
+
+    def density(i):
+        """n / n_c of cell plane i (profile at the cell centre)."""
+        x = (i + 0.5) * h[0]
+        if foil0 <= i < foil1:
+            return 40.0
+        if pre0 <= i < foil0:
+            return max(40.0 * np.exp(-(foil0 * h[0] - x) / 0.5), 0.1)
+        return 0.1
+
+    def chunks():
+        """Records of planes [i0, i0 + planes_per_chunk), alternately in two
+        reused buffers (a chunk stays valid while the next one is built):
+        positions on the sub-lattice by broadcasting per axis (the order of
+        _quiet_cells: cells i, j, k, then sub-points, last axis fastest),
+        electrons then protons."""
+        nj, nk = n[1] - 2, n[2] - 2
+        sx, sy, sz = sub
+        bufs = [np.zeros(2 * planes_per_chunk * jk * ppc, dtype=MARKER_DTYPE) for _ in range(2)]
+        oj = (np.arange(1, n[1] - 1)[:, None] + (np.arange(sy) + 0.5) / sy) * h[1]      # (nj, sy)
+        ok = (np.arange(1, n[2] - 1)[:, None] + (np.arange(sz) + 0.5) / sz) * h[2]      # (nk, sz)
+        nid = 1
+        for c, i0 in enumerate(range(i_lo, i_hi + 1, planes_per_chunk)):
+            i1 = min(i0 + planes_per_chunk, i_hi + 1)
+            P = i1 - i0
+            N = P * jk * ppc
+            M = bufs[c % 2][:2 * N]
+            rng = np.random.default_rng((seed, c))
+            oi = (np.arange(i0, i1)[:, None] + (np.arange(sx) + 0.5) / sx) * h[0]      # (P, sx)
+            w = (np.array([density(i) for i in range(i0, i1)]) * (RHO_CRIT / ppc))
+            for s, (sign, qm, vth) in enumerate(((-1.0, -1.0, 0.01), (1.0, QM_PROTON, 0.0))):
+                V = M[s * N:(s + 1) * N].reshape(P, nj, nk, sx, sy, sz)
+                V["x"] = oi[:, None, None, :, None, None]
+                V["y"] = oj[None, :, None, None, :, None]
+                V["z"] = ok[None, None, :, None, None, :]
+                V["rho"] = (sign * w)[:, None, None, None, None, None]
+                V["qDivM"] = qm
+                Vf = M[s * N:(s + 1) * N]
+                if vth > 0:
+                    for f in ("vx", "vy", "vz"):
+                        Vf[f] = rng.normal(0.0, vth, size=N)
+                else:
+                    for f in ("vx", "vy", "vz"):
+                        Vf[f] = 0.0
+            M["id"] = np.arange(nid, nid + 2 * N, dtype=np.int32)
+            nid += 2 * N
+            yield M
+
+    cells_filled = (i_hi - i_lo + 1) * jk
+    plane = 20
+    focus_x = foil0 * h[0]
+    lp = laser_params(10.0, plane, h, focus_x, 15.0, focus_x - plane * h[0] + 2.0)
+    lmap = gaussian_map(n, h, plane, 10.0, 4.0, (focus_x, L[1] / 2, L[2] / 2))
+    return dict(name="large 512x256x256 h=lambda/24, full-box fill 16 ppc x 2 species, 40 n_c foil "
+                     "1 lambda + 2 lambda preplasma, a0=10 w0=4 lambda, Mur",
+                n=n, h=h, L=L, tau=courant_tau(h), bc=(BC_OPEN,) * 6, markers=None, chunks=chunks,
+                n_markers=2 * ppc * cells_filled, laser=(lp, lmap), ppc=2 * ppc,
+                plasma_cells=cells_filled)
+
+
 def two_stream_box(n=32, seed=7):
     """32^3 periodic box, h = 0.05, electrons (vth 0.05 + beams +-0.2 c
     along x) and cold ions, 8 markers per cell each, n_e = n_i = 1 n_c."""

@@ -154,3 +154,46 @@ def test_fp32_charge_conservation():
         res = g.gauss_after()
         print("fp32 continuity residual %.2e of scale" % (res[0] / scale))
         assert res[0] < 1e-5 * scale
+
+
+@pytest.mark.parametrize("prec", [ma.FP64, ma.FP32_SPLIT])
+def test_append_in_pieces_equals_one_load(prec):
+    """mpic_markers_append (plasma_load's buffered read): a record loaded in
+    three ragged pieces, one of them empty, gives the run a record loaded at
+    once gives: the same markers bit for bit after the first sort, and after
+    five steps the same markers and fields to 1e-12 (FP64 form) or 1e-5
+    (split form).  Two runs are not bit-identical after steps: the sort
+    reserves a cell's slots by atomics, so the order of a cell's markers,
+    and with it the order of J sums, varies from run to run.  Appends past
+    the capacity and after mpic_init are refused."""
+    fx = load_fixture("mixed")
+    M0 = fx["M0"]
+    runs, first = [], []
+    for pieces in ([M0], [M0[:1000], M0[1000:1000], M0[1000:]]):
+        r = ma.Run(fx["n"], fx["h"], fx["L"], fx["tau"][0], fx["bc"], 0.0, precision=prec,
+                   max_markers=M0.size)
+        r.set_fields(fx["E0"], fx["H0"])
+        r.load_markers(pieces[0])
+        for p in pieces[1:]:
+            r.append_markers(p)
+        if len(pieces) > 1:
+            with pytest.raises(RuntimeError):
+                r.append_markers(M0)                       # capacity
+        r.init()
+        with pytest.raises(RuntimeError):
+            r.append_markers(M0[:0])                       # after init
+        first.append(r.markers())
+        for _ in range(5):
+            r.step(diag=False)
+        runs.append(r)
+    assert by_id(first[0]).tobytes() == by_id(first[1]).tobytes()
+    a, b = runs
+    Ma, Mb = by_id(a.markers()), by_id(b.markers())
+    (Ea, Ha), (Eb, Hb) = a.fields(), b.fields()
+    assert np.array_equal(Ma["id"], Mb["id"])
+    tol = 1e-12 if prec == ma.FP64 else 1e-5
+    for f in ("vx", "vy", "vz"):
+        assert rel_l2(Ma[f], Mb[f]) <= tol, f
+    for f, h in zip(("x", "y", "z"), fx["h"]):
+        assert np.max(np.abs(Ma[f] - Mb[f])) <= tol * h, f
+    assert rel_l2(Ea, Eb) <= tol and rel_l2(Ha, Hb) <= tol
