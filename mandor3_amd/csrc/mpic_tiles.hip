@@ -107,22 +107,27 @@ __global__ void k_scan_b(int *blocksum, int nb)
 }
 
 /* Tile offsets, the active-tile list and, for every non-empty tile, the
- * per-key offsets and cursors with its counters reset: one wavefront per
- * tile, one lane per cell key (coalesced).  The tile's total comes from the
- * tile scan, so the wavefront of an empty tile (its counters are all 0)
- * writes two words and leaves: most tiles of a sparse plasma are empty. */
+ * per-key offsets and cursors with its counters reset.  One wavefront per
+ * SC_TPW consecutive tiles: lane l < SC_TPW keeps the books of tile l (its
+ * total comes from the tile scan), then the wavefront takes the non-empty
+ * tiles one after the other, one lane per cell key (coalesced).  Empty
+ * tiles (their counters are all 0) cost two words: most tiles of a sparse
+ * plasma are empty, and fewer wavefronts than tiles keep the launch short. */
+#define SC_TPW 8
 __global__ void k_scan_c(TileView tv, const int *blocksum, const int *tileloc, int nb,
                          int parity_next)
 {
-    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int t = (int)(q / TILE_CELLS), lane = threadIdx.x & 63;
-    if (t >= tv.ntiles) return;                      /* whole waves exit together */
-    const int tb = t / SB, tn = t + 1;
-    const int base = blocksum[NSCAN * tb] + tileloc[NSCAN * t];
-    const int next = tn < tv.ntiles ? blocksum[NSCAN * (tn / SB)] + tileloc[NSCAN * tn]
-                                    : blocksum[NSCAN * nb];
-    const int tot = next - base;
-    if (lane == 0) {
+    const int lane = threadIdx.x & 63;
+    const int w = (int)(((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const int t0 = w * SC_TPW;
+    if (t0 >= tv.ntiles) return;                     /* whole waves exit together */
+    int base = 0, tot = 0;
+    if (lane < SC_TPW && t0 + lane < tv.ntiles) {
+        const int t = t0 + lane, tb = t / SB, tn = t + 1;
+        base = blocksum[NSCAN * tb] + tileloc[NSCAN * t];
+        const int next = tn < tv.ntiles ? blocksum[NSCAN * (tn / SB)] + tileloc[NSCAN * tn]
+                                        : blocksum[NSCAN * nb];
+        tot = next - base;
         tv.toff[t] = base;
         const int a = blocksum[NSCAN * tb + 1] + tileloc[NSCAN * t + 1];
         if (tot > 0) {
@@ -134,7 +139,7 @@ __global__ void k_scan_c(TileView tv, const int *blocksum, const int *tileloc, i
         }
         tv.slot[t] = tot > 0 ? a : -1;
     }
-    if (q == 0) {
+    if (w == 0 && lane == 0) {
         tv.off[tv.ncellp] = blocksum[NSCAN * nb];
         tv.toff[tv.ntiles] = blocksum[NSCAN * nb];
         *tv.nactive = blocksum[NSCAN * nb + 1];
@@ -143,17 +148,21 @@ __global__ void k_scan_c(TileView tv, const int *blocksum, const int *tileloc, i
         tv.nmark[parity_next] = blocksum[NSCAN * nb];
         for (int x = 0; x < MPIC_XCDS; ++x) tv.queue[x] = 0;
     }
-    if (tot == 0) return;                            /* wave-uniform */
-    const int c = tv.count[q];
-    int incl = c;
-    for (int d = 1; d < 64; d <<= 1) {
-        const int v = __shfl_up(incl, d, 64);
-        if (lane >= d) incl += v;
+    for (int it = 0; it < SC_TPW; ++it) {
+        if (__shfl(tot, it, 64) == 0) continue;      /* wave-uniform */
+        const int tb0 = __shfl(base, it, 64);
+        const long q = (long)(t0 + it) * TILE_CELLS + lane;
+        const int c = tv.count[q];
+        int incl = c;
+        for (int d = 1; d < 64; d <<= 1) {
+            const int v = __shfl_up(incl, d, 64);
+            if (lane >= d) incl += v;
+        }
+        const int o = tb0 + incl - c;
+        tv.off[q] = o;
+        tv.cursor[q] = o;
+        tv.count[q] = 0;
     }
-    const int o = base + incl - c;
-    tv.off[q] = o;
-    tv.cursor[q] = o;
-    tv.count[q] = 0;
 }
 
 int launch_tile_scan(hipStream_t s, const TileView &tv, int *blocksum, int parity_next)
@@ -164,7 +173,8 @@ int launch_tile_scan(hipStream_t s, const TileView &tv, int *blocksum, int parit
     MPIC_CHECK(hipGetLastError());
     k_scan_b<<<1, SB, 0, s>>>(blocksum, nb);
     MPIC_CHECK(hipGetLastError());
-    k_scan_c<<<(unsigned)((tv.ncellp + 255) / 256), 256, 0, s>>>(tv, blocksum, tileloc, nb,
+    const long waves = (tv.ntiles + SC_TPW - 1) / SC_TPW;
+    k_scan_c<<<(unsigned)((waves * 64 + 255) / 256), 256, 0, s>>>(tv, blocksum, tileloc, nb,
                                                                    parity_next);
     MPIC_CHECK(hipGetLastError());
     return MPIC_OK;
