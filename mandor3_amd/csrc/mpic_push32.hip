@@ -201,9 +201,9 @@ struct MarkerRegs {
     float ox, oy, oz, ux, uy, uz, w;
 };
 
-/* Element p of a 4-byte marker array as base + 32-bit byte offset, so the
- * access uses a scalar base and a 32-bit vector offset (runs hold fewer
- * than 2^30 markers). */
+/* Element p of a 4-byte array as base + 32-bit byte offset, so the access
+ * uses a scalar base and a 32-bit vector offset (used for the field staging,
+ * whose offsets from the tile's first node span six planes). */
 template <class T>
 __device__ static inline T &at(T *base, unsigned p)
 {
