@@ -27,3 +27,41 @@ def test_python_binding_covers_header():
     lib = mandor3_amd.lib()
     for n in declared():
         assert getattr(lib, n).restype is not None or n in (), n
+
+
+def _config(n=(16, 16, 16), h=0.05, tau=0.01, precision=0, max_markers=1000):
+    import mandor3_amd
+    c = mandor3_amd.Config()
+    for a in range(3):
+        c.n[a] = n[a]
+        c.h[a] = h
+        c.L[a] = n[a] * h
+    c.tau = tau
+    for b in range(6):
+        c.bc[b] = 0
+    c.precision = precision
+    c.max_markers = max_markers
+    return c
+
+
+def test_create_rejects_bad_configs_before_touching_a_device():
+    """mpic_create's argument checks (the reference's ENSURE/DIE cases and
+    the split form's limits) return MPIC_ERR_ARG with a message before any
+    HIP call, so they run here; the marker entry points refuse a null run."""
+    import mandor3_amd
+    lib = mandor3_amd.lib()
+    run = ctypes.c_void_p()
+    cases = [
+        (_config(n=(6, 16, 16)), ">= 7"),                          # core region
+        (_config(tau=0.05), "unstable"),                           # Courant
+        (_config(precision=7), "precision"),
+        (_config(n=(2048, 2048, 1024), h=0.05, tau=0.01, precision=1), "2^29"),
+        (_config(max_markers=1 << 31), "2^31"),
+    ]
+    for cfg, words in cases:
+        rc = lib.mpic_create(ctypes.byref(cfg), ctypes.byref(run))
+        assert rc == -1, (words, rc)
+        assert words in lib.mpic_last_error().decode(), lib.mpic_last_error()
+    assert lib.mpic_create(None, ctypes.byref(run)) == -1
+    assert lib.mpic_markers_load(None, None, 0, 64) == -1
+    assert lib.mpic_markers_append(None, None, 0, 64) == -1
