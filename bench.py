@@ -100,10 +100,11 @@ def measured_traffic(kernel, workload):
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        rec = json.load(f)
-    if rec.get("workload") != workload:
-        return None
-    return rec.get("bytes_per_launch")
+        recs = json.load(f)
+    for rec in recs if isinstance(recs, list) else [recs]:
+        if rec.get("workload") == workload:
+            return rec.get("bytes_per_launch")
+    return None
 
 
 def run_gpu(ma, wl, precision, steps, warmup):
@@ -257,6 +258,8 @@ def main_large(a):
     from mandor3_amd import workloads
     if ma.device_count() < 1:
         raise SystemExit("no HIP device visible")
+    if a.copy_calibration > 0:
+        copy_calibration(a.copy_calibration)
     wl = workloads.large_foil()
     r = ma.Run(wl["n"], wl["h"], wl["L"], wl["tau"], wl["bc"], 0.0, precision=ma.FP32_SPLIT,
                max_markers=wl["n_markers"])
@@ -312,7 +315,7 @@ def main_large(a):
                    "rest_of_step_ms": rest_ms / a.steps, "setup_s": t1 - t0},
         "roofline": {"bound": "hbm", "kernel": "k_push32_tiles", "achieved": gbs,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
-                     "traffic": None},
+                     "traffic": measured_traffic("k_push32_tiles", wl["name"])},
         "checks": {"sort_counts_sum_to_live": counts_ok,
                    "markers_only_removed": bool(live_end <= live)}}))
 
