@@ -184,6 +184,7 @@ def large_foil(seed=4000, planes_per_chunk=8):
     return dict(name="large 512x256x256 h=lambda/24, full-box fill 16 ppc x 2 species, 40 n_c foil "
                      "1 lambda + 2 lambda preplasma, a0=10 w0=4 lambda, Mur",
                 n=n, h=h, L=L, tau=courant_tau(h), bc=(BC_OPEN,) * 6, markers=None, chunks=chunks,
+                density=density,
                 n_markers=2 * ppc * cells_filled, laser=(lp, lmap), ppc=2 * ppc,
                 plasma_cells=cells_filled)
 
