@@ -720,18 +720,21 @@ int launch_jgather32(hipStream_t s, const Fields32 &f, const Grid &g, const Tile
     return MPIC_OK;
 }
 
-/* Keys of the loaded markers (the cell word already holds them): counts. */
-__global__ void k_keys32(Markers32 m, long n, TileView tv)
+/* Keys of the current markers (the cell word already holds them): counts.
+ * Only the first nmark[parity] markers are live; the buffer past them holds
+ * records of earlier orders, which must not be counted. */
+__global__ void k_keys32(Markers32 m, long n, TileView tv, int parity)
 {
     const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int word = (p < n) ? ((const int *)m.pos)[4 * p] : -1;
+    const long live = tv.nmark[parity];
+    const int word = (p < n && p < live) ? ((const int *)m.pos)[4 * p] : -1;
     wave_run_count(tv.count, word < 0 ? -1 : (word & KEY_MASK));
 }
 
-int launch_keys32(hipStream_t s, const Markers32 &m, long n, const TileView &tv)
+int launch_keys32(hipStream_t s, const Markers32 &m, long n, const TileView &tv, int parity)
 {
     if (n <= 0) return MPIC_OK;
-    k_keys32<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(m, n, tv);
+    k_keys32<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(m, n, tv, parity);
     MPIC_CHECK(hipGetLastError());
     return MPIC_OK;
 }

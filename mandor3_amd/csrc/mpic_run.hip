@@ -1275,11 +1275,11 @@ static int sort_pass(mpic_run *r, int keys)
 {
     const int other = r->cur ^ 1;
     if (r->fp32) {
-        if (keys) TRY(launch_keys32(r->stream, r->m32, r->n, r->tv));
+        if (keys) TRY(launch_keys32(r->stream, r->m32, r->n, r->tv, r->parity));
         TRY(launch_tile_scan(r->stream, r->tv, r->blocksum, r->parity ^ 1));
         TRY(launch_scatter32(r->stream, r->m32, r->m32buf[other], r->n, r->tv, r->parity));
     } else {
-        if (keys) TRY(launch_keys64(r->stream, r->m, r->n, r->g, r->tv));
+        if (keys) TRY(launch_keys64(r->stream, r->m, r->n, r->g, r->tv, r->parity));
         TRY(launch_tile_scan(r->stream, r->tv, r->blocksum, r->parity ^ 1));
         TRY(launch_scatter64(r->stream, r->m, r->mbuf[other], r->n, r->tv, r->parity));
     }

@@ -114,7 +114,8 @@ __device__ static inline void wave_run_count(int *count, int key)
     if (head && key >= 0) atomicAdd(count + key, len);
 }
 
-int launch_keys64(hipStream_t s, const Markers64 &m, long n, const Grid &g, const TileView &tv);
+int launch_keys64(hipStream_t s, const Markers64 &m, long n, const Grid &g, const TileView &tv,
+                  int parity);
 int launch_tile_scan(hipStream_t s, const TileView &tv, int *blocksum, int parity_next);
 int launch_scatter64(hipStream_t s, const Markers64 &src, const Markers64 &dst, long nbound,
                      const TileView &tv, int parity);
@@ -133,7 +134,7 @@ int launch_push32(hipStream_t s, const Markers32 &m, const Fields32 &f, const Gr
                   double *Wacc, int nblocks);
 int launch_jgather32(hipStream_t s, const Fields32 &f, const Grid &g, const TileView &tv,
                      const float *tj, int nblocks);
-int launch_keys32(hipStream_t s, const Markers32 &m, long n, const TileView &tv);
+int launch_keys32(hipStream_t s, const Markers32 &m, long n, const TileView &tv, int parity);
 int launch_scatter32(hipStream_t s, const Markers32 &src, const Markers32 &dst, long nbound,
                      const TileView &tv, int parity);
 int launch_rho32(hipStream_t s, const Markers32 &m, long n, double *rho, const Grid &g,

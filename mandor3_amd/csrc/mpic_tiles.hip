@@ -16,12 +16,16 @@
 
 #define SB MPIC_SCAN_BLOCK       /* tiles per scan block */
 
-/* Keys of an unsorted array (after loading), with per-run counting. */
-__global__ void k_keys64(Markers64 m, long n, Grid g, TileView tv)
+/* Keys of the current array, with per-run counting.  Only the first
+ * nmark[parity] markers are live; the buffer past them holds records of
+ * earlier orders (or of loaded markers that the walls deleted), which must
+ * not be counted. */
+__global__ void k_keys64(Markers64 m, long n, Grid g, TileView tv, int parity)
 {
     const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long live = tv.nmark[parity];
     int key = -1;
-    if (p < n && m.qm[p] != 0.0f) {
+    if (p < n && p < live && m.qm[p] != 0.0f) {
         const double r[3] = {m.x[p], m.y[p], m.z[p]};
         key = cell_key_of(tv, g, r);
     }
@@ -29,10 +33,11 @@ __global__ void k_keys64(Markers64 m, long n, Grid g, TileView tv)
     wave_run_count(tv.count, key);
 }
 
-int launch_keys64(hipStream_t s, const Markers64 &m, long n, const Grid &g, const TileView &tv)
+int launch_keys64(hipStream_t s, const Markers64 &m, long n, const Grid &g, const TileView &tv,
+                  int parity)
 {
     if (n <= 0) return MPIC_OK;
-    k_keys64<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(m, n, g, tv);
+    k_keys64<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(m, n, g, tv, parity);
     MPIC_CHECK(hipGetLastError());
     return MPIC_OK;
 }

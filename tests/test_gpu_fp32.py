@@ -240,3 +240,55 @@ def test_fp32_energy_drift_and_gauss_against_fp64():
     d32, c32 = out[ma.FP32_SPLIT]
     assert d32 <= 2.0 * d64 + 1e-6
     assert c64 < 1e-12 and c32 < 1e-5
+
+
+def test_laser_foil_full_size_split_against_fp64():
+    """The 256x128x128 laser-foil state at full size (8 573 040 markers),
+    20 steps in both forms, then one staged step with the continuity check.
+    Checks that hold at any size: markers are only removed (open walls),
+    the per-cell counts of a sort add up to the live markers, the
+    continuity residual of the last step (physical here: markers leave
+    through the open walls) is the same in both forms; and the split form's
+    error against
+    the FP64 form at this size, printed and held to the fixture bars (E, H
+    rel. L2 <= 2e-4, energies rel. <= 2e-4)."""
+    from mandor3_amd import workloads
+    wl = workloads.laser_foil()
+    M = wl["markers"]
+    res = {}
+    for prec in (ma.FP64, ma.FP32_SPLIT):
+        r = ma.Run(wl["n"], wl["h"], wl["L"], wl["tau"], wl["bc"], 0.0, precision=prec,
+                   max_markers=M.size)
+        z = np.zeros(3 * int(np.prod([v + 1 for v in wl["n"]])))
+        r.set_fields(z, z)
+        r.load_markers(M)
+        r.add_laser(*wl["laser"])
+        r.init()
+        live0 = r.marker_count()
+        r.advance(19)
+        d = r.step(diag=True)
+        scale = np.abs(M["rho"]).max() / wl["tau"]
+        r.stage("em_h_half")
+        r.gauss_before()
+        r.stage("push_deposit")
+        r.stage("em_h_step")
+        r.stage("em_e_start")
+        r.stage("j_finish")
+        r.stage("em_e_finish")
+        cont = r.gauss_after()[0] / scale
+        live = r.marker_count()
+        counts = r.sort(with_counts=True)
+        assert int(counts.sum()) == live <= live0
+        E, H = r.fields()
+        res[prec] = (E, H, d, cont)
+        r.close()
+        print("%s: live %d -> %d, continuity %.2e of scale, W_E %.6e W_M %.6e"
+              % ("fp64" if prec == ma.FP64 else "fp32", live0, live, cont, d[1], d[2]))
+    E64, H64, d64, c64 = res[ma.FP64]
+    E32, H32, d32, c32 = res[ma.FP32_SPLIT]
+    eE, eH = rel_l2(E32, E64), rel_l2(H32, H64)
+    eW = [abs(d32[q] - d64[q]) / max(abs(d64[q]), 1e-300) for q in (1, 2, 3, 4, 5)]
+    print("split vs fp64 after 20 steps: E %.2e H %.2e, energies %s" % (eE, eH, ["%.1e" % v for v in eW]))
+    assert abs(c32 - c64) <= 1e-4 * c64
+    assert eE <= 2e-4 and eH <= 2e-4
+    assert max(eW) <= 2e-4
