@@ -296,7 +296,7 @@ def main_large(a):
     push_ms, dep_ms, rest_ms = r.timing()
     live_end = r.marker_count()
     counts = r.sort(with_counts=True)
-    counts_ok = int(counts.sum()) == r.marker_count()
+    counts_ok = int(counts.sum()) == live_end == r.marker_count()
     r.close()
     avg_push = push_ms / a.steps
     alg = push_bytes(live, wl["plasma_cells"], "fp32")
