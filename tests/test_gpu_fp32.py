@@ -292,3 +292,26 @@ def test_laser_foil_full_size_split_against_fp64():
     assert abs(c32 - c64) <= 1e-4 * c64
     assert eE <= 2e-4 and eH <= 2e-4
     assert max(eW) <= 2e-4
+
+
+@pytest.mark.parametrize("prec", [ma.FP64, ma.FP32_SPLIT])
+def test_repeated_sorts_after_wall_deletions_keep_the_live_count(prec):
+    """Regression: an explicit sort after markers have left through the open
+    walls counts only the live markers, not the stale records past them in
+    the buffer, so repeated sorts and later steps keep the count exact."""
+    fx = load_fixture("mixed")
+    r = ma.Run(fx["n"], fx["h"], fx["L"], fx["tau"][0], fx["bc"], 0.0, precision=prec,
+               max_markers=fx["M0"].size)
+    r.set_fields(fx["E0"], fx["H0"])
+    r.load_markers(fx["M0"])
+    r.init()
+    for rounds in range(3):
+        for _ in range(4):
+            r.step(diag=False)
+        live = r.marker_count()
+        for _ in range(2):
+            counts = r.sort(with_counts=True)
+            assert int(counts.sum()) == live == r.marker_count()
+        M = r.markers()
+        assert M.size == live and np.unique(M["id"]).size == live
+    assert live < fx["M0"].size                      # the walls did delete markers
