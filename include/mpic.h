@@ -84,8 +84,9 @@ int  mpic_get_fields(mpic_run_t *run, double *E, double *H);
 /* J after jbc_finish, nodes [-2,n+2], interleaved, k fastest. */
 int  mpic_get_J(mpic_run_t *run, double *J);
 
-/* plasma_load: records in file order.  plasma_save: live markers in device
- * order; *n in: capacity of 'rec', out: markers written. */
+/* plasma_load: records in file order, before mpic_init (a load after it
+ * returns MPIC_ERR_STATE).  plasma_save: live markers in device order; *n
+ * in: capacity of 'rec', out: markers written. */
 int  mpic_markers_load(mpic_run_t *run, const void *rec, long n, int recsize);
 /* plasma_load's buffered read (source/core/plasma_IO.c:227-264: BUFFER_N
  * records at a time, file after file): appends n records after the markers
@@ -154,7 +155,8 @@ int  mpic_advance(mpic_run_t *run, int nsteps);
 int  mpic_set_graphs(mpic_run_t *run, int on);
 /* An ensemble of runs advanced together, step by step across the runs,
  * each replaying its captured step on its own stream (no counterpart in the
- * reference, which runs one simulation per job). */
+ * reference, which runs one simulation per job).  Every run must be between
+ * steps (a step begun through the stage calls: MPIC_ERR_ORDER). */
 int  mpic_batch_advance(mpic_run_t *const *runs, int nruns, int nsteps);
 /* Picks the workgroup count of the FP32 push and J gather by timing
  * 'steps' real steps per candidate (the steps advance the run; the choice

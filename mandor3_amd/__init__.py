@@ -31,6 +31,54 @@ class Config(ctypes.Structure):
                 ("device", ctypes.c_int)]
 
 
+_P, _I, _D, _LG = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_long
+# (restype, argtypes) of every entry point of include/mpic.h
+SIGNATURES = {
+    "mpic_last_error": (ctypes.c_char_p, []),
+    "mpic_device_count": (_I, []),
+    "mpic_create": (_I, [ctypes.POINTER(Config), ctypes.POINTER(_P)]),
+    "mpic_destroy": (_I, [_P]),
+    "mpic_footprint": (_I, [ctypes.POINTER(Config), ctypes.POINTER(ctypes.c_size_t)]),
+    "mpic_set_fields": (_I, [_P, _P, _P]),
+    "mpic_get_fields": (_I, [_P, _P, _P]),
+    "mpic_get_J": (_I, [_P, _P]),
+    "mpic_markers_load": (_I, [_P, _P, _LG, _I]),
+    "mpic_markers_append": (_I, [_P, _P, _LG, _I]),
+    "mpic_markers_save": (_I, [_P, _P, ctypes.POINTER(_LG), _I]),
+    "mpic_marker_count": (_LG, [_P]),
+    "mpic_add_laser": (_I, [_P, _D, _D, _D, _D, _D, _I, _I, _P]),
+    "mpic_add_gauss_spot": (_I, [_P, _P]),
+    "mpic_add_tfsf": (_I, [_P, _P, _I, _I, _P, _P]),
+    "mpic_init": (_I, [_P]),
+    "mpic_em_h_half": (_I, [_P]),
+    "mpic_em_energy": (_I, [_P, _P, _P]),
+    "mpic_push_deposit": (_I, [_P]),
+    "mpic_temperature": (_I, [_P, _P, _P, _P]),
+    "mpic_em_h_step": (_I, [_P]),
+    "mpic_em_e_start": (_I, [_P]),
+    "mpic_j_finish": (_I, [_P]),
+    "mpic_em_e_finish": (_I, [_P]),
+    "mpic_step": (_I, [_P, _P]),
+    "mpic_advance": (_I, [_P, _I]),
+    "mpic_set_graphs": (_I, [_P, _I]),
+    "mpic_batch_advance": (_I, [_P, _I, _I]),
+    "mpic_autotune": (_I, [_P, _I, _P]),
+    "mpic_sync": (_I, [_P]),
+    "mpic_time": (_D, [_P]),
+    "mpic_rho": (_I, [_P, _P]),
+    "mpic_node_fields": (_I, [_P, _I, _P]),
+    "mpic_tfsf_record": (_I, [_P, _P, _I]),
+    "mpic_tfsf_recorded": (_I, [_P, _P, _P, _P]),
+    "mpic_set_probes": (_I, [_P, _P, _I]),
+    "mpic_probe_sample": (_I, [_P, _P]),
+    "mpic_gauss_before": (_I, [_P]),
+    "mpic_gauss_after": (_I, [_P, _P]),
+    "mpic_sort": (_I, [_P, _P]),
+    "mpic_marker_cells": (_I, [_P, _P, ctypes.POINTER(_LG)]),
+    "mpic_last_timing": (_I, [_P, _P, _P, _P]),
+}
+
+
 class MpicError(RuntimeError):
     pass
 
@@ -45,52 +93,7 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise MpicError("libmpic.so is not built (python -m mandor3_amd.build)")
         L = ctypes.CDLL(LIB_PATH)
-        P, I, D, LG = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_long
-        sig = {
-            "mpic_last_error": (ctypes.c_char_p, []),
-            "mpic_device_count": (I, []),
-            "mpic_create": (I, [ctypes.POINTER(Config), ctypes.POINTER(P)]),
-            "mpic_destroy": (I, [P]),
-            "mpic_footprint": (I, [ctypes.POINTER(Config), ctypes.POINTER(ctypes.c_size_t)]),
-            "mpic_set_fields": (I, [P, P, P]),
-            "mpic_get_fields": (I, [P, P, P]),
-            "mpic_get_J": (I, [P, P]),
-            "mpic_markers_load": (I, [P, P, LG, I]),
-            "mpic_markers_append": (I, [P, P, LG, I]),
-            "mpic_markers_save": (I, [P, P, ctypes.POINTER(LG), I]),
-            "mpic_marker_count": (LG, [P]),
-            "mpic_add_laser": (I, [P, D, D, D, D, D, I, I, P]),
-            "mpic_add_gauss_spot": (I, [P, P]),
-            "mpic_add_tfsf": (I, [P, P, I, I, P, P]),
-            "mpic_init": (I, [P]),
-            "mpic_em_h_half": (I, [P]),
-            "mpic_em_energy": (I, [P, P, P]),
-            "mpic_push_deposit": (I, [P]),
-            "mpic_temperature": (I, [P, P, P, P]),
-            "mpic_em_h_step": (I, [P]),
-            "mpic_em_e_start": (I, [P]),
-            "mpic_j_finish": (I, [P]),
-            "mpic_em_e_finish": (I, [P]),
-            "mpic_step": (I, [P, P]),
-            "mpic_advance": (I, [P, I]),
-            "mpic_set_graphs": (I, [P, I]),
-            "mpic_batch_advance": (I, [P, I, I]),
-            "mpic_autotune": (I, [P, I, P]),
-            "mpic_sync": (I, [P]),
-            "mpic_time": (D, [P]),
-            "mpic_rho": (I, [P, P]),
-            "mpic_node_fields": (I, [P, I, P]),
-            "mpic_tfsf_record": (I, [P, P, I]),
-            "mpic_tfsf_recorded": (I, [P, P, P, P]),
-            "mpic_set_probes": (I, [P, P, I]),
-            "mpic_probe_sample": (I, [P, P]),
-            "mpic_gauss_before": (I, [P]),
-            "mpic_gauss_after": (I, [P, P]),
-            "mpic_sort": (I, [P, P]),
-            "mpic_marker_cells": (I, [P, P, ctypes.POINTER(LG)]),
-            "mpic_last_timing": (I, [P, P, P, P]),
-        }
-        for name, (res, args) in sig.items():
+        for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name)
             fn.restype = res
             fn.argtypes = args

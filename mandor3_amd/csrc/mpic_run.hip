@@ -913,6 +913,12 @@ static int load_records(mpic_run *r, const void *rec, long n, int recsize, long 
 
 extern "C" int mpic_markers_load(mpic_run_t *r, const void *rec, long n, int recsize)
 {
+    /* after mpic_init the tile ranges, live counts and work items describe
+     * the loaded order: a new record would be pushed through stale ones */
+    if (r && r->inited) {
+        mpic_set_error("mpic_markers_load: markers are loaded before mpic_init");
+        return MPIC_ERR_STATE;
+    }
     return load_records(r, rec, n, recsize, 0, "mpic_markers_load");
 }
 
@@ -1815,6 +1821,16 @@ extern "C" int mpic_batch_advance(mpic_run_t *const *runs, int nruns, int nsteps
             mpic_set_error("mpic_batch_advance: run %d is not an initialised tiled run", q);
             return MPIC_ERR_ARG;
         }
+        /* a replay runs whole steps: a step begun through the stage calls
+         * has to be finished through them first */
+        if (r->stage != 0) {
+            mpic_set_error("stage order violated: run %d is at stage %d, a replayed step "
+                           "starts at stage 0", q, r->stage);
+            return MPIC_ERR_ORDER;
+        }
+    }
+    for (int q = 0; q < nruns; ++q) {
+        mpic_run *r = runs[q];
         r->ev_steps = 0;
     }
     for (long left = nsteps; left > 0;) {
@@ -2000,9 +2016,10 @@ extern "C" int mpic_marker_cells(mpic_run_t *r, int32_t *cells, long *n)
     return MPIC_OK;
 }
 
-extern "C" int mpic_sort(mpic_run_t *r, int32_t *counts)
+/* mpic_sort with the per-cell counts staged in 'stage' (device memory of at
+ * least n0*n1*n2 int32, or nullptr: one is allocated for the call). */
+static int sort_counts(mpic_run *r, int32_t *counts, int32_t *stage)
 {
-    if (!r) return MPIC_ERR_ARG;
     if (!r->inited) {
         mpic_set_error("mpic_sort: run is not initialised");
         return MPIC_ERR_STATE;
@@ -2017,22 +2034,26 @@ extern "C" int mpic_sort(mpic_run_t *r, int32_t *counts)
     TRY(sort_pass(r, 1));
     if (!r->tiled) {
         /* the untiled push keeps working on the compacted array */
-        long n = 0;
         int v = 0;
         MPIC_CHECK(hipStreamSynchronize(r->stream));
         MPIC_CHECK(hipMemcpy(&v, r->tv.nmark + r->parity, sizeof(int), hipMemcpyDeviceToHost));
-        n = v;
-        r->n = n;
+        r->n = v;
     }
     if (counts) {
         const long nc = (long)r->g.n[0] * r->g.n[1] * r->g.n[2];
-        int32_t *d = nullptr;
-        MPIC_CHECK(hipMallocAsync((void **)&d, sizeof(int32_t) * nc, r->stream));
+        int32_t *d = stage;
+        if (!d) MPIC_CHECK(hipMallocAsync((void **)&d, sizeof(int32_t) * nc, r->stream));
         TRY(launch_counts_natural(r->stream, r->tv, r->g, d));
         MPIC_CHECK(hipMemcpyAsync(counts, d, sizeof(int32_t) * nc, hipMemcpyDeviceToHost,
                                   r->stream));
         MPIC_CHECK(hipStreamSynchronize(r->stream));
-        MPIC_CHECK(hipFreeAsync(d, r->stream));
+        if (!stage) MPIC_CHECK(hipFreeAsync(d, r->stream));
     }
     return MPIC_OK;
+}
+
+extern "C" int mpic_sort(mpic_run_t *r, int32_t *counts)
+{
+    if (!r) return MPIC_ERR_ARG;
+    return sort_counts(r, counts, nullptr);
 }

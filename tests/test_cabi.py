@@ -10,7 +10,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def declared():
     txt = open(os.path.join(ROOT, "include", "mpic.h")).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-    return sorted(set(re.findall(r"\b(mpic_[a-z0-9_]+)\s*\(", txt)))
+    return sorted(set(re.findall(r"\b(mpic_[A-Za-z0-9_]+)\s*\(", txt)))
 
 
 def test_library_exports_header():
@@ -23,10 +23,31 @@ def test_library_exports_header():
 
 
 def test_python_binding_covers_header():
+    """The binding's signature table names exactly the header's entry
+    points, and the entry points that return something other than int are
+    bound with their C return types."""
     import mandor3_amd
+    sig = mandor3_amd.SIGNATURES
+    names = set(declared())
+    assert set(sig) == names, (sorted(names - set(sig)), sorted(set(sig) - names))
+    assert sig["mpic_time"][0] is ctypes.c_double
+    assert sig["mpic_marker_count"][0] is ctypes.c_long
+    assert sig["mpic_last_error"][0] is ctypes.c_char_p
     lib = mandor3_amd.lib()
-    for n in declared():
-        assert getattr(lib, n).restype is not None or n in (), n
+    for n in names:
+        assert getattr(lib, n).restype is sig[n][0], n
+        assert list(getattr(lib, n).argtypes) == list(sig[n][1]), n
+    ret = {}
+    txt = open(os.path.join(ROOT, "include", "mpic.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    for t, n in re.findall(r"^\s*(int|long|double|const char \*)\s*(mpic_[A-Za-z0-9_]+)\s*\(",
+                           txt, flags=re.M):
+        ret[n] = t.strip()
+    assert set(ret) == names
+    cmap = {"int": ctypes.c_int, "long": ctypes.c_long, "double": ctypes.c_double,
+            "const char *": ctypes.c_char_p}
+    for n, t in ret.items():
+        assert sig[n][0] is cmap[t], (n, t)
 
 
 def _config(n=(16, 16, 16), h=0.05, tau=0.01, precision=0, max_markers=1000):

@@ -2,14 +2,14 @@
 
 The split form stores a marker as an int32 cell key plus FP32 in-cell
 offsets; push, deposit and field update run in FP32, J is summed in FP64.
-It cannot match the FP64 path bit for bit; the bars below are the FP32
-error budget measured over the 20-step fixture horizon (the measured
-values are printed, and recorded with the design notes):
+It cannot match the FP64 path bit for bit; the bars below are about ten
+times the error measured on MI355X over the 20-step fixture horizon, per
+fixture (the measured values are printed with every run):
 
-  E, H relative L2 vs oracle             <= 2e-4
-  W_E, W_M, kinetic energies (relative)  <= 2e-4
+  E, H relative L2 vs oracle             FIELD_BAR[case]
+  W_E, W_M, kinetic energies (relative)  ENERGY_BAR[case]
   markers whose cell differs from the oracle's, per step <= 0.2 %
-  positions (cells units) max |error|    <= 1e-3 cells
+  positions (cells units) max |error|    <= 2e-5 cells
 """
 import numpy as np
 import pytest
@@ -20,6 +20,12 @@ from oracle_lib import MARKER_DTYPE, Oracle, load_fixture, oracle_from_fixture, 
 pytestmark = pytest.mark.gpu
 
 CASES = ["periodic", "laser_mur", "mixed", "spot", "tfsf", "tfsf_back"]
+
+# about 10x the measured 20-step errors of each fixture (printed by the tests)
+FIELD_BAR = {"periodic": 1e-5, "laser_mur": 1e-5, "mixed": 1e-5, "spot": 1e-5, "tfsf": 1e-5,
+             "tfsf_back": 1e-5}
+ENERGY_BAR = {"periodic": 5e-6, "laser_mur": 5e-6, "mixed": 5e-6, "spot": 5e-6, "tfsf": 5e-6,
+              "tfsf_back": 5e-6}
 
 
 def run32(fx, markers=True, E=None, H=None):
@@ -110,10 +116,10 @@ def test_fp32_twenty_steps(case):
                 worst_d = max(worst_d, abs(dg[q] - do[q]) / abs(do[q]))
     print("fp32 %s: fields %.2e, diag %.2e, cell mismatch %.2e, max dx %.2e cells"
           % (case, worst_f, worst_d, worst_c, worst_x))
-    assert worst_f <= 2e-4
-    assert worst_d <= 2e-4
+    assert worst_f <= FIELD_BAR[case]
+    assert worst_d <= ENERGY_BAR[case]
     assert worst_c <= 2e-3
-    assert worst_x <= 1e-3
+    assert worst_x <= 2e-5
 
 
 @pytest.mark.parametrize("prec", [ma.FP64, ma.FP32_SPLIT])

@@ -124,8 +124,10 @@ def test_twenty_steps_parity(case):
         assert np.array_equal(np.bincount(cg, minlength=ncell), np.bincount(co, minlength=ncell))
         Eg, Hg = g.fields()
         Eo, Ho = o.fields()
-        worst = max(worst, rel_l2(Eg, Eo), rel_l2(Hg, Ho))
+        Jg, Jo = g.J(), o.J()
+        worst = max(worst, rel_l2(Eg, Eo), rel_l2(Hg, Ho), rel_l2(Jg, Jo))
         assert rel_l2(Eg, Eo) <= 1e-10 and rel_l2(Hg, Ho) <= 1e-10, (case, r)
+        assert rel_l2(Jg, Jo) <= 1e-10, (case, r)
         np.testing.assert_allclose(dg[1:], do[1:], rtol=1e-10, atol=1e-300)
     # the final record against the reference's own (through the oracle pin)
     Eo20 = fx["E20"]
@@ -169,3 +171,35 @@ def test_stage_order_enforced():
     g.stage("em_h_half")
     with pytest.raises(ma.MpicError):
         g.stage("em_e_start")
+    # a replayed step starts at stage 0: a half-done step is refused, and the
+    # run stays usable once the stage calls have finished it
+    g.set_graphs(True)
+    with pytest.raises(ma.MpicError, match="stage order"):
+        g.advance(1)
+    with pytest.raises(ma.MpicError, match="stage order"):
+        ma.batch_advance([g], 1)
+    for s in ("push_deposit", "em_h_step", "em_e_start", "j_finish", "em_e_finish"):
+        g.stage(s)
+    g.advance(2)
+    g.sync()
+    assert g.time == pytest.approx(3 * fx["tau"][0], rel=1e-14)
+
+
+def test_markers_load_refused_after_init():
+    """A record loaded after mpic_init would be pushed through the tile
+    ranges and live counts of the old one: the load is refused and the run
+    keeps its markers."""
+    fx = load_fixture("mixed")
+    for prec in (ma.FP64, ma.FP32_SPLIT):
+        r = ma.Run(fx["n"], fx["h"], fx["L"], fx["tau"][0], fx["bc"], 0.0, precision=prec,
+                   max_markers=fx["M0"].size)
+        r.set_fields(fx["E0"], fx["H0"])
+        r.load_markers(fx["M0"])
+        r.init()
+        live = r.marker_count()
+        with pytest.raises(ma.MpicError, match="before mpic_init"):
+            r.load_markers(fx["M0"][:100])
+        assert r.marker_count() == live
+        r.step(diag=False)
+        r.sync()
+        r.close()
