@@ -8,9 +8,13 @@ deposit, per-step sort by cell, H step, E step, J fold, laser source, Mur /
 periodic walls) over the whole synthetic state, resident in HBM before the
 timed region; the K steps are enqueued without host synchronisation.  Prints
 one JSON line: marker-pushes/s (live markers x steps / wall) of the FP32
-split form with the FP64 plain form beside it, the roofline of the push
-kernel from HIP events on the run's stream, and the CPU oracle timed on a
-bounded sample of the same workload on this host.
+split form with the FP64 plain form beside it, the energy diagnostics of
+every step computed inside the timed region (em_energy and
+plasma_temperature of the reference loop, kept in a device ring), the
+roofline of the push kernel from HIP events on the run's stream, p50/p99
+time-to-result of a bounded serving stream in both forms, and the CPU oracle
+timed on a bounded sample of the same workload on this host's cores (one
+replicated copy per core, run as separate processes).
 """
 import argparse
 import json
@@ -44,7 +48,15 @@ def parse():
     p.add_argument("--requests", type=int, default=48, help="serving: number of requests")
     p.add_argument("--rate", type=float, default=4.0, help="serving: mean arrivals per second")
     p.add_argument("--no-cpu", action="store_true")
-    p.add_argument("--cpu-seconds", type=float, default=12.0)
+    p.add_argument("--cpu-seconds", type=float, default=10.0,
+                   help="CPU baseline: seconds of oracle steps per core")
+    p.add_argument("--cpu-cores", type=int, default=0,
+                   help="CPU baseline: copies run at once (0: the cores this process may use, "
+                        "at most 16)")
+    p.add_argument("--cpu-worker", default=None, help=argparse.SUPPRESS)
+    p.add_argument("--serve-requests", type=int, default=24,
+                   help="default workload: requests of the bounded serving stream giving "
+                        "p50/p99 (0: skip)")
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                    help="first run a plain device copy of MIB MiB three times: the reference "
                         "point that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings")
@@ -127,7 +139,10 @@ def run_gpu(ma, wl, precision, steps, warmup):
     wall = time.perf_counter() - t0
     push_ms, dep_ms, rest_ms = r.timing()
     live_end = r.marker_count()
+    hist = r.diag_history(steps)
     r.close()
+    if hist.shape[0] != steps or not np.all(np.isfinite(hist)):
+        raise SystemExit("energy diagnostics of the timed steps missing or not finite")
     avg_push = push_ms / steps
     alg = push_bytes(live, wl["plasma_cells"], precision)
     return {"pushes_per_s": 0.5 * (live + live_end) * steps / wall,
@@ -137,14 +152,19 @@ def run_gpu(ma, wl, precision, steps, warmup):
             "push_kernel_ms": avg_push, "deposit_rest_ms": dep_ms / steps,
             "rest_of_step_ms": rest_ms / steps,
             "push_kernel_bytes": alg,
-            "push_kernel_GBps": alg / (avg_push * 1e-3) / 1e9}
+            "push_kernel_GBps": alg / (avg_push * 1e-3) / 1e9,
+            "last_diag": [float(v) for v in hist[-1]]}
 
 
-def cpu_baseline(wl, seconds):
-    """The CPU oracle (serial FP64 restatement of the reference path) on the
-    same state, timed for a bounded number of steps."""
+def cpu_worker(spec):
+    """One CPU-baseline process: the oracle (serial FP64 restatement of the
+    reference path) on a fresh copy of the workload, full steps for the given
+    seconds; prints one JSON line {markers, steps, seconds}."""
+    workload, seconds = spec.split(":")
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_lib import Oracle
+    from mandor3_amd import workloads
+    wl = workloads.laser_foil() if workload == "laser-foil" else workloads.two_stream_box()
     n = wl["n"]
     o = Oracle(n, wl["h"], wl["L"], wl["tau"], wl["bc"], 0.0)
     z = np.zeros(3 * int(np.prod([v + 1 for v in n])))
@@ -154,14 +174,45 @@ def cpu_baseline(wl, seconds):
         o.add_laser(wl["laser"][0], wl["laser"][1])
     o.init()
     nm = o.marker_count()
+    del wl, z
     steps = 0
     t0 = time.perf_counter()
     while True:
         o.step()
         steps += 1
         el = time.perf_counter() - t0
-        if el >= seconds or steps >= 50:
+        if el >= float(seconds) or steps >= 200:
             break
+    o.close()
+    print(json.dumps({"markers": nm, "steps": steps, "seconds": el}))
+
+
+def host_cores():
+    try:
+        c = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        c = os.cpu_count() or 1
+    return max(1, min(c, 16))
+
+
+def cpu_baseline(workload, seconds, cores):
+    """The CPU oracle on this host's cores: `cores` processes, each running
+    full steps of its own copy of the same workload for `seconds` (the
+    reference's ensemble decomposition, independent members, one per core,
+    with no communication: an upper bound of what its MPI path reaches on
+    these cores).  The rate is the sum of the copies' marker-pushes/s."""
+    import subprocess
+    cores = cores if cores > 0 else host_cores()
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    procs = [subprocess.Popen([sys.executable, os.path.abspath(__file__), "--cpu-worker",
+                               "%s:%g" % (workload, seconds)], stdout=subprocess.PIPE, env=env)
+             for _ in range(cores)]
+    outs = []
+    for p in procs:
+        out, _ = p.communicate()
+        if p.returncode != 0:
+            raise SystemExit("CPU baseline worker failed (%d)" % p.returncode)
+        outs.append(json.loads(out.decode().strip().splitlines()[-1]))
     cpu = "unknown"
     try:
         for line in open("/proc/cpuinfo"):
@@ -170,10 +221,38 @@ def cpu_baseline(wl, seconds):
                 break
     except OSError:
         pass
-    o.close()
-    return {"value": nm * steps / el, "unit": "marker-pushes/s", "cores": 1, "kind": "port",
-            "sample": "%d full steps of the same workload (%d markers) on the serial FP64 "
-                      "oracle, %.1f s, %s" % (steps, nm, el, cpu)}
+    rate = sum(o["markers"] * o["steps"] / o["seconds"] for o in outs)
+    per = [o["markers"] * o["steps"] / o["seconds"] for o in outs]
+    return {"value": rate, "unit": "marker-pushes/s", "cores": cores, "kind": "port",
+            "sample": "%d processes, one per core, each %d-%d full steps of its own copy of the "
+                      "same workload (%d markers) on the serial FP64 oracle, %.1f-%.1f s; "
+                      "per-core %.3e-%.3e marker-pushes/s; %s"
+                      % (cores, min(o["steps"] for o in outs), max(o["steps"] for o in outs),
+                         outs[0]["markers"], min(o["seconds"] for o in outs),
+                         max(o["seconds"] for o in outs), min(per), max(per), cpu)}
+
+
+def serve_latency(requests, rate, steps=(500, 3000), seed=2024):
+    """A bounded serving stream (Poisson arrivals at `rate`, scan requests
+    with 10 % 256x128x128 requests at 16 markers per cell, one of them
+    forced into the stream) through the continuous-batching loop in both
+    forms: p50/p99 time-to-result and aggregate marker-pushes/s."""
+    from mandor3_amd import serve
+    import mandor3_amd as ma
+    out = {}
+    for p in ("fp32", "fp64"):
+        reqs = serve.poisson_requests(requests, rate, seed, 0.1, steps)
+        if not any(tuple(r.params["n"]) == (256, 128, 128) for r in reqs):
+            from mandor3_amd import workloads
+            rng = np.random.default_rng(seed + 1)
+            reqs[len(reqs) // 2].params = workloads.scan_request(rng, 1.0, steps)
+        srv = serve.Server(ma.FP32_SPLIT if p == "fp32" else ma.FP64, 240.0, 25)
+        for req in reqs:
+            srv.submit(req)
+        out[p] = srv.serve()
+        print("serve %s: %d requests, p50 %.2f s p99 %.2f s" % (p, out[p]["requests"],
+              out[p]["p50_s"], out[p]["p99_s"]), file=sys.stderr, flush=True)
+    return out
 
 
 def run_ensemble(ma, workloads, precision, runs, steps, warmup, seed=64):
@@ -322,6 +401,8 @@ def main_large(a):
 
 def main():
     a = parse()
+    if a.cpu_worker:
+        return cpu_worker(a.cpu_worker)
     if a.gpus == 1 and a.workload == "ensemble":
         return main_ensemble(a)
     if a.gpus == 1 and a.workload == "serve":
@@ -376,8 +457,20 @@ def main():
                              "push_kernel_ms": f["push_kernel_ms"],
                              "push_kernel_GBps": f["push_kernel_GBps"]}
         out["fp32_over_fp64"] = m["pushes_per_s"] / f["pushes_per_s"]
+    out["config"]["energy_diagnostics"] = "every step, on the device (last row {Time, W_E, W_M, " \
+                                          "WTx, WTy, WTz}: %s)" % (
+        ", ".join("%.6g" % v for v in m["last_diag"]))
+    if a.serve_requests > 0:
+        sv = serve_latency(a.serve_requests, a.rate)
+        out["p50_s"] = sv["fp32"]["p50_s"]
+        out["p99_s"] = sv["fp32"]["p99_s"]
+        out["serving"] = {
+            "stream": "Poisson %.1f req/s, %d laser-foil scan requests (grids 32^3..128x64x64, "
+                      "4-32 ppc, 500-3000 steps; 10%% at 256x128x128 16 ppc), 240 GB budget"
+                      % (a.rate, a.serve_requests),
+            "fp32": sv["fp32"], "fp64_plain": sv["fp64"]}
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(wl, a.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(a.workload, a.cpu_seconds, a.cpu_cores)
     print(json.dumps(out))
 
 

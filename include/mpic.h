@@ -126,6 +126,13 @@ int  mpic_add_gauss_spot(mpic_run_t *run, const double *p);
 int  mpic_add_tfsf(mpic_run_t *run, const int reg[6], int mode, int nframes,
                    const double *const dataE[12], const double *const dataH[12]);
 
+/* VSP_configure (source/core/plasma_VSP.c:18-27; core.out's --VSP:weight
+ * option, source/core/main.c:314-320): the (beta, alpha, beta) convolution,
+ * beta = (1 - alpha)/2, of J at the end of mpic_j_finish and of the charge
+ * density of mpic_rho / the Gauss test, along z, y, x.  alpha in (-0.1,
+ * 1.1]; 1 (the default) switches it off.  Between steps only. */
+int  mpic_set_vsp(mpic_run_t *run, double alpha);
+
 /* em_init + plasma_init (caps, Mur buffers, ghost nodes, walls applied to
  * the loaded markers). */
 int  mpic_init(mpic_run_t *run);
@@ -147,6 +154,13 @@ int  mpic_step(mpic_run_t *run, double *diag);
 /* n steps enqueued on the run's stream without host synchronisation; with
  * graph replay on, as replays of the captured step. */
 int  mpic_advance(mpic_run_t *run, int nsteps);
+/* The energy history of the steps run by mpic_step(run, NULL),
+ * mpic_advance and mpic_batch_advance, which compute em_energy and
+ * plasma_temperature on the device every step (source/core/main.c:371,383)
+ * into a ring of 1024 rows: the last min(max_rows, 1024) of those steps,
+ * oldest first, each {Time, W_E, W_M, WTx, WTy, WTz} as wDensity_addPoint
+ * records it (source/core/diag_WDensity.c:136); *nrows rows written. */
+int  mpic_diag_history(mpic_run_t *run, double *rows, int max_rows, int *nrows);
 /* Graph replay (no counterpart in the reference: its loop calls the stages
  * one by one): on = 1 captures one step per marker-buffer parity into HIP
  * graphs at the next advance and replays them; results are those of the

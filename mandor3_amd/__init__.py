@@ -49,6 +49,7 @@ SIGNATURES = {
     "mpic_add_laser": (_I, [_P, _D, _D, _D, _D, _D, _I, _I, _P]),
     "mpic_add_gauss_spot": (_I, [_P, _P]),
     "mpic_add_tfsf": (_I, [_P, _P, _I, _I, _P, _P]),
+    "mpic_set_vsp": (_I, [_P, _D]),
     "mpic_init": (_I, [_P]),
     "mpic_em_h_half": (_I, [_P]),
     "mpic_em_energy": (_I, [_P, _P, _P]),
@@ -60,6 +61,7 @@ SIGNATURES = {
     "mpic_em_e_finish": (_I, [_P]),
     "mpic_step": (_I, [_P, _P]),
     "mpic_advance": (_I, [_P, _I]),
+    "mpic_diag_history": (_I, [_P, _P, _I, _P]),
     "mpic_set_graphs": (_I, [_P, _I]),
     "mpic_batch_advance": (_I, [_P, _I, _I]),
     "mpic_autotune": (_I, [_P, _I, _P]),
@@ -210,6 +212,10 @@ class Run:
         _check(lib().mpic_add_laser(self.handle, A, T0, delay, dt, phi0, int(fwd),
                                     int(plane), _ptr(lmap)), "mpic_add_laser")
 
+    def set_vsp(self, alpha):
+        """The VSP (beta, alpha, beta) convolution of J and rho (1: off)."""
+        _check(lib().mpic_set_vsp(self.handle, float(alpha)), "mpic_set_vsp")
+
     def init(self):
         _check(lib().mpic_init(self.handle), "mpic_init")
 
@@ -226,6 +232,15 @@ class Run:
 
     def advance(self, nsteps):
         _check(lib().mpic_advance(self.handle, int(nsteps)), "mpic_advance")
+
+    def diag_history(self, max_rows=1024):
+        """Rows {Time, W_E, W_M, WTx, WTy, WTz} of the last asynchronous
+        steps (mpic_diag_history), oldest first."""
+        rows = np.zeros((max_rows, 6))
+        n = np.zeros(1, dtype=np.int32)
+        _check(lib().mpic_diag_history(self.handle, _ptr(rows), int(max_rows), _ptr(n)),
+               "mpic_diag_history")
+        return rows[:int(n[0])]
 
     def add_gauss_spot(self, params):
         """Hard Gaussian spot: the 11 values of .gaussSpot.cfg in file order."""

@@ -19,6 +19,7 @@
  * (3): 72 B/node.
  */
 #include "mpic_internal.h"
+#include "mpic_launch.h"
 
 #define TPB 256
 
@@ -65,11 +66,12 @@ static inline unsigned grid_span(const Span &sp, int planes)
 }
 
 /* Plane i, row j and first node k0 of this thread's group. */
-__device__ static inline bool span_group(const Span &sp, int V, int &i, int &j, int &k0)
+__device__ static inline bool span_group(const Span &sp, int V, const VBlock &b,
+                                         const VBlock &gd, int &i, int &j, int &k0)
 {
     const unsigned per = (unsigned)sp.nj * (unsigned)sp.ng;
     const unsigned bpp = (per + TPB - 1) / TPB;              /* workgroups per plane */
-    const unsigned lb = xcd_block(blockIdx.x, gridDim.x);
+    const unsigned lb = xcd_block(b.x, gd.x);
     const unsigned ip = lb / bpp;
     const unsigned t = (lb - ip * bpp) * TPB + threadIdx.x;
     if (t >= per) return false;
@@ -98,10 +100,18 @@ __device__ static inline bool box_decode(const BoxSet &bs, long t, int &i, int &
 }
 
 /* Cap H^{n+1/2} = H + c3*(dEy/dz) - c2*(dEz/dy)..., source/core/em_caps.c:418-439. */
+template <class T> struct CapArgs {
+    FieldsT<T> f;
+    Grid       g;
+    BoxSet     bs;
+    T          c1, c2, c3;
+    int        E;                  /* 0: cap H, 1: cap E */
+};
+
 template <class T>
-__global__ void k_cap_H(FieldsT<T> f, Grid g, BoxSet bs, T c1, T c2, T c3)
+__device__ static inline void cap_H(const FieldsT<T> &f, const Grid &g, const BoxSet &bs, T c1,
+                                    T c2, T c3, long t)
 {
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     int i, j, k;
     if (!box_decode(bs, t, i, j, k)) return;
     long o = gidx(g, i, j, k);
@@ -118,9 +128,9 @@ __global__ void k_cap_H(FieldsT<T> f, Grid g, BoxSet bs, T c1, T c2, T c3)
 
 /* Cap E^{n+1} from the cap H, source/core/em_caps.c:580-599. */
 template <class T>
-__global__ void k_cap_E(FieldsT<T> f, Grid g, BoxSet bs, T c1, T c2, T c3)
+__device__ static inline void cap_E(const FieldsT<T> &f, const Grid &g, const BoxSet &bs, T c1,
+                                    T c2, T c3, long t)
 {
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     int i, j, k;
     if (!box_decode(bs, t, i, j, k)) return;
     long o = gidx(g, i, j, k);
@@ -135,26 +145,37 @@ __global__ void k_cap_E(FieldsT<T> f, Grid g, BoxSet bs, T c1, T c2, T c3)
     f.Ec[2][o] = e2 - c2 * (hxj - hx) + c1 * (hyi - hy);
 }
 
+template <class T> struct KernelDef<CapArgs<T>> {
+    static constexpr int kBlock = TPB;
+    __device__ static void body(const CapArgs<T> &a, VBlock b, VBlock)
+    {
+        const long t = (long)b.x * TPB + threadIdx.x;
+        if (a.E)
+            cap_E<T>(a.f, a.g, a.bs, a.c1, a.c2, a.c3, t);
+        else
+            cap_H<T>(a.f, a.g, a.bs, a.c1, a.c2, a.c3, t);
+    }
+};
+
+template <class T>
+static int launch_cap(hipStream_t s, const FieldsT<T> &f, const Grid &g, const BoxSet &bs, int E)
+{
+    const long n = bs.off[bs.n];
+    if (n <= 0) return MPIC_OK;
+    CapArgs<T> a = {f, g, bs, (T)(g.tau / g.h[0]), (T)(g.tau / g.h[1]), (T)(g.tau / g.h[2]), E};
+    return launch_def(s, a, nblk(n));
+}
+
 template <class T>
 int launch_cap_H(hipStream_t s, const FieldsT<T> &f, const Grid &g, const BoxSet &bs)
 {
-    long n = bs.off[bs.n];
-    if (n <= 0) return MPIC_OK;
-    k_cap_H<T><<<nblk(n), TPB, 0, s>>>(f, g, bs, (T)(g.tau / g.h[0]), (T)(g.tau / g.h[1]),
-                                        (T)(g.tau / g.h[2]));
-    MPIC_CHECK(hipGetLastError());
-    return MPIC_OK;
+    return launch_cap(s, f, g, bs, 0);
 }
 
 template <class T>
 int launch_cap_E(hipStream_t s, const FieldsT<T> &f, const Grid &g, const BoxSet &bs)
 {
-    long n = bs.off[bs.n];
-    if (n <= 0) return MPIC_OK;
-    k_cap_E<T><<<nblk(n), TPB, 0, s>>>(f, g, bs, (T)(g.tau / g.h[0]), (T)(g.tau / g.h[1]),
-                                        (T)(g.tau / g.h[2]));
-    MPIC_CHECK(hipGetLastError());
-    return MPIC_OK;
+    return launch_cap(s, f, g, bs, 1);
 }
 
 /* Walls of one face: periodic, mirror and first-order Mur
@@ -266,29 +287,30 @@ __device__ static inline void face_apply(const Grid &g, const FaceOp &op, long t
 #undef AT
 }
 
-template <class T>
-__global__ void k_face(Grid g, FaceOp op)
-{
-    face_apply<T>(g, op, (long)blockIdx.x * blockDim.x + threadIdx.x);
-}
+/* One wall, or the two walls of one axis (block y picks the wall). */
+template <class T> struct FaceArgs {
+    Grid   g;
+    FaceOp op[2];
+};
 
-/* The two walls of one axis in one launch (blockIdx.y picks the wall). */
-template <class T>
-__global__ void k_face2(Grid g, FaceOp a, FaceOp b)
-{
-    face_apply<T>(g, blockIdx.y ? b : a, (long)blockIdx.x * blockDim.x + threadIdx.x);
-}
+template <class T> struct KernelDef<FaceArgs<T>> {
+    static constexpr int kBlock = TPB;
+    __device__ static void body(const FaceArgs<T> &a, VBlock b, VBlock)
+    {
+        face_apply<T>(a.g, a.op[b.y], (long)b.x * TPB + threadIdx.x);
+    }
+};
 
 int launch_face(hipStream_t s, const Grid &g, const FaceOp &op)
 {
     long n = (long)(op.phi - op.plo + 1) * (op.qhi - op.qlo + 1);
     if (n <= 0) return MPIC_OK;
-    if (op.fp32)
-        k_face<float><<<nblk(n), TPB, 0, s>>>(g, op);
-    else
-        k_face<double><<<nblk(n), TPB, 0, s>>>(g, op);
-    MPIC_CHECK(hipGetLastError());
-    return MPIC_OK;
+    if (op.fp32) {
+        FaceArgs<float> a = {g, {op, op}};
+        return launch_def(s, a, nblk(n));
+    }
+    FaceArgs<double> a = {g, {op, op}};
+    return launch_def(s, a, nblk(n));
 }
 
 /* a and b: the min and max wall of one axis with at least 8 cells, so that
@@ -300,13 +322,12 @@ int launch_face_pair(hipStream_t s, const Grid &g, const FaceOp &a, const FaceOp
     const long nb = (long)(b.phi - b.plo + 1) * (b.qhi - b.qlo + 1);
     const long n = na > nb ? na : nb;
     if (n <= 0) return MPIC_OK;
-    const dim3 grid((unsigned)nblk(n), 2);
-    if (a.fp32)
-        k_face2<float><<<grid, TPB, 0, s>>>(g, a, b);
-    else
-        k_face2<double><<<grid, TPB, 0, s>>>(g, a, b);
-    MPIC_CHECK(hipGetLastError());
-    return MPIC_OK;
+    if (a.fp32) {
+        FaceArgs<float> fa = {g, {a, b}};
+        return launch_def(s, fa, nblk(n), 2);
+    }
+    FaceArgs<double> fa = {g, {a, b}};
+    return launch_def(s, fa, nblk(n), 2);
 }
 
 __device__ static inline int in_boxes(const BoxSet &bs, int i, int j, int k)
@@ -324,14 +345,24 @@ __device__ static inline int in_boxes(const BoxSet &bs, int i, int j, int k)
  * cap layer: H = (H + Hc)/2 after the first half step (cap_flushHalfH,
  * em_caps.c:730-749) or H = Hc after the second (cap_flushH, :754-770).
  * Nodes [0, n+1]^3; interior and flush regions are disjoint. */
+template <class T> struct HUpdateArgs {
+    FieldsT<T> f;
+    Grid       g;
+    BoxSet     fl;
+    int        full;
+    Span       sp;
+    T          c1, c2, c3;
+};
+
 template <class T>
-__global__ void __launch_bounds__(TPB) k_h_update(FieldsT<T> f, Grid g, BoxSet fl, int full, Span sp,
-                                                  T c1, T c2, T c3)
+__device__ static inline void h_update(const FieldsT<T> &f, const Grid &g, const BoxSet &fl,
+                                       int full, const Span &sp, T c1, T c2, T c3,
+                                       const VBlock &b, const VBlock &gd)
 {
     typedef typename Vec<T>::v VT;
     constexpr int V = Vec<T>::N;
     int i, j, k0;
-    if (!span_group(sp, V, i, j, k0)) return;
+    if (!span_group(sp, V, b, gd, i, j, k0)) return;
     const long o = gidx(g, i, j, k0);
     const long si = (long)g.GY * g.GZ, sj = g.GZ;
     const T *Ex = f.E[0], *Ey = f.E[1], *Ez = f.E[2];
@@ -366,29 +397,44 @@ __global__ void __launch_bounds__(TPB) k_h_update(FieldsT<T> f, Grid g, BoxSet f
     *(VT *)(f.H[2] + o) = hz;
 }
 
+template <class T> struct KernelDef<HUpdateArgs<T>> {
+    static constexpr int kBlock = TPB;
+    __device__ static void body(const HUpdateArgs<T> &a, VBlock b, VBlock gd)
+    {
+        h_update<T>(a.f, a.g, a.fl, a.full, a.sp, a.c1, a.c2, a.c3, b, gd);
+    }
+};
+
 template <class T>
 int launch_h_update(hipStream_t s, const FieldsT<T> &f, const Grid &g, const BoxSet &flush,
                     int full)
 {
     const Span sp = span_of<Vec<T>::N>(0, 0, g.n[1] + 1, 0, g.n[2] + 1);
-    k_h_update<T><<<grid_span(sp, g.n[0] + 2), TPB, 0, s>>>(f, g, flush, full, sp,
-                                                            (T)(0.5 * g.tau / g.h[0]),
-                                                            (T)(0.5 * g.tau / g.h[1]),
-                                                            (T)(0.5 * g.tau / g.h[2]));
-    MPIC_CHECK(hipGetLastError());
-    return MPIC_OK;
+    HUpdateArgs<T> a = {f, g, flush, full, sp, (T)(0.5 * g.tau / g.h[0]),
+                        (T)(0.5 * g.tau / g.h[1]), (T)(0.5 * g.tau / g.h[2])};
+    return launch_def(s, a, grid_span(sp, g.n[0] + 2));
 }
 
 /* Interior E step (source/core/em.c:134-161) and/or the flush of the cap
  * layer E = Ec (cap_flushE, em_caps.c:775-794). */
+template <class T> struct EUpdateArgs {
+    FieldsT<T> f;
+    Grid       g;
+    BoxSet     fl;
+    int        interior, do_flush;
+    Span       sp;
+    T          c1, c2, c3;
+};
+
 template <class T>
-__global__ void __launch_bounds__(TPB) k_e_update(FieldsT<T> f, Grid g, BoxSet fl, int interior,
-                                                  int do_flush, Span sp, T c1, T c2, T c3)
+__device__ static inline void e_update(const FieldsT<T> &f, const Grid &g, const BoxSet &fl,
+                                       int interior, int do_flush, const Span &sp, T c1, T c2,
+                                       T c3, const VBlock &b, const VBlock &gd)
 {
     typedef typename Vec<T>::v VT;
     constexpr int V = Vec<T>::N;
     int i, j, k0;
-    if (!span_group(sp, V, i, j, k0)) return;
+    if (!span_group(sp, V, b, gd, i, j, k0)) return;
     const long o = gidx(g, i, j, k0);
     const long si = (long)g.GY * g.GZ, sj = g.GZ;
     const T *Hx = f.H[0], *Hy = f.H[1], *Hz = f.H[2];
@@ -419,30 +465,44 @@ __global__ void __launch_bounds__(TPB) k_e_update(FieldsT<T> f, Grid g, BoxSet f
     *(VT *)(f.E[2] + o) = ez;
 }
 
+template <class T> struct KernelDef<EUpdateArgs<T>> {
+    static constexpr int kBlock = TPB;
+    __device__ static void body(const EUpdateArgs<T> &a, VBlock b, VBlock gd)
+    {
+        e_update<T>(a.f, a.g, a.fl, a.interior, a.do_flush, a.sp, a.c1, a.c2, a.c3, b, gd);
+    }
+};
+
 template <class T>
 int launch_e_update(hipStream_t s, const FieldsT<T> &f, const Grid &g, const BoxSet &flush,
                     int interior, int do_flush)
 {
     const Span sp = span_of<Vec<T>::N>(0, 0, g.n[1] + 1, 0, g.n[2] + 1);
-    k_e_update<T><<<grid_span(sp, g.n[0] + 2), TPB, 0, s>>>(f, g, flush, interior, do_flush, sp,
-                                                            (T)(g.tau / g.h[0]), (T)(g.tau / g.h[1]),
-                                                            (T)(g.tau / g.h[2]));
-    MPIC_CHECK(hipGetLastError());
-    return MPIC_OK;
+    EUpdateArgs<T> a = {f, g, flush, interior, do_flush, sp, (T)(g.tau / g.h[0]),
+                        (T)(g.tau / g.h[1]), (T)(g.tau / g.h[2])};
+    return launch_def(s, a, grid_span(sp, g.n[0] + 2));
 }
 
 /* E -= tau*J over nodes [-1, n+1]^3, source/core/em.c:166-184.  FP32
  * fields: the FP64 J sum is applied in FP64 and rounded once. */
+template <class T> struct ETauJArgs {
+    FieldsT<T> f;
+    Grid       g;
+    Span       sp;
+    int       *tick;
+};
+
 template <class T>
-__global__ void __launch_bounds__(TPB) k_e_tauJ(FieldsT<T> f, Grid g, Span sp, int *tick)
+__device__ static inline void e_tauJ(const FieldsT<T> &f, const Grid &g, const Span &sp,
+                                     int *tick, const VBlock &b, const VBlock &gd)
 {
     typedef typename Vec<T>::v VT;
     typedef double VD __attribute__((ext_vector_type(Vec<T>::N)));
     constexpr int V = Vec<T>::N;
     /* the run's step counter, when this is the step's last kernel */
-    if (tick && blockIdx.x == 0 && threadIdx.x == 0) *tick += 1;
+    if (tick && b.x == 0 && threadIdx.x == 0) *tick += 1;
     int i, j, k0;
-    if (!span_group(sp, V, i, j, k0)) return;
+    if (!span_group(sp, V, b, gd, i, j, k0)) return;
     const long o = gidx(g, i, j, k0);
     const double tau = g.tau;
 #pragma unroll
@@ -458,24 +518,40 @@ __global__ void __launch_bounds__(TPB) k_e_tauJ(FieldsT<T> f, Grid g, Span sp, i
     }
 }
 
+template <class T> struct KernelDef<ETauJArgs<T>> {
+    static constexpr int kBlock = TPB;
+    __device__ static void body(const ETauJArgs<T> &a, VBlock b, VBlock gd)
+    {
+        e_tauJ<T>(a.f, a.g, a.sp, a.tick, b, gd);
+    }
+};
+
 template <class T>
 int launch_e_tauJ(hipStream_t s, const FieldsT<T> &f, const Grid &g, int *tick)
 {
     const Span sp = span_of<Vec<T>::N>(-1, -1, g.n[1] + 1, -1, g.n[2] + 1);
-    k_e_tauJ<T><<<grid_span(sp, g.n[0] + 3), TPB, 0, s>>>(f, g, sp, tick);
-    MPIC_CHECK(hipGetLastError());
-    return MPIC_OK;
+    ETauJArgs<T> a = {f, g, sp, tick};
+    return launch_def(s, a, grid_span(sp, g.n[0] + 3));
 }
 
 /* One-plane laser soft source, source/scpic/em_mirror.c:667-693.  The time
  * factors (envelope, cos, sin of the carrier) are evaluated in FP64 on the
  * host once per step and passed in. */
+template <class T> struct LaserArgs {
+    T            *Fy, *Fz;
+    Grid          g;
+    const double *map;
+    int           pos, JMIN, KMIN, KMAX, jmin, jmax, kmin, kmax;
+    const double *tab;
+    const int    *step;
+    int           cap;
+};
+
 template <class T>
-__global__ void k_laser(T *Fy, T *Fz, Grid g, const double *map, int pos, int JMIN, int KMIN,
-                        int KMAX, int jmin, int jmax, int kmin, int kmax, const double *tab,
-                        const int *step, int cap)
+__device__ static inline void laser(T *Fy, T *Fz, const Grid &g, const double *map, int pos,
+                                    int JMIN, int KMIN, int KMAX, int jmin, int jmax, int kmin,
+                                    int kmax, const double *tab, const int *step, int cap, long t)
 {
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     int nk = kmax - kmin + 1;
     if (t >= (long)(jmax - jmin + 1) * nk) return;
     /* {A, cos, sin} of this step, evaluated on the host */
@@ -490,6 +566,15 @@ __global__ void k_laser(T *Fy, T *Fz, Grid g, const double *map, int pos, int JM
     Fz[o] = (T)(Fz[o] + A * (w[0] * cs + w[1] * sn));
 }
 
+template <class T> struct KernelDef<LaserArgs<T>> {
+    static constexpr int kBlock = TPB;
+    __device__ static void body(const LaserArgs<T> &a, VBlock b, VBlock)
+    {
+        laser<T>(a.Fy, a.Fz, a.g, a.map, a.pos, a.JMIN, a.KMIN, a.KMAX, a.jmin, a.jmax, a.kmin,
+                 a.kmax, a.tab, a.step, a.cap, (long)b.x * TPB + threadIdx.x);
+    }
+};
+
 template <class T>
 int launch_laser(hipStream_t s, T *Fy, T *Fz, const Grid &g, const double *map, int pos, int JMIN,
                  int KMIN, int KMAX, int jmin, int jmax, int kmin, int kmax, const double *tab,
@@ -497,21 +582,28 @@ int launch_laser(hipStream_t s, T *Fy, T *Fz, const Grid &g, const double *map, 
 {
     long n = (long)(jmax - jmin + 1) * (kmax - kmin + 1);
     if (jmax < jmin || kmax < kmin) return MPIC_OK;
-    k_laser<T><<<nblk(n), TPB, 0, s>>>(Fy, Fz, g, map, pos, JMIN, KMIN, KMAX, jmin, jmax, kmin,
-                                        kmax, tab, step, cap);
-    MPIC_CHECK(hipGetLastError());
-    return MPIC_OK;
+    LaserArgs<T> a = {Fy, Fz, g, map, pos, JMIN, KMIN, KMAX, jmin, jmax, kmin, kmax, tab, step, cap};
+    return launch_def(s, a, nblk(n));
 }
 
 /* Hard Gaussian spot (source/core/em_gaussSpot.c:129-156): Ey, Ez on plane
  * i over j, k in [-1, n+1] set to the step's amplitudes times the spatial
  * profiles (both evaluated on the host). */
+template <class T> struct SpotArgs {
+    T            *Ey, *Ez;
+    Grid          g;
+    int           i;
+    const double *py, *pz, *tab;
+    const int    *step;
+    int           cap;
+};
+
 template <class T>
-__global__ void k_spot(T *Ey, T *Ez, Grid g, int i, const double *py, const double *pz,
-                       const double *tab, const int *step, int cap)
+__device__ static inline void spot(T *Ey, T *Ez, const Grid &g, int i, const double *py,
+                                   const double *pz, const double *tab, const int *step, int cap,
+                                   long t)
 {
     const int nk = g.n[2] + 3;
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long)(g.n[1] + 3) * nk) return;
     const unsigned jq = (unsigned)t / (unsigned)nk;
     const int j = -1 + (int)jq, k = -1 + (int)((unsigned)t - jq * (unsigned)nk);
@@ -521,18 +613,26 @@ __global__ void k_spot(T *Ey, T *Ez, Grid g, int i, const double *py, const doub
     Ez[o] = (T)(e[1] * pz[t]);
 }
 
+template <class T> struct KernelDef<SpotArgs<T>> {
+    static constexpr int kBlock = TPB;
+    __device__ static void body(const SpotArgs<T> &a, VBlock b, VBlock)
+    {
+        spot<T>(a.Ey, a.Ez, a.g, a.i, a.py, a.pz, a.tab, a.step, a.cap,
+                (long)b.x * TPB + threadIdx.x);
+    }
+};
+
 template <class T>
 int launch_spot(hipStream_t s, T *Ey, T *Ez, const Grid &g, int i, const double *py,
                 const double *pz, const double *tab, const int *step, int cap)
 {
     const long n = (long)(g.n[1] + 3) * (g.n[2] + 3);
-    k_spot<T><<<nblk(n), TPB, 0, s>>>(Ey, Ez, g, i, py, pz, tab, step, cap);
-    MPIC_CHECK(hipGetLastError());
-    return MPIC_OK;
+    SpotArgs<T> a = {Ey, Ez, g, i, py, pz, tab, step, cap};
+    return launch_def(s, a, nblk(n));
 }
 
 /* TFSF_addSource / TFSF_subSource (source/core/em_TFSF.c:666-772) for all
- * twelve faces of one kind in one launch: blockIdx.y is the face, a thread
+ * twelve faces of one kind in one launch: block y is the face, a thread
  * one node of it.  Faces of the same component meet along the interface's
  * edges, and the reference adds them one face after the other, so a node
  * shared by several faces is updated by the thread of the first of them,
@@ -545,14 +645,23 @@ __device__ static inline bool tfsf_has(const TfsfFace &f, const int r[3])
            r[1] <= f.hi[1] && r[2] >= f.lo[2] && r[2] <= f.hi[2];
 }
 
+template <class T> struct TfsfArgs {
+    T            *F0, *F1, *F2;
+    Grid          g;
+    TfsfSet       fs;
+    const double *data;
+    const int    *tab, *step;
+    int           cap, slot;
+    double        sign;
+};
+
 template <class T>
-__global__ void k_tfsf(T *F0, T *F1, T *F2, Grid g, TfsfSet fs, const double *data,
-                       const int *tab, const int *step, int cap, int slot, double sign)
+__device__ static inline void tfsf(T *F0, T *F1, T *F2, const Grid &g, const TfsfSet &fs,
+                                   const double *data, const int *tab, const int *step, int cap,
+                                   int slot, double sign, int f, long t)
 {
-    const int f = blockIdx.y;
     const TfsfFace &fc = fs.f[f];
     if (fc.off < 0) return;
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= fc.vol) return;
     const int frame = tab[2 * (*step % cap) + slot];
     if (frame < 0) return;
@@ -581,16 +690,23 @@ __global__ void k_tfsf(T *F0, T *F1, T *F2, Grid g, TfsfSet fs, const double *da
     F[o] = (T)v;
 }
 
+template <class T> struct KernelDef<TfsfArgs<T>> {
+    static constexpr int kBlock = TPB;
+    __device__ static void body(const TfsfArgs<T> &a, VBlock b, VBlock)
+    {
+        tfsf<T>(a.F0, a.F1, a.F2, a.g, a.fs, a.data, a.tab, a.step, a.cap, a.slot, a.sign,
+                (int)b.y, (long)b.x * TPB + threadIdx.x);
+    }
+};
+
 template <class T>
 int launch_tfsf(hipStream_t s, T *const F[3], const Grid &g, const TfsfSet &fs, long maxvol,
                 const double *data, const int *tab, const int *step, int cap, int slot,
                 double sign)
 {
     if (maxvol <= 0) return MPIC_OK;
-    k_tfsf<T><<<dim3(nblk(maxvol), 12), TPB, 0, s>>>(F[0], F[1], F[2], g, fs, data, tab, step,
-                                                     cap, slot, sign);
-    MPIC_CHECK(hipGetLastError());
-    return MPIC_OK;
+    TfsfArgs<T> a = {F[0], F[1], F[2], g, fs, data, tab, step, cap, slot, sign};
+    return launch_def(s, a, nblk(maxvol), 12);
 }
 
 template <class T>
@@ -600,17 +716,27 @@ struct Ptr3 {
 
 /* TFSF recording (TFSF_preHStep / TFSF_writeMesh, source/core/em_TFSF.c:
  * 573-624): at the start of a step, E on the 12 E faces and H on the 12 H
- * faces (blockIdx.y = slot, E faces first), each value times its face's
+ * faces (block y = slot, E faces first), each value times its face's
  * factor, into frame (*step - step0) of the face's frames (ep outer, eq
  * inner).  Frames past 'cap' are not stored. */
+template <class T> struct TfsfRecArgs {
+    Ptr3<T>       E, H;
+    Grid          g;
+    TfsfSet       fe, fh;
+    const double *factor;
+    double       *out;
+    const int    *step;
+    int           step0, cap;
+};
+
 template <class T>
-__global__ void k_tfsf_record(Ptr3<T> E, Ptr3<T> H, Grid g, TfsfSet fe, TfsfSet fh,
-                              const double *factor, double *out, const int *step, int step0,
-                              int cap)
+__device__ static inline void tfsf_record(const Ptr3<T> &E, const Ptr3<T> &H, const Grid &g,
+                                          const TfsfSet &fe, const TfsfSet &fh,
+                                          const double *factor, double *out, const int *step,
+                                          int step0, int cap, int slot, long t)
 {
-    const int slot = blockIdx.y, f = slot % 12;
+    const int f = slot % 12;
     const TfsfFace &fc = slot < 12 ? fe.f[f] : fh.f[f];
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= fc.vol) return;
     const int frame = *step - step0;
     if (frame < 0 || frame >= cap) return;
@@ -623,6 +749,15 @@ __global__ void k_tfsf_record(Ptr3<T> E, Ptr3<T> H, Grid g, TfsfSet fe, TfsfSet 
     out[fc.off + (long)frame * fc.vol + t] = (double)F[gidx(g, r[0], r[1], r[2])] * factor[f];
 }
 
+template <class T> struct KernelDef<TfsfRecArgs<T>> {
+    static constexpr int kBlock = TPB;
+    __device__ static void body(const TfsfRecArgs<T> &a, VBlock b, VBlock)
+    {
+        tfsf_record<T>(a.E, a.H, a.g, a.fe, a.fh, a.factor, a.out, a.step, a.step0, a.cap,
+                       (int)b.y, (long)b.x * TPB + threadIdx.x);
+    }
+};
+
 template <class T>
 int launch_tfsf_record(hipStream_t s, T *const E[3], T *const H[3], const Grid &g,
                        const TfsfSet &fe, const TfsfSet &fh, long maxvol, const double *factor,
@@ -630,34 +765,44 @@ int launch_tfsf_record(hipStream_t s, T *const E[3], T *const H[3], const Grid &
 {
     if (maxvol <= 0) return MPIC_OK;
     Ptr3<T> Pe = {{E[0], E[1], E[2]}}, Ph = {{H[0], H[1], H[2]}};
-    k_tfsf_record<T><<<dim3(nblk(maxvol), 24), TPB, 0, s>>>(Pe, Ph, g, fe, fh, factor, out, step,
-                                                            step0, cap);
-    MPIC_CHECK(hipGetLastError());
-    return MPIC_OK;
+    TfsfRecArgs<T> a = {Pe, Ph, g, fe, fh, factor, out, step, step0, cap};
+    return launch_def(s, a, nblk(maxvol), 24);
 }
 
 /* The run's step counter, advanced once per step on the stream. */
-__global__ void k_tick(int *step)
-{
-    if (threadIdx.x == 0 && blockIdx.x == 0) *step += 1;
-}
+struct TickArgs {
+    int *step;
+};
+template <> struct KernelDef<TickArgs> {
+    static constexpr int kBlock = 64;
+    __device__ static void body(const TickArgs &a, VBlock b, VBlock)
+    {
+        if (threadIdx.x == 0 && b.x == 0) *a.step += 1;
+    }
+};
 
 int launch_tick(hipStream_t s, int *step)
 {
-    k_tick<<<1, 64, 0, s>>>(step);
-    MPIC_CHECK(hipGetLastError());
-    return MPIC_OK;
+    TickArgs a = {step};
+    return launch_def(s, a, 1);
 }
 
 /* J boundary fold of one face b (order x-min, x-max, y-min, y-max, z-min,
  * z-max), source/core/plasma_currentWalls.c:132-227. */
-__global__ void k_jfold(JPtrs f, Grid g, int ea, int bc0, int bc1, int top0)
+struct JFoldArgs {
+    JPtrs f;
+    Grid  g;
+    int   ea, bc0, bc1, top0;
+};
+
+__device__ static inline void jfold(const JPtrs &f, const Grid &g, int ea, int bc0, int bc1,
+                                    int top0, const VBlock &b)
 {
-    /* wall top0 + blockIdx.y of the axis: 0 min, 1 max */
-    const int top = top0 + (int)blockIdx.y, bc = top ? bc1 : bc0;
+    /* wall top0 + block y of the axis: 0 min, 1 max */
+    const int top = top0 + (int)b.y, bc = top ? bc1 : bc0;
     const int ep = (ea == 0) ? 1 : 0, eq = (ea == 2) ? 1 : 2;
     const int np = g.n[ep] + 5, nq = g.n[eq] + 5;
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long t = (long)b.x * TPB + threadIdx.x;
     if (t >= (long)np * nq) return;
     int r[3];
     const unsigned pq = (unsigned)t / (unsigned)nq;
@@ -730,22 +875,31 @@ __global__ void k_jfold(JPtrs f, Grid g, int ea, int bc0, int bc1, int top0)
 /* The J fold of both walls of axis ea (min first).  A periodic axis is one
  * fold; two non-periodic walls of an axis with at least 8 cells touch the
  * disjoint layers [-2, 2] and [N-2, N+2] and share one launch. */
+template <> struct KernelDef<JFoldArgs> {
+    static constexpr int kBlock = TPB;
+    __device__ static void body(const JFoldArgs &a, VBlock b, VBlock)
+    {
+        jfold(a.f, a.g, a.ea, a.bc0, a.bc1, a.top0, b);
+    }
+};
+
+#define TRY_LAUNCH(x)                   \
+    do {                                \
+        const int rc_ = (x);            \
+        if (rc_ != MPIC_OK) return rc_; \
+    } while (0)
+
 int launch_jfold(hipStream_t s, const JPtrs &f, const Grid &g, int ea, int bc0, int bc1)
 {
     const int ep = (ea == 0) ? 1 : 0, eq = (ea == 2) ? 1 : 2;
     const long n = (long)(g.n[ep] + 5) * (g.n[eq] + 5);
     const unsigned nb = (unsigned)nblk(n);
-    if (bc0 == MPIC_BC_PERIODIC) {
-        k_jfold<<<dim3(nb, 1), TPB, 0, s>>>(f, g, ea, bc0, bc1, 0);
-    } else if (g.n[ea] >= 8) {
-        k_jfold<<<dim3(nb, 2), TPB, 0, s>>>(f, g, ea, bc0, bc1, 0);
-    } else {
-        k_jfold<<<dim3(nb, 1), TPB, 0, s>>>(f, g, ea, bc0, bc1, 0);
-        MPIC_CHECK(hipGetLastError());
-        k_jfold<<<dim3(nb, 1), TPB, 0, s>>>(f, g, ea, bc0, bc1, 1);
-    }
-    MPIC_CHECK(hipGetLastError());
-    return MPIC_OK;
+    JFoldArgs a = {f, g, ea, bc0, bc1, 0};
+    if (bc0 == MPIC_BC_PERIODIC) return launch_def(s, a, nb, 1);
+    if (g.n[ea] >= 8) return launch_def(s, a, nb, 2);
+    TRY_LAUNCH(launch_def(s, a, nb, 1));
+    a.top0 = 1;
+    return launch_def(s, a, nb, 1);
 }
 
 /* Field energy sums, source/core/em.c:226-252 (FP64; block partial sums
@@ -794,6 +948,201 @@ int launch_energy(hipStream_t s, const FieldsT<T> &f, const Grid &g, double *acc
     return MPIC_OK;
 }
 
+/* VSP_vectorSweep / VSP_scalarSweep (source/core/plasma_VSP.c:32-176): the
+ * (beta, alpha, beta) convolution along axis 'ax' of every line of the
+ * padded grid (nodes [-2, n+2] on every axis), in place; one thread per
+ * line keeps the untouched previous value in a register, as the reference
+ * sweep does, so each new value takes the old values of its neighbours and
+ * the arithmetic is the reference's expression for expression.  Lines are
+ * dealt so that consecutive threads take consecutive k (ax = 0, 1). */
+struct VspArgs {
+    double *F0, *F1, *F2;
+    int     ncomp;
+    Grid    g;
+    int     ax;
+    double  alpha, beta;
+};
+
+__device__ static inline void vsp(double *F0, double *F1, double *F2, int ncomp, const Grid &g,
+                                  int ax, double alpha, double beta, long t)
+{
+    const int p = ax == 0 ? 1 : 0, q = ax == 2 ? 1 : 2;
+    const int np = g.n[p] + 5, nq = g.n[q] + 5;
+    if (t >= (long)np * nq) return;
+    const int a = (int)(t / nq) - 2, b = (int)(t % nq) - 2;
+    int r[3];
+    r[p] = a;
+    r[q] = b;
+    r[ax] = -2;
+    const long base = gidx(g, r[0], r[1], r[2]);
+    const long st = ax == 0 ? (long)g.GY * g.GZ : (ax == 1 ? (long)g.GZ : 1L);
+    const int len = g.n[ax] + 5;
+    double *F[3] = {F0, F1, F2};
+    for (int c = 0; c < ncomp; ++c) {
+        double *f = F[c] + base;
+        double un = f[0];
+        f[0] = beta * f[st] + alpha * f[0];
+        double cur = f[st];
+        for (int k = 1; k < len - 1; ++k) {
+            const double nx = f[(k + 1) * st];
+            const double v = alpha * (cur) + beta * (un + nx);
+            un = cur;
+            f[k * st] = v;
+            cur = nx;
+        }
+        f[(len - 1) * st] = beta * un + alpha * cur;
+    }
+}
+
+template <> struct KernelDef<VspArgs> {
+    static constexpr int kBlock = TPB;
+    __device__ static void body(const VspArgs &a, VBlock b, VBlock)
+    {
+        vsp(a.F0, a.F1, a.F2, a.ncomp, a.g, a.ax, a.alpha, a.beta, (long)b.x * TPB + threadIdx.x);
+    }
+};
+
+int launch_vsp(hipStream_t s, double *const F[3], int ncomp, const Grid &g, double alpha)
+{
+    const double beta = 0.5 * (1.0 - alpha);
+    for (int ax = 2; ax >= 0; --ax) {
+        const int p = ax == 0 ? 1 : 0, q = ax == 2 ? 1 : 2;
+        const long lines = (long)(g.n[p] + 5) * (g.n[q] + 5);
+        VspArgs a = {F[0], ncomp > 1 ? F[1] : nullptr, ncomp > 2 ? F[2] : nullptr, ncomp, g, ax,
+                     alpha, beta};
+        TRY_LAUNCH(launch_def(s, a, nblk(lines)));
+    }
+    return MPIC_OK;
+}
+
+/* em_energy of the loop body (source/core/em.c:226-252) without atomics:
+ * one wavefront per cell row (i, j), lanes along k, each workgroup's partial
+ * sums stored in part[2 * block + {0, 1}] and combined in fixed order by
+ * k_diag_ring. */
+template <class T> struct EnergyPartArgs {
+    FieldsT<T> f;
+    Grid       g;
+    double    *part;
+};
+
+template <class T>
+__device__ static inline void energy_part(const FieldsT<T> &f, const Grid &g, double *part,
+                                          const VBlock &b, const VBlock &gd)
+{
+    __shared__ double se[TPB / 64], sb[TPB / 64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int rows = g.n[0] * g.n[1];
+    double we = 0, wb = 0;
+    for (int row = b.x * (TPB / 64) + wv; row < rows; row += gd.x * (TPB / 64)) {
+        const int i = row / g.n[1], j = row - i * g.n[1];
+        const long b = gidx(g, i, j, 0);
+        const long dI = (long)g.GY * g.GZ, dJ = g.GZ;
+        for (int k = lane; k < g.n[2]; k += 64) {
+            const double ex = f.E[0][b + dI + k], ey = f.E[1][b + dJ + k],
+                         ez = f.E[2][b + k + 1], hx = f.H[0][b + dJ + k + 1],
+                         hy = f.H[1][b + dI + k + 1], hz = f.H[2][b + dI + dJ + k];
+            we += ex * ex + ey * ey + ez * ez;
+            wb += hx * hx + hy * hy + hz * hz;
+        }
+    }
+    for (int d = 32; d > 0; d >>= 1) {
+        we += __shfl_down(we, d, 64);
+        wb += __shfl_down(wb, d, 64);
+    }
+    if (lane == 0) {
+        se[wv] = we;
+        sb[wv] = wb;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0, c = 0;
+        for (int q = 0; q < TPB / 64; ++q) {
+            a += se[q];
+            c += sb[q];
+        }
+        part[2 * b.x] = a;
+        part[2 * b.x + 1] = c;
+    }
+}
+
+template <class T> struct KernelDef<EnergyPartArgs<T>> {
+    static constexpr int kBlock = TPB;
+    __device__ static void body(const EnergyPartArgs<T> &a, VBlock b, VBlock gd)
+    {
+        energy_part<T>(a.f, a.g, a.part, b, gd);
+    }
+};
+
+/* Row of the diagnostics ring of the current step (device step counter
+ * *step): the two energy sums of k_energy_part's nparts partials (fixed
+ * order) and the three kinetic sums wk[0..2] of the push. */
+struct DiagRingArgs {
+    const double *part;
+    int           nparts;
+    const double *wk;
+    double       *ring;
+    const int    *step;
+    int           cap;
+};
+
+__device__ static inline void diag_ring(const double *part, int nparts, const double *wk,
+                                        double *ring, const int *step, int cap)
+{
+    __shared__ double se[TPB], sb[TPB];
+    double a = 0, c = 0;
+    for (int q = threadIdx.x; q < nparts; q += TPB) {
+        a += part[2 * q];
+        c += part[2 * q + 1];
+    }
+    se[threadIdx.x] = a;
+    sb[threadIdx.x] = c;
+    __syncthreads();
+    for (int w = TPB / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            se[threadIdx.x] += se[threadIdx.x + w];
+            sb[threadIdx.x] += sb[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        double *row = ring + 5 * (long)(*step % cap);
+        row[0] = se[0];
+        row[1] = sb[0];
+        row[2] = wk[0];
+        row[3] = wk[1];
+        row[4] = wk[2];
+    }
+}
+
+int energy_parts(const Grid &g)
+{
+    const int rows = g.n[0] * g.n[1];
+    int nb = (rows + TPB / 64 - 1) / (TPB / 64);
+    return nb > 1024 ? 1024 : nb;
+}
+
+template <> struct KernelDef<DiagRingArgs> {
+    static constexpr int kBlock = TPB;
+    __device__ static void body(const DiagRingArgs &a, VBlock, VBlock)
+    {
+        diag_ring(a.part, a.nparts, a.wk, a.ring, a.step, a.cap);
+    }
+};
+
+template <class T>
+int launch_energy_part(hipStream_t s, const FieldsT<T> &f, const Grid &g, double *part)
+{
+    EnergyPartArgs<T> a = {f, g, part};
+    return launch_def(s, a, energy_parts(g));
+}
+
+int launch_diag_ring(hipStream_t s, const double *part, int nparts, const double *wk,
+                     double *ring, const int *step, int cap)
+{
+    DiagRingArgs a = {part, nparts, wk, ring, step, cap};
+    return launch_def(s, a, 1);
+}
+
 __global__ void k_zero(double *p, long n)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -810,21 +1159,29 @@ int launch_zero(hipStream_t s, double *p, long n)
 }
 
 /* Two ranges cleared by one launch. */
-__global__ void k_zero2(double *p, long n, double *q, long m)
-{
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < m) q[t] = 0.0;
-    for (; t < n; t += (long)gridDim.x * blockDim.x) p[t] = 0.0;
-}
+struct Zero2Args {
+    double *p;
+    long    n;
+    double *q;
+    long    m;
+};
+template <> struct KernelDef<Zero2Args> {
+    static constexpr int kBlock = TPB;
+    __device__ static void body(const Zero2Args &a, VBlock b, VBlock gd)
+    {
+        long t = (long)b.x * TPB + threadIdx.x;
+        if (t < a.m) a.q[t] = 0.0;
+        for (; t < a.n; t += (long)gd.x * TPB) a.p[t] = 0.0;
+    }
+};
 
 int launch_zero2(hipStream_t s, double *p, long n, double *q, long m)
 {
     long nb = ((n > m ? n : m) + TPB - 1) / TPB;
     if (nb > 65536) nb = 65536;
     if (nb < 1) nb = 1;
-    k_zero2<<<(unsigned)nb, TPB, 0, s>>>(p, n, q, m);
-    MPIC_CHECK(hipGetLastError());
-    return MPIC_OK;
+    Zero2Args a = {p, n, q, m};
+    return launch_def(s, a, (unsigned)nb);
 }
 
 /* CIC rho fold of one face, source/core/plasma_rho.c:23-96. */
@@ -1001,6 +1358,7 @@ int launch_probes(hipStream_t s, T *const E[3], T *const H[3], const Grid &g, co
     template int launch_spot<T>(hipStream_t, T *, T *, const Grid &, int, const double *,        \
                                 const double *, const double *, const int *, int);              \
     template int launch_energy<T>(hipStream_t, const FieldsT<T> &, const Grid &, double *);      \
+    template int launch_energy_part<T>(hipStream_t, const FieldsT<T> &, const Grid &, double *); \
     template int launch_tfsf<T>(hipStream_t, T *const[3], const Grid &, const TfsfSet &, long,   \
                                 const double *, const int *, const int *, int, int, double);   \
     template int launch_gauss<T>(hipStream_t, const FieldsT<T> &, const double *,                \

@@ -166,6 +166,8 @@ template <class T>
 int launch_spot(hipStream_t s, T *Ey, T *Ez, const Grid &g, int i, const double *py,
                 const double *pz, const double *tab, const int *step, int cap);
 int launch_jfold(hipStream_t s, const JPtrs &f, const Grid &g, int ea, int bc0, int bc1);
+/* VSP (beta, alpha, beta) sweeps along z, y, x of ncomp arrays on the padded grid. */
+int launch_vsp(hipStream_t s, double *const F[3], int ncomp, const Grid &g, double alpha);
 template <class T>
 int launch_node_fields(hipStream_t s, T *const F[3], const Grid &g, int kind, double *out);
 template <class T>
@@ -176,6 +178,14 @@ template <class T>
 int launch_probes(hipStream_t s, T *const E[3], T *const H[3], const Grid &g, const int *ijk,
                   int np, double *out);
 template <class T> int launch_energy(hipStream_t s, const FieldsT<T> &f, const Grid &g, double *acc);
+/* Loop-body diagnostics: energy partial sums (energy_parts(g) workgroups,
+ * two doubles each) and the ring row {W_E sum, W_M sum, WTx, WTy, WTz} of
+ * the device step counter's step. */
+int energy_parts(const Grid &g);
+template <class T>
+int launch_energy_part(hipStream_t s, const FieldsT<T> &f, const Grid &g, double *part);
+int launch_diag_ring(hipStream_t s, const double *part, int nparts, const double *wk,
+                     double *ring, const int *step, int cap);
 int launch_zero(hipStream_t s, double *p, long n);
 int launch_zero2(hipStream_t s, double *p, long n, double *q, long m);
 

@@ -1,0 +1,177 @@
+/*
+ * mpic_launch.h -- kernels as (argument struct, body) pairs, so that one
+ * launch can serve many runs.
+ *
+ * Every per-step kernel of the library is written as the body of a
+ * KernelDef<A> specialisation: a __device__ function of its argument struct
+ * A and of a block index and grid size passed in (VBlock), instead of the
+ * blockIdx / gridDim builtins.  Two __global__ wrappers run a body:
+ *
+ *   k_one<A>    one run: the launch's own grid (what the stage calls use);
+ *   k_multi<A>  many runs in one launch: run r owns blocks [tab[r],
+ *               tab[r+1]) of a 1-D grid, each mapped back to a block of that
+ *               run's own grid, and reads its arguments from a device array.
+ *
+ * launch_def<A>() launches a body on a stream, or, while a Recorder is set
+ * on the calling thread, appends the launch to it as data (kind, grid,
+ * argument bytes).  The ensemble code records one step of every run and
+ * fuses the launches of equal kind across runs into k_multi launches: one
+ * launch per stage per step for the whole ensemble.
+ */
+#ifndef MPIC_LAUNCH_H
+#define MPIC_LAUNCH_H
+
+#include <string.h>
+
+#include <type_traits>
+#include <vector>
+
+#include "mpic_internal.h"
+
+struct VBlock {
+    unsigned x, y, z;
+};
+
+/* Specialised per argument struct: kBlock (threads per workgroup) and
+ * body(const A &, VBlock block, VBlock grid); optionally kWaves, the
+ * wavefronts per SIMD the compiler must keep room for (its register
+ * budget). */
+template <class A> struct KernelDef;
+
+template <class A, class = void> struct WavesOf {
+    static constexpr int v = 0;
+};
+template <class A> struct WavesOf<A, std::void_t<decltype(KernelDef<A>::kWaves)>> {
+    static constexpr int v = KernelDef<A>::kWaves;
+};
+
+template <class A>
+__device__ static inline void run_one(const A &a)
+{
+    const VBlock b = {blockIdx.x, blockIdx.y, blockIdx.z}, g = {gridDim.x, gridDim.y, gridDim.z};
+    KernelDef<A>::body(a, b, g);
+}
+
+/* tab: [n + 1] first block of every run, then [3 n] the runs' own grids. */
+template <class A>
+__device__ static inline void run_multi(const A *__restrict__ args,
+                                        const unsigned *__restrict__ tab, int n)
+{
+    const unsigned blk = blockIdx.x;
+    int lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (tab[mid] <= blk) lo = mid;
+        else hi = mid;
+    }
+    const unsigned *gr = tab + (n + 1) + 3 * lo;
+    const VBlock g = {gr[0], gr[1], gr[2]};
+    const unsigned L = blk - tab[lo];
+    const VBlock b = {L % g.x, (L / g.x) % g.y, L / (g.x * g.y)};
+    KernelDef<A>::body(args[lo], b, g);
+}
+
+template <class A> __global__ void __launch_bounds__(KernelDef<A>::kBlock) k_one(A a)
+{
+    run_one(a);
+}
+template <class A>
+__global__ void __launch_bounds__(KernelDef<A>::kBlock)
+    k_multi(const A *__restrict__ args, const unsigned *__restrict__ tab, int n)
+{
+    run_multi(args, tab, n);
+}
+template <class A>
+__global__ void __launch_bounds__(KernelDef<A>::kBlock)
+    __attribute__((amdgpu_waves_per_eu(WavesOf<A>::v))) k_one_w(A a)
+{
+    run_one(a);
+}
+template <class A>
+__global__ void __launch_bounds__(KernelDef<A>::kBlock)
+    __attribute__((amdgpu_waves_per_eu(WavesOf<A>::v)))
+    k_multi_w(const A *__restrict__ args, const unsigned *__restrict__ tab, int n)
+{
+    run_multi(args, tab, n);
+}
+
+template <class A> static inline void start_one(hipStream_t s, const A &a, dim3 grid)
+{
+    if constexpr (WavesOf<A>::v > 0)
+        k_one_w<A><<<grid, KernelDef<A>::kBlock, 0, s>>>(a);
+    else
+        k_one<A><<<grid, KernelDef<A>::kBlock, 0, s>>>(a);
+}
+
+/* One launch as data. */
+struct LaunchRec {
+    const void *kind;                 /* identifies the body (KernelDef<A>::tag) */
+    unsigned    grid[3];
+    std::vector<unsigned char> args;
+    /* k_multi<A> over n runs: dargs = device array of n argument structs,
+     * dtab = the block table, blocks = tab[n] */
+    int (*multi)(hipStream_t s, const void *dargs, const unsigned *dtab, int n,
+                 unsigned blocks);
+    /* k_one<A> of this record */
+    int (*one)(hipStream_t s, const void *args, const unsigned grid[3]);
+};
+
+struct Recorder {
+    std::vector<LaunchRec> recs;
+};
+
+/* Set while a run's step is recorded instead of launched (calling thread). */
+extern thread_local Recorder *g_recorder;
+
+template <class A> struct KernelTag {
+    static const char tag;
+};
+template <class A> const char KernelTag<A>::tag = 0;
+
+template <class A>
+static int multi_launch(hipStream_t s, const void *dargs, const unsigned *dtab, int n,
+                        unsigned blocks)
+{
+    if constexpr (WavesOf<A>::v > 0)
+        k_multi_w<A><<<blocks, KernelDef<A>::kBlock, 0, s>>>((const A *)dargs, dtab, n);
+    else
+        k_multi<A><<<blocks, KernelDef<A>::kBlock, 0, s>>>((const A *)dargs, dtab, n);
+    MPIC_CHECK(hipGetLastError());
+    return MPIC_OK;
+}
+
+template <class A>
+static int one_launch(hipStream_t s, const void *args, const unsigned grid[3])
+{
+    A a;
+    memcpy(&a, args, sizeof(A));
+    start_one(s, a, dim3(grid[0], grid[1], grid[2]));
+    MPIC_CHECK(hipGetLastError());
+    return MPIC_OK;
+}
+
+/* Launches (or records) body A over a gx x gy x gz grid; an empty grid is a
+ * no-op. */
+template <class A>
+static int launch_def(hipStream_t s, const A &a, unsigned gx, unsigned gy = 1, unsigned gz = 1)
+{
+    if (gx == 0 || gy == 0 || gz == 0) return MPIC_OK;
+    if (g_recorder) {
+        LaunchRec r;
+        r.kind = &KernelTag<A>::tag;
+        r.grid[0] = gx;
+        r.grid[1] = gy;
+        r.grid[2] = gz;
+        r.args.resize(sizeof(A));
+        memcpy(r.args.data(), &a, sizeof(A));
+        r.multi = &multi_launch<A>;
+        r.one = &one_launch<A>;
+        g_recorder->recs.push_back(std::move(r));
+        return MPIC_OK;
+    }
+    start_one(s, a, dim3(gx, gy, gz));
+    MPIC_CHECK(hipGetLastError());
+    return MPIC_OK;
+}
+
+#endif

@@ -31,6 +31,7 @@
  * and 48 B of FP64 J read-modify-write.
  */
 #include "mpic_internal.h"
+#include "mpic_launch.h"
 #include "mpic_tiles.h"
 
 #define TPB 256
@@ -457,8 +458,18 @@ __device__ static inline void write_back(const Markers32 &m, const TileView &tv,
  * tile's E/H nodes are staged in LDS, the item's J is accumulated in an
  * FP32 LDS block and stored, without atomics, as the item's block in 'tj';
  * the next batch of markers is loaded before the current one is pushed. */
-__global__ void __launch_bounds__(TW) __attribute__((amdgpu_waves_per_eu(4)))
-k_push32_tiles(Markers32 m, F32View v, Push32Consts c, TileView tv, float *tj, double *Wacc)
+struct Push32Args {
+    Markers32    m;
+    F32View      v;
+    Push32Consts c;
+    TileView     tv;
+    float       *tj;
+    double      *Wacc;
+};
+
+__device__ static inline void push32_tiles(const Markers32 &m, const F32View &v,
+                                           const Push32Consts &c, const TileView &tv, float *tj,
+                                           double *Wacc, const VBlock &bk, const VBlock &gd)
 {
     __shared__ float lj[TJB];
     __shared__ float lf[6 * TFN3];
@@ -468,7 +479,7 @@ k_push32_tiles(Markers32 m, F32View v, Push32Consts c, TileView tv, float *tj, d
     const int lane = threadIdx.x & 63;
     const int nitems = *tv.nitems;
     float w0 = 0, w1 = 0, w2 = 0;         /* per-thread kinetic energy partials */
-    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+    for (int it = (int)bk.x; it < nitems; it += (int)gd.x) {
         const int sl = tv.item[it];
         const int t = tv.active[sl];
         const int u = it - tv.istart[sl], nu = tv.istart[sl + 1] - tv.istart[sl];
@@ -584,6 +595,16 @@ k_push32_tiles(Markers32 m, F32View v, Push32Consts c, TileView tv, float *tj, d
     }
 }
 
+/* One wavefront per workgroup, register budget for 4 wavefronts per SIMD. */
+template <> struct KernelDef<Push32Args> {
+    static constexpr int kBlock = TW;
+    static constexpr int kWaves = 4;
+    __device__ static void body(const Push32Args &a, VBlock b, VBlock gd)
+    {
+        push32_tiles(a.m, a.v, a.c, a.tv, a.tj, a.Wacc, b, gd);
+    }
+};
+
 /* Range of tiles u (along one axis) whose J block [4u-1, 4u+5] holds node i. */
 __device__ static inline void covering(int i, int nt, int &lo, int &hi)
 {
@@ -604,12 +625,19 @@ __device__ static inline void covering(int i, int nt, int &lo, int &hi)
  * of each of the (at most 8) covering tiles is loaded before any is added,
  * so the gather waits for one memory latency per node, not one per block. */
 #define TJN3 (TJN * TJN * TJN)
-__global__ void __launch_bounds__(TPB) k_jgather32(F32View v, TileView tv, const float *tj)
+struct JGatherArgs {
+    F32View      v;
+    TileView     tv;
+    const float *tj;
+};
+
+__device__ static inline void jgather32(const F32View &v, const TileView &tv, const float *tj,
+                                        const VBlock &bk, const VBlock &gd)
 {
     __shared__ int nfirst[27], nitem[27];    /* items of the 3x3x3 tiles around t (0: none) */
     const Grid &g = v.g;
     const int nact = *tv.nactive;
-    for (int sl = blockIdx.x; sl < nact; sl += gridDim.x) {
+    for (int sl = (int)bk.x; sl < nact; sl += (int)gd.x) {
         const int t = tv.active[sl];
         int t0, t1, t2;
         tile_coords(tv, t, t0, t1, t2);
@@ -680,6 +708,14 @@ __global__ void __launch_bounds__(TPB) k_jgather32(F32View v, TileView tv, const
     }
 }
 
+template <> struct KernelDef<JGatherArgs> {
+    static constexpr int kBlock = TPB;
+    __device__ static void body(const JGatherArgs &a, VBlock b, VBlock gd)
+    {
+        jgather32(a.v, a.tv, a.tj, b, gd);
+    }
+};
+
 int launch_push32(hipStream_t s, const Markers32 &m, const Fields32 &f, const Grid &g,
                   const PushConsts &pc, const float *qm, const TileView &tv, float *tj,
                   double *Wacc, int nblocks)
@@ -700,9 +736,8 @@ int launch_push32(hipStream_t s, const Markers32 &m, const Fields32 &f, const Gr
     c.vol = g.h[0] * g.h[1] * g.h[2];
     for (int q = 0; q < 4; ++q) c.qm[q] = qm[q];
     for (int b = 0; b < 6; ++b) c.bc[b] = pc.bc[b];
-    k_push32_tiles<<<nblocks, TW, 0, s>>>(m, v, c, tv, tj, Wacc);
-    MPIC_CHECK(hipGetLastError());
-    return MPIC_OK;
+    Push32Args a = {m, v, c, tv, tj, Wacc};
+    return launch_def(s, a, (unsigned)nblocks);
 }
 
 int launch_jgather32(hipStream_t s, const Fields32 &f, const Grid &g, const TileView &tv,
@@ -715,9 +750,8 @@ int launch_jgather32(hipStream_t s, const Fields32 &f, const Grid &g, const Tile
         v.J[a] = f.J[a];
     }
     v.g = g;
-    k_jgather32<<<nblocks, TPB, 0, s>>>(v, tv, tj);
-    MPIC_CHECK(hipGetLastError());
-    return MPIC_OK;
+    JGatherArgs a = {v, tv, tj};
+    return launch_def(s, a, (unsigned)nblocks);
 }
 
 /* Keys of the current markers (the cell word already holds them): counts.
@@ -742,9 +776,16 @@ int launch_keys32(hipStream_t s, const Markers32 &m, long n, const TileView &tv,
 /* The scatter streams every marker once: nontemporal loads and stores keep
  * it from displacing the field arrays in the caches. */
 typedef float v4f_nt __attribute__((ext_vector_type(4)));
-__global__ void k_scatter32(Markers32 src, Markers32 dst, long nbound, TileView tv, int parity)
+struct Scatter32Args {
+    Markers32 src, dst;
+    long      nbound;
+    TileView  tv;
+    int       parity;
+};
+
+__device__ static inline void scatter32(const Markers32 &src, const Markers32 &dst, long nbound,
+                                        const TileView &tv, int parity, long p)
 {
-    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long n = tv.nmark[parity];
     float4 a = make_float4(__int_as_float(-1), 0.0f, 0.0f, 0.0f);
     if (p < n && p < nbound) {
@@ -769,13 +810,20 @@ __global__ void k_scatter32(Markers32 src, Markers32 dst, long nbound, TileView 
     __builtin_nontemporal_store(idv, dst.id + d);
 }
 
+template <> struct KernelDef<Scatter32Args> {
+    static constexpr int kBlock = 256;
+    __device__ static void body(const Scatter32Args &a, VBlock b, VBlock)
+    {
+        scatter32(a.src, a.dst, a.nbound, a.tv, a.parity, (long)b.x * 256 + threadIdx.x);
+    }
+};
+
 int launch_scatter32(hipStream_t s, const Markers32 &src, const Markers32 &dst, long nbound,
                      const TileView &tv, int parity)
 {
     if (nbound <= 0) return MPIC_OK;
-    k_scatter32<<<(unsigned)((nbound + 255) / 256), 256, 0, s>>>(src, dst, nbound, tv, parity);
-    MPIC_CHECK(hipGetLastError());
-    return MPIC_OK;
+    Scatter32Args a = {src, dst, nbound, tv, parity};
+    return launch_def(s, a, (unsigned)((nbound + 255) / 256));
 }
 
 /* CIC charge density of the split-precision markers, FP64. */

@@ -16,6 +16,7 @@
  * markers per cell.
  */
 #include "mpic_internal.h"
+#include "mpic_launch.h"
 #include "mpic_tiles.h"
 
 #define TPB 256
@@ -307,10 +308,20 @@ __device__ static void block_sum3(double w3[3], double *Wacc)
  * the walls only (plasma_init).  A marker whose cell lies outside [-1, n]
  * (a state the reference never produces) is deleted and flagged in *err
  * instead of touching memory off the grid. */
-__global__ void k_push64(Markers64 m, long n, F64View v, PushConsts c, double *Wacc, int mode,
-                         unsigned *err)
+struct Push64Args {
+    Markers64  m;
+    long       n;
+    F64View    v;
+    PushConsts c;
+    double    *Wacc;
+    int        mode;
+    unsigned  *err;
+};
+
+__device__ static inline void push64(const Markers64 &m, long n, const F64View &v,
+                                     const PushConsts &c, double *Wacc, int mode, unsigned *err,
+                                     long p)
 {
-    long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
     double w3[3] = {0, 0, 0};
     if (p < n && m.qm[p] != 0.0f) {
         double r[3] = {m.x[p], m.y[p], m.z[p]};
@@ -336,6 +347,14 @@ __global__ void k_push64(Markers64 m, long n, F64View v, PushConsts c, double *W
     block_sum3(w3, Wacc);
 }
 
+template <> struct KernelDef<Push64Args> {
+    static constexpr int kBlock = TPB;
+    __device__ static void body(const Push64Args &a, VBlock b, VBlock)
+    {
+        push64(a.m, a.n, a.v, a.c, a.Wacc, a.mode, a.err, (long)b.x * TPB + threadIdx.x);
+    }
+};
+
 int launch_push64(hipStream_t s, const Markers64 &m, long n, const Fields64 &f, const Grid &g,
                   const PushConsts &c, double *Wacc, int mode, unsigned *err)
 {
@@ -347,22 +366,31 @@ int launch_push64(hipStream_t s, const Markers64 &m, long n, const Fields64 &f, 
         v.J[a] = f.J[a];
     }
     v.g = g;
-    k_push64<<<(unsigned)((n + TPB - 1) / TPB), TPB, 0, s>>>(m, n, v, c, Wacc, mode, err);
-    MPIC_CHECK(hipGetLastError());
-    return MPIC_OK;
+    Push64Args a = {m, n, v, c, Wacc, mode, err};
+    return launch_def(s, a, (unsigned)((n + TPB - 1) / TPB));
 }
 
 /* Tiled push: persistent workgroups pull 4x4x4-cell tiles (markers sorted
  * by tile-major cell key) from per-XCD queues, accumulate the tile's J in
  * LDS and flush it to HBM once; every marker leaves with its new cell key
  * and the per-cell counts of the new positions for the next sort. */
-__global__ void __launch_bounds__(TPB) k_push64_tiled(Markers64 m, F64View v, PushConsts c,
-                                                      TileView tv, double *Wacc, unsigned *err)
+struct Push64TArgs {
+    Markers64  m;
+    F64View    v;
+    PushConsts c;
+    TileView   tv;
+    double    *Wacc;
+    unsigned  *err;
+};
+
+__device__ static inline void push64_tiled(const Markers64 &m, const F64View &v,
+                                           const PushConsts &c, const TileView &tv, double *Wacc,
+                                           unsigned *err, const VBlock &bk)
 {
     __shared__ double lj[TJN * TJN * TJN * 3];
     __shared__ int s_tile;
     const Grid &g = v.g;
-    const int xcd = blockIdx.x % MPIC_XCDS;
+    const int xcd = bk.x % MPIC_XCDS;
     double w3[3] = {0, 0, 0};
     GlobalSink gs;
     for (int a = 0; a < 3; ++a) gs.J[a] = v.J[a];
@@ -424,11 +452,21 @@ __global__ void __launch_bounds__(TPB) k_push64_tiled(Markers64 m, F64View v, Pu
     block_sum3(w3, Wacc);
 }
 
+template <> struct KernelDef<Push64TArgs> {
+    static constexpr int kBlock = TPB;
+    __device__ static void body(const Push64TArgs &a, VBlock b, VBlock)
+    {
+        push64_tiled(a.m, a.v, a.c, a.tv, a.Wacc, a.err, b);
+    }
+};
+
 /* Workgroups of the persistent push that fit on one CU at once. */
 int push64_blocks_per_cu(void)
 {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_push64_tiled, TPB, 0) != hipSuccess || nb < 1)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_one<Push64TArgs>, TPB, 0) !=
+            hipSuccess ||
+        nb < 1)
         nb = 1;
     return nb;
 }
@@ -444,9 +482,8 @@ int launch_push64_tiled(hipStream_t s, const Markers64 &m, const Fields64 &f, co
         v.J[a] = f.J[a];
     }
     v.g = g;
-    k_push64_tiled<<<nblocks, TPB, 0, s>>>(m, v, c, tv, Wacc, err);
-    MPIC_CHECK(hipGetLastError());
-    return MPIC_OK;
+    Push64TArgs a = {m, v, c, tv, Wacc, err};
+    return launch_def(s, a, (unsigned)nblocks);
 }
 
 /* CIC charge density of the live markers, source/core/plasma_rho.c:160-173. */

@@ -10,10 +10,14 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <mutex>
 #include <vector>
 
 #include "mpic_internal.h"
+#include "mpic_launch.h"
 #include "mpic_tiles.h"
+
+thread_local Recorder *g_recorder = nullptr;
 
 static thread_local char g_err[512] = "";
 
@@ -51,6 +55,8 @@ extern "C" int mpic_device_count(void)
 
 /* Steps covered by one fill of the laser tables. */
 #define LASER_CAP 1024
+/* Steps the diagnostics ring holds. */
+#define DIAG_RING 1024
 
 struct Laser {
     double  A, T0, delay, dt, phi0;
@@ -153,7 +159,25 @@ struct mpic_run {
     int          *dprobe;     /* probe nodes, 3 ints each */
     double       *dsample;    /* probe samples, 6 doubles each */
     int           nprobes;
+    /* diagnostics of the loop body (em_energy, plasma_temperature), one row
+     * per step in a device ring indexed by the device step counter */
+    double       *dpart;      /* energy partial sums, 2 per workgroup */
+    int           nparts;
+    double       *dring;      /* DIAG_RING rows {W_E sum, W_M sum, WTx, WTy, WTz} */
+    std::vector<long>   ring_step;  /* step whose row a slot holds (-1: none) */
+    std::vector<double> ring_time;  /* Time at the start of that step */
+    int           capturing;  /* step_body is being captured, not run */
+    double        vsp;        /* VSP convolution weight alpha (1: off) */
+    long          topo;       /* bumped whenever the launch sequence of a step changes */
 };
+
+/* The launch sequence of the run's step changed: captured graphs and fused
+ * ensembles holding it are stale. */
+static void invalidate(mpic_run *r)
+{
+    r->graphs_ok = 0;
+    ++r->topo;
+}
 
 /* ---------------- boundary-cap geometry (single rank) ---------------- */
 /* Tables of source/core/em_caps.c:68-113 indexed [bc][top][lo/hi]. */
@@ -494,6 +518,11 @@ extern "C" int mpic_create(const mpic_config_t *cfg, mpic_run_t **out)
     r->timed = -1;
     MPIC_CHECK(hipMalloc(&r->dstep, sizeof(int)));
     MPIC_CHECK(hipMemsetAsync(r->dstep, 0, sizeof(int), r->stream));
+    r->nparts = energy_parts(g);
+    MPIC_CHECK(hipMalloc(&r->dpart, sizeof(double) * 2 * r->nparts));
+    MPIC_CHECK(hipMalloc(&r->dring, sizeof(double) * 5 * DIAG_RING));
+    r->ring_step.assign(DIAG_RING, -1);
+    r->ring_time.assign(DIAG_RING, 0.0);
     cap_geometry(cfg->n, cfg->bc, kUpdH, kFlH, r->cH);
     cap_geometry(cfg->n, cfg->bc, kUpdE, kFlE, r->cE);
     r->updH = make_set(r->cH.upd_s);
@@ -516,14 +545,18 @@ extern "C" int mpic_create(const mpic_config_t *cfg, mpic_run_t **out)
     pc.tau = cfg->tau;
     r->time = cfg->time;
     r->stage = 0;
+    r->vsp = 1.0;
     MPIC_CHECK(hipStreamSynchronize(r->stream));
     *out = r;
     return MPIC_OK;
 }
 
+static void ensemble_forget(mpic_run *r);
+
 extern "C" int mpic_destroy(mpic_run_t *r)
 {
     if (!r) return MPIC_OK;
+    ensemble_forget(r);
     hipSetDevice(r->dev);
     if (r->stream) hipStreamSynchronize(r->stream);
     hipFree(r->fblock);
@@ -552,6 +585,8 @@ extern "C" int mpic_destroy(mpic_run_t *r)
         hipHostFree(l.htab);
     }
     hipFree(r->dstep);
+    hipFree(r->dpart);
+    hipFree(r->dring);
     hipFree(r->dnode);
     hipFree(r->dprobe);
     hipFree(r->dsample);
@@ -877,7 +912,7 @@ static int load_records(mpic_run *r, const void *rec, long n, int recsize, long 
         mpic_set_error("%s: bad arguments", who);
         return MPIC_ERR_ARG;
     }
-    r->graphs_ok = 0;
+    invalidate(r);
     if (n > r->cap - at) {
         mpic_set_error("%s: %ld markers exceed the capacity %ld", who, at + n, r->cap);
         return MPIC_ERR_CAPACITY;
@@ -1018,7 +1053,7 @@ extern "C" int mpic_add_laser(mpic_run_t *r, double A, double T0, double delay, 
     MPIC_CHECK(hipHostMalloc(&l.htab, sizeof(double) * 6 * LASER_CAP, hipHostMallocDefault));
     r->lasers.push_back(l);
     r->tab_hi = r->tab_lo = 0;      /* refill before the next step */
-    r->graphs_ok = 0;
+    invalidate(r);
     return MPIC_OK;
 }
 
@@ -1066,7 +1101,7 @@ extern "C" int mpic_add_gauss_spot(mpic_run_t *r, const double *p)
     /* outside the E mesh [-1, n+1] the reference skips the spot */
     sp.on = sp.i >= -1 && sp.i <= g.n[0] + 1;
     r->tab_hi = r->tab_lo = 0;
-    r->graphs_ok = 0;
+    invalidate(r);
     return MPIC_OK;
 }
 
@@ -1168,7 +1203,7 @@ extern "C" int mpic_add_tfsf(mpic_run_t *r, const int reg[6], int mode, int nfra
     tf.step0 = r->step;
     tf.on = 1;
     r->tab_hi = r->tab_lo = 0;
-    r->graphs_ok = 0;
+    invalidate(r);
     return MPIC_OK;
 }
 
@@ -1208,7 +1243,7 @@ extern "C" int mpic_tfsf_record(mpic_run_t *r, const int reg[6], int max_frames)
     tr.cap = max_frames;
     tr.step0 = r->step;
     tr.on = 1;
-    r->graphs_ok = 0;
+    invalidate(r);
     return MPIC_OK;
 }
 
@@ -1622,6 +1657,25 @@ extern "C" int mpic_j_finish(mpic_run_t *r)
     MPIC_CHECK(hipSetDevice(r->dev));
     const int *bc = r->cfg.bc;
     for (int ea = 0; ea < 3; ++ea) TRY(launch_jfold(r->stream, r->jp, r->g, ea, bc[ea], bc[3 + ea]));
+    /* VSP_vectorSweep after the boundary fold (source/core/plasma_currentWalls.c:204) */
+    if (fabs(r->vsp - 1.0) >= 1e-10) TRY(launch_vsp(r->stream, r->jp.J, 3, r->g, r->vsp));
+    return MPIC_OK;
+}
+
+/* VSP_configure (source/core/plasma_VSP.c:18-27). */
+extern "C" int mpic_set_vsp(mpic_run_t *r, double alpha)
+{
+    if (!r) return MPIC_ERR_ARG;
+    if (!(alpha > -0.1 && alpha <= 1.1)) {
+        mpic_set_error("mpic_set_vsp: weight %g outside (-0.1, 1.1]", alpha);
+        return MPIC_ERR_ARG;
+    }
+    if (r->stage != 0) {
+        mpic_set_error("mpic_set_vsp: set the weight between steps");
+        return MPIC_ERR_ORDER;
+    }
+    r->vsp = alpha;
+    invalidate(r);
     return MPIC_OK;
 }
 
@@ -1665,16 +1719,31 @@ extern "C" int mpic_em_e_finish(mpic_run_t *r)
     return MPIC_OK;
 }
 
+/* The ring slot of the step about to run holds that step's row. */
+static void mark_ring(mpic_run *r)
+{
+    const long slot = r->step % DIAG_RING;
+    r->ring_step[slot] = r->step;
+    r->ring_time[slot] = r->time;
+}
+
 /* One step; with idx >= 0 its 5 events (start, push kernel start/end,
  * deposit end, step end) are recorded into the pool. */
 static int step_body(mpic_run *r)
 {
     if (r->timed >= 0) TRY(record(r, 0));
+    if (!r->capturing) mark_ring(r);
     TRY(mpic_em_h_half(r));
+    /* em_energy after the H half step, plasma_temperature after the push
+     * (source/core/main.c:371,383): one ring row per step, nothing copied
+     * back, so the step stays asynchronous */
+    FCALL(launch_energy_part, r->g, r->dpart);
     TRY(mpic_push_deposit(r));
     TRY(mpic_em_h_step(r));
     TRY(mpic_em_e_start(r));
     TRY(mpic_j_finish(r));
+    TRY(launch_diag_ring(r->stream, r->dpart, r->nparts, r->dacc + 2, r->dring, r->dstep,
+                         DIAG_RING));
     TRY(mpic_em_e_finish(r));
     if (r->timed >= 0) TRY(record(r, 4));
     return MPIC_OK;
@@ -1725,6 +1794,7 @@ static int advance_timed(mpic_run *r, int nsteps)
 /* Host bookkeeping of a step that a replayed graph performs on the device. */
 static void host_step(mpic_run *r)
 {
+    mark_ring(r);
     if (r->tiled) swap_markers(r);
     r->time = r->time + r->g.tau;
     ++r->step;
@@ -1761,7 +1831,9 @@ static int capture(mpic_run *r)
             rc = MPIC_ERR_DEVICE;
             break;
         }
+        r->capturing = 1;
         rc = step_async(r, -1);
+        r->capturing = 0;
         const hipError_t e = hipStreamEndCapture(r->stream, &gr);
         if (rc == MPIC_OK && e != hipSuccess) {
             mpic_set_error("hipStreamEndCapture: %s", hipGetErrorString(e));
@@ -1803,18 +1875,312 @@ extern "C" int mpic_set_graphs(mpic_run_t *r, int on)
     return MPIC_OK;
 }
 
+static int batch_streams(mpic_run *const *runs, int nruns, long chunk);
+
 extern "C" int mpic_advance(mpic_run_t *r, int nsteps)
 {
     if (!r || nsteps < 0) return MPIC_ERR_ARG;
     MPIC_CHECK(hipSetDevice(r->dev));
     if (!r->use_graphs) return advance_timed(r, nsteps);
-    mpic_run_t *one[1] = {r};
-    return mpic_batch_advance(one, 1, nsteps);
+    if (!r->inited || r->stage != 0) {
+        mpic_set_error("stage order violated: run is at stage %d, a replayed step starts at "
+                       "stage 0", r->stage);
+        return r->inited ? MPIC_ERR_ORDER : MPIC_ERR_STATE;
+    }
+    /* the run's own captured step, replayed on its stream */
+    r->ev_steps = 0;
+    mpic_run *one[1] = {r};
+    for (long left = nsteps; left > 0;) {
+        const long chunk = left < LASER_CAP - 2 ? left : LASER_CAP - 2;
+        TRY(laser_tables(r, chunk + 2));
+        TRY(batch_streams(one, 1, chunk));
+        left -= chunk;
+    }
+    return MPIC_OK;
+}
+
+/* ---------------- fused ensembles ---------------- */
+/* One launch of the fused step: the runs' launches of one kind as one
+ * k_multi launch (or the run's own launch when only one run has it). */
+struct FusedLaunch {
+    const LaunchRec *rec;          /* the first run's record (kind, launchers) */
+    int              n;            /* runs in the launch */
+    unsigned         blocks;       /* total blocks */
+    size_t           args_off, tab_off;   /* in the device block */
+};
+
+/* The fused step of a set of runs: per marker-buffer phase (the runs
+ * double-buffer their markers, so the step's arguments alternate) the
+ * launches in order and a captured graph of them. */
+struct Ensemble {
+    std::vector<mpic_run *> runs;
+    std::vector<long>       topo;      /* runs' topo counters when built */
+    std::vector<int>        rel;       /* parity of run q relative to run 0 */
+    int                     dev;
+    int                     parity0;   /* run 0's parity at phase 0 */
+    hipStream_t             stream;
+    void                   *dmem;
+    hipGraphExec_t          exec[2];
+    int                     nlaunch[2];
+    std::vector<hipEvent_t> ev;        /* one per run, plus the ensemble's */
+};
+
+static Ensemble *g_ens = nullptr;        /* the last fused ensemble */
+static std::mutex g_ens_mu;
+
+static void ensemble_free(Ensemble *e)
+{
+    if (!e) return;
+    hipSetDevice(e->dev);
+    if (e->stream) hipStreamSynchronize(e->stream);
+    for (int k = 0; k < 2; ++k)
+        if (e->exec[k]) hipGraphExecDestroy(e->exec[k]);
+    hipFree(e->dmem);
+    for (hipEvent_t v : e->ev) hipEventDestroy(v);
+    if (e->stream) hipStreamDestroy(e->stream);
+    delete e;
+}
+
+/* A run is destroyed: a fused ensemble holding it goes with it. */
+static void ensemble_forget(mpic_run *r)
+{
+    std::lock_guard<std::mutex> lk(g_ens_mu);
+    if (!g_ens) return;
+    for (mpic_run *q : g_ens->runs)
+        if (q == r) {
+            ensemble_free(g_ens);
+            g_ens = nullptr;
+            return;
+        }
+}
+
+static bool ensemble_matches(const Ensemble *e, mpic_run *const *runs, int nruns)
+{
+    if (!e || (int)e->runs.size() != nruns) return false;
+    for (int q = 0; q < nruns; ++q)
+        if (e->runs[q] != runs[q] || e->topo[q] != runs[q]->topo ||
+            e->rel[q] != (runs[q]->parity ^ runs[0]->parity))
+            return false;
+    return true;
+}
+
+/* Records one step of run r at each marker-buffer phase (k = 0: its current
+ * parity) as launch data; the host state is restored afterwards. */
+static int record_steps(mpic_run *r, Recorder rec[2])
+{
+    const double time = r->time;
+    const long step = r->step;
+    const int cur = r->cur, parity = r->parity;
+    const Markers32 m32 = r->m32;
+    const Markers64 m = r->m;
+    int rc = MPIC_OK;
+    for (int k = 0; k < 2 && rc == MPIC_OK; ++k) {
+        g_recorder = &rec[k];
+        r->capturing = 1;
+        rc = step_async(r, -1);
+        r->capturing = 0;
+        g_recorder = nullptr;
+    }
+    r->time = time;
+    r->step = step;
+    r->cur = cur;
+    r->parity = parity;
+    r->m32 = m32;
+    r->m = m;
+    r->stage = 0;
+    return rc;
+}
+
+/* Orders the runs' launches into fused launches: repeatedly the kind that
+ * is next for the most runs is launched for all of them, so every run's
+ * launches keep their order (runs are independent of each other). */
+static void fuse(const std::vector<Recorder> &recs, std::vector<std::vector<const LaunchRec *>> &out)
+{
+    const int n = (int)recs.size();
+    std::vector<size_t> at(n, 0);
+    for (;;) {
+        const void *best = nullptr;
+        int bestc = 0;
+        for (int q = 0; q < n; ++q) {
+            if (at[q] >= recs[q].recs.size()) continue;
+            const void *k = recs[q].recs[at[q]].kind;
+            int c = 0;
+            for (int u = 0; u < n; ++u)
+                if (at[u] < recs[u].recs.size() && recs[u].recs[at[u]].kind == k) ++c;
+            if (c > bestc) {
+                bestc = c;
+                best = k;
+            }
+        }
+        if (!best) return;
+        std::vector<const LaunchRec *> grp;
+        for (int q = 0; q < n; ++q)
+            if (at[q] < recs[q].recs.size() && recs[q].recs[at[q]].kind == best)
+                grp.push_back(&recs[q].recs[at[q]++]);
+        out.push_back(grp);
+    }
+}
+
+static int ensemble_build(mpic_run *const *runs, int nruns, Ensemble **out)
+{
+    Ensemble *e = new Ensemble();
+    e->dev = runs[0]->dev;
+    e->parity0 = runs[0]->parity;
+    for (int q = 0; q < nruns; ++q) {
+        e->runs.push_back(runs[q]);
+        e->topo.push_back(runs[q]->topo);
+        e->rel.push_back(runs[q]->parity ^ runs[0]->parity);
+    }
+    int rc = MPIC_OK;
+    std::vector<Recorder> recs[2];
+    recs[0].resize(nruns);
+    recs[1].resize(nruns);
+    for (int q = 0; q < nruns && rc == MPIC_OK; ++q) {
+        Recorder two[2];
+        rc = record_steps(runs[q], two);
+        recs[0][q] = std::move(two[0]);
+        recs[1][q] = std::move(two[1]);
+    }
+    std::vector<std::vector<const LaunchRec *>> seq[2];
+    std::vector<FusedLaunch> fl[2];
+    size_t bytes = 0;
+    for (int k = 0; k < 2 && rc == MPIC_OK; ++k) {
+        fuse(recs[k], seq[k]);
+        for (auto &grp : seq[k]) {
+            FusedLaunch f;
+            f.rec = grp[0];
+            f.n = (int)grp.size();
+            f.blocks = 0;
+            for (const LaunchRec *x : grp) f.blocks += x->grid[0] * x->grid[1] * x->grid[2];
+            bytes = (bytes + 255) & ~(size_t)255;
+            f.args_off = bytes;
+            bytes += grp[0]->args.size() * grp.size();
+            bytes = (bytes + 255) & ~(size_t)255;
+            f.tab_off = bytes;
+            bytes += sizeof(unsigned) * (4 * grp.size() + 1);
+            fl[k].push_back(f);
+        }
+        e->nlaunch[k] = (int)fl[k].size();
+    }
+    std::vector<unsigned char> host(bytes ? bytes : 1);
+    for (int k = 0; k < 2 && rc == MPIC_OK; ++k)
+        for (size_t u = 0; u < fl[k].size(); ++u) {
+            const auto &grp = seq[k][u];
+            const FusedLaunch &f = fl[k][u];
+            const size_t as = grp[0]->args.size();
+            unsigned *tab = (unsigned *)(host.data() + f.tab_off);
+            unsigned acc = 0;
+            for (size_t q = 0; q < grp.size(); ++q) {
+                memcpy(host.data() + f.args_off + q * as, grp[q]->args.data(), as);
+                tab[q] = acc;
+                for (int d = 0; d < 3; ++d) tab[grp.size() + 1 + 3 * q + d] = grp[q]->grid[d];
+                acc += grp[q]->grid[0] * grp[q]->grid[1] * grp[q]->grid[2];
+            }
+            tab[grp.size()] = acc;
+        }
+    if (rc == MPIC_OK && hipSetDevice(e->dev) != hipSuccess) rc = MPIC_ERR_DEVICE;
+    if (rc == MPIC_OK && (hipMalloc(&e->dmem, host.size()) != hipSuccess ||
+                          hipMemcpy(e->dmem, host.data(), host.size(), hipMemcpyHostToDevice) !=
+                              hipSuccess ||
+                          hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess)) {
+        mpic_set_error("fused ensemble: device set-up failed");
+        rc = MPIC_ERR_DEVICE;
+    }
+    for (int q = 0; q <= nruns && rc == MPIC_OK; ++q) {
+        hipEvent_t v;
+        if (hipEventCreateWithFlags(&v, hipEventDisableTiming) != hipSuccess) {
+            mpic_set_error("fused ensemble: event creation failed");
+            rc = MPIC_ERR_DEVICE;
+            break;
+        }
+        e->ev.push_back(v);
+    }
+    /* one graph per phase: the fused launches in order */
+    for (int k = 0; k < 2 && rc == MPIC_OK; ++k) {
+        hipGraph_t gr = nullptr;
+        if (hipStreamBeginCapture(e->stream, hipStreamCaptureModeRelaxed) != hipSuccess) {
+            mpic_set_error("fused ensemble: hipStreamBeginCapture failed");
+            rc = MPIC_ERR_DEVICE;
+            break;
+        }
+        for (size_t u = 0; u < fl[k].size() && rc == MPIC_OK; ++u) {
+            const FusedLaunch &f = fl[k][u];
+            const unsigned char *base = (const unsigned char *)e->dmem;
+            if (f.n == 1)
+                rc = f.rec->one(e->stream, f.rec->args.data(), f.rec->grid);
+            else
+                rc = f.rec->multi(e->stream, base + f.args_off, (const unsigned *)(base + f.tab_off),
+                                  f.n, f.blocks);
+        }
+        const hipError_t ce = hipStreamEndCapture(e->stream, &gr);
+        if (rc == MPIC_OK && ce != hipSuccess) {
+            mpic_set_error("fused ensemble: hipStreamEndCapture: %s", hipGetErrorString(ce));
+            rc = MPIC_ERR_DEVICE;
+        }
+        if (rc == MPIC_OK && hipGraphInstantiate(&e->exec[k], gr, nullptr, nullptr, 0) != hipSuccess) {
+            mpic_set_error("fused ensemble: hipGraphInstantiate failed");
+            rc = MPIC_ERR_DEVICE;
+        }
+        if (gr) hipGraphDestroy(gr);
+    }
+    if (rc != MPIC_OK) {
+        ensemble_free(e);
+        return rc;
+    }
+    *out = e;
+    return MPIC_OK;
+}
+
+/* The runs' steps as replays of per-run graphs, each run on its own stream
+ * (MPIC_BATCH_STREAMS=1, and runs on several devices). */
+static int batch_streams(mpic_run *const *runs, int nruns, long chunk)
+{
+    for (int q = 0; q < nruns; ++q)
+        if (!runs[q]->graphs_ok) TRY(capture(runs[q]));
+    int dev = -1;
+    for (long s = 0; s < chunk; ++s)
+        for (int q = 0; q < nruns; ++q) {
+            mpic_run *r = runs[q];
+            if (r->dev != dev) {
+                MPIC_CHECK(hipSetDevice(r->dev));
+                dev = r->dev;
+            }
+            TRY(replay(r));
+        }
+    return MPIC_OK;
+}
+
+/* The runs' steps as replays of the fused ensemble graph on the ensemble's
+ * stream, ordered after the work already on the runs' streams and before
+ * whatever they get next. */
+static int batch_fused(mpic_run *const *runs, int nruns, long chunk)
+{
+    std::lock_guard<std::mutex> lk(g_ens_mu);
+    if (!ensemble_matches(g_ens, runs, nruns)) {
+        ensemble_free(g_ens);
+        g_ens = nullptr;
+        TRY(ensemble_build(runs, nruns, &g_ens));
+    }
+    Ensemble *e = g_ens;
+    MPIC_CHECK(hipSetDevice(e->dev));
+    for (int q = 0; q < nruns; ++q) {
+        MPIC_CHECK(hipEventRecord(e->ev[q], runs[q]->stream));
+        MPIC_CHECK(hipStreamWaitEvent(e->stream, e->ev[q], 0));
+    }
+    for (long s = 0; s < chunk; ++s) {
+        const int k = runs[0]->parity == e->parity0 ? 0 : 1;
+        MPIC_CHECK(hipGraphLaunch(e->exec[k], e->stream));
+        for (int q = 0; q < nruns; ++q) host_step(runs[q]);
+    }
+    MPIC_CHECK(hipEventRecord(e->ev[nruns], e->stream));
+    for (int q = 0; q < nruns; ++q) MPIC_CHECK(hipStreamWaitEvent(runs[q]->stream, e->ev[nruns], 0));
+    return MPIC_OK;
 }
 
 extern "C" int mpic_batch_advance(mpic_run_t *const *runs, int nruns, int nsteps)
 {
     if (!runs || nruns < 0 || nsteps < 0) return MPIC_ERR_ARG;
+    bool one_dev = true;
     for (int q = 0; q < nruns; ++q) {
         mpic_run *r = runs[q];
         if (!r || !r->inited || !r->tiled) {
@@ -1828,30 +2194,24 @@ extern "C" int mpic_batch_advance(mpic_run_t *const *runs, int nruns, int nsteps
                            "starts at stage 0", q, r->stage);
             return MPIC_ERR_ORDER;
         }
+        for (int u = 0; u < q; ++u)
+            if (runs[u] == r) {
+                mpic_set_error("mpic_batch_advance: run %d is listed twice", q);
+                return MPIC_ERR_ARG;
+            }
+        one_dev = one_dev && r->dev == runs[0]->dev;
     }
-    for (int q = 0; q < nruns; ++q) {
-        mpic_run *r = runs[q];
-        r->ev_steps = 0;
-    }
+    if (nruns == 0 || nsteps == 0) return MPIC_OK;
+    const char *env = getenv("MPIC_BATCH_STREAMS");
+    const bool streams = !one_dev || (env && env[0] == '1');
+    for (int q = 0; q < nruns; ++q) runs[q]->ev_steps = 0;
     for (long left = nsteps; left > 0;) {
         const long chunk = left < LASER_CAP - 2 ? left : LASER_CAP - 2;
         for (int q = 0; q < nruns; ++q) {
-            mpic_run *r = runs[q];
-            MPIC_CHECK(hipSetDevice(r->dev));
-            TRY(laser_tables(r, chunk + 2));
-            if (!r->graphs_ok) TRY(capture(r));
+            MPIC_CHECK(hipSetDevice(runs[q]->dev));
+            TRY(laser_tables(runs[q], chunk + 2));
         }
-        /* step by step across the runs, each on its own stream */
-        int dev = -1;
-        for (long s = 0; s < chunk; ++s)
-            for (int q = 0; q < nruns; ++q) {
-                mpic_run *r = runs[q];
-                if (r->dev != dev) {
-                    MPIC_CHECK(hipSetDevice(r->dev));
-                    dev = r->dev;
-                }
-                TRY(replay(r));
-            }
+        TRY(streams ? batch_streams(runs, nruns, chunk) : batch_fused(runs, nruns, chunk));
         left -= chunk;
     }
     return MPIC_OK;
@@ -1881,7 +2241,7 @@ extern "C" int mpic_autotune(mpic_run_t *r, int steps, int *chosen)
         if (nb == r->items_cap) break;
     }
     r->tile_blocks = best;
-    r->graphs_ok = 0;
+    invalidate(r);
     r->ev_steps = 0;
     if (chosen) *chosen = best;
     return MPIC_OK;
@@ -1934,6 +2294,39 @@ extern "C" int mpic_last_timing(mpic_run_t *r, double *push_ms, double *deposit_
     return MPIC_OK;
 }
 
+/* wDensity_addPoint's rows (source/core/diag_WDensity.c:136) of the steps
+ * the loop body ran (mpic_step without diag, mpic_advance, replays): the
+ * last min(max_rows, DIAG_RING) such steps, oldest first, each {Time, W_E,
+ * W_M, WTx, WTy, WTz}. */
+extern "C" int mpic_diag_history(mpic_run_t *r, double *rows, int max_rows, int *nrows)
+{
+    if (!r || !nrows || max_rows < 0 || (max_rows && !rows)) return MPIC_ERR_ARG;
+    MPIC_CHECK(hipSetDevice(r->dev));
+    std::vector<double> ring(5 * DIAG_RING);
+    MPIC_CHECK(hipStreamSynchronize(r->stream));
+    MPIC_CHECK(hipMemcpy(ring.data(), r->dring, sizeof(double) * ring.size(),
+                         hipMemcpyDeviceToHost));
+    const double vol = 0.5 * r->g.h[0] * r->g.h[1] * r->g.h[2];
+    std::vector<long> steps;
+    for (long s = r->step - 1; s >= 0 && s >= r->step - DIAG_RING && (int)steps.size() < max_rows;
+         --s)
+        if (r->ring_step[s % DIAG_RING] == s) steps.push_back(s);
+    const int m = (int)steps.size();
+    for (int q = 0; q < m; ++q) {
+        const long s = steps[m - 1 - q], slot = s % DIAG_RING;
+        double *o = rows + 6 * q;
+        const double *v = ring.data() + 5 * slot;
+        o[0] = r->ring_time[slot];
+        o[1] = vol * v[0];
+        o[2] = vol * v[1];
+        o[3] = v[2];
+        o[4] = v[3];
+        o[5] = v[4];
+    }
+    *nrows = m;
+    return MPIC_OK;
+}
+
 /* ---------------- rho, Gauss residuals, cells ---------------- */
 static int rho_into(mpic_run *r, double *rho)
 {
@@ -1947,6 +2340,11 @@ static int rho_into(mpic_run *r, double *rho)
     const int *bc = r->cfg.bc;
     const int BCv[6] = {bc[0], bc[3], bc[1], bc[4], bc[2], bc[5]};
     for (int b = 0; b < 6; ++b) TRY(launch_rho_fold(r->stream, rho, r->g, b, BCv[b]));
+    /* VSP_scalarSweep after the boundary fold (source/core/plasma_rho.c:153) */
+    if (fabs(r->vsp - 1.0) >= 1e-10) {
+        double *F[3] = {rho, nullptr, nullptr};
+        TRY(launch_vsp(r->stream, F, 1, r->g, r->vsp));
+    }
     return MPIC_OK;
 }
 
@@ -2057,3 +2455,4 @@ extern "C" int mpic_sort(mpic_run_t *r, int32_t *counts)
     if (!r) return MPIC_ERR_ARG;
     return sort_counts(r, counts, nullptr);
 }
+

@@ -12,6 +12,7 @@
  * and the marker and writes the marker: FP64 plain form 4 + 60 + 60 B per
  * marker.
  */
+#include "mpic_launch.h"
 #include "mpic_tiles.h"
 
 #define SB MPIC_SCAN_BLOCK       /* tiles per scan block */
@@ -45,10 +46,16 @@ int launch_keys64(hipStream_t s, const Markers64 &m, long n, const Grid &g, cons
 /* Block-local exclusive scans of (markers per tile, tile non-empty, work
  * items of the tile), three ints per tile and per block. */
 #define NSCAN 3
-__global__ void __launch_bounds__(SB) k_scan_a(TileView tv, int *blocksum, int *tileloc)
+struct ScanAArgs {
+    TileView tv;
+    int     *blocksum, *tileloc;
+};
+
+__device__ static inline void scan_a(const TileView &tv, int *blocksum, int *tileloc,
+                                     const VBlock &bk)
 {
     __shared__ int wsum[NSCAN][SB / 64];
-    const int t = blockIdx.x * SB + threadIdx.x, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int t = bk.x * SB + threadIdx.x, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     int c = 0;
     if (t < tv.ntiles) {
         const int4 *q = (const int4 *)(tv.count + (long)t * TILE_CELLS);
@@ -74,11 +81,24 @@ __global__ void __launch_bounds__(SB) k_scan_a(TileView tv, int *blocksum, int *
     if (t < tv.ntiles)
         for (int a = 0; a < NSCAN; ++a) tileloc[NSCAN * t + a] = inc[a] - own[a];
     if (threadIdx.x == SB - 1)
-        for (int a = 0; a < NSCAN; ++a) blocksum[NSCAN * blockIdx.x + a] = inc[a];
+        for (int a = 0; a < NSCAN; ++a) blocksum[NSCAN * bk.x + a] = inc[a];
 }
 
+template <> struct KernelDef<ScanAArgs> {
+    static constexpr int kBlock = SB;
+    __device__ static void body(const ScanAArgs &a, VBlock b, VBlock)
+    {
+        scan_a(a.tv, a.blocksum, a.tileloc, b);
+    }
+};
+
 /* Exclusive scan of the block sums (one block, loops for any count). */
-__global__ void k_scan_b(int *blocksum, int nb)
+struct ScanBArgs {
+    int *blocksum;
+    int  nb;
+};
+
+__device__ static inline void scan_b(int *blocksum, int nb)
 {
     __shared__ int sv[NSCAN][SB];
     __shared__ int carry[NSCAN];
@@ -111,6 +131,11 @@ __global__ void k_scan_b(int *blocksum, int nb)
     if (threadIdx.x < NSCAN) blocksum[NSCAN * nb + threadIdx.x] = carry[threadIdx.x];
 }
 
+template <> struct KernelDef<ScanBArgs> {
+    static constexpr int kBlock = SB;
+    __device__ static void body(const ScanBArgs &a, VBlock, VBlock) { scan_b(a.blocksum, a.nb); }
+};
+
 /* Tile offsets, the active-tile list and, for every non-empty tile, the
  * per-key offsets and cursors with its counters reset.  One wavefront per
  * SC_TPW consecutive tiles: lane l < SC_TPW keeps the books of tile l (its
@@ -119,11 +144,17 @@ __global__ void k_scan_b(int *blocksum, int nb)
  * tiles (their counters are all 0) cost two words: most tiles of a sparse
  * plasma are empty, and fewer wavefronts than tiles keep the launch short. */
 #define SC_TPW 8
-__global__ void k_scan_c(TileView tv, const int *blocksum, const int *tileloc, int nb,
-                         int parity_next)
+struct ScanCArgs {
+    TileView   tv;
+    const int *blocksum, *tileloc;
+    int        nb, parity_next;
+};
+
+__device__ static inline void scan_c(const TileView &tv, const int *blocksum, const int *tileloc,
+                                     int nb, int parity_next, const VBlock &bk)
 {
     const int lane = threadIdx.x & 63;
-    const int w = (int)(((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const int w = (int)(((long)bk.x * 256 + threadIdx.x) >> 6);
     const int t0 = w * SC_TPW;
     if (t0 >= tv.ntiles) return;                     /* whole waves exit together */
     int base = 0, tot = 0;
@@ -170,25 +201,40 @@ __global__ void k_scan_c(TileView tv, const int *blocksum, const int *tileloc, i
     }
 }
 
+template <> struct KernelDef<ScanCArgs> {
+    static constexpr int kBlock = 256;
+    __device__ static void body(const ScanCArgs &a, VBlock b, VBlock)
+    {
+        scan_c(a.tv, a.blocksum, a.tileloc, a.nb, a.parity_next, b);
+    }
+};
+
 int launch_tile_scan(hipStream_t s, const TileView &tv, int *blocksum, int parity_next)
 {
     const int nb = (tv.ntiles + SB - 1) / SB;
     int *tileloc = blocksum + NSCAN * (nb + 1);
-    k_scan_a<<<nb, SB, 0, s>>>(tv, blocksum, tileloc);
-    MPIC_CHECK(hipGetLastError());
-    k_scan_b<<<1, SB, 0, s>>>(blocksum, nb);
-    MPIC_CHECK(hipGetLastError());
+    ScanAArgs a = {tv, blocksum, tileloc};
+    int rc = launch_def(s, a, (unsigned)nb);
+    if (rc != MPIC_OK) return rc;
+    ScanBArgs b = {blocksum, nb};
+    rc = launch_def(s, b, 1);
+    if (rc != MPIC_OK) return rc;
     const long waves = (tv.ntiles + SC_TPW - 1) / SC_TPW;
-    k_scan_c<<<(unsigned)((waves * 64 + 255) / 256), 256, 0, s>>>(tv, blocksum, tileloc, nb,
-                                                                   parity_next);
-    MPIC_CHECK(hipGetLastError());
-    return MPIC_OK;
+    ScanCArgs c = {tv, blocksum, tileloc, nb, parity_next};
+    return launch_def(s, c, (unsigned)((waves * 64 + 255) / 256));
 }
 
 /* Moves each live marker of src to its slot in dst. */
-__global__ void k_scatter64(Markers64 src, Markers64 dst, long nbound, TileView tv, int parity)
+struct Scatter64Args {
+    Markers64 src, dst;
+    long      nbound;
+    TileView  tv;
+    int       parity;
+};
+
+__device__ static inline void scatter64(const Markers64 &src, const Markers64 &dst, long nbound,
+                                        const TileView &tv, int parity, long p)
 {
-    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long n = tv.nmark[parity];
     const int key = (p < n && p < nbound) ? tv.key[p] : -1;
     int start, len;
@@ -211,13 +257,20 @@ __global__ void k_scatter64(Markers64 src, Markers64 dst, long nbound, TileView 
     __builtin_nontemporal_store(__builtin_nontemporal_load(src.id + p), dst.id + d);
 }
 
+template <> struct KernelDef<Scatter64Args> {
+    static constexpr int kBlock = 256;
+    __device__ static void body(const Scatter64Args &a, VBlock b, VBlock)
+    {
+        scatter64(a.src, a.dst, a.nbound, a.tv, a.parity, (long)b.x * 256 + threadIdx.x);
+    }
+};
+
 int launch_scatter64(hipStream_t s, const Markers64 &src, const Markers64 &dst, long nbound,
                      const TileView &tv, int parity)
 {
     if (nbound <= 0) return MPIC_OK;
-    k_scatter64<<<(unsigned)((nbound + 255) / 256), 256, 0, s>>>(src, dst, nbound, tv, parity);
-    MPIC_CHECK(hipGetLastError());
-    return MPIC_OK;
+    Scatter64Args a = {src, dst, nbound, tv, parity};
+    return launch_def(s, a, (unsigned)((nbound + 255) / 256));
 }
 
 /* Per-cell marker counts in natural (i, j, k) order from the offsets. */

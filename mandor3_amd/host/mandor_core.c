@@ -237,16 +237,23 @@ static void step_stages(state_t *s, double *pt, int tec)
 int main(int argc, char **argv)
 {
     int precision = MPIC_FP64, device = 0;
+    double vsp_weight = 1.0;
     for (int a = 1; a < argc; ++a) {
         if (!strcmp(argv[a], "--fp32")) {
             precision = MPIC_FP32_SPLIT;
+        } else if (!strncmp(argv[a], "--VSP:weight(", 13)) {
+            /* core.out's option (source/import.pl) and its clamp
+             * (source/core/main.c:316-320) */
+            vsp_weight = atof(argv[a] + 13);
+            vsp_weight = (vsp_weight > 0.5 && vsp_weight < 1.00001) ? vsp_weight : 1.00;
         } else if (!strcmp(argv[a], "--device") && a + 1 < argc) {
             device = atoi(argv[++a]);
         } else if (!strcmp(argv[a], "--tfsf-defrag")) {
             HOST(mh_tfsf_defrag(".EM_sources/TFSF"), "TFSF defragmentation");
             return 0;
         } else {
-            fprintf(stderr, "usage: %s [--fp32] [--device N] | --tfsf-defrag\n", argv[0]);
+            fprintf(stderr, "usage: %s [--fp32] [--device N] [--VSP:weight(w)] | --tfsf-defrag\n",
+                    argv[0]);
             return EXIT_FAILURE;
         }
     }
@@ -332,6 +339,7 @@ int main(int argc, char **argv)
     cfg.max_markers = nm > 0 ? nm : 1;
     cfg.device = device;
     GPU(mpic_create(&cfg, &s.run), "mpic_create");
+    GPU(mpic_set_vsp(s.run, vsp_weight), "mpic_set_vsp");
     GPU(mpic_set_fields(s.run, s.E, s.H), "mpic_set_fields");
     GPU(mpic_markers_load(s.run, rec, nm, recsize), "mpic_markers_load");
     s.rec = rec;

@@ -807,6 +807,21 @@ void mh_ranmar(mh_ranmar_t *r, double *out, int len)
     }
 }
 
+/* len deviates of N(0, sigma^2) from the stream: the Box-Muller transform
+ * (G. E. P. Box and M. E. Muller, 1958) of consecutive uniform pairs
+ * (u1, u2), r = sqrt(-2 ln(1 - u1)) (1 - u1 lies in (0, 1]), angle 2 pi u2,
+ * giving r cos and r sin in that order; an odd len drops the last sine. */
+void mh_ranmar_normal(mh_ranmar_t *r, double *out, long len, double sigma)
+{
+    double u[2];
+    for (long q = 0; q < len; q += 2) {
+        mh_ranmar(r, u, 2);
+        const double rad = sigma * sqrt(-2.0 * log1p(-u[0])), a = 2.0 * M_PI * u[1];
+        out[q] = rad * cos(a);
+        if (q + 1 < len) out[q + 1] = rad * sin(a);
+    }
+}
+
 /* ---- TFSF recordings, source/core/em_TFSF.c ---------------------------- */
 
 /* Face file names by slot axis*4 + top*2 + comp (tfsfE / tfsfH,

@@ -182,6 +182,9 @@ typedef struct {
 } mh_ranmar_t;
 int  mh_ranmar_init(mh_ranmar_t *r, int ij, int kl);
 void mh_ranmar(mh_ranmar_t *r, double *out, int len);
+/* Normal deviates N(0, sigma^2) by the Box-Muller transform of the stream's
+ * consecutive uniform pairs (r cos, r sin of each pair, in that order). */
+void mh_ranmar_normal(mh_ranmar_t *r, double *out, long len, double sigma);
 
 /* Last error message of this library. */
 const char *mh_error(void);

@@ -164,6 +164,7 @@ struct opic_s {
     laser_t laser[4];
     spot_t spot;
     tfsf_t tfsf;
+    double vsp;            /* VSP convolution weight alpha (1: identity) */
     int    inited;
 };
 
@@ -187,7 +188,61 @@ opic_t *opic_create(const int n[3], const double h[3], const double L[3],
     mesh_alloc(&s->rho1, box_make(-2, -2, -2, n[0] + 2, n[1] + 2, n[2] + 2), 1);
     mesh_alloc(&s->rho2, box_make(-2, -2, -2, n[0] + 2, n[1] + 2, n[2] + 2), 1);
     mesh_zero(&s->J);
+    s->vsp = 1.0;
     return s;
+}
+
+/* VSP_configure, source/core/plasma_VSP.c:18-27. */
+int opic_set_vsp(opic_t *s, double alpha)
+{
+    if (!(alpha > -0.1 && alpha <= 1.1)) return -1;
+    s->vsp = alpha;
+    return 0;
+}
+
+/* VSP_vectorSweep / VSP_scalarSweep, source/core/plasma_VSP.c:32-176: the
+ * (beta, alpha, beta) convolution, beta = (1 - alpha)/2, along z, then y,
+ * then x, over every line of the mesh (nodes [-2, n+2]); each new value
+ * takes the old values of its neighbours (the sweep keeps the untouched
+ * previous value), and the end nodes take their one inner neighbour.  A
+ * no-op for |alpha - 1| < 1e-10. */
+static void vsp_sweep(const opic_t *s, mesh_t *m)
+{
+    if (fabs(s->vsp - 1.0) < 1e-10) return;
+    const double alpha = s->vsp, beta = 0.5 * (1.0 - alpha);
+    double un[3];
+    for (int ax = 2; ax >= 0; --ax) {
+        const int p = ax == 0 ? 1 : 0, q = ax == 2 ? 1 : 2;
+        const int lo = -2, hi = s->n[ax] + 2;
+        int r[3];
+        for (int a = m->b.lo[p]; a <= m->b.hi[p]; ++a)
+            for (int b = m->b.lo[q]; b <= m->b.hi[q]; ++b) {
+                r[p] = a;
+                r[q] = b;
+                r[ax] = lo;
+                double *f0 = ATv(m, r);
+                r[ax] = lo + 1;
+                double *f1 = ATv(m, r);
+                for (int c = 0; c < m->nc; ++c) {
+                    un[c] = f0[c];
+                    f0[c] = beta * f1[c] + alpha * f0[c];
+                }
+                for (int k = lo + 1; k <= hi - 1; ++k) {
+                    r[ax] = k;
+                    double *f = ATv(m, r);
+                    r[ax] = k + 1;
+                    double *fn = ATv(m, r);
+                    for (int c = 0; c < m->nc; ++c) {
+                        const double v = alpha * (f[c]) + beta * (un[c] + fn[c]);
+                        un[c] = f[c];
+                        f[c] = v;
+                    }
+                }
+                r[ax] = hi;
+                double *fe = ATv(m, r);
+                for (int c = 0; c < m->nc; ++c) fe[c] = beta * un[c] + alpha * fe[c];
+            }
+    }
 }
 
 void opic_destroy(opic_t *s)
@@ -1237,10 +1292,11 @@ double *Jin = jat(J, r, ea, pl + in_), *Jex = jat(J, r, ea, pl + ex_);
                 }
         }
     }
+    vsp_sweep(s, &s->J);                        /* jbc_finish, plasma_currentWalls.c:204 */
 }
 
 /* CIC charge density with boundary fold, source/core/plasma_rho.c:23-96,
- * 160-173 (VSP at the default weight 1 is the identity). */
+ * 160-173, then the VSP sweep (:153). */
 static void rho_compute(opic_t *s, mesh_t *rho)
 {
     mesh_zero(rho);
@@ -1298,6 +1354,7 @@ double *Rm1 = jat(rho, r, ea, pl - so), *Rp1 = jat(rho, r, ea, pl + so);
                 }
         }
     }
+    vsp_sweep(s, rho);                          /* source/core/plasma_rho.c:153 */
 }
 
 void opic_rho(opic_t *s, double *rho)

@@ -65,6 +65,11 @@ int     opic_add_gauss_spot(opic_t *s, const double p[11]);
 int     opic_add_tfsf(opic_t *s, const int reg[6], int mode, int nframes,
                       const double *const dataE[12], const double *const dataH[12]);
 
+/* VSP_configure (source/core/plasma_VSP.c:18-27): the (beta, alpha, beta)
+ * convolution of J in jbc_finish and of rho; alpha in (-0.1, 1.1], 1 =
+ * off (the default).  Returns -1 for other values. */
+int     opic_set_vsp(opic_t *s, double alpha);
+
 int     opic_init(opic_t *s);              /* em_init + plasma_init */
 /* One main-loop iteration, source/core/main.c:353-476.
  * diag (may be NULL) receives {Time, W_E, W_M, WTx, WTy, WTz} as passed to

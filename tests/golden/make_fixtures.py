@@ -127,6 +127,16 @@ CASES = {
                  dict(sub=(1, 1, 1, 1), rho=1.5, qm=5.4466e-4,
                       u=(0, 0, 0), du=(0.01, 0.01, 0.01))],
         laser=None),
+    # The mixed box with the VSP (beta, alpha, beta) convolution of J and rho
+    # at alpha = 0.8 (core.out's --VSP:weight option, source/core/main.c:
+    # 314-320, source/core/plasma_VSP.c).
+    "vsp": dict(
+        n=(10, 12, 9), L=(0.5, 0.6, 0.405), bc=(0, 1, 2, 0, 1, 2),
+        species=[dict(sub=(1, 2, 1, 1), rho=-1.5, qm=-1.0,
+                      u=(0.1, 0.2, -0.3), du=(1.6, 1.6, 1.6)),
+                 dict(sub=(1, 1, 1, 1), rho=1.5, qm=5.4466e-4,
+                      u=(0, 0, 0), du=(0.01, 0.01, 0.01))],
+        laser=None, args=["--VSP:weight(0.8)"], vsp=0.8),
 }
 
 
@@ -349,6 +359,8 @@ def make_case(name, cfg, workroot):
         out["spot_params"] = np.array([sp[k] for k in keys] + [sp["Ey"], sp["Ez"]], dtype=float)
     if cfg.get("tfsf") is not None:
         out.update(write_tfsf(wd, cfg, d7[:3], d7[6]))
+    if cfg.get("vsp") is not None:
+        out["vsp"] = np.array([cfg["vsp"]])
     # setup.out seeds rand() from the clock (source/misc_partition.c:255):
     # the Gauss build runs on a copy of the same record 0.
     wg = os.path.join(workroot, name + "_gauss")
@@ -359,7 +371,7 @@ def make_case(name, cfg, workroot):
         rec0 = dict(E=read_mesh(os.path.join(wd, "binData/sys_E_000000(000).bin"), n),
                     H=read_mesh(os.path.join(wd, "binData/sys_H_000000(000).bin"), n),
                     M=read_markers(os.path.join(wd, "binData/plasma_000000_000.bin")))
-        run(["mpirun", "-np", "1", os.path.join(BIN, exe)], wd, env)
+        run(["mpirun", "-np", "1", os.path.join(BIN, exe)] + list(cfg.get("args", [])), wd, env)
         if variant == "plain":
             out["h"] = d7[:3].copy()
             out["L"] = d7[3:6].copy()

@@ -52,6 +52,7 @@ def lib():
         L.opic_time.argtypes = [P]
         L.opic_get_J.argtypes = [P, P]
         L.opic_rho.argtypes = [P, P]
+        L.opic_set_vsp.argtypes = [P, D]
         _lib = L
     return _lib
 
@@ -132,6 +133,9 @@ class Oracle:
         r = np.ascontiguousarray(reg, dtype=np.int32)
         assert lib().opic_add_tfsf(self.s, _ptr(r), int(mode), int(nframes), pe, ph) == 0
 
+    def set_vsp(self, alpha):
+        assert lib().opic_set_vsp(self.s, float(alpha)) == 0
+
     def init(self):
         rc = lib().opic_init(self.s)
         assert rc == 0, rc
@@ -193,5 +197,7 @@ def oracle_from_fixture(fx):
         o.add_gauss_spot(fx["spot_params"])
     if "tfsf_reg" in fx:
         o.add_tfsf(*tfsf_arrays(fx))
+    if "vsp" in fx:
+        o.set_vsp(float(fx["vsp"][0]))
     o.init()
     return o

@@ -19,13 +19,13 @@ from oracle_lib import MARKER_DTYPE, Oracle, load_fixture, oracle_from_fixture, 
 
 pytestmark = pytest.mark.gpu
 
-CASES = ["periodic", "laser_mur", "mixed", "spot", "tfsf", "tfsf_back"]
+CASES = ["periodic", "laser_mur", "mixed", "spot", "tfsf", "tfsf_back", "vsp"]
 
 # about 10x the measured 20-step errors of each fixture (printed by the tests)
 FIELD_BAR = {"periodic": 1e-5, "laser_mur": 1e-5, "mixed": 1e-5, "spot": 1e-5, "tfsf": 1e-5,
-             "tfsf_back": 1e-5}
+             "tfsf_back": 1e-5, "vsp": 1e-5}
 ENERGY_BAR = {"periodic": 5e-6, "laser_mur": 5e-6, "mixed": 5e-6, "spot": 5e-6, "tfsf": 5e-6,
-              "tfsf_back": 5e-6}
+              "tfsf_back": 5e-6, "vsp": 5e-6}
 
 
 def run32(fx, markers=True, E=None, H=None):
@@ -40,6 +40,8 @@ def run32(fx, markers=True, E=None, H=None):
         r.add_gauss_spot(fx["spot_params"])
     if "tfsf_reg" in fx:
         r.add_tfsf(*tfsf_arrays(fx))
+    if "vsp" in fx:
+        r.set_vsp(float(fx["vsp"][0]))
     r.init()
     return r
 
@@ -76,6 +78,8 @@ def test_fp32_vacuum_fields(case):
         o.add_gauss_spot(fx["spot_params"])
     if "tfsf_reg" in fx:
         o.add_tfsf(*tfsf_arrays(fx))
+    if "vsp" in fx:
+        o.set_vsp(float(fx["vsp"][0]))
     o.init()
     worst = 0.0
     for _ in range(12):

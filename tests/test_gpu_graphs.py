@@ -30,6 +30,8 @@ def make(fx, precision, markers=True, E=None, H=None, graphs=False):
         r.add_gauss_spot(fx["spot_params"])
     if "tfsf_reg" in fx:
         r.add_tfsf(*tfsf_arrays(fx))
+    if "vsp" in fx:
+        r.set_vsp(float(fx["vsp"][0]))
     r.init()
     if graphs:
         r.set_graphs(True)
@@ -75,16 +77,38 @@ def test_replay_equals_stage_calls(case, prec):
     assert a.time == b.time
 
 
+@pytest.fixture(params=["fused", "streams"])
+def batch_mode(request, monkeypatch):
+    """mpic_batch_advance's two forms: the fused ensemble (one launch per
+    stage for all runs, the default) and per-run graph replays on per-run
+    streams (MPIC_BATCH_STREAMS=1)."""
+    if request.param == "streams":
+        monkeypatch.setenv("MPIC_BATCH_STREAMS", "1")
+    else:
+        monkeypatch.delenv("MPIC_BATCH_STREAMS", raising=False)
+    return request.param
+
+
+ALL_CASES = ("periodic", "laser_mur", "mixed", "spot", "tfsf", "tfsf_back", "vsp")
+
+
 @pytest.mark.parametrize("prec", [ma.FP64, ma.FP32_SPLIT])
-def test_ensemble_equals_alone(prec):
-    fxs = [load_fixture(c) for c in ("periodic", "laser_mur", "mixed")]
+def test_ensemble_equals_alone(prec, batch_mode):
+    """Every fixture box (periodic, Mur with the laser, mirror, the hard
+    spot, TFSF forward and backward, VSP: steps of different launch
+    sequences) advanced together gives what it gives alone; the second run
+    starts one step ahead (its markers sit in the other buffer), and the
+    ensemble is advanced in two calls (the fused step is reused)."""
+    fxs = [load_fixture(c) for c in ALL_CASES]
     alone = [make(fx, prec) for fx in fxs]
     mix = [make(fx, prec) for fx in fxs]
-    for r in alone:
-        for _ in range(9):
+    mix[1].step(diag=False)
+    for q, r in enumerate(alone):
+        for _ in range(9 + (q == 1)):
             r.step(diag=False)
         r.sync()
-    ma.batch_advance(mix, 9)
+    ma.batch_advance(mix, 4)
+    ma.batch_advance(mix, 5)
     tol = 1e-12 if prec == ma.FP64 else 2e-5
     for fx, a, b in zip(fxs, alone, mix):
         b.sync()
@@ -113,3 +137,39 @@ def test_autotune_is_result_neutral():
     Eb, Hb = b.fields()
     assert rel_l2(Eb, Ea) <= 2e-5 and rel_l2(Hb, Ha) <= 2e-5
     assert a.time == pytest.approx(b.time, rel=1e-14)
+
+
+@pytest.mark.parametrize("prec", [ma.FP64, ma.FP32_SPLIT])
+@pytest.mark.parametrize("graphs", [False, True])
+def test_diag_history_equals_stepwise_diagnostics(prec, graphs):
+    """The energy history the loop body keeps on the device (em_energy and
+    plasma_temperature of every asynchronous step, mpic_diag_history) equals
+    the rows mpic_step(diag) hands out step by step and, in the FP64 form,
+    the oracle's: Time exactly, energies to the summation order."""
+    fx = load_fixture("laser_mur")
+    a = make(fx, prec)
+    b = make(fx, prec, graphs=graphs)
+    o = oracle(fx)
+    rows_a, rows_o = [], []
+    for _ in range(14):
+        rows_a.append(a.step(diag=True))
+        rows_o.append(o.step()[0])
+    b.advance(6)
+    b.step(diag=False)
+    b.advance(7)
+    h = b.diag_history()
+    assert h.shape == (14, 6)
+    ra, ro = np.array(rows_a), np.array(rows_o)
+    assert np.array_equal(h[:, 0], ra[:, 0])
+    tol = 1e-12 if prec == ma.FP64 else 2e-5
+    np.testing.assert_allclose(h[:, 1:], ra[:, 1:], rtol=tol, atol=1e-300)
+    if prec == ma.FP64:
+        np.testing.assert_allclose(h[:, 1:], ro[:, 1:], rtol=1e-10, atol=1e-300)
+    # staged steps keep no row; the ring keeps the last 1024 rows
+    b.stage("em_h_half")
+    for s in ("push_deposit", "em_h_step", "em_e_start", "j_finish", "em_e_finish"):
+        b.stage(s)
+    b.advance(2)
+    h2 = b.diag_history(4)
+    assert h2.shape == (4, 6)
+    assert np.array_equal(h2[:2, 0], ra[12:14, 0]) and h2[2, 0] > ra[13, 0] + 1.5 * fx["tau"][0]

@@ -15,7 +15,7 @@ from oracle_lib import MARKER_DTYPE, Oracle, load_fixture, oracle_from_fixture, 
 
 pytestmark = pytest.mark.gpu
 
-CASES = ["periodic", "laser_mur", "mixed", "spot", "tfsf", "tfsf_back"]
+CASES = ["periodic", "laser_mur", "mixed", "spot", "tfsf", "tfsf_back", "vsp"]
 
 
 def gpu_from_fixture(fx, markers=True, E=None, H=None):
@@ -30,6 +30,8 @@ def gpu_from_fixture(fx, markers=True, E=None, H=None):
         r.add_gauss_spot(fx["spot_params"])
     if "tfsf_reg" in fx:
         r.add_tfsf(*tfsf_arrays(fx))
+    if "vsp" in fx:
+        r.set_vsp(float(fx["vsp"][0]))
     r.init()
     return r
 
@@ -44,6 +46,8 @@ def oracle(fx, markers=True, E=None, H=None):
         o.add_gauss_spot(fx["spot_params"])
     if "tfsf_reg" in fx:
         o.add_tfsf(*tfsf_arrays(fx))
+    if "vsp" in fx:
+        o.set_vsp(float(fx["vsp"][0]))
     o.init()
     return o
 
@@ -134,7 +138,7 @@ def test_twenty_steps_parity(case):
     assert rel_l2(Eg, Eo20) <= 1e-10
 
 
-@pytest.mark.parametrize("case", ["periodic", "mixed"])
+@pytest.mark.parametrize("case", ["periodic", "mixed", "vsp"])
 def test_gauss_residuals(case):
     fx = load_fixture(case)
     g = gpu_from_fixture(fx)

@@ -62,3 +62,35 @@ def test_admission_respects_budget():
     srv2.submit(big)
     rep2 = srv2.serve()
     assert rep2["rejected"] == 1 and "error" in big.result
+
+
+@pytest.mark.parametrize("prec,tol", [(ma.FP32_SPLIT, 2e-5), (ma.FP64, 1e-12)])
+def test_stream_with_a_large_request_same_alone_and_in_the_mix(prec, tol):
+    """The serving configuration's mix: Poisson arrivals of scan requests and
+    one 256x128x128 request at 16 markers per cell (the large tenth of the
+    stream), admitted against the 240 GB budget while the others run; the
+    large request and two small ones give the result they give alone."""
+    rng = np.random.default_rng(11)
+    reqs, t = [], 0.0
+    for rid in range(6):
+        t += rng.exponential(0.25)
+        p = workloads.scan_request(rng, 1.0 if rid == 2 else 0.0, (20, 40))
+        reqs.append(serve.Request(rid, p, t))
+    assert tuple(reqs[2].params["n"]) == (256, 128, 128) and reqs[2].params["ppc"] == 16
+    srv = serve.Server(prec, budget_gb=240.0, slice_steps=7, full_results=True)
+    for r in reqs:
+        srv.submit(r)
+    rep = srv.serve()
+    assert rep["requests"] == 6 and rep["rejected"] == 0
+    print("serve mix with a large request: p50 %.2f s p99 %.2f s, %.3e marker-pushes/s"
+          % (rep["p50_s"], rep["p99_s"], rep["marker_pushes_per_s"]))
+    for req in (reqs[2], reqs[0], reqs[5]):
+        state = workloads.foil_request(**req.params)
+        alone = serve.start_run(state, prec, graphs=False)
+        alone.advance(req.params["steps"])
+        alone.sync()
+        E, H = alone.fields()
+        assert rel_l2(req.result["E"], E) <= tol and rel_l2(req.result["H"], H) <= tol
+        assert req.result["markers"] == alone.marker_count()
+        assert req.result["time"] == pytest.approx(alone.time, rel=1e-14)
+        alone.close()

@@ -52,6 +52,30 @@ def test_ranmar_known_answer(host):
     assert np.all(v * 2.0 ** 24 == np.floor(v * 2.0 ** 24))
 
 
+def test_ranmar_normal_is_box_muller_of_the_stream(host):
+    """mh_ranmar_normal (the workloads' thermal momenta) is the Box-Muller
+    transform of the RANMAR stream's consecutive pairs, r cos then r sin,
+    and the workloads' seed pair covers the generator's seed range."""
+    from mandor3_amd import rng as mrng
+    u = ranmar_stream(host, 1802, 9373, 2000)
+    r = Ranmar()
+    assert host.mh_ranmar_init(ctypes.byref(r), 1802, 9373) == 0
+    z = np.zeros(1999)
+    host.mh_ranmar_normal.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_long,
+                                      ctypes.c_double]
+    host.mh_ranmar_normal(ctypes.byref(r), z.ctypes.data_as(ctypes.c_void_p), z.size, 0.5)
+    rad = 0.5 * np.sqrt(-2.0 * np.log1p(-u[0::2]))
+    ang = 2.0 * np.pi * u[1::2]
+    want = np.empty(2000)
+    want[0::2], want[1::2] = rad * np.cos(ang), rad * np.sin(ang)
+    np.testing.assert_allclose(z, want[:1999], rtol=1e-13, atol=1e-15)
+    g = mrng.Ranmar(12345).normal(400000, 0.01)
+    assert abs(g.mean()) < 1e-4 and abs(g.std() - 0.01) < 1e-4
+    assert mrng.seed_pair(0) == (0, 0)
+    assert mrng.seed_pair(31329 * 30082 - 1) == (31328, 30081)
+    assert mrng.seed_pair(31329) == (0, 1)
+
+
 def test_ranmar_rejects_bad_seeds(host):
     r = Ranmar()
     assert host.mh_ranmar_init(ctypes.byref(r), 31329, 0) != 0

@@ -14,7 +14,7 @@ import pytest
 from oracle_lib import (field_digest, load_fixture, marker_digest,
                         oracle_from_fixture)
 
-CASES = ["periodic", "laser_mur", "mixed", "spot", "tfsf", "tfsf_back"]
+CASES = ["periodic", "laser_mur", "mixed", "spot", "tfsf", "tfsf_back", "vsp"]
 
 
 @pytest.mark.parametrize("case", CASES)
@@ -90,3 +90,33 @@ def test_tfsf_fixture_pins_the_interface(case):
         ext = [None if x is None else np.concatenate([x, x[-x.size // nf:]]) for x in dataE]
         exth = [None if x is None else np.concatenate([x, x[-x.size // nf:]]) for x in dataH]
         assert _tfsf_first_mismatch(fx, ext, exth, nframes=nf + 1) == 14
+
+
+def test_vsp_fixture_pins_the_convolution():
+    """The VSP fixture holds a run with the (beta, alpha, beta) convolution at
+    alpha = 0.8: without the convolution, or with the weight mistaken, the
+    oracle leaves the reference's records at the first step; the sweep is
+    the identity at alpha = 1 and keeps sums (2 beta + alpha = 1) along
+    periodic lines."""
+    from oracle_lib import Oracle
+    fx = load_fixture("vsp")
+
+    def first_mismatch(alpha):
+        o = Oracle(fx["n"], fx["h"], fx["L"], fx["tau"][0], fx["bc"], 0.0)
+        o.set_fields(fx["E0"], fx["H0"])
+        o.set_markers(fx["M0"])
+        if alpha is not None:
+            o.set_vsp(alpha)
+        o.init()
+        for r in range(1, len(fx["digest_E"]) + 1):
+            o.step()
+            E, _ = o.fields()
+            if field_digest(E) != fx["digest_E"][r - 1]:
+                return r
+        return 0
+    assert first_mismatch(0.8) == 0
+    assert first_mismatch(None) == 1
+    assert first_mismatch(0.75) == 1
+    # the same box without the option: the "mixed" fixture's deck and record 0
+    mx = load_fixture("mixed")
+    assert np.array_equal(mx["M0"]["x"], fx["M0"]["x"])
