@@ -21,6 +21,10 @@ MARKER_DTYPE = np.dtype([("x", "<f8"), ("y", "<f8"), ("z", "<f8"),
                          ("vx", "<f8"), ("vy", "<f8"), ("vz", "<f8"),
                          ("rho", "<f4"), ("qDivM", "<f4"), ("id", "<i4"),
                          ("pad", "<i4")])
+# the reference's marker_t without the tracer id (source/type_marker.h:12-25)
+MARKER56_DTYPE = np.dtype([("x", "<f8"), ("y", "<f8"), ("z", "<f8"),
+                           ("vx", "<f8"), ("vy", "<f8"), ("vz", "<f8"),
+                           ("rho", "<f4"), ("qDivM", "<f4")])
 
 
 class Config(ctypes.Structure):
@@ -183,9 +187,18 @@ class Run:
         _check(lib().mpic_get_J(self.handle, _ptr(J)), "mpic_get_J")
         return J
 
-    def load_markers(self, M):
-        M = np.ascontiguousarray(M, dtype=MARKER_DTYPE)
-        _check(lib().mpic_markers_load(self.handle, _ptr(M), M.size, 64),
+    def load_markers(self, M, recsize=64):
+        """64-byte records (with id) or 56-byte records (no id: the run
+        numbers the markers 1..N in record order)."""
+        dt = MARKER_DTYPE if recsize == 64 else MARKER56_DTYPE
+        if M.dtype != dt:
+            A = np.zeros(M.shape, dtype=dt)
+            for f in dt.names:
+                if f in M.dtype.names:
+                    A[f] = M[f]
+            M = A
+        M = np.ascontiguousarray(M)
+        _check(lib().mpic_markers_load(self.handle, _ptr(M), M.size, recsize),
                "mpic_markers_load")
 
     def append_markers(self, M):
@@ -195,11 +208,11 @@ class Run:
         _check(lib().mpic_markers_append(self.handle, _ptr(M), M.size, 64),
                "mpic_markers_append")
 
-    def markers(self):
+    def markers(self, recsize=64):
         cap = max(int(lib().mpic_marker_count(self.handle)), 1)
-        M = np.zeros(cap, dtype=MARKER_DTYPE)
+        M = np.zeros(cap, dtype=MARKER_DTYPE if recsize == 64 else MARKER56_DTYPE)
         n = ctypes.c_long(cap)
-        _check(lib().mpic_markers_save(self.handle, _ptr(M), ctypes.byref(n), 64),
+        _check(lib().mpic_markers_save(self.handle, _ptr(M), ctypes.byref(n), recsize),
                "mpic_markers_save")
         return M[:n.value]
 

@@ -147,7 +147,7 @@ __device__ static inline void cap_E(const FieldsT<T> &f, const Grid &g, const Bo
 
 template <class T> struct KernelDef<CapArgs<T>> {
     static constexpr int kBlock = TPB;
-    __device__ static void body(const CapArgs<T> &a, VBlock b, VBlock)
+    __device__ __forceinline__ static void body(const CapArgs<T> &a, VBlock b, VBlock)
     {
         const long t = (long)b.x * TPB + threadIdx.x;
         if (a.E)
@@ -295,7 +295,7 @@ template <class T> struct FaceArgs {
 
 template <class T> struct KernelDef<FaceArgs<T>> {
     static constexpr int kBlock = TPB;
-    __device__ static void body(const FaceArgs<T> &a, VBlock b, VBlock)
+    __device__ __forceinline__ static void body(const FaceArgs<T> &a, VBlock b, VBlock)
     {
         face_apply<T>(a.g, a.op[b.y], (long)b.x * TPB + threadIdx.x);
     }
@@ -399,7 +399,7 @@ __device__ static inline void h_update(const FieldsT<T> &f, const Grid &g, const
 
 template <class T> struct KernelDef<HUpdateArgs<T>> {
     static constexpr int kBlock = TPB;
-    __device__ static void body(const HUpdateArgs<T> &a, VBlock b, VBlock gd)
+    __device__ __forceinline__ static void body(const HUpdateArgs<T> &a, VBlock b, VBlock gd)
     {
         h_update<T>(a.f, a.g, a.fl, a.full, a.sp, a.c1, a.c2, a.c3, b, gd);
     }
@@ -467,7 +467,7 @@ __device__ static inline void e_update(const FieldsT<T> &f, const Grid &g, const
 
 template <class T> struct KernelDef<EUpdateArgs<T>> {
     static constexpr int kBlock = TPB;
-    __device__ static void body(const EUpdateArgs<T> &a, VBlock b, VBlock gd)
+    __device__ __forceinline__ static void body(const EUpdateArgs<T> &a, VBlock b, VBlock gd)
     {
         e_update<T>(a.f, a.g, a.fl, a.interior, a.do_flush, a.sp, a.c1, a.c2, a.c3, b, gd);
     }
@@ -520,7 +520,7 @@ __device__ static inline void e_tauJ(const FieldsT<T> &f, const Grid &g, const S
 
 template <class T> struct KernelDef<ETauJArgs<T>> {
     static constexpr int kBlock = TPB;
-    __device__ static void body(const ETauJArgs<T> &a, VBlock b, VBlock gd)
+    __device__ __forceinline__ static void body(const ETauJArgs<T> &a, VBlock b, VBlock gd)
     {
         e_tauJ<T>(a.f, a.g, a.sp, a.tick, b, gd);
     }
@@ -568,7 +568,7 @@ __device__ static inline void laser(T *Fy, T *Fz, const Grid &g, const double *m
 
 template <class T> struct KernelDef<LaserArgs<T>> {
     static constexpr int kBlock = TPB;
-    __device__ static void body(const LaserArgs<T> &a, VBlock b, VBlock)
+    __device__ __forceinline__ static void body(const LaserArgs<T> &a, VBlock b, VBlock)
     {
         laser<T>(a.Fy, a.Fz, a.g, a.map, a.pos, a.JMIN, a.KMIN, a.KMAX, a.jmin, a.jmax, a.kmin,
                  a.kmax, a.tab, a.step, a.cap, (long)b.x * TPB + threadIdx.x);
@@ -615,7 +615,7 @@ __device__ static inline void spot(T *Ey, T *Ez, const Grid &g, int i, const dou
 
 template <class T> struct KernelDef<SpotArgs<T>> {
     static constexpr int kBlock = TPB;
-    __device__ static void body(const SpotArgs<T> &a, VBlock b, VBlock)
+    __device__ __forceinline__ static void body(const SpotArgs<T> &a, VBlock b, VBlock)
     {
         spot<T>(a.Ey, a.Ez, a.g, a.i, a.py, a.pz, a.tab, a.step, a.cap,
                 (long)b.x * TPB + threadIdx.x);
@@ -692,7 +692,7 @@ __device__ static inline void tfsf(T *F0, T *F1, T *F2, const Grid &g, const Tfs
 
 template <class T> struct KernelDef<TfsfArgs<T>> {
     static constexpr int kBlock = TPB;
-    __device__ static void body(const TfsfArgs<T> &a, VBlock b, VBlock)
+    __device__ __forceinline__ static void body(const TfsfArgs<T> &a, VBlock b, VBlock)
     {
         tfsf<T>(a.F0, a.F1, a.F2, a.g, a.fs, a.data, a.tab, a.step, a.cap, a.slot, a.sign,
                 (int)b.y, (long)b.x * TPB + threadIdx.x);
@@ -751,7 +751,7 @@ __device__ static inline void tfsf_record(const Ptr3<T> &E, const Ptr3<T> &H, co
 
 template <class T> struct KernelDef<TfsfRecArgs<T>> {
     static constexpr int kBlock = TPB;
-    __device__ static void body(const TfsfRecArgs<T> &a, VBlock b, VBlock)
+    __device__ __forceinline__ static void body(const TfsfRecArgs<T> &a, VBlock b, VBlock)
     {
         tfsf_record<T>(a.E, a.H, a.g, a.fe, a.fh, a.factor, a.out, a.step, a.step0, a.cap,
                        (int)b.y, (long)b.x * TPB + threadIdx.x);
@@ -775,7 +775,7 @@ struct TickArgs {
 };
 template <> struct KernelDef<TickArgs> {
     static constexpr int kBlock = 64;
-    __device__ static void body(const TickArgs &a, VBlock b, VBlock)
+    __device__ __forceinline__ static void body(const TickArgs &a, VBlock b, VBlock)
     {
         if (threadIdx.x == 0 && b.x == 0) *a.step += 1;
     }
@@ -877,7 +877,7 @@ __device__ static inline void jfold(const JPtrs &f, const Grid &g, int ea, int b
  * disjoint layers [-2, 2] and [N-2, N+2] and share one launch. */
 template <> struct KernelDef<JFoldArgs> {
     static constexpr int kBlock = TPB;
-    __device__ static void body(const JFoldArgs &a, VBlock b, VBlock)
+    __device__ __forceinline__ static void body(const JFoldArgs &a, VBlock b, VBlock)
     {
         jfold(a.f, a.g, a.ea, a.bc0, a.bc1, a.top0, b);
     }
@@ -996,7 +996,7 @@ __device__ static inline void vsp(double *F0, double *F1, double *F2, int ncomp,
 
 template <> struct KernelDef<VspArgs> {
     static constexpr int kBlock = TPB;
-    __device__ static void body(const VspArgs &a, VBlock b, VBlock)
+    __device__ __forceinline__ static void body(const VspArgs &a, VBlock b, VBlock)
     {
         vsp(a.F0, a.F1, a.F2, a.ncomp, a.g, a.ax, a.alpha, a.beta, (long)b.x * TPB + threadIdx.x);
     }
@@ -1067,7 +1067,7 @@ __device__ static inline void energy_part(const FieldsT<T> &f, const Grid &g, do
 
 template <class T> struct KernelDef<EnergyPartArgs<T>> {
     static constexpr int kBlock = TPB;
-    __device__ static void body(const EnergyPartArgs<T> &a, VBlock b, VBlock gd)
+    __device__ __forceinline__ static void body(const EnergyPartArgs<T> &a, VBlock b, VBlock gd)
     {
         energy_part<T>(a.f, a.g, a.part, b, gd);
     }
@@ -1123,7 +1123,7 @@ int energy_parts(const Grid &g)
 
 template <> struct KernelDef<DiagRingArgs> {
     static constexpr int kBlock = TPB;
-    __device__ static void body(const DiagRingArgs &a, VBlock, VBlock)
+    __device__ __forceinline__ static void body(const DiagRingArgs &a, VBlock, VBlock)
     {
         diag_ring(a.part, a.nparts, a.wk, a.ring, a.step, a.cap);
     }
@@ -1167,7 +1167,7 @@ struct Zero2Args {
 };
 template <> struct KernelDef<Zero2Args> {
     static constexpr int kBlock = TPB;
-    __device__ static void body(const Zero2Args &a, VBlock b, VBlock gd)
+    __device__ __forceinline__ static void body(const Zero2Args &a, VBlock b, VBlock gd)
     {
         long t = (long)b.x * TPB + threadIdx.x;
         if (t < a.m) a.q[t] = 0.0;

@@ -52,10 +52,15 @@ __device__ static inline void run_one(const A &a)
     KernelDef<A>::body(a, b, g);
 }
 
+/* Read-only data of a multi-run launch in the constant address space: its
+ * loads are invariant, so the bodies read argument fields in place (as from
+ * the kernel-argument segment) instead of copying whole structs to scratch. */
+#define CAS __attribute__((address_space(4)))
+
 /* tab: [n + 1] first block of every run, then [3 n] the runs' own grids. */
 template <class A>
-__device__ static inline void run_multi(const A *__restrict__ args,
-                                        const unsigned *__restrict__ tab, int n)
+__device__ static inline void run_multi(const CAS A *__restrict__ args,
+                                        const CAS unsigned *__restrict__ tab, int n)
 {
     const unsigned blk = blockIdx.x;
     int lo = 0, hi = n;
@@ -64,11 +69,11 @@ __device__ static inline void run_multi(const A *__restrict__ args,
         if (tab[mid] <= blk) lo = mid;
         else hi = mid;
     }
-    const unsigned *gr = tab + (n + 1) + 3 * lo;
+    const CAS unsigned *gr = tab + (n + 1) + 3 * lo;
     const VBlock g = {gr[0], gr[1], gr[2]};
     const unsigned L = blk - tab[lo];
     const VBlock b = {L % g.x, (L / g.x) % g.y, L / (g.x * g.y)};
-    KernelDef<A>::body(args[lo], b, g);
+    KernelDef<A>::body(*(const A *)(args + lo), b, g);
 }
 
 template <class A> __global__ void __launch_bounds__(KernelDef<A>::kBlock) k_one(A a)
@@ -79,7 +84,7 @@ template <class A>
 __global__ void __launch_bounds__(KernelDef<A>::kBlock)
     k_multi(const A *__restrict__ args, const unsigned *__restrict__ tab, int n)
 {
-    run_multi(args, tab, n);
+    run_multi((const CAS A *)args, (const CAS unsigned *)tab, n);
 }
 template <class A>
 __global__ void __launch_bounds__(KernelDef<A>::kBlock)
@@ -92,7 +97,7 @@ __global__ void __launch_bounds__(KernelDef<A>::kBlock)
     __attribute__((amdgpu_waves_per_eu(WavesOf<A>::v)))
     k_multi_w(const A *__restrict__ args, const unsigned *__restrict__ tab, int n)
 {
-    run_multi(args, tab, n);
+    run_multi((const CAS A *)args, (const CAS unsigned *)tab, n);
 }
 
 template <class A> static inline void start_one(hipStream_t s, const A &a, dim3 grid)

@@ -467,7 +467,7 @@ struct Push32Args {
     double      *Wacc;
 };
 
-__device__ static inline void push32_tiles(const Markers32 &m, const F32View &v,
+__device__ __forceinline__ static void push32_tiles(const Markers32 &m, const F32View &v,
                                            const Push32Consts &c, const TileView &tv, float *tj,
                                            double *Wacc, const VBlock &bk, const VBlock &gd)
 {
@@ -599,7 +599,7 @@ __device__ static inline void push32_tiles(const Markers32 &m, const F32View &v,
 template <> struct KernelDef<Push32Args> {
     static constexpr int kBlock = TW;
     static constexpr int kWaves = 4;
-    __device__ static void body(const Push32Args &a, VBlock b, VBlock gd)
+    __device__ __forceinline__ static void body(const Push32Args &a, VBlock b, VBlock gd)
     {
         push32_tiles(a.m, a.v, a.c, a.tv, a.tj, a.Wacc, b, gd);
     }
@@ -631,7 +631,7 @@ struct JGatherArgs {
     const float *tj;
 };
 
-__device__ static inline void jgather32(const F32View &v, const TileView &tv, const float *tj,
+__device__ __forceinline__ static void jgather32(const F32View &v, const TileView &tv, const float *tj,
                                         const VBlock &bk, const VBlock &gd)
 {
     __shared__ int nfirst[27], nitem[27];    /* items of the 3x3x3 tiles around t (0: none) */
@@ -710,7 +710,7 @@ __device__ static inline void jgather32(const F32View &v, const TileView &tv, co
 
 template <> struct KernelDef<JGatherArgs> {
     static constexpr int kBlock = TPB;
-    __device__ static void body(const JGatherArgs &a, VBlock b, VBlock gd)
+    __device__ __forceinline__ static void body(const JGatherArgs &a, VBlock b, VBlock gd)
     {
         jgather32(a.v, a.tv, a.tj, b, gd);
     }
@@ -812,7 +812,7 @@ __device__ static inline void scatter32(const Markers32 &src, const Markers32 &d
 
 template <> struct KernelDef<Scatter32Args> {
     static constexpr int kBlock = 256;
-    __device__ static void body(const Scatter32Args &a, VBlock b, VBlock)
+    __device__ __forceinline__ static void body(const Scatter32Args &a, VBlock b, VBlock)
     {
         scatter32(a.src, a.dst, a.nbound, a.tv, a.parity, (long)b.x * 256 + threadIdx.x);
     }

@@ -318,7 +318,7 @@ struct Push64Args {
     unsigned  *err;
 };
 
-__device__ static inline void push64(const Markers64 &m, long n, const F64View &v,
+__device__ __forceinline__ static void push64(const Markers64 &m, long n, const F64View &v,
                                      const PushConsts &c, double *Wacc, int mode, unsigned *err,
                                      long p)
 {
@@ -349,7 +349,7 @@ __device__ static inline void push64(const Markers64 &m, long n, const F64View &
 
 template <> struct KernelDef<Push64Args> {
     static constexpr int kBlock = TPB;
-    __device__ static void body(const Push64Args &a, VBlock b, VBlock)
+    __device__ __forceinline__ static void body(const Push64Args &a, VBlock b, VBlock)
     {
         push64(a.m, a.n, a.v, a.c, a.Wacc, a.mode, a.err, (long)b.x * TPB + threadIdx.x);
     }
@@ -383,7 +383,7 @@ struct Push64TArgs {
     unsigned  *err;
 };
 
-__device__ static inline void push64_tiled(const Markers64 &m, const F64View &v,
+__device__ __forceinline__ static void push64_tiled(const Markers64 &m, const F64View &v,
                                            const PushConsts &c, const TileView &tv, double *Wacc,
                                            unsigned *err, const VBlock &bk)
 {
@@ -454,7 +454,7 @@ __device__ static inline void push64_tiled(const Markers64 &m, const F64View &v,
 
 template <> struct KernelDef<Push64TArgs> {
     static constexpr int kBlock = TPB;
-    __device__ static void body(const Push64TArgs &a, VBlock b, VBlock)
+    __device__ __forceinline__ static void body(const Push64TArgs &a, VBlock b, VBlock)
     {
         push64_tiled(a.m, a.v, a.c, a.tv, a.Wacc, a.err, b);
     }

@@ -86,7 +86,7 @@ __device__ static inline void scan_a(const TileView &tv, int *blocksum, int *til
 
 template <> struct KernelDef<ScanAArgs> {
     static constexpr int kBlock = SB;
-    __device__ static void body(const ScanAArgs &a, VBlock b, VBlock)
+    __device__ __forceinline__ static void body(const ScanAArgs &a, VBlock b, VBlock)
     {
         scan_a(a.tv, a.blocksum, a.tileloc, b);
     }
@@ -133,7 +133,7 @@ __device__ static inline void scan_b(int *blocksum, int nb)
 
 template <> struct KernelDef<ScanBArgs> {
     static constexpr int kBlock = SB;
-    __device__ static void body(const ScanBArgs &a, VBlock, VBlock) { scan_b(a.blocksum, a.nb); }
+    __device__ __forceinline__ static void body(const ScanBArgs &a, VBlock, VBlock) { scan_b(a.blocksum, a.nb); }
 };
 
 /* Tile offsets, the active-tile list and, for every non-empty tile, the
@@ -203,7 +203,7 @@ __device__ static inline void scan_c(const TileView &tv, const int *blocksum, co
 
 template <> struct KernelDef<ScanCArgs> {
     static constexpr int kBlock = 256;
-    __device__ static void body(const ScanCArgs &a, VBlock b, VBlock)
+    __device__ __forceinline__ static void body(const ScanCArgs &a, VBlock b, VBlock)
     {
         scan_c(a.tv, a.blocksum, a.tileloc, a.nb, a.parity_next, b);
     }
@@ -259,7 +259,7 @@ __device__ static inline void scatter64(const Markers64 &src, const Markers64 &d
 
 template <> struct KernelDef<Scatter64Args> {
     static constexpr int kBlock = 256;
-    __device__ static void body(const Scatter64Args &a, VBlock b, VBlock)
+    __device__ __forceinline__ static void body(const Scatter64Args &a, VBlock b, VBlock)
     {
         scatter64(a.src, a.dst, a.nbound, a.tv, a.parity, (long)b.x * 256 + threadIdx.x);
     }

@@ -207,3 +207,50 @@ def test_markers_load_refused_after_init():
         r.step(diag=False)
         r.sync()
         r.close()
+
+
+def test_reference_56_byte_records():
+    """marker_t without the tracer id (source/type_marker.h:12-25, 56 B):
+    the run numbers the markers 1..N in record order, and 20 steps from
+    those records match the oracle started from the same records with the
+    same numbers (cells by id and per-cell counts bit-exact, E/H/J <=
+    1e-10).  A 56-byte save holds the 64-byte save's values without the id,
+    in the same order."""
+    fx = load_fixture("mixed")
+    M0 = fx["M0"]
+    rec56 = np.zeros(M0.size, dtype=ma.MARKER56_DTYPE)
+    for f in ma.MARKER56_DTYPE.names:
+        rec56[f] = M0[f]
+    g = ma.Run(fx["n"], fx["h"], fx["L"], fx["tau"][0], fx["bc"], 0.0, max_markers=M0.size)
+    g.set_fields(fx["E0"], fx["H0"])
+    g.load_markers(rec56, recsize=56)
+    g.init()
+    Mn = M0.copy()
+    Mn["id"] = np.arange(1, M0.size + 1)
+    o = Oracle(fx["n"], fx["h"], fx["L"], fx["tau"][0], fx["bc"], 0.0)
+    o.set_fields(fx["E0"], fx["H0"])
+    o.set_markers(Mn)
+    o.init()
+    M, Mo = by_id(g.markers()), by_id(o.markers())
+    assert np.array_equal(M["id"], Mo["id"]) and M["id"].max() <= M0.size
+    for f in ma.MARKER56_DTYPE.names:
+        assert np.array_equal(M[f], Mo[f]), f
+    ncell = int(np.prod(fx["n"]))
+    for s in range(1, 21):
+        g.step(diag=False)
+        o.step()
+        Mg, Mo = by_id(g.markers()), by_id(o.markers())
+        assert np.array_equal(Mg["id"], Mo["id"]), s
+        cg, co = cells(Mg, fx), cells(Mo, fx)
+        assert np.array_equal(cg, co), (s, int(np.sum(cg != co)))
+        assert np.array_equal(np.bincount(cg, minlength=ncell), np.bincount(co, minlength=ncell))
+        Eg, Hg = g.fields()
+        Eo, Ho = o.fields()
+        assert rel_l2(Eg, Eo) <= 1e-10 and rel_l2(Hg, Ho) <= 1e-10, s
+        assert rel_l2(g.J(), o.J()) <= 1e-10, s
+    s64, s56 = g.markers(), g.markers(recsize=56)
+    assert s56.dtype.itemsize == 56 and s56.size == s64.size
+    for f in ma.MARKER56_DTYPE.names:
+        assert np.array_equal(s56[f], s64[f]), f
+    g.close()
+    o.close()
