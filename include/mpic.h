@@ -209,6 +209,16 @@ int  mpic_node_fields(mpic_run_t *run, int what, double *out);
  * after em_HHalfStep). */
 int  mpic_set_probes(mpic_run_t *run, const int *ijk, int nprobes);
 int  mpic_probe_sample(mpic_run_t *run, double *out);
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops).  The samples
+ * of mpic_set_probes(ijk, nprobes) + mpic_probe_sample(out) in one call
+ * that allocates nothing: the nodes and samples pass through a static
+ * device scratch of MPIC_PROBE_AT_MAX probes (1 MiB, held by every GPU the
+ * library runs on until the process ends).  More probes: MPIC_ERR_CAPACITY.
+ * The run's own probe set is left as it is.  All calls on one GPU share the
+ * scratch: calls that overlap on one GPU (from several host threads) race. */
+#define MPIC_PROBE_AT_MAX 16384
+int  mpic_probe_sample_at(mpic_run_t *run, const int *ijk, int nprobes, double *out);
 /* gauss_before / gauss_after residuals {max|drho/dt + div J|, max|div E - rho|}:
  * call mpic_gauss_before() before mpic_push_deposit and mpic_gauss_after()
  * after mpic_em_e_finish. */

@@ -15,6 +15,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "libmpic.so")
 
 BC_PERIODIC, BC_MIRROR, BC_OPEN = 0, 1, 2
+PROBE_AT_MAX = 16384          # MPIC_PROBE_AT_MAX of include/mpic.h
 FP64, FP32_SPLIT = 0, 1
 
 MARKER_DTYPE = np.dtype([("x", "<f8"), ("y", "<f8"), ("z", "<f8"),
@@ -77,6 +78,7 @@ SIGNATURES = {
     "mpic_tfsf_recorded": (_I, [_P, _P, _P, _P]),
     "mpic_set_probes": (_I, [_P, _P, _I]),
     "mpic_probe_sample": (_I, [_P, _P]),
+    "mpic_probe_sample_at": (_I, [_P, _P, _I, _P]),
     "mpic_gauss_before": (_I, [_P]),
     "mpic_gauss_after": (_I, [_P, _P]),
     "mpic_sort": (_I, [_P, _P]),
@@ -337,6 +339,15 @@ class Run:
     def probe_sample(self):
         out = np.empty(6 * getattr(self, "_nprobes", 0))
         _check(lib().mpic_probe_sample(self.handle, _ptr(out)), "mpic_probe_sample")
+        return out.reshape(-1, 6)
+
+    def probe_sample_at(self, ijk):
+        """Samples at the nodes ijk without allocating device memory
+        (mpic_probe_sample_at: at most PROBE_AT_MAX probes)."""
+        ijk = np.ascontiguousarray(ijk, dtype=np.int32).reshape(-1, 3)
+        out = np.empty(6 * ijk.shape[0])
+        _check(lib().mpic_probe_sample_at(self.handle, _ptr(ijk), ijk.shape[0], _ptr(out)),
+               "mpic_probe_sample_at")
         return out.reshape(-1, 6)
 
     def gauss_before(self):

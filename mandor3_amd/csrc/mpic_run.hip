@@ -44,6 +44,8 @@ extern "C" int mpic_device_count(void)
         if (rc_ != MPIC_OK) return rc_; \
     } while (0)
 
+static int scratch_setup(int dev);
+
 /* Calls a field launcher with the run's field set (FP64 or FP32). */
 #define FCALL(fn, ...)                                                   \
     do {                                                                 \
@@ -396,6 +398,8 @@ extern "C" int mpic_create(const mpic_config_t *cfg, mpic_run_t **out)
         mpic_set_error("mpic_create: capacity %ld exceeds 2^31 - 1 markers", cfg->max_markers);
         return MPIC_ERR_ARG;
     }
+    MPIC_CHECK(hipSetDevice(cfg->device));
+    TRY(scratch_setup(cfg->device));
     mpic_run *r = new mpic_run();
     r->cfg = *cfg;
     r->dev = cfg->device;
@@ -749,6 +753,79 @@ extern "C" int mpic_probe_sample(mpic_run_t *r, double *out)
                                   r->dsample));
     MPIC_CHECK(hipMemcpyAsync(out, r->dsample, sizeof(double) * 6 * r->nprobes,
                               hipMemcpyDeviceToHost, r->stream));
+    MPIC_CHECK(hipStreamSynchronize(r->stream));
+    return MPIC_OK;
+}
+
+/* ---------------- probe samples without allocation ---------------- */
+/* Device scratch of mpic_probe_sample_at: the samples (6 doubles per probe)
+ * then the probe nodes (3 ints per probe, 2 doubles reserved).  Every GPU
+ * holds its own copy until the process ends; all calls on one GPU share it,
+ * so calls that overlap on one GPU race. */
+static __device__ __align__(256) double scratch[8 * MPIC_PROBE_AT_MAX];
+
+#define MPIC_MAX_DEVICES 64
+static void *g_scratch_ptr[MPIC_MAX_DEVICES];
+static std::mutex g_scratch_mu;
+
+/* The device address of this device's scratch, taken once per device with
+ * that device current (mpic_create calls it for the run's device).  Host
+ * code uses this address, never the array's name. */
+static int scratch_setup(int dev)
+{
+    if (dev < 0 || dev >= MPIC_MAX_DEVICES) {
+        mpic_set_error("device %d beyond the %d devices the library tracks", dev,
+                       MPIC_MAX_DEVICES);
+        return MPIC_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[dev]) return MPIC_OK;
+    MPIC_CHECK(hipSetDevice(dev));
+    void *p = nullptr;
+    MPIC_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) {
+        mpic_set_error("hipGetSymbolAddress returned no address for the probe scratch");
+        return MPIC_ERR_DEVICE;
+    }
+    g_scratch_ptr[dev] = p;
+    return MPIC_OK;
+}
+
+/* mpic_set_probes + mpic_probe_sample in one call that allocates nothing:
+ * the nodes and the samples pass through the device's static scratch.  The
+ * kernel writes every sample it hands back, so nothing is cleared. */
+extern "C" int mpic_probe_sample_at(mpic_run_t *r, const int *ijk, int nprobes, double *out)
+{
+    if (!r || nprobes < 0 || (nprobes && (!ijk || !out))) return MPIC_ERR_ARG;
+    if (nprobes > MPIC_PROBE_AT_MAX) {
+        mpic_set_error("mpic_probe_sample_at: %d probes exceed the scratch limit of %d", nprobes,
+                       MPIC_PROBE_AT_MAX);
+        return MPIC_ERR_CAPACITY;
+    }
+    for (int p = 0; p < nprobes; ++p)
+        for (int a = 0; a < 3; ++a)
+            if (ijk[3 * p + a] < 0 || ijk[3 * p + a] >= r->g.n[a]) {
+                mpic_set_error("mpic_probe_sample_at: probe %d node (%d, %d, %d) outside "
+                               "[0, n-1]", p, ijk[3 * p], ijk[3 * p + 1], ijk[3 * p + 2]);
+                return MPIC_ERR_ARG;
+            }
+    if (!nprobes) return MPIC_OK;
+    MPIC_CHECK(hipSetDevice(r->dev));
+    void *base = g_scratch_ptr[r->dev];
+    if (!base) {
+        mpic_set_error("mpic_probe_sample_at: scratch of device %d not set up", r->dev);
+        return MPIC_ERR_STATE;
+    }
+    double *dout = (double *)base;
+    int *dijk = (int *)(dout + 6 * MPIC_PROBE_AT_MAX);
+    MPIC_CHECK(hipMemcpyAsync(dijk, ijk, sizeof(int) * 3 * nprobes, hipMemcpyHostToDevice,
+                              r->stream));
+    if (r->fp32)
+        TRY(launch_probes<float>(r->stream, r->f32.E, r->f32.H, r->g, dijk, nprobes, dout));
+    else
+        TRY(launch_probes<double>(r->stream, r->f.E, r->f.H, r->g, dijk, nprobes, dout));
+    MPIC_CHECK(hipMemcpyAsync(out, dout, sizeof(double) * 6 * nprobes, hipMemcpyDeviceToHost,
+                              r->stream));
     MPIC_CHECK(hipStreamSynchronize(r->stream));
     return MPIC_OK;
 }

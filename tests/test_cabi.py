@@ -86,3 +86,14 @@ def test_create_rejects_bad_configs_before_touching_a_device():
     assert lib.mpic_create(None, ctypes.byref(run)) == -1
     assert lib.mpic_markers_load(None, None, 0, 64) == -1
     assert lib.mpic_markers_append(None, None, 0, 64) == -1
+
+
+def test_probe_scratch_limit_matches_the_header():
+    """The binding's PROBE_AT_MAX is the header's MPIC_PROBE_AT_MAX, and the
+    no-allocation probe call refuses a null run before touching a device."""
+    import mandor3_amd
+    txt = open(os.path.join(ROOT, "include", "mpic.h")).read()
+    m = re.search(r"#define\s+MPIC_PROBE_AT_MAX\s+(\d+)", txt)
+    assert m and int(m.group(1)) == mandor3_amd.PROBE_AT_MAX
+    lib = mandor3_amd.lib()
+    assert lib.mpic_probe_sample_at(None, None, 0, None) == -1
