@@ -119,7 +119,7 @@ def measured_traffic(kernel, workload):
     return None
 
 
-def run_gpu(ma, wl, precision, steps, warmup):
+def run_gpu(ma, wl, precision, steps, warmup, autotune=True):
     prec = ma.FP64 if precision == "fp64" else ma.FP32_SPLIT
     M = wl["markers"]
     r = ma.Run(wl["n"], wl["h"], wl["L"], wl["tau"], wl["bc"], 0.0, precision=prec,
@@ -130,6 +130,11 @@ def run_gpu(ma, wl, precision, steps, warmup):
     if wl["laser"] is not None:
         r.add_laser(*wl["laser"])
     r.init()
+    tuned = None
+    if precision == "fp32" and autotune:
+        # the launch shape of the push (markers per work item, workgroups),
+        # picked by timing real steps before the timed region
+        tuned = r.autotune(steps=2)
     live = r.marker_count()
     r.advance(warmup)
     r.sync()
@@ -153,6 +158,8 @@ def run_gpu(ma, wl, precision, steps, warmup):
             "rest_of_step_ms": rest_ms / steps,
             "push_kernel_bytes": alg,
             "push_kernel_GBps": alg / (avg_push * 1e-3) / 1e9,
+            "autotune": None if tuned is None else {"workgroups": tuned[0],
+                                                   "markers_per_item": tuned[1]},
             "last_diag": [float(v) for v in hist[-1]]}
 
 
@@ -445,7 +452,8 @@ def main():
                    "cell_updates_per_s": m["cell_updates_per_s"],
                    "push_kernel_ms": m["push_kernel_ms"],
                    "deposit_gather_ms": m["deposit_rest_ms"],
-                   "rest_of_step_ms": m["rest_of_step_ms"]},
+                   "rest_of_step_ms": m["rest_of_step_ms"],
+                   "push_launch_shape": m["autotune"]},
         "roofline": {"bound": "hbm", "kernel": kernel, "achieved": m["push_kernel_GBps"],
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": m["push_kernel_GBps"] / HBM_PEAK_GBS,

@@ -167,14 +167,24 @@ int  mpic_diag_history(mpic_run_t *run, double *rows, int max_rows, int *nrows);
  * stage calls.  Kernel timing (mpic_last_timing) is recorded only with
  * replay off. */
 int  mpic_set_graphs(mpic_run_t *run, int on);
-/* An ensemble of runs advanced together, step by step across the runs,
- * each replaying its captured step on its own stream (no counterpart in the
- * reference, which runs one simulation per job).  Every run must be between
+/* An ensemble of runs advanced together (no counterpart in the reference,
+ * which runs one simulation per job): one step of every run is recorded,
+ * the launches of each kernel kind are fused across the runs into one
+ * launch over a table of per-run arguments and block ranges, and the fused
+ * step is captured in one graph (per marker-buffer phase) and replayed on
+ * the ensemble's own stream, ordered after and before the runs' streams.
+ * The fused step is kept for the next call with the same runs.  With
+ * MPIC_BATCH_STREAMS=1, or runs on several devices, every run replays its
+ * own captured step on its own stream instead.  Every run must be between
  * steps (a step begun through the stage calls: MPIC_ERR_ORDER). */
 int  mpic_batch_advance(mpic_run_t *const *runs, int nruns, int nsteps);
-/* Picks the workgroup count of the FP32 push and J gather by timing
- * 'steps' real steps per candidate (the steps advance the run; the choice
- * does not change results).  *chosen receives the count. */
+/* Picks the launch shape of the FP32 push and J gather -- markers per work
+ * item (a tile's markers split into even items of at most 2048, 1024 or 512,
+ * or one item per tile) and workgroup count -- by timing 'steps' real steps
+ * per candidate (the steps advance the run; the choice changes results only
+ * by the summation order of J).  chosen (may be NULL) receives {workgroups,
+ * markers per item (0: one item per tile)}.  The work-item buffers may grow
+ * by up to an eighth of the marker buffers' size.  Between steps only. */
 int  mpic_autotune(mpic_run_t *run, int steps, int *chosen);
 int  mpic_sync(mpic_run_t *run);
 double mpic_time(mpic_run_t *run);

@@ -282,11 +282,12 @@ class Run:
         _check(lib().mpic_set_graphs(self.handle, 1 if on else 0), "mpic_set_graphs")
 
     def autotune(self, steps=3):
-        """Times `steps` real steps per candidate workgroup count of the FP32
-        push and keeps the fastest; returns the count."""
-        c = np.zeros(1, dtype=np.int32)
+        """Times `steps` real steps per candidate launch shape of the FP32
+        push (markers per work item x workgroup count) and keeps the fastest;
+        returns (workgroups, markers per item, 0 = one item per tile)."""
+        c = np.zeros(2, dtype=np.int32)
         _check(lib().mpic_autotune(self.handle, int(steps), _ptr(c)), "mpic_autotune")
-        return int(c[0])
+        return int(c[0]), int(c[1])
 
     def sync(self):
         _check(lib().mpic_sync(self.handle), "mpic_sync")

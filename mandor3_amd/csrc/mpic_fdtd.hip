@@ -352,17 +352,62 @@ template <class T> struct HUpdateArgs {
     int        full;
     Span       sp;
     T          c1, c2, c3;
+    double    *part;       /* non-null: em_energy partial sums, 2 per workgroup */
 };
 
+/* Sum over the workgroup of (we, wb), stored as part[0..1]; every thread of
+ * the workgroup calls it. */
+__device__ static inline void block_sum2(double we, double wb, double *part)
+{
+    __shared__ double se[TPB / 64], sb[TPB / 64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int d = 32; d > 0; d >>= 1) {
+        we += __shfl_down(we, d, 64);
+        wb += __shfl_down(wb, d, 64);
+    }
+    if (lane == 0) {
+        se[wv] = we;
+        sb[wv] = wb;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0, c = 0;
+        for (int q = 0; q < TPB / 64; ++q) {
+            a += se[q];
+            c += sb[q];
+        }
+        part[0] = a;
+        part[1] = c;
+    }
+}
+
+/* em_energy's terms of node (i,j,k) (source/core/em.c:227-252): each
+ * component squared where the node is that component's point of a cell of
+ * [0,n)^3 -- Ex at (i+1,j,k), Ey at (i,j+1,k), Ez at (i,j,k+1), Hx at
+ * (i,j+1,k+1), Hy at (i+1,j,k+1), Hz at (i+1,j+1,k) of cell (i,j,k). */
 template <class T>
-__device__ static inline void h_update(const FieldsT<T> &f, const Grid &g, const BoxSet &fl,
-                                       int full, const Span &sp, T c1, T c2, T c3,
-                                       const VBlock &b, const VBlock &gd)
+__device__ static inline void energy_terms(const Grid &g, int i, int j, int k, T ex, T ey, T ez,
+                                           T hx, T hy, T hz, double &we, double &wb)
+{
+    const bool i0 = i >= 0 && i < g.n[0], i1 = i >= 1 && i <= g.n[0];
+    const bool j0 = j >= 0 && j < g.n[1], j1 = j >= 1 && j <= g.n[1];
+    const bool k0 = k >= 0 && k < g.n[2], k1 = k >= 1 && k <= g.n[2];
+    const double dx = ex, dy = ey, dz = ez, bx = hx, by = hy, bz = hz;
+    if (i1 && j0 && k0) we += dx * dx;
+    if (i0 && j1 && k0) we += dy * dy;
+    if (i0 && j0 && k1) we += dz * dz;
+    if (i0 && j1 && k1) wb += bx * bx;
+    if (i1 && j0 && k1) wb += by * by;
+    if (i1 && j1 && k0) wb += bz * bz;
+}
+
+template <class T>
+__device__ static inline void h_update_group(const FieldsT<T> &f, const Grid &g, const BoxSet &fl,
+                                             int full, T c1, T c2, T c3, bool energy, int i,
+                                             int j, int k0, double &we, double &wb)
 {
     typedef typename Vec<T>::v VT;
     constexpr int V = Vec<T>::N;
-    int i, j, k0;
-    if (!span_group(sp, V, b, gd, i, j, k0)) return;
     const long o = gidx(g, i, j, k0);
     const long si = (long)g.GY * g.GZ, sj = g.GZ;
     const T *Ex = f.E[0], *Ey = f.E[1], *Ez = f.E[2];
@@ -391,27 +436,53 @@ __device__ static inline void h_update(const FieldsT<T> &f, const Grid &g, const
                 hz[e] = (T)0.5 * (hz[e] + f.Hc[2][o + e]);
             }
         }
+        if (energy) energy_terms<T>(g, i, j, k, ex[e], ey[e], ez[e], hx[e], hy[e], hz[e], we, wb);
     }
     *(VT *)(f.H[0] + o) = hx;
     *(VT *)(f.H[1] + o) = hy;
     *(VT *)(f.H[2] + o) = hz;
 }
 
+template <class T>
+__device__ static inline void h_update(const FieldsT<T> &f, const Grid &g, const BoxSet &fl,
+                                       int full, const Span &sp, T c1, T c2, T c3, double *part,
+                                       const VBlock &b, const VBlock &gd)
+{
+    int i, j, k0;
+    const bool on = span_group(sp, Vec<T>::N, b, gd, i, j, k0);
+    double we = 0.0, wb = 0.0;
+    if (on) h_update_group<T>(f, g, fl, full, c1, c2, c3, part != nullptr, i, j, k0, we, wb);
+    /* the energy of H after this update (the half step: H^n) and of E */
+    if (part) block_sum2(we, wb, part + 2 * (long)b.x);
+}
+
 template <class T> struct KernelDef<HUpdateArgs<T>> {
     static constexpr int kBlock = TPB;
     __device__ __forceinline__ static void body(const HUpdateArgs<T> &a, VBlock b, VBlock gd)
     {
-        h_update<T>(a.f, a.g, a.fl, a.full, a.sp, a.c1, a.c2, a.c3, b, gd);
+        h_update<T>(a.f, a.g, a.fl, a.full, a.sp, a.c1, a.c2, a.c3, a.part, b, gd);
     }
 };
 
+static Span h_span(const Grid &g, int V)
+{
+    return V == 4 ? span_of<4>(0, 0, g.n[1] + 1, 0, g.n[2] + 1)
+                  : span_of<2>(0, 0, g.n[1] + 1, 0, g.n[2] + 1);
+}
+
+/* Workgroups of the H update: the energy partial sums it writes. */
+int h_update_blocks(const Grid &g, int fp32)
+{
+    return (int)grid_span(h_span(g, fp32 ? Vec<float>::N : Vec<double>::N), g.n[0] + 2);
+}
+
 template <class T>
 int launch_h_update(hipStream_t s, const FieldsT<T> &f, const Grid &g, const BoxSet &flush,
-                    int full)
+                    int full, double *part)
 {
-    const Span sp = span_of<Vec<T>::N>(0, 0, g.n[1] + 1, 0, g.n[2] + 1);
+    const Span sp = h_span(g, Vec<T>::N);
     HUpdateArgs<T> a = {f, g, flush, full, sp, (T)(0.5 * g.tau / g.h[0]),
-                        (T)(0.5 * g.tau / g.h[1]), (T)(0.5 * g.tau / g.h[2])};
+                        (T)(0.5 * g.tau / g.h[1]), (T)(0.5 * g.tau / g.h[2]), part};
     return launch_def(s, a, grid_span(sp, g.n[0] + 2));
 }
 
@@ -531,6 +602,85 @@ int launch_e_tauJ(hipStream_t s, const FieldsT<T> &f, const Grid &g, int *tick)
 {
     const Span sp = span_of<Vec<T>::N>(-1, -1, g.n[1] + 1, -1, g.n[2] + 1);
     ETauJArgs<T> a = {f, g, sp, tick};
+    return launch_def(s, a, grid_span(sp, g.n[0] + 3));
+}
+
+/* The E step of the loop body in one pass: the interior update of
+ * em_EStep_start, the cap flush E = Ec and E -= tau J of em_EStep_finish,
+ * node by node in that order, over nodes [-1, n+1]^3 (the J fold has run
+ * before).  Per node the arithmetic is that of e_update then e_tauJ; the
+ * loop body uses it where no source adds to E between the two (no laser, no
+ * TFSF interface) in the FP64 form, and always in the FP32 split form,
+ * whose sources then add after E -= tau J (a different rounding order). */
+template <class T> struct EFusedArgs {
+    FieldsT<T> f;
+    Grid       g;
+    BoxSet     fl;
+    Span       sp;
+    T          c1, c2, c3;
+};
+
+template <class T>
+__device__ static inline void e_fused(const FieldsT<T> &f, const Grid &g, const BoxSet &fl,
+                                      const Span &sp, T c1, T c2, T c3, const VBlock &b,
+                                      const VBlock &gd)
+{
+    typedef typename Vec<T>::v VT;
+    typedef double VD __attribute__((ext_vector_type(Vec<T>::N)));
+    constexpr int V = Vec<T>::N;
+    int i, j, k0;
+    if (!span_group(sp, V, b, gd, i, j, k0)) return;
+    const long o = gidx(g, i, j, k0);
+    const long si = (long)g.GY * g.GZ, sj = g.GZ;
+    const T *Hx = f.H[0], *Hy = f.H[1], *Hz = f.H[2];
+    const VT hx = *(const VT *)(Hx + o), hy = *(const VT *)(Hy + o), hz = *(const VT *)(Hz + o);
+    const VT hxj = *(const VT *)(Hx + o + sj), hzj = *(const VT *)(Hz + o + sj);
+    const VT hyi = *(const VT *)(Hy + o + si), hzi = *(const VT *)(Hz + o + si);
+    const T hxkV = Hx[o + V], hykV = Hy[o + V];          /* k + 1 of the last node */
+    VT ex = *(const VT *)(f.E[0] + o), ey = *(const VT *)(f.E[1] + o), ez = *(const VT *)(f.E[2] + o);
+    const VD jx = *(const VD *)(f.J[0] + o), jy = *(const VD *)(f.J[1] + o),
+             jz = *(const VD *)(f.J[2] + o);
+    const bool ij_in = i >= 2 && i <= g.n[0] - 2 && j >= 2 && j <= g.n[1] - 2;
+    const bool ij_fl = i >= 0 && j >= 0;                 /* e_update's nodes start at 0 */
+    const double tau = g.tau;
+#pragma unroll
+    for (int e = 0; e < V; ++e) {
+        const int k = k0 + e;
+        const T hxk = e < V - 1 ? hx[e + 1] : hxkV, hyk = e < V - 1 ? hy[e + 1] : hykV;
+        if (ij_in && k >= 2 && k <= g.n[2] - 2) {
+            ex[e] -= c3 * (hyk - hy[e]) - c2 * (hzj[e] - hz[e]);
+            ey[e] -= c1 * (hzi[e] - hz[e]) - c3 * (hxk - hx[e]);
+            ez[e] -= c2 * (hxj[e] - hx[e]) - c1 * (hyi[e] - hy[e]);
+        } else if (ij_fl && k >= 0 && k <= g.n[2] + 1 && in_boxes(fl, i, j, k)) {
+            ex[e] = f.Ec[0][o + e];
+            ey[e] = f.Ec[1][o + e];
+            ez[e] = f.Ec[2][o + e];
+        }
+        if (k >= -1 && k <= g.n[2] + 1) {
+            ex[e] = (T)(ex[e] - tau * jx[e]);
+            ey[e] = (T)(ey[e] - tau * jy[e]);
+            ez[e] = (T)(ez[e] - tau * jz[e]);
+        }
+    }
+    *(VT *)(f.E[0] + o) = ex;
+    *(VT *)(f.E[1] + o) = ey;
+    *(VT *)(f.E[2] + o) = ez;
+}
+
+template <class T> struct KernelDef<EFusedArgs<T>> {
+    static constexpr int kBlock = TPB;
+    __device__ __forceinline__ static void body(const EFusedArgs<T> &a, VBlock b, VBlock gd)
+    {
+        e_fused<T>(a.f, a.g, a.fl, a.sp, a.c1, a.c2, a.c3, b, gd);
+    }
+};
+
+template <class T>
+int launch_e_fused(hipStream_t s, const FieldsT<T> &f, const Grid &g, const BoxSet &flush)
+{
+    const Span sp = span_of<Vec<T>::N>(-1, -1, g.n[1] + 1, -1, g.n[2] + 1);
+    EFusedArgs<T> a = {f, g, flush, sp, (T)(g.tau / g.h[0]), (T)(g.tau / g.h[1]),
+                       (T)(g.tau / g.h[2])};
     return launch_def(s, a, grid_span(sp, g.n[0] + 3));
 }
 
@@ -1019,60 +1169,6 @@ int launch_vsp(hipStream_t s, double *const F[3], int ncomp, const Grid &g, doub
  * one wavefront per cell row (i, j), lanes along k, each workgroup's partial
  * sums stored in part[2 * block + {0, 1}] and combined in fixed order by
  * k_diag_ring. */
-template <class T> struct EnergyPartArgs {
-    FieldsT<T> f;
-    Grid       g;
-    double    *part;
-};
-
-template <class T>
-__device__ static inline void energy_part(const FieldsT<T> &f, const Grid &g, double *part,
-                                          const VBlock &b, const VBlock &gd)
-{
-    __shared__ double se[TPB / 64], sb[TPB / 64];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int rows = g.n[0] * g.n[1];
-    double we = 0, wb = 0;
-    for (int row = b.x * (TPB / 64) + wv; row < rows; row += gd.x * (TPB / 64)) {
-        const int i = row / g.n[1], j = row - i * g.n[1];
-        const long b = gidx(g, i, j, 0);
-        const long dI = (long)g.GY * g.GZ, dJ = g.GZ;
-        for (int k = lane; k < g.n[2]; k += 64) {
-            const double ex = f.E[0][b + dI + k], ey = f.E[1][b + dJ + k],
-                         ez = f.E[2][b + k + 1], hx = f.H[0][b + dJ + k + 1],
-                         hy = f.H[1][b + dI + k + 1], hz = f.H[2][b + dI + dJ + k];
-            we += ex * ex + ey * ey + ez * ez;
-            wb += hx * hx + hy * hy + hz * hz;
-        }
-    }
-    for (int d = 32; d > 0; d >>= 1) {
-        we += __shfl_down(we, d, 64);
-        wb += __shfl_down(wb, d, 64);
-    }
-    if (lane == 0) {
-        se[wv] = we;
-        sb[wv] = wb;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double a = 0, c = 0;
-        for (int q = 0; q < TPB / 64; ++q) {
-            a += se[q];
-            c += sb[q];
-        }
-        part[2 * b.x] = a;
-        part[2 * b.x + 1] = c;
-    }
-}
-
-template <class T> struct KernelDef<EnergyPartArgs<T>> {
-    static constexpr int kBlock = TPB;
-    __device__ __forceinline__ static void body(const EnergyPartArgs<T> &a, VBlock b, VBlock gd)
-    {
-        energy_part<T>(a.f, a.g, a.part, b, gd);
-    }
-};
-
 /* Row of the diagnostics ring of the current step (device step counter
  * *step): the two energy sums of k_energy_part's nparts partials (fixed
  * order) and the three kinetic sums wk[0..2] of the push. */
@@ -1081,23 +1177,26 @@ struct DiagRingArgs {
     int           nparts;
     const double *wk;
     double       *ring;
-    const int    *step;
+    int          *step;
     int           cap;
+    int           tick;    /* advance the step counter after the row (the step's last kernel) */
 };
 
+#define RING_TPB 1024
+
 __device__ static inline void diag_ring(const double *part, int nparts, const double *wk,
-                                        double *ring, const int *step, int cap)
+                                        double *ring, int *step, int cap, int tick)
 {
-    __shared__ double se[TPB], sb[TPB];
+    __shared__ double se[RING_TPB], sb[RING_TPB];
     double a = 0, c = 0;
-    for (int q = threadIdx.x; q < nparts; q += TPB) {
+    for (int q = threadIdx.x; q < nparts; q += RING_TPB) {
         a += part[2 * q];
         c += part[2 * q + 1];
     }
     se[threadIdx.x] = a;
     sb[threadIdx.x] = c;
     __syncthreads();
-    for (int w = TPB / 2; w > 0; w >>= 1) {
+    for (int w = RING_TPB / 2; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w) {
             se[threadIdx.x] += se[threadIdx.x + w];
             sb[threadIdx.x] += sb[threadIdx.x + w];
@@ -1111,35 +1210,22 @@ __device__ static inline void diag_ring(const double *part, int nparts, const do
         row[2] = wk[0];
         row[3] = wk[1];
         row[4] = wk[2];
+        if (tick) *step += 1;
     }
-}
-
-int energy_parts(const Grid &g)
-{
-    const int rows = g.n[0] * g.n[1];
-    int nb = (rows + TPB / 64 - 1) / (TPB / 64);
-    return nb > 1024 ? 1024 : nb;
 }
 
 template <> struct KernelDef<DiagRingArgs> {
-    static constexpr int kBlock = TPB;
+    static constexpr int kBlock = RING_TPB;
     __device__ __forceinline__ static void body(const DiagRingArgs &a, VBlock, VBlock)
     {
-        diag_ring(a.part, a.nparts, a.wk, a.ring, a.step, a.cap);
+        diag_ring(a.part, a.nparts, a.wk, a.ring, a.step, a.cap, a.tick);
     }
 };
 
-template <class T>
-int launch_energy_part(hipStream_t s, const FieldsT<T> &f, const Grid &g, double *part)
-{
-    EnergyPartArgs<T> a = {f, g, part};
-    return launch_def(s, a, energy_parts(g));
-}
-
 int launch_diag_ring(hipStream_t s, const double *part, int nparts, const double *wk,
-                     double *ring, const int *step, int cap)
+                     double *ring, int *step, int cap, int tick)
 {
-    DiagRingArgs a = {part, nparts, wk, ring, step, cap};
+    DiagRingArgs a = {part, nparts, wk, ring, step, cap, tick};
     return launch_def(s, a, 1);
 }
 
@@ -1348,7 +1434,9 @@ int launch_probes(hipStream_t s, T *const E[3], T *const H[3], const Grid &g, co
     template int launch_cap_H<T>(hipStream_t, const FieldsT<T> &, const Grid &, const BoxSet &); \
     template int launch_cap_E<T>(hipStream_t, const FieldsT<T> &, const Grid &, const BoxSet &); \
     template int launch_h_update<T>(hipStream_t, const FieldsT<T> &, const Grid &,                \
-                                    const BoxSet &, int);                                       \
+                                    const BoxSet &, int, double *);                             \
+    template int launch_e_fused<T>(hipStream_t, const FieldsT<T> &, const Grid &,                 \
+                                   const BoxSet &);                                             \
     template int launch_e_update<T>(hipStream_t, const FieldsT<T> &, const Grid &,                \
                                     const BoxSet &, int, int);                                  \
     template int launch_e_tauJ<T>(hipStream_t, const FieldsT<T> &, const Grid &, int *);         \
@@ -1358,7 +1446,6 @@ int launch_probes(hipStream_t s, T *const E[3], T *const H[3], const Grid &g, co
     template int launch_spot<T>(hipStream_t, T *, T *, const Grid &, int, const double *,        \
                                 const double *, const double *, const int *, int);              \
     template int launch_energy<T>(hipStream_t, const FieldsT<T> &, const Grid &, double *);      \
-    template int launch_energy_part<T>(hipStream_t, const FieldsT<T> &, const Grid &, double *); \
     template int launch_tfsf<T>(hipStream_t, T *const[3], const Grid &, const TfsfSet &, long,   \
                                 const double *, const int *, const int *, int, int, double);   \
     template int launch_gauss<T>(hipStream_t, const FieldsT<T> &, const double *,                \

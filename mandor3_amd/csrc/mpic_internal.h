@@ -148,7 +148,9 @@ int launch_face(hipStream_t s, const Grid &g, const FaceOp &op);
 int launch_face_pair(hipStream_t s, const Grid &g, const FaceOp &a, const FaceOp &b);
 template <class T>
 int launch_h_update(hipStream_t s, const FieldsT<T> &f, const Grid &g, const BoxSet &flush,
-                    int full);
+                    int full, double *part = nullptr);
+template <class T>
+int launch_e_fused(hipStream_t s, const FieldsT<T> &f, const Grid &g, const BoxSet &flush);
 template <class T>
 int launch_e_update(hipStream_t s, const FieldsT<T> &f, const Grid &g, const BoxSet &flush,
                     int interior, int do_flush);
@@ -178,14 +180,13 @@ template <class T>
 int launch_probes(hipStream_t s, T *const E[3], T *const H[3], const Grid &g, const int *ijk,
                   int np, double *out);
 template <class T> int launch_energy(hipStream_t s, const FieldsT<T> &f, const Grid &g, double *acc);
-/* Loop-body diagnostics: energy partial sums (energy_parts(g) workgroups,
- * two doubles each) and the ring row {W_E sum, W_M sum, WTx, WTy, WTz} of
- * the device step counter's step. */
-int energy_parts(const Grid &g);
-template <class T>
-int launch_energy_part(hipStream_t s, const FieldsT<T> &f, const Grid &g, double *part);
+/* Loop-body diagnostics: the H update writes em_energy partial sums (two
+ * doubles per workgroup, h_update_blocks(g, fp32) workgroups) when given
+ * 'part'; the ring row {W_E sum, W_M sum, WTx, WTy, WTz} of the device step
+ * counter's step sums them, and with 'tick' advances the counter. */
+int h_update_blocks(const Grid &g, int fp32);
 int launch_diag_ring(hipStream_t s, const double *part, int nparts, const double *wk,
-                     double *ring, const int *step, int cap);
+                     double *ring, int *step, int cap, int tick);
 int launch_zero(hipStream_t s, double *p, long n);
 int launch_zero2(hipStream_t s, double *p, long n, double *q, long m);
 

@@ -522,7 +522,7 @@ extern "C" int mpic_create(const mpic_config_t *cfg, mpic_run_t **out)
     r->timed = -1;
     MPIC_CHECK(hipMalloc(&r->dstep, sizeof(int)));
     MPIC_CHECK(hipMemsetAsync(r->dstep, 0, sizeof(int), r->stream));
-    r->nparts = energy_parts(g);
+    r->nparts = h_update_blocks(g, cfg->precision == MPIC_FP32_SPLIT);
     MPIC_CHECK(hipMalloc(&r->dpart, sizeof(double) * 2 * r->nparts));
     MPIC_CHECK(hipMalloc(&r->dring, sizeof(double) * 5 * DIAG_RING));
     r->ring_step.assign(DIAG_RING, -1);
@@ -1603,9 +1603,18 @@ static int laser_E(mpic_run *r, T *Ey, T *Ez, const Box &tu)
     return MPIC_OK;
 }
 
+/* em_HHalfStep; with 'part' the H update also writes em_energy's partial
+ * sums of E and of the H it leaves (H^n). */
+static int h_half(mpic_run *r, double *part);
+
 extern "C" int mpic_em_h_half(mpic_run_t *r)
 {
     if (!r) return MPIC_ERR_ARG;
+    return h_half(r, nullptr);
+}
+
+static int h_half(mpic_run *r, double *part)
+{
     TRY(check_stage(r, 0));
     MPIC_CHECK(hipSetDevice(r->dev));
     TRY(laser_tables(r, 1));
@@ -1625,7 +1634,7 @@ extern "C" int mpic_em_h_half(mpic_run_t *r)
     FCALL(launch_cap_E, r->g, r->updE);
     TRY(walls_on(r, r->bcE));
     /* em_HSmallStep + cap_flushHalfH */
-    FCALL(launch_h_update, r->g, r->flH, 0);
+    FCALL(launch_h_update, r->g, r->flH, 0, part);
     r->stage = 1;
     return MPIC_OK;
 }
@@ -1756,11 +1765,10 @@ extern "C" int mpic_set_vsp(mpic_run_t *r, double alpha)
     return MPIC_OK;
 }
 
-extern "C" int mpic_em_e_finish(mpic_run_t *r)
+/* The sources em_EStep_finish adds to E: the TFSF interface
+ * (TFSF_postEStep, source/core/em.c:200) and the lasers. */
+static int e_sources(mpic_run *r)
 {
-    if (!r) return MPIC_ERR_ARG;
-    TRY(check_stage(r, 3));
-    MPIC_CHECK(hipSetDevice(r->dev));
     if (r->tfsf.on) {                    /* TFSF_postEStep, source/core/em.c:200 */
         const Tfsf &tf = r->tfsf;
         const double sign = tf.mode == 0 ? 1.0 : -1.0;
@@ -1776,9 +1784,12 @@ extern "C" int mpic_em_e_finish(mpic_run_t *r)
         TRY(laser_E(r, r->f32.E[1], r->f32.E[2], dom));
     else
         TRY(laser_E(r, r->f.E[1], r->f.E[2], dom));
-    /* E -= tau J; the step counter advances in the same launch unless the
-     * spot source still has to read it */
-    FCALL(launch_e_tauJ, r->g, r->spot.on ? nullptr : r->dstep);
+    return MPIC_OK;
+}
+
+/* The hard spot after E -= tau J (gaussSpot_forceE). */
+static int e_spot(mpic_run *r)
+{
     if (r->spot.on) {
         const Spot &sp = r->spot;
         const long np = (long)(r->g.n[1] + 3) * (r->g.n[2] + 3);
@@ -1789,10 +1800,29 @@ extern "C" int mpic_em_e_finish(mpic_run_t *r)
             TRY(launch_spot(r->stream, r->f.E[1], r->f.E[2], r->g, sp.i, sp.dprof, sp.dprof + np,
                             sp.dtab, r->dstep, LASER_CAP));
     }
-    if (r->spot.on) TRY(launch_tick(r->stream, r->dstep));
+    return MPIC_OK;
+}
+
+/* The host clock of a finished step. */
+static void step_done(mpic_run *r)
+{
     r->time = r->time + r->g.tau;          /* parameter_setTime (Time + tau) */
     ++r->step;
     r->stage = 0;
+}
+
+extern "C" int mpic_em_e_finish(mpic_run_t *r)
+{
+    if (!r) return MPIC_ERR_ARG;
+    TRY(check_stage(r, 3));
+    MPIC_CHECK(hipSetDevice(r->dev));
+    TRY(e_sources(r));
+    /* E -= tau J; the step counter advances in the same launch unless the
+     * spot source still has to read it */
+    FCALL(launch_e_tauJ, r->g, r->spot.on ? nullptr : r->dstep);
+    TRY(e_spot(r));
+    if (r->spot.on) TRY(launch_tick(r->stream, r->dstep));
+    step_done(r);
     return MPIC_OK;
 }
 
@@ -1810,18 +1840,33 @@ static int step_body(mpic_run *r)
 {
     if (r->timed >= 0) TRY(record(r, 0));
     if (!r->capturing) mark_ring(r);
-    TRY(mpic_em_h_half(r));
-    /* em_energy after the H half step, plasma_temperature after the push
-     * (source/core/main.c:371,383): one ring row per step, nothing copied
-     * back, so the step stays asynchronous */
-    FCALL(launch_energy_part, r->g, r->dpart);
+    /* em_energy after the H half step (its sums come out of the H update),
+     * plasma_temperature after the push (source/core/main.c:371,383): one
+     * ring row per step, nothing copied back, so the step stays
+     * asynchronous */
+    TRY(h_half(r, r->dpart));
     TRY(mpic_push_deposit(r));
-    TRY(mpic_em_h_step(r));
-    TRY(mpic_em_e_start(r));
+    /* jbc_finish reads and writes J only and the H step does not read J:
+     * the fold goes first, so that E -= tau J can join the E update */
     TRY(mpic_j_finish(r));
+    TRY(mpic_em_h_step(r));
+    /* one pass for em_EStep_start and E -= tau J where no source adds to E
+     * between them (FP64 form), always in the FP32 split form (its TFSF and
+     * laser sources then add after E -= tau J) */
+    const bool fused = r->fp32 || (!r->tfsf.on && r->lasers.empty());
+    if (fused) {
+        FCALL(launch_e_fused, r->g, r->flE);
+        TRY(e_sources(r));
+    } else {
+        TRY(mpic_em_e_start(r));
+        TRY(e_sources(r));
+        FCALL(launch_e_tauJ, r->g, nullptr);
+    }
+    TRY(e_spot(r));
+    /* the ring row of this step, then the device step counter advances */
     TRY(launch_diag_ring(r->stream, r->dpart, r->nparts, r->dacc + 2, r->dring, r->dstep,
-                         DIAG_RING));
-    TRY(mpic_em_e_finish(r));
+                         DIAG_RING, 1));
+    step_done(r);
     if (r->timed >= 0) TRY(record(r, 4));
     return MPIC_OK;
 }
@@ -2294,33 +2339,89 @@ extern "C" int mpic_batch_advance(mpic_run_t *const *runs, int nruns, int nsteps
     return MPIC_OK;
 }
 
+/* Work-item list and FP32 J blocks for items of at most chm markers (the
+ * buffers only grow; graphs and fused ensembles holding the old ones are
+ * dropped). */
+static int ensure_items(mpic_run *r, int chm)
+{
+    const long need = items_capacity(r->tv.ntiles, r->cap, chm);
+    if (need <= r->items_cap) return MPIC_OK;
+    MPIC_CHECK(hipStreamSynchronize(r->stream));
+    int *item = nullptr;
+    float *tj = nullptr;
+    if (hipMalloc(&item, sizeof(int) * need) != hipSuccess ||
+        hipMalloc(&tj, sizeof(float) * TJB * (size_t)need) != hipSuccess) {
+        hipFree(item);
+        mpic_set_error("work items of %d markers: %ld items do not fit on the device", chm, need);
+        return MPIC_ERR_CAPACITY;
+    }
+    hipFree(r->tv.item);
+    hipFree(r->tj);
+    r->tv.item = item;
+    r->tj = tj;
+    r->items_cap = need;
+    invalidate(r);
+    return MPIC_OK;
+}
+
+/* The FP32 push's launch shape: markers per work item (a tile's markers
+ * split into even items of at most that many, or one item per tile) and
+ * workgroup count.  Each candidate runs 'steps' real steps, timed with HIP
+ * events (push + J gather); item sizes whose J blocks would take more than
+ * an eighth of the marker buffers' memory are not tried. */
 extern "C" int mpic_autotune(mpic_run_t *r, int steps, int *chosen)
 {
     if (!r || steps < 1) return MPIC_ERR_ARG;
+    if (!r->inited || r->stage != 0) {
+        mpic_set_error("mpic_autotune: the run must be initialised and between steps");
+        return r->inited ? MPIC_ERR_ORDER : MPIC_ERR_STATE;
+    }
     MPIC_CHECK(hipSetDevice(r->dev));
     if (!r->fp32 || !r->tiled) {
-        if (chosen) *chosen = r->push_blocks;
+        if (chosen) {
+            chosen[0] = r->push_blocks;
+            chosen[1] = 0;
+        }
         return MPIC_OK;
     }
-    int best = r->tile_blocks;
+    const int chms[4] = {0, 2048, 1024, 512};
+    const char *verbose = getenv("MPIC_AUTOTUNE_VERBOSE");
+    int best_nb = r->tile_blocks, best_chm = r->tv.chm;
     double best_ms = 1e300;
-    for (int mult = 4; mult <= 64; mult *= 2) {
-        int nb = mult * r->ncu;
-        if (nb > r->items_cap) nb = (int)r->items_cap;
-        r->tile_blocks = nb;
-        TRY(advance_timed(r, steps));
-        double p, d, rest;
-        TRY(mpic_last_timing(r, &p, &d, &rest));
-        if (p + d < best_ms) {
-            best_ms = p + d;
-            best = nb;
+    for (int c = 0; c < 4; ++c) {
+        const int chm = chms[c];
+        if (chm > 0 && (double)TJB * sizeof(float) / chm > 0.125 * 2 * 36.0) continue;
+        if (ensure_items(r, chm) != MPIC_OK) continue;
+        /* the item list of the current order follows the item size */
+        r->tv.chm = chm;
+        TRY(sort_pass(r, 1));
+        for (int mult = 2; mult <= 64; mult *= 2) {
+            int nb = mult * r->ncu;
+            if (nb > r->items_cap) nb = (int)r->items_cap;
+            r->tile_blocks = nb;
+            TRY(advance_timed(r, steps));
+            double p, d, rest;
+            TRY(mpic_last_timing(r, &p, &d, &rest));
+            if (verbose && verbose[0] == '1')
+                fprintf(stderr, "mpic_autotune: %d markers per item, %d workgroups: push %.1f us, "
+                        "J gather %.1f us per step\n", chm, nb, 1e3 * p / steps, 1e3 * d / steps);
+            if (p + d < best_ms) {
+                best_ms = p + d;
+                best_nb = nb;
+                best_chm = chm;
+            }
+            if (nb == r->items_cap) break;
         }
-        if (nb == r->items_cap) break;
     }
-    r->tile_blocks = best;
+    r->tv.chm = best_chm;
+    r->tile_blocks = best_nb;
+    TRY(sort_pass(r, 1));
     invalidate(r);
     r->ev_steps = 0;
-    if (chosen) *chosen = best;
+    if (chosen) {
+        chosen[0] = best_nb;
+        chosen[1] = best_chm;
+    }
     return MPIC_OK;
 }
 

@@ -5,10 +5,15 @@ over time.  The loop keeps every admitted run resident in HBM (fields,
 markers, tiles, per-run stream and captured step graphs), admits waiting
 requests whenever their device footprint (mpic_footprint) fits the HBM
 budget, and advances all resident runs together in slices of steps with
-mpic_batch_advance, each run replaying its captured step on its own stream
-so that the runs' kernels overlap on the device.  Between slices it retires
-finished runs (their result is read back) and admits new arrivals: a
-request that arrives while others are on the GPU joins at the next slice.
+mpic_batch_advance: the resident runs' steps are fused into one launch per
+kernel kind for all of them (one captured graph per step), so the runs'
+work shares every launch.  Between slices it retires finished runs (their
+result is read back) and admits new arrivals: a request that arrives while
+others are on the GPU joins at the next slice.
+
+Throughput counts the live markers of each run (the mean of its first and
+last count) times its steps, over the whole wall time (the offered load)
+and over the time some run was resident (the busy rate).
 
     python -m mandor3_amd.serve --requests 40 --rate 8 --precision fp32
 
@@ -99,6 +104,7 @@ class Server:
         self.peak_resident = 0
         self.pushes = 0.0
         self.t0 = None
+        self.idle = 0.0
 
     def submit(self, req):
         heapq.heappush(self.waiting, (req.arrival, req.rid, req))
@@ -137,6 +143,8 @@ class Server:
                 continue
             req.result = result_of(req.run, self.full)
             req.done = self._now()
+            steps = int(req.params["steps"])
+            self.pushes += 0.5 * (req.markers + req.result["markers"]) * steps
             req.run.close()
             req.run = None
             self.resident -= req.footprint
@@ -145,11 +153,14 @@ class Server:
 
     def serve(self):
         self.t0 = time.perf_counter()
+        self.idle = 0.0
         while self.waiting or self.active:
             self._admit()
             if not self.active:
+                t = self._now()
                 if self.waiting:
                     time.sleep(max(0.0, min(self.waiting[0][0] - self._now(), 0.05)))
+                self.idle += self._now() - t
                 continue
             # one slice for every resident run: the shortest remaining work
             # bounds it so that finished runs leave at the slice boundary
@@ -157,7 +168,6 @@ class Server:
             batch_advance([r.run for r in self.active], k)
             for r in self.active:
                 r.left -= k
-                self.pushes += r.markers * k
             if any(r.left == 0 for r in self.active):
                 self._retire()
         for r in self.active:
@@ -172,7 +182,10 @@ class Server:
                 "p50_s": float(np.percentile(lat, 50)) if lat.size else None,
                 "p99_s": float(np.percentile(lat, 99)) if lat.size else None,
                 "marker_pushes_per_s": self.pushes / wall if wall > 0 else 0.0,
+                "marker_pushes_per_s_busy": (self.pushes / (wall - self.idle)
+                                             if wall > self.idle else 0.0),
                 "wall_s": wall,
+                "busy_s": wall - self.idle,
                 "peak_resident_gb": self.peak_resident / GB}
 
 

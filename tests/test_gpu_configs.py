@@ -184,11 +184,15 @@ def test_c1_split_form_error_statement():
     assert s64["gauss"] <= 2 * oracle_stat["gauss"] + 1e-14
     assert s64["cont"] <= 2 * oracle_stat["cont"] + 1e-14
     assert abs(s64["drift"] - oracle_stat["drift"]) <= 1e-6 * oracle_stat["drift"]
-    # split form: energy drift within 2x of the oracle's; the rest at FP32
-    # round-off, held to about 10x the measured values
+    # split form: energy drift within 2x of the oracle's; the rest held to
+    # about 10x the values measured on MI355X (E 4.1e-5, H 7.4e-5, J 1.0e-4,
+    # energies 5.8e-6 after 200 steps, E/H 1.4e-6 after 20; Gauss residual
+    # 4.0e-6 of max|rho| against the oracle's 1.9e-13, continuity 8.9e-8 of
+    # max|rho|/tau against 4.0e-14: each form at its own round-off)
     assert s32["drift"] <= 2 * oracle_stat["drift"]
-    assert s32["at20"] <= 1e-5
-    assert s32["gauss"] <= 1e-5 and s32["cont"] <= 1e-5
+    assert s32["at20"] <= 1.5e-5
+    assert s32["E"] <= 4e-4 and s32["H"] <= 7e-4 and s32["J"] <= 1e-3 and s32["W"] <= 6e-5
+    assert s32["gauss"] <= 4e-5 and s32["cont"] <= 1e-6
 
 
 # ------------------------------------------------------------------ C2
@@ -230,9 +234,11 @@ def test_c2_laser_foil_against_oracle():
                       ro[0] / scaleC, r64[0] / scaleC, r32[1] / scaleR, ro[1] / scaleR,
                       r64[1] / scaleR))
     assert abs(r64[0] - ro[0]) <= 1e-9 * ro[0] and abs(r64[1] - ro[1]) <= 1e-9 * ro[1]
+    # about 10x the values measured on MI355X: E 2.1e-6, H 1.5e-6, J 1.1e-5,
+    # energies 3.3e-7, no marker in another cell, positions 4.7e-7 cells
     assert common.size >= 0.9999 * Mo.size
-    assert eE <= 2e-5 and eH <= 2e-5 and eJ <= 1e-4 and eW <= 5e-6
-    assert mism <= 1e-4 and dx <= 2e-5
+    assert eE <= 2e-5 and eH <= 1.5e-5 and eJ <= 1e-4 and eW <= 3.5e-6
+    assert mism <= 1e-5 and dx <= 5e-6
     # markers leave through the open walls: both residuals are physical
     # there, the same in every form
     assert abs(r32[0] - ro[0]) <= 1e-3 * ro[0] and abs(r32[1] - ro[1]) <= 1e-3 * ro[1]
@@ -294,19 +300,26 @@ def scan_draws(runs=64, seed=64):
     return [workloads.scan_request(rng, 0.0, (500, 3000)) for _ in range(runs)]
 
 
-def advance_to_completion(rs, steps, slice_steps=100):
+def advance_to_completion(rs, steps, slice_steps=100, first=None, on_first=None):
     """Advances every run its own number of steps, all resident runs
     together through mpic_batch_advance, in slices bounded by the shortest
-    remaining run."""
+    remaining run; the first slice is 'first' steps long when given, and
+    on_first() is called after it."""
     left = list(steps)
     while True:
         act = [q for q in range(len(rs)) if left[q] > 0]
         if not act:
             return
         k = min(slice_steps, min(left[q] for q in act))
+        if first is not None:
+            k = min(k, first)
         ma.batch_advance([rs[q] for q in act], k)
         for q in act:
             left[q] -= k
+        if first is not None:
+            first = None
+            if on_first:
+                on_first()
 
 
 def test_c4_scan_ensemble_runs_to_completion_and_equals_alone():
@@ -320,8 +333,15 @@ def test_c4_scan_ensemble_runs_to_completion_and_equals_alone():
         pushes += st["markers"].size * p["steps"]
     say("C4: 64 runs set up (%.0f s)" % (time.perf_counter() - t0))
     live0 = [r.marker_count() for r in rs]
+    sampled = (0, 31, 63)
+    early = {}
+
+    def snapshot():
+        for q in sampled:
+            early[q] = rs[q].fields()
+
     t1 = time.perf_counter()
-    advance_to_completion(rs, [p["steps"] for p in draws])
+    advance_to_completion(rs, [p["steps"] for p in draws], first=20, on_first=snapshot)
     for r in rs:
         r.sync()
     wall = time.perf_counter() - t1
@@ -330,21 +350,35 @@ def test_c4_scan_ensemble_runs_to_completion_and_equals_alone():
     for q, (r, p) in enumerate(zip(rs, draws)):
         assert r.time == pytest.approx(p["steps"] * workloads.courant_tau((0.05,) * 3), rel=1e-12)
         assert 0 < r.marker_count() <= live0[q]
-    for q in (0, 31, 63):
+    for q in sampled:
         p = draws[q]
         E, H = rs[q].fields()
         alone = []
-        for _ in range(2):
+        for rep in range(2):
             a = serve.start_run(workloads.foil_request(**p), prec, graphs=False)
-            a.advance(p["steps"])
+            if rep == 0:
+                a.advance(20)
+                E20, H20 = a.fields()
+                a.advance(p["steps"] - 20)
+            else:
+                a.advance(p["steps"])
             a.sync()
             alone.append(a.fields() + (a.marker_count(),))
             a.close()
         (Ea, Ha, na), (Eb, Hb, nb) = alone
+        # after 20 steps, before the laser-plasma interaction amplifies
+        # round-off: the mix equals the run alone to FP32 round-off
+        m20 = max(rel_l2(early[q][0], E20), rel_l2(early[q][1], H20))
+        # at the end the split form's own run-to-run spread (the order of a
+        # cell's markers after the sort is not fixed, so J sums round
+        # differently, and the plasma amplifies it) sets the bar
         floor = max(rel_l2(Eb, Ea), rel_l2(Hb, Ha))
         mix = max(rel_l2(E, Ea), rel_l2(H, Ha))
-        say("C4 run %d (%s, %d steps): mix vs alone %.2e, alone vs alone %.2e, live %d / %d"
-            % (q, "x".join(map(str, p["n"])), p["steps"], mix, floor, rs[q].marker_count(), na))
+        say("C4 run %d (%s, %d steps): mix vs alone after 20 steps %.2e; at the end %.2e, "
+            "alone vs alone %.2e, live %d / %d"
+            % (q, "x".join(map(str, p["n"])), p["steps"], m20, mix, floor,
+               rs[q].marker_count(), na))
+        assert m20 <= 2e-5
         assert mix <= max(3 * floor, 1e-5)
         assert abs(rs[q].marker_count() - na) <= max(2 * abs(na - nb), 0.001 * na)
     for r in rs:
