@@ -159,7 +159,8 @@ def run_gpu(ma, wl, precision, steps, warmup, autotune=True):
             "push_kernel_bytes": alg,
             "push_kernel_GBps": alg / (avg_push * 1e-3) / 1e9,
             "autotune": None if tuned is None else {"workgroups": tuned[0],
-                                                   "markers_per_item": tuned[1]},
+                                                   "wavefronts_per_workgroup": tuned[1],
+                                                   "markers_per_item": tuned[2]},
             "last_diag": [float(v) for v in hist[-1]]}
 
 

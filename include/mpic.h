@@ -178,13 +178,15 @@ int  mpic_set_graphs(mpic_run_t *run, int on);
  * own captured step on its own stream instead.  Every run must be between
  * steps (a step begun through the stage calls: MPIC_ERR_ORDER). */
 int  mpic_batch_advance(mpic_run_t *const *runs, int nruns, int nsteps);
-/* Picks the launch shape of the FP32 push and J gather -- markers per work
- * item (a tile's markers split into even items of at most 2048, 1024 or 512,
- * or one item per tile) and workgroup count -- by timing 'steps' real steps
+/* Picks the launch shape of the FP32 push -- markers per work item (a
+ * tile's markers split into even items of at most 2048 or 1024, or one item
+ * per tile), wavefronts per workgroup (1, 2 or 4 sharing one tile's staged
+ * fields and J block) and workgroup count -- by timing 'steps' real steps
  * per candidate (the steps advance the run; the choice changes results only
  * by the summation order of J).  chosen (may be NULL) receives {workgroups,
- * markers per item (0: one item per tile)}.  The work-item buffers may grow
- * by up to an eighth of the marker buffers' size.  Between steps only. */
+ * wavefronts per workgroup, markers per item (0: one item per tile)}.  The
+ * work-item buffers may grow by up to an eighth of the marker buffers'
+ * size.  Between steps only. */
 int  mpic_autotune(mpic_run_t *run, int steps, int *chosen);
 int  mpic_sync(mpic_run_t *run);
 double mpic_time(mpic_run_t *run);

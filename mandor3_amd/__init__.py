@@ -283,11 +283,12 @@ class Run:
 
     def autotune(self, steps=3):
         """Times `steps` real steps per candidate launch shape of the FP32
-        push (markers per work item x workgroup count) and keeps the fastest;
-        returns (workgroups, markers per item, 0 = one item per tile)."""
-        c = np.zeros(2, dtype=np.int32)
+        push (markers per work item x wavefronts per workgroup x workgroups)
+        and keeps the fastest; returns (workgroups, wavefronts per
+        workgroup, markers per item, 0 = one item per tile)."""
+        c = np.zeros(3, dtype=np.int32)
         _check(lib().mpic_autotune(self.handle, int(steps), _ptr(c)), "mpic_autotune")
-        return int(c[0]), int(c[1])
+        return int(c[0]), int(c[1]), int(c[2])
 
     def sync(self):
         _check(lib().mpic_sync(self.handle), "mpic_sync")

@@ -35,11 +35,10 @@
 #include "mpic_tiles.h"
 
 #define TPB 256
-#define TW 64                  /* the tile push: one wavefront per workgroup */
+#define TW 64                  /* wavefront width */
 #define TILE 4
 #define TFN (TILE + 2)         /* E/H nodes per axis staged for a tile's gathers */
 #define TFN3 (TFN * TFN * TFN)
-#define NSTG ((TFN3 + TW - 1) / TW)  /* staged nodes per lane and component */
 #define SPECIES_SHIFT 29
 #define KEY_MASK ((1 << SPECIES_SHIFT) - 1)
 
@@ -451,13 +450,16 @@ __device__ static inline void write_back(const Markers32 &m, const TileView &tv,
     if (q.ckey >= 0) atomicAdd(tv.count + q.ckey, q.clen);
 }
 
-/* Tiled push: one wavefront per work item, an even share of one active
- * tile's (4x4x4 cells) markers, so that tiles of very different occupancy
- * still keep the wavefront slots busy to the end; items are taken with
- * stride = grid size (a shared queue would serialise on its counter).  The
- * tile's E/H nodes are staged in LDS, the item's J is accumulated in an
- * FP32 LDS block and stored, without atomics, as the item's block in 'tj';
- * the next batch of markers is loaded before the current one is pushed. */
+/* Tiled push: one workgroup of W wavefronts per work item (an even share
+ * of one active tile's (4x4x4 cells) markers, or the whole tile); items are
+ * taken with stride = grid size (a shared queue would serialise on its
+ * counter).  The tile's E/H nodes are staged in LDS once for the W
+ * wavefronts, which push interleaved batches of 64 markers each and add to
+ * one FP32 LDS J block; the block is stored, without atomics, as the item's
+ * block in 'tj'.  More wavefronts per tile keep every SIMD busy when tiles
+ * are few (a 32^3 box has 512) and let the hardware balance tiles of very
+ * different occupancy (workgroups finish at different times and new ones
+ * start); each wavefront loads its next batch before pushing the current. */
 struct Push32Args {
     Markers32    m;
     F32View      v;
@@ -467,10 +469,13 @@ struct Push32Args {
     double      *Wacc;
 };
 
+template <int W>
 __device__ __forceinline__ static void push32_tiles(const Markers32 &m, const F32View &v,
                                            const Push32Consts &c, const TileView &tv, float *tj,
                                            double *Wacc, const VBlock &bk, const VBlock &gd)
 {
+    constexpr int BS = TW * W;                       /* threads of the workgroup */
+    constexpr int NSTG = (TFN3 + BS - 1) / BS;       /* staged nodes per thread and component */
     __shared__ float lj[TJB];
     __shared__ float lf[6 * TFN3];
     __shared__ float lqm[4];
@@ -494,7 +499,7 @@ __device__ __forceinline__ static void push32_tiles(const Markers32 &m, const F3
         tp.edge = !(c0 >= TILE && c0 + 2 * TILE <= g.n[0] && c1 >= TILE && c1 + 2 * TILE <= g.n[1] &&
                     c2 >= TILE && c2 + 2 * TILE <= g.n[2]);
         __syncthreads();                  /* previous tile's LDS reads are done */
-        for (int q = threadIdx.x; q < TJB; q += TW) lj[q] = 0.0f;
+        for (int q = threadIdx.x; q < TJB; q += BS) lj[q] = 0.0f;
         /* nodes [c, c + TFN) per axis; those past the last ghost layer (a
          * partial tile at the high edge) are never read by a gather and are
          * staged as 0.  Every load of a lane is issued before the first LDS
@@ -505,7 +510,7 @@ __device__ __forceinline__ static void push32_tiles(const Markers32 &m, const F3
             bool ok[NSTG];
 #pragma unroll
             for (int r = 0; r < NSTG; ++r) {
-                const int rr = lane + TW * r;
+                const int rr = (int)threadIdx.x + BS * r;
                 const int a = rr / (TFN * TFN), b = (rr / TFN) % TFN, cc = rr % TFN;
                 ok[r] = rr < TFN3 && c0 + a <= g.n[0] + 2 && c1 + b <= g.n[1] + 2 &&
                         c2 + cc <= g.n[2] + 2;
@@ -522,7 +527,7 @@ __device__ __forceinline__ static void push32_tiles(const Markers32 &m, const F3
             for (int comp = 0; comp < 6; ++comp)
 #pragma unroll
                 for (int r = 0; r < NSTG; ++r) {
-                    const int rr = lane + TW * r;
+                    const int rr = (int)threadIdx.x + BS * r;
                     if (rr < TFN3) lf[comp * TFN3 + rr] = ok[r] ? st[comp][r] : 0.0f;
                 }
         }
@@ -539,10 +544,10 @@ __device__ __forceinline__ static void push32_tiles(const Markers32 &m, const F3
         Pending pend;
         pend.valid = false;
         pend.ckey = -1;
-        for (unsigned base = beg; base < end; base += TW) {
+        for (unsigned base = beg; base < end; base += BS) {
             const unsigned p = base + threadIdx.x;
             MarkerRegs nxt;
-            load_marker(m, p + TW, end - 1, nxt);
+            load_marker(m, p + BS, end - 1, nxt);
             write_back(m, tv, pend);
             int rkey = -1, newword = -1, ci = 0, cj = 0, ck = 0;
             Seg12 first;
@@ -580,7 +585,7 @@ __device__ __forceinline__ static void push32_tiles(const Markers32 &m, const F3
         __syncthreads();
         /* the tile's J block, stored whole (coalesced, no atomics) */
         float *dst = tj + (long)it * TJB;
-        for (int q = threadIdx.x; q < TJB; q += TW) dst[q] = lj[q];
+        for (int q = threadIdx.x; q < TJB; q += BS) dst[q] = lj[q];
     }
     double d0 = w0, d1 = w1, d2 = w2;     /* wave sums in FP64 */
     for (int d = 32; d > 0; d >>= 1) {
@@ -595,13 +600,18 @@ __device__ __forceinline__ static void push32_tiles(const Markers32 &m, const F3
     }
 }
 
-/* One wavefront per workgroup, register budget for 4 wavefronts per SIMD. */
-template <> struct KernelDef<Push32Args> {
-    static constexpr int kBlock = TW;
+/* W wavefronts per workgroup (a distinct argument type per W, so that the
+ * fused ensemble launches each shape on its own), register budget for 4
+ * wavefronts per SIMD. */
+template <int W> struct Push32ArgsW : Push32Args {
+};
+
+template <int W> struct KernelDef<Push32ArgsW<W>> {
+    static constexpr int kBlock = TW * W;
     static constexpr int kWaves = 4;
-    __device__ __forceinline__ static void body(const Push32Args &a, VBlock b, VBlock gd)
+    __device__ __forceinline__ static void body(const Push32ArgsW<W> &a, VBlock b, VBlock gd)
     {
-        push32_tiles(a.m, a.v, a.c, a.tv, a.tj, a.Wacc, b, gd);
+        push32_tiles<W>(a.m, a.v, a.c, a.tv, a.tj, a.Wacc, b, gd);
     }
 };
 
@@ -718,7 +728,7 @@ template <> struct KernelDef<JGatherArgs> {
 
 int launch_push32(hipStream_t s, const Markers32 &m, const Fields32 &f, const Grid &g,
                   const PushConsts &pc, const float *qm, const TileView &tv, float *tj,
-                  double *Wacc, int nblocks)
+                  double *Wacc, int nblocks, int waves)
 {
     F32View v;
     for (int a = 0; a < 3; ++a) {
@@ -736,8 +746,15 @@ int launch_push32(hipStream_t s, const Markers32 &m, const Fields32 &f, const Gr
     c.vol = g.h[0] * g.h[1] * g.h[2];
     for (int q = 0; q < 4; ++q) c.qm[q] = qm[q];
     for (int b = 0; b < 6; ++b) c.bc[b] = pc.bc[b];
-    Push32Args a = {m, v, c, tv, tj, Wacc};
-    return launch_def(s, a, (unsigned)nblocks);
+    const Push32Args a = {m, v, c, tv, tj, Wacc};
+    switch (waves) {
+    case 1: return launch_def(s, Push32ArgsW<1>{a}, (unsigned)nblocks);
+    case 2: return launch_def(s, Push32ArgsW<2>{a}, (unsigned)nblocks);
+    case 4: return launch_def(s, Push32ArgsW<4>{a}, (unsigned)nblocks);
+    default:
+        mpic_set_error("launch_push32: %d wavefronts per tile (1, 2 or 4)", waves);
+        return MPIC_ERR_ARG;
+    }
 }
 
 int launch_jgather32(hipStream_t s, const Fields32 &f, const Grid &g, const TileView &tv,

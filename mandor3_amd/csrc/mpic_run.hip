@@ -129,6 +129,8 @@ struct mpic_run {
     int           parity;     /* tv.nmark[parity] = markers in the current order */
     int           push_blocks;  /* FP64: resident workgroups of the persistent push */
     int           tile_blocks;  /* FP32: workgroups of the tile push (work items taken in turn) */
+    int           tile_waves;   /* FP32: wavefronts per workgroup of the tile push */
+    int           gather_blocks;/* FP32: workgroups of the J gather */
     long          items_cap;    /* work items (and FP32 J blocks) the buffers hold */
     int           ncu;          /* compute units of the device */
     float        *tj;           /* FP32: per-tile J blocks, [ntiles][TJB] */
@@ -507,12 +509,16 @@ extern "C" int mpic_create(const mpic_config_t *cfg, mpic_run_t **out)
         /* exactly the resident workgroups of the persistent push: extra ones
          * only arrive once the tile queues are drained */
         r->push_blocks = ncu * push64_blocks_per_cu();
-        /* the resident wavefronts of the tile push (16 per CU); work items
-         * beyond them are taken from a queue as wavefronts finish */
+        /* the resident wavefronts of the tile push (16 per CU), one per
+         * workgroup; a workgroup takes work items with stride = grid size */
         r->tile_blocks = 16 * ncu < r->items_cap ? 16 * ncu : (int)r->items_cap;
+        r->tile_waves = 1;
+        r->gather_blocks = 16 * ncu;
         r->ncu = ncu;
         const char *e = getenv("MPIC_PUSH_BLOCKS");
         if (e && atoi(e) > 0) r->push_blocks = r->tile_blocks = atoi(e);
+        const char *w = getenv("MPIC_TILE_WAVES");
+        if (w && (atoi(w) == 2 || atoi(w) == 4)) r->tile_waves = atoi(w);
     }
     r->n = 0;
     MPIC_CHECK(hipMalloc(&r->dacc, sizeof(double) * 8));
@@ -1670,9 +1676,9 @@ extern "C" int mpic_push_deposit(mpic_run_t *r)
     if (r->timed >= 0) TRY(record(r, 1));
     if (r->fp32) {
         TRY(launch_push32(r->stream, r->m32, r->f32, r->g, r->pc, r->qm, r->tv, r->tj, r->dacc + 2,
-                          r->tile_blocks));
+                          r->tile_blocks, r->tile_waves));
         if (r->timed >= 0) TRY(record(r, 2));
-        TRY(launch_jgather32(r->stream, r->f32, r->g, r->tv, r->tj, r->tile_blocks));
+        TRY(launch_jgather32(r->stream, r->f32, r->g, r->tv, r->tj, r->gather_blocks));
     } else if (r->tiled) {
         TRY(launch_push64_tiled(r->stream, r->m, r->f, r->g, r->pc, r->tv, r->dacc + 2, r->derr,
                                 r->push_blocks));
@@ -2365,10 +2371,11 @@ static int ensure_items(mpic_run *r, int chm)
 }
 
 /* The FP32 push's launch shape: markers per work item (a tile's markers
- * split into even items of at most that many, or one item per tile) and
- * workgroup count.  Each candidate runs 'steps' real steps, timed with HIP
- * events (push + J gather); item sizes whose J blocks would take more than
- * an eighth of the marker buffers' memory are not tried. */
+ * split into even items of at most that many, or one item per tile),
+ * wavefronts per workgroup (one workgroup per item at a time) and workgroup
+ * count.  Each candidate runs 'steps' real steps, timed with HIP events
+ * (push + J gather); item sizes whose J blocks would take more than an
+ * eighth of the marker buffers' memory are not tried. */
 extern "C" int mpic_autotune(mpic_run_t *r, int steps, int *chosen)
 {
     if (!r || steps < 1) return MPIC_ERR_ARG;
@@ -2380,47 +2387,58 @@ extern "C" int mpic_autotune(mpic_run_t *r, int steps, int *chosen)
     if (!r->fp32 || !r->tiled) {
         if (chosen) {
             chosen[0] = r->push_blocks;
-            chosen[1] = 0;
+            chosen[1] = 1;
+            chosen[2] = 0;
         }
         return MPIC_OK;
     }
-    const int chms[4] = {0, 2048, 1024, 512};
+    const int chms[3] = {0, 2048, 1024};
+    const int wavess[3] = {1, 2, 4};
     const char *verbose = getenv("MPIC_AUTOTUNE_VERBOSE");
-    int best_nb = r->tile_blocks, best_chm = r->tv.chm;
+    int best_nb = r->tile_blocks, best_w = r->tile_waves, best_chm = r->tv.chm;
     double best_ms = 1e300;
-    for (int c = 0; c < 4; ++c) {
+    for (int c = 0; c < 3; ++c) {
         const int chm = chms[c];
         if (chm > 0 && (double)TJB * sizeof(float) / chm > 0.125 * 2 * 36.0) continue;
         if (ensure_items(r, chm) != MPIC_OK) continue;
         /* the item list of the current order follows the item size */
         r->tv.chm = chm;
         TRY(sort_pass(r, 1));
-        for (int mult = 2; mult <= 64; mult *= 2) {
-            int nb = mult * r->ncu;
-            if (nb > r->items_cap) nb = (int)r->items_cap;
-            r->tile_blocks = nb;
-            TRY(advance_timed(r, steps));
-            double p, d, rest;
-            TRY(mpic_last_timing(r, &p, &d, &rest));
-            if (verbose && verbose[0] == '1')
-                fprintf(stderr, "mpic_autotune: %d markers per item, %d workgroups: push %.1f us, "
-                        "J gather %.1f us per step\n", chm, nb, 1e3 * p / steps, 1e3 * d / steps);
-            if (p + d < best_ms) {
-                best_ms = p + d;
-                best_nb = nb;
-                best_chm = chm;
+        TRY(advance_timed(r, 1));          /* untimed: first launches of the new lists */
+        for (int wi = 0; wi < 3; ++wi) {
+            const int w = wavess[wi];
+            for (int per_cu = 4; per_cu <= 64; per_cu *= 2) {
+                int nb = per_cu * r->ncu / w;
+                if (nb > r->items_cap) nb = (int)r->items_cap;
+                r->tile_blocks = nb;
+                r->tile_waves = w;
+                TRY(advance_timed(r, steps));
+                double p, d, rest;
+                TRY(mpic_last_timing(r, &p, &d, &rest));
+                if (verbose && verbose[0] == '1')
+                    fprintf(stderr, "mpic_autotune: %d markers per item, %d x %d wavefronts: push "
+                            "%.1f us, J gather %.1f us per step\n", chm, nb, w, 1e3 * p / steps,
+                            1e3 * d / steps);
+                if (p + d < best_ms) {
+                    best_ms = p + d;
+                    best_nb = nb;
+                    best_w = w;
+                    best_chm = chm;
+                }
+                if (nb == r->items_cap) break;
             }
-            if (nb == r->items_cap) break;
         }
     }
     r->tv.chm = best_chm;
     r->tile_blocks = best_nb;
+    r->tile_waves = best_w;
     TRY(sort_pass(r, 1));
     invalidate(r);
     r->ev_steps = 0;
     if (chosen) {
         chosen[0] = best_nb;
-        chosen[1] = best_chm;
+        chosen[1] = best_w;
+        chosen[2] = best_chm;
     }
     return MPIC_OK;
 }

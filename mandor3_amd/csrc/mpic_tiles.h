@@ -129,9 +129,10 @@ int launch_counts_natural(hipStream_t s, const TileView &tv, const Grid &g, int3
 #define TJN 7
 #define TJB (TJN * TJN * TJN * 3)
 
+/* nblocks workgroups of 'waves' (1, 2 or 4) wavefronts each. */
 int launch_push32(hipStream_t s, const Markers32 &m, const Fields32 &f, const Grid &g,
                   const PushConsts &pc, const float *qm, const TileView &tv, float *tj,
-                  double *Wacc, int nblocks);
+                  double *Wacc, int nblocks, int waves);
 int launch_jgather32(hipStream_t s, const Fields32 &f, const Grid &g, const TileView &tv,
                      const float *tj, int nblocks);
 int launch_keys32(hipStream_t s, const Markers32 &m, long n, const TileView &tv, int parity);
