@@ -48,6 +48,10 @@ def parse():
     p.add_argument("--requests", type=int, default=48, help="serving: number of requests")
     p.add_argument("--rate", type=float, default=4.0, help="serving: mean arrivals per second")
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--autotune", action="store_true",
+                   help="pick the FP32 push's launch shape with mpic_autotune before the timed "
+                        "steps (the default shape is one wavefront per tile, 16 workgroups per "
+                        "CU)")
     p.add_argument("--cpu-seconds", type=float, default=10.0,
                    help="CPU baseline: seconds of oracle steps per core")
     p.add_argument("--cpu-cores", type=int, default=0,
@@ -108,18 +112,22 @@ def measured_traffic(kernel, workload):
     counter summary (rocprofv3 FETCH_SIZE and WRITE_SIZE runs, each scaled
     by the same counter's reading for a plain copy of known size), or None
     when no summary covers this kernel and workload."""
-    path = os.path.join(ROOT, "profiles", "traffic_%s.json" % kernel)
+    path = os.path.join(ROOT, "profiles", "push_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
         recs = json.load(f)
-    for rec in recs if isinstance(recs, list) else [recs]:
-        if rec.get("workload") == workload:
+    for rec in recs:
+        if rec.get("kernel") == kernel and rec.get("workload") == workload:
             return rec.get("bytes_per_launch")
     return None
 
 
-def run_gpu(ma, wl, precision, steps, warmup, autotune=True):
+PUSH32_KERNEL = "k_one_w<Push32ArgsW<%d> >"     # the FP32 push as the kernel trace names it
+PUSH64_KERNEL = "k_one<Push64TArgs>"
+
+
+def run_gpu(ma, wl, precision, steps, warmup, autotune=False):
     prec = ma.FP64 if precision == "fp64" else ma.FP32_SPLIT
     M = wl["markers"]
     r = ma.Run(wl["n"], wl["h"], wl["L"], wl["tau"], wl["bc"], 0.0, precision=prec,
@@ -158,9 +166,12 @@ def run_gpu(ma, wl, precision, steps, warmup, autotune=True):
             "rest_of_step_ms": rest_ms / steps,
             "push_kernel_bytes": alg,
             "push_kernel_GBps": alg / (avg_push * 1e-3) / 1e9,
-            "autotune": None if tuned is None else {"workgroups": tuned[0],
-                                                   "wavefronts_per_workgroup": tuned[1],
-                                                   "markers_per_item": tuned[2]},
+            "launch_shape": ({"workgroups": tuned[0], "wavefronts_per_workgroup": tuned[1],
+                              "markers_per_item": tuned[2], "picked_by": "mpic_autotune"}
+                             if tuned is not None else
+                             {"workgroups": "16 per CU", "wavefronts_per_workgroup": 1,
+                              "markers_per_item": 0, "picked_by": "default"}
+                             if precision == "fp32" else None),
             "last_diag": [float(v) for v in hist[-1]]}
 
 
@@ -400,9 +411,9 @@ def main_large(a):
                    "cell_updates_per_s": int(np.prod(wl["n"])) * a.steps / wall,
                    "push_kernel_ms": avg_push, "deposit_gather_ms": dep_ms / a.steps,
                    "rest_of_step_ms": rest_ms / a.steps, "setup_s": t1 - t0},
-        "roofline": {"bound": "hbm", "kernel": "k_push32_tiles", "achieved": gbs,
+        "roofline": {"bound": "hbm", "kernel": PUSH32_KERNEL % 1, "achieved": gbs,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
-                     "traffic": measured_traffic("k_push32_tiles", wl["name"])},
+                     "traffic": measured_traffic(PUSH32_KERNEL % 1, wl["name"])},
         "checks": {"sort_counts_sum_to_live": counts_ok,
                    "markers_only_removed": bool(live_end <= live)}}))
 
@@ -429,11 +440,12 @@ def main():
         copy_calibration(a.copy_calibration)
     wl = workloads.laser_foil() if a.workload == "laser-foil" else workloads.two_stream_box()
     main_prec = "fp64" if a.precision == "fp64" else "fp32"
-    res = {main_prec: run_gpu(ma, wl, main_prec, a.steps, a.warmup)}
+    res = {main_prec: run_gpu(ma, wl, main_prec, a.steps, a.warmup, a.autotune)}
     if a.precision == "both":
         res["fp64"] = run_gpu(ma, wl, "fp64", a.steps, a.warmup)
     m = res[main_prec]
-    kernel = "k_push32_tiles" if main_prec == "fp32" else "k_push64_tiled"
+    kernel = (PUSH32_KERNEL % (m["launch_shape"]["wavefronts_per_workgroup"])
+              if main_prec == "fp32" else PUSH64_KERNEL)
     out = {
         "metric": "marker-pushes/s incl. deposit+FDTD, FP32 vs FP64 on 1 GPU; p50/p99 run latency",
         "value": m["pushes_per_s"],
@@ -454,7 +466,7 @@ def main():
                    "push_kernel_ms": m["push_kernel_ms"],
                    "deposit_gather_ms": m["deposit_rest_ms"],
                    "rest_of_step_ms": m["rest_of_step_ms"],
-                   "push_launch_shape": m["autotune"]},
+                   "push_launch_shape": m["launch_shape"]},
         "roofline": {"bound": "hbm", "kernel": kernel, "achieved": m["push_kernel_GBps"],
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": m["push_kernel_GBps"] / HBM_PEAK_GBS,

@@ -246,9 +246,11 @@ int  mpic_sort(mpic_run_t *run, int32_t *counts);
 int  mpic_marker_cells(mpic_run_t *run, int32_t *cells, long *n);
 
 /* Kernel timing of the last mpic_advance, from HIP events on the run's
- * stream (waits for those steps to finish): milliseconds, summed over the
- * steps, in the push kernel, in the rest of the deposit (the FP32 form's J
- * gather; 0 for FP64) and in everything else of the step. */
+ * stream (three per step: before the push, after it, after the deposit;
+ * waits for those steps to finish): milliseconds, summed over the steps, in
+ * the push kernel, in the rest of the deposit (the FP32 form's J gather; 0
+ * for FP64) and in everything else of the step (from the deposit's end to
+ * the next push, scaled to all the steps). */
 int  mpic_last_timing(mpic_run_t *run, double *push_ms, double *deposit_ms, double *rest_ms);
 
 #ifdef __cplusplus

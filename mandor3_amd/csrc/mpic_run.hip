@@ -1840,11 +1840,10 @@ static void mark_ring(mpic_run *r)
     r->ring_time[slot] = r->time;
 }
 
-/* One step; with idx >= 0 its 5 events (start, push kernel start/end,
- * deposit end, step end) are recorded into the pool. */
+/* One step; with r->timed >= 0 its events (before the push, after it,
+ * after the deposit: slots 1, 2, 3 of the step's five) are recorded. */
 static int step_body(mpic_run *r)
 {
-    if (r->timed >= 0) TRY(record(r, 0));
     if (!r->capturing) mark_ring(r);
     /* em_energy after the H half step (its sums come out of the H update),
      * plasma_temperature after the push (source/core/main.c:371,383): one
@@ -1873,7 +1872,6 @@ static int step_body(mpic_run *r)
     TRY(launch_diag_ring(r->stream, r->dpart, r->nparts, r->dacc + 2, r->dring, r->dstep,
                          DIAG_RING, 1));
     step_done(r);
-    if (r->timed >= 0) TRY(record(r, 4));
     return MPIC_OK;
 }
 
@@ -2469,20 +2467,24 @@ extern "C" int mpic_last_timing(mpic_run_t *r, double *push_ms, double *deposit_
 {
     if (!r) return MPIC_ERR_ARG;
     double t[3] = {0, 0, 0};
-    if (r->ev_steps > 0) {
+    const int n = r->ev_steps;
+    if (n > 0) {
         MPIC_CHECK(hipSetDevice(r->dev));
-        MPIC_CHECK(hipEventSynchronize(r->evpool[5 * r->ev_steps - 1]));
-        for (int s = 0; s < r->ev_steps; ++s) {
+        MPIC_CHECK(hipEventSynchronize(r->evpool[5 * (n - 1) + 3]));
+        for (int s = 0; s < n; ++s) {
             const hipEvent_t *e = &r->evpool[5 * s];
-            float a, b, c, d;
-            MPIC_CHECK(hipEventElapsedTime(&a, e[0], e[1]));
+            float b, c;
             MPIC_CHECK(hipEventElapsedTime(&b, e[1], e[2]));
             MPIC_CHECK(hipEventElapsedTime(&c, e[2], e[3]));
-            MPIC_CHECK(hipEventElapsedTime(&d, e[3], e[4]));
             t[0] += b;
             t[1] += c;
-            t[2] += a + d;
+            if (s + 1 < n) {              /* from this step's deposit end to the next push */
+                float a;
+                MPIC_CHECK(hipEventElapsedTime(&a, e[3], r->evpool[5 * (s + 1) + 1]));
+                t[2] += a;
+            }
         }
+        if (n > 1) t[2] *= (double)n / (n - 1);
     }
     if (push_ms) *push_ms = t[0];
     if (deposit_ms) *deposit_ms = t[1];

@@ -90,7 +90,8 @@ def test_fp32_vacuum_fields(case):
         ok = np.isfinite(Eo)
         worst = max(worst, rel_l2(Eg[ok], Eo[ok]), rel_l2(Hg[np.isfinite(Ho)], Ho[np.isfinite(Ho)]))
     print("fp32 vacuum %s: worst field rel L2 %.2e" % (case, worst))
-    assert worst < 1e-5
+    # measured 2.2e-7 at most (FP32 storage of the fields)
+    assert worst < 3e-6
 
 
 @pytest.mark.parametrize("case", CASES)
@@ -248,8 +249,10 @@ def test_fp32_energy_drift_and_gauss_against_fp64():
               % ("fp64" if prec == ma.FP64 else "fp32", drift, res[0] / scale))
     d64, c64 = out[ma.FP64]
     d32, c32 = out[ma.FP32_SPLIT]
+    # drift: both forms 1.03e-2; continuity 1.0e-7 (split) and 3.9e-14 (FP64)
+    # of max rho/tau, each at its own round-off
     assert d32 <= 2.0 * d64 + 1e-6
-    assert c64 < 1e-12 and c32 < 1e-5
+    assert c64 < 1e-12 and c32 < 1e-6
 
 
 def test_laser_foil_full_size_split_against_fp64():
@@ -299,9 +302,10 @@ def test_laser_foil_full_size_split_against_fp64():
     eE, eH = rel_l2(E32, E64), rel_l2(H32, H64)
     eW = [abs(d32[q] - d64[q]) / max(abs(d64[q]), 1e-300) for q in (1, 2, 3, 4, 5)]
     print("split vs fp64 after 20 steps: E %.2e H %.2e, energies %s" % (eE, eH, ["%.1e" % v for v in eW]))
+    # about 10x the measured E 2.1e-6, H 1.5e-6, energies 3.3e-7
     assert abs(c32 - c64) <= 1e-4 * c64
-    assert eE <= 2e-4 and eH <= 2e-4
-    assert max(eW) <= 2e-4
+    assert eE <= 2e-5 and eH <= 1.5e-5
+    assert max(eW) <= 3.5e-6
 
 
 @pytest.mark.parametrize("prec", [ma.FP64, ma.FP32_SPLIT])
