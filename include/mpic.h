@@ -21,6 +21,11 @@
  * code, with a message in mpic_last_error().  The C host driver maps a
  * failure to "log + exit(1)" like the reference.
  *
+ * Threads: each run has its own stream and state, the last error is per
+ * thread, and the fused ensemble cache is guarded by a lock, so one thread
+ * may create, load, initialise and close runs while another advances other
+ * runs (the serving loop does).  Calls naming the same run must not overlap.
+ *
  * Layouts at this boundary are the reference's binData layouts:
  *   fields  node block [0,n] per axis, interleaved {x,y,z} doubles, k
  *           fastest (the payload of sys_E/H_*.bin, source/type_mesh.c:285-311)

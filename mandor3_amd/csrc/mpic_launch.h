@@ -14,9 +14,10 @@
  *
  * launch_def<A>() launches a body on a stream, or, while a Recorder is set
  * on the calling thread, appends the launch to it as data (kind, grid,
- * argument bytes).  The ensemble code records one step of every run and
- * fuses the launches of equal kind across runs into k_multi launches: one
- * launch per stage per step for the whole ensemble.
+ * argument bytes).  A run's step graphs are the recorded launches as
+ * kernel nodes; the ensemble code records one step of every run and fuses
+ * the launches of equal kind across runs into k_multi nodes: one launch
+ * per stage per step for the whole ensemble.
  */
 #ifndef MPIC_LAUNCH_H
 #define MPIC_LAUNCH_H
@@ -113,12 +114,14 @@ struct LaunchRec {
     const void *kind;                 /* identifies the body (KernelDef<A>::tag) */
     unsigned    grid[3];
     std::vector<unsigned char> args;
-    /* k_multi<A> over n runs: dargs = device array of n argument structs,
-     * dtab = the block table, blocks = tab[n] */
-    int (*multi)(hipStream_t s, const void *dargs, const unsigned *dtab, int n,
-                 unsigned blocks);
-    /* k_one<A> of this record */
-    int (*one)(hipStream_t s, const void *args, const unsigned grid[3]);
+    /* k_multi<A> over n runs as a kernel node of graph gr, after node dep
+     * (none: nullptr): dargs = device array of n argument structs, dtab =
+     * the block table, blocks = tab[n] */
+    int (*multi_node)(hipGraph_t gr, hipGraphNode_t dep, hipGraphNode_t *node, const void *dargs,
+                      const unsigned *dtab, int n, unsigned blocks);
+    /* k_one<A> of this record as a kernel node */
+    int (*one_node)(hipGraph_t gr, hipGraphNode_t dep, hipGraphNode_t *node, const void *args,
+                    const unsigned grid[3]);
 };
 
 struct Recorder {
@@ -133,26 +136,55 @@ template <class A> struct KernelTag {
 };
 template <class A> const char KernelTag<A>::tag = 0;
 
-template <class A>
-static int multi_launch(hipStream_t s, const void *dargs, const unsigned *dtab, int n,
-                        unsigned blocks)
+/* Graphs are built node by node from recorded launches, not by stream
+ * capture: building one then needs no stream, and device calls made by
+ * other host threads meanwhile cannot invalidate it.  The node copies the
+ * kernel arguments. */
+static inline int add_kernel_node(hipGraph_t gr, hipGraphNode_t dep, hipGraphNode_t *node,
+                                  const void *func, dim3 grid, unsigned block, void **params)
 {
-    if constexpr (WavesOf<A>::v > 0)
-        k_multi_w<A><<<blocks, KernelDef<A>::kBlock, 0, s>>>((const A *)dargs, dtab, n);
-    else
-        k_multi<A><<<blocks, KernelDef<A>::kBlock, 0, s>>>((const A *)dargs, dtab, n);
-    MPIC_CHECK(hipGetLastError());
+    hipKernelNodeParams p;
+    memset(&p, 0, sizeof(p));
+    p.func = (void *)func;
+    p.gridDim = grid;
+    p.blockDim = dim3(block);
+    p.sharedMemBytes = 0;
+    p.kernelParams = params;
+    p.extra = nullptr;
+    MPIC_CHECK(hipGraphAddKernelNode(node, gr, dep ? &dep : nullptr, dep ? 1 : 0, &p));
     return MPIC_OK;
 }
 
 template <class A>
-static int one_launch(hipStream_t s, const void *args, const unsigned grid[3])
+static int multi_node(hipGraph_t gr, hipGraphNode_t dep, hipGraphNode_t *node, const void *dargs,
+                      const unsigned *dtab, int n, unsigned blocks)
+{
+    const A *pa = (const A *)dargs;
+    const unsigned *pt = dtab;
+    int pn = n;
+    void *params[3] = {(void *)&pa, (void *)&pt, (void *)&pn};
+    const void *f;
+    if constexpr (WavesOf<A>::v > 0)
+        f = (const void *)&k_multi_w<A>;
+    else
+        f = (const void *)&k_multi<A>;
+    return add_kernel_node(gr, dep, node, f, dim3(blocks), KernelDef<A>::kBlock, params);
+}
+
+template <class A>
+static int one_node(hipGraph_t gr, hipGraphNode_t dep, hipGraphNode_t *node, const void *args,
+                    const unsigned grid[3])
 {
     A a;
     memcpy(&a, args, sizeof(A));
-    start_one(s, a, dim3(grid[0], grid[1], grid[2]));
-    MPIC_CHECK(hipGetLastError());
-    return MPIC_OK;
+    void *params[1] = {(void *)&a};
+    const void *f;
+    if constexpr (WavesOf<A>::v > 0)
+        f = (const void *)&k_one_w<A>;
+    else
+        f = (const void *)&k_one<A>;
+    return add_kernel_node(gr, dep, node, f, dim3(grid[0], grid[1], grid[2]), KernelDef<A>::kBlock,
+                           params);
 }
 
 /* Launches (or records) body A over a gx x gy x gz grid; an empty grid is a
@@ -169,8 +201,8 @@ static int launch_def(hipStream_t s, const A &a, unsigned gx, unsigned gy = 1, u
         r.grid[2] = gz;
         r.args.resize(sizeof(A));
         memcpy(r.args.data(), &a, sizeof(A));
-        r.multi = &multi_launch<A>;
-        r.one = &one_launch<A>;
+        r.multi_node = &multi_node<A>;
+        r.one_node = &one_node<A>;
         g_recorder->recs.push_back(std::move(r));
         return MPIC_OK;
     }

@@ -170,7 +170,7 @@ struct mpic_run {
     double       *dring;      /* DIAG_RING rows {W_E sum, W_M sum, WTx, WTy, WTz} */
     std::vector<long>   ring_step;  /* step whose row a slot holds (-1: none) */
     std::vector<double> ring_time;  /* Time at the start of that step */
-    int           capturing;  /* step_body is being captured, not run */
+    int           capturing;  /* step_body is being recorded into graphs, not run */
     double        vsp;        /* VSP convolution weight alpha (1: off) */
     long          topo;       /* bumped whenever the launch sequence of a step changes */
 };
@@ -1927,10 +1927,33 @@ static void host_step(mpic_run *r)
     r->stage = 0;
 }
 
-/* Captures one step from each marker-buffer parity into exec[0..1]: the
+static int record_steps(mpic_run *r, Recorder rec[2]);
+
+/* The recorded launches as a chain of kernel nodes, instantiated. */
+static int graph_of(const Recorder &rec, hipGraphExec_t *exec)
+{
+    hipGraph_t gr = nullptr;
+    MPIC_CHECK(hipGraphCreate(&gr, 0));
+    hipGraphNode_t prev = nullptr;
+    int rc = MPIC_OK;
+    for (const LaunchRec &x : rec.recs) {
+        hipGraphNode_t node = nullptr;
+        rc = x.one_node(gr, prev, &node, x.args.data(), x.grid);
+        if (rc != MPIC_OK) break;
+        prev = node;
+    }
+    if (rc == MPIC_OK && hipGraphInstantiate(exec, gr, nullptr, nullptr, 0) != hipSuccess) {
+        mpic_set_error("hipGraphInstantiate failed");
+        rc = MPIC_ERR_DEVICE;
+    }
+    hipGraphDestroy(gr);
+    return rc;
+}
+
+/* Graphs of one step from each marker-buffer parity in exec[0..1]: the
  * kernel arguments of a step differ only by which buffer holds the sorted
- * markers, so two graphs replay every step.  The host state is restored
- * afterwards (capturing executes nothing). */
+ * markers, so two graphs replay every step.  The step is recorded, not run,
+ * and the host state is restored afterwards. */
 static int capture(mpic_run *r)
 {
     if (!r->tiled || !r->inited || r->stage != 0) {
@@ -1943,41 +1966,10 @@ static int capture(mpic_run *r)
             hipGraphExecDestroy(r->exec[k]);
             r->exec[k] = nullptr;
         }
-    const double time = r->time;
-    const long step = r->step;
-    const int cur = r->cur, parity = r->parity;
-    const Markers32 m32 = r->m32;
-    const Markers64 m = r->m;
     r->exec_parity = r->parity;
-    int rc = MPIC_OK;
-    for (int k = 0; k < 2 && rc == MPIC_OK; ++k) {
-        hipGraph_t gr = nullptr;
-        if (hipStreamBeginCapture(r->stream, hipStreamCaptureModeRelaxed) != hipSuccess) {
-            mpic_set_error("hipStreamBeginCapture failed");
-            rc = MPIC_ERR_DEVICE;
-            break;
-        }
-        r->capturing = 1;
-        rc = step_async(r, -1);
-        r->capturing = 0;
-        const hipError_t e = hipStreamEndCapture(r->stream, &gr);
-        if (rc == MPIC_OK && e != hipSuccess) {
-            mpic_set_error("hipStreamEndCapture: %s", hipGetErrorString(e));
-            rc = MPIC_ERR_DEVICE;
-        }
-        if (rc == MPIC_OK && hipGraphInstantiate(&r->exec[k], gr, nullptr, nullptr, 0) != hipSuccess) {
-            mpic_set_error("hipGraphInstantiate failed");
-            rc = MPIC_ERR_DEVICE;
-        }
-        if (gr) hipGraphDestroy(gr);
-    }
-    r->time = time;
-    r->step = step;
-    r->cur = cur;
-    r->parity = parity;
-    r->m32 = m32;
-    r->m = m;
-    r->stage = 0;
+    Recorder two[2];
+    int rc = record_steps(r, two);
+    for (int k = 0; k < 2 && rc == MPIC_OK; ++k) rc = graph_of(two[k], &r->exec[k]);
     r->graphs_ok = rc == MPIC_OK;
     return rc;
 }
@@ -2221,33 +2213,31 @@ static int ensemble_build(mpic_run *const *runs, int nruns, Ensemble **out)
         }
         e->ev.push_back(v);
     }
-    /* one graph per phase: the fused launches in order */
+    /* one graph per phase: the fused launches in order, as kernel nodes */
     for (int k = 0; k < 2 && rc == MPIC_OK; ++k) {
         hipGraph_t gr = nullptr;
-        if (hipStreamBeginCapture(e->stream, hipStreamCaptureModeRelaxed) != hipSuccess) {
-            mpic_set_error("fused ensemble: hipStreamBeginCapture failed");
+        if (hipGraphCreate(&gr, 0) != hipSuccess) {
+            mpic_set_error("fused ensemble: hipGraphCreate failed");
             rc = MPIC_ERR_DEVICE;
             break;
         }
+        hipGraphNode_t prev = nullptr;
         for (size_t u = 0; u < fl[k].size() && rc == MPIC_OK; ++u) {
             const FusedLaunch &f = fl[k][u];
             const unsigned char *base = (const unsigned char *)e->dmem;
+            hipGraphNode_t node = nullptr;
             if (f.n == 1)
-                rc = f.rec->one(e->stream, f.rec->args.data(), f.rec->grid);
+                rc = f.rec->one_node(gr, prev, &node, f.rec->args.data(), f.rec->grid);
             else
-                rc = f.rec->multi(e->stream, base + f.args_off, (const unsigned *)(base + f.tab_off),
-                                  f.n, f.blocks);
-        }
-        const hipError_t ce = hipStreamEndCapture(e->stream, &gr);
-        if (rc == MPIC_OK && ce != hipSuccess) {
-            mpic_set_error("fused ensemble: hipStreamEndCapture: %s", hipGetErrorString(ce));
-            rc = MPIC_ERR_DEVICE;
+                rc = f.rec->multi_node(gr, prev, &node, base + f.args_off,
+                                       (const unsigned *)(base + f.tab_off), f.n, f.blocks);
+            prev = node;
         }
         if (rc == MPIC_OK && hipGraphInstantiate(&e->exec[k], gr, nullptr, nullptr, 0) != hipSuccess) {
             mpic_set_error("fused ensemble: hipGraphInstantiate failed");
             rc = MPIC_ERR_DEVICE;
         }
-        if (gr) hipGraphDestroy(gr);
+        hipGraphDestroy(gr);
     }
     if (rc != MPIC_OK) {
         ensemble_free(e);

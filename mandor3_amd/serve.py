@@ -8,8 +8,9 @@ budget, and advances all resident runs together in slices of steps with
 mpic_batch_advance: the resident runs' steps are fused into one launch per
 kernel kind for all of them (one captured graph per step), so the runs'
 work shares every launch.  Between slices it retires finished runs (their
-result is read back) and admits new arrivals: a request that arrives while
-others are on the GPU joins at the next slice.
+result is read back) and makes prepared arrivals resident: a request is
+prepared on an admission thread (host set-up, upload, initial sort) while
+the resident runs advance, and joins at the next slice.
 
 Throughput counts the live markers of each run (the mean of its first and
 last count) times its steps, over the whole wall time (the offered load)
@@ -23,7 +24,10 @@ and the admission statistics.
 import argparse
 import heapq
 import json
+import queue
+import threading
 import time
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
@@ -92,6 +96,15 @@ def result_of(run, full):
 
 
 class Server:
+    """The serving loop.  One admission thread prepares the arrived requests
+    in arrival order -- the request's markers and laser map on the host, its
+    device footprint, then a reservation of that footprint against the HBM
+    budget (it waits while the resident runs leave too little) and the run
+    itself (allocation, upload, initial sort, captured step graphs) -- while
+    the main thread advances the resident runs slice by slice; a prepared
+    run joins at the next slice boundary.  Run set-up and the slices of
+    other runs overlap on the device: each run has its own stream."""
+
     def __init__(self, precision=FP32_SPLIT, budget_gb=240.0, slice_steps=25, full_results=False):
         self.precision = precision
         self.budget = int(budget_gb * GB)
@@ -100,11 +113,16 @@ class Server:
         self.waiting = []          # heap of (arrival, rid, request)
         self.active = []
         self.finished = []
-        self.resident = 0
+        self.resident = 0          # reserved device bytes (prepared and resident runs)
         self.peak_resident = 0
         self.pushes = 0.0
         self.t0 = None
         self.idle = 0.0
+        self._cv = threading.Condition()
+        self._ready = queue.Queue()
+        self._pending = 0          # requests handed to the admission thread
+        self._pool = None
+        self._closing = False
 
     def submit(self, req):
         heapq.heappush(self.waiting, (req.arrival, req.rid, req))
@@ -112,27 +130,56 @@ class Server:
     def _now(self):
         return time.perf_counter() - self.t0
 
-    def _admit(self):
-        """Admit arrived requests in arrival order while they fit."""
-        while self.waiting and self.waiting[0][0] <= self._now():
-            req = self.waiting[0][2]
+    def _prepare(self, req):
+        """Admission thread: set-up, footprint, reservation, run start."""
+        try:
             state = workloads.foil_request(**req.params)
             need = footprint(state, self.precision)
             if need > self.budget:
-                heapq.heappop(self.waiting)
-                req.done = self._now()
                 req.result = {"error": "footprint %.1f GB exceeds the budget" % (need / GB)}
+                self._ready.put(req)
+                return
+            with self._cv:
+                while self.resident + need > self.budget and not self._closing:
+                    self._cv.wait()                    # until a resident run retires
+                if self._closing:
+                    return
+                self.resident += need
+                self.peak_resident = max(self.peak_resident, self.resident)
+            req.footprint = need
+            req.markers = int(state["markers"].size)
+            req.run = start_run(state, self.precision)
+            self._ready.put(req)
+        except BaseException as e:                     # re-raised by the main thread
+            self._ready.put((req, e))
+
+    def _release(self, need):
+        with self._cv:
+            self.resident -= need
+            self._cv.notify_all()
+
+    def _admit(self, block_s=0.0):
+        """Hands arrived requests to the admission thread and makes the
+        prepared runs resident; waits up to block_s for one to be ready."""
+        while self.waiting and self.waiting[0][0] <= self._now():
+            req = heapq.heappop(self.waiting)[2]
+            self._pending += 1
+            self._pool.submit(self._prepare, req)
+        while self._pending:
+            try:
+                item = self._ready.get(timeout=block_s) if block_s > 0 else self._ready.get_nowait()
+            except queue.Empty:
+                return
+            block_s = 0.0
+            self._pending -= 1
+            if isinstance(item, tuple):
+                raise item[1]
+            req = item
+            if req.result is not None:                 # refused: larger than the budget
+                req.done = self._now()
                 self.finished.append(req)
                 continue
-            if self.resident + need > self.budget:
-                break                                  # waits for memory to free up
-            heapq.heappop(self.waiting)
-            req.run = start_run(state, self.precision)
-            req.markers = int(state["markers"].size)
-            req.footprint = need
             req.start = self._now()
-            self.resident += need
-            self.peak_resident = max(self.peak_resident, self.resident)
             self.active.append(req)
 
     def _retire(self):
@@ -147,31 +194,47 @@ class Server:
             self.pushes += 0.5 * (req.markers + req.result["markers"]) * steps
             req.run.close()
             req.run = None
-            self.resident -= req.footprint
+            self._release(req.footprint)
             self.finished.append(req)
         self.active = keep
 
     def serve(self):
         self.t0 = time.perf_counter()
         self.idle = 0.0
-        while self.waiting or self.active:
-            self._admit()
-            if not self.active:
-                t = self._now()
-                if self.waiting:
-                    time.sleep(max(0.0, min(self.waiting[0][0] - self._now(), 0.05)))
-                self.idle += self._now() - t
-                continue
-            # one slice for every resident run: the shortest remaining work
-            # bounds it so that finished runs leave at the slice boundary
-            k = min(self.slice, min(r.left for r in self.active))
-            batch_advance([r.run for r in self.active], k)
-            for r in self.active:
-                r.left -= k
-            if any(r.left == 0 for r in self.active):
-                self._retire()
-        for r in self.active:
-            r.run.sync()
+        self._pool = ThreadPoolExecutor(max_workers=1, thread_name_prefix="admission")
+        try:
+            while self.waiting or self._pending or self.active:
+                self._admit()
+                if not self.active:
+                    t = self._now()
+                    nxt = self.waiting[0][0] - t if self.waiting else 0.05
+                    if self._pending:
+                        self._admit(block_s=min(max(nxt, 0.001), 0.05))
+                    elif self.waiting:
+                        time.sleep(max(0.0, min(nxt, 0.05)))
+                    self.idle += self._now() - t
+                    continue
+                # one slice for every resident run: the shortest remaining
+                # work bounds it so that finished runs leave at the slice
+                # boundary; the slice is waited for, so that a run prepared
+                # meanwhile joins at the next boundary
+                k = min(self.slice, min(r.left for r in self.active))
+                batch_advance([r.run for r in self.active], k)
+                self.active[0].run.sync()
+                for r in self.active:
+                    r.left -= k
+                if any(r.left == 0 for r in self.active):
+                    self._retire()
+        finally:
+            with self._cv:                             # an admission waiting for memory ends
+                self._closing = True
+                self._cv.notify_all()
+            self._pool.shutdown(wait=True)
+            while not self._ready.empty():             # prepared runs never made resident
+                item = self._ready.get_nowait()
+                if not isinstance(item, tuple) and item.run is not None:
+                    item.run.close()
+                    item.run = None
         return self.report()
 
     def report(self):

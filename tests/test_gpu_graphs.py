@@ -173,3 +173,61 @@ def test_diag_history_equals_stepwise_diagnostics(prec, graphs):
     h2 = b.diag_history(4)
     assert h2.shape == (4, 6)
     assert np.array_equal(h2[:2, 0], ra[12:14, 0]) and h2[2, 0] > ra[13, 0] + 1.5 * fx["tau"][0]
+
+
+@pytest.mark.parametrize("prec", [ma.FP64, ma.FP32_SPLIT])
+def test_runs_set_up_on_another_thread_while_an_ensemble_advances(prec):
+    """The serving pattern: one thread creates, loads, initialises, replays
+    and closes runs while another advances an ensemble whose membership
+    changes every call (so its fused graphs are rebuilt meanwhile); both
+    sides give what they give alone."""
+    import threading
+    cases = ("laser_mur", "periodic", "mixed")
+    fxs = [load_fixture(c) for c in cases]
+    alone = [make(fx, prec) for fx in fxs]
+    mix = [make(fx, prec) for fx in fxs]
+    side_fx = load_fixture("spot")
+    side_ref = make(side_fx, prec)
+    for _ in range(6):
+        side_ref.step(diag=False)
+    side_ref.sync()
+    E_ref, H_ref = side_ref.fields()
+    side_ref.close()
+    errors, side_out = [], []
+
+    def side():
+        try:
+            for _ in range(4):
+                r = make(side_fx, prec, graphs=True)
+                r.advance(6)
+                r.sync()
+                side_out.append(r.fields())
+                r.close()
+        except BaseException as e:          # reported by the main thread
+            errors.append(e)
+
+    t = threading.Thread(target=side)
+    t.start()
+    done = [0, 0, 0]
+    for s in range(12):
+        members = [0, 1, 2] if s % 2 == 0 else [0, 2]
+        ma.batch_advance([mix[q] for q in members], 2)
+        for q in members:
+            done[q] += 2
+        mix[0].sync()
+    t.join()
+    assert not errors, errors[0]
+    tol = 1e-12 if prec == ma.FP64 else 2e-5
+    for a, b, n in zip(alone, mix, done):
+        for _ in range(n):
+            a.step(diag=False)
+        a.sync()
+        b.sync()
+        Ea, Ha = a.fields()
+        Eb, Hb = b.fields()
+        assert rel_l2(Eb, Ea) <= tol and rel_l2(Hb, Ha) <= tol
+        assert np.array_equal(by_id(a.markers())["id"], by_id(b.markers())["id"])
+        assert b.time == a.time
+    assert len(side_out) == 4
+    for E, H in side_out:
+        assert rel_l2(E, E_ref) <= tol and rel_l2(H, H_ref) <= tol
