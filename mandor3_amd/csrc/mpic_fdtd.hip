@@ -330,15 +330,35 @@ int launch_face_pair(hipStream_t s, const Grid &g, const FaceOp &a, const FaceOp
     return launch_def(s, fa, nblk(n), 2);
 }
 
-__device__ static inline int in_boxes(const BoxSet &bs, int i, int j, int k)
+/* Cap-flush nodes of a vector group (row i, j, nodes k0 .. k0+V-1): bit e
+ * is set when node k0+e is not an interior node ('interior': bits of the
+ * nodes the interior formula takes), lies in [0, n+1] along k and in a box
+ * of bs.  The boxes' i, j ranges are tested once per group.  Only groups at
+ * the edges of the volume have any bit set; they also load the cap values
+ * of the whole group together with the fields, so that no wavefront waits
+ * twice on memory. */
+__device__ static inline unsigned flush_mask(const BoxSet &bs, int i, int j, int k0, int V,
+                                             int nk, unsigned interior)
 {
+    unsigned m = 0;
     for (int b = 0; b < bs.n; ++b) {
         const Box &B = bs.b[b];
-        if (i >= B.lo[0] && i <= B.hi[0] && j >= B.lo[1] && j <= B.hi[1] &&
-            k >= B.lo[2] && k <= B.hi[2])
-            return 1;
+        if (i < B.lo[0] || i > B.hi[0] || j < B.lo[1] || j > B.hi[1]) continue;
+        for (int e = 0; e < V; ++e) {
+            const int k = k0 + e;
+            if (k >= B.lo[2] && k <= B.hi[2] && k >= 0 && k <= nk + 1) m |= 1u << e;
+        }
     }
-    return 0;
+    return m & ~interior;
+}
+
+/* Bits of the nodes k0 .. k0+V-1 inside [lo, hi]. */
+__device__ static inline unsigned k_bits(int k0, int V, int lo, int hi)
+{
+    unsigned m = 0;
+    for (int e = 0; e < V; ++e)
+        if (k0 + e >= lo && k0 + e <= hi) m |= 1u << e;
+    return m;
 }
 
 /* Interior H half step (source/core/em.c:55-77) fused with the flush of the
@@ -417,23 +437,34 @@ __device__ static inline void h_update_group(const FieldsT<T> &f, const Grid &g,
     const T exk0 = Ex[o - 1], eyk0 = Ey[o - 1];          /* k - 1 of the first node */
     VT hx = *(const VT *)(f.H[0] + o), hy = *(const VT *)(f.H[1] + o), hz = *(const VT *)(f.H[2] + o);
     const bool ij_in = i >= 3 && i <= g.n[0] - 2 && j >= 3 && j <= g.n[1] - 2;
+    const unsigned inner = ij_in ? k_bits(k0, V, 3, g.n[2] - 2) : 0u;
+    unsigned fm = 0;
+    VT cx = {}, cy = {}, cz = {};
+    if (inner != (1u << V) - 1) {
+        fm = flush_mask(fl, i, j, k0, V, g.n[2], inner);
+        if (fm) {
+            cx = *(const VT *)(f.Hc[0] + o);
+            cy = *(const VT *)(f.Hc[1] + o);
+            cz = *(const VT *)(f.Hc[2] + o);
+        }
+    }
 #pragma unroll
     for (int e = 0; e < V; ++e) {
         const int k = k0 + e;
         const T exk = e ? ex[e - 1] : exk0, eyk = e ? ey[e - 1] : eyk0;
-        if (ij_in && k >= 3 && k <= g.n[2] - 2) {
+        if (inner >> e & 1) {
             hx[e] += c3 * (ey[e] - eyk) - c2 * (ez[e] - ezj[e]);
             hy[e] += c1 * (ez[e] - ezi[e]) - c3 * (ex[e] - exk);
             hz[e] += c2 * (ex[e] - exj[e]) - c1 * (ey[e] - eyi[e]);
-        } else if (k >= 0 && k <= g.n[2] + 1 && in_boxes(fl, i, j, k)) {
+        } else if (fm >> e & 1) {
             if (full) {
-                hx[e] = f.Hc[0][o + e];
-                hy[e] = f.Hc[1][o + e];
-                hz[e] = f.Hc[2][o + e];
+                hx[e] = cx[e];
+                hy[e] = cy[e];
+                hz[e] = cz[e];
             } else {
-                hx[e] = (T)0.5 * (hx[e] + f.Hc[0][o + e]);
-                hy[e] = (T)0.5 * (hy[e] + f.Hc[1][o + e]);
-                hz[e] = (T)0.5 * (hz[e] + f.Hc[2][o + e]);
+                hx[e] = (T)0.5 * (hx[e] + cx[e]);
+                hy[e] = (T)0.5 * (hy[e] + cy[e]);
+                hz[e] = (T)0.5 * (hz[e] + cz[e]);
             }
         }
         if (energy) energy_terms<T>(g, i, j, k, ex[e], ey[e], ez[e], hx[e], hy[e], hz[e], we, wb);
@@ -515,20 +546,30 @@ __device__ static inline void e_update(const FieldsT<T> &f, const Grid &g, const
     const T hxkV = Hx[o + V], hykV = Hy[o + V];          /* k + 1 of the last node */
     VT ex = *(const VT *)(f.E[0] + o), ey = *(const VT *)(f.E[1] + o), ez = *(const VT *)(f.E[2] + o);
     const bool ij_in = i >= 2 && i <= g.n[0] - 2 && j >= 2 && j <= g.n[1] - 2;
+    const unsigned inner = ij_in ? k_bits(k0, V, 2, g.n[2] - 2) : 0u;
+    unsigned fm = 0;
+    VT cx = {}, cy = {}, cz = {};
+    if (do_flush && inner != (1u << V) - 1) {
+        fm = flush_mask(fl, i, j, k0, V, g.n[2], inner);
+        if (fm) {
+            cx = *(const VT *)(f.Ec[0] + o);
+            cy = *(const VT *)(f.Ec[1] + o);
+            cz = *(const VT *)(f.Ec[2] + o);
+        }
+    }
 #pragma unroll
     for (int e = 0; e < V; ++e) {
-        const int k = k0 + e;
         const T hxk = e < V - 1 ? hx[e + 1] : hxkV, hyk = e < V - 1 ? hy[e + 1] : hykV;
-        if (ij_in && k >= 2 && k <= g.n[2] - 2) {
+        if (inner >> e & 1) {
             if (interior) {
                 ex[e] -= c3 * (hyk - hy[e]) - c2 * (hzj[e] - hz[e]);
                 ey[e] -= c1 * (hzi[e] - hz[e]) - c3 * (hxk - hx[e]);
                 ez[e] -= c2 * (hxj[e] - hx[e]) - c1 * (hyi[e] - hy[e]);
             }
-        } else if (do_flush && k >= 0 && k <= g.n[2] + 1 && in_boxes(fl, i, j, k)) {
-            ex[e] = f.Ec[0][o + e];
-            ey[e] = f.Ec[1][o + e];
-            ez[e] = f.Ec[2][o + e];
+        } else if (fm >> e & 1) {
+            ex[e] = cx[e];
+            ey[e] = cy[e];
+            ez[e] = cz[e];
         }
     }
     *(VT *)(f.E[0] + o) = ex;
@@ -642,19 +683,30 @@ __device__ static inline void e_fused(const FieldsT<T> &f, const Grid &g, const 
              jz = *(const VD *)(f.J[2] + o);
     const bool ij_in = i >= 2 && i <= g.n[0] - 2 && j >= 2 && j <= g.n[1] - 2;
     const bool ij_fl = i >= 0 && j >= 0;                 /* e_update's nodes start at 0 */
+    const unsigned inner = ij_in ? k_bits(k0, V, 2, g.n[2] - 2) : 0u;
+    unsigned fm = 0;
+    VT cx = {}, cy = {}, cz = {};
+    if (ij_fl && inner != (1u << V) - 1) {
+        fm = flush_mask(fl, i, j, k0, V, g.n[2], inner);
+        if (fm) {
+            cx = *(const VT *)(f.Ec[0] + o);
+            cy = *(const VT *)(f.Ec[1] + o);
+            cz = *(const VT *)(f.Ec[2] + o);
+        }
+    }
     const double tau = g.tau;
 #pragma unroll
     for (int e = 0; e < V; ++e) {
         const int k = k0 + e;
         const T hxk = e < V - 1 ? hx[e + 1] : hxkV, hyk = e < V - 1 ? hy[e + 1] : hykV;
-        if (ij_in && k >= 2 && k <= g.n[2] - 2) {
+        if (inner >> e & 1) {
             ex[e] -= c3 * (hyk - hy[e]) - c2 * (hzj[e] - hz[e]);
             ey[e] -= c1 * (hzi[e] - hz[e]) - c3 * (hxk - hx[e]);
             ez[e] -= c2 * (hxj[e] - hx[e]) - c1 * (hyi[e] - hy[e]);
-        } else if (ij_fl && k >= 0 && k <= g.n[2] + 1 && in_boxes(fl, i, j, k)) {
-            ex[e] = f.Ec[0][o + e];
-            ey[e] = f.Ec[1][o + e];
-            ez[e] = f.Ec[2][o + e];
+        } else if (fm >> e & 1) {
+            ex[e] = cx[e];
+            ey[e] = cy[e];
+            ez[e] = cz[e];
         }
         if (k >= -1 && k <= g.n[2] + 1) {
             ex[e] = (T)(ex[e] - tau * jx[e]);
