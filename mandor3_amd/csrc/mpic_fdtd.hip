@@ -1296,29 +1296,91 @@ int launch_zero(hipStream_t s, double *p, long n)
     return MPIC_OK;
 }
 
-/* Two ranges cleared by one launch. */
-struct Zero2Args {
-    double *p;
-    long    n;
-    double *q;
-    long    m;
+/* J cleared before the push, only on the x planes the previous step's
+ * deposit can have written, and the three temperature sums.
+ *
+ * A tiled push deposits the markers of the active tiles only (the list is
+ * in tile order, tiles i-major), each into nodes [4u-1, 4u+5] along x of
+ * its tile u; the fold and the VSP filter leave zero where J is zero except
+ * near the x walls (a periodic fold moves current across the box), and the
+ * filter spreads it by one node.  So the planes written in a step are those
+ * of [4 u_first - 3, 4 u_last + 7] for the step's first and last active
+ * tiles, or the whole grid when that range comes within 4 planes of an x
+ * wall.  Each launch clears the range recorded by the previous step's
+ * launch (slot step % 2) and records the range of the list it sees, which
+ * the coming push uses (slot (step + 1) % 2).  'full' (an untiled push)
+ * clears everything and records the whole grid. */
+struct JClearArgs {
+    double       *J;          /* 3 components, ncell apart */
+    long          ncell, plane;
+    int           n0, tpp;    /* cells along x, tiles per x layer of tiles */
+    double       *acc;        /* 3 sums */
+    int          *range;      /* [2][2] plane ranges by step parity */
+    const int    *step, *active, *nactive;
+    int           full;
 };
-template <> struct KernelDef<Zero2Args> {
+
+__device__ static inline void jclear(const JClearArgs &a, const VBlock &b, const VBlock &gd)
+{
+    const int s = *a.step;
+    int lo = -2, hi = a.n0 + 2;
+    if (!a.full) {
+        lo = a.range[2 * (s & 1)];
+        hi = a.range[2 * (s & 1) + 1];
+    }
+    if (b.x == 0 && threadIdx.x == 0) {
+        a.acc[0] = 0.0;
+        a.acc[1] = 0.0;
+        a.acc[2] = 0.0;
+        int nlo = -2, nhi = a.n0 + 2;
+        if (!a.full) {
+            const int na = *a.nactive;
+            if (na <= 0) {
+                nlo = 0;
+                nhi = -1;                                    /* nothing written */
+            } else {
+                nlo = 4 * (a.active[0] / a.tpp) - 3;
+                nhi = 4 * (a.active[na - 1] / a.tpp) + 7;
+                if (nlo <= 3 || nhi >= a.n0 - 3) {
+                    nlo = -2;
+                    nhi = a.n0 + 2;
+                }
+            }
+        }
+        a.range[2 * ((s + 1) & 1)] = nlo;
+        a.range[2 * ((s + 1) & 1) + 1] = nhi;
+    }
+    if (hi < lo) return;
+    typedef double D2 __attribute__((ext_vector_type(2)));
+    const long base = (long)(lo + MPIC_GHOST) * a.plane;
+    const long cnt2 = (long)(hi - lo + 1) * a.plane / 2;   /* planes hold an even count */
+    const long stride = (long)gd.x * TPB;
+    const D2 z = {0.0, 0.0};
+    for (int c = 0; c < 3; ++c) {
+        D2 *p = (D2 *)(a.J + c * a.ncell + base);
+        for (long u = (long)b.x * TPB + threadIdx.x; u < cnt2; u += stride) p[u] = z;
+    }
+}
+
+template <> struct KernelDef<JClearArgs> {
     static constexpr int kBlock = TPB;
-    __device__ __forceinline__ static void body(const Zero2Args &a, VBlock b, VBlock gd)
+    __device__ __forceinline__ static void body(const JClearArgs &a, VBlock b, VBlock gd)
     {
-        long t = (long)b.x * TPB + threadIdx.x;
-        if (t < a.m) a.q[t] = 0.0;
-        for (; t < a.n; t += (long)gd.x * TPB) a.p[t] = 0.0;
+        jclear(a, b, gd);
     }
 };
 
-int launch_zero2(hipStream_t s, double *p, long n, double *q, long m)
+int launch_jclear(hipStream_t s, double *J, const Grid &g, double *acc, int *range,
+                  const int *step, const int *active, const int *nactive, int tpp, int full)
 {
-    long nb = ((n > m ? n : m) + TPB - 1) / TPB;
-    if (nb > 65536) nb = 65536;
+    JClearArgs a = {J, g.ncell, (long)g.GY * g.GZ, g.n[0], tpp, acc, range, step, active,
+                    nactive, full};
+    /* 16-byte stores; enough workgroups to stream the whole grid at full
+     * rate, few enough that a narrow range costs little more than a launch */
+    const long cnt2 = (long)(g.n[0] + 5) * g.GY * g.GZ / 2;
+    long nb = (cnt2 + 4 * TPB - 1) / (4 * TPB);
+    if (nb > 4096) nb = 4096;
     if (nb < 1) nb = 1;
-    Zero2Args a = {p, n, q, m};
     return launch_def(s, a, (unsigned)nb);
 }
 

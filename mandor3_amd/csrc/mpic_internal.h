@@ -188,7 +188,8 @@ int h_update_blocks(const Grid &g, int fp32);
 int launch_diag_ring(hipStream_t s, const double *part, int nparts, const double *wk,
                      double *ring, int *step, int cap, int tick);
 int launch_zero(hipStream_t s, double *p, long n);
-int launch_zero2(hipStream_t s, double *p, long n, double *q, long m);
+int launch_jclear(hipStream_t s, double *J, const Grid &g, double *acc, int *range,
+                  const int *step, const int *active, const int *nactive, int tpp, int full);
 
 int launch_push64(hipStream_t s, const Markers64 &m, long n, const Fields64 &f,
                   const Grid &g, const PushConsts &c, double *Wacc, int mode, unsigned *err);

@@ -149,6 +149,7 @@ struct mpic_run {
     double        time;
     std::vector<hipEvent_t> evpool;  /* 5 events per timed step */
     int          *dstep;      /* device step counter (indexes the laser tables) */
+    int          *djr;        /* device: x plane ranges of J written, by step parity [2][2] */
     Spot          spot;
     Tfsf          tfsf;
     TfsfRec       trec;
@@ -528,6 +529,12 @@ extern "C" int mpic_create(const mpic_config_t *cfg, mpic_run_t **out)
     r->timed = -1;
     MPIC_CHECK(hipMalloc(&r->dstep, sizeof(int)));
     MPIC_CHECK(hipMemsetAsync(r->dstep, 0, sizeof(int), r->stream));
+    {
+        /* the first J clear clears the whole grid */
+        const int full[4] = {-2, g.n[0] + 2, -2, g.n[0] + 2};
+        MPIC_CHECK(hipMalloc(&r->djr, sizeof(full)));
+        MPIC_CHECK(hipMemcpy(r->djr, full, sizeof(full), hipMemcpyHostToDevice));
+    }
     r->nparts = h_update_blocks(g, cfg->precision == MPIC_FP32_SPLIT);
     MPIC_CHECK(hipMalloc(&r->dpart, sizeof(double) * 2 * r->nparts));
     MPIC_CHECK(hipMalloc(&r->dring, sizeof(double) * 5 * DIAG_RING));
@@ -595,6 +602,7 @@ extern "C" int mpic_destroy(mpic_run_t *r)
         hipHostFree(l.htab);
     }
     hipFree(r->dstep);
+    hipFree(r->djr);
     hipFree(r->dpart);
     hipFree(r->dring);
     hipFree(r->dnode);
@@ -1671,8 +1679,11 @@ extern "C" int mpic_push_deposit(mpic_run_t *r)
     if (!r) return MPIC_ERR_ARG;
     TRY(check_stage(r, 1));
     MPIC_CHECK(hipSetDevice(r->dev));
-    /* J and the temperature sums: a kernel, not a memset (the step may be captured) */
-    TRY(launch_zero2(r->stream, r->jp.J[0], 3 * r->g.ncell, r->dacc + 2, 3));
+    /* J (the planes the last deposit wrote) and the temperature sums: a
+     * kernel, not a memset (the step may be replayed from a graph) */
+    TRY(launch_jclear(r->stream, r->jp.J[0], r->g, r->dacc + 2, r->djr, r->dstep,
+                      r->tiled ? r->tv.active : nullptr, r->tiled ? r->tv.nactive : nullptr,
+                      r->tiled ? r->tv.nt[1] * r->tv.nt[2] : 1, r->tiled ? 0 : 1));
     if (r->timed >= 0) TRY(record(r, 1));
     if (r->fp32) {
         TRY(launch_push32(r->stream, r->m32, r->f32, r->g, r->pc, r->qm, r->tv, r->tj, r->dacc + 2,
