@@ -602,11 +602,12 @@ template <class T> struct ETauJArgs {
     Grid       g;
     Span       sp;
     int       *tick;
+    const int *jr;         /* x planes [jr[0], jr[1]] of nodes where J may be non-zero */
 };
 
 template <class T>
 __device__ static inline void e_tauJ(const FieldsT<T> &f, const Grid &g, const Span &sp,
-                                     int *tick, const VBlock &b, const VBlock &gd)
+                                     int *tick, const int *jr, const VBlock &b, const VBlock &gd)
 {
     typedef typename Vec<T>::v VT;
     typedef double VD __attribute__((ext_vector_type(Vec<T>::N)));
@@ -615,6 +616,7 @@ __device__ static inline void e_tauJ(const FieldsT<T> &f, const Grid &g, const S
     if (tick && b.x == 0 && threadIdx.x == 0) *tick += 1;
     int i, j, k0;
     if (!span_group(sp, V, b, gd, i, j, k0)) return;
+    if (i < jr[0] || i > jr[1]) return;                  /* J = 0: E - tau*0 is E */
     const long o = gidx(g, i, j, k0);
     const double tau = g.tau;
 #pragma unroll
@@ -634,15 +636,15 @@ template <class T> struct KernelDef<ETauJArgs<T>> {
     static constexpr int kBlock = TPB;
     __device__ __forceinline__ static void body(const ETauJArgs<T> &a, VBlock b, VBlock gd)
     {
-        e_tauJ<T>(a.f, a.g, a.sp, a.tick, b, gd);
+        e_tauJ<T>(a.f, a.g, a.sp, a.tick, a.jr, b, gd);
     }
 };
 
 template <class T>
-int launch_e_tauJ(hipStream_t s, const FieldsT<T> &f, const Grid &g, int *tick)
+int launch_e_tauJ(hipStream_t s, const FieldsT<T> &f, const Grid &g, int *tick, const int *jr)
 {
     const Span sp = span_of<Vec<T>::N>(-1, -1, g.n[1] + 1, -1, g.n[2] + 1);
-    ETauJArgs<T> a = {f, g, sp, tick};
+    ETauJArgs<T> a = {f, g, sp, tick, jr};
     return launch_def(s, a, grid_span(sp, g.n[0] + 3));
 }
 
@@ -659,12 +661,13 @@ template <class T> struct EFusedArgs {
     BoxSet     fl;
     Span       sp;
     T          c1, c2, c3;
+    const int *jr;         /* x planes [jr[0], jr[1]] of nodes where J may be non-zero */
 };
 
 template <class T>
 __device__ static inline void e_fused(const FieldsT<T> &f, const Grid &g, const BoxSet &fl,
-                                      const Span &sp, T c1, T c2, T c3, const VBlock &b,
-                                      const VBlock &gd)
+                                      const Span &sp, T c1, T c2, T c3, const int *jr,
+                                      const VBlock &b, const VBlock &gd)
 {
     typedef typename Vec<T>::v VT;
     typedef double VD __attribute__((ext_vector_type(Vec<T>::N)));
@@ -679,8 +682,14 @@ __device__ static inline void e_fused(const FieldsT<T> &f, const Grid &g, const 
     const VT hyi = *(const VT *)(Hy + o + si), hzi = *(const VT *)(Hz + o + si);
     const T hxkV = Hx[o + V], hykV = Hy[o + V];          /* k + 1 of the last node */
     VT ex = *(const VT *)(f.E[0] + o), ey = *(const VT *)(f.E[1] + o), ez = *(const VT *)(f.E[2] + o);
-    const VD jx = *(const VD *)(f.J[0] + o), jy = *(const VD *)(f.J[1] + o),
-             jz = *(const VD *)(f.J[2] + o);
+    /* J is read only on the planes where it may be non-zero (elsewhere
+     * E - tau*0 is E, the same value) */
+    VD jx = {}, jy = {}, jz = {};
+    if (i >= jr[0] && i <= jr[1]) {
+        jx = *(const VD *)(f.J[0] + o);
+        jy = *(const VD *)(f.J[1] + o);
+        jz = *(const VD *)(f.J[2] + o);
+    }
     const bool ij_in = i >= 2 && i <= g.n[0] - 2 && j >= 2 && j <= g.n[1] - 2;
     const bool ij_fl = i >= 0 && j >= 0;                 /* e_update's nodes start at 0 */
     const unsigned inner = ij_in ? k_bits(k0, V, 2, g.n[2] - 2) : 0u;
@@ -723,16 +732,17 @@ template <class T> struct KernelDef<EFusedArgs<T>> {
     static constexpr int kBlock = TPB;
     __device__ __forceinline__ static void body(const EFusedArgs<T> &a, VBlock b, VBlock gd)
     {
-        e_fused<T>(a.f, a.g, a.fl, a.sp, a.c1, a.c2, a.c3, b, gd);
+        e_fused<T>(a.f, a.g, a.fl, a.sp, a.c1, a.c2, a.c3, a.jr, b, gd);
     }
 };
 
 template <class T>
-int launch_e_fused(hipStream_t s, const FieldsT<T> &f, const Grid &g, const BoxSet &flush)
+int launch_e_fused(hipStream_t s, const FieldsT<T> &f, const Grid &g, const BoxSet &flush,
+                   const int *jr)
 {
     const Span sp = span_of<Vec<T>::N>(-1, -1, g.n[1] + 1, -1, g.n[2] + 1);
     EFusedArgs<T> a = {f, g, flush, sp, (T)(g.tau / g.h[0]), (T)(g.tau / g.h[1]),
-                       (T)(g.tau / g.h[2])};
+                       (T)(g.tau / g.h[2]), jr};
     return launch_def(s, a, grid_span(sp, g.n[0] + 3));
 }
 
@@ -1308,14 +1318,15 @@ int launch_zero(hipStream_t s, double *p, long n)
  * tiles, or the whole grid when that range comes within 4 planes of an x
  * wall.  Each launch clears the range recorded by the previous step's
  * launch (slot step % 2) and records the range of the list it sees, which
- * the coming push uses (slot (step + 1) % 2).  'full' (an untiled push)
- * clears everything and records the whole grid. */
+ * the coming push uses (slot (step + 1) % 2), also as range[4..5], where
+ * the E updates of the same step find the planes on which J may be non-zero.
+ * 'full' (an untiled push) clears everything and records the whole grid. */
 struct JClearArgs {
     double       *J;          /* 3 components, ncell apart */
     long          ncell, plane;
     int           n0, tpp;    /* cells along x, tiles per x layer of tiles */
     double       *acc;        /* 3 sums */
-    int          *range;      /* [2][2] plane ranges by step parity */
+    int          *range;      /* [2][2] plane ranges by step parity, [2] this step's */
     const int    *step, *active, *nactive;
     int           full;
 };
@@ -1349,6 +1360,8 @@ __device__ static inline void jclear(const JClearArgs &a, const VBlock &b, const
         }
         a.range[2 * ((s + 1) & 1)] = nlo;
         a.range[2 * ((s + 1) & 1) + 1] = nhi;
+        a.range[4] = nlo;                                    /* this step's, for E -= tau J */
+        a.range[5] = nhi;
     }
     if (hi < lo) return;
     typedef double D2 __attribute__((ext_vector_type(2)));
@@ -1550,10 +1563,11 @@ int launch_probes(hipStream_t s, T *const E[3], T *const H[3], const Grid &g, co
     template int launch_h_update<T>(hipStream_t, const FieldsT<T> &, const Grid &,                \
                                     const BoxSet &, int, double *);                             \
     template int launch_e_fused<T>(hipStream_t, const FieldsT<T> &, const Grid &,                 \
-                                   const BoxSet &);                                             \
+                                   const BoxSet &, const int *);                                \
     template int launch_e_update<T>(hipStream_t, const FieldsT<T> &, const Grid &,                \
                                     const BoxSet &, int, int);                                  \
-    template int launch_e_tauJ<T>(hipStream_t, const FieldsT<T> &, const Grid &, int *);         \
+    template int launch_e_tauJ<T>(hipStream_t, const FieldsT<T> &, const Grid &, int *,         \
+                                  const int *);         \
     template int launch_laser<T>(hipStream_t, T *, T *, const Grid &, const double *, int, int,  \
                                  int, int, int, int, int, int, const double *, const int *,     \
                                  int);                                                          \

@@ -150,11 +150,13 @@ template <class T>
 int launch_h_update(hipStream_t s, const FieldsT<T> &f, const Grid &g, const BoxSet &flush,
                     int full, double *part = nullptr);
 template <class T>
-int launch_e_fused(hipStream_t s, const FieldsT<T> &f, const Grid &g, const BoxSet &flush);
+int launch_e_fused(hipStream_t s, const FieldsT<T> &f, const Grid &g, const BoxSet &flush,
+                   const int *jr);
 template <class T>
 int launch_e_update(hipStream_t s, const FieldsT<T> &f, const Grid &g, const BoxSet &flush,
                     int interior, int do_flush);
-template <class T> int launch_e_tauJ(hipStream_t s, const FieldsT<T> &f, const Grid &g, int *tick);
+template <class T>
+int launch_e_tauJ(hipStream_t s, const FieldsT<T> &f, const Grid &g, int *tick, const int *jr);
 template <class T>
 int launch_laser(hipStream_t s, T *Fy, T *Fz, const Grid &g, const double *map, int pos,
                  int JMIN, int KMIN, int KMAX, int jmin, int jmax, int kmin, int kmax,

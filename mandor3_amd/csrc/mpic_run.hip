@@ -149,7 +149,8 @@ struct mpic_run {
     double        time;
     std::vector<hipEvent_t> evpool;  /* 5 events per timed step */
     int          *dstep;      /* device step counter (indexes the laser tables) */
-    int          *djr;        /* device: x plane ranges of J written, by step parity [2][2] */
+    int          *djr;        /* device: x plane ranges of J written: by step parity [2][2],
+                                 then the current step's [2] */
     Spot          spot;
     Tfsf          tfsf;
     TfsfRec       trec;
@@ -531,7 +532,7 @@ extern "C" int mpic_create(const mpic_config_t *cfg, mpic_run_t **out)
     MPIC_CHECK(hipMemsetAsync(r->dstep, 0, sizeof(int), r->stream));
     {
         /* the first J clear clears the whole grid */
-        const int full[4] = {-2, g.n[0] + 2, -2, g.n[0] + 2};
+        const int full[6] = {-2, g.n[0] + 2, -2, g.n[0] + 2, -2, g.n[0] + 2};
         MPIC_CHECK(hipMalloc(&r->djr, sizeof(full)));
         MPIC_CHECK(hipMemcpy(r->djr, full, sizeof(full), hipMemcpyHostToDevice));
     }
@@ -1836,7 +1837,7 @@ extern "C" int mpic_em_e_finish(mpic_run_t *r)
     TRY(e_sources(r));
     /* E -= tau J; the step counter advances in the same launch unless the
      * spot source still has to read it */
-    FCALL(launch_e_tauJ, r->g, r->spot.on ? nullptr : r->dstep);
+    FCALL(launch_e_tauJ, r->g, r->spot.on ? nullptr : r->dstep, r->djr + 4);
     TRY(e_spot(r));
     if (r->spot.on) TRY(launch_tick(r->stream, r->dstep));
     step_done(r);
@@ -1871,12 +1872,12 @@ static int step_body(mpic_run *r)
      * laser sources then add after E -= tau J) */
     const bool fused = r->fp32 || (!r->tfsf.on && r->lasers.empty());
     if (fused) {
-        FCALL(launch_e_fused, r->g, r->flE);
+        FCALL(launch_e_fused, r->g, r->flE, r->djr + 4);
         TRY(e_sources(r));
     } else {
         TRY(mpic_em_e_start(r));
         TRY(e_sources(r));
-        FCALL(launch_e_tauJ, r->g, nullptr);
+        FCALL(launch_e_tauJ, r->g, nullptr, r->djr + 4);
     }
     TRY(e_spot(r));
     /* the ring row of this step, then the device step counter advances */
