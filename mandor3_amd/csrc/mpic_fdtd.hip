@@ -1002,25 +1002,35 @@ int launch_tick(hipStream_t s, int *step)
 /* J boundary fold of one face b (order x-min, x-max, y-min, y-max, z-min,
  * z-max), source/core/plasma_currentWalls.c:132-227. */
 struct JFoldArgs {
-    JPtrs f;
-    Grid  g;
-    int   ea, bc0, bc1, top0;
+    JPtrs      f;
+    Grid       g;
+    int        ea, bc0, bc1, top0;
+    const int *jr;         /* x planes [jr[0], jr[1]] of nodes where J may be non-zero */
 };
 
+/* A line whose nodes all lie on x planes where J is zero is left alone: the
+ * fold of zeros writes zeros.  For y and z walls that is a line on a plane
+ * outside jr; an x wall is skipped whole when its layers miss jr (a
+ * periodic x fold reads both walls' layers). */
 __device__ static inline void jfold(const JPtrs &f, const Grid &g, int ea, int bc0, int bc1,
-                                    int top0, const VBlock &b)
+                                    int top0, const int *jr, const VBlock &b)
 {
     /* wall top0 + block y of the axis: 0 min, 1 max */
     const int top = top0 + (int)b.y, bc = top ? bc1 : bc0;
     const int ep = (ea == 0) ? 1 : 0, eq = (ea == 2) ? 1 : 2;
     const int np = g.n[ep] + 5, nq = g.n[eq] + 5;
+    const int N = g.n[ea];
+    if (ea == 0) {
+        const bool lo_wall = jr[0] <= 2 && jr[1] >= -2, hi_wall = jr[0] <= N + 2 && jr[1] >= N - 2;
+        if (bc == MPIC_BC_PERIODIC ? !(lo_wall || hi_wall) : !(top ? hi_wall : lo_wall)) return;
+    }
     long t = (long)b.x * TPB + threadIdx.x;
     if (t >= (long)np * nq) return;
     int r[3];
     const unsigned pq = (unsigned)t / (unsigned)nq;
     r[ep] = -2 + (int)pq;
     r[eq] = -2 + (int)((unsigned)t - pq * (unsigned)nq);
-    const int N = g.n[ea];
+    if (ea != 0 && (r[0] < jr[0] || r[0] > jr[1])) return;
 #define AT(a) (r[ea] = (a), gidx(g, r[0], r[1], r[2]))
     /* every value is loaded before any is stored: the nodes of one line are
      * distinct, so this is the reference's sequence with one memory latency */
@@ -1091,7 +1101,7 @@ template <> struct KernelDef<JFoldArgs> {
     static constexpr int kBlock = TPB;
     __device__ __forceinline__ static void body(const JFoldArgs &a, VBlock b, VBlock)
     {
-        jfold(a.f, a.g, a.ea, a.bc0, a.bc1, a.top0, b);
+        jfold(a.f, a.g, a.ea, a.bc0, a.bc1, a.top0, a.jr, b);
     }
 };
 
@@ -1101,12 +1111,13 @@ template <> struct KernelDef<JFoldArgs> {
         if (rc_ != MPIC_OK) return rc_; \
     } while (0)
 
-int launch_jfold(hipStream_t s, const JPtrs &f, const Grid &g, int ea, int bc0, int bc1)
+int launch_jfold(hipStream_t s, const JPtrs &f, const Grid &g, int ea, int bc0, int bc1,
+                 const int *jr)
 {
     const int ep = (ea == 0) ? 1 : 0, eq = (ea == 2) ? 1 : 2;
     const long n = (long)(g.n[ep] + 5) * (g.n[eq] + 5);
     const unsigned nb = (unsigned)nblk(n);
-    JFoldArgs a = {f, g, ea, bc0, bc1, 0};
+    JFoldArgs a = {f, g, ea, bc0, bc1, 0, jr};
     if (bc0 == MPIC_BC_PERIODIC) return launch_def(s, a, nb, 1);
     if (g.n[ea] >= 8) return launch_def(s, a, nb, 2);
     TRY_LAUNCH(launch_def(s, a, nb, 1));

@@ -169,7 +169,8 @@ int launch_tfsf(hipStream_t s, T *const F[3], const Grid &g, const TfsfSet &fs, 
 template <class T>
 int launch_spot(hipStream_t s, T *Ey, T *Ez, const Grid &g, int i, const double *py,
                 const double *pz, const double *tab, const int *step, int cap);
-int launch_jfold(hipStream_t s, const JPtrs &f, const Grid &g, int ea, int bc0, int bc1);
+int launch_jfold(hipStream_t s, const JPtrs &f, const Grid &g, int ea, int bc0, int bc1,
+                 const int *jr);
 /* VSP (beta, alpha, beta) sweeps along z, y, x of ncomp arrays on the padded grid. */
 int launch_vsp(hipStream_t s, double *const F[3], int ncomp, const Grid &g, double alpha);
 template <class T>

@@ -1760,7 +1760,8 @@ extern "C" int mpic_j_finish(mpic_run_t *r)
     if (!r) return MPIC_ERR_ARG;
     MPIC_CHECK(hipSetDevice(r->dev));
     const int *bc = r->cfg.bc;
-    for (int ea = 0; ea < 3; ++ea) TRY(launch_jfold(r->stream, r->jp, r->g, ea, bc[ea], bc[3 + ea]));
+    for (int ea = 0; ea < 3; ++ea)
+        TRY(launch_jfold(r->stream, r->jp, r->g, ea, bc[ea], bc[3 + ea], r->djr + 4));
     /* VSP_vectorSweep after the boundary fold (source/core/plasma_currentWalls.c:204) */
     if (fabs(r->vsp - 1.0) >= 1e-10) TRY(launch_vsp(r->stream, r->jp.J, 3, r->g, r->vsp));
     return MPIC_OK;
