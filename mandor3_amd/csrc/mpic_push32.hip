@@ -662,7 +662,12 @@ __device__ __forceinline__ static void jgather32(const F32View &v, const TileVie
             nitem[threadIdx.x] = su >= 0 ? tv.istart[su + 1] - tv.istart[su] : 0;
         }
         __syncthreads();
-        for (int node = threadIdx.x; node < TJN3; node += TPB) {
+        /* nodes in reverse order: the nodes an interior tile owns (block
+         * positions 3..6 along each axis, the others belong to the tiles
+         * below) all fall in the first pass of the workgroup, so a tile
+         * inside the plasma waits on memory once */
+        for (int pass = threadIdx.x; pass < TJN3; pass += TPB) {
+            const int node = TJN3 - 1 - pass;
             const int cc = node % TJN, bb = (node / TJN) % TJN, aa = node / (TJN * TJN);
             const int i = TILE * t0 - 1 + aa, j = TILE * t1 - 1 + bb, k = TILE * t2 - 1 + cc;
             if (i < -2 || j < -2 || k < -2 || i > g.n[0] + 2 || j > g.n[1] + 2 || k > g.n[2] + 2)
@@ -687,6 +692,8 @@ __device__ __forceinline__ static void jgather32(const F32View &v, const TileVie
                 if (use[q] && own < 0) own = (ux * tv.nt[1] + uy) * tv.nt[2] + uz;
             }
             if (own != t) continue;
+            const long o = gidx(g, i, j, k);
+            const double J0 = v.J[0][o], J1 = v.J[1][o], J2 = v.J[2][o];
             float f[8][3];
 #pragma unroll
             for (int q = 0; q < 8; ++q) {
@@ -710,10 +717,9 @@ __device__ __forceinline__ static void jgather32(const F32View &v, const TileVie
                     s2 += (double)blk[2];
                 }
             }
-            const long o = gidx(g, i, j, k);
-            v.J[0][o] += s0;
-            v.J[1][o] += s1;
-            v.J[2][o] += s2;
+            v.J[0][o] = J0 + s0;
+            v.J[1][o] = J1 + s1;
+            v.J[2][o] = J2 + s2;
         }
     }
 }
