@@ -361,6 +361,40 @@ __device__ static inline unsigned k_bits(int k0, int V, int lo, int hi)
     return m;
 }
 
+/* One-plane laser soft source, source/scpic/em_mirror.c:667-693.  The time
+ * factors (envelope, cos, sin of the carrier) are evaluated in FP64 on the
+ * host once per step and passed in. */
+template <class T>
+__device__ static inline void laser_node(const LaserPlane &lp, int j, int k, double A, double cs,
+                                         double sn, T &fy, T &fz)
+{
+    const long cnt = (long)(j - lp.JMIN) * (lp.KMAX - lp.KMIN + 1) + k - lp.KMIN;
+    const double *w = lp.map + 4 * cnt;
+    fy = (T)(fy + A * (w[2] * cs + w[3] * sn));
+    fz = (T)(fz + A * (w[0] * cs + w[1] * sn));
+}
+
+/* The source added to a vector group (nodes k0 .. k0+V-1 of row (i, j))
+ * whose new Fy/Fz values are in registers: the same arithmetic on the
+ * same values as the separate launch after the update stored them. */
+template <class T, class VT>
+__device__ static inline void laser_group(const LaserPlane &lp, int i, int j, int k0, int V,
+                                          VT &fy, VT &fz)
+{
+    if (!lp.on || i != lp.pos || j < lp.jmin || j > lp.jmax) return;
+    if (k0 + V - 1 < lp.kmin || k0 > lp.kmax) return;
+    const double *e = lp.tab + 3 * (*lp.step % lp.cap);
+    const double A = e[0], cs = e[1], sn = e[2];
+    for (int q = 0; q < V; ++q) {
+        const int k = k0 + q;
+        if (k < lp.kmin || k > lp.kmax) continue;
+        T y = fy[q], z = fz[q];
+        laser_node<T>(lp, j, k, A, cs, sn, y, z);
+        fy[q] = y;
+        fz[q] = z;
+    }
+}
+
 /* Interior H half step (source/core/em.c:55-77) fused with the flush of the
  * cap layer: H = (H + Hc)/2 after the first half step (cap_flushHalfH,
  * em_caps.c:730-749) or H = Hc after the second (cap_flushH, :754-770).
@@ -373,6 +407,7 @@ template <class T> struct HUpdateArgs {
     Span       sp;
     T          c1, c2, c3;
     double    *part;       /* non-null: em_energy partial sums, 2 per workgroup */
+    LaserPlane lp;         /* a laser's H plane added after the update (on = 0: none) */
 };
 
 /* Sum over the workgroup of (we, wb), stored as part[0..1]; every thread of
@@ -424,7 +459,8 @@ __device__ static inline void energy_terms(const Grid &g, int i, int j, int k, T
 template <class T>
 __device__ static inline void h_update_group(const FieldsT<T> &f, const Grid &g, const BoxSet &fl,
                                              int full, T c1, T c2, T c3, bool energy, int i,
-                                             int j, int k0, double &we, double &wb)
+                                             int j, int k0, double &we, double &wb,
+                                             const LaserPlane &lp)
 {
     typedef typename Vec<T>::v VT;
     constexpr int V = Vec<T>::N;
@@ -469,6 +505,7 @@ __device__ static inline void h_update_group(const FieldsT<T> &f, const Grid &g,
         }
         if (energy) energy_terms<T>(g, i, j, k, ex[e], ey[e], ez[e], hx[e], hy[e], hz[e], we, wb);
     }
+    laser_group<T>(lp, i, j, k0, V, hy, hz);
     *(VT *)(f.H[0] + o) = hx;
     *(VT *)(f.H[1] + o) = hy;
     *(VT *)(f.H[2] + o) = hz;
@@ -477,12 +514,12 @@ __device__ static inline void h_update_group(const FieldsT<T> &f, const Grid &g,
 template <class T>
 __device__ static inline void h_update(const FieldsT<T> &f, const Grid &g, const BoxSet &fl,
                                        int full, const Span &sp, T c1, T c2, T c3, double *part,
-                                       const VBlock &b, const VBlock &gd)
+                                       const LaserPlane &lp, const VBlock &b, const VBlock &gd)
 {
     int i, j, k0;
     const bool on = span_group(sp, Vec<T>::N, b, gd, i, j, k0);
     double we = 0.0, wb = 0.0;
-    if (on) h_update_group<T>(f, g, fl, full, c1, c2, c3, part != nullptr, i, j, k0, we, wb);
+    if (on) h_update_group<T>(f, g, fl, full, c1, c2, c3, part != nullptr, i, j, k0, we, wb, lp);
     /* the energy of H after this update (the half step: H^n) and of E */
     if (part) block_sum2(we, wb, part + 2 * (long)b.x);
 }
@@ -491,7 +528,7 @@ template <class T> struct KernelDef<HUpdateArgs<T>> {
     static constexpr int kBlock = TPB;
     __device__ __forceinline__ static void body(const HUpdateArgs<T> &a, VBlock b, VBlock gd)
     {
-        h_update<T>(a.f, a.g, a.fl, a.full, a.sp, a.c1, a.c2, a.c3, a.part, b, gd);
+        h_update<T>(a.f, a.g, a.fl, a.full, a.sp, a.c1, a.c2, a.c3, a.part, a.lp, b, gd);
     }
 };
 
@@ -509,11 +546,11 @@ int h_update_blocks(const Grid &g, int fp32)
 
 template <class T>
 int launch_h_update(hipStream_t s, const FieldsT<T> &f, const Grid &g, const BoxSet &flush,
-                    int full, double *part)
+                    int full, double *part, const LaserPlane &lp)
 {
     const Span sp = h_span(g, Vec<T>::N);
     HUpdateArgs<T> a = {f, g, flush, full, sp, (T)(0.5 * g.tau / g.h[0]),
-                        (T)(0.5 * g.tau / g.h[1]), (T)(0.5 * g.tau / g.h[2]), part};
+                        (T)(0.5 * g.tau / g.h[1]), (T)(0.5 * g.tau / g.h[2]), part, lp};
     return launch_def(s, a, grid_span(sp, g.n[0] + 2));
 }
 
@@ -662,12 +699,13 @@ template <class T> struct EFusedArgs {
     Span       sp;
     T          c1, c2, c3;
     const int *jr;         /* x planes [jr[0], jr[1]] of nodes where J may be non-zero */
+    LaserPlane lp;         /* a laser's E plane added after E -= tau J (on = 0: none) */
 };
 
 template <class T>
 __device__ static inline void e_fused(const FieldsT<T> &f, const Grid &g, const BoxSet &fl,
                                       const Span &sp, T c1, T c2, T c3, const int *jr,
-                                      const VBlock &b, const VBlock &gd)
+                                      const LaserPlane &lp, const VBlock &b, const VBlock &gd)
 {
     typedef typename Vec<T>::v VT;
     typedef double VD __attribute__((ext_vector_type(Vec<T>::N)));
@@ -723,6 +761,7 @@ __device__ static inline void e_fused(const FieldsT<T> &f, const Grid &g, const 
             ez[e] = (T)(ez[e] - tau * jz[e]);
         }
     }
+    laser_group<T>(lp, i, j, k0, V, ey, ez);
     *(VT *)(f.E[0] + o) = ex;
     *(VT *)(f.E[1] + o) = ey;
     *(VT *)(f.E[2] + o) = ez;
@@ -732,69 +771,58 @@ template <class T> struct KernelDef<EFusedArgs<T>> {
     static constexpr int kBlock = TPB;
     __device__ __forceinline__ static void body(const EFusedArgs<T> &a, VBlock b, VBlock gd)
     {
-        e_fused<T>(a.f, a.g, a.fl, a.sp, a.c1, a.c2, a.c3, a.jr, b, gd);
+        e_fused<T>(a.f, a.g, a.fl, a.sp, a.c1, a.c2, a.c3, a.jr, a.lp, b, gd);
     }
 };
 
 template <class T>
 int launch_e_fused(hipStream_t s, const FieldsT<T> &f, const Grid &g, const BoxSet &flush,
-                   const int *jr)
+                   const int *jr, const LaserPlane &lp)
 {
     const Span sp = span_of<Vec<T>::N>(-1, -1, g.n[1] + 1, -1, g.n[2] + 1);
     EFusedArgs<T> a = {f, g, flush, sp, (T)(g.tau / g.h[0]), (T)(g.tau / g.h[1]),
-                       (T)(g.tau / g.h[2]), jr};
+                       (T)(g.tau / g.h[2]), jr, lp};
     return launch_def(s, a, grid_span(sp, g.n[0] + 3));
 }
 
-/* One-plane laser soft source, source/scpic/em_mirror.c:667-693.  The time
- * factors (envelope, cos, sin of the carrier) are evaluated in FP64 on the
- * host once per step and passed in. */
+/* The laser plane as a launch of its own. */
 template <class T> struct LaserArgs {
-    T            *Fy, *Fz;
-    Grid          g;
-    const double *map;
-    int           pos, JMIN, KMIN, KMAX, jmin, jmax, kmin, kmax;
-    const double *tab;
-    const int    *step;
-    int           cap;
+    T         *Fy, *Fz;
+    Grid       g;
+    LaserPlane lp;
 };
 
 template <class T>
-__device__ static inline void laser(T *Fy, T *Fz, const Grid &g, const double *map, int pos,
-                                    int JMIN, int KMIN, int KMAX, int jmin, int jmax, int kmin,
-                                    int kmax, const double *tab, const int *step, int cap, long t)
+__device__ static inline void laser(T *Fy, T *Fz, const Grid &g, const LaserPlane &lp, long t)
 {
-    int nk = kmax - kmin + 1;
-    if (t >= (long)(jmax - jmin + 1) * nk) return;
+    int nk = lp.kmax - lp.kmin + 1;
+    if (t >= (long)(lp.jmax - lp.jmin + 1) * nk) return;
     /* {A, cos, sin} of this step, evaluated on the host */
-    const double *e = tab + 3 * (*step % cap);
+    const double *e = lp.tab + 3 * (*lp.step % lp.cap);
     const double A = e[0], cs = e[1], sn = e[2];
     const unsigned jq = (unsigned)t / (unsigned)nk;
-    int j = jmin + (int)jq, k = kmin + (int)((unsigned)t - jq * (unsigned)nk);
-    long cnt = (long)(j - JMIN) * (KMAX - KMIN + 1) + k - KMIN;
-    const double *w = map + 4 * cnt;
-    long o = gidx(g, pos, j, k);
-    Fy[o] = (T)(Fy[o] + A * (w[2] * cs + w[3] * sn));
-    Fz[o] = (T)(Fz[o] + A * (w[0] * cs + w[1] * sn));
+    int j = lp.jmin + (int)jq, k = lp.kmin + (int)((unsigned)t - jq * (unsigned)nk);
+    long o = gidx(g, lp.pos, j, k);
+    T y = Fy[o], z = Fz[o];
+    laser_node<T>(lp, j, k, A, cs, sn, y, z);
+    Fy[o] = y;
+    Fz[o] = z;
 }
 
 template <class T> struct KernelDef<LaserArgs<T>> {
     static constexpr int kBlock = TPB;
     __device__ __forceinline__ static void body(const LaserArgs<T> &a, VBlock b, VBlock)
     {
-        laser<T>(a.Fy, a.Fz, a.g, a.map, a.pos, a.JMIN, a.KMIN, a.KMAX, a.jmin, a.jmax, a.kmin,
-                 a.kmax, a.tab, a.step, a.cap, (long)b.x * TPB + threadIdx.x);
+        laser<T>(a.Fy, a.Fz, a.g, a.lp, (long)b.x * TPB + threadIdx.x);
     }
 };
 
 template <class T>
-int launch_laser(hipStream_t s, T *Fy, T *Fz, const Grid &g, const double *map, int pos, int JMIN,
-                 int KMIN, int KMAX, int jmin, int jmax, int kmin, int kmax, const double *tab,
-                 const int *step, int cap)
+int launch_laser(hipStream_t s, T *Fy, T *Fz, const Grid &g, const LaserPlane &lp)
 {
-    long n = (long)(jmax - jmin + 1) * (kmax - kmin + 1);
-    if (jmax < jmin || kmax < kmin) return MPIC_OK;
-    LaserArgs<T> a = {Fy, Fz, g, map, pos, JMIN, KMIN, KMAX, jmin, jmax, kmin, kmax, tab, step, cap};
+    if (!lp.on || lp.jmax < lp.jmin || lp.kmax < lp.kmin) return MPIC_OK;
+    long n = (long)(lp.jmax - lp.jmin + 1) * (lp.kmax - lp.kmin + 1);
+    LaserArgs<T> a = {Fy, Fz, g, lp};
     return launch_def(s, a, nblk(n));
 }
 
@@ -1572,16 +1600,14 @@ int launch_probes(hipStream_t s, T *const E[3], T *const H[3], const Grid &g, co
     template int launch_cap_H<T>(hipStream_t, const FieldsT<T> &, const Grid &, const BoxSet &); \
     template int launch_cap_E<T>(hipStream_t, const FieldsT<T> &, const Grid &, const BoxSet &); \
     template int launch_h_update<T>(hipStream_t, const FieldsT<T> &, const Grid &,                \
-                                    const BoxSet &, int, double *);                             \
+                                    const BoxSet &, int, double *, const LaserPlane &);         \
     template int launch_e_fused<T>(hipStream_t, const FieldsT<T> &, const Grid &,                 \
-                                   const BoxSet &, const int *);                                \
+                                   const BoxSet &, const int *, const LaserPlane &);            \
     template int launch_e_update<T>(hipStream_t, const FieldsT<T> &, const Grid &,                \
                                     const BoxSet &, int, int);                                  \
     template int launch_e_tauJ<T>(hipStream_t, const FieldsT<T> &, const Grid &, int *,         \
                                   const int *);         \
-    template int launch_laser<T>(hipStream_t, T *, T *, const Grid &, const double *, int, int,  \
-                                 int, int, int, int, int, int, const double *, const int *,     \
-                                 int);                                                          \
+    template int launch_laser<T>(hipStream_t, T *, T *, const Grid &, const LaserPlane &);       \
     template int launch_spot<T>(hipStream_t, T *, T *, const Grid &, int, const double *,        \
                                 const double *, const double *, const int *, int);              \
     template int launch_energy<T>(hipStream_t, const FieldsT<T> &, const Grid &, double *);      \

@@ -57,6 +57,18 @@ struct BoxSet {
     long off[7];            /* prefix sums of the box volumes */
 };
 
+/* One laser's soft-source plane (source/scpic/em_mirror.c:667-693): the
+ * nodes (pos, j, k), j in [jmin, jmax], k in [kmin, kmax], of its Fy/Fz
+ * pair get A * (map . {cos, sin}) added, A and the carrier's cos and sin of
+ * the step from tab[step % cap]; map rows are numbered from (JMIN, KMIN)
+ * over [KMIN, KMAX].  on = 0: no source. */
+struct LaserPlane {
+    const double *map = nullptr, *tab = nullptr;
+    const int    *step = nullptr;
+    int           on = 0, pos = 0, JMIN = 0, KMIN = 0, KMAX = 0, jmin = 0, jmax = -1, kmin = 0,
+                  kmax = -1, cap = 1;
+};
+
 /* Field arrays of one precision; J is always accumulated in FP64. */
 template <class T>
 struct FieldsT {
@@ -148,19 +160,17 @@ int launch_face(hipStream_t s, const Grid &g, const FaceOp &op);
 int launch_face_pair(hipStream_t s, const Grid &g, const FaceOp &a, const FaceOp &b);
 template <class T>
 int launch_h_update(hipStream_t s, const FieldsT<T> &f, const Grid &g, const BoxSet &flush,
-                    int full, double *part = nullptr);
+                    int full, double *part = nullptr, const LaserPlane &lp = LaserPlane());
 template <class T>
 int launch_e_fused(hipStream_t s, const FieldsT<T> &f, const Grid &g, const BoxSet &flush,
-                   const int *jr);
+                   const int *jr, const LaserPlane &lp = LaserPlane());
 template <class T>
 int launch_e_update(hipStream_t s, const FieldsT<T> &f, const Grid &g, const BoxSet &flush,
                     int interior, int do_flush);
 template <class T>
 int launch_e_tauJ(hipStream_t s, const FieldsT<T> &f, const Grid &g, int *tick, const int *jr);
 template <class T>
-int launch_laser(hipStream_t s, T *Fy, T *Fz, const Grid &g, const double *map, int pos,
-                 int JMIN, int KMIN, int KMAX, int jmin, int jmax, int kmin, int kmax,
-                 const double *tab, const int *step, int cap);
+int launch_laser(hipStream_t s, T *Fy, T *Fz, const Grid &g, const LaserPlane &lp);
 int launch_tick(hipStream_t s, int *step);
 template <class T>
 int launch_tfsf(hipStream_t s, T *const F[3], const Grid &g, const TfsfSet &fs, long maxvol,

@@ -1584,37 +1584,66 @@ static int laser_tables(mpic_run *r, long n)
     return MPIC_OK;
 }
 
+/* Plane and node ranges of laser l's H (h = 1) or E source within box tu
+ * (source/scpic/em_mirror.c:667-693: rows [5, n-5] of the map's
+ * [-1, n+1] x [-1, n+1] frame, H on the source plane, E one plane before it
+ * for a forward beam); on = 0 when the plane lies outside tu. */
+static LaserPlane laser_plane(const mpic_run *r, const Laser &l, int h, const Box &tu)
+{
+    const Grid &g = r->g;
+    LaserPlane lp;
+    const int I = l.forward ? (h ? l.plane : l.plane - 1) : g.n[0] - l.plane + 1;
+    if (I < tu.lo[0] || I > tu.hi[0]) return lp;
+    lp.JMIN = (int)(fmax(-1, 5) + 0.1);
+    const int JMAX = (int)(fmin(g.n[1] + 1, g.n[1] - 5) + 0.1);
+    lp.KMIN = (int)(fmax(-1, 5) + 0.1);
+    lp.KMAX = (int)(fmin(g.n[2] + 1, g.n[2] - 5) + 0.1);
+    lp.jmin = (int)(fmax(tu.lo[1], lp.JMIN) + 0.1);
+    lp.jmax = (int)(fmin(tu.hi[1], JMAX) + 0.1);
+    lp.kmin = (int)(fmax(tu.lo[2], lp.KMIN) + 0.1);
+    lp.kmax = (int)(fmin(tu.hi[2], lp.KMAX) + 0.1);
+    lp.pos = I;
+    lp.map = h ? l.dmap : l.dmap + 4 * l.nodes;
+    lp.tab = h ? l.dtab : l.dtab + 3 * LASER_CAP;
+    lp.step = r->dstep;
+    lp.cap = LASER_CAP;
+    lp.on = lp.jmax >= lp.jmin && lp.kmax >= lp.kmin;
+    return lp;
+}
+
+/* The box over which the H source applies (em_HStep's update box) and the
+ * one of the E source. */
+static Box laser_box_H(const mpic_run *r)
+{
+    return Box{{3, 3, 3}, {r->g.n[0] - 2, r->g.n[1] - 2, r->g.n[2] - 2}};
+}
+static Box laser_box_E(const mpic_run *r)
+{
+    return Box{{-1, -1, -1}, {r->g.n[0] + 1, r->g.n[1] + 1, r->g.n[2] + 1}};
+}
+
+/* The run's laser folded into the H step (h = 1) or the fused E update:
+ * only with exactly one laser and no TFSF interface, whose source would
+ * come between the update and the laser's. */
+static LaserPlane laser_fold(const mpic_run *r, int h)
+{
+    if (r->lasers.size() != 1 || r->tfsf.on) return LaserPlane();
+    return laser_plane(r, r->lasers[0], h, h ? laser_box_H(r) : laser_box_E(r));
+}
+
 template <class T>
 static int laser_H(mpic_run *r, T *Fy, T *Fz, const Box &tu)
 {
-    const Grid &g = r->g;
-    for (const Laser &l : r->lasers) {
-        const int I = l.forward ? l.plane : g.n[0] - l.plane + 1;
-        if (I < tu.lo[0] || I > tu.hi[0]) continue;
-        const int JMIN = (int)(fmax(-1, 5) + 0.1), JMAX = (int)(fmin(g.n[1] + 1, g.n[1] - 5) + 0.1);
-        const int KMIN = (int)(fmax(-1, 5) + 0.1), KMAX = (int)(fmin(g.n[2] + 1, g.n[2] - 5) + 0.1);
-        const int jmin = (int)(fmax(tu.lo[1], JMIN) + 0.1), jmax = (int)(fmin(tu.hi[1], JMAX) + 0.1);
-        const int kmin = (int)(fmax(tu.lo[2], KMIN) + 0.1), kmax = (int)(fmin(tu.hi[2], KMAX) + 0.1);
-        TRY(launch_laser(r->stream, Fy, Fz, g, l.dmap, I, JMIN, KMIN, KMAX, jmin, jmax, kmin, kmax,
-                         l.dtab, r->dstep, LASER_CAP));
-    }
+    for (const Laser &l : r->lasers)
+        TRY(launch_laser(r->stream, Fy, Fz, r->g, laser_plane(r, l, 1, tu)));
     return MPIC_OK;
 }
 
 template <class T>
 static int laser_E(mpic_run *r, T *Ey, T *Ez, const Box &tu)
 {
-    const Grid &g = r->g;
-    for (const Laser &l : r->lasers) {
-        const int I = l.forward ? l.plane - 1 : g.n[0] - l.plane + 1;
-        if (I < tu.lo[0] || I > tu.hi[0]) continue;
-        const int JMIN = (int)(fmax(-1, 5) + 0.1), JMAX = (int)(fmin(g.n[1] + 1, g.n[1] - 5) + 0.1);
-        const int KMIN = (int)(fmax(-1, 5) + 0.1), KMAX = (int)(fmin(g.n[2] + 1, g.n[2] - 5) + 0.1);
-        const int jmin = (int)(fmax(tu.lo[1], JMIN) + 0.1), jmax = (int)(fmin(tu.hi[1], JMAX) + 0.1);
-        const int kmin = (int)(fmax(tu.lo[2], KMIN) + 0.1), kmax = (int)(fmin(tu.hi[2], KMAX) + 0.1);
-        TRY(launch_laser(r->stream, Ey, Ez, g, l.dmap + 4 * l.nodes, I, JMIN, KMIN, KMAX, jmin, jmax,
-                         kmin, kmax, l.dtab + 3 * LASER_CAP, r->dstep, LASER_CAP));
-    }
+    for (const Laser &l : r->lasers)
+        TRY(launch_laser(r->stream, Ey, Ez, r->g, laser_plane(r, l, 0, tu)));
     return MPIC_OK;
 }
 
@@ -1724,7 +1753,9 @@ extern "C" int mpic_em_h_step(mpic_run_t *r)
     if (!r) return MPIC_ERR_ARG;
     TRY(check_stage(r, 1));
     MPIC_CHECK(hipSetDevice(r->dev));
-    FCALL(launch_h_update, r->g, r->flH, 1);
+    /* the laser's H source in the same pass where nothing comes between */
+    const LaserPlane lf = laser_fold(r, 1);
+    FCALL(launch_h_update, r->g, r->flH, 1, nullptr, lf);
     if (r->tfsf.on) {                    /* TFSF_postHStep, source/core/em.c:125 */
         const Tfsf &tf = r->tfsf;
         if (r->fp32)
@@ -1734,11 +1765,13 @@ extern "C" int mpic_em_h_step(mpic_run_t *r)
             TRY(launch_tfsf(r->stream, r->f.H, r->g, tf.fh, tf.maxvol, tf.ddata, tf.dtab,
                             r->dstep, LASER_CAP, 0, 1.0));
     }
-    Box inner = {{3, 3, 3}, {r->g.n[0] - 2, r->g.n[1] - 2, r->g.n[2] - 2}};
-    if (r->fp32)
-        TRY(laser_H(r, r->f32.H[1], r->f32.H[2], inner));
-    else
-        TRY(laser_H(r, r->f.H[1], r->f.H[2], inner));
+    if (!lf.on) {
+        const Box inner = laser_box_H(r);
+        if (r->fp32)
+            TRY(laser_H(r, r->f32.H[1], r->f32.H[2], inner));
+        else
+            TRY(laser_H(r, r->f.H[1], r->f.H[2], inner));
+    }
     r->stage = 2;
     return MPIC_OK;
 }
@@ -1786,7 +1819,7 @@ extern "C" int mpic_set_vsp(mpic_run_t *r, double alpha)
 
 /* The sources em_EStep_finish adds to E: the TFSF interface
  * (TFSF_postEStep, source/core/em.c:200) and the lasers. */
-static int e_sources(mpic_run *r)
+static int e_sources(mpic_run *r, bool laser_done = false)
 {
     if (r->tfsf.on) {                    /* TFSF_postEStep, source/core/em.c:200 */
         const Tfsf &tf = r->tfsf;
@@ -1798,7 +1831,8 @@ static int e_sources(mpic_run *r)
             TRY(launch_tfsf(r->stream, r->f.E, r->g, tf.fe, tf.maxvol, tf.ddata, tf.dtab,
                             r->dstep, LASER_CAP, 1, sign));
     }
-    Box dom = {{-1, -1, -1}, {r->g.n[0] + 1, r->g.n[1] + 1, r->g.n[2] + 1}};
+    if (laser_done) return MPIC_OK;     /* added by the fused E update */
+    const Box dom = laser_box_E(r);
     if (r->fp32)
         TRY(laser_E(r, r->f32.E[1], r->f32.E[2], dom));
     else
@@ -1873,8 +1907,11 @@ static int step_body(mpic_run *r)
      * laser sources then add after E -= tau J) */
     const bool fused = r->fp32 || (!r->tfsf.on && r->lasers.empty());
     if (fused) {
-        FCALL(launch_e_fused, r->g, r->flE, r->djr + 4);
-        TRY(e_sources(r));
+        /* the laser's E source in the same pass where no TFSF interface
+         * comes first (FP32 split form: the sources add after E -= tau J) */
+        const LaserPlane lf = laser_fold(r, 0);
+        FCALL(launch_e_fused, r->g, r->flE, r->djr + 4, lf);
+        TRY(e_sources(r, lf.on != 0));
     } else {
         TRY(mpic_em_e_start(r));
         TRY(e_sources(r));
