@@ -7,7 +7,7 @@
  *         -1 deleted), ox oy oz FP32 offset inside the cell, cell units
  *   mom   ux uy uz FP32 momentum gamma*v/c, w FP32 charge density of the
  *         marker (the reference's rho)
- *   id    int32 reference marker id (moved by the sort only)
+ *   id    int32 reference marker id (carried along by the push)
  * Position = (cell index + offset) * h: the integer part carries the
  * position exactly, so FP32 resolves 6e-8 of a cell anywhere in the box.
  *
@@ -26,9 +26,17 @@
  * flushed to the FP64 J in HBM once per tile.  The tile's E/H nodes are
  * staged in LDS, so gathers never leave the CU.
  *
+ * Order: the markers of a step are taken in tile-major cell order through a
+ * permutation (indices into the current buffer) that the sort's rank pass
+ * builds from the push's new keys; the push writes each pushed marker to
+ * the other buffer at its position in that order.  So no pass moves the
+ * markers only to sort them: the next step reads the other buffer through
+ * the next permutation.
+ *
  * Roofline: HBM-bound.  Algorithmic bytes per marker-step 32 B read + 32 B
- * written (id is not touched), plus per occupied cell 24 B of FP32 E/H reads
- * and 48 B of FP64 J read-modify-write.
+ * written, plus per occupied cell 24 B of FP32 E/H reads and 48 B of FP64 J
+ * read-modify-write (the id, the permutation index and the new key are
+ * 4 B each on top).
  */
 #include "mpic_internal.h"
 #include "mpic_launch.h"
@@ -199,6 +207,7 @@ __device__ static inline float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
 struct MarkerRegs {
     int   word;
     float ox, oy, oz, ux, uy, uz, w;
+    int   id;
 };
 
 /* Element p of a 4-byte array as base + 32-bit byte offset, so the access
@@ -210,15 +219,20 @@ __device__ static inline T &at(T *base, unsigned p)
     return *(T *)((char *)base + (p << 2));
 }
 
-/* Loads marker min(p, last): unconditional, so that every path through the
- * push loop issues the same memory operations and the compiler's wait
- * counts stay exact (a skippable load makes it wait for later loads too).
- * Lanes past the tile's end carry a duplicate, masked by the caller. */
-__device__ static inline void load_marker(const Markers32 &m, unsigned p, unsigned last,
-                                          MarkerRegs &r)
+/* Index of position min(p, last) of the order: unconditional, so that every
+ * path through the push loop issues the same memory operations and the
+ * compiler's wait counts stay exact (a skippable load makes it wait for
+ * later loads too).  Lanes past the item's end carry a duplicate, masked by
+ * the caller. */
+__device__ static inline unsigned perm_at(const int *perm, unsigned p, unsigned last)
 {
-    p = p < last ? p : last;
-    const float4 a = m.pos[p], b = m.mom[p];
+    return (unsigned)perm[p < last ? p : last];
+}
+
+/* The records of marker s of the current buffer. */
+__device__ static inline void load_marker(const Markers32 &m, unsigned s, MarkerRegs &r)
+{
+    const float4 a = m.pos[s], b = m.mom[s];
     r.word = __float_as_int(a.x);
     r.ox = a.y;
     r.oy = a.z;
@@ -227,6 +241,7 @@ __device__ static inline void load_marker(const Markers32 &m, unsigned p, unsign
     r.uy = b.y;
     r.uz = b.z;
     r.w = b.w;
+    r.id = m.id[s];
 }
 
 /* Trilinear gather from a tile's staged field block (TFN^3 nodes, node
@@ -441,11 +456,15 @@ struct Pending {
     int        ckey, clen;/* count run head: key (-1 none), length */
 };
 
-__device__ static inline void write_back(const Markers32 &m, const TileView &tv, const Pending &q)
+/* Position p of the order in the output buffer d: the records, the id and
+ * the new key (-1: deleted, which the rank pass leaves out). */
+__device__ static inline void write_back(const Markers32 &d, const TileView &tv, const Pending &q)
 {
     if (q.valid) {
-        m.pos[q.p] = make_float4(__int_as_float(q.word), q.r.ox, q.r.oy, q.r.oz);
-        if (q.word >= 0) m.mom[q.p] = make_float4(q.r.ux, q.r.uy, q.r.uz, q.r.w);
+        d.pos[q.p] = make_float4(__int_as_float(q.word), q.r.ox, q.r.oy, q.r.oz);
+        if (q.word >= 0) d.mom[q.p] = make_float4(q.r.ux, q.r.uy, q.r.uz, q.r.w);
+        d.id[q.p] = q.r.id;
+        tv.key[q.p] = q.word < 0 ? -1 : (q.word & KEY_MASK);
     }
     if (q.ckey >= 0) atomicAdd(tv.count + q.ckey, q.clen);
 }
@@ -461,7 +480,7 @@ __device__ static inline void write_back(const Markers32 &m, const TileView &tv,
  * different occupancy (workgroups finish at different times and new ones
  * start); each wavefront loads its next batch before pushing the current. */
 struct Push32Args {
-    Markers32    m;
+    Markers32    m, d;         /* current buffer (read through tv.perm), output buffer */
     F32View      v;
     Push32Consts c;
     TileView     tv;
@@ -470,9 +489,10 @@ struct Push32Args {
 };
 
 template <int W>
-__device__ __forceinline__ static void push32_tiles(const Markers32 &m, const F32View &v,
-                                           const Push32Consts &c, const TileView &tv, float *tj,
-                                           double *Wacc, const VBlock &bk, const VBlock &gd)
+__device__ __forceinline__ static void push32_tiles(const Markers32 &m, const Markers32 &d,
+                                           const F32View &v, const Push32Consts &c,
+                                           const TileView &tv, float *tj, double *Wacc,
+                                           const VBlock &bk, const VBlock &gd)
 {
     constexpr int BS = TW * W;                       /* threads of the workgroup */
     constexpr int NSTG = (TFN3 + BS - 1) / BS;       /* staged nodes per thread and component */
@@ -535,8 +555,11 @@ __device__ __forceinline__ static void push32_tiles(const Markers32 &m, const F3
         const unsigned tb = tv.toff[t], cnt = tv.toff[t + 1] - tb;
         const unsigned beg = tb + (unsigned)((unsigned long long)u * cnt / nu),
                        end = tb + (unsigned)((unsigned long long)(u + 1) * cnt / nu);
+        /* the order's indices are loaded two batches ahead and the records
+         * one batch ahead, so no load waits for an index just loaded */
         MarkerRegs cur;
-        load_marker(m, beg + threadIdx.x, end - 1, cur);
+        load_marker(m, perm_at(tv.perm, beg + threadIdx.x, end - 1), cur);
+        unsigned snx = perm_at(tv.perm, beg + BS + threadIdx.x, end - 1);
         /* everything outstanding completes here, so that the loop's waits
          * only ever concern the prefetch of the batch in flight */
         __builtin_amdgcn_s_waitcnt(0);
@@ -547,8 +570,9 @@ __device__ __forceinline__ static void push32_tiles(const Markers32 &m, const F3
         for (unsigned base = beg; base < end; base += BS) {
             const unsigned p = base + threadIdx.x;
             MarkerRegs nxt;
-            load_marker(m, p + BS, end - 1, nxt);
-            write_back(m, tv, pend);
+            load_marker(m, snx, nxt);
+            snx = perm_at(tv.perm, p + 2 * BS, end - 1);
+            write_back(d, tv, pend);
             int rkey = -1, newword = -1, ci = 0, cj = 0, ck = 0;
             Seg12 first;
             for (int q = 0; q < 12; ++q) first.v[q] = 0.0f;
@@ -563,6 +587,7 @@ __device__ __forceinline__ static void push32_tiles(const Markers32 &m, const F3
                 newword = push32_marker(pend.r, cur, li, lj_, lk, ci, cj, ck, lf, lqm, lj, o0, o1, o2,
                                         v, c, tv, tp, first, w0, w1, w2);
             }
+            pend.r.id = cur.id;
             pend.valid = in;
             pend.p = p;
             pend.word = newword;
@@ -581,7 +606,7 @@ __device__ __forceinline__ static void push32_tiles(const Markers32 &m, const F3
             pend.clen = len;
             cur = nxt;
         }
-        write_back(m, tv, pend);
+        write_back(d, tv, pend);
         __syncthreads();
         /* the tile's J block, stored whole (coalesced, no atomics) */
         float *dst = tj + (long)it * TJB;
@@ -611,7 +636,7 @@ template <int W> struct KernelDef<Push32ArgsW<W>> {
     static constexpr int kWaves = 4;
     __device__ __forceinline__ static void body(const Push32ArgsW<W> &a, VBlock b, VBlock gd)
     {
-        push32_tiles<W>(a.m, a.v, a.c, a.tv, a.tj, a.Wacc, b, gd);
+        push32_tiles<W>(a.m, a.d, a.v, a.c, a.tv, a.tj, a.Wacc, b, gd);
     }
 };
 
@@ -732,9 +757,9 @@ template <> struct KernelDef<JGatherArgs> {
     }
 };
 
-int launch_push32(hipStream_t s, const Markers32 &m, const Fields32 &f, const Grid &g,
-                  const PushConsts &pc, const float *qm, const TileView &tv, float *tj,
-                  double *Wacc, int nblocks, int waves)
+int launch_push32(hipStream_t s, const Markers32 &m, const Markers32 &dst, const Fields32 &f,
+                  const Grid &g, const PushConsts &pc, const float *qm, const TileView &tv,
+                  float *tj, double *Wacc, int nblocks, int waves)
 {
     F32View v;
     for (int a = 0; a < 3; ++a) {
@@ -752,7 +777,7 @@ int launch_push32(hipStream_t s, const Markers32 &m, const Fields32 &f, const Gr
     c.vol = g.h[0] * g.h[1] * g.h[2];
     for (int q = 0; q < 4; ++q) c.qm[q] = qm[q];
     for (int b = 0; b < 6; ++b) c.bc[b] = pc.bc[b];
-    const Push32Args a = {m, v, c, tv, tj, Wacc};
+    const Push32Args a = {m, dst, v, c, tv, tj, Wacc};
     switch (waves) {
     case 1: return launch_def(s, Push32ArgsW<1>{a}, (unsigned)nblocks);
     case 2: return launch_def(s, Push32ArgsW<2>{a}, (unsigned)nblocks);
@@ -777,15 +802,17 @@ int launch_jgather32(hipStream_t s, const Fields32 &f, const Grid &g, const Tile
     return launch_def(s, a, (unsigned)nblocks);
 }
 
-/* Keys of the current markers (the cell word already holds them): counts.
- * Only the first nmark[parity] markers are live; the buffer past them holds
- * records of earlier orders, which must not be counted. */
+/* Keys and counts of the current buffer's first nmark[parity] markers
+ * (the cell word holds the key; a deleted marker's is -1): the start of a
+ * sort that moves nothing (loaded markers, or a re-sort). */
 __global__ void k_keys32(Markers32 m, long n, TileView tv, int parity)
 {
     const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long live = tv.nmark[parity];
     const int word = (p < n && p < live) ? ((const int *)m.pos)[4 * p] : -1;
-    wave_run_count(tv.count, word < 0 ? -1 : (word & KEY_MASK));
+    const int key = word < 0 ? -1 : (word & KEY_MASK);
+    if (p < n) tv.key[p] = key;
+    wave_run_count(tv.count, key);
 }
 
 int launch_keys32(hipStream_t s, const Markers32 &m, long n, const TileView &tv, int parity)
@@ -796,27 +823,20 @@ int launch_keys32(hipStream_t s, const Markers32 &m, long n, const TileView &tv,
     return MPIC_OK;
 }
 
-/* The scatter streams every marker once: nontemporal loads and stores keep
- * it from displacing the field arrays in the caches. */
-typedef float v4f_nt __attribute__((ext_vector_type(4)));
-struct Scatter32Args {
-    Markers32 src, dst;
-    long      nbound;
-    TileView  tv;
-    int       parity;
+/* The next order: every position p < nmark[parity] with a key (the push's
+ * output, or the markers keyed above) takes slot cursor[key]++ among its
+ * key's, one atomic per run of equal keys in a wavefront, and perm[slot] =
+ * p.  4 B read and 4 B written per marker instead of moving its 36 B. */
+struct Rank32Args {
+    TileView tv;
+    long     nbound;
+    int      parity;
 };
 
-__device__ static inline void scatter32(const Markers32 &src, const Markers32 &dst, long nbound,
-                                        const TileView &tv, int parity, long p)
+__device__ static inline void rank32(const TileView &tv, long nbound, int parity, long p)
 {
     const long n = tv.nmark[parity];
-    float4 a = make_float4(__int_as_float(-1), 0.0f, 0.0f, 0.0f);
-    if (p < n && p < nbound) {
-        const v4f_nt t = __builtin_nontemporal_load((const v4f_nt *)(src.pos + p));
-        a = make_float4(t.x, t.y, t.z, t.w);
-    }
-    const int word = __float_as_int(a.x);
-    const int key = word < 0 ? -1 : (word & KEY_MASK);
+    const int key = (p < n && p < nbound) ? tv.key[p] : -1;
     int start, len;
     bool head;
     wave_run(key, start, len, head);
@@ -824,36 +844,30 @@ __device__ static inline void scatter32(const Markers32 &src, const Markers32 &d
     if (head && key >= 0) base = atomicAdd(tv.cursor + key, len);
     base = __shfl(base, start, 64);
     if (key < 0) return;
-    const long d = base + ((int)(threadIdx.x & 63) - start);
-    const v4f_nt mo = __builtin_nontemporal_load((const v4f_nt *)(src.mom + p));
-    const int idv = __builtin_nontemporal_load(src.id + p);
-    const v4f_nt ao = {a.x, a.y, a.z, a.w};
-    __builtin_nontemporal_store(ao, (v4f_nt *)(dst.pos + d));
-    __builtin_nontemporal_store(mo, (v4f_nt *)(dst.mom + d));
-    __builtin_nontemporal_store(idv, dst.id + d);
+    tv.perm[base + ((int)(threadIdx.x & 63) - start)] = (int)p;
 }
 
-template <> struct KernelDef<Scatter32Args> {
+template <> struct KernelDef<Rank32Args> {
     static constexpr int kBlock = 256;
-    __device__ __forceinline__ static void body(const Scatter32Args &a, VBlock b, VBlock)
+    __device__ __forceinline__ static void body(const Rank32Args &a, VBlock b, VBlock)
     {
-        scatter32(a.src, a.dst, a.nbound, a.tv, a.parity, (long)b.x * 256 + threadIdx.x);
+        rank32(a.tv, a.nbound, a.parity, (long)b.x * 256 + threadIdx.x);
     }
 };
 
-int launch_scatter32(hipStream_t s, const Markers32 &src, const Markers32 &dst, long nbound,
-                     const TileView &tv, int parity)
+int launch_rank32(hipStream_t s, long nbound, const TileView &tv, int parity)
 {
     if (nbound <= 0) return MPIC_OK;
-    Scatter32Args a = {src, dst, nbound, tv, parity};
+    Rank32Args a = {tv, nbound, parity};
     return launch_def(s, a, (unsigned)((nbound + 255) / 256));
 }
 
-/* CIC charge density of the split-precision markers, FP64. */
-__global__ void k_rho32(Markers32 m, long n, double *rho, Grid g, TileView tv)
+/* CIC charge density of the split-precision markers of the order, FP64. */
+__global__ void k_rho32(Markers32 m, long n, double *rho, Grid g, TileView tv, int parity)
 {
-    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n || (tv.perm && q >= tv.nmark[parity])) return;
+    const long p = tv.perm ? tv.perm[q] : q;            /* no permutation: the loaded order */
     const float4 a = m.pos[p];
     const int word = __float_as_int(a.x);
     if (word < 0) return;
@@ -871,34 +885,36 @@ __global__ void k_rho32(Markers32 m, long n, double *rho, Grid g, TileView tv)
 }
 
 int launch_rho32(hipStream_t s, const Markers32 &m, long n, double *rho, const Grid &g,
-                 const TileView &tv)
+                 const TileView &tv, int parity)
 {
     if (n <= 0) return MPIC_OK;
-    k_rho32<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(m, n, rho, g, tv);
+    k_rho32<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(m, n, rho, g, tv, parity);
     MPIC_CHECK(hipGetLastError());
     return MPIC_OK;
 }
 
-/* Natural cell index (i*n1 + j)*n2 + k of every marker (-1: deleted). */
-__global__ void k_cells32(Markers32 m, long n, Grid g, TileView tv, int32_t *cell)
+/* Natural cell index (i*n1 + j)*n2 + k of every marker of the order
+ * (-1: deleted). */
+__global__ void k_cells32(Markers32 m, long n, Grid g, TileView tv, int parity, int32_t *cell)
 {
-    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    const int word = ((const int *)m.pos)[4 * p];
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const long p = tv.perm ? (q < tv.nmark[parity] ? tv.perm[q] : -1) : q;
+    const int word = p >= 0 ? ((const int *)m.pos)[4 * p] : -1;
     if (word < 0) {
-        cell[p] = -1;
+        cell[q] = -1;
         return;
     }
     int i, j, k;
     key_to_ijk(tv, word & KEY_MASK, i, j, k);
-    cell[p] = (i * g.n[1] + j) * g.n[2] + k;
+    cell[q] = (i * g.n[1] + j) * g.n[2] + k;
 }
 
 int launch_cells32(hipStream_t s, const Markers32 &m, long n, const Grid &g, const TileView &tv,
-                   int32_t *cell)
+                   int parity, int32_t *cell)
 {
     if (n <= 0) return MPIC_OK;
-    k_cells32<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(m, n, g, tv, cell);
+    k_cells32<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(m, n, g, tv, parity, cell);
     MPIC_CHECK(hipGetLastError());
     return MPIC_OK;
 }

@@ -355,7 +355,9 @@ extern "C" int mpic_footprint(const mpic_config_t *cfg, size_t *bytes)
     if (!cfg || !bytes) return MPIC_ERR_ARG;
     const size_t cells = (size_t)(cfg->n[0] + 5) * (cfg->n[1] + 5) * row_pitch(cfg->n[2]);
     const size_t tiles = (size_t)((cfg->n[0] + 3) / 4) * ((cfg->n[1] + 3) / 4) * ((cfg->n[2] + 3) / 4);
-    const size_t mk = (cfg->precision == MPIC_FP32_SPLIT) ? (4 + 7 * 4 + 4) : (6 * 8 + 2 * 4 + 4 + 4);
+    /* per marker and buffer: records and id; plus the new key (and, FP32, the
+     * order's index) once */
+    const size_t mk = (cfg->precision == MPIC_FP32_SPLIT) ? (4 + 7 * 4 + 4 + 4) : (6 * 8 + 2 * 4 + 4 + 4);
     const int fp32 = cfg->precision == MPIC_FP32_SPLIT;
     const long cap = cfg->max_markers > 0 ? cfg->max_markers : 1;
     const size_t items = (size_t)items_capacity((int)tiles, cap, fp32 ? push32_item_markers() : 0);
@@ -486,7 +488,9 @@ extern "C" int mpic_create(const mpic_config_t *cfg, mpic_run_t **out)
     MPIC_CHECK(hipMalloc(&tv.count, sizeof(int) * tv.ncellp));
     MPIC_CHECK(hipMemsetAsync(tv.count, 0, sizeof(int) * tv.ncellp, r->stream));
     MPIC_CHECK(hipMemsetAsync(tv.off, 0, sizeof(int) * (tv.ncellp + 1), r->stream));
-    MPIC_CHECK(hipMalloc(&tv.key, sizeof(int) * (r->fp32 ? 1 : r->cap)));
+    MPIC_CHECK(hipMalloc(&tv.key, sizeof(int) * r->cap));
+    tv.perm = nullptr;
+    if (r->fp32) MPIC_CHECK(hipMalloc(&tv.perm, sizeof(int) * r->cap));
     MPIC_CHECK(hipMalloc(&tv.active, sizeof(int) * tv.ntiles));
     MPIC_CHECK(hipMalloc(&tv.slot, sizeof(int) * tv.ntiles));
     tv.chm = r->fp32 ? push32_item_markers() : 0;
@@ -589,6 +593,7 @@ extern "C" int mpic_destroy(mpic_run_t *r)
         hipFree(m3.pos); hipFree(m3.mom); hipFree(m3.id);
     }
     hipFree(r->tv.off); hipFree(r->tv.toff); hipFree(r->tv.cursor); hipFree(r->tv.count); hipFree(r->tv.key);
+    hipFree(r->tv.perm);
     hipFree(r->tv.active); hipFree(r->tv.slot); hipFree(r->tv.nactive); hipFree(r->blocksum);
     hipFree(r->tv.item); hipFree(r->tv.istart);
     hipFree(r->tj);
@@ -959,14 +964,33 @@ static int load_split(mpic_run *r, std::vector<double> *d, std::vector<float> &w
 static int save_split(mpic_run *r, void *rec, long *n, int recsize, long N)
 {
     Markers32 &M = r->m32;
-    std::vector<int32_t> id(N);
-    std::vector<float> pos(4 * N), mom(4 * N);
-    MPIC_CHECK(hipMemcpy(pos.data(), M.pos, sizeof(float4) * N, hipMemcpyDeviceToHost));
-    MPIC_CHECK(hipMemcpy(mom.data(), M.mom, sizeof(float4) * N, hipMemcpyDeviceToHost));
-    MPIC_CHECK(hipMemcpy(id.data(), M.id, sizeof(int32_t) * N, hipMemcpyDeviceToHost));
+    /* the N markers of the order: through the permutation once initialised
+     * (the buffer's first E records are written), else the loaded records */
+    long E = N;
+    std::vector<int32_t> order;
+    if (r->inited) {
+        int ext = 0;
+        MPIC_CHECK(hipMemcpy(&ext, r->tv.nmark + (r->parity ^ 1), sizeof(int), hipMemcpyDeviceToHost));
+        E = ext;
+        order.resize(N);
+        if (N)
+            MPIC_CHECK(hipMemcpy(order.data(), r->tv.perm, sizeof(int32_t) * N, hipMemcpyDeviceToHost));
+    }
+    std::vector<int32_t> id(E);
+    std::vector<float> pos(4 * E), mom(4 * E);
+    if (E) {
+        MPIC_CHECK(hipMemcpy(pos.data(), M.pos, sizeof(float4) * E, hipMemcpyDeviceToHost));
+        MPIC_CHECK(hipMemcpy(mom.data(), M.mom, sizeof(float4) * E, hipMemcpyDeviceToHost));
+        MPIC_CHECK(hipMemcpy(id.data(), M.id, sizeof(int32_t) * E, hipMemcpyDeviceToHost));
+    }
     char *dst = (char *)rec;
     long o = 0;
-    for (long q = 0; q < N; ++q) {
+    for (long u = 0; u < N; ++u) {
+        const long q = r->inited ? order[u] : u;
+        if (q < 0 || q >= E) {
+            mpic_set_error("mpic_markers_save: order entry %ld names record %ld of %ld", u, q, E);
+            return MPIC_ERR_STATE;
+        }
         int32_t word;
         memcpy(&word, &pos[4 * q], sizeof(word));
         if (word < 0) continue;
@@ -1408,9 +1432,25 @@ static int sort_pass(mpic_run *r, int keys)
 {
     const int other = r->cur ^ 1;
     if (r->fp32) {
-        if (keys) TRY(launch_keys32(r->stream, r->m32, r->n, r->tv, r->parity));
+        /* The order is a permutation of the current buffer: tv.nmark[parity]
+         * markers, the buffer's first tv.nmark[parity ^ 1] records written.
+         * After a push (keys == 0) the push's output buffer and keys are
+         * ranked and become current; keys == 1 ranks the current buffer in
+         * place (loaded markers: nmark[parity] set by the caller, or a
+         * re-sort: the buffer's extent), moving nothing. */
+        if (keys) {
+            if (r->inited)
+                MPIC_CHECK(hipMemcpyAsync(r->tv.nmark + r->parity, r->tv.nmark + (r->parity ^ 1),
+                                          sizeof(int), hipMemcpyDeviceToDevice, r->stream));
+            TRY(launch_keys32(r->stream, r->m32, r->n, r->tv, r->parity));
+        }
         TRY(launch_tile_scan(r->stream, r->tv, r->blocksum, r->parity ^ 1));
-        TRY(launch_scatter32(r->stream, r->m32, r->m32buf[other], r->n, r->tv, r->parity));
+        TRY(launch_rank32(r->stream, r->n, r->tv, r->parity));
+        if (keys)
+            r->parity ^= 1;
+        else
+            swap_markers(r);
+        return MPIC_OK;
     } else {
         if (keys) TRY(launch_keys64(r->stream, r->m, r->n, r->g, r->tv, r->parity));
         TRY(launch_tile_scan(r->stream, r->tv, r->blocksum, r->parity ^ 1));
@@ -1716,7 +1756,8 @@ extern "C" int mpic_push_deposit(mpic_run_t *r)
                       r->tiled ? r->tv.nt[1] * r->tv.nt[2] : 1, r->tiled ? 0 : 1));
     if (r->timed >= 0) TRY(record(r, 1));
     if (r->fp32) {
-        TRY(launch_push32(r->stream, r->m32, r->f32, r->g, r->pc, r->qm, r->tv, r->tj, r->dacc + 2,
+        TRY(launch_push32(r->stream, r->m32, r->m32buf[r->cur ^ 1], r->f32, r->g, r->pc, r->qm,
+                          r->tv, r->tj, r->dacc + 2,
                           r->tile_blocks, r->tile_waves));
         if (r->timed >= 0) TRY(record(r, 2));
         TRY(launch_jgather32(r->stream, r->f32, r->g, r->tv, r->tj, r->gather_blocks));
@@ -2566,13 +2607,22 @@ extern "C" int mpic_diag_history(mpic_run_t *r, double *rows, int max_rows, int 
 }
 
 /* ---------------- rho, Gauss residuals, cells ---------------- */
+/* The tile view for kernels that visit the markers of the order: before
+ * mpic_init the loaded records are the order (no permutation yet). */
+static TileView order_view(const mpic_run *r)
+{
+    TileView tv = r->tv;
+    if (!r->inited) tv.perm = nullptr;
+    return tv;
+}
+
 static int rho_into(mpic_run *r, double *rho)
 {
     long n = 0;
     TRY(live_n(r, &n));
     TRY(launch_zero(r->stream, rho, r->g.ncell));
     if (r->fp32)
-        TRY(launch_rho32(r->stream, r->m32, n, rho, r->g, r->tv));
+        TRY(launch_rho32(r->stream, r->m32, n, rho, r->g, order_view(r), r->parity));
     else
         TRY(launch_rho64(r->stream, r->m, n, rho, r->g));
     const int *bc = r->cfg.bc;
@@ -2634,7 +2684,7 @@ extern "C" int mpic_marker_cells(mpic_run_t *r, int32_t *cells, long *n)
     int32_t *d = nullptr;
     MPIC_CHECK(hipMallocAsync((void **)&d, sizeof(int32_t) * (N ? N : 1), r->stream));
     if (r->fp32)
-        TRY(launch_cells32(r->stream, r->m32, N, r->g, r->tv, d));
+        TRY(launch_cells32(r->stream, r->m32, N, r->g, order_view(r), r->parity, d));
     else
         TRY(launch_cells64(r->stream, r->m, N, r->g, d));
     std::vector<int32_t> all(N);

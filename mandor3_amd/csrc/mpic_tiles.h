@@ -25,6 +25,8 @@ struct TileView {
     int *cursor;             /* [ncellp] scatter cursors */
     int *count;              /* [ncellp] markers per key after the push (zeroed by the scan) */
     int *key;                /* [capacity] new key of every marker, -1 = deleted */
+    int *perm;               /* FP32: [capacity] the current order as indices into the
+                              * current buffer (nmark[parity] of them) */
     int *active;             /* [ntiles] tiles holding markers, in tile order */
     int *slot;               /* [ntiles] position of a tile in 'active', -1 if empty */
     int *nactive;            /* [1] */
@@ -130,17 +132,16 @@ int launch_counts_natural(hipStream_t s, const TileView &tv, const Grid &g, int3
 #define TJB (TJN * TJN * TJN * 3)
 
 /* nblocks workgroups of 'waves' (1, 2 or 4) wavefronts each. */
-int launch_push32(hipStream_t s, const Markers32 &m, const Fields32 &f, const Grid &g,
-                  const PushConsts &pc, const float *qm, const TileView &tv, float *tj,
-                  double *Wacc, int nblocks, int waves);
+int launch_push32(hipStream_t s, const Markers32 &m, const Markers32 &dst, const Fields32 &f,
+                  const Grid &g, const PushConsts &pc, const float *qm, const TileView &tv,
+                  float *tj, double *Wacc, int nblocks, int waves);
 int launch_jgather32(hipStream_t s, const Fields32 &f, const Grid &g, const TileView &tv,
                      const float *tj, int nblocks);
 int launch_keys32(hipStream_t s, const Markers32 &m, long n, const TileView &tv, int parity);
-int launch_scatter32(hipStream_t s, const Markers32 &src, const Markers32 &dst, long nbound,
-                     const TileView &tv, int parity);
+int launch_rank32(hipStream_t s, long nbound, const TileView &tv, int parity);
 int launch_rho32(hipStream_t s, const Markers32 &m, long n, double *rho, const Grid &g,
-                 const TileView &tv);
+                 const TileView &tv, int parity);
 int launch_cells32(hipStream_t s, const Markers32 &m, long n, const Grid &g, const TileView &tv,
-                   int32_t *cell);
+                   int parity, int32_t *cell);
 
 #endif
