@@ -229,10 +229,15 @@ __device__ static inline unsigned perm_at(const int *perm, unsigned p, unsigned 
     return (unsigned)perm[p < last ? p : last];
 }
 
-/* The records of marker s of the current buffer. */
+/* The records of marker s of the current buffer.  Marker records are read
+ * once and written once per step: nontemporal, so that they do not push the
+ * field arrays (which every kernel of the step reads again) out of the
+ * caches. */
+typedef float v4f_nt __attribute__((ext_vector_type(4)));
 __device__ static inline void load_marker(const Markers32 &m, unsigned s, MarkerRegs &r)
 {
-    const float4 a = m.pos[s], b = m.mom[s];
+    const v4f_nt a = __builtin_nontemporal_load((const v4f_nt *)(m.pos + s));
+    const v4f_nt b = __builtin_nontemporal_load((const v4f_nt *)(m.mom + s));
     r.word = __float_as_int(a.x);
     r.ox = a.y;
     r.oy = a.z;
@@ -241,7 +246,7 @@ __device__ static inline void load_marker(const Markers32 &m, unsigned s, Marker
     r.uy = b.y;
     r.uz = b.z;
     r.w = b.w;
-    r.id = m.id[s];
+    r.id = __builtin_nontemporal_load(m.id + s);
 }
 
 /* Trilinear gather from a tile's staged field block (TFN^3 nodes, node
@@ -461,10 +466,14 @@ struct Pending {
 __device__ static inline void write_back(const Markers32 &d, const TileView &tv, const Pending &q)
 {
     if (q.valid) {
-        d.pos[q.p] = make_float4(__int_as_float(q.word), q.r.ox, q.r.oy, q.r.oz);
-        if (q.word >= 0) d.mom[q.p] = make_float4(q.r.ux, q.r.uy, q.r.uz, q.r.w);
-        d.id[q.p] = q.r.id;
-        tv.key[q.p] = q.word < 0 ? -1 : (q.word & KEY_MASK);
+        const v4f_nt a = {__int_as_float(q.word), q.r.ox, q.r.oy, q.r.oz};
+        __builtin_nontemporal_store(a, (v4f_nt *)(d.pos + q.p));
+        if (q.word >= 0) {
+            const v4f_nt b = {q.r.ux, q.r.uy, q.r.uz, q.r.w};
+            __builtin_nontemporal_store(b, (v4f_nt *)(d.mom + q.p));
+        }
+        __builtin_nontemporal_store(q.r.id, d.id + q.p);
+        tv.key[q.p] = q.word < 0 ? -1 : (q.word & KEY_MASK);   /* read by the rank pass next */
     }
     if (q.ckey >= 0) atomicAdd(tv.count + q.ckey, q.clen);
 }
@@ -826,32 +835,47 @@ int launch_keys32(hipStream_t s, const Markers32 &m, long n, const TileView &tv,
 /* The next order: every position p < nmark[parity] with a key (the push's
  * output, or the markers keyed above) takes slot cursor[key]++ among its
  * key's, one atomic per run of equal keys in a wavefront, and perm[slot] =
- * p.  4 B read and 4 B written per marker instead of moving its 36 B. */
+ * p.  4 B read and 4 B written per marker instead of moving its 36 B.  A
+ * workgroup ranks RANK_BATCHES consecutive batches of 256 positions with
+ * all their loads, then all their atomics, in flight together: one
+ * batch's chain of dependent accesses would leave the pass latency-bound. */
+#define RANK_BATCHES 4
 struct Rank32Args {
     TileView tv;
     long     nbound;
     int      parity;
 };
 
-__device__ static inline void rank32(const TileView &tv, long nbound, int parity, long p)
+__device__ static inline void rank32(const TileView &tv, long nbound, int parity, long b0)
 {
     const long n = tv.nmark[parity];
-    const int key = (p < n && p < nbound) ? tv.key[p] : -1;
-    int start, len;
-    bool head;
-    wave_run(key, start, len, head);
-    int base = 0;
-    if (head && key >= 0) base = atomicAdd(tv.cursor + key, len);
-    base = __shfl(base, start, 64);
-    if (key < 0) return;
-    tv.perm[base + ((int)(threadIdx.x & 63) - start)] = (int)p;
+    int key[RANK_BATCHES], start[RANK_BATCHES], base[RANK_BATCHES];
+#pragma unroll
+    for (int u = 0; u < RANK_BATCHES; ++u) {
+        const long p = b0 + 256 * u + threadIdx.x;
+        key[u] = (p < n && p < nbound) ? tv.key[p] : -1;
+    }
+#pragma unroll
+    for (int u = 0; u < RANK_BATCHES; ++u) {
+        int len;
+        bool head;
+        wave_run(key[u], start[u], len, head);
+        base[u] = 0;
+        if (head && key[u] >= 0) base[u] = atomicAdd(tv.cursor + key[u], len);
+    }
+#pragma unroll
+    for (int u = 0; u < RANK_BATCHES; ++u) {
+        const int bs = __shfl(base[u], start[u], 64);
+        if (key[u] >= 0)
+            tv.perm[bs + ((int)(threadIdx.x & 63) - start[u])] = (int)(b0 + 256 * u + threadIdx.x);
+    }
 }
 
 template <> struct KernelDef<Rank32Args> {
     static constexpr int kBlock = 256;
     __device__ __forceinline__ static void body(const Rank32Args &a, VBlock b, VBlock)
     {
-        rank32(a.tv, a.nbound, a.parity, (long)b.x * 256 + threadIdx.x);
+        rank32(a.tv, a.nbound, a.parity, (long)b.x * (256 * RANK_BATCHES));
     }
 };
 
@@ -859,7 +883,8 @@ int launch_rank32(hipStream_t s, long nbound, const TileView &tv, int parity)
 {
     if (nbound <= 0) return MPIC_OK;
     Rank32Args a = {tv, nbound, parity};
-    return launch_def(s, a, (unsigned)((nbound + 255) / 256));
+    const long per = 256L * RANK_BATCHES;
+    return launch_def(s, a, (unsigned)((nbound + per - 1) / per));
 }
 
 /* CIC charge density of the split-precision markers of the order, FP64. */
