@@ -341,6 +341,11 @@ __device__ static inline unsigned flush_mask(const BoxSet &bs, int i, int j, int
                                              int nk, unsigned interior)
 {
     unsigned m = 0;
+    if (bs.shell >= 0) {                     /* every non-interior node of [0, n+1]^3 */
+        for (int e = 0; e < V; ++e)
+            if (k0 + e >= 0 && k0 + e <= nk + 1) m |= 1u << e;
+        return m & ~interior;
+    }
     for (int b = 0; b < bs.n; ++b) {
         const Box &B = bs.b[b];
         if (i < B.lo[0] || i > B.hi[0] || j < B.lo[1] || j > B.hi[1]) continue;
@@ -439,27 +444,28 @@ __device__ static inline void block_sum2(double we, double wb, double *part)
 /* em_energy's terms of node (i,j,k) (source/core/em.c:227-252): each
  * component squared where the node is that component's point of a cell of
  * [0,n)^3 -- Ex at (i+1,j,k), Ey at (i,j+1,k), Ez at (i,j,k+1), Hx at
- * (i,j+1,k+1), Hy at (i+1,j,k+1), Hz at (i+1,j+1,k) of cell (i,j,k). */
+ * (i,j+1,k+1), Hy at (i+1,j,k+1), Hz at (i+1,j+1,k) of cell (i,j,k).
+ * A thread's partial over its few nodes is kept in the field precision
+ * (FP32 form: ~1e-7 relative); the workgroup sums are FP64. */
 template <class T>
 __device__ static inline void energy_terms(const Grid &g, int i, int j, int k, T ex, T ey, T ez,
-                                           T hx, T hy, T hz, double &we, double &wb)
+                                           T hx, T hy, T hz, T &we, T &wb)
 {
     const bool i0 = i >= 0 && i < g.n[0], i1 = i >= 1 && i <= g.n[0];
     const bool j0 = j >= 0 && j < g.n[1], j1 = j >= 1 && j <= g.n[1];
     const bool k0 = k >= 0 && k < g.n[2], k1 = k >= 1 && k <= g.n[2];
-    const double dx = ex, dy = ey, dz = ez, bx = hx, by = hy, bz = hz;
-    if (i1 && j0 && k0) we += dx * dx;
-    if (i0 && j1 && k0) we += dy * dy;
-    if (i0 && j0 && k1) we += dz * dz;
-    if (i0 && j1 && k1) wb += bx * bx;
-    if (i1 && j0 && k1) wb += by * by;
-    if (i1 && j1 && k0) wb += bz * bz;
+    if (i1 && j0 && k0) we += ex * ex;
+    if (i0 && j1 && k0) we += ey * ey;
+    if (i0 && j0 && k1) we += ez * ez;
+    if (i0 && j1 && k1) wb += hx * hx;
+    if (i1 && j0 && k1) wb += hy * hy;
+    if (i1 && j1 && k0) wb += hz * hz;
 }
 
 template <class T>
 __device__ static inline void h_update_group(const FieldsT<T> &f, const Grid &g, const BoxSet &fl,
                                              int full, T c1, T c2, T c3, bool energy, int i,
-                                             int j, int k0, double &we, double &wb,
+                                             int j, int k0, T &we, T &wb,
                                              const LaserPlane &lp)
 {
     typedef typename Vec<T>::v VT;
@@ -518,10 +524,10 @@ __device__ static inline void h_update(const FieldsT<T> &f, const Grid &g, const
 {
     int i, j, k0;
     const bool on = span_group(sp, Vec<T>::N, b, gd, i, j, k0);
-    double we = 0.0, wb = 0.0;
+    T we = 0, wb = 0;
     if (on) h_update_group<T>(f, g, fl, full, c1, c2, c3, part != nullptr, i, j, k0, we, wb, lp);
     /* the energy of H after this update (the half step: H^n) and of E */
-    if (part) block_sum2(we, wb, part + 2 * (long)b.x);
+    if (part) block_sum2((double)we, (double)wb, part + 2 * (long)b.x);
 }
 
 template <class T> struct KernelDef<HUpdateArgs<T>> {

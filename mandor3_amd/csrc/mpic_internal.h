@@ -55,6 +55,8 @@ struct BoxSet {
     int  n;
     Box  b[6];
     long off[7];            /* prefix sums of the box volumes */
+    int  shell;             /* >= 0: the boxes are exactly the nodes of [0, n+1]^3 outside
+                             * [shell, n-2]^3 (a flush set; tested by ranges), else -1 */
 };
 
 /* One laser's soft-source plane (source/scpic/em_mirror.c:667-693): the

@@ -239,6 +239,7 @@ static BoxSet make_set(const Box *bx)
 {
     BoxSet s;
     s.n = 0;
+    s.shell = -1;
     s.off[0] = 0;
     for (int b = 0; b < 6; ++b) {
         long v = 1;
@@ -252,6 +253,37 @@ static BoxSet make_set(const Box *bx)
         s.n++;
     }
     return s;
+}
+
+/* lo when the boxes of s are pairwise disjoint, lie in [0, n+1]^3, miss
+ * [lo, n-2]^3 and together hold every other node of [0, n+1]^3 (so a node
+ * of [0, n+1]^3 is in s exactly when it is outside [lo, n-2]^3), else -1. */
+static int complement_shell(const BoxSet &s, const int n[3], int lo)
+{
+    long vol = 0;
+    for (int b = 0; b < s.n; ++b) {
+        const Box &B = s.b[b];
+        bool hits_inner = true;
+        for (int a = 0; a < 3; ++a) {
+            if (B.lo[a] < 0 || B.hi[a] > n[a] + 1) return -1;
+            if (B.hi[a] < lo || B.lo[a] > n[a] - 2) hits_inner = false;
+        }
+        if (hits_inner) return -1;
+        for (int c = 0; c < b; ++c) {
+            const Box &C = s.b[c];
+            bool meet = true;
+            for (int a = 0; a < 3; ++a)
+                if (B.hi[a] < C.lo[a] || C.hi[a] < B.lo[a]) meet = false;
+            if (meet) return -1;
+        }
+        vol += s.off[b + 1] - s.off[b];
+    }
+    long all = 1, inner = 1;
+    for (int a = 0; a < 3; ++a) {
+        all *= n[a] + 2;
+        inner *= (n[a] - 2 - lo + 1 > 0) ? n[a] - 2 - lo + 1 : 0;
+    }
+    return vol == all - inner ? lo : -1;
 }
 
 template <class T>
@@ -551,6 +583,9 @@ extern "C" int mpic_create(const mpic_config_t *cfg, mpic_run_t **out)
     r->updE = make_set(r->cE.upd_s);
     r->flH = make_set(r->cH.fl_s);
     r->flE = make_set(r->cE.fl_s);
+    /* the interiors of the H and E updates start at node 3 and 2 */
+    r->flH.shell = complement_shell(r->flH, cfg->n, 3);
+    r->flE.shell = complement_shell(r->flE, cfg->n, 2);
     int rc = build_wall_ops(r);
     if (rc != MPIC_OK) {
         mpic_destroy(r);
