@@ -209,12 +209,12 @@ int launch_jclear(hipStream_t s, double *J, const Grid &g, double *acc, int *ran
 int launch_push64(hipStream_t s, const Markers64 &m, long n, const Fields64 &f,
                   const Grid &g, const PushConsts &c, double *Wacc, int mode, unsigned *err);
 int launch_rho64(hipStream_t s, const Markers64 &m, long n, double *rho,
-                 const Grid &g);
+                 const Grid &g, const int *perm, const int *nmark, int parity);
 int launch_rho_fold(hipStream_t s, double *rho, const Grid &g, int b, int bc);
 template <class T>
 int launch_gauss(hipStream_t s, const FieldsT<T> &f, const double *rho1, const double *rho2,
                  const Grid &g, double *acc);
-int launch_cells64(hipStream_t s, const Markers64 &m, long n, const Grid &g,
-                   int32_t *cell);
+int launch_cells64(hipStream_t s, const Markers64 &m, long n, const Grid &g, const int *perm,
+                   const int *nmark, int parity, int32_t *cell);
 
 #endif

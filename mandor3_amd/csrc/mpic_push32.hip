@@ -832,61 +832,6 @@ int launch_keys32(hipStream_t s, const Markers32 &m, long n, const TileView &tv,
     return MPIC_OK;
 }
 
-/* The next order: every position p < nmark[parity] with a key (the push's
- * output, or the markers keyed above) takes slot cursor[key]++ among its
- * key's, one atomic per run of equal keys in a wavefront, and perm[slot] =
- * p.  4 B read and 4 B written per marker instead of moving its 36 B.  A
- * workgroup ranks RANK_BATCHES consecutive batches of 256 positions with
- * all their loads, then all their atomics, in flight together: one
- * batch's chain of dependent accesses would leave the pass latency-bound. */
-#define RANK_BATCHES 4
-struct Rank32Args {
-    TileView tv;
-    long     nbound;
-    int      parity;
-};
-
-__device__ static inline void rank32(const TileView &tv, long nbound, int parity, long b0)
-{
-    const long n = tv.nmark[parity];
-    int key[RANK_BATCHES], start[RANK_BATCHES], base[RANK_BATCHES];
-#pragma unroll
-    for (int u = 0; u < RANK_BATCHES; ++u) {
-        const long p = b0 + 256 * u + threadIdx.x;
-        key[u] = (p < n && p < nbound) ? tv.key[p] : -1;
-    }
-#pragma unroll
-    for (int u = 0; u < RANK_BATCHES; ++u) {
-        int len;
-        bool head;
-        wave_run(key[u], start[u], len, head);
-        base[u] = 0;
-        if (head && key[u] >= 0) base[u] = atomicAdd(tv.cursor + key[u], len);
-    }
-#pragma unroll
-    for (int u = 0; u < RANK_BATCHES; ++u) {
-        const int bs = __shfl(base[u], start[u], 64);
-        if (key[u] >= 0)
-            tv.perm[bs + ((int)(threadIdx.x & 63) - start[u])] = (int)(b0 + 256 * u + threadIdx.x);
-    }
-}
-
-template <> struct KernelDef<Rank32Args> {
-    static constexpr int kBlock = 256;
-    __device__ __forceinline__ static void body(const Rank32Args &a, VBlock b, VBlock)
-    {
-        rank32(a.tv, a.nbound, a.parity, (long)b.x * (256 * RANK_BATCHES));
-    }
-};
-
-int launch_rank32(hipStream_t s, long nbound, const TileView &tv, int parity)
-{
-    if (nbound <= 0) return MPIC_OK;
-    Rank32Args a = {tv, nbound, parity};
-    const long per = 256L * RANK_BATCHES;
-    return launch_def(s, a, (unsigned)((nbound + per - 1) / per));
-}
-
 /* CIC charge density of the split-precision markers of the order, FP64. */
 __global__ void k_rho32(Markers32 m, long n, double *rho, Grid g, TileView tv, int parity)
 {

@@ -370,12 +370,14 @@ int launch_push64(hipStream_t s, const Markers64 &m, long n, const Fields64 &f, 
     return launch_def(s, a, (unsigned)((n + TPB - 1) / TPB));
 }
 
-/* Tiled push: persistent workgroups pull 4x4x4-cell tiles (markers sorted
- * by tile-major cell key) from per-XCD queues, accumulate the tile's J in
- * LDS and flush it to HBM once; every marker leaves with its new cell key
- * and the per-cell counts of the new positions for the next sort. */
+/* Tiled push: persistent workgroups pull 4x4x4-cell tiles (the order's
+ * markers by tile-major cell key, read through its permutation) from
+ * per-XCD queues, accumulate the tile's J in LDS and flush it to HBM once;
+ * every marker is written, with its id and new cell key, to the output
+ * buffer at its position in the order, and the per-cell counts of the new
+ * positions are kept for the next sort. */
 struct Push64TArgs {
-    Markers64  m;
+    Markers64  m, d;           /* current buffer (read through tv.perm), output buffer */
     F64View    v;
     PushConsts c;
     TileView   tv;
@@ -383,9 +385,10 @@ struct Push64TArgs {
     unsigned  *err;
 };
 
-__device__ __forceinline__ static void push64_tiled(const Markers64 &m, const F64View &v,
-                                           const PushConsts &c, const TileView &tv, double *Wacc,
-                                           unsigned *err, const VBlock &bk)
+__device__ __forceinline__ static void push64_tiled(const Markers64 &m, const Markers64 &d,
+                                           const F64View &v, const PushConsts &c,
+                                           const TileView &tv, double *Wacc, unsigned *err,
+                                           const VBlock &bk)
 {
     __shared__ double lj[TJN * TJN * TJN * 3];
     __shared__ int s_tile;
@@ -415,22 +418,27 @@ __device__ __forceinline__ static void push64_tiled(const Markers64 &m, const F6
             const long p = base + threadIdx.x;
             int key = -1;
             if (p < end) {
-                double r[3] = {m.x[p], m.y[p], m.z[p]};
-                double u[3] = {m.ux[p], m.uy[p], m.uz[p]};
-                const float qm = m.qm[p];
-                int alive = (qm != 0.0f) && push_marker(ts, v, c, r, u, m.w[p], qm, w3, err);
+                const long q = tv.perm[p];
+                double r[3] = {m.x[q], m.y[q], m.z[q]};
+                double u[3] = {m.ux[q], m.uy[q], m.uz[q]};
+                const float qm = m.qm[q], w = m.w[q];
+                const int32_t id = m.id[q];
+                int alive = (qm != 0.0f) && push_marker(ts, v, c, r, u, w, qm, w3, err);
                 if (!alive) {
-                    m.qm[p] = 0.0f;
-                    m.w[p] = 0.0f;
+                    d.qm[p] = 0.0f;
+                    d.w[p] = 0.0f;
                 } else {
-                    m.x[p] = r[0];
-                    m.y[p] = r[1];
-                    m.z[p] = r[2];
-                    m.ux[p] = u[0];
-                    m.uy[p] = u[1];
-                    m.uz[p] = u[2];
+                    d.x[p] = r[0];
+                    d.y[p] = r[1];
+                    d.z[p] = r[2];
+                    d.ux[p] = u[0];
+                    d.uy[p] = u[1];
+                    d.uz[p] = u[2];
+                    d.w[p] = w;
+                    d.qm[p] = qm;
                     key = cell_key_of(tv, g, r);
                 }
+                d.id[p] = id;
                 tv.key[p] = key;
             }
             wave_run_count(tv.count, key);
@@ -456,7 +464,7 @@ template <> struct KernelDef<Push64TArgs> {
     static constexpr int kBlock = TPB;
     __device__ __forceinline__ static void body(const Push64TArgs &a, VBlock b, VBlock)
     {
-        push64_tiled(a.m, a.v, a.c, a.tv, a.Wacc, a.err, b);
+        push64_tiled(a.m, a.d, a.v, a.c, a.tv, a.Wacc, a.err, b);
     }
 };
 
@@ -471,9 +479,9 @@ int push64_blocks_per_cu(void)
     return nb;
 }
 
-int launch_push64_tiled(hipStream_t s, const Markers64 &m, const Fields64 &f, const Grid &g,
-                        const PushConsts &c, const TileView &tv, double *Wacc, unsigned *err,
-                        int nblocks)
+int launch_push64_tiled(hipStream_t s, const Markers64 &m, const Markers64 &dst,
+                        const Fields64 &f, const Grid &g, const PushConsts &c,
+                        const TileView &tv, double *Wacc, unsigned *err, int nblocks)
 {
     F64View v;
     for (int a = 0; a < 3; ++a) {
@@ -482,15 +490,20 @@ int launch_push64_tiled(hipStream_t s, const Markers64 &m, const Fields64 &f, co
         v.J[a] = f.J[a];
     }
     v.g = g;
-    Push64TArgs a = {m, v, c, tv, Wacc, err};
+    Push64TArgs a = {m, dst, v, c, tv, Wacc, err};
     return launch_def(s, a, (unsigned)nblocks);
 }
 
-/* CIC charge density of the live markers, source/core/plasma_rho.c:160-173. */
-__global__ void k_rho64(Markers64 m, long n, double *rho, Grid g)
+/* CIC charge density of the live markers, source/core/plasma_rho.c:160-173:
+ * the order's markers through its permutation (perm non-null: the first
+ * nmark[parity] entries), else the first n records. */
+__global__ void k_rho64(Markers64 m, long n, double *rho, Grid g, const int *perm,
+                        const int *nmark, int parity)
 {
-    long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n || m.qm[p] == 0.0f) return;
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n || (perm && q >= nmark[parity])) return;
+    const long p = perm ? perm[q] : q;
+    if (m.qm[p] == 0.0f) return;
     int i, j, k;
     double sx, sy, sz;
     sx = m.x[p] / g.h[0]; i = DBL_TO_INT(sx); sx -= i;
@@ -507,32 +520,37 @@ __global__ void k_rho64(Markers64 m, long n, double *rho, Grid g)
     unsafeAtomicAdd(rho + gidx(g, i + 1, j + 1, k + 1), sx * (sy * (sz * val)));
 }
 
-int launch_rho64(hipStream_t s, const Markers64 &m, long n, double *rho, const Grid &g)
+int launch_rho64(hipStream_t s, const Markers64 &m, long n, double *rho, const Grid &g,
+                 const int *perm, const int *nmark, int parity)
 {
     if (n <= 0) return MPIC_OK;
-    k_rho64<<<(unsigned)((n + TPB - 1) / TPB), TPB, 0, s>>>(m, n, rho, g);
+    k_rho64<<<(unsigned)((n + TPB - 1) / TPB), TPB, 0, s>>>(m, n, rho, g, perm, nmark, parity);
     MPIC_CHECK(hipGetLastError());
     return MPIC_OK;
 }
 
-/* Linear cell index of floor(x/h) for every marker (-1 for deleted ones). */
-__global__ void k_cells64(Markers64 m, long n, Grid g, int32_t *cell)
+/* Linear cell index of floor(x/h) for every marker of the order (through
+ * its permutation when perm is non-null; -1 for deleted ones). */
+__global__ void k_cells64(Markers64 m, long n, Grid g, const int *perm, const int *nmark,
+                          int parity, int32_t *cell)
 {
-    long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    if (m.qm[p] == 0.0f) {
-        cell[p] = -1;
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const long p = perm ? (q < nmark[parity] ? perm[q] : -1) : q;
+    if (p < 0 || m.qm[p] == 0.0f) {
+        cell[q] = -1;
         return;
     }
     int i = DBL_TO_INT(m.x[p] / g.h[0]), j = DBL_TO_INT(m.y[p] / g.h[1]),
         k = DBL_TO_INT(m.z[p] / g.h[2]);
-    cell[p] = (i * g.n[1] + j) * g.n[2] + k;
+    cell[q] = (i * g.n[1] + j) * g.n[2] + k;
 }
 
-int launch_cells64(hipStream_t s, const Markers64 &m, long n, const Grid &g, int32_t *cell)
+int launch_cells64(hipStream_t s, const Markers64 &m, long n, const Grid &g, const int *perm,
+                   const int *nmark, int parity, int32_t *cell)
 {
     if (n <= 0) return MPIC_OK;
-    k_cells64<<<(unsigned)((n + TPB - 1) / TPB), TPB, 0, s>>>(m, n, g, cell);
+    k_cells64<<<(unsigned)((n + TPB - 1) / TPB), TPB, 0, s>>>(m, n, g, perm, nmark, parity, cell);
     MPIC_CHECK(hipGetLastError());
     return MPIC_OK;
 }

@@ -522,7 +522,7 @@ extern "C" int mpic_create(const mpic_config_t *cfg, mpic_run_t **out)
     MPIC_CHECK(hipMemsetAsync(tv.off, 0, sizeof(int) * (tv.ncellp + 1), r->stream));
     MPIC_CHECK(hipMalloc(&tv.key, sizeof(int) * r->cap));
     tv.perm = nullptr;
-    if (r->fp32) MPIC_CHECK(hipMalloc(&tv.perm, sizeof(int) * r->cap));
+    if (r->tiled) MPIC_CHECK(hipMalloc(&tv.perm, sizeof(int) * r->cap));
     MPIC_CHECK(hipMalloc(&tv.active, sizeof(int) * tv.ntiles));
     MPIC_CHECK(hipMalloc(&tv.slot, sizeof(int) * tv.ntiles));
     tv.chm = r->fp32 ? push32_item_markers() : 0;
@@ -1125,20 +1125,41 @@ extern "C" int mpic_markers_save(mpic_run_t *r, void *rec, long *n, int recsize)
     long N = 0;
     TRY(live_n(r, &N));
     if (r->fp32) return save_split(r, rec, n, recsize, N);
+    /* the N markers of the order: through the permutation once a tiled run
+     * is initialised (the buffer's first E records are written), else the
+     * first N records */
+    const bool through = r->tiled && r->inited;
+    long E = N;
+    std::vector<int32_t> order;
+    if (through) {
+        int ext = 0;
+        MPIC_CHECK(hipMemcpy(&ext, r->tv.nmark + (r->parity ^ 1), sizeof(int), hipMemcpyDeviceToHost));
+        E = ext;
+        order.resize(N);
+        if (N)
+            MPIC_CHECK(hipMemcpy(order.data(), r->tv.perm, sizeof(int32_t) * N, hipMemcpyDeviceToHost));
+    }
     std::vector<double> d[6];
-    std::vector<float> w(N), qm(N);
-    std::vector<int32_t> id(N);
+    std::vector<float> w(E), qm(E);
+    std::vector<int32_t> id(E);
     double *src[6] = {r->m.x, r->m.y, r->m.z, r->m.ux, r->m.uy, r->m.uz};
     for (int c = 0; c < 6; ++c) {
-        d[c].resize(N);
-        MPIC_CHECK(hipMemcpy(d[c].data(), src[c], sizeof(double) * N, hipMemcpyDeviceToHost));
+        d[c].resize(E);
+        if (E) MPIC_CHECK(hipMemcpy(d[c].data(), src[c], sizeof(double) * E, hipMemcpyDeviceToHost));
     }
-    MPIC_CHECK(hipMemcpy(w.data(), r->m.w, sizeof(float) * N, hipMemcpyDeviceToHost));
-    MPIC_CHECK(hipMemcpy(qm.data(), r->m.qm, sizeof(float) * N, hipMemcpyDeviceToHost));
-    MPIC_CHECK(hipMemcpy(id.data(), r->m.id, sizeof(int32_t) * N, hipMemcpyDeviceToHost));
+    if (E) {
+        MPIC_CHECK(hipMemcpy(w.data(), r->m.w, sizeof(float) * E, hipMemcpyDeviceToHost));
+        MPIC_CHECK(hipMemcpy(qm.data(), r->m.qm, sizeof(float) * E, hipMemcpyDeviceToHost));
+        MPIC_CHECK(hipMemcpy(id.data(), r->m.id, sizeof(int32_t) * E, hipMemcpyDeviceToHost));
+    }
     long o = 0;
     char *dst = (char *)rec;
-    for (long q = 0; q < N; ++q) {
+    for (long u = 0; u < N; ++u) {
+        const long q = through ? order[u] : u;
+        if (q < 0 || q >= E) {
+            mpic_set_error("mpic_markers_save: order entry %ld names record %ld of %ld", u, q, E);
+            return MPIC_ERR_STATE;
+        }
         if (qm[q] == 0.0f) continue;
         if (o >= *n) {
             mpic_set_error("mpic_markers_save: output holds %ld markers, more are live", *n);
@@ -1466,7 +1487,7 @@ static void swap_markers(mpic_run *r)
 static int sort_pass(mpic_run *r, int keys)
 {
     const int other = r->cur ^ 1;
-    if (r->fp32) {
+    if (r->tiled) {
         /* The order is a permutation of the current buffer: tv.nmark[parity]
          * markers, the buffer's first tv.nmark[parity ^ 1] records written.
          * After a push (keys == 0) the push's output buffer and keys are
@@ -1477,20 +1498,23 @@ static int sort_pass(mpic_run *r, int keys)
             if (r->inited)
                 MPIC_CHECK(hipMemcpyAsync(r->tv.nmark + r->parity, r->tv.nmark + (r->parity ^ 1),
                                           sizeof(int), hipMemcpyDeviceToDevice, r->stream));
-            TRY(launch_keys32(r->stream, r->m32, r->n, r->tv, r->parity));
+            if (r->fp32)
+                TRY(launch_keys32(r->stream, r->m32, r->n, r->tv, r->parity));
+            else
+                TRY(launch_keys64(r->stream, r->m, r->n, r->g, r->tv, r->parity));
         }
         TRY(launch_tile_scan(r->stream, r->tv, r->blocksum, r->parity ^ 1));
-        TRY(launch_rank32(r->stream, r->n, r->tv, r->parity));
+        TRY(launch_rank(r->stream, r->n, r->tv, r->parity));
         if (keys)
             r->parity ^= 1;
         else
             swap_markers(r);
         return MPIC_OK;
-    } else {
-        if (keys) TRY(launch_keys64(r->stream, r->m, r->n, r->g, r->tv, r->parity));
-        TRY(launch_tile_scan(r->stream, r->tv, r->blocksum, r->parity ^ 1));
-        TRY(launch_scatter64(r->stream, r->m, r->mbuf[other], r->n, r->tv, r->parity));
     }
+    /* the untiled FP64 push keeps no order: a sort compacts its array */
+    if (keys) TRY(launch_keys64(r->stream, r->m, r->n, r->g, r->tv, r->parity));
+    TRY(launch_tile_scan(r->stream, r->tv, r->blocksum, r->parity ^ 1));
+    TRY(launch_scatter64(r->stream, r->m, r->mbuf[other], r->n, r->tv, r->parity));
     swap_markers(r);
     return MPIC_OK;
 }
@@ -1797,7 +1821,8 @@ extern "C" int mpic_push_deposit(mpic_run_t *r)
         if (r->timed >= 0) TRY(record(r, 2));
         TRY(launch_jgather32(r->stream, r->f32, r->g, r->tv, r->tj, r->gather_blocks));
     } else if (r->tiled) {
-        TRY(launch_push64_tiled(r->stream, r->m, r->f, r->g, r->pc, r->tv, r->dacc + 2, r->derr,
+        TRY(launch_push64_tiled(r->stream, r->m, r->mbuf[r->cur ^ 1], r->f, r->g, r->pc, r->tv,
+                                r->dacc + 2, r->derr,
                                 r->push_blocks));
     } else {
         TRY(launch_push64(r->stream, r->m, r->n, r->f, r->g, r->pc, r->dacc + 2, 0, r->derr));
@@ -2647,7 +2672,7 @@ extern "C" int mpic_diag_history(mpic_run_t *r, double *rows, int max_rows, int 
 static TileView order_view(const mpic_run *r)
 {
     TileView tv = r->tv;
-    if (!r->inited) tv.perm = nullptr;
+    if (!r->inited || !r->tiled) tv.perm = nullptr;
     return tv;
 }
 
@@ -2659,7 +2684,7 @@ static int rho_into(mpic_run *r, double *rho)
     if (r->fp32)
         TRY(launch_rho32(r->stream, r->m32, n, rho, r->g, order_view(r), r->parity));
     else
-        TRY(launch_rho64(r->stream, r->m, n, rho, r->g));
+        TRY(launch_rho64(r->stream, r->m, n, rho, r->g, order_view(r).perm, r->tv.nmark, r->parity));
     const int *bc = r->cfg.bc;
     const int BCv[6] = {bc[0], bc[3], bc[1], bc[4], bc[2], bc[5]};
     for (int b = 0; b < 6; ++b) TRY(launch_rho_fold(r->stream, rho, r->g, b, BCv[b]));
@@ -2721,7 +2746,7 @@ extern "C" int mpic_marker_cells(mpic_run_t *r, int32_t *cells, long *n)
     if (r->fp32)
         TRY(launch_cells32(r->stream, r->m32, N, r->g, order_view(r), r->parity, d));
     else
-        TRY(launch_cells64(r->stream, r->m, N, r->g, d));
+        TRY(launch_cells64(r->stream, r->m, N, r->g, order_view(r).perm, r->tv.nmark, r->parity, d));
     std::vector<int32_t> all(N);
     MPIC_CHECK(hipMemcpyAsync(all.data(), d, sizeof(int32_t) * N, hipMemcpyDeviceToHost,
                               r->stream));

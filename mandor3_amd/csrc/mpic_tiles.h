@@ -25,8 +25,8 @@ struct TileView {
     int *cursor;             /* [ncellp] scatter cursors */
     int *count;              /* [ncellp] markers per key after the push (zeroed by the scan) */
     int *key;                /* [capacity] new key of every marker, -1 = deleted */
-    int *perm;               /* FP32: [capacity] the current order as indices into the
-                              * current buffer (nmark[parity] of them) */
+    int *perm;               /* [capacity] the current order as indices into the
+                              * current buffer (nmark[parity] of them; tiled runs) */
     int *active;             /* [ntiles] tiles holding markers, in tile order */
     int *slot;               /* [ntiles] position of a tile in 'active', -1 if empty */
     int *nactive;            /* [1] */
@@ -119,11 +119,12 @@ __device__ static inline void wave_run_count(int *count, int key)
 int launch_keys64(hipStream_t s, const Markers64 &m, long n, const Grid &g, const TileView &tv,
                   int parity);
 int launch_tile_scan(hipStream_t s, const TileView &tv, int *blocksum, int parity_next);
+int launch_rank(hipStream_t s, long nbound, const TileView &tv, int parity);
 int launch_scatter64(hipStream_t s, const Markers64 &src, const Markers64 &dst, long nbound,
                      const TileView &tv, int parity);
-int launch_push64_tiled(hipStream_t s, const Markers64 &m, const Fields64 &f, const Grid &g,
-                        const PushConsts &c, const TileView &tv, double *Wacc, unsigned *err,
-                        int nblocks);
+int launch_push64_tiled(hipStream_t s, const Markers64 &m, const Markers64 &dst,
+                        const Fields64 &f, const Grid &g, const PushConsts &c,
+                        const TileView &tv, double *Wacc, unsigned *err, int nblocks);
 int push64_blocks_per_cu(void);
 int launch_counts_natural(hipStream_t s, const TileView &tv, const Grid &g, int32_t *out);
 
@@ -138,7 +139,6 @@ int launch_push32(hipStream_t s, const Markers32 &m, const Markers32 &dst, const
 int launch_jgather32(hipStream_t s, const Fields32 &f, const Grid &g, const TileView &tv,
                      const float *tj, int nblocks);
 int launch_keys32(hipStream_t s, const Markers32 &m, long n, const TileView &tv, int parity);
-int launch_rank32(hipStream_t s, long nbound, const TileView &tv, int parity);
 int launch_rho32(hipStream_t s, const Markers32 &m, long n, double *rho, const Grid &g,
                  const TileView &tv, int parity);
 int launch_cells32(hipStream_t s, const Markers32 &m, long n, const Grid &g, const TileView &tv,

@@ -5,12 +5,15 @@
  * Per step: the push kernel writes each marker's new key and bumps
  * count[key] once per run of equal keys in a wavefront; the scan turns the
  * counts into per-key offsets, builds the list of non-empty tiles and resets
- * the tile queues; the scatter moves every live marker to its slot
- * (cursor[key] reserved once per run), dropping deleted markers.
+ * the tile queues; the rank pass gives every live marker its slot
+ * (cursor[key] reserved once per run) and records the marker's position in
+ * the next order's permutation, dropping deleted markers.  The next push
+ * reads its markers through that permutation and writes them to the other
+ * buffer in order, so no pass moves markers only to sort them (the untiled
+ * FP64 push, which keeps no order, is compacted by a scatter instead).
  *
- * Roofline (HBM-bound): scan ~ 3 x 4 B per cell key; scatter reads key (4 B)
- * and the marker and writes the marker: FP64 plain form 4 + 60 + 60 B per
- * marker.
+ * Roofline (HBM-bound): scan ~ 3 x 4 B per cell key; rank 4 B read and 4 B
+ * written per marker; scatter 4 + 60 + 60 B per FP64 marker.
  */
 #include "mpic_launch.h"
 #include "mpic_tiles.h"
@@ -224,7 +227,64 @@ int launch_tile_scan(hipStream_t s, const TileView &tv, int *blocksum, int parit
     return launch_def(s, c, (unsigned)((waves * 64 + 255) / 256));
 }
 
-/* Moves each live marker of src to its slot in dst. */
+/* The next order: every position p < nmark[parity] with a key (the push's
+ * output, or the markers keyed by k_keys32 / k_keys64) takes slot
+ * cursor[key]++ among its key's, one atomic per run of equal keys in a
+ * wavefront, and perm[slot] = p.  4 B read and 4 B written per marker
+ * instead of moving its records (36 B FP32, 60 B FP64).  A
+ * workgroup ranks RANK_BATCHES consecutive batches of 256 positions with
+ * all their loads, then all their atomics, in flight together: one
+ * batch's chain of dependent accesses would leave the pass latency-bound. */
+#define RANK_BATCHES 4
+struct RankArgs {
+    TileView tv;
+    long     nbound;
+    int      parity;
+};
+
+__device__ static inline void rank_pass(const TileView &tv, long nbound, int parity, long b0)
+{
+    const long n = tv.nmark[parity];
+    int key[RANK_BATCHES], start[RANK_BATCHES], base[RANK_BATCHES];
+#pragma unroll
+    for (int u = 0; u < RANK_BATCHES; ++u) {
+        const long p = b0 + 256 * u + threadIdx.x;
+        key[u] = (p < n && p < nbound) ? tv.key[p] : -1;
+    }
+#pragma unroll
+    for (int u = 0; u < RANK_BATCHES; ++u) {
+        int len;
+        bool head;
+        wave_run(key[u], start[u], len, head);
+        base[u] = 0;
+        if (head && key[u] >= 0) base[u] = atomicAdd(tv.cursor + key[u], len);
+    }
+#pragma unroll
+    for (int u = 0; u < RANK_BATCHES; ++u) {
+        const int bs = __shfl(base[u], start[u], 64);
+        if (key[u] >= 0)
+            tv.perm[bs + ((int)(threadIdx.x & 63) - start[u])] = (int)(b0 + 256 * u + threadIdx.x);
+    }
+}
+
+template <> struct KernelDef<RankArgs> {
+    static constexpr int kBlock = 256;
+    __device__ __forceinline__ static void body(const RankArgs &a, VBlock b, VBlock)
+    {
+        rank_pass(a.tv, a.nbound, a.parity, (long)b.x * (256 * RANK_BATCHES));
+    }
+};
+
+int launch_rank(hipStream_t s, long nbound, const TileView &tv, int parity)
+{
+    if (nbound <= 0) return MPIC_OK;
+    RankArgs a = {tv, nbound, parity};
+    const long per = 256L * RANK_BATCHES;
+    return launch_def(s, a, (unsigned)((nbound + per - 1) / per));
+}
+
+/* Moves each live marker of src to its slot in dst (the untiled FP64 push's
+ * compacting sort). */
 struct Scatter64Args {
     Markers64 src, dst;
     long      nbound;
