@@ -95,16 +95,18 @@ def copy_calibration(mib):
 def push_bytes(n_markers, plasma_cells, precision):
     """Algorithmic HBM bytes of one launch of the push kernel.
 
-    FP64 plain form: marker read 56 B (x y z ux uy uz FP64, w qm FP32) and
-    write 48 B; per cell holding markers E/H read 48 B and the J
-    read-modify-write 48 B (the kernel flushes J itself).
+    FP64 plain form: marker read 64 B (its index in the order's
+    permutation, x y z ux uy uz FP64, w qm FP32, id) and write 64 B (the
+    state and id at its position in the order, and its new key); per cell
+    holding markers E/H read 48 B and the J read-modify-write 48 B (the
+    kernel flushes J itself).
     FP32 split form: marker read 40 B (its index in the order's permutation,
     two 16-byte records: cell word + 3 offsets, 3 momenta + w, and the id)
     and write 40 B (the records and id at its position in the order, and
     its new key); per cell holding markers E/H read 24 B.  Its J (48 B per
     node, FP64 read-modify-write) is summed by the separate gather kernel."""
     if precision == "fp64":
-        return n_markers * (56 + 48) + plasma_cells * (48 + 48)
+        return n_markers * (64 + 64) + plasma_cells * (48 + 48)
     return n_markers * (40 + 40) + plasma_cells * 24
 
 
