@@ -313,6 +313,94 @@ int launch_face(hipStream_t s, const Grid &g, const FaceOp &op)
     return launch_def(s, a, nblk(n));
 }
 
+/* The y and z wall ops of a list, in list order, one workgroup per x plane:
+ * every line of a y or z wall keeps i fixed, so a plane's lines depend only
+ * on earlier ops' lines of the same plane, and a workgroup barrier between
+ * consecutive ops gives the list's sequence exactly (edges included).  The
+ * x walls, whose lines cross planes, are launched before. */
+template <class T> struct FacePlanesArgs {
+    Grid   g;
+    FaceOp op[4];
+    int    phase[4];      /* ops of one phase touch disjoint nodes; a barrier between phases */
+    int    nop, ilo;
+};
+
+/* Lines of op on plane i: [first, first + count). */
+__device__ static inline void plane_lines(const FaceOp &op, int i, long &first, long &count)
+{
+    const long np = op.phi - op.plo + 1, nq = op.qhi - op.qlo + 1;
+    first = 0;
+    count = 0;
+    if (op.ea == 1) {                 /* eq = 0: q is i, consecutive lines along p */
+        if (i < op.qlo || i > op.qhi) return;
+        first = (long)(i - op.qlo) * np;
+        count = np;
+    } else {                          /* ea == 2, ep = 0: p is i */
+        if (i < op.plo || i > op.phi) return;
+        first = (long)(i - op.plo) * nq;
+        count = nq;
+    }
+}
+
+template <class T> struct KernelDef<FacePlanesArgs<T>> {
+    static constexpr int kBlock = TPB;
+    __device__ __forceinline__ static void body(const FacePlanesArgs<T> &a, VBlock b, VBlock)
+    {
+        const int i = a.ilo + (int)b.x;
+        for (int q = 0; q < a.nop; ++q) {
+            if (q && a.phase[q] != a.phase[q - 1])
+                __syncthreads();          /* the plane's lines of the earlier phase are done */
+            long first, count;
+            plane_lines(a.op[q], i, first, count);
+            for (long u = threadIdx.x; u < count; u += TPB) face_apply<T>(a.g, a.op[q], first + u);
+        }
+    }
+};
+
+int launch_face_planes(hipStream_t s, const Grid &g, const FaceOp *ops, int nop)
+{
+    if (nop <= 0) return MPIC_OK;
+    int ilo = 1 << 30, ihi = -(1 << 30);
+    for (int q = 0; q < nop; ++q) {
+        const FaceOp &op = ops[q];
+        const int lo = op.ea == 1 ? op.qlo : op.plo, hi = op.ea == 1 ? op.qhi : op.phi;
+        ilo = lo < ilo ? lo : ilo;
+        ihi = hi > ihi ? hi : ihi;
+    }
+    if (ihi < ilo) return MPIC_OK;
+    /* the min and max wall of an axis with at least 8 cells touch disjoint
+     * node layers (as launch_face_pair uses): one phase */
+    int phase[4], ph = 0;
+    for (int q = 0; q < nop; ++q) {
+        const bool pair = q > 0 && ops[q].ea == ops[q - 1].ea && ops[q - 1].top == 0 &&
+                          ops[q].top == 1 && ops[q].kind == ops[q - 1].kind &&
+                          ops[q].kind != FACE_PERIODIC_H && ops[q].kind != FACE_PERIODIC_E &&
+                          ops[q].N >= 8 && (q < 2 || phase[q - 1] != phase[q - 2]);
+        if (q > 0 && !pair) ++ph;
+        phase[q] = ph;
+    }
+    if (ops[0].fp32) {
+        FacePlanesArgs<float> a;
+        a.g = g;
+        for (int q = 0; q < nop; ++q) {
+            a.op[q] = ops[q];
+            a.phase[q] = phase[q];
+        }
+        a.nop = nop;
+        a.ilo = ilo;
+        return launch_def(s, a, (unsigned)(ihi - ilo + 1));
+    }
+    FacePlanesArgs<double> a;
+    a.g = g;
+    for (int q = 0; q < nop; ++q) {
+        a.op[q] = ops[q];
+        a.phase[q] = phase[q];
+    }
+    a.nop = nop;
+    a.ilo = ilo;
+    return launch_def(s, a, (unsigned)(ihi - ilo + 1));
+}
+
 /* a and b: the min and max wall of one axis with at least 8 cells, so that
  * the node layers they touch ([0, 1] and [N-1, N+1]) are disjoint and one
  * launch gives the result of the two in sequence. */
@@ -1046,19 +1134,14 @@ struct JFoldArgs {
  * fold of zeros writes zeros.  For y and z walls that is a line on a plane
  * outside jr; an x wall is skipped whole when its layers miss jr (a
  * periodic x fold reads both walls' layers). */
-__device__ static inline void jfold(const JPtrs &f, const Grid &g, int ea, int bc0, int bc1,
-                                    int top0, const int *jr, const VBlock &b)
+/* Line t of the fold of wall 'top' (0 min, 1 max) of axis ea, boundary
+ * code bc. */
+__device__ static inline void jfold_line(const JPtrs &f, const Grid &g, int ea, int bc, int top,
+                                         const int *jr, long t)
 {
-    /* wall top0 + block y of the axis: 0 min, 1 max */
-    const int top = top0 + (int)b.y, bc = top ? bc1 : bc0;
     const int ep = (ea == 0) ? 1 : 0, eq = (ea == 2) ? 1 : 2;
     const int np = g.n[ep] + 5, nq = g.n[eq] + 5;
     const int N = g.n[ea];
-    if (ea == 0) {
-        const bool lo_wall = jr[0] <= 2 && jr[1] >= -2, hi_wall = jr[0] <= N + 2 && jr[1] >= N - 2;
-        if (bc == MPIC_BC_PERIODIC ? !(lo_wall || hi_wall) : !(top ? hi_wall : lo_wall)) return;
-    }
-    long t = (long)b.x * TPB + threadIdx.x;
     if (t >= (long)np * nq) return;
     int r[3];
     const unsigned pq = (unsigned)t / (unsigned)nq;
@@ -1128,6 +1211,19 @@ __device__ static inline void jfold(const JPtrs &f, const Grid &g, int ea, int b
 #undef AT
 }
 
+__device__ static inline void jfold(const JPtrs &f, const Grid &g, int ea, int bc0, int bc1,
+                                    int top0, const int *jr, const VBlock &b)
+{
+    /* wall top0 + block y of the axis: 0 min, 1 max */
+    const int top = top0 + (int)b.y, bc = top ? bc1 : bc0;
+    const int N = g.n[ea];
+    if (ea == 0) {
+        const bool lo_wall = jr[0] <= 2 && jr[1] >= -2, hi_wall = jr[0] <= N + 2 && jr[1] >= N - 2;
+        if (bc == MPIC_BC_PERIODIC ? !(lo_wall || hi_wall) : !(top ? hi_wall : lo_wall)) return;
+    }
+    jfold_line(f, g, ea, bc, top, jr, (long)b.x * TPB + threadIdx.x);
+}
+
 /* The J fold of both walls of axis ea (min first).  A periodic axis is one
  * fold; two non-periodic walls of an axis with at least 8 cells touch the
  * disjoint layers [-2, 2] and [N-2, N+2] and share one launch. */
@@ -1138,6 +1234,54 @@ template <> struct KernelDef<JFoldArgs> {
         jfold(a.f, a.g, a.ea, a.bc0, a.bc1, a.top0, a.jr, b);
     }
 };
+
+/* The y and z folds (min then max wall of y, then of z) of x plane
+ * b.x - 2, one workgroup per plane, a barrier between consecutive walls: a
+ * y or z fold line keeps i fixed (see the wall planes above).  Planes
+ * outside jr hold no current and are left alone. */
+struct JFoldPlanesArgs {
+    JPtrs      f;
+    Grid       g;
+    int        bc[6];
+    const int *jr;
+};
+
+template <> struct KernelDef<JFoldPlanesArgs> {
+    static constexpr int kBlock = TPB;
+    __device__ __forceinline__ static void body(const JFoldPlanesArgs &a, VBlock b, VBlock)
+    {
+        const int i = (int)b.x - 2;
+        if (i < a.jr[0] || i > a.jr[1]) return;          /* the whole workgroup */
+        bool first = true;
+        for (int ea = 1; ea < 3; ++ea) {
+            const int eq = (ea == 2) ? 1 : 2;            /* ep = 0: a line's p is its plane */
+            const int nq = a.g.n[eq] + 5;
+            const int bc0 = a.bc[ea], bc1 = a.bc[3 + ea];
+            const int walls = bc0 == MPIC_BC_PERIODIC ? 1 : 2;   /* a periodic axis folds once */
+            /* the two walls of an axis with at least 8 cells touch disjoint
+             * layers ([-2, 2] and [N-2, N+2]): one phase */
+            const bool together = walls == 2 && a.g.n[ea] >= 8;
+            for (int top = 0; top < walls; ++top) {
+                if (!first && !(together && top == 1)) __syncthreads();
+                first = false;
+                const long t0 = (long)(i + 2) * nq;
+                for (int u = threadIdx.x; u < nq; u += TPB)
+                    jfold_line(a.f, a.g, ea, top ? bc1 : bc0, top, a.jr, t0 + u);
+            }
+        }
+    }
+};
+
+int launch_jfold_planes(hipStream_t s, const JPtrs &f, const Grid &g, const int bc[6],
+                        const int *jr)
+{
+    JFoldPlanesArgs a;
+    a.f = f;
+    a.g = g;
+    for (int q = 0; q < 6; ++q) a.bc[q] = bc[q];
+    a.jr = jr;
+    return launch_def(s, a, (unsigned)(g.n[0] + 5));
+}
 
 #define TRY_LAUNCH(x)                   \
     do {                                \

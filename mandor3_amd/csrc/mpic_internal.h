@@ -160,6 +160,7 @@ template <class T> int launch_cap_H(hipStream_t s, const FieldsT<T> &f, const Gr
 template <class T> int launch_cap_E(hipStream_t s, const FieldsT<T> &f, const Grid &g, const BoxSet &bs);
 int launch_face(hipStream_t s, const Grid &g, const FaceOp &op);
 int launch_face_pair(hipStream_t s, const Grid &g, const FaceOp &a, const FaceOp &b);
+int launch_face_planes(hipStream_t s, const Grid &g, const FaceOp *ops, int nop);
 template <class T>
 int launch_h_update(hipStream_t s, const FieldsT<T> &f, const Grid &g, const BoxSet &flush,
                     int full, double *part = nullptr, const LaserPlane &lp = LaserPlane());
@@ -183,6 +184,8 @@ int launch_spot(hipStream_t s, T *Ey, T *Ez, const Grid &g, int i, const double 
                 const double *pz, const double *tab, const int *step, int cap);
 int launch_jfold(hipStream_t s, const JPtrs &f, const Grid &g, int ea, int bc0, int bc1,
                  const int *jr);
+int launch_jfold_planes(hipStream_t s, const JPtrs &f, const Grid &g, const int bc[6],
+                        const int *jr);
 /* VSP (beta, alpha, beta) sweeps along z, y, x of ncomp arrays on the padded grid. */
 int launch_vsp(hipStream_t s, double *const F[3], int ncomp, const Grid &g, double alpha);
 template <class T>

@@ -1446,10 +1446,33 @@ extern "C" int mpic_tfsf_recorded(mpic_run_t *r, int *nframes, long vol[24], dou
 }
 
 /* ---------------- stages ---------------- */
-/* The walls in list order; the min and max wall of one axis with at least
- * 8 cells (adjacent in the list, disjoint node layers) share a launch. */
+/* The walls in list order (x, then y, then z): the x walls with the min and
+ * max wall of an axis with at least 8 cells (adjacent in the list, disjoint
+ * node layers) sharing a launch, then the y and z walls in one launch of a
+ * workgroup per x plane (their lines keep i fixed). */
 static int walls_on(mpic_run *r, std::vector<FaceOp> &ops)
 {
+    size_t nx = 0;
+    while (nx < ops.size() && ops[nx].ea == 0) ++nx;
+    if (ops.size() - nx <= 4) {
+        for (size_t q = nx; q < ops.size(); ++q)
+            if (ops[q].ea == 0) goto generic;       /* not in x, y, z order */
+        for (size_t q = 0; q < nx; ++q) {
+            const FaceOp &a = ops[q];
+            if (q + 1 < nx) {
+                const FaceOp &b = ops[q + 1];
+                if (a.top == 0 && b.top == 1 && a.kind == b.kind && a.kind != FACE_PERIODIC_H &&
+                    a.kind != FACE_PERIODIC_E && a.N >= 8) {
+                    TRY(launch_face_pair(r->stream, r->g, a, b));
+                    ++q;
+                    continue;
+                }
+            }
+            TRY(launch_face(r->stream, r->g, a));
+        }
+        return launch_face_planes(r->stream, r->g, ops.data() + nx, (int)(ops.size() - nx));
+    }
+generic:
     for (size_t q = 0; q < ops.size(); ++q) {
         const FaceOp &a = ops[q];
         if (q + 1 < ops.size()) {
@@ -1894,8 +1917,9 @@ extern "C" int mpic_j_finish(mpic_run_t *r)
     if (!r) return MPIC_ERR_ARG;
     MPIC_CHECK(hipSetDevice(r->dev));
     const int *bc = r->cfg.bc;
-    for (int ea = 0; ea < 3; ++ea)
-        TRY(launch_jfold(r->stream, r->jp, r->g, ea, bc[ea], bc[3 + ea], r->djr + 4));
+    /* x walls, then the y and z walls plane by plane */
+    TRY(launch_jfold(r->stream, r->jp, r->g, 0, bc[0], bc[3], r->djr + 4));
+    TRY(launch_jfold_planes(r->stream, r->jp, r->g, bc, r->djr + 4));
     /* VSP_vectorSweep after the boundary fold (source/core/plasma_currentWalls.c:204) */
     if (fabs(r->vsp - 1.0) >= 1e-10) TRY(launch_vsp(r->stream, r->jp.J, 3, r->g, r->vsp));
     return MPIC_OK;
