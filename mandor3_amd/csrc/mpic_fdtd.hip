@@ -639,13 +639,21 @@ int h_update_blocks(const Grid &g, int fp32)
 }
 
 template <class T>
-int launch_h_update(hipStream_t s, const FieldsT<T> &f, const Grid &g, const BoxSet &flush,
-                    int full, double *part, const LaserPlane &lp)
+static HUpdateArgs<T> h_args(const FieldsT<T> &f, const Grid &g, const BoxSet &flush, int full,
+                             double *part, const LaserPlane &lp)
 {
     const Span sp = h_span(g, Vec<T>::N);
     HUpdateArgs<T> a = {f, g, flush, full, sp, (T)(0.5 * g.tau / g.h[0]),
                         (T)(0.5 * g.tau / g.h[1]), (T)(0.5 * g.tau / g.h[2]), part, lp};
-    return launch_def(s, a, grid_span(sp, g.n[0] + 2));
+    return a;
+}
+
+template <class T>
+int launch_h_update(hipStream_t s, const FieldsT<T> &f, const Grid &g, const BoxSet &flush,
+                    int full, double *part, const LaserPlane &lp)
+{
+    const HUpdateArgs<T> a = h_args(f, g, flush, full, part, lp);
+    return launch_def(s, a, grid_span(a.sp, g.n[0] + 2));
 }
 
 /* Interior E step (source/core/em.c:134-161) and/or the flush of the cap
@@ -1572,18 +1580,30 @@ template <> struct KernelDef<JClearArgs> {
     }
 };
 
-int launch_jclear(hipStream_t s, double *J, const Grid &g, double *acc, int *range,
-                  const int *step, const int *active, const int *nactive, int tpp, int full)
+static JClearArgs jclear_args(double *J, const Grid &g, double *acc, int *range, const int *step,
+                              const int *active, const int *nactive, int tpp, int full)
 {
     JClearArgs a = {J, g.ncell, (long)g.GY * g.GZ, g.n[0], tpp, acc, range, step, active,
                     nactive, full};
-    /* 16-byte stores; enough workgroups to stream the whole grid at full
-     * rate, few enough that a narrow range costs little more than a launch */
+    return a;
+}
+
+/* 16-byte stores; enough workgroups to stream the whole grid at full rate,
+ * few enough that a narrow range costs little more than a launch. */
+static unsigned jclear_blocks(const Grid &g)
+{
     const long cnt2 = (long)(g.n[0] + 5) * g.GY * g.GZ / 2;
     long nb = (cnt2 + 4 * TPB - 1) / (4 * TPB);
     if (nb > 4096) nb = 4096;
     if (nb < 1) nb = 1;
-    return launch_def(s, a, (unsigned)nb);
+    return (unsigned)nb;
+}
+
+int launch_jclear(hipStream_t s, double *J, const Grid &g, double *acc, int *range,
+                  const int *step, const int *active, const int *nactive, int tpp, int full)
+{
+    return launch_def(s, jclear_args(J, g, acc, range, step, active, nactive, tpp, full),
+                      jclear_blocks(g));
 }
 
 /* CIC rho fold of one face, source/core/plasma_rho.c:23-96. */
@@ -1745,12 +1765,48 @@ int launch_probes(hipStream_t s, T *const E[3], T *const H[3], const Grid &g, co
     return MPIC_OK;
 }
 
+/* The H update paired with the J clear (the H half step of the loop body:
+ * the update never touches J) or with the y and z J folds (the H step: the
+ * x folds have run before). */
+template <class T>
+int launch_h_update_jclear(hipStream_t s, const FieldsT<T> &f, const Grid &g, const BoxSet &flush,
+                           int full, double *part, double *J, double *acc, int *range,
+                           const int *step, const int *active, const int *nactive, int tpp,
+                           int jfull)
+{
+    const HUpdateArgs<T> a = h_args(f, g, flush, full, part, LaserPlane());
+    return launch_pair(s, a, dim3(grid_span(a.sp, g.n[0] + 2)),
+                       jclear_args(J, g, acc, range, step, active, nactive, tpp, jfull),
+                       dim3(jclear_blocks(g)));
+}
+
+template <class T>
+int launch_h_update_jfold(hipStream_t s, const FieldsT<T> &f, const Grid &g, const BoxSet &flush,
+                          int full, const LaserPlane &lp, const JPtrs &jp, const int bc[6],
+                          const int *jr)
+{
+    const HUpdateArgs<T> a = h_args(f, g, flush, full, nullptr, lp);
+    JFoldPlanesArgs b;
+    b.f = jp;
+    b.g = g;
+    for (int q = 0; q < 6; ++q) b.bc[q] = bc[q];
+    b.jr = jr;
+    return launch_pair(s, a, dim3(grid_span(a.sp, g.n[0] + 2)), b, dim3(g.n[0] + 5));
+}
+
 /* explicit instantiations for the two field precisions */
 #define INST(T)                                                                               \
     template int launch_cap_H<T>(hipStream_t, const FieldsT<T> &, const Grid &, const BoxSet &); \
     template int launch_cap_E<T>(hipStream_t, const FieldsT<T> &, const Grid &, const BoxSet &); \
     template int launch_h_update<T>(hipStream_t, const FieldsT<T> &, const Grid &,                \
                                     const BoxSet &, int, double *, const LaserPlane &);         \
+    template int launch_h_update_jclear<T>(hipStream_t, const FieldsT<T> &, const Grid &,         \
+                                           const BoxSet &, int, double *, double *, double *,   \
+                                           int *, const int *, const int *, const int *, int,   \
+                                           int);                                                \
+    template int launch_h_update_jfold<T>(hipStream_t, const FieldsT<T> &, const Grid &,          \
+                                          const BoxSet &, int, const LaserPlane &,              \
+                                          const JPtrs &, const int *, const int *);             \
     template int launch_e_fused<T>(hipStream_t, const FieldsT<T> &, const Grid &,                 \
                                    const BoxSet &, const int *, const LaserPlane &);            \
     template int launch_e_update<T>(hipStream_t, const FieldsT<T> &, const Grid &,                \

@@ -162,6 +162,15 @@ int launch_face(hipStream_t s, const Grid &g, const FaceOp &op);
 int launch_face_pair(hipStream_t s, const Grid &g, const FaceOp &a, const FaceOp &b);
 int launch_face_planes(hipStream_t s, const Grid &g, const FaceOp *ops, int nop);
 template <class T>
+int launch_h_update_jclear(hipStream_t s, const FieldsT<T> &f, const Grid &g, const BoxSet &flush,
+                           int full, double *part, double *J, double *acc, int *range,
+                           const int *step, const int *active, const int *nactive, int tpp,
+                           int jfull);
+template <class T>
+int launch_h_update_jfold(hipStream_t s, const FieldsT<T> &f, const Grid &g, const BoxSet &flush,
+                          int full, const LaserPlane &lp, const JPtrs &jp, const int bc[6],
+                          const int *jr);
+template <class T>
 int launch_h_update(hipStream_t s, const FieldsT<T> &f, const Grid &g, const BoxSet &flush,
                     int full, double *part = nullptr, const LaserPlane &lp = LaserPlane());
 template <class T>

@@ -109,6 +109,36 @@ template <class A> static inline void start_one(hipStream_t s, const A &a, dim3 
         k_one<A><<<grid, KernelDef<A>::kBlock, 0, s>>>(a);
 }
 
+/* Two bodies with no order between them in one launch (same block size):
+ * blocks [0, na) run A over its own grid ga, the others B over gb (both
+ * flattened, x fastest).  Every tiny launch costs a few microseconds of
+ * stream time, so independent neighbours of a step share one. */
+template <class A, class B> struct PairArgs {
+    A        a;
+    B        b;
+    VBlock   ga, gb;
+    unsigned na;
+};
+
+__device__ static inline VBlock unflat(unsigned L, const VBlock &g)
+{
+    const VBlock b = {L % g.x, (L / g.x) % g.y, L / (g.x * g.y)};
+    return b;
+}
+
+template <class A, class B> struct KernelDef<PairArgs<A, B>> {
+    static_assert(KernelDef<A>::kBlock == KernelDef<B>::kBlock, "a pair shares one block size");
+    static constexpr int kBlock = KernelDef<A>::kBlock;
+    __device__ __forceinline__ static void body(const PairArgs<A, B> &p, VBlock blk, VBlock)
+    {
+        const unsigned L = blk.x;            /* pairs are launched over a 1-D grid */
+        if (L < p.na)
+            KernelDef<A>::body(p.a, unflat(L, p.ga), p.ga);
+        else
+            KernelDef<B>::body(p.b, unflat(L - p.na, p.gb), p.gb);
+    }
+};
+
 /* One launch as data. */
 struct LaunchRec {
     const void *kind;                 /* identifies the body (KernelDef<A>::tag) */
@@ -209,6 +239,23 @@ static int launch_def(hipStream_t s, const A &a, unsigned gx, unsigned gy = 1, u
     start_one(s, a, dim3(gx, gy, gz));
     MPIC_CHECK(hipGetLastError());
     return MPIC_OK;
+}
+
+/* A over grid ga and B over grid gb in one launch (one of them alone when
+ * the other's grid is empty). */
+template <class A, class B>
+static int launch_pair(hipStream_t s, const A &a, dim3 ga, const B &b, dim3 gb)
+{
+    const unsigned na = ga.x * ga.y * ga.z, nb = gb.x * gb.y * gb.z;
+    if (na == 0) return launch_def(s, b, gb.x, gb.y, gb.z);
+    if (nb == 0) return launch_def(s, a, ga.x, ga.y, ga.z);
+    PairArgs<A, B> p;
+    p.a = a;
+    p.b = b;
+    p.ga = {ga.x, ga.y, ga.z};
+    p.gb = {gb.x, gb.y, gb.z};
+    p.na = na;
+    return launch_def(s, p, na + nb);
 }
 
 #endif

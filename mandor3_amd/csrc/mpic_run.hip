@@ -173,6 +173,7 @@ struct mpic_run {
     std::vector<long>   ring_step;  /* step whose row a slot holds (-1: none) */
     std::vector<double> ring_time;  /* Time at the start of that step */
     int           capturing;  /* step_body is being recorded into graphs, not run */
+    int           jclear_done;  /* the H half step cleared J and the temperature sums */
     double        vsp;        /* VSP convolution weight alpha (1: off) */
     long          topo;       /* bumped whenever the launch sequence of a step changes */
 };
@@ -1771,7 +1772,7 @@ static int laser_E(mpic_run *r, T *Ey, T *Ez, const Box &tu)
 
 /* em_HHalfStep; with 'part' the H update also writes em_energy's partial
  * sums of E and of the H it leaves (H^n). */
-static int h_half(mpic_run *r, double *part);
+static int h_half(mpic_run *r, double *part, bool with_clear = false);
 
 extern "C" int mpic_em_h_half(mpic_run_t *r)
 {
@@ -1779,7 +1780,9 @@ extern "C" int mpic_em_h_half(mpic_run_t *r)
     return h_half(r, nullptr);
 }
 
-static int h_half(mpic_run *r, double *part)
+/* The J clear of the push that follows the H half step; with_clear runs it
+ * in the H update's launch (nothing between them reads or writes J). */
+static int h_half(mpic_run *r, double *part, bool with_clear)
 {
     TRY(check_stage(r, 0));
     MPIC_CHECK(hipSetDevice(r->dev));
@@ -1800,7 +1803,14 @@ static int h_half(mpic_run *r, double *part)
     FCALL(launch_cap_E, r->g, r->updE);
     TRY(walls_on(r, r->bcE));
     /* em_HSmallStep + cap_flushHalfH */
-    FCALL(launch_h_update, r->g, r->flH, 0, part);
+    if (with_clear) {
+        FCALL(launch_h_update_jclear, r->g, r->flH, 0, part, r->jp.J[0], r->dacc + 2, r->djr,
+              r->dstep, r->tiled ? r->tv.active : nullptr, r->tiled ? r->tv.nactive : nullptr,
+              r->tiled ? r->tv.nt[1] * r->tv.nt[2] : 1, r->tiled ? 0 : 1);
+        r->jclear_done = 1;
+    } else {
+        FCALL(launch_h_update, r->g, r->flH, 0, part);
+    }
     r->stage = 1;
     return MPIC_OK;
 }
@@ -1832,10 +1842,13 @@ extern "C" int mpic_push_deposit(mpic_run_t *r)
     TRY(check_stage(r, 1));
     MPIC_CHECK(hipSetDevice(r->dev));
     /* J (the planes the last deposit wrote) and the temperature sums: a
-     * kernel, not a memset (the step may be replayed from a graph) */
-    TRY(launch_jclear(r->stream, r->jp.J[0], r->g, r->dacc + 2, r->djr, r->dstep,
-                      r->tiled ? r->tv.active : nullptr, r->tiled ? r->tv.nactive : nullptr,
-                      r->tiled ? r->tv.nt[1] * r->tv.nt[2] : 1, r->tiled ? 0 : 1));
+     * kernel, not a memset (the step may be replayed from a graph), unless
+     * the H half step's launch cleared them */
+    if (!r->jclear_done)
+        TRY(launch_jclear(r->stream, r->jp.J[0], r->g, r->dacc + 2, r->djr, r->dstep,
+                          r->tiled ? r->tv.active : nullptr, r->tiled ? r->tv.nactive : nullptr,
+                          r->tiled ? r->tv.nt[1] * r->tv.nt[2] : 1, r->tiled ? 0 : 1));
+    r->jclear_done = 0;
     if (r->timed >= 0) TRY(record(r, 1));
     if (r->fp32) {
         TRY(launch_push32(r->stream, r->m32, r->m32buf[r->cur ^ 1], r->f32, r->g, r->pc, r->qm,
@@ -1872,14 +1885,21 @@ extern "C" int mpic_temperature(mpic_run_t *r, double *WTx, double *WTy, double 
     return MPIC_OK;
 }
 
-extern "C" int mpic_em_h_step(mpic_run_t *r)
+/* The H step; with_fold also runs the y and z J folds in the H update's
+ * launch (the H step neither reads nor writes J) and then the VSP sweep,
+ * the rest of mpic_j_finish after its x folds. */
+static int h_step(mpic_run *r, bool with_fold)
 {
-    if (!r) return MPIC_ERR_ARG;
     TRY(check_stage(r, 1));
     MPIC_CHECK(hipSetDevice(r->dev));
     /* the laser's H source in the same pass where nothing comes between */
     const LaserPlane lf = laser_fold(r, 1);
-    FCALL(launch_h_update, r->g, r->flH, 1, nullptr, lf);
+    if (with_fold) {
+        FCALL(launch_h_update_jfold, r->g, r->flH, 1, lf, r->jp, r->cfg.bc, r->djr + 4);
+        if (fabs(r->vsp - 1.0) >= 1e-10) TRY(launch_vsp(r->stream, r->jp.J, 3, r->g, r->vsp));
+    } else {
+        FCALL(launch_h_update, r->g, r->flH, 1, nullptr, lf);
+    }
     if (r->tfsf.on) {                    /* TFSF_postHStep, source/core/em.c:125 */
         const Tfsf &tf = r->tfsf;
         if (r->fp32)
@@ -1900,6 +1920,12 @@ extern "C" int mpic_em_h_step(mpic_run_t *r)
     return MPIC_OK;
 }
 
+extern "C" int mpic_em_h_step(mpic_run_t *r)
+{
+    if (!r) return MPIC_ERR_ARG;
+    return h_step(r, false);
+}
+
 extern "C" int mpic_em_e_start(mpic_run_t *r)
 {
     if (!r) return MPIC_ERR_ARG;
@@ -1912,13 +1938,20 @@ extern "C" int mpic_em_e_start(mpic_run_t *r)
     return MPIC_OK;
 }
 
+/* jbc_finish's x walls: the first part of mpic_j_finish. */
+static int j_fold_x(mpic_run *r)
+{
+    const int *bc = r->cfg.bc;
+    return launch_jfold(r->stream, r->jp, r->g, 0, bc[0], bc[3], r->djr + 4);
+}
+
 extern "C" int mpic_j_finish(mpic_run_t *r)
 {
     if (!r) return MPIC_ERR_ARG;
     MPIC_CHECK(hipSetDevice(r->dev));
     const int *bc = r->cfg.bc;
     /* x walls, then the y and z walls plane by plane */
-    TRY(launch_jfold(r->stream, r->jp, r->g, 0, bc[0], bc[3], r->djr + 4));
+    TRY(j_fold_x(r));
     TRY(launch_jfold_planes(r->stream, r->jp, r->g, bc, r->djr + 4));
     /* VSP_vectorSweep after the boundary fold (source/core/plasma_currentWalls.c:204) */
     if (fabs(r->vsp - 1.0) >= 1e-10) TRY(launch_vsp(r->stream, r->jp.J, 3, r->g, r->vsp));
@@ -2021,12 +2054,14 @@ static int step_body(mpic_run *r)
      * plasma_temperature after the push (source/core/main.c:371,383): one
      * ring row per step, nothing copied back, so the step stays
      * asynchronous */
-    TRY(h_half(r, r->dpart));
+    /* the J clear joins the H half step's launch, the y and z J folds the
+     * H step's */
+    TRY(h_half(r, r->dpart, true));
     TRY(mpic_push_deposit(r));
     /* jbc_finish reads and writes J only and the H step does not read J:
      * the fold goes first, so that E -= tau J can join the E update */
-    TRY(mpic_j_finish(r));
-    TRY(mpic_em_h_step(r));
+    TRY(j_fold_x(r));
+    TRY(h_step(r, true));
     /* one pass for em_EStep_start and E -= tau J where no source adds to E
      * between them (FP64 form), always in the FP32 split form (its TFSF and
      * laser sources then add after E -= tau J) */
