@@ -46,17 +46,24 @@ template <class A> struct WavesOf<A, std::void_t<decltype(KernelDef<A>::kWaves)>
     static constexpr int v = KernelDef<A>::kWaves;
 };
 
-template <class A>
-__device__ static inline void run_one(const A &a)
-{
-    const VBlock b = {blockIdx.x, blockIdx.y, blockIdx.z}, g = {gridDim.x, gridDim.y, gridDim.z};
-    KernelDef<A>::body(a, b, g);
-}
-
-/* Read-only data of a multi-run launch in the constant address space: its
- * loads are invariant, so the bodies read argument fields in place (as from
- * the kernel-argument segment) instead of copying whole structs to scratch. */
+/* Read-only data of a launch in the constant address space: its loads are
+ * invariant, so the bodies read argument fields in place, when they are
+ * used, instead of holding whole structs in registers (or copying them to
+ * scratch) for the kernel's life. */
 #define CAS __attribute__((address_space(4)))
+
+/* A one-run launch's argument struct is its first kernel argument, at the
+ * start of the kernel-argument segment: the body reads it there.  With the
+ * struct passed by value the compiler keeps every field it uses live in
+ * SGPRs; the push kernels' structs then spilled (46 SGPRs in the FP32 push,
+ * 247 VGPRs for the FP64 push instead of 164). */
+template <class A>
+__device__ static inline void run_one()
+{
+    const CAS A *ap = (const CAS A *)__builtin_amdgcn_kernarg_segment_ptr();
+    const VBlock b = {blockIdx.x, blockIdx.y, blockIdx.z}, g = {gridDim.x, gridDim.y, gridDim.z};
+    KernelDef<A>::body(*(const A *)ap, b, g);
+}
 
 /* tab: [n + 1] first block of every run, then [3 n] the runs' own grids. */
 template <class A>
@@ -77,9 +84,9 @@ __device__ static inline void run_multi(const CAS A *__restrict__ args,
     KernelDef<A>::body(*(const A *)(args + lo), b, g);
 }
 
-template <class A> __global__ void __launch_bounds__(KernelDef<A>::kBlock) k_one(A a)
+template <class A> __global__ void __launch_bounds__(KernelDef<A>::kBlock) k_one(A)
 {
-    run_one(a);
+    run_one<A>();
 }
 template <class A>
 __global__ void __launch_bounds__(KernelDef<A>::kBlock)
@@ -89,9 +96,9 @@ __global__ void __launch_bounds__(KernelDef<A>::kBlock)
 }
 template <class A>
 __global__ void __launch_bounds__(KernelDef<A>::kBlock)
-    __attribute__((amdgpu_waves_per_eu(WavesOf<A>::v))) k_one_w(A a)
+    __attribute__((amdgpu_waves_per_eu(WavesOf<A>::v))) k_one_w(A)
 {
-    run_one(a);
+    run_one<A>();
 }
 template <class A>
 __global__ void __launch_bounds__(KernelDef<A>::kBlock)

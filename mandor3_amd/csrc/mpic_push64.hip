@@ -138,8 +138,10 @@ __device__ static void deposit_multi(const S &v, const Grid &g, int i1, int j1, 
         const double sgZ = dk * ((k1 + k2 + 1) * 0.5 * h3 - z1) / (z2 - z1 + ((1 - dk) << 16));
         double k1s = sgX + (1 - di) * sgY;
         double k2s = sgZ + (1 - dk) * sgY;
-        int dI0[3] = {di * (i2 - i1), (1 - di) * (j2 - j1), 0};
-        int dI1[3] = {0, (1 - dk) * (j2 - j1), dk * (k2 - k1)};
+        /* the two crossings' cell steps, in the order they happen (selected
+         * by value: an array picked by pointer would live in scratch) */
+        const int dI0[3] = {di * (i2 - i1), (1 - di) * (j2 - j1), 0};
+        const int dI1[3] = {0, (1 - dk) * (j2 - j1), dk * (k2 - k1)};
         int st = 0;
         if (k1s > k2s) {
             double t = k1s;
@@ -149,15 +151,14 @@ __device__ static void deposit_multi(const S &v, const Grid &g, int i1, int j1, 
         }
         const double xA = x1 + k1s * (x2 - x1), yA = y1 + k1s * (y2 - y1), zA = z1 + k1s * (z2 - z1);
         const double xB = x1 + k2s * (x2 - x1), yB = y1 + k2s * (y2 - y1), zB = z1 + k2s * (z2 - z1);
-        const int *fst = st ? dI1 : dI0, *snd = st ? dI0 : dI1;
         deposit_seg(v, g, i, j, k, x1, xA, y1, yA, z1, zA, rp);
-        i += fst[0];
-        j += fst[1];
-        k += fst[2];
+        i += st ? dI1[0] : dI0[0];
+        j += st ? dI1[1] : dI0[1];
+        k += st ? dI1[2] : dI0[2];
         deposit_seg(v, g, i, j, k, xA, xB, yA, yB, zA, zB, rp);
-        i += snd[0];
-        j += snd[1];
-        k += snd[2];
+        i += st ? dI0[0] : dI1[0];
+        j += st ? dI0[1] : dI1[1];
+        k += st ? dI0[2] : dI1[2];
         deposit_seg(v, g, i, j, k, xB, x2, yB, y2, zB, z2, rp);
         return;
     }
