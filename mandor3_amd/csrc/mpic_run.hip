@@ -125,7 +125,7 @@ struct mpic_run {
     long          n, cap;     /* n: markers loaded (upper bound of the live count) */
     int           tiled;      /* 1: sorted tiled push (default), 0: untiled push */
     TileView      tv;
-    int          *blocksum;
+    unsigned long long *scanw;  /* the tile scan's published words and sequence number */
     int           parity;     /* tv.nmark[parity] = markers in the current order */
     int           push_blocks;  /* FP64: resident workgroups of the persistent push */
     int           tile_blocks;  /* FP32: workgroups of the tile push (work items taken in turn) */
@@ -142,7 +142,8 @@ struct mpic_run {
     PushConsts    pc;
     double       *dacc;       /* device scratch: [0..1] energy, [2..4] W, [5..6] gauss */
     double       *hacc;       /* pinned host mirror */
-    unsigned     *derr;       /* device error flags (bit 0: marker off the grid) */
+    unsigned     *derr;       /* device error flags (bit 0: marker off the grid, bit 1: a
+                                 tile scan's wait for an earlier workgroup ran out) */
     int           stage;
     int           inited;
     int           timed;      /* >= 0: index of the timed step of mpic_advance */
@@ -536,10 +537,10 @@ extern "C" int mpic_create(const mpic_config_t *cfg, mpic_run_t **out)
     tv.nmark = tv.nactive + 1 + MPIC_XCDS;
     tv.nitems = tv.nactive + 3 + MPIC_XCDS;
     MPIC_CHECK(hipMemsetAsync(tv.nactive, 0, sizeof(int) * (4 + MPIC_XCDS), r->stream));
-    /* scan scratch: block sums (3 per block + totals) and 3 ints per tile */
-    MPIC_CHECK(hipMalloc(&r->blocksum,
-                         sizeof(int) * (3 * ((tv.ntiles + MPIC_SCAN_BLOCK - 1) / MPIC_SCAN_BLOCK + 1) +
-                                       3 * (long)tv.ntiles)));
+    /* the tile scan's published words: zero never matches a scan's tag */
+    MPIC_CHECK(hipMalloc(&r->scanw, sizeof(unsigned long long) * tile_scan_words(tv.ntiles)));
+    MPIC_CHECK(hipMemsetAsync(r->scanw, 0, sizeof(unsigned long long) * tile_scan_words(tv.ntiles),
+                              r->stream));
     r->parity = 0;
     {
         int ncu = 256;
@@ -632,7 +633,7 @@ extern "C" int mpic_destroy(mpic_run_t *r)
     }
     hipFree(r->tv.off); hipFree(r->tv.toff); hipFree(r->tv.cursor); hipFree(r->tv.count); hipFree(r->tv.key);
     hipFree(r->tv.perm);
-    hipFree(r->tv.active); hipFree(r->tv.slot); hipFree(r->tv.nactive); hipFree(r->blocksum);
+    hipFree(r->tv.active); hipFree(r->tv.slot); hipFree(r->tv.nactive); hipFree(r->scanw);
     hipFree(r->tv.item); hipFree(r->tv.istart);
     hipFree(r->tj);
     hipFree(r->dacc);
@@ -1529,7 +1530,7 @@ static int sort_pass(mpic_run *r, int keys)
             else
                 TRY(launch_keys64(r->stream, r->m, r->n, r->g, r->tv, r->parity));
         }
-        TRY(launch_tile_scan(r->stream, r->tv, r->blocksum, r->parity ^ 1));
+        TRY(launch_tile_scan(r->stream, r->tv, r->scanw, r->derr, r->parity ^ 1));
         TRY(launch_rank(r->stream, r->n, r->tv, r->parity));
         if (keys)
             r->parity ^= 1;
@@ -1539,7 +1540,7 @@ static int sort_pass(mpic_run *r, int keys)
     }
     /* the untiled FP64 push keeps no order: a sort compacts its array */
     if (keys) TRY(launch_keys64(r->stream, r->m, r->n, r->g, r->tv, r->parity));
-    TRY(launch_tile_scan(r->stream, r->tv, r->blocksum, r->parity ^ 1));
+    TRY(launch_tile_scan(r->stream, r->tv, r->scanw, r->derr, r->parity ^ 1));
     TRY(launch_scatter64(r->stream, r->m, r->mbuf[other], r->n, r->tv, r->parity));
     swap_markers(r);
     return MPIC_OK;
@@ -2659,6 +2660,11 @@ static int check_flags(mpic_run *r)
 {
     unsigned f = 0;
     MPIC_CHECK(hipMemcpy(&f, r->derr, sizeof(unsigned), hipMemcpyDeviceToHost));
+    if (f & 2u) {
+        mpic_set_error("the tile scan's wait for an earlier workgroup ran out: the marker order "
+                       "is invalid (flags %#x)", f);
+        return MPIC_ERR_DEVICE;
+    }
     if (f) {
         mpic_set_error("markers outside the grid were deleted (flags %#x)", f);
         return MPIC_ERR_STATE;

@@ -118,7 +118,11 @@ __device__ static inline void wave_run_count(int *count, int key)
 
 int launch_keys64(hipStream_t s, const Markers64 &m, long n, const Grid &g, const TileView &tv,
                   int parity);
-int launch_tile_scan(hipStream_t s, const TileView &tv, int *blocksum, int parity_next);
+/* The tile scan's word buffer (zeroed once at allocation) and launcher;
+ * err bit 1 is set if a wait of the scan ran out (never expected). */
+long tile_scan_words(int ntiles);
+int launch_tile_scan(hipStream_t s, const TileView &tv, unsigned long long *words, unsigned *err,
+                     int parity_next);
 int launch_rank(hipStream_t s, long nbound, const TileView &tv, int parity);
 int launch_scatter64(hipStream_t s, const Markers64 &src, const Markers64 &dst, long nbound,
                      const TileView &tv, int parity);

@@ -46,185 +46,217 @@ int launch_keys64(hipStream_t s, const Markers64 &m, long n, const Grid &g, cons
     return MPIC_OK;
 }
 
-/* Block-local exclusive scans of (markers per tile, tile non-empty, work
- * items of the tile), three ints per tile and per block. */
+/* The tile scan, in one pass: every workgroup takes SB consecutive tiles,
+ * sums their per-key counts, scans (markers, tile non-empty, work items)
+ * over its tiles, and finds the sum of all earlier tiles by a decoupled
+ * look-back (Merrill and Garland, "Single-pass parallel prefix scan with
+ * decoupled look-back", NVIDIA technical report NVR-2016-002): it publishes
+ * its own totals at once, then adds up its predecessors' published values
+ * back to the first one that has published its inclusive prefix, and
+ * publishes its own inclusive prefix.  A workgroup only waits for
+ * workgroups of lower index, which the hardware has dispatched before it,
+ * so every wait ends; the wait is bounded all the same (SCAN_SPIN polls,
+ * then an error flag, and the kernel still exits).
+ *
+ * Published values are 64-bit words (tag << 32 | value), three for a
+ * workgroup's totals and three for its inclusive prefix; the tag is the
+ * scan's sequence number (+1), so words of earlier scans never match and
+ * the words are never cleared.  The last workgroup advances the sequence
+ * number after every workgroup has published (its inclusive prefix needs
+ * all of them), i.e. after every workgroup has read it.
+ *
+ * Layout of the counts: a tile's 64 per-key counts are 16 int4; 16 lanes of
+ * a wavefront read one tile (coalesced), a wavefront four tiles per load,
+ * and keep them in registers for the per-key offsets written at the end. */
 #define NSCAN 3
-struct ScanAArgs {
-    TileView tv;
-    int     *blocksum, *tileloc;
+#define SCAN_SPIN (1 << 20)
+struct ScanArgs {
+    TileView            tv;
+    unsigned long long *flags;        /* [nb][6] published words */
+    int                *seq;          /* scan sequence number */
+    unsigned           *err;          /* bit 1: a look-back wait ran out */
+    int                 nb, parity_next;
 };
 
-__device__ static inline void scan_a(const TileView &tv, int *blocksum, int *tileloc,
-                                     const VBlock &bk)
+__device__ static inline unsigned long long scan_word(unsigned tag, int v)
 {
+    return ((unsigned long long)tag << 32) | (unsigned)v;
+}
+
+__device__ static inline void scan_pass(const ScanArgs &A, const VBlock &bk)
+{
+    const TileView &tv = A.tv;
     __shared__ int wsum[NSCAN][SB / 64];
-    const int t = bk.x * SB + threadIdx.x, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    int c = 0;
-    if (t < tv.ntiles) {
-        const int4 *q = (const int4 *)(tv.count + (long)t * TILE_CELLS);
-        for (int v = 0; v < TILE_CELLS / 4; ++v) {
-            const int4 x = q[v];
-            c += x.x + x.y + x.z + x.w;
+    __shared__ int spre[NSCAN];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, blk = (int)bk.x;
+    const int wt0 = blk * SB + w * 64;               /* first tile of the wavefront */
+    const int4 *cnt4 = (const int4 *)tv.count;
+    int4 x[16];
+    int ex[16];                                      /* this lane's first key, within its tile */
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+        const int t = wt0 + v * 4 + (lane >> 4);
+        x[v] = t < tv.ntiles ? cnt4[(long)t * 16 + (lane & 15)] : make_int4(0, 0, 0, 0);
+    }
+    int c = 0;                                       /* markers of tile wt0 + lane */
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+        const int s = x[v].x + x[v].y + x[v].z + x[v].w;
+        int inc = s;
+        for (int d = 1; d < 16; d <<= 1) {
+            const int y = __shfl_up(inc, d, 16);
+            if ((lane & 15) >= d) inc += y;
         }
+        ex[v] = inc - s;
+        const int tot = __shfl(inc, (lane & 3) * 16 + 15, 64);   /* tile v*4 + (lane & 3) */
+        if ((lane >> 2) == v) c = tot;
     }
     const int own[NSCAN] = {c, c > 0, tile_items(c, tv.chm)};
     int inc[NSCAN];
-    for (int a = 0; a < NSCAN; ++a) {        /* wavefront inclusive scan */
-        int x = own[a];
+    for (int a = 0; a < NSCAN; ++a) {                /* wavefront inclusive scan */
+        int y = own[a];
         for (int d = 1; d < 64; d <<= 1) {
-            const int y = __shfl_up(x, d, 64);
-            if (lane >= d) x += y;
+            const int z = __shfl_up(y, d, 64);
+            if (lane >= d) y += z;
         }
-        inc[a] = x;
-        if (lane == 63) wsum[a][w] = x;
+        inc[a] = y;
+        if (lane == 63) wsum[a][w] = y;
     }
     __syncthreads();
-    for (int a = 0; a < NSCAN; ++a)
-        for (int q = 0; q < w; ++q) inc[a] += wsum[a][q];
-    if (t < tv.ntiles)
-        for (int a = 0; a < NSCAN; ++a) tileloc[NSCAN * t + a] = inc[a] - own[a];
-    if (threadIdx.x == SB - 1)
-        for (int a = 0; a < NSCAN; ++a) blocksum[NSCAN * bk.x + a] = inc[a];
-}
-
-template <> struct KernelDef<ScanAArgs> {
-    static constexpr int kBlock = SB;
-    __device__ __forceinline__ static void body(const ScanAArgs &a, VBlock b, VBlock)
-    {
-        scan_a(a.tv, a.blocksum, a.tileloc, b);
+    int agg[NSCAN];
+    for (int a = 0; a < NSCAN; ++a) {
+        agg[a] = 0;
+        for (int q = 0; q < SB / 64; ++q) {
+            if (q < w) inc[a] += wsum[a][q];
+            agg[a] += wsum[a][q];
+        }
     }
-};
-
-/* Exclusive scan of the block sums (one block, loops for any count). */
-struct ScanBArgs {
-    int *blocksum;
-    int  nb;
-};
-
-__device__ static inline void scan_b(int *blocksum, int nb)
-{
-    __shared__ int sv[NSCAN][SB];
-    __shared__ int carry[NSCAN];
-    if (threadIdx.x < NSCAN) carry[threadIdx.x] = 0;
+    if (w == 0) {                                    /* the look-back, one wavefront */
+        unsigned long long *F = A.flags;
+        const int seq = *A.seq;
+        const unsigned tag = (unsigned)(seq % 0x7fffffff) + 1u;
+        int pre[NSCAN] = {0, 0, 0};
+        if (blk > 0) {
+            if (lane < NSCAN)
+                __hip_atomic_store(F + 6 * blk + lane, scan_word(tag, agg[lane]), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            int end = blk - 1;                       /* the window's nearest predecessor */
+            unsigned polls = 0;
+            for (;;) {
+                const int j = end - lane;
+                bool ready = true, incl = false;
+                int v[NSCAN] = {0, 0, 0};
+                if (j >= 0) {
+                    unsigned long long q[NSCAN];
+                    bool ok = true;
+                    for (int a = 0; a < NSCAN; ++a) {
+                        q[a] = __hip_atomic_load(F + 6 * j + 3 + a, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT);
+                        ok = ok && (unsigned)(q[a] >> 32) == tag;
+                    }
+                    incl = ok;
+                    if (!ok) {
+                        ok = true;
+                        for (int a = 0; a < NSCAN; ++a) {
+                            q[a] = __hip_atomic_load(F + 6 * j + a, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT);
+                            ok = ok && (unsigned)(q[a] >> 32) == tag;
+                        }
+                    }
+                    ready = ok;
+                    for (int a = 0; a < NSCAN; ++a) v[a] = (int)(unsigned)q[a];
+                }
+                /* lanes up to the nearest inclusive prefix (or all) are needed */
+                const unsigned long long im = __ballot(incl), nr = __ballot(!ready);
+                const int L = im ? __ffsll((long long)im) - 1 : 63;
+                const unsigned long long need = L == 63 ? ~0ull : ((2ull << L) - 1ull);
+                if (nr & need) {
+                    if (++polls > SCAN_SPIN) {       /* never expected: give up, flagged */
+                        if (lane == 0 && A.err) atomicOr(A.err, 2u);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                for (int a = 0; a < NSCAN; ++a) {
+                    int y = (lane <= L && j >= 0) ? v[a] : 0;
+                    for (int d = 32; d > 0; d >>= 1) y += __shfl_xor(y, d, 64);
+                    pre[a] += y;
+                }
+                if (im) break;
+                end -= 64;                           /* block 0 is always inclusive: end stays >= 0 */
+            }
+        }
+        if (lane < NSCAN) {
+            __hip_atomic_store(F + 6 * blk + 3 + lane, scan_word(tag, pre[lane] + agg[lane]),
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            spre[lane] = pre[lane];
+        }
+        if (blk == A.nb - 1 && lane == 0) {          /* totals, then the next sequence number */
+            const int tot = pre[0] + agg[0], na = pre[1] + agg[1], ni = pre[2] + agg[2];
+            tv.off[tv.ncellp] = tot;
+            tv.toff[tv.ntiles] = tot;
+            *tv.nactive = na;
+            tv.istart[na] = ni;
+            *tv.nitems = ni;
+            tv.nmark[A.parity_next] = tot;
+            for (int q = 0; q < MPIC_XCDS; ++q) tv.queue[q] = 0;
+            *A.seq = seq + 1;
+        }
+    }
     __syncthreads();
-    for (int b0 = 0; b0 < nb; b0 += SB) {
-        const int b = b0 + threadIdx.x;
-        int own[NSCAN];
-        for (int a = 0; a < NSCAN; ++a) {
-            own[a] = (b < nb) ? blocksum[NSCAN * b + a] : 0;
-            sv[a][threadIdx.x] = own[a];
-        }
-        __syncthreads();
-        for (int d = 1; d < SB; d <<= 1) {
-            int x[NSCAN] = {0, 0, 0};
-            if ((int)threadIdx.x >= d)
-                for (int a = 0; a < NSCAN; ++a) x[a] = sv[a][threadIdx.x - d];
-            __syncthreads();
-            for (int a = 0; a < NSCAN; ++a) sv[a][threadIdx.x] += x[a];
-            __syncthreads();
-        }
-        if (b < nb)
-            for (int a = 0; a < NSCAN; ++a)
-                blocksum[NSCAN * b + a] = carry[a] + sv[a][threadIdx.x] - own[a];
-        __syncthreads();
-        if (threadIdx.x == SB - 1)
-            for (int a = 0; a < NSCAN; ++a) carry[a] += sv[a][SB - 1];
-        __syncthreads();
-    }
-    if (threadIdx.x < NSCAN) blocksum[NSCAN * nb + threadIdx.x] = carry[threadIdx.x];
-}
-
-template <> struct KernelDef<ScanBArgs> {
-    static constexpr int kBlock = SB;
-    __device__ __forceinline__ static void body(const ScanBArgs &a, VBlock, VBlock) { scan_b(a.blocksum, a.nb); }
-};
-
-/* Tile offsets, the active-tile list and, for every non-empty tile, the
- * per-key offsets and cursors with its counters reset.  One wavefront per
- * SC_TPW consecutive tiles: lane l < SC_TPW keeps the books of tile l (its
- * total comes from the tile scan), then the wavefront takes the non-empty
- * tiles one after the other, one lane per cell key (coalesced).  Empty
- * tiles (their counters are all 0) cost two words: most tiles of a sparse
- * plasma are empty, and fewer wavefronts than tiles keep the launch short. */
-#define SC_TPW 8
-struct ScanCArgs {
-    TileView   tv;
-    const int *blocksum, *tileloc;
-    int        nb, parity_next;
-};
-
-__device__ static inline void scan_c(const TileView &tv, const int *blocksum, const int *tileloc,
-                                     int nb, int parity_next, const VBlock &bk)
-{
-    const int lane = threadIdx.x & 63;
-    const int w = (int)(((long)bk.x * 256 + threadIdx.x) >> 6);
-    const int t0 = w * SC_TPW;
-    if (t0 >= tv.ntiles) return;                     /* whole waves exit together */
-    int base = 0, tot = 0;
-    if (lane < SC_TPW && t0 + lane < tv.ntiles) {
-        const int t = t0 + lane, tb = t / SB, tn = t + 1;
-        base = blocksum[NSCAN * tb] + tileloc[NSCAN * t];
-        const int next = tn < tv.ntiles ? blocksum[NSCAN * (tn / SB)] + tileloc[NSCAN * tn]
-                                        : blocksum[NSCAN * nb];
-        tot = next - base;
+    /* tile offsets, the active list, the work items */
+    const int base = spre[0] + inc[0] - own[0];
+    const int t = wt0 + lane;
+    if (t < tv.ntiles) {
         tv.toff[t] = base;
-        const int a = blocksum[NSCAN * tb + 1] + tileloc[NSCAN * t + 1];
-        if (tot > 0) {
-            const int is = blocksum[NSCAN * tb + 2] + tileloc[NSCAN * t + 2];
+        const int a = spre[1] + inc[1] - own[1];
+        if (c > 0) {
+            const int is = spre[2] + inc[2] - own[2];
             tv.active[a] = t;
             tv.istart[a] = is;
-            const int ni = tile_items(tot, tv.chm);
-            for (int u = 0; u < ni; ++u) tv.item[is + u] = a;
+            for (int u = 0; u < own[2]; ++u) tv.item[is + u] = a;
         }
-        tv.slot[t] = tot > 0 ? a : -1;
+        tv.slot[t] = c > 0 ? a : -1;
     }
-    if (w == 0 && lane == 0) {
-        tv.off[tv.ncellp] = blocksum[NSCAN * nb];
-        tv.toff[tv.ntiles] = blocksum[NSCAN * nb];
-        *tv.nactive = blocksum[NSCAN * nb + 1];
-        tv.istart[blocksum[NSCAN * nb + 1]] = blocksum[NSCAN * nb + 2];
-        *tv.nitems = blocksum[NSCAN * nb + 2];
-        tv.nmark[parity_next] = blocksum[NSCAN * nb];
-        for (int x = 0; x < MPIC_XCDS; ++x) tv.queue[x] = 0;
-    }
-    for (int it = 0; it < SC_TPW; ++it) {
-        if (__shfl(tot, it, 64) == 0) continue;      /* wave-uniform */
-        const int tb0 = __shfl(base, it, 64);
-        const long q = (long)(t0 + it) * TILE_CELLS + lane;
-        const int c = tv.count[q];
-        int incl = c;
-        for (int d = 1; d < 64; d <<= 1) {
-            const int v = __shfl_up(incl, d, 64);
-            if (lane >= d) incl += v;
+    /* per-key offsets and cursors of the non-empty tiles, counters reset
+     * (entries of empty tiles are not maintained) */
+    int4 *off4 = (int4 *)tv.off, *cur4 = (int4 *)tv.cursor, *cnt4w = (int4 *)tv.count;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+        const int tt = v * 4 + (lane >> 4);
+        const int tb = __shfl(base, tt, 64), tc = __shfl(c, tt, 64);
+        if (tc > 0) {
+            const long q = (long)(wt0 + tt) * 16 + (lane & 15);
+            const int o0 = tb + ex[v];
+            const int4 o = make_int4(o0, o0 + x[v].x, o0 + x[v].x + x[v].y,
+                                     o0 + x[v].x + x[v].y + x[v].z);
+            off4[q] = o;
+            cur4[q] = o;
+            cnt4w[q] = make_int4(0, 0, 0, 0);
         }
-        const int o = tb0 + incl - c;
-        tv.off[q] = o;
-        tv.cursor[q] = o;
-        tv.count[q] = 0;
     }
 }
 
-template <> struct KernelDef<ScanCArgs> {
-    static constexpr int kBlock = 256;
-    __device__ __forceinline__ static void body(const ScanCArgs &a, VBlock b, VBlock)
+template <> struct KernelDef<ScanArgs> {
+    static constexpr int kBlock = SB;
+    __device__ __forceinline__ static void body(const ScanArgs &a, VBlock b, VBlock)
     {
-        scan_c(a.tv, a.blocksum, a.tileloc, a.nb, a.parity_next, b);
+        scan_pass(a, b);
     }
 };
 
-int launch_tile_scan(hipStream_t s, const TileView &tv, int *blocksum, int parity_next)
+long tile_scan_words(int ntiles)
+{
+    return 6L * ((ntiles + SB - 1) / SB) + 1;
+}
+
+int launch_tile_scan(hipStream_t s, const TileView &tv, unsigned long long *words, unsigned *err,
+                     int parity_next)
 {
     const int nb = (tv.ntiles + SB - 1) / SB;
-    int *tileloc = blocksum + NSCAN * (nb + 1);
-    ScanAArgs a = {tv, blocksum, tileloc};
-    int rc = launch_def(s, a, (unsigned)nb);
-    if (rc != MPIC_OK) return rc;
-    ScanBArgs b = {blocksum, nb};
-    rc = launch_def(s, b, 1);
-    if (rc != MPIC_OK) return rc;
-    const long waves = (tv.ntiles + SC_TPW - 1) / SC_TPW;
-    ScanCArgs c = {tv, blocksum, tileloc, nb, parity_next};
-    return launch_def(s, c, (unsigned)((waves * 64 + 255) / 256));
+    ScanArgs a = {tv, words, (int *)(words + 6L * nb), err, nb, parity_next};
+    return launch_def(s, a, (unsigned)nb);
 }
 
 /* The next order: every position p < nmark[parity] with a key (the push's
