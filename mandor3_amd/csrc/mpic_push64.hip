@@ -82,6 +82,12 @@ struct TileSink {
             gs.seg(i, j, k, vx, vy, vz);
             return;
         }
+        lds_add(a, b, c, vx, vy, vz);
+    }
+    /* the 12 values of a segment in the block's cell (a, b, c) */
+    __device__ void lds_add(int a, int b, int c, const double vx[4], const double vy[4],
+                            const double vz[4]) const
+    {
 #define L(aa, bb, cc, comp) (lj + 3 * (((aa) * TJN + (bb)) * TJN + (cc)) + (comp))
         atomicAdd(L(a + 1, b, c, 0), vx[0]);
         atomicAdd(L(a + 1, b + 1, c, 0), vx[1]);
@@ -99,11 +105,10 @@ struct TileSink {
     }
 };
 
-/* Deposit of one straight segment inside cell (i,j,k). */
-template <class S>
-__device__ static void deposit_seg(const S &v, const Grid &g, int i, int j, int k, double x1,
-                                   double x2, double y1, double y2, double z1, double z2,
-                                   double rp)
+/* The 12 current values of one straight segment inside cell (i,j,k). */
+__device__ static inline void seg_values(const Grid &g, int i, int j, int k, double x1, double x2,
+                                         double y1, double y2, double z1, double z2, double rp,
+                                         double vx[4], double vy[4], double vz[4])
 {
     const double h1 = g.h[0], h2 = g.h[1], h3 = g.h[2], tau = g.tau;
     const double dX = x2 - x1, dY = y2 - y1, dZ = z2 - z1;
@@ -114,20 +119,65 @@ __device__ static void deposit_seg(const S &v, const Grid &g, int i, int j, int 
     const double ax = rp * dX / (tau * h2 * h3);
     const double ay = rp * dY / (tau * h1 * h3);
     const double az = rp * dZ / (tau * h1 * h2);
-    const double vx[4] = {ax * ((h2 - tY) * (h3 - tZ) + yz), ax * (tY * (h3 - tZ) - yz),
-                          ax * ((h2 - tY) * tZ - yz), ax * (tY * tZ + yz)};
-    const double vy[4] = {ay * ((h1 - tX) * (h3 - tZ) + xz), ay * (tX * (h3 - tZ) - xz),
-                          ay * ((h1 - tX) * tZ - xz), ay * (tX * tZ + xz)};
-    const double vz[4] = {az * ((h1 - tX) * (h2 - tY) + xy), az * (tX * (h2 - tY) - xy),
-                          az * ((h1 - tX) * tY - xy), az * (tX * tY + xy)};
+    vx[0] = ax * ((h2 - tY) * (h3 - tZ) + yz);
+    vx[1] = ax * (tY * (h3 - tZ) - yz);
+    vx[2] = ax * ((h2 - tY) * tZ - yz);
+    vx[3] = ax * (tY * tZ + yz);
+    vy[0] = ay * ((h1 - tX) * (h3 - tZ) + xz);
+    vy[1] = ay * (tX * (h3 - tZ) - xz);
+    vy[2] = ay * ((h1 - tX) * tZ - xz);
+    vy[3] = ay * (tX * tZ + xz);
+    vz[0] = az * ((h1 - tX) * (h2 - tY) + xy);
+    vz[1] = az * (tX * (h2 - tY) - xy);
+    vz[2] = az * ((h1 - tX) * tY - xy);
+    vz[3] = az * (tX * tY + xy);
+}
+
+/* Deposit of one straight segment inside cell (i,j,k). */
+template <class S>
+__device__ static void deposit_seg(const S &v, const Grid &g, int i, int j, int k, double x1,
+                                   double x2, double y1, double y2, double z1, double z2,
+                                   double rp)
+{
+    double vx[4], vy[4], vz[4];
+    seg_values(g, i, j, k, x1, x2, y1, y2, z1, z2, rp, vx, vy, vz);
     v.seg(i, j, k, vx, vy, vz);
 }
 
+/* The first straight segment of a marker's path, in its start cell: the
+ * tiled push keeps it (kKeep) to sum it over the run of markers that share
+ * the cell (the markers are sorted by cell) before it reaches LDS. */
+struct FirstSeg {
+    int    i, j, k, on;
+    double x1, x2, y1, y2, z1, z2;
+};
+
+template <bool kKeep, class S>
+__device__ static inline void first_seg(const S &v, const Grid &g, FirstSeg *fs, int i, int j,
+                                        int k, double x1, double x2, double y1, double y2,
+                                        double z1, double z2, double rp)
+{
+    if constexpr (kKeep) {
+        fs->i = i;
+        fs->j = j;
+        fs->k = k;
+        fs->on = 1;
+        fs->x1 = x1;
+        fs->x2 = x2;
+        fs->y1 = y1;
+        fs->y2 = y2;
+        fs->z1 = z1;
+        fs->z2 = z2;
+    } else {
+        deposit_seg(v, g, i, j, k, x1, x2, y1, y2, z1, z2, rp);
+    }
+}
+
 /* Two or three face crossings, source/core/plasma_currentKernel.c:156-279. */
-template <class S>
+template <bool kKeep, class S>
 __device__ static void deposit_multi(const S &v, const Grid &g, int i1, int j1, int k1, int i2,
                                      int j2, int k2, double x1, double y1, double z1, double x2,
-                                     double y2, double z2, double rp)
+                                     double y2, double z2, double rp, FirstSeg *fs)
 {
     const double h1 = g.h[0], h2 = g.h[1], h3 = g.h[2];
     const int di = abs(i2 - i1), dj = abs(j2 - j1), dk = abs(k2 - k1);
@@ -151,7 +201,7 @@ __device__ static void deposit_multi(const S &v, const Grid &g, int i1, int j1, 
         }
         const double xA = x1 + k1s * (x2 - x1), yA = y1 + k1s * (y2 - y1), zA = z1 + k1s * (z2 - z1);
         const double xB = x1 + k2s * (x2 - x1), yB = y1 + k2s * (y2 - y1), zB = z1 + k2s * (z2 - z1);
-        deposit_seg(v, g, i, j, k, x1, xA, y1, yA, z1, zA, rp);
+        first_seg<kKeep>(v, g, fs, i, j, k, x1, xA, y1, yA, z1, zA, rp);
         i += st ? dI1[0] : dI0[0];
         j += st ? dI1[1] : dI0[1];
         k += st ? dI1[2] : dI0[2];
@@ -182,7 +232,8 @@ __device__ static void deposit_multi(const S &v, const Grid &g, int i1, int j1, 
     if (a1 > a3) { int t = s1; s1 = s3; s3 = t; double d = a1; a1 = a3; a3 = d; }
     if (a2 > a3) { int t = s2; s2 = s3; s3 = t; double d = a2; a2 = a3; a3 = d; }
     int ii[3] = {i1, j1, k1};
-    deposit_seg(v, g, ii[0], ii[1], ii[2], x1, rc[s1][0], y1, rc[s1][1], z1, rc[s1][2], rp);
+    first_seg<kKeep>(v, g, fs, ii[0], ii[1], ii[2], x1, rc[s1][0], y1, rc[s1][1], z1, rc[s1][2],
+                     rp);
     ii[s1] += dI[s1];
     deposit_seg(v, g, ii[0], ii[1], ii[2], rc[s1][0], rc[s2][0], rc[s1][1], rc[s2][1], rc[s1][2],
                 rc[s2][2], rp);
@@ -217,9 +268,10 @@ __device__ static inline int walls(const PushConsts &c, double *r, double *u)
 /* Gather, Boris and deposit of one live marker; returns 0 for a marker off
  * the grid (deleted and flagged), else applies the walls and returns
  * whether the marker survives.  r/u are in/out. */
-template <class S>
+template <bool kKeep = false, class S>
 __device__ static int push_marker(const S &sink, const F64View &v, const PushConsts &c, double r[3],
-                                  double u[3], float rhof, float qmf, double w3[3], unsigned *err)
+                                  double u[3], float rhof, float qmf, double w3[3], unsigned *err,
+                                  FirstSeg *fs = nullptr)
 {
     const Grid &g = v.g;
     const double h1 = g.h[0], h2 = g.h[1], h3 = g.h[2], tau = g.tau;
@@ -265,17 +317,17 @@ __device__ static int push_marker(const S &sink, const F64View &v, const PushCon
     if (fabs(z2 - z1) < 1e-10 * h3) { dk = 0; k2 = k; }
     const double rp = rhof;
     if (di + dj + dk == 0) {
-        deposit_seg(sink, g, i, j, k, x1, x2, y1, y2, z1, z2, rp);
+        first_seg<kKeep>(sink, g, fs, i, j, k, x1, x2, y1, y2, z1, z2, rp);
     } else if (di + dj + dk == 1) {
         const double alpha = dk * ((k + k2 + 1) * 0.5 * h3 - z1) / (z2 - z1 + ((1 - dk) << 16))
                            + dj * ((j + j2 + 1) * 0.5 * h2 - y1) / (y2 - y1 + ((1 - dj) << 16))
                            + di * ((i + i2 + 1) * 0.5 * h1 - x1) / (x2 - x1 + ((1 - di) << 16));
         const double xC = x1 + alpha * (x2 - x1), yC = y1 + alpha * (y2 - y1),
                      zC = z1 + alpha * (z2 - z1);
-        deposit_seg(sink, g, i, j, k, x1, xC, y1, yC, z1, zC, rp);
+        first_seg<kKeep>(sink, g, fs, i, j, k, x1, xC, y1, yC, z1, zC, rp);
         deposit_seg(sink, g, i2, j2, k2, xC, x2, yC, y2, zC, z2, rp);
     } else {
-        deposit_multi(sink, g, i, j, k, i2, j2, k2, x1, y1, z1, x2, y2, z2, rp);
+        deposit_multi<kKeep>(sink, g, i, j, k, i2, j2, k2, x1, y1, z1, x2, y2, z2, rp, fs);
     }
     r[0] = x2;
     r[1] = y2;
@@ -371,6 +423,62 @@ int launch_push64(hipStream_t s, const Markers64 &m, long n, const Fields64 &f, 
     return launch_def(s, a, (unsigned)((n + TPB - 1) / TPB));
 }
 
+/* Row shift of a double by D lanes inside rows of 16 (DPP row_shr on
+ * both halves; 0 from outside the row).  Every lane must be active. */
+template <int D> __device__ static inline double row_shr(double x)
+{
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, 0x110 + D, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x110 + D, 0xf, 0xf, true);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+/* The first segments of a batch (one marker per lane, lanes in cell
+ * order): summed over each run of lanes in one cell within each row of 16
+ * lanes, then added to LDS by the run's last lane in the row (at most 4
+ * atomics per value and run instead of one per marker: LDS atomics to one
+ * address serialise).  Called by every lane of the wavefront. */
+__device__ static inline void first_run_add(const TileSink &ts, const Grid &g, const FirstSeg &fs,
+                                            double rp)
+{
+    double vv[12];
+    int cell = -1, a = 0, b = 0, c = 0;
+    if (fs.on) {
+        seg_values(g, fs.i, fs.j, fs.k, fs.x1, fs.x2, fs.y1, fs.y2, fs.z1, fs.z2, rp, vv, vv + 4,
+                   vv + 8);
+        a = fs.i - ts.o0;
+        b = fs.j - ts.o1;
+        c = fs.k - ts.o2;
+        if (a < 0 || b < 0 || c < 0 || a > TJN - 2 || b > TJN - 2 || c > TJN - 2)
+            ts.gs.seg(fs.i, fs.j, fs.k, vv, vv + 4, vv + 8);   /* outside the block: to HBM */
+        else
+            cell = (a * TJN + b) * TJN + c;
+    }
+    if (cell < 0)
+        for (int q = 0; q < 12; ++q) vv[q] = 0.0;
+    int start, len;
+    bool head;
+    wave_run(cell, start, len, head);
+    const int lane = threadIdx.x & 63;
+    const bool m1 = lane - 1 >= start, m2 = lane - 2 >= start, m4 = lane - 4 >= start,
+               m8 = lane - 8 >= start;
+#pragma unroll
+    for (int q = 0; q < 12; ++q) {
+        double x = vv[q], y;
+        y = row_shr<1>(x);
+        x += m1 ? y : 0.0;
+        y = row_shr<2>(x);
+        x += m2 ? y : 0.0;
+        y = row_shr<4>(x);
+        x += m4 ? y : 0.0;
+        y = row_shr<8>(x);
+        x += m8 ? y : 0.0;
+        vv[q] = x;
+    }
+    const bool tail = lane == start + len - 1 || ((lane & 15) == 15 && lane < start + len - 1);
+    if (tail && cell >= 0) ts.lds_add(a, b, c, vv, vv + 4, vv + 8);
+}
+
 /* Tiled push: persistent workgroups pull 4x4x4-cell tiles (the order's
  * markers by tile-major cell key, read through its permutation) from
  * per-XCD queues, accumulate the tile's J in LDS and flush it to HBM once;
@@ -418,13 +526,17 @@ __device__ __forceinline__ static void push64_tiled(const Markers64 &m, const Ma
         for (long base = beg; base < end; base += TPB) {
             const long p = base + threadIdx.x;
             int key = -1;
+            FirstSeg fs;
+            fs.on = 0;
+            float wf = 0.0f;
             if (p < end) {
                 const long q = tv.perm[p];
                 double r[3] = {m.x[q], m.y[q], m.z[q]};
                 double u[3] = {m.ux[q], m.uy[q], m.uz[q]};
                 const float qm = m.qm[q], w = m.w[q];
                 const int32_t id = m.id[q];
-                int alive = (qm != 0.0f) && push_marker(ts, v, c, r, u, w, qm, w3, err);
+                wf = w;
+                int alive = (qm != 0.0f) && push_marker<true>(ts, v, c, r, u, w, qm, w3, err, &fs);
                 if (!alive) {
                     d.qm[p] = 0.0f;
                     d.w[p] = 0.0f;
@@ -442,6 +554,7 @@ __device__ __forceinline__ static void push64_tiled(const Markers64 &m, const Ma
                 d.id[p] = id;
                 tv.key[p] = key;
             }
+            first_run_add(ts, g, fs, wf);
             wave_run_count(tv.count, key);
         }
         __syncthreads();
@@ -463,6 +576,7 @@ __device__ __forceinline__ static void push64_tiled(const Markers64 &m, const Ma
 
 template <> struct KernelDef<Push64TArgs> {
     static constexpr int kBlock = TPB;
+    static constexpr int kWaves = 3;   /* register budget for 3 wavefronts per SIMD */
     __device__ __forceinline__ static void body(const Push64TArgs &a, VBlock b, VBlock)
     {
         push64_tiled(a.m, a.d, a.v, a.c, a.tv, a.Wacc, a.err, b);
@@ -473,7 +587,7 @@ template <> struct KernelDef<Push64TArgs> {
 int push64_blocks_per_cu(void)
 {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_one<Push64TArgs>, TPB, 0) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_one_w<Push64TArgs>, TPB, 0) !=
             hipSuccess ||
         nb < 1)
         nb = 1;
