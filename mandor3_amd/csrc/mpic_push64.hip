@@ -460,19 +460,17 @@ __device__ static inline void first_run_add(const TileSink &ts, const Grid &g, c
     bool head;
     wave_run(cell, start, len, head);
     const int lane = threadIdx.x & 63;
-    const bool m1 = lane - 1 >= start, m2 = lane - 2 >= start, m4 = lane - 4 >= start,
-               m8 = lane - 8 >= start;
+    /* fma(y, 1, x) is x + y rounded once, fma(y, 0, x) is x: one
+     * instruction instead of a select of y and an add */
+    const double m1 = lane - 1 >= start ? 1.0 : 0.0, m2 = lane - 2 >= start ? 1.0 : 0.0,
+                 m4 = lane - 4 >= start ? 1.0 : 0.0, m8 = lane - 8 >= start ? 1.0 : 0.0;
 #pragma unroll
     for (int q = 0; q < 12; ++q) {
-        double x = vv[q], y;
-        y = row_shr<1>(x);
-        x += m1 ? y : 0.0;
-        y = row_shr<2>(x);
-        x += m2 ? y : 0.0;
-        y = row_shr<4>(x);
-        x += m4 ? y : 0.0;
-        y = row_shr<8>(x);
-        x += m8 ? y : 0.0;
+        double x = vv[q];
+        x = __builtin_fma(row_shr<1>(x), m1, x);
+        x = __builtin_fma(row_shr<2>(x), m2, x);
+        x = __builtin_fma(row_shr<4>(x), m4, x);
+        x = __builtin_fma(row_shr<8>(x), m8, x);
         vv[q] = x;
     }
     const bool tail = lane == start + len - 1 || ((lane & 15) == 15 && lane < start + len - 1);
