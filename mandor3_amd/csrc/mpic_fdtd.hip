@@ -801,7 +801,8 @@ template <class T> struct EFusedArgs {
     Span       sp;
     T          c1, c2, c3;
     const int *jr;         /* x planes [jr[0], jr[1]] of nodes where J may be non-zero */
-    LaserPlane lp;         /* a laser's E plane added after E -= tau J (on = 0: none) */
+    LaserPlane lp;         /* a laser's E plane, added between the update and E -= tau J
+                            * as the reference's sequence does (on = 0: none) */
 };
 
 template <class T>
@@ -857,13 +858,19 @@ __device__ static inline void e_fused(const FieldsT<T> &f, const Grid &g, const 
             ey[e] = cy[e];
             ez[e] = cz[e];
         }
+    }
+    /* em_EStep_start, the laser's plane, then em_EStep_finish: per node the
+     * operations of the separate launches, in their order */
+    laser_group<T>(lp, i, j, k0, V, ey, ez);
+#pragma unroll
+    for (int e = 0; e < V; ++e) {
+        const int k = k0 + e;
         if (k >= -1 && k <= g.n[2] + 1) {
             ex[e] = (T)(ex[e] - tau * jx[e]);
             ey[e] = (T)(ey[e] - tau * jy[e]);
             ez[e] = (T)(ez[e] - tau * jz[e]);
         }
     }
-    laser_group<T>(lp, i, j, k0, V, ey, ez);
     *(VT *)(f.E[0] + o) = ex;
     *(VT *)(f.E[1] + o) = ey;
     *(VT *)(f.E[2] + o) = ez;

@@ -2065,13 +2065,13 @@ static int step_body(mpic_run *r)
      * the fold goes first, so that E -= tau J can join the E update */
     TRY(j_fold_x(r));
     TRY(h_step(r, true));
-    /* one pass for em_EStep_start and E -= tau J where no source adds to E
-     * between them (FP64 form), always in the FP32 split form (its TFSF and
-     * laser sources then add after E -= tau J) */
-    const bool fused = r->fp32 || (!r->tfsf.on && r->lasers.empty());
+    /* one pass for em_EStep_start, the laser's E plane and E -= tau J where
+     * no other source adds to E between them (FP64 form: no TFSF interface,
+     * at most one laser, which the pass adds in the reference's order),
+     * always in the FP32 split form (its TFSF and further laser sources
+     * then add after E -= tau J) */
+    const bool fused = r->fp32 || (!r->tfsf.on && r->lasers.size() <= 1);
     if (fused) {
-        /* the laser's E source in the same pass where no TFSF interface
-         * comes first (FP32 split form: the sources add after E -= tau J) */
         const LaserPlane lf = laser_fold(r, 0);
         FCALL(launch_e_fused, r->g, r->flE, r->djr + 4, lf);
         TRY(e_sources(r, lf.on != 0));
