@@ -127,7 +127,7 @@ def measured_traffic(kernel, workload):
 
 
 PUSH32_KERNEL = "k_one_w<Push32ArgsW<%d> >"     # the FP32 push as the kernel trace names it
-PUSH64_KERNEL = "k_one<Push64TArgs>"
+PUSH64_KERNEL = "k_one_w<Push64TArgs>"
 
 
 def run_gpu(ma, wl, precision, steps, warmup, autotune=False):
